@@ -1,0 +1,215 @@
+"""Benchmark: pair-FFT operator applies/s of the Fock-exchange apply (hf2x) on one MI355X.
+
+Workload (one "step" = one full H*phi apply of the exchange term):
+  fcc a=10.26 bohr, 4x4x4 k (64 k, q over all k), 16 bands/k all occupied (f=2),
+  FFT grid 40^3, plane-wave sphere ~2738 G/k, synthetic seeded orbitals;
+  1,048,576 (k,j,q,i) pair applies per step.
+The timed steps run the FP32-split path (FFTs and pointwise products in FP32,
+band-pair sums and outputs in FP64) with inputs and outputs resident in HBM.
+The same apply in plain FP64 is timed for the FP32/FP64 ratio, and the two
+outputs are compared (per-band relative error, exchange-energy difference).
+
+Output: one JSON line (see the repository's benchmark contract) with
+"roofline" for the dominant kernel (HIP events on the engine's stream around
+every launch inside the timed region) and "cpu_baseline" (the FP64 CPU oracle,
+OpenMP, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pair-FFT operator applies/s + HBM GB/s: FP32-split vs FP64 at ΔE<1e-6 Ha"
+HBM_PEAK_GBS = 8000.0
+CLASS_NAMES = ["pass_A_zpencil", "pass_B_xyplane", "pass_C_zpencil_acc", "orbital_and_final_fft"]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(sys_, threads):
+    """FP64 oracle (reference loop nest, OpenMP over q) on one applied band at k=0:
+    all 64 q x 16 i = 1024 pairs, about 40 core-seconds."""
+    from oracle import oracle as O
+    o = O.Oracle(sys_)
+    nb = np.zeros(sys_.nk, np.int32)
+    nb[0] = 1
+    phi = sys_.occ_block(0)[:1].reshape(-1).copy()
+    pairs = int(sys_.nocc.sum())
+    t0 = time.perf_counter()
+    o.hf2x(nb, phi, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"hf2x FP64 oracle, 1 applied band at k=0 x {pairs} occupied (q,i) = {pairs} pairs, "
+                      f"{dt:.2f} s wall on {threads} threads; full apply extrapolates x{sys_.nk * sys_.nocc[0]}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--max-pairs", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tc_amd import energy as E
+    from tc_amd import system as S
+    from tc_amd import tcx
+
+    log("bench: building workload fcc 4x4x4k 16b 40^3")
+    sys_ = S.fcc_config1()
+    nb = sys_.nocc.copy()
+    phi = sys_.phi_occ
+    pairs_per_step = int(np.sum(nb) * np.sum(sys_.nocc))
+    N = sys_.ngrid
+
+    eng = tcx.Engine(0)
+    eng.load_system(sys_)
+    if args.max_pairs:
+        eng.set_max_pairs(args.max_pairs)
+    d_phi = torch.from_numpy(phi.view(np.float64).copy()).cuda()
+    d_out32 = torch.zeros_like(d_phi)
+    d_out64 = torch.zeros_like(d_phi)
+    torch.cuda.synchronize()
+
+    def step(prec, out):
+        eng._term_device("hf2x", nb, d_phi.data_ptr(), out.data_ptr(), prec)
+
+    log(f"bench: warmup {args.warmup} FP32-split step(s)")
+    for _ in range(args.warmup):
+        d_out32.zero_()
+        torch.cuda.synchronize()
+        step(tcx.PREC_FP32, d_out32)
+
+    log(f"bench: timing {args.steps} FP32-split step(s)")
+    eng.enable_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(tcx.PREC_FP32, d_out32)
+        log(f"bench: step {i + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms, launches, kpairs = eng.get_timing()
+    eng.enable_timing(False)
+    elapsed = t1 - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = pairs_per_step * args.steps / elapsed
+
+    # dominant kernel of the pair pipeline (classes 0..2) and its roofline
+    dom = int(np.argmax(ms[:3]))
+    avg_ms = ms[dom] / max(1, launches[dom])
+    pairs_per_launch = kpairs[dom] / max(1, launches[dom])
+    bytes_per_launch = 2.0 * N * 8 * pairs_per_launch  # each pass: one read + one write of a FP32 pair grid
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    pipeline_gbs = 6.0 * N * 8 * pairs_per_step / (ms_per_step * 1e-3) / 1e9
+    traffic = read_traffic(CLASS_NAMES[dom])
+
+    # FP64 apply for the ratio, then parity of the two outputs
+    log(f"bench: FP64 apply x{args.fp64_steps}")
+    d_out64.zero_()
+    torch.cuda.synchronize()
+    step(tcx.PREC_FP64, d_out64)  # also the parity reference
+    torch.cuda.synchronize()
+    d_scratch = torch.zeros_like(d_out64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        step(tcx.PREC_FP64, d_scratch)
+    torch.cuda.synchronize()
+    fp64_s = (time.perf_counter() - t0) / args.fp64_steps
+    value64 = pairs_per_step / fp64_s
+
+    d_out32.zero_()
+    torch.cuda.synchronize()
+    step(tcx.PREC_FP32, d_out32)
+    torch.cuda.synchronize()
+    h32 = d_out32.cpu().numpy().view(np.complex128)
+    h64 = d_out64.cpu().numpy().view(np.complex128)
+    bo = sys_.band_offsets(nb)
+    rel = []
+    for ik in range(sys_.nk):
+        a = h32[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+        b = h64[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+        rel.extend(np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1))
+    ex32 = E.term_energy(sys_, h32, 2)
+    ex64 = E.term_energy(sys_, h64, 2)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"bench: CPU baseline on {threads} threads")
+        cpu = cpu_baseline(sys_, threads)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded Gaussian orbitals x exp(-|k+G|^2/2), Gram-Schmidt per k)",
+        "config": {
+            "workload": "hf2x fcc a=10.26 bohr, 4x4x4 k, 16 bands/k (all occupied), FFT 40^3, FP32-split",
+            "pairs_per_step": pairs_per_step, "grid": list(sys_.n), "nk": sys_.nk, "bands": int(nb[0]),
+            "npw_mean": float(np.mean(sys_.npw)), "div_correction": True,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": CLASS_NAMES[dom], "avg_launch_ms": avg_ms, "pairs_per_launch": pairs_per_launch,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+        },
+        "cpu_baseline": cpu,
+        "pipeline": {
+            "algorithmic_GBps_6NBc_per_pair": pipeline_gbs, "frac_of_hbm": pipeline_gbs / HBM_PEAK_GBS,
+            "class_ms_per_step": {CLASS_NAMES[c]: float(ms[c]) / args.steps for c in range(4)},
+            "launches_per_step": {CLASS_NAMES[c]: int(launches[c]) // args.steps for c in range(4)},
+        },
+        "fp64": {"value": value64, "ms_per_step": fp64_s * 1e3, "fp32_over_fp64": value / value64},
+        "parity_fp32_vs_fp64": {
+            "per_band_rel_max": float(np.max(rel)), "per_band_rel_median": float(np.median(rel)),
+            "E_x_fp64_Ha": float(ex64.real), "dE_x_Ha": float(abs(ex32 - ex64)),
+        },
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

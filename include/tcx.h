@@ -1,0 +1,148 @@
+/*
+ * tcx.h -- C ABI of the MI355X pair-FFT operator-apply engine.
+ *
+ * Drop-in boundary for the H*phi terms of TC++ (masaochi/TC).  Each term
+ * entry point replaces one function of namespace calc_hamiltonian declared in
+ * the reference's src/include/calc_hamiltonian.hpp; the state setters carry
+ * the pieces of the reference's CrystalStructure / Kpoints / PlaneWaveBasis /
+ * BlochStates / Potentials objects that those functions read.
+ *
+ * Conventions (the reference's):
+ *   - complex numbers are complex128, interleaved (re, im) doubles;
+ *   - orbital arrays are "ragged contiguous": k-major, then band, then plane
+ *     wave, i.e. phi[k][band][0..npw_k) with the k blocks concatenated
+ *     (reference layout phi[spin][k][band][spinor] of Eigen::VectorXcd,
+ *     spin and spinor of extent 1 here);
+ *   - grid index ipw = i1 + n1*(i2 + n2*i3), i1 fastest
+ *     (src/plane_wave_basis.cpp:137-140);
+ *   - reciprocal vectors are the rows b_i, Cartesian G = recip^T * m
+ *     (src/calc_hamiltonian_hf2x.cpp:35-36);
+ *   - term entry points ACCUMULATE into their output (+=), as the reference
+ *     terms do (e.g. src/calc_hamiltonian_hf2x.cpp:146).
+ *
+ * Errors: every function returns a tcx_status; nothing aborts or throws
+ * across this boundary (the reference calls MPI_Abort instead,
+ * src/error_messages.cpp:60-64).  tcx_last_error() gives the message.
+ * Threading: one context is used by one host thread at a time; work runs on
+ * the context's own HIP stream and entry points return after it completes.
+ */
+#ifndef TCX_H
+#define TCX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tcx_context tcx_context;
+
+typedef enum {
+    TCX_OK = 0,
+    TCX_ERR_ARG = 1,          /* invalid argument or size mismatch */
+    TCX_ERR_STATE = 2,        /* a required setter has not been called */
+    TCX_ERR_HIP = 3,          /* HIP runtime error */
+    TCX_ERR_UNSUPPORTED = 4,  /* valid input outside what this build implements */
+    TCX_ERR_NOMEM = 5,        /* device allocation failed */
+    TCX_ERR_INTERNAL = 6
+} tcx_status;
+
+typedef enum { TCX_METHOD_HF = 0, TCX_METHOD_TC = 1, TCX_METHOD_BITC = 2 } tcx_method;
+
+/* Arithmetic of the pair pipeline.  FP64: every FFT and product in double.
+ * FP32: FFTs and pointwise products in float, band-pair sums accumulated in
+ * double, orbital transforms, final transforms, divergence corrections and
+ * all outputs in double. */
+typedef enum { TCX_PREC_FP64 = 0, TCX_PREC_FP32 = 1 } tcx_precision;
+
+/* flags of the term entry points */
+#define TCX_FLAG_DEVICE_PTRS 1u   /* phi / hphi are device pointers on the context's device */
+
+/* ---------------------------------------------------------------- context */
+int tcx_create(tcx_context **ctx, int device);
+int tcx_destroy(tcx_context *ctx);
+const char *tcx_last_error(const tcx_context *ctx);
+int tcx_synchronize(tcx_context *ctx);
+
+/* ---------------------------------------------------------------- state */
+/* Cell, FFT grid, spin and method.  Replaces what the terms read from
+ * CrystalStructure (reciprocal_vectors, unit_cell_volume:
+ * src/include/crystal_structure.hpp), PlaneWaveBasis::setup_FFT
+ * (src/plane_wave_basis.cpp:175-198), Spin::num_independent_spins and
+ * Method::calc_method.  nspin must be 1 (spin-unpolarised) in this build. */
+int tcx_set_cell(tcx_context *ctx, const double recip_rows[9], double volume,
+                 const int grid[3], int nspin, int method);
+
+/* SCF k-points in crystal coordinates, identity symmetry (each k irreducible
+ * with one isym, no time reversal).  num_kpoints is Kpoints::num_kpoints(),
+ * the Nk of the 1/(Nk V) prefactors.  Replaces Kpoints::kvectors_scf /
+ * index_all_kscf (src/kpoints.cpp:34-110). */
+int tcx_set_kpoints(tcx_context *ctx, int nk, const double *kfrac, int num_kpoints);
+
+/* Plane-wave basis: npw[k] and the concatenated FFT-grid indices of every
+ * G at every k.  Replaces PlaneWaveBasis::num_G_at_k_scf / Gindex_at_k_scf
+ * (src/plane_wave_basis.cpp:81-164). */
+int tcx_set_basis(tcx_context *ctx, const int *npw, const int *gindex);
+
+/* Occupied orbitals (BlochStates::phik_scf, filling, num_occupied_bands;
+ * src/include/bloch_states.hpp:31-40).  nocc[k] orbitals at k with their
+ * fillings (spin factor included, as in the reference).  Density-matrix
+ * mixing's "old" bands (phik_scf_old, filling_old) are simply extra entries.
+ * chi_occ: BITC left orbitals (phik_left_scf), NULL otherwise.  For TC/BITC
+ * the real-space density (BlochStates::set_density,
+ * src/bloch_states_scfloop_filling_density.cpp:256-358) is formed here. */
+int tcx_set_states(tcx_context *ctx, const int *nocc, const double *filling,
+                   const double *phi_occ, const double *chi_occ);
+
+/* RPA Jastrow parameters in atomic units, [s1*2+s2] (Jastrow::A_long_,
+ * F_long_ after unnormalize_A_long and impose_cusp;
+ * src/jastrow_initialize.cpp:28-110).  All |A| <= 1e-8 means "no Jastrow"
+ * (Jastrow::is_A_zero_). */
+int tcx_set_jastrow_rpa(tcx_context *ctx, const double A[4], const double F[4]);
+
+/* Divergence (G=0) correction: Potentials::includes_div_correction,
+ * alpha_Vaux and sum_of_Vaux_scf[nk] (src/potentials_coulomb.cpp:9-161). */
+int tcx_set_div_correction(tcx_context *ctx, int enabled, double alpha, const double *sum_of_vaux);
+
+/* Host utility computing alpha and sum_of_Vaux for a k-set, restating
+ * Potentials::set_sum_of_Vaux_each (src/potentials_coulomb.cpp:60-161). */
+int tcx_sum_of_vaux(const double recip_rows[9], double volume, int nk, const double *kfrac,
+                    double *alpha, double *sum_of_vaux);
+
+/* ---------------------------------------------------------------- terms */
+/* nb[k] bands at every k in phi (ragged, see above); hphi has the same
+ * layout and is accumulated into. */
+
+/* Fock exchange Vx*phi: replaces calc_hamiltonian::hf2x
+ * (src/include/calc_hamiltonian.hpp:54-63; src/calc_hamiltonian_hf2x.cpp:8-205). */
+int tcx_hf2x(tcx_context *ctx, const int *nb, const double *phi, double *h2phi,
+             int precision, unsigned flags);
+
+/* TC/BITC two-body exchange [2a_x, 2b_x1, 2b_x2]: replaces
+ * calc_hamiltonian::tc2x (src/include/calc_hamiltonian.hpp:77-86;
+ * src/calc_hamiltonian_tc2x.cpp:11-516). */
+int tcx_tc2x(tcx_context *ctx, const int *nb, const double *phi, double *h2phi,
+             int precision, unsigned flags);
+
+/* Three-body terms that reduce to a local potential: replace
+ * calc_hamiltonian::tc3a1 and tc3b1 (src/include/calc_hamiltonian.hpp:89-108;
+ * src/calc_hamiltonian_tc3a1.cpp:8-117, src/calc_hamiltonian_tc3b1.cpp:8-135). */
+int tcx_tc3a1(tcx_context *ctx, const int *nb, const double *phi, double *h3phi,
+              int precision, unsigned flags);
+int tcx_tc3b1(tcx_context *ctx, const int *nb, const double *phi, double *h3phi,
+              int precision, unsigned flags);
+
+/* ---------------------------------------------------------------- tuning / timing */
+/* Upper bound of (k,j,q,i) pairs held in flight per launch (pair-buffer size). */
+int tcx_set_max_pairs(tcx_context *ctx, long max_pairs);
+
+/* Per-kernel-class timing with HIP events on the context's stream.
+ * Classes: 0 pair pass A (z-pencils), 1 pair pass B (xy-planes),
+ * 2 pair pass C (z-pencils + accumulate), 3 orbital/final transforms.
+ * out_ms[c] = summed duration (ms), out_launches[c] = number of launches,
+ * out_pairs[c] = pairs processed by that class. */
+int tcx_enable_timing(tcx_context *ctx, int on);
+int tcx_get_timing(tcx_context *ctx, double out_ms[4], long out_launches[4], long out_pairs[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TCX_H */

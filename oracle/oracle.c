@@ -1,0 +1,624 @@
+/*
+ * oracle.c -- FP64 CPU restatement of TC++'s pair-FFT operator apply.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Every function below restates
+ * the reference algorithm from reading its source; the reference file:line
+ * it follows is cited at each function.  The loop nests deliberately keep
+ * the reference's structure (orbital FFTs recomputed inside the pair loop,
+ * OpenMP over q, accumulation under a critical section) because this code
+ * is also the CPU baseline that the GPU path is timed against.
+ *
+ * Parity pins (tests/): orc_fft3d against orc_dft3d_naive and numpy.fft;
+ * orc_hf2x / orc_tc2x against a brute-force G-space convolution (no FFT) on
+ * tiny grids, the plane-wave closed form of the exchange operator, and
+ * operator invariants (Hermiticity, A=0 => tc2x == hf2x, BITC(chi=phi) ==
+ * TC, quadratic dependence on lambda*u); orc_tc3a1 / orc_tc3b1 against a
+ * brute-force G-space restatement and the uniform-density zero.
+ */
+#include "oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+static const double PI = 3.14159265358979323846; /* header.hpp:43 */
+#define FOURPI (4.0 * PI)                          /* header.hpp:44 */
+
+/* ------------------------------------------------------------------ */
+/* FFT: recursive mixed-radix decimation in time, any length.          */
+/* ------------------------------------------------------------------ */
+
+static int smallest_factor(int n)
+{
+    if (n % 4 == 0) return 4;
+    for (int p = 2; p * p <= n; ++p)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* tw[j] = exp(sign * 2*pi*i * j / n0) for j < n0 */
+static void make_twiddles(cplx *tw, int n0, int sign)
+{
+    for (int j = 0; j < n0; ++j) {
+        double a = 2.0 * PI * (double)j / (double)n0;
+        tw[j] = cos(a) + sign * I * sin(a);
+    }
+}
+
+/* out[k] = sum_t in[t*is] * W_n^{t k}, W_n = tw[step] with n*step = n0 */
+static void fft_rec(const cplx *in, int is, cplx *out, int n, int step, int n0, const cplx *tw)
+{
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = smallest_factor(n);
+    int m = n / p;
+    for (int r = 0; r < p; ++r)
+        fft_rec(in + (long)r * is, is * p, out + r * m, m, step * p, n0, tw);
+    cplx t[64], y[64];
+    for (int k = 0; k < m; ++k) {
+        for (int r = 0; r < p; ++r)
+            t[r] = out[r * m + k] * tw[((long)r * k * step) % n0];
+        for (int s = 0; s < p; ++s) {
+            cplx acc = 0;
+            for (int r = 0; r < p; ++r)
+                acc += t[r] * tw[((long)r * s * m * step) % n0];
+            y[s] = acc;
+        }
+        for (int s = 0; s < p; ++s) out[k + s * m] = y[s];
+    }
+}
+
+void orc_fft3d(double *grid_d, const int n[3], int sign)
+{
+    cplx *grid = (cplx *)grid_d;
+    const int n1 = n[0], n2 = n[1], n3 = n[2];
+    const long N = (long)n1 * n2 * n3;
+    int nmax = n1 > n2 ? n1 : n2;
+    if (n3 > nmax) nmax = n3;
+    cplx *tw = malloc(sizeof(cplx) * nmax);
+    cplx *line = malloc(sizeof(cplx) * nmax);
+    cplx *res = malloc(sizeof(cplx) * nmax);
+    /* axis 1 (contiguous) */
+    make_twiddles(tw, n1, sign);
+    for (long b = 0; b < (long)n2 * n3; ++b) {
+        cplx *row = grid + b * n1;
+        fft_rec(row, 1, res, n1, 1, n1, tw);
+        memcpy(row, res, sizeof(cplx) * n1);
+    }
+    /* axis 2 */
+    make_twiddles(tw, n2, sign);
+    for (int i3 = 0; i3 < n3; ++i3)
+        for (int i1 = 0; i1 < n1; ++i1) {
+            cplx *base = grid + (long)i3 * n1 * n2 + i1;
+            for (int i2 = 0; i2 < n2; ++i2) line[i2] = base[(long)i2 * n1];
+            fft_rec(line, 1, res, n2, 1, n2, tw);
+            for (int i2 = 0; i2 < n2; ++i2) base[(long)i2 * n1] = res[i2];
+        }
+    /* axis 3 */
+    make_twiddles(tw, n3, sign);
+    for (long b = 0; b < (long)n1 * n2; ++b) {
+        cplx *base = grid + b;
+        for (int i3 = 0; i3 < n3; ++i3) line[i3] = base[(long)i3 * n1 * n2];
+        fft_rec(line, 1, res, n3, 1, n3, tw);
+        for (int i3 = 0; i3 < n3; ++i3) base[(long)i3 * n1 * n2] = res[i3];
+    }
+    if (sign < 0) /* FFT_forward divides by N: plane_wave_basis.cpp:279 */
+        for (long i = 0; i < N; ++i) grid[i] /= (double)N;
+    free(tw); free(line); free(res);
+}
+
+void orc_dft3d_naive(const double *in_d, double *out_d, const int n[3], int sign)
+{
+    const cplx *in = (const cplx *)in_d;
+    cplx *out = (cplx *)out_d;
+    const long N = (long)n[0] * n[1] * n[2];
+    for (long o = 0; o < N; ++o) {
+        int o1 = o % n[0], o2 = (o / n[0]) % n[1], o3 = o / ((long)n[0] * n[1]);
+        cplx acc = 0;
+        for (long s = 0; s < N; ++s) {
+            int s1 = s % n[0], s2 = (s / n[0]) % n[1], s3 = s / ((long)n[0] * n[1]);
+            double ph = 2.0 * PI * ((double)((long)o1 * s1 % n[0]) / n[0] +
+                                    (double)((long)o2 * s2 % n[1]) / n[1] +
+                                    (double)((long)o3 * s3 % n[2]) / n[2]);
+            acc += in[s] * (cos(ph) + sign * I * sin(ph));
+        }
+        out[o] = sign < 0 ? acc / (double)N : acc;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Grid helpers                                                         */
+/* ------------------------------------------------------------------ */
+
+/* plane_wave_basis.cpp:205-216 (Miller decode) and the Cartesian product
+ * reciprocal_vectors().transpose() * m (calc_hamiltonian_hf2x.cpp:35-36). */
+static void grid_gcart(const orc_problem *p, double *G /* N*3 */)
+{
+    const int n1 = p->n[0], n2 = p->n[1], n3 = p->n[2];
+    const long N = (long)n1 * n2 * n3;
+    for (long ipw = 0; ipw < N; ++ipw) {
+        int i1 = ipw % n1, i2 = (ipw / n1) % n2, i3 = ipw / ((long)n1 * n2);
+        int m1 = i1 <= n1 / 2 ? i1 : i1 - n1;
+        int m2 = i2 <= n2 / 2 ? i2 : i2 - n2;
+        int m3 = i3 <= n3 / 2 ? i3 : i3 - n3;
+        for (int d = 0; d < 3; ++d)
+            G[ipw * 3 + d] = p->recip[0 * 3 + d] * m1 + p->recip[1 * 3 + d] * m2 + p->recip[2 * 3 + d] * m3;
+    }
+}
+
+static void kcart(const orc_problem *p, const double *kf, double *kc)
+{
+    for (int d = 0; d < 3; ++d)
+        kc[d] = p->recip[0 * 3 + d] * kf[0] + p->recip[1 * 3 + d] * kf[1] + p->recip[2 * 3 + d] * kf[2];
+}
+
+static long pw_offset(const orc_problem *p, int ik)
+{
+    long off = 0;
+    for (int k = 0; k < ik; ++k) off += p->npw[k];
+    return off;
+}
+
+static long occ_offset(const orc_problem *p, int ik) /* in orbitals */
+{
+    long off = 0;
+    for (int k = 0; k < ik; ++k) off += p->nocc[k];
+    return off;
+}
+
+static long occ_coef_offset(const orc_problem *p, int ik) /* in complex numbers */
+{
+    long off = 0;
+    for (int k = 0; k < ik; ++k) off += (long)p->nocc[k] * p->npw[k];
+    return off;
+}
+
+static long app_coef_offset(const orc_problem *p, const int *nb, int ik)
+{
+    long off = 0;
+    for (int k = 0; k < ik; ++k) off += (long)nb[k] * p->npw[k];
+    return off;
+}
+
+/* get_orbital_FFTgrid, identity symmetry: plane_wave_basis.cpp:221-271 */
+static void scatter(const orc_problem *p, int ik, const cplx *c, cplx *grid, long N)
+{
+    memset(grid, 0, sizeof(cplx) * N);
+    const int *gi = p->gindex + pw_offset(p, ik);
+    for (int l = 0; l < p->npw[ik]; ++l) grid[gi[l]] = c[l];
+}
+
+static cplx zdotc(const cplx *a, const cplx *b, int n) /* Eigen a.dot(b) = sum conj(a) b */
+{
+    cplx s = 0;
+    for (int i = 0; i < n; ++i) s += conj(a[i]) * b[i];
+    return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* Jastrow (RPA), jastrow_functions.cpp:6-97                            */
+/* ------------------------------------------------------------------ */
+
+double orc_uk(const orc_problem *p, double G2, int s1, int s2)
+{
+    if (p->jastrow_A_zero) return 0.0;                    /* uk :9-16, no polynomial */
+    const double A = p->A[s1 * 2 + s2], F = p->F[s1 * 2 + s2];
+    const double F2 = F * F;
+    if (G2 < 1e-12) return -FOURPI * A * F2;              /* uk_RPA :63-67 */
+    return FOURPI * A / (G2 * (G2 * F2 + 1.0));           /* :70 */
+}
+
+double orc_tc_2body(const orc_problem *p, double G2, int s1, int s2)
+{
+    if (p->jastrow_A_zero) {                              /* tc_2body :22-39 */
+        if (G2 > 1e-12) return FOURPI / G2;
+        return 0.0;
+    }
+    const double A = p->A[s1 * 2 + s2], F = p->F[s1 * 2 + s2];
+    const double F2 = F * F;
+    const double PIAAF2 = PI * A * A / F2;                /* PI_AAFinv2_long_, jastrow_initialize.cpp:107 */
+    if (G2 < 1e-12) return -2.0 * PIAAF2 * F;             /* tc_2body_RPA :79-85 */
+    const double G1 = sqrt(G2);
+    const double FG1 = F * G1;
+    const double F2G2 = F2 * G2;
+    return FOURPI / G2 - FOURPI * A / (1.0 + F2G2)
+         + (PIAAF2 / G1) * (PI * F2G2 + (4.0 + 2.0 * F2G2) * atan(FG1 / 2.0)
+                            - (4.0 + 4.0 * F2G2) * atan(FG1)); /* :88-95 */
+}
+
+/* ------------------------------------------------------------------ */
+/* Divergence correction sums, potentials_coulomb.cpp:60-161           */
+/* ------------------------------------------------------------------ */
+
+void orc_sum_of_vaux(const double recip[9], double volume, int nk, const double *kfrac,
+                     double *alpha_out, double *sum_out)
+{
+    double alpha = cbrt(volume) / (2.0 * PI);
+    alpha = alpha * alpha;                                  /* :69-71 */
+    for (int ik = 0; ik < nk; ++ik) {
+        double sum = 0.0;
+        int if_match = 0;
+        for (int iq = 0; iq < nk; ++iq) {
+            double kq[3];
+            for (int d = 0; d < 3; ++d) {
+                kq[d] = kfrac[ik * 3 + d] - kfrac[iq * 3 + d];
+                while (kq[d] < -1e-8) kq[d] += 1.0;         /* :92-96 */
+                while (kq[d] > 1.0 - 1e-8) kq[d] -= 1.0;
+            }
+            int match = (kq[0] * kq[0] + kq[1] * kq[1] + kq[2] * kq[2]) < 1e-8;
+            if (match) if_match = 1;
+            if (!match) {                                   /* nshell = 0: :109-114 */
+                double c[3];
+                for (int d = 0; d < 3; ++d)
+                    c[d] = recip[0 * 3 + d] * kq[0] + recip[1 * 3 + d] * kq[1] + recip[2 * 3 + d] * kq[2];
+                double kq2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+                sum += exp(-alpha * kq2) / kq2;
+            }
+            for (int nshell = 1;; ++nshell) {               /* :115-136 */
+                double shell = 0.0;
+                for (int gx = -nshell; gx <= nshell; ++gx)
+                    for (int gy = -nshell; gy <= nshell; ++gy)
+                        for (int gz = -nshell; gz <= nshell; ++gz) {
+                            if (gx != nshell && -gx != nshell && gy != nshell && -gy != nshell &&
+                                gz != nshell && -gz != nshell) continue;
+                            double m[3] = {kq[0] + gx, kq[1] + gy, kq[2] + gz}, c[3];
+                            for (int d = 0; d < 3; ++d)
+                                c[d] = recip[0 * 3 + d] * m[0] + recip[1 * 3 + d] * m[1] + recip[2 * 3 + d] * m[2];
+                            double g2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+                            shell += exp(-alpha * g2) / g2;
+                        }
+                sum += shell;
+                if (shell < 1e-20) break;
+            }
+        }
+        sum *= FOURPI;                                      /* :139 */
+        if (if_match) sum -= FOURPI * alpha;                /* :149-160 (k on the SCF mesh) */
+        sum_out[ik] = sum;
+    }
+    *alpha_out = alpha;
+}
+
+/* ------------------------------------------------------------------ */
+/* Density, bloch_states_scfloop_filling_density.cpp:256-358            */
+/* ------------------------------------------------------------------ */
+
+void orc_density(const orc_problem *p, double *out_d)
+{
+    cplx *out = (cplx *)out_d;
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const int bitc = p->method == 2;
+    cplx *orb = malloc(sizeof(cplx) * N), *orbl = malloc(sizeof(cplx) * N);
+    memset(out, 0, sizeof(cplx) * 2 * N);
+    const cplx *phi = (const cplx *)p->phi_occ;
+    const cplx *chi = p->chi_occ ? (const cplx *)p->chi_occ : phi;
+    for (int ik = 0; ik < p->nk; ++ik) {
+        long co = occ_coef_offset(p, ik), fo = occ_offset(p, ik);
+        for (int ib = 0; ib < p->nocc[ik]; ++ib) {
+            scatter(p, ik, phi + co + (long)ib * p->npw[ik], orb, N);
+            orc_fft3d((double *)orb, p->n, +1);
+            if (bitc) {
+                scatter(p, ik, chi + co + (long)ib * p->npw[ik], orbl, N);
+                orc_fft3d((double *)orbl, p->n, +1);
+            }
+            double f = p->filling[fo + ib];
+            for (long r = 0; r < N; ++r)
+                out[r] += f * (bitc ? conj(orbl[r]) * orb[r] : (cplx)(creal(orb[r]) * creal(orb[r]) + cimag(orb[r]) * cimag(orb[r])));
+        }
+    }
+    if (p->nspin == 1) {                                    /* :343, :356-358 */
+        for (long r = 0; r < N; ++r) { out[r] /= 2.0; out[N + r] = out[r]; }
+    }
+    free(orb); free(orbl);
+}
+
+/* ------------------------------------------------------------------ */
+/* hf2x, calc_hamiltonian_hf2x.cpp:8-205                                */
+/* ------------------------------------------------------------------ */
+
+static double div_corr_value(const orc_problem *p, int ik) /* hf2x.cpp:153-163, tc2x.cpp:412-422 */
+{
+    double dc = 1.0 / sqrt(PI * p->alpha);
+    dc -= p->sum_vaux[ik] / (p->num_kpoints * p->volume);
+    dc *= -1.0;
+    if (p->nspin == 1) dc /= 2.0;
+    return dc;
+}
+
+void orc_hf2x(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d, int nthreads)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *phi = (const cplx *)phi_d;
+    cplx *hphi = (cplx *)hphi_d;
+    const cplx *occ = (const cplx *)p->phi_occ;
+    double factor = -FOURPI / (p->num_kpoints * p->volume);  /* :29 */
+    if (p->nspin == 1) factor /= 2.0;                        /* :30 */
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+    cplx *phij = malloc(sizeof(cplx) * N), *hsub = malloc(sizeof(cplx) * N);
+    if (nthreads <= 0) nthreads = 0;
+
+    for (int ik = 0; ik < p->nk; ++ik) {
+        const long ao = app_coef_offset(p, nb, ik);
+        const int npwk = p->npw[ik];
+        const int *gi = p->gindex + pw_offset(p, ik);
+        double kc[3];
+        kcart(p, p->kfrac + 3 * ik, kc);
+        for (int jb = 0; jb < nb[ik]; ++jb) {
+            scatter(p, ik, phi + ao + (long)jb * npwk, phij, N);  /* :60-64 */
+            orc_fft3d((double *)phij, p->n, +1);
+            memset(hsub, 0, sizeof(cplx) * N);
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+            {
+                cplx *phiq = malloc(sizeof(cplx) * N), *phiqj = malloc(sizeof(cplx) * N);
+                cplx *hloc = calloc(N, sizeof(cplx));
+                double *V = malloc(sizeof(double) * N);
+#pragma omp for schedule(dynamic)
+                for (int iq = 0; iq < p->nk; ++iq) {                  /* :80 */
+                    if (p->nocc[iq] == 0) continue;                    /* :90 */
+                    double qc[3];
+                    kcart(p, p->kfrac + 3 * iq, qc);
+                    for (long r = 0; r < N; ++r) {                     /* :91-104 */
+                        double x = kc[0] - qc[0] + G[3 * r], y = kc[1] - qc[1] + G[3 * r + 1],
+                               z = kc[2] - qc[2] + G[3 * r + 2];
+                        double t = x * x + y * y + z * z;
+                        V[r] = t < 1e-8 ? 0.0 : 1.0 / t;
+                    }
+                    const long co = occ_coef_offset(p, iq), fo = occ_offset(p, iq);
+                    for (int ib = 0; ib < p->nocc[iq]; ++ib) {         /* :107 */
+                        scatter(p, iq, occ + co + (long)ib * p->npw[iq], phiq, N);
+                        orc_fft3d((double *)phiq, p->n, +1);           /* :123 */
+                        for (long r = 0; r < N; ++r) phiqj[r] = conj(phiq[r]) * phij[r];
+                        orc_fft3d((double *)phiqj, p->n, -1);          /* :125 */
+                        for (long r = 0; r < N; ++r) phiqj[r] *= V[r];
+                        orc_fft3d((double *)phiqj, p->n, +1);          /* :127 */
+                        const double f = p->filling[fo + ib];
+                        for (long r = 0; r < N; ++r) hloc[r] += f * phiqj[r] * phiq[r]; /* :134 */
+                    }
+                }
+#pragma omp critical
+                for (long r = 0; r < N; ++r) hsub[r] += hloc[r];
+                free(phiq); free(phiqj); free(hloc); free(V);
+            }
+            orc_fft3d((double *)hsub, p->n, -1);                       /* :143 */
+            cplx *out = hphi + ao + (long)jb * npwk;
+            for (int l = 0; l < npwk; ++l) out[l] += factor * hsub[gi[l]]; /* :144-147 */
+
+            if (p->div_corr) {                                          /* :150-201 */
+                const double dc = div_corr_value(p, ik);
+                const long co = occ_coef_offset(p, ik), fo = occ_offset(p, ik);
+                for (int b = 0; b < p->nocc[ik]; ++b) {
+                    const double f = p->filling[fo + b];
+                    if (f < 1e-8) continue;
+                    const cplx *pk = occ + co + (long)b * npwk;
+                    cplx prod = zdotc(pk, phi + ao + (long)jb * npwk, npwk) * f * dc;
+                    for (int l = 0; l < npwk; ++l) out[l] += prod * pk[l];
+                }
+            }
+        }
+    }
+    free(G); free(phij); free(hsub);
+}
+
+/* ------------------------------------------------------------------ */
+/* tc2x, calc_hamiltonian_tc2x.cpp:11-516                               */
+/* (the kVaux branches :117-222, :483-508 vanish for SCF k on the mesh,  */
+/*  potentials_coulomb.cpp:187-191, and are not restated)               */
+/* ------------------------------------------------------------------ */
+
+void orc_tc2x(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d, int nthreads)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *phi = (const cplx *)phi_d;
+    cplx *hphi = (cplx *)hphi_d;
+    const cplx *occ = (const cplx *)p->phi_occ;
+    const int bitc = p->method == 2;
+    const cplx *occl = (bitc && p->chi_occ) ? (const cplx *)p->chi_occ : occ;
+    const int s = 0; /* ispin, num_independent_spins == 1 */
+    double factor = -1.0 / (p->num_kpoints * p->volume);   /* :32 */
+    if (p->nspin == 1) factor /= 2.0;                       /* :33 */
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+
+    for (int ik = 0; ik < p->nk; ++ik) {
+        const long ao = app_coef_offset(p, nb, ik);
+        const int npwk = p->npw[ik];
+        const int *gi = p->gindex + pw_offset(p, ik);
+        double kc[3];
+        kcart(p, p->kfrac + 3 * ik, kc);                    /* :115 */
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+        {
+            double *V2 = malloc(sizeof(double) * N), *duk = malloc(sizeof(double) * 3 * N);
+            cplx *phij = malloc(sizeof(cplx) * N), *gphij = malloc(sizeof(cplx) * 3 * N);
+            cplx *phiq = malloc(sizeof(cplx) * N), *chiq = malloc(sizeof(cplx) * N);
+            cplx *gphiq = malloc(sizeof(cplx) * 3 * N), *phiqj = malloc(sizeof(cplx) * N);
+            cplx *pgp = malloc(sizeof(cplx) * 3 * N), *hsub = malloc(sizeof(cplx) * N);
+#pragma omp for schedule(dynamic)
+            for (int iq = 0; iq < p->nk; ++iq) {                              /* :241 */
+                if (p->nocc[iq] == 0) continue;                                /* :251 */
+                double qc[3];
+                kcart(p, p->kfrac + 3 * iq, qc);                               /* :253-255 */
+                for (long r = 0; r < N; ++r) {                                 /* :256-276 */
+                    double kq[3], g2 = 0;
+                    for (int d = 0; d < 3; ++d) { kq[d] = kc[d] - qc[d] + G[3 * r + d]; g2 += kq[d] * kq[d]; }
+                    V2[r] = orc_tc_2body(p, g2, s, s);
+                    double u = orc_uk(p, g2, s, s);
+                    for (int d = 0; d < 3; ++d) duk[d * N + r] = kq[d] * u;
+                }
+                const long co = occ_coef_offset(p, iq), fo = occ_offset(p, iq);
+                for (int jb = 0; jb < nb[ik]; ++jb) {                          /* :279 */
+                    scatter(p, ik, phi + ao + (long)jb * npwk, phij, N);      /* :283-298 */
+                    for (int d = 0; d < 3; ++d) {
+                        for (long r = 0; r < N; ++r) gphij[d * N + r] = (kc[d] + G[3 * r + d]) * phij[r];
+                        orc_fft3d((double *)(gphij + d * N), p->n, +1);
+                    }
+                    orc_fft3d((double *)phij, p->n, +1);
+                    memset(hsub, 0, sizeof(cplx) * N);                         /* :300 */
+                    for (int ib = 0; ib < p->nocc[iq]; ++ib) {                 /* :303 */
+                        scatter(p, iq, occ + co + (long)ib * p->npw[iq], phiq, N); /* :310-313 */
+                        if (bitc) {                                             /* :323-340 */
+                            scatter(p, iq, occl + co + (long)ib * p->npw[iq], chiq, N);
+                            orc_fft3d((double *)chiq, p->n, +1);
+                        }
+                        for (int d = 0; d < 3; ++d) {                           /* :342-346 */
+                            for (long r = 0; r < N; ++r) gphiq[d * N + r] = (qc[d] + G[3 * r + d]) * phiq[r];
+                            orc_fft3d((double *)(gphiq + d * N), p->n, +1);
+                        }
+                        orc_fft3d((double *)phiq, p->n, +1);                    /* :347 */
+                        const cplx *chi = bitc ? chiq : phiq;
+                        for (long r = 0; r < N; ++r) phiqj[r] = conj(chi[r]) * phij[r]; /* :350 */
+                        orc_fft3d((double *)phiqj, p->n, -1);                   /* :351 */
+                        for (int d = 0; d < 3; ++d) {                           /* :354-358 */
+                            for (long r = 0; r < N; ++r) pgp[d * N + r] = phiqj[r] * duk[d * N + r];
+                            orc_fft3d((double *)(pgp + d * N), p->n, +1);
+                        }
+                        const double f = p->filling[fo + ib];                   /* :360 */
+                        for (long r = 0; r < N; ++r) {                          /* :363-369 */
+                            cplx t = 0;
+                            for (int d = 0; d < 3; ++d) t += pgp[d * N + r] * gphiq[d * N + r];
+                            hsub[r] -= f * t;
+                        }
+                        for (int d = 0; d < 3; ++d) {                           /* :372-376 */
+                            for (long r = 0; r < N; ++r) pgp[d * N + r] = conj(chi[r]) * gphij[d * N + r];
+                            orc_fft3d((double *)(pgp + d * N), p->n, -1);
+                        }
+                        for (long r = 0; r < N; ++r) {                          /* :379-385 */
+                            cplx t = phiqj[r] * V2[r];
+                            for (int d = 0; d < 3; ++d) t += pgp[d * N + r] * duk[d * N + r];
+                            phiqj[r] = t;
+                        }
+                        orc_fft3d((double *)phiqj, p->n, +1);                   /* :386 */
+                        for (long r = 0; r < N; ++r) hsub[r] += f * phiqj[r] * phiq[r]; /* :387-388 */
+                    }
+                    orc_fft3d((double *)hsub, p->n, -1);                        /* :391 */
+#pragma omp critical
+                    {
+                        cplx *out = hphi + ao + (long)jb * npwk;                /* :397-400 */
+                        for (int l = 0; l < npwk; ++l) out[l] += factor * hsub[gi[l]];
+                    }
+                }
+            }
+            free(V2); free(duk); free(phij); free(gphij); free(phiq); free(chiq);
+            free(gphiq); free(phiqj); free(pgp); free(hsub);
+        }
+
+        if (p->div_corr) {                                                      /* :409-513 */
+            const double dc = div_corr_value(p, ik);
+            const long co = occ_coef_offset(p, ik), fo = occ_offset(p, ik);
+            for (int jb = 0; jb < nb[ik]; ++jb) {
+                cplx *out = hphi + ao + (long)jb * npwk;
+                for (int b = 0; b < p->nocc[ik]; ++b) {
+                    const double f = p->filling[fo + b];
+                    if (f < 1e-8) continue;                                     /* :463 */
+                    const cplx *pk = occ + co + (long)b * npwk;
+                    const cplx *ck = occl + co + (long)b * npwk;                /* :476-479 */
+                    cplx prod = zdotc(ck, phi + ao + (long)jb * npwk, npwk);    /* :481 */
+                    prod *= dc;                                                 /* :496 */
+                    prod *= f;                                                  /* :509 */
+                    for (int l = 0; l < npwk; ++l) out[l] += prod * pk[l];      /* :510 */
+                }
+            }
+        }
+    }
+    free(G);
+}
+
+/* ------------------------------------------------------------------ */
+/* tc3a1, calc_hamiltonian_tc3a1.cpp:8-117                              */
+/* ------------------------------------------------------------------ */
+
+static void apply_local_potential(const orc_problem *p, const int *nb, const double *phi_d,
+                                  double *hphi_d, const cplx *V, double factor)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *phi = (const cplx *)phi_d;
+    cplx *hphi = (cplx *)hphi_d;
+    cplx *orb = malloc(sizeof(cplx) * N);
+    for (int ik = 0; ik < p->nk; ++ik) {
+        const long ao = app_coef_offset(p, nb, ik);
+        const int npwk = p->npw[ik];
+        const int *gi = p->gindex + pw_offset(p, ik);
+        for (int jb = 0; jb < nb[ik]; ++jb) {
+            scatter(p, ik, phi + ao + (long)jb * npwk, orb, N);
+            orc_fft3d((double *)orb, p->n, +1);
+            for (long r = 0; r < N; ++r) orb[r] *= V[r];
+            orc_fft3d((double *)orb, p->n, -1);
+            cplx *out = hphi + ao + (long)jb * npwk;
+            for (int l = 0; l < npwk; ++l) out[l] += factor * orb[gi[l]];
+        }
+    }
+    free(orb);
+}
+
+void orc_tc3a1(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const double NkV = p->num_kpoints * p->volume;
+    const double factor = 0.5 / (NkV * NkV);                         /* :26-28 */
+    const cplx *dens = (const cplx *)p->density;
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+    cplx *dnu = calloc(3 * N, sizeof(cplx)), *nG = malloc(sizeof(cplx) * N);
+    cplx *V = calloc(N, sizeof(cplx));
+    const int s = 0;
+    for (int js = 0; js < 2; ++js) {                                  /* :56-70 */
+        memcpy(nG, dens + js * N, sizeof(cplx) * N);
+        orc_fft3d((double *)nG, p->n, -1);
+        for (long r = 0; r < N; ++r) {
+            double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
+            double u = orc_uk(p, g2, s, js);
+            for (int d = 0; d < 3; ++d) dnu[d * N + r] += u * G[3 * r + d] * nG[r];
+        }
+    }
+    for (int d = 0; d < 3; ++d) orc_fft3d((double *)(dnu + d * N), p->n, +1); /* :71-74 */
+    for (int d = 0; d < 3; ++d)                                       /* :78-84 */
+        for (long r = 0; r < N; ++r) V[r] += dnu[d * N + r] * dnu[d * N + r];
+    apply_local_potential(p, nb, phi_d, hphi_d, V, factor);          /* :86-116 */
+    free(G); free(dnu); free(nG); free(V);
+}
+
+/* ------------------------------------------------------------------ */
+/* tc3b1, calc_hamiltonian_tc3b1.cpp:8-135                              */
+/* ------------------------------------------------------------------ */
+
+void orc_tc3b1(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const double NkV = p->num_kpoints * p->volume;
+    const double factor = -1.0 / (NkV * NkV);                        /* :26-28 */
+    const cplx *dens = (const cplx *)p->density;
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+    cplx *dnu = calloc(3 * N, sizeof(cplx)), *nG = malloc(sizeof(cplx) * N);
+    cplx *V = calloc(N, sizeof(cplx));
+    const int s2 = 0; /* ispin2 < num_independent_spins == 1 */
+    for (int s3 = 0; s3 < 2; ++s3) {                                  /* :55-70 */
+        memcpy(nG, dens + s3 * N, sizeof(cplx) * N);
+        orc_fft3d((double *)nG, p->n, -1);
+        for (long r = 0; r < N; ++r) {
+            double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
+            double u = orc_uk(p, g2, s2, s3);
+            for (int d = 0; d < 3; ++d) dnu[d * N + r] += u * G[3 * r + d] * nG[r];
+        }
+    }
+    for (int d = 0; d < 3; ++d) {                                     /* :71-82 */
+        orc_fft3d((double *)(dnu + d * N), p->n, +1);
+        for (long r = 0; r < N; ++r) dnu[d * N + r] *= dens[s2 * N + r];
+        orc_fft3d((double *)(dnu + d * N), p->n, -1);
+    }
+    const int s1 = 0;
+    for (int s2b = 0; s2b < 2; ++s2b) {                               /* :86-100 (ispin2_ref = 0) */
+        for (long r = 0; r < N; ++r) {
+            double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
+            double u = orc_uk(p, g2, s1, s2b);
+            for (int d = 0; d < 3; ++d) V[r] += u * G[3 * r + d] * dnu[d * N + r];
+        }
+    }
+    orc_fft3d((double *)V, p->n, +1);                                 /* :101 */
+    apply_local_potential(p, nb, phi_d, hphi_d, V, factor);          /* :104-134 */
+    free(G); free(dnu); free(nG); free(V);
+}

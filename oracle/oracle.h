@@ -1,0 +1,94 @@
+/*
+ * oracle.h -- FP64 CPU restatement of TC++'s pair-FFT operator apply.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (tc_amd/, include/,
+ * libtcx.so) may include, link or call this code.  Only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py use it, and
+ * only as the checker / the timed CPU baseline.
+ *
+ * The reference (masaochi/TC) needs FFTW3, Eigen3, Boost and MPI, none of
+ * which exist in this image, so it cannot be compiled here; this file is a
+ * restatement written from reading the reference sources.  Each function
+ * cites the reference file:line it follows.  Conventions (reference
+ * src/plane_wave_basis.cpp:175-289, src/include/header.hpp:43-44):
+ *   - grid index ipw = i1 + n1*(i2 + n2*i3), i1 fastest;
+ *   - FFT_forward (R->G) = exp(-iGr) with 1/N; FFT_backward (G->R) = exp(+iGr), unscaled;
+ *   - Cartesian G = recip^T * m with recip rows b_i; Miller m_d = i_d if i_d <= n_d/2 else i_d-n_d.
+ *
+ * Scope of this restatement: identity-only symmetry (every k irreducible,
+ * one isym, no time reversal, phase factor 1), SCF mode, collinear spins
+ * with num_independent_spins = 1 (the spin-unpolarised configurations).
+ */
+#ifndef TCX_ORACLE_H
+#define TCX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    /* FFT grid and cell */
+    int n[3];
+    double recip[9];        /* reciprocal vectors as rows b_i (Cartesian, bohr^-1) */
+    double volume;          /* unit-cell volume (bohr^3) */
+    int nspin;              /* num_independent_spins (only 1 supported) */
+    int method;             /* 0 = HF, 1 = TC, 2 = BITC */
+    /* k-points (identity symmetry) */
+    int nk;
+    const double *kfrac;    /* nk*3 crystal coordinates */
+    int num_kpoints;        /* Nk in the 1/(Nk V) prefactors */
+    /* plane-wave basis */
+    const int *npw;         /* nk */
+    const int *gindex;      /* concatenated over k, FFT-grid linear indices */
+    /* occupied (right) and left orbitals at every k; "old" density-matrix
+     * bands are simply extra entries of this list */
+    const int *nocc;        /* nk */
+    const double *filling;  /* concatenated over k */
+    const double *phi_occ;  /* complex128 interleaved, [k][i][npw_k] */
+    const double *chi_occ;  /* BITC left orbitals (same layout) or NULL (= phi_occ) */
+    /* RPA Jastrow, [s1*2+s2], atomic units (already "unnormalised") */
+    double A[4];
+    double F[4];
+    int jastrow_A_zero;     /* Jastrow::is_A_zero_ */
+    /* divergence (G=0) correction */
+    int div_corr;
+    double alpha;           /* Potentials::alpha_Vaux_ */
+    const double *sum_vaux; /* nk, Potentials::sum_of_Vaux_scf_ */
+    /* real-space density [2][N] complex128 (BlochStates::density_); may be NULL
+     * for terms that do not use it */
+    const double *density;
+} orc_problem;
+
+/* 3D FFT in place on a complex128 grid (interleaved doubles).
+ * sign = -1: FFT_forward (with 1/N); sign = +1: FFT_backward (unscaled).
+ * Reference: src/plane_wave_basis.cpp:274-289 (FFTW plan dims (n3,n2,n1), :189-196). */
+void orc_fft3d(double *grid, const int n[3], int sign);
+
+/* Naive O(N^2) 3D DFT with the same convention (for pinning orc_fft3d). */
+void orc_dft3d_naive(const double *in, double *out, const int n[3], int sign);
+
+/* Jastrow RPA kernels, reference src/jastrow_functions.cpp:6-97. */
+double orc_uk(const orc_problem *p, double G2, int s1, int s2);
+double orc_tc_2body(const orc_problem *p, double G2, int s1, int s2);
+
+/* Divergence-correction shell sums, reference src/potentials_coulomb.cpp:60-161.
+ * Writes alpha and sum_of_Vaux[nk] (single process: ibegin..iend covers everything). */
+void orc_sum_of_vaux(const double recip[9], double volume, int nk, const double *kfrac,
+                     double *alpha_out, double *sum_out);
+
+/* density_s(r), reference src/bloch_states_scfloop_filling_density.cpp:256-358.
+ * out: [2][N] complex128. */
+void orc_density(const orc_problem *p, double *out);
+
+/* H*phi terms.  phi and hphi are complex128 ragged [k][j][npw_k] with nb[k]
+ * applied bands at k; results are ACCUMULATED into hphi (+=), like the
+ * reference.  nthreads <= 0 uses the OpenMP default. */
+void orc_hf2x(const orc_problem *p, const int *nb, const double *phi, double *hphi, int nthreads);
+void orc_tc2x(const orc_problem *p, const int *nb, const double *phi, double *hphi, int nthreads);
+void orc_tc3a1(const orc_problem *p, const int *nb, const double *phi, double *hphi);
+void orc_tc3b1(const orc_problem *p, const int *nb, const double *phi, double *hphi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,191 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [
+        ("n", C.c_int * 3),
+        ("recip", C.c_double * 9),
+        ("volume", C.c_double),
+        ("nspin", C.c_int),
+        ("method", C.c_int),
+        ("nk", C.c_int),
+        ("kfrac", _dp),
+        ("num_kpoints", C.c_int),
+        ("npw", _ip),
+        ("gindex", _ip),
+        ("nocc", _ip),
+        ("filling", _dp),
+        ("phi_occ", _dp),
+        ("chi_occ", _dp),
+        ("A", C.c_double * 4),
+        ("F", C.c_double * 4),
+        ("jastrow_A_zero", C.c_int),
+        ("div_corr", C.c_int),
+        ("alpha", C.c_double),
+        ("sum_vaux", _dp),
+        ("density", _dp),
+    ]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(OrcProblem)
+        L.orc_fft3d.argtypes = [_dp, _ip, C.c_int]
+        L.orc_dft3d_naive.argtypes = [_dp, _dp, _ip, C.c_int]
+        L.orc_uk.argtypes = [P, C.c_double, C.c_int, C.c_int]
+        L.orc_uk.restype = C.c_double
+        L.orc_tc_2body.argtypes = [P, C.c_double, C.c_int, C.c_int]
+        L.orc_tc_2body.restype = C.c_double
+        L.orc_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.orc_density.argtypes = [P, _dp]
+        for name in ("orc_hf2x", "orc_tc2x"):
+            getattr(L, name).argtypes = [P, _ip, _dp, _dp, C.c_int]
+        for name in ("orc_tc3a1", "orc_tc3b1"):
+            getattr(L, name).argtypes = [P, _ip, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def fft3d(grid: np.ndarray, sign: int) -> np.ndarray:
+    """grid: complex128 array shaped (n3, n2, n1) (i1 fastest); returns a new array."""
+    g = np.ascontiguousarray(grid, dtype=np.complex128).copy()
+    n = np.array([g.shape[2], g.shape[1], g.shape[0]], dtype=np.int32)
+    lib().orc_fft3d(g.view(np.float64).ctypes.data_as(_dp), _p(n, _ip), sign)
+    return g
+
+
+def dft3d_naive(grid: np.ndarray, sign: int) -> np.ndarray:
+    g = np.ascontiguousarray(grid, dtype=np.complex128)
+    out = np.empty_like(g)
+    n = np.array([g.shape[2], g.shape[1], g.shape[0]], dtype=np.int32)
+    lib().orc_dft3d_naive(g.view(np.float64).ctypes.data_as(_dp), out.view(np.float64).ctypes.data_as(_dp),
+                          _p(n, _ip), sign)
+    return out
+
+
+def sum_of_vaux(recip, volume, kfrac):
+    r = np.ascontiguousarray(recip, dtype=np.float64).reshape(9)
+    kf = np.ascontiguousarray(kfrac, dtype=np.float64).reshape(-1)
+    nk = len(kf) // 3
+    out = np.zeros(nk)
+    alpha = np.zeros(1)
+    lib().orc_sum_of_vaux(_p(r), float(volume), nk, _p(kf), _p(alpha), _p(out))
+    return float(alpha[0]), out
+
+
+class Oracle:
+    """Holds the ctypes problem struct (and keeps its arrays alive) for one System."""
+
+    def __init__(self, sys, density: np.ndarray | None = None, div_corr: bool | None = None):
+        self.sys = sys
+        self._keep = []
+        P = OrcProblem()
+        P.n[:] = list(sys.n)
+        P.recip[:] = list(np.asarray(sys.recip, dtype=np.float64).reshape(9))
+        P.volume = sys.volume
+        P.nspin = sys.nspin
+        P.method = sys.method_id
+        P.nk = sys.nk
+        P.num_kpoints = sys.num_kpoints
+        self.kfrac = self._hold(np.ascontiguousarray(sys.kfrac, dtype=np.float64).reshape(-1))
+        self.npw = self._hold(np.ascontiguousarray(sys.npw, dtype=np.int32))
+        self.gindex = self._hold(np.ascontiguousarray(sys.gindex, dtype=np.int32))
+        self.nocc = self._hold(np.ascontiguousarray(sys.nocc, dtype=np.int32))
+        self.filling = self._hold(np.ascontiguousarray(sys.filling, dtype=np.float64))
+        self.phi_occ = self._hold(np.ascontiguousarray(sys.phi_occ, dtype=np.complex128))
+        self.chi_occ = (self._hold(np.ascontiguousarray(sys.chi_occ, dtype=np.complex128))
+                        if sys.chi_occ is not None else None)
+        P.kfrac = _p(self.kfrac)
+        P.npw = _p(self.npw, _ip)
+        P.gindex = _p(self.gindex, _ip)
+        P.nocc = _p(self.nocc, _ip)
+        P.filling = _p(self.filling)
+        P.phi_occ = self.phi_occ.view(np.float64).ctypes.data_as(_dp)
+        P.chi_occ = self.chi_occ.view(np.float64).ctypes.data_as(_dp) if self.chi_occ is not None else None
+        P.A[:] = list(np.asarray(sys.A, dtype=np.float64).reshape(4))
+        P.F[:] = list(np.asarray(sys.F, dtype=np.float64).reshape(4))
+        P.jastrow_A_zero = int(sys.jastrow_A_zero)
+        P.div_corr = int(sys.div_corr if div_corr is None else div_corr)
+        alpha, svaux = sum_of_vaux(sys.recip, sys.volume, sys.kfrac)
+        P.alpha = alpha
+        self.svaux = self._hold(svaux)
+        P.sum_vaux = _p(self.svaux)
+        self.P = P
+        if density is None and sys.method in ("TC", "BITC"):
+            density = self.density()
+        if density is not None:
+            self.dens = self._hold(np.ascontiguousarray(density, dtype=np.complex128))
+            self.P.density = self.dens.view(np.float64).ctypes.data_as(_dp)
+
+    def _hold(self, a):
+        self._keep.append(a)
+        return a
+
+    def density(self) -> np.ndarray:
+        out = np.zeros(2 * self.sys.ngrid, dtype=np.complex128)
+        lib().orc_density(C.byref(self.P), out.view(np.float64).ctypes.data_as(_dp))
+        return out
+
+    def _apply(self, fname, nb, phi, hphi=None, nthreads=0):
+        nb = np.ascontiguousarray(nb, dtype=np.int32)
+        phi = np.ascontiguousarray(phi, dtype=np.complex128)
+        out = np.zeros_like(phi) if hphi is None else hphi
+        f = getattr(lib(), fname)
+        args = [C.byref(self.P), _p(nb, _ip), phi.view(np.float64).ctypes.data_as(_dp),
+                out.view(np.float64).ctypes.data_as(_dp)]
+        if fname in ("orc_hf2x", "orc_tc2x"):
+            args.append(int(nthreads))
+        f(*args)
+        return out
+
+    def hf2x(self, nb, phi, nthreads=0):
+        return self._apply("orc_hf2x", nb, phi, nthreads=nthreads)
+
+    def tc2x(self, nb, phi, nthreads=0):
+        return self._apply("orc_tc2x", nb, phi, nthreads=nthreads)
+
+    def tc3a1(self, nb, phi):
+        return self._apply("orc_tc3a1", nb, phi)
+
+    def tc3b1(self, nb, phi):
+        return self._apply("orc_tc3b1", nb, phi)
+
+    def uk(self, G2, s1=0, s2=0):
+        return lib().orc_uk(C.byref(self.P), float(G2), s1, s2)
+
+    def tc_2body(self, G2, s1=0, s2=0):
+        return lib().orc_tc_2body(C.byref(self.P), float(G2), s1, s2)

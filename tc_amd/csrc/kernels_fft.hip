@@ -1,0 +1,263 @@
+// kernels_fft.hip -- 3D FFTs, scatter/gather between plane-wave spheres and
+// FFT grids, precision conversion and the divergence-correction projector.
+//
+// Reference pieces replaced here:
+//   PlaneWaveBasis::FFT_forward / FFT_backward (src/plane_wave_basis.cpp:274-289)
+//   PlaneWaveBasis::get_orbital_FFTgrid, identity symmetry (:221-271)
+//   the gather H(l) += c * W(Gindex(l)) (e.g. src/calc_hamiltonian_hf2x.cpp:144-147)
+//   the divergence-correction projector (src/calc_hamiltonian_hf2x.cpp:150-201,
+//   src/calc_hamiltonian_tc2x.cpp:409-513 with kVaux = 0)
+#include "kernels_common.hpp"
+#include "size_switch.hpp"
+
+namespace tcx {
+
+bool size_supported(int n)
+{
+    for (int s : kSizes)
+        if (s == n) return true;
+    return false;
+}
+
+// ----------------------------------------------------------------- zero fill
+__global__ __launch_bounds__(256) void k_zero(uint4* __restrict__ p, size_t n16, unsigned char* tail, size_t ntail)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+    if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+void launch_zero(void* p, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return;
+    const size_t n16 = bytes / 16, ntail = bytes % 16;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)p, n16,
+                       (unsigned char*)p + n16 * 16, ntail);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+// ----------------------------------------------------------------- scatter / gather
+__global__ __launch_bounds__(256) void k_scatter(const double2* __restrict__ coef, const int* __restrict__ gindex,
+                                                 const OrbEntry* __restrict__ jobs, double2* __restrict__ grids,
+                                                 long long ngrid, int dir, const double* __restrict__ kcart, Geom g)
+{
+    const OrbEntry e = jobs[blockIdx.y];
+    const int l = blockIdx.x * 256 + threadIdx.x;
+    if (l >= e.npw) return;
+    const int idx = gindex[e.gidx_off + l];
+    double2 c = coef[e.coef_off + l];
+    if (dir >= 0) {
+        const int i1 = idx % g.n1, i2 = (idx / g.n1) % g.n2, i3 = idx / (g.n1 * g.n2);
+        const double kg = kcart[3 * e.kidx + dir] + g.b[dir] * centered(i1, g.n1) +
+                          g.b[3 + dir] * centered(i2, g.n2) + g.b[6 + dir] * centered(i3, g.n3);
+        c.x *= kg;
+        c.y *= kg;
+    }
+    grids[(long long)blockIdx.y * ngrid + idx] = c;
+}
+
+void launch_scatter(const double* coef, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
+                    double* grids, long long ngrid, int dir, const double* kcart, const Geom& g, hipStream_t s)
+{
+    if (njobs == 0 || max_npw == 0) return;
+    for (int j0 = 0; j0 < njobs; j0 += 65535) {
+        const int nj = njobs - j0 < 65535 ? njobs - j0 : 65535;
+        hipLaunchKernelGGL(k_scatter, dim3((max_npw + 255) / 256, nj), dim3(256), 0, s, (const double2*)coef,
+                           gindex, jobs + j0, (double2*)grids + (size_t)j0 * ngrid, ngrid, dir, kcart, g);
+        TCX_HIP_CHECK(hipGetLastError());
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_add(const double2* __restrict__ W, const int* __restrict__ gindex,
+                                                    const OrbEntry* __restrict__ jobs, double2* __restrict__ out,
+                                                    long long ngrid, double factor)
+{
+    const OrbEntry e = jobs[blockIdx.y];
+    const int l = blockIdx.x * 256 + threadIdx.x;
+    if (l >= e.npw) return;
+    const double2 w = W[(long long)blockIdx.y * ngrid + gindex[e.gidx_off + l]];
+    double2 o = out[e.coef_off + l];
+    o.x += factor * w.x;
+    o.y += factor * w.y;
+    out[e.coef_off + l] = o;
+}
+
+void launch_gather_add(const double* W, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
+                       double* out, long long ngrid, double factor, hipStream_t s)
+{
+    if (njobs == 0 || max_npw == 0) return;
+    for (int j0 = 0; j0 < njobs; j0 += 65535) {
+        const int nj = njobs - j0 < 65535 ? njobs - j0 : 65535;
+        hipLaunchKernelGGL(k_gather_add, dim3((max_npw + 255) / 256, nj), dim3(256), 0, s,
+                           (const double2*)W + (size_t)j0 * ngrid, gindex, jobs + j0, (double2*)out, ngrid, factor);
+        TCX_HIP_CHECK(hipGetLastError());
+    }
+}
+
+__global__ __launch_bounds__(256) void k_convert_d2f(const double2* __restrict__ in, float2* __restrict__ out,
+                                                     long long n)
+{
+    long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long stride = (long long)gridDim.x * 256;
+    for (; i < n; i += stride) {
+        const double2 v = in[i];
+        out[i] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+void launch_convert_d2f(const double* in, float* out, long long n, hipStream_t s)
+{
+    if (n == 0) return;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_convert_d2f, dim3((unsigned)blocks), dim3(256), 0, s, (const double2*)in, (float2*)out, n);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+// ----------------------------------------------------------------- 3D FFT
+template <typename T, int NZ, int SIGN>
+__global__ __launch_bounds__(kPencilBlock) void k_fft_z(cpx<T>* __restrict__ grids, int nxy, T sc)
+{
+    const int xy = blockIdx.x * kPencilBlock + threadIdx.x;
+    if (xy >= nxy) return;
+    cpx<T>* p = grids + (size_t)blockIdx.y * nxy * NZ + xy;
+    cpx<T> v[NZ];
+    load_pencil<T, NZ>(v, p, nxy);
+    fft<T, NZ, SIGN>(v);
+    if (sc != T(1))
+        static_for<0, NZ>([&](auto zc) { v[decltype(zc)::value] = scale(v[decltype(zc)::value], sc); });
+    store_pencil<T, NZ>(v, p, nxy);
+}
+
+template <typename T, int NP, int SIGN>
+__global__ __launch_bounds__(kPlaneBlock) void k_fft_xy(cpx<T>* __restrict__ grids, int nz)
+{
+    __shared__ PlaneTile<T, NP> tile;
+    const int t = threadIdx.x;
+    cpx<T>* plane = grids + ((size_t)blockIdx.y * nz + blockIdx.x) * (NP * NP);
+    cpx<T> v[NP];
+    if (t < NP) {
+        load_pencil<T, NP>(v, plane + t, NP);  // column x = t
+        fft<T, NP, SIGN>(v);                   // along y
+        tile_put_col<T, NP>(tile, v, t);
+    }
+    __syncthreads();
+    if (t < NP) {
+        tile_get_row<T, NP>(tile, v, t);       // row y = t
+        fft<T, NP, SIGN>(v);                   // along x
+        tile_put_row<T, NP>(tile, v, t);
+    }
+    __syncthreads();
+    if (t < NP) {
+        tile_get_col<T, NP>(tile, v, t);
+        store_pencil<T, NP>(v, plane + t, NP);
+    }
+}
+
+template <typename T>
+static void fft3d_impl(int sign, T* grids, int ngrids, const Geom& g, double sc, hipStream_t s)
+{
+    if (g.n1 != g.n2) throw Error(TCX_ERR_UNSUPPORTED, "FFT grid must have n1 == n2");
+    const int nxy = g.n1 * g.n2;
+    cpx<T>* cg = reinterpret_cast<cpx<T>*>(grids);
+    for (int b0 = 0; b0 < ngrids; b0 += 65535) {
+        const int nb = ngrids - b0 < 65535 ? ngrids - b0 : 65535;
+        cpx<T>* base = cg + (size_t)b0 * nxy * g.n3;
+#define TCX_XY(NP)                                                                                     \
+    if (sign < 0)                                                                                      \
+        hipLaunchKernelGGL((k_fft_xy<T, NP, -1>), dim3(g.n3, nb), dim3(kPlaneBlock), 0, s, base, g.n3); \
+    else                                                                                               \
+        hipLaunchKernelGGL((k_fft_xy<T, NP, 1>), dim3(g.n3, nb), dim3(kPlaneBlock), 0, s, base, g.n3);
+        TCX_SIZE_SWITCH(g.n1, TCX_XY)
+#undef TCX_XY
+        TCX_HIP_CHECK(hipGetLastError());
+        const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, nb);
+#define TCX_Z(NZ)                                                                                            \
+    if (sign < 0)                                                                                            \
+        hipLaunchKernelGGL((k_fft_z<T, NZ, -1>), grid, dim3(kPencilBlock), 0, s, base, nxy, (T)sc);          \
+    else                                                                                                     \
+        hipLaunchKernelGGL((k_fft_z<T, NZ, 1>), grid, dim3(kPencilBlock), 0, s, base, nxy, (T)sc);
+        TCX_SIZE_SWITCH(g.n3, TCX_Z)
+#undef TCX_Z
+        TCX_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void launch_fft3d(int prec, int sign, void* grids, int ngrids, const Geom& g, double sc, hipStream_t s)
+{
+    if (ngrids == 0) return;
+    if (prec == TCX_PREC_FP64)
+        fft3d_impl<double>(sign, (double*)grids, ngrids, g, sc, s);
+    else
+        fft3d_impl<float>(sign, (float*)grids, ngrids, g, sc, s);
+}
+
+// ----------------------------------------------------------------- divergence correction
+// One workgroup per applied band: c_b = dc * f_b * <chi_kb|phi_kj> (f_b >= 1e-8),
+// then hphi_kj += sum_b c_b phi_kb.
+constexpr int kDivMaxOcc = 512;
+
+__global__ __launch_bounds__(256) void k_divcorr(const double2* __restrict__ phi, const double2* __restrict__ occ_r,
+                                                 const double2* __restrict__ occ_l, const DivEntry* __restrict__ jobs,
+                                                 const double* __restrict__ fill, const double* __restrict__ dcs,
+                                                 double2* __restrict__ out)
+{
+    const DivEntry e = jobs[blockIdx.x];
+    const double dc = dcs[e.kidx];
+    __shared__ double2 coef[kDivMaxOcc];
+    __shared__ double2 red[256];
+    const int t = threadIdx.x;
+    const double2* pj = phi + e.app_off;
+    for (int b = 0; b < e.nocc; ++b) {
+        const double f = fill[e.fill_off + b];
+        const double2* cb = occ_l + e.occ_off + (long long)b * e.npw;
+        double re = 0.0, im = 0.0;
+        for (int l = t; l < e.npw; l += 256) {
+            const double2 a = cb[l], x = pj[l];
+            re += a.x * x.x + a.y * x.y;  // conj(a) * x
+            im += a.x * x.y - a.y * x.x;
+        }
+        red[t] = make_double2(re, im);
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) {
+                red[t].x += red[t + w].x;
+                red[t].y += red[t + w].y;
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            const double s = f < 1e-8 ? 0.0 : dc * f;
+            coef[b] = make_double2(red[0].x * s, red[0].y * s);
+        }
+        __syncthreads();
+    }
+    double2* o = out + e.app_off;
+    for (int l = t; l < e.npw; l += 256) {
+        double re = 0.0, im = 0.0;
+        for (int b = 0; b < e.nocc; ++b) {
+            const double2 c = coef[b], v = occ_r[e.occ_off + (long long)b * e.npw + l];
+            re += c.x * v.x - c.y * v.y;
+            im += c.x * v.y + c.y * v.x;
+        }
+        double2 cur = o[l];
+        cur.x += re;
+        cur.y += im;
+        o[l] = cur;
+    }
+}
+
+void launch_divcorr(const double* phi, const double* occ_right, const double* occ_left, const DivEntry* jobs,
+                    int njobs, const double* fill, const double* dc, double* out, hipStream_t s)
+{
+    if (njobs == 0) return;
+    hipLaunchKernelGGL(k_divcorr, dim3(njobs), dim3(256), 0, s, (const double2*)phi, (const double2*)occ_right,
+                       (const double2*)occ_left, jobs, fill, dc, (double2*)out);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace tcx

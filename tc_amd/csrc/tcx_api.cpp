@@ -1,0 +1,656 @@
+// tcx_api.cpp -- C ABI (include/tcx.h) over the gfx950 kernels.
+//
+// The context keeps, on one device and one HIP stream:
+//   * the cell/grid geometry, k-points and plane-wave index lists;
+//   * the occupied orbitals in G space (complex128) and, cached per
+//     precision, in real space (one FFT grid per orbital);
+//   * workspaces: applied orbitals in real space, the FP64 accumulation grids
+//     W_kj, and the pair buffer whose size bounds the pairs in flight.
+// Each term call builds (and caches) a launch plan: the (k,j | q,i) pairs are
+// grouped into launches of "k-chunk x q-chunk" so that every launch owns a
+// set of accumulation targets (k,j) exclusively and W is updated without
+// atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "tcx_internal.hpp"
+#include "tcx_terms.hpp"
+
+using namespace tcx;
+
+namespace tcx {
+
+void DBuf::ensure(size_t bytes)
+{
+    if (bytes <= cap) return;
+    release();
+    if (bytes == 0) return;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        throw Error(TCX_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    cap = bytes;
+}
+
+void DBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+void Context::require(bool cond, int code, const char* what)
+{
+    if (!cond) throw Error(code, what);
+}
+
+void Context::timed(int cls, long pairs, const std::function<void()>& launch)
+{
+    if (!timing_on) {
+        launch();
+        return;
+    }
+    hipEvent_t a, b;
+    TCX_HIP_CHECK(hipEventCreate(&a));
+    TCX_HIP_CHECK(hipEventCreate(&b));
+    TCX_HIP_CHECK(hipEventRecord(a, stream));
+    launch();
+    TCX_HIP_CHECK(hipEventRecord(b, stream));
+    pending.push_back({cls, pairs, a, b});
+}
+
+void Context::collect_timing()
+{
+    if (pending.empty()) return;
+    TCX_HIP_CHECK(hipStreamSynchronize(stream));
+    for (auto& ev : pending) {
+        float ms = 0.f;
+        TCX_HIP_CHECK(hipEventElapsedTime(&ms, ev.a, ev.b));
+        t_ms[ev.cls] += ms;
+        t_launches[ev.cls] += 1;
+        t_pairs[ev.cls] += ev.pairs;
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    pending.clear();
+}
+
+// ------------------------------------------------------------------ state setters
+void Context::set_cell(const double recip_rows[9], double vol, const int grid[3], int ns, int meth)
+{
+    require(recip_rows && grid, TCX_ERR_ARG, "set_cell: null pointer");
+    require(vol > 1e-8, TCX_ERR_ARG, "set_cell: non-positive volume");
+    require(grid[0] > 0 && grid[1] > 0 && grid[2] > 0, TCX_ERR_ARG, "set_cell: bad grid");
+    require(meth >= TCX_METHOD_HF && meth <= TCX_METHOD_BITC, TCX_ERR_ARG, "set_cell: bad method");
+    require(ns == 1, TCX_ERR_UNSUPPORTED, "set_cell: only num_independent_spins == 1 is implemented");
+    require(grid[0] == grid[1], TCX_ERR_UNSUPPORTED, "set_cell: grid must have n1 == n2");
+    require(size_supported(grid[0]) && size_supported(grid[2]), TCX_ERR_UNSUPPORTED,
+            "set_cell: grid size not compiled (supported: 6 8 12 16 20 24 32 36 40 48)");
+    geo.n1 = grid[0];
+    geo.n2 = grid[1];
+    geo.n3 = grid[2];
+    for (int i = 0; i < 9; ++i) geo.b[i] = recip_rows[i];
+    N = (long long)grid[0] * grid[1] * grid[2];
+    volume = vol;
+    nspin = ns;
+    method = meth;
+    cell_set = true;
+    basis_set = false;
+    states_set = false;
+    invalidate_states();
+}
+
+void Context::set_kpoints(int nk_, const double* kf, int numk)
+{
+    require(cell_set, TCX_ERR_STATE, "set_kpoints: call tcx_set_cell first");
+    require(nk_ > 0 && kf, TCX_ERR_ARG, "set_kpoints: bad k list");
+    require(numk > 0, TCX_ERR_ARG, "set_kpoints: num_kpoints must be positive");
+    nk = nk_;
+    num_kpoints = numk;
+    kfrac.assign(kf, kf + 3 * nk);
+    kcart.assign(3 * nk, 0.0);
+    for (int k = 0; k < nk; ++k)
+        for (int d = 0; d < 3; ++d)
+            kcart[3 * k + d] = geo.b[d] * kfrac[3 * k] + geo.b[3 + d] * kfrac[3 * k + 1] + geo.b[6 + d] * kfrac[3 * k + 2];
+    d_kcart.ensure(kcart.size() * sizeof(double));
+    TCX_HIP_CHECK(hipMemcpy(d_kcart.p, kcart.data(), kcart.size() * sizeof(double), hipMemcpyHostToDevice));
+    k_set = true;
+    basis_set = false;
+    states_set = false;
+    div_enabled = false;
+    invalidate_states();
+}
+
+void Context::set_basis(const int* npw_, const int* gindex)
+{
+    require(k_set, TCX_ERR_STATE, "set_basis: call tcx_set_kpoints first");
+    require(npw_ && gindex, TCX_ERR_ARG, "set_basis: null pointer");
+    npw.assign(npw_, npw_ + nk);
+    pw_off.assign(nk + 1, 0);
+    for (int k = 0; k < nk; ++k) {
+        require(npw[k] > 0, TCX_ERR_ARG, "set_basis: npw must be positive");
+        pw_off[k + 1] = pw_off[k] + npw[k];
+    }
+    for (long long l = 0; l < pw_off[nk]; ++l)
+        require(gindex[l] >= 0 && gindex[l] < N, TCX_ERR_ARG, "set_basis: gindex outside the FFT grid");
+    max_npw = *std::max_element(npw.begin(), npw.end());
+    d_gindex.ensure(pw_off[nk] * sizeof(int));
+    TCX_HIP_CHECK(hipMemcpy(d_gindex.p, gindex, pw_off[nk] * sizeof(int), hipMemcpyHostToDevice));
+    basis_set = true;
+    states_set = false;
+    invalidate_states();
+}
+
+void Context::invalidate_states()
+{
+    occR_valid[0] = occR_valid[1] = false;
+    chiR_valid[0] = chiR_valid[1] = false;
+    gradR_valid[0] = gradR_valid[1] = false;
+    density_valid = false;
+    plans.clear();
+}
+
+void Context::set_states(const int* nocc_, const double* filling, const double* phi_occ, const double* chi_occ)
+{
+    require(basis_set, TCX_ERR_STATE, "set_states: call tcx_set_basis first");
+    require(nocc_ && filling && phi_occ, TCX_ERR_ARG, "set_states: null pointer");
+    require(method == TCX_METHOD_BITC || chi_occ == nullptr, TCX_ERR_ARG,
+            "set_states: left orbitals are only meaningful for BITC");
+    require(method != TCX_METHOD_BITC || chi_occ != nullptr, TCX_ERR_ARG, "set_states: BITC needs left orbitals");
+    nocc.assign(nocc_, nocc_ + nk);
+    occ_first.assign(nk + 1, 0);
+    occ_coef.assign(nk + 1, 0);
+    for (int k = 0; k < nk; ++k) {
+        require(nocc[k] >= 0 && nocc[k] <= 512, TCX_ERR_ARG, "set_states: nocc must be in [0, 512]");
+        occ_first[k + 1] = occ_first[k] + nocc[k];
+        occ_coef[k + 1] = occ_coef[k] + (long long)nocc[k] * npw[k];
+    }
+    n_occ = (int)occ_first[nk];
+    fill.assign(filling, filling + n_occ);
+    for (double f : fill) require(std::isfinite(f) && f >= 0.0, TCX_ERR_ARG, "set_states: filling must be >= 0");
+    const size_t cb = (size_t)occ_coef[nk] * 16;
+    for (size_t i = 0; i < (size_t)occ_coef[nk] * 2; ++i)
+        require(std::isfinite(phi_occ[i]), TCX_ERR_ARG, "set_states: non-finite orbital coefficient");
+    d_phi_occ.ensure(std::max<size_t>(cb, 16));
+    TCX_HIP_CHECK(hipMemcpy(d_phi_occ.p, phi_occ, cb, hipMemcpyHostToDevice));
+    if (chi_occ) {
+        d_chi_occ.ensure(std::max<size_t>(cb, 16));
+        TCX_HIP_CHECK(hipMemcpy(d_chi_occ.p, chi_occ, cb, hipMemcpyHostToDevice));
+    }
+    upload(d_fill, fill.data(), fill.size() * sizeof(double));
+    std::vector<OrbEntry> jobs;
+    for (int q = 0; q < nk; ++q)
+        for (int i = 0; i < nocc[q]; ++i) jobs.push_back({occ_coef[q] + (long long)i * npw[q], npw[q], (int)pw_off[q], q, 0});
+    upload(d_occjobs, jobs.data(), jobs.size() * sizeof(OrbEntry));
+    states_set = true;
+    invalidate_states();
+}
+
+void Context::set_jastrow(const double A_[4], const double F_[4])
+{
+    require(A_ && F_, TCX_ERR_ARG, "set_jastrow: null pointer");
+    jastrow_zero = true;
+    for (int i = 0; i < 4; ++i) {
+        A[i] = A_[i];
+        F[i] = F_[i];
+        if (std::fabs(A[i]) > 1e-8) jastrow_zero = false;
+    }
+    if (!jastrow_zero)
+        for (int i = 0; i < 4; ++i)
+            require(std::fabs(F[i]) > 1e-8, TCX_ERR_ARG, "set_jastrow: F must be non-zero");
+    jastrow_set = true;
+    plans.clear();
+}
+
+void Context::set_div(int enabled, double alpha_, const double* svaux)
+{
+    require(k_set, TCX_ERR_STATE, "set_div_correction: call tcx_set_kpoints first");
+    if (enabled) {
+        require(svaux != nullptr, TCX_ERR_ARG, "set_div_correction: sum_of_vaux is null");
+        require(alpha_ > 0.0, TCX_ERR_ARG, "set_div_correction: alpha must be positive");
+        alpha = alpha_;
+        sum_vaux.assign(svaux, svaux + nk);
+        std::vector<double> dc(nk);
+        for (int k = 0; k < nk; ++k) dc[k] = div_factor(k);
+        upload(d_dc, dc.data(), dc.size() * sizeof(double));
+    }
+    div_enabled = enabled != 0;
+}
+
+// ------------------------------------------------------------------ real-space grids
+// Transforms ncoef-addressed orbitals (complex128, G space) to real space in
+// precision prec: FP64 scatter + FP64 inverse FFT, then rounding to float for
+// FP32.  dir >= 0 multiplies by (k+G)_dir first (gradient components).
+void Context::orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n, int prec, void* out, int dir)
+{
+    if (n == 0) return;
+    const size_t g64 = (size_t)N * 16;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n, ((size_t)1 << 30) / g64));
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+        const int nc = std::min(chunk, n - c0);
+        double* dst64;
+        if (prec == TCX_PREC_FP64) {
+            dst64 = (double*)out + (size_t)c0 * N * 2;
+        } else {
+            d_tmp64.ensure(g64 * chunk);
+            dst64 = (double*)d_tmp64.p;
+        }
+        launch_zero(dst64, g64 * nc, stream);
+        launch_scatter(coef, (const int*)d_gindex.p, d_jobs + c0, nc, max_npw, dst64, N, dir,
+                       (const double*)d_kcart.p, geo, stream);
+        launch_fft3d(TCX_PREC_FP64, +1, dst64, nc, geo, 1.0, stream);
+        if (prec == TCX_PREC_FP32)
+            launch_convert_d2f(dst64, (float*)out + (size_t)c0 * N * 2, (long long)nc * N, stream);
+    }
+}
+
+void upload(DBuf& buf, const void* src, size_t bytes)
+{
+    buf.ensure(std::max<size_t>(bytes, 16));
+    if (bytes) TCX_HIP_CHECK(hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+void Context::ensure_occR(int prec)
+{
+    if (occR_valid[prec]) return;
+    const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
+    d_occR[prec].ensure(std::max<size_t>((size_t)n_occ * N * es, 16));
+    orbitals_to_real((const double*)d_phi_occ.p, (const OrbEntry*)d_occjobs.p, n_occ, prec, d_occR[prec].p, -1);
+    occR_valid[prec] = true;
+}
+
+// ------------------------------------------------------------------ launch plans
+const Plan& Context::get_plan(const std::vector<int>& nb)
+{
+    auto key = std::make_pair(nb, max_pairs);
+    auto it = plans.find(key);
+    if (it != plans.end()) return it->second;
+    Plan P;
+    P.app_first.assign(nk + 1, 0);
+    P.app_coef.assign(nk + 1, 0);
+    for (int k = 0; k < nk; ++k) {
+        P.app_first[k + 1] = P.app_first[k] + nb[k];
+        P.app_coef[k + 1] = P.app_coef[k] + (long long)nb[k] * npw[k];
+    }
+    P.n_app = (int)P.app_first[nk];
+    const int nbmax = std::max(1, *std::max_element(nb.begin(), nb.end()));
+    const int noccmax = std::max(1, *std::max_element(nocc.begin(), nocc.end()));
+    // Enough accumulation targets per launch to fill the chip in pass C,
+    // then as many q as the pair budget allows.
+    const long target_goal = std::max<long>(1, (2048L * 64) / std::max<long long>(1, (long long)geo.n1 * geo.n2));
+    int kc = (int)std::min<long>(nk, std::max<long>(1, (target_goal + nbmax - 1) / nbmax));
+    while (kc > 1 && (long)kc * nbmax * noccmax > max_pairs) --kc;
+    int qc = (int)std::max<long>(1, std::min<long>(nk, max_pairs / ((long)kc * nbmax * noccmax)));
+    for (int k0 = 0; k0 < nk; k0 += kc) {
+        const int k1 = std::min(nk, k0 + kc);
+        for (int q0 = 0; q0 < nk; q0 += qc) {
+            const int q1 = std::min(nk, q0 + qc);
+            Launch L;
+            L.pair0 = (int)P.pairs.size();
+            L.target0 = (int)P.targets.size();
+            L.kq0 = (int)P.kq.size() / 3;
+            for (int k = k0; k < k1; ++k)
+                for (int q = q0; q < q1; ++q)
+                    for (int d = 0; d < 3; ++d) P.kq.push_back(kcart[3 * k + d] - kcart[3 * q + d]);
+            int local = 0;
+            for (int k = k0; k < k1; ++k) {
+                for (int j = 0; j < nb[k]; ++j) {
+                    TargetEntry te{(int)P.app_first[k] + j, local, 0, 0};
+                    for (int q = q0; q < q1; ++q) {
+                        const int kq = L.kq0 + (k - k0) * (q1 - q0) + (q - q0);
+                        for (int i = 0; i < nocc[q]; ++i) {
+                            P.pairs.push_back({(int)P.app_first[k] + j, (int)occ_first[q] + i, kq, 0});
+                            ++local;
+                            ++te.npair;
+                        }
+                    }
+                    if (te.npair > 0) P.targets.push_back(te);
+                }
+            }
+            L.npairs = local;
+            L.ntargets = (int)P.targets.size() - L.target0;
+            if (L.npairs > 0) P.launches.push_back(L);
+        }
+    }
+    for (const auto& L : P.launches) P.max_launch_pairs = std::max(P.max_launch_pairs, L.npairs);
+    upload(P.d_pairs, P.pairs.data(), P.pairs.size() * sizeof(PairEntry));
+    upload(P.d_targets, P.targets.data(), P.targets.size() * sizeof(TargetEntry));
+    upload(P.d_kq, P.kq.data(), P.kq.size() * sizeof(double));
+    // one scatter/gather job and one divergence job per applied band, in band order
+    std::vector<OrbEntry> appjobs;
+    std::vector<DivEntry> divjobs;
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nb[k]; ++j) {
+            appjobs.push_back({P.app_coef[k] + (long long)j * npw[k], npw[k], (int)pw_off[k], k, 0});
+            divjobs.push_back({P.app_coef[k] + (long long)j * npw[k], occ_coef[k], npw[k], nocc[k], (int)occ_first[k], k});
+        }
+    upload(P.d_appjobs, appjobs.data(), appjobs.size() * sizeof(OrbEntry));
+    upload(P.d_divjobs, divjobs.data(), divjobs.size() * sizeof(DivEntry));
+    auto res = plans.emplace(key, std::move(P));
+    return res.first->second;
+}
+
+// ------------------------------------------------------------------ term driver
+void Context::check_apply_args(const int* nb, const double* phi, const double* out, int prec)
+{
+    require(states_set, TCX_ERR_STATE, "term: call tcx_set_states first");
+    require(nb && phi && out, TCX_ERR_ARG, "term: null pointer");
+    require(prec == TCX_PREC_FP64 || prec == TCX_PREC_FP32, TCX_ERR_ARG, "term: bad precision");
+    for (int k = 0; k < nk; ++k) require(nb[k] >= 0, TCX_ERR_ARG, "term: negative band count");
+}
+
+ApplyIO Context::begin_apply(const int* nb, const double* phi, double* out, unsigned flags, const Plan& P)
+{
+    ApplyIO io;
+    io.host = !(flags & TCX_FLAG_DEVICE_PTRS);
+    io.ncoef = P.app_coef[nk];
+    io.out_host = out;
+    if (io.host) {
+        for (long long i = 0; i < io.ncoef * 2; ++i)
+            require(std::isfinite(phi[i]), TCX_ERR_ARG, "term: non-finite input coefficient");
+        d_phi_app.ensure(std::max<size_t>(16, (size_t)io.ncoef * 16));
+        d_out.ensure(std::max<size_t>(16, (size_t)io.ncoef * 16));
+        TCX_HIP_CHECK(hipMemcpyAsync(d_phi_app.p, phi, (size_t)io.ncoef * 16, hipMemcpyHostToDevice, stream));
+        launch_zero(d_out.p, (size_t)io.ncoef * 16, stream);
+        io.phi = (const double*)d_phi_app.p;
+        io.out = (double*)d_out.p;
+    } else {
+        io.phi = phi;
+        io.out = out;
+    }
+    return io;
+}
+
+void Context::end_apply(const ApplyIO& io)
+{
+    if (io.host && io.ncoef > 0) {
+        std::vector<double> tmp((size_t)io.ncoef * 2);
+        TCX_HIP_CHECK(hipMemcpyAsync(tmp.data(), io.out, (size_t)io.ncoef * 16, hipMemcpyDeviceToHost, stream));
+        TCX_HIP_CHECK(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < tmp.size(); ++i) io.out_host[i] += tmp[i];
+    }
+    TCX_HIP_CHECK(hipStreamSynchronize(stream));
+    collect_timing();
+}
+
+double Context::div_factor(int k) const
+{
+    // src/calc_hamiltonian_hf2x.cpp:153-163 (identical in tc2x.cpp:412-422)
+    double dc = 1.0 / std::sqrt(kPiHost * alpha);
+    dc -= sum_vaux[k] / (num_kpoints * volume);
+    dc *= -1.0;
+    if (nspin == 1) dc /= 2.0;
+    return dc;
+}
+
+void Context::apply_divcorr(const Plan& P, const ApplyIO& io, const double* occ_left)
+{
+    if (!div_enabled || P.n_app == 0) return;
+    launch_divcorr(io.phi, (const double*)d_phi_occ.p, occ_left, (const DivEntry*)P.d_divjobs.p, P.n_app,
+                   (const double*)d_fill.p, (const double*)d_dc.p, io.out, stream);
+}
+
+// W (FP64 real-space accumulators, one grid per applied band) -> FFT_forward -> gather with factor
+void Context::finish_W(const Plan& P, const ApplyIO& io, double factor)
+{
+    timed(3, 0, [&] { launch_fft3d(TCX_PREC_FP64, -1, d_W.p, P.n_app, geo, 1.0 / (double)N, stream); });
+    launch_gather_add((const double*)d_W.p, (const int*)d_gindex.p, (const OrbEntry*)P.d_appjobs.p, P.n_app,
+                      max_npw, io.out, N, factor, stream);
+}
+
+void Context::hf2x(const int* nb_, const double* phi, double* out, int prec, unsigned flags)
+{
+    check_apply_args(nb_, phi, out, prec);
+    std::vector<int> nb(nb_, nb_ + nk);
+    const Plan& P = get_plan(nb);
+    ApplyIO io = begin_apply(nb_, phi, out, flags, P);
+    if (P.n_app > 0) {
+        ensure_occR(prec);
+        const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
+        d_appR.ensure((size_t)P.n_app * N * es);
+        timed(3, 0, [&] { orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p, -1); });
+        d_W.ensure((size_t)P.n_app * N * 16);
+        launch_zero(d_W.p, (size_t)P.n_app * N * 16, stream);
+        d_pairbuf.ensure((size_t)std::max(1, P.max_launch_pairs) * N * es);
+        for (const Launch& L : P.launches) {
+            const PairEntry* pairs = (const PairEntry*)P.d_pairs.p + L.pair0;
+            const TargetEntry* targets = (const TargetEntry*)P.d_targets.p + L.target0;
+            timed(0, L.npairs, [&] {
+                launch_hf2x_A(prec, geo, d_appR.p, d_occR[prec].p, pairs, L.npairs, d_pairbuf.p, stream);
+            });
+            timed(1, L.npairs, [&] {
+                launch_hf2x_B(prec, geo, d_pairbuf.p, pairs, L.npairs, (const double*)P.d_kq.p, stream);
+            });
+            timed(2, L.npairs, [&] {
+                launch_hf2x_C(prec, geo, d_pairbuf.p, d_occR[prec].p, pairs, targets, L.ntargets,
+                              (const double*)d_fill.p, (double*)d_W.p, stream);
+            });
+        }
+        // two_body_factor, src/calc_hamiltonian_hf2x.cpp:29-30
+        double factor = -4.0 * kPiHost / (num_kpoints * volume);
+        if (nspin == 1) factor /= 2.0;
+        finish_W(P, io, factor);
+        apply_divcorr(P, io, (const double*)d_phi_occ.p);
+    }
+    end_apply(io);
+}
+
+}  // namespace tcx
+
+// ======================================================================= C ABI
+namespace {
+
+template <typename F>
+int guarded(tcx_context* ctx, F&& f)
+{
+    if (!ctx) return TCX_ERR_ARG;
+    try {
+        TCX_HIP_CHECK(hipSetDevice(ctx->impl.device));
+        f();
+        ctx->impl.err.clear();
+        return TCX_OK;
+    } catch (const Error& e) {
+        ctx->impl.err = e.what();
+        ctx->impl.pending.clear();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->impl.err = e.what();
+        return TCX_ERR_INTERNAL;
+    } catch (...) {
+        ctx->impl.err = "unknown error";
+        return TCX_ERR_INTERNAL;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcx_create(tcx_context** out, int device)
+{
+    if (!out) return TCX_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return TCX_ERR_HIP;
+    if (device < 0 || device >= ndev) return TCX_ERR_ARG;
+    tcx_context* c = new (std::nothrow) tcx_context();
+    if (!c) return TCX_ERR_NOMEM;
+    c->impl.device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->impl.stream) != hipSuccess) {
+        delete c;
+        return TCX_ERR_HIP;
+    }
+    *out = c;
+    return TCX_OK;
+}
+
+int tcx_destroy(tcx_context* ctx)
+{
+    if (!ctx) return TCX_ERR_ARG;
+    (void)hipSetDevice(ctx->impl.device);
+    (void)hipStreamSynchronize(ctx->impl.stream);
+    (void)hipStreamDestroy(ctx->impl.stream);
+    delete ctx;
+    return TCX_OK;
+}
+
+const char* tcx_last_error(const tcx_context* ctx)
+{
+    if (!ctx) return "null context";
+    return ctx->impl.err.c_str();
+}
+
+int tcx_synchronize(tcx_context* ctx)
+{
+    return guarded(ctx, [&] { TCX_HIP_CHECK(hipStreamSynchronize(ctx->impl.stream)); });
+}
+
+int tcx_set_cell(tcx_context* ctx, const double recip_rows[9], double volume, const int grid[3], int nspin, int method)
+{
+    return guarded(ctx, [&] { ctx->impl.set_cell(recip_rows, volume, grid, nspin, method); });
+}
+
+int tcx_set_kpoints(tcx_context* ctx, int nk, const double* kfrac, int num_kpoints)
+{
+    return guarded(ctx, [&] { ctx->impl.set_kpoints(nk, kfrac, num_kpoints); });
+}
+
+int tcx_set_basis(tcx_context* ctx, const int* npw, const int* gindex)
+{
+    return guarded(ctx, [&] { ctx->impl.set_basis(npw, gindex); });
+}
+
+int tcx_set_states(tcx_context* ctx, const int* nocc, const double* filling, const double* phi_occ,
+                   const double* chi_occ)
+{
+    return guarded(ctx, [&] { ctx->impl.set_states(nocc, filling, phi_occ, chi_occ); });
+}
+
+int tcx_set_jastrow_rpa(tcx_context* ctx, const double A[4], const double F[4])
+{
+    return guarded(ctx, [&] { ctx->impl.set_jastrow(A, F); });
+}
+
+int tcx_set_div_correction(tcx_context* ctx, int enabled, double alpha, const double* sum_of_vaux)
+{
+    return guarded(ctx, [&] { ctx->impl.set_div(enabled, alpha, sum_of_vaux); });
+}
+
+int tcx_hf2x(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision, unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.hf2x(nb, phi, h2phi, precision, flags); });
+}
+
+int tcx_tc2x(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision, unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.tc2x(nb, phi, h2phi, precision, flags); });
+}
+
+int tcx_tc3a1(tcx_context* ctx, const int* nb, const double* phi, double* h3phi, int precision, unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.tc3_local(0, nb, phi, h3phi, precision, flags); });
+}
+
+int tcx_tc3b1(tcx_context* ctx, const int* nb, const double* phi, double* h3phi, int precision, unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.tc3_local(1, nb, phi, h3phi, precision, flags); });
+}
+
+int tcx_set_max_pairs(tcx_context* ctx, long max_pairs)
+{
+    return guarded(ctx, [&] {
+        if (max_pairs < 1 || max_pairs > 65535) throw Error(TCX_ERR_ARG, "max_pairs must be in [1, 65535]");
+        ctx->impl.max_pairs = max_pairs;
+    });
+}
+
+int tcx_enable_timing(tcx_context* ctx, int on)
+{
+    return guarded(ctx, [&] {
+        ctx->impl.timing_on = on != 0;
+        for (int c = 0; c < 4; ++c) {
+            ctx->impl.t_ms[c] = 0;
+            ctx->impl.t_launches[c] = 0;
+            ctx->impl.t_pairs[c] = 0;
+        }
+    });
+}
+
+int tcx_get_timing(tcx_context* ctx, double out_ms[4], long out_launches[4], long out_pairs[4])
+{
+    return guarded(ctx, [&] {
+        ctx->impl.collect_timing();
+        for (int c = 0; c < 4; ++c) {
+            if (out_ms) out_ms[c] = ctx->impl.t_ms[c];
+            if (out_launches) out_launches[c] = ctx->impl.t_launches[c];
+            if (out_pairs) out_pairs[c] = ctx->impl.t_pairs[c];
+        }
+    });
+}
+
+int tcx_sum_of_vaux(const double recip[9], double volume, int nk, const double* kfrac, double* alpha_out,
+                    double* sum_out)
+{
+    if (!recip || !kfrac || !alpha_out || !sum_out || nk <= 0 || !(volume > 0)) return TCX_ERR_ARG;
+    // Restates Potentials::set_sum_of_Vaux_each (src/potentials_coulomb.cpp:60-161) for an
+    // identity-symmetry k set that is also the q set.
+    double alpha = std::cbrt(volume) / (2.0 * kPiHost);
+    alpha = alpha * alpha;
+    auto cart2 = [&](const double m[3]) {
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double c = recip[d] * m[0] + recip[3 + d] * m[1] + recip[6 + d] * m[2];
+            s += c * c;
+        }
+        return s;
+    };
+    for (int ik = 0; ik < nk; ++ik) {
+        double sum = 0.0;
+        bool match_any = false;
+        for (int iq = 0; iq < nk; ++iq) {
+            double kq[3];
+            for (int d = 0; d < 3; ++d) {
+                kq[d] = kfrac[3 * ik + d] - kfrac[3 * iq + d];
+                while (kq[d] < -1e-8) kq[d] += 1.0;
+                while (kq[d] > 1.0 - 1e-8) kq[d] -= 1.0;
+            }
+            const bool match = kq[0] * kq[0] + kq[1] * kq[1] + kq[2] * kq[2] < 1e-8;
+            match_any = match_any || match;
+            if (!match) {
+                const double k2 = cart2(kq);
+                sum += std::exp(-alpha * k2) / k2;
+            }
+            for (int shell = 1;; ++shell) {
+                double on_shell = 0.0;
+                for (int gx = -shell; gx <= shell; ++gx)
+                    for (int gy = -shell; gy <= shell; ++gy)
+                        for (int gz = -shell; gz <= shell; ++gz) {
+                            if (std::abs(gx) != shell && std::abs(gy) != shell && std::abs(gz) != shell) continue;
+                            const double m[3] = {kq[0] + gx, kq[1] + gy, kq[2] + gz};
+                            const double g2 = cart2(m);
+                            on_shell += std::exp(-alpha * g2) / g2;
+                        }
+                sum += on_shell;
+                if (on_shell < 1e-20) break;
+            }
+        }
+        sum *= 4.0 * kPiHost;
+        if (match_any) sum -= 4.0 * kPiHost * alpha;
+        sum_out[ik] = sum;
+    }
+    *alpha_out = alpha;
+    return TCX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// tcx_internal.hpp -- shared declarations of the HIP engine behind include/tcx.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/tcx.h"
+
+namespace tcx {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define TCX_HIP_CHECK(call)                                                                   \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::tcx::Error(e_ == hipErrorOutOfMemory ? TCX_ERR_NOMEM : TCX_ERR_HIP,       \
+                               std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+// Grid sizes (per axis) with compiled kernels.
+constexpr int kSizes[] = {6, 8, 12, 16, 20, 24, 32, 36, 40, 48};
+bool size_supported(int n);
+
+// Geometry passed by value to kernels.
+struct Geom {
+    int n1, n2, n3;
+    double b[9];   // reciprocal rows b_i: b[3*i + d]
+};
+
+// One (k,j,q,i) pair of a launch: indices into the real-space grid arrays.
+struct PairEntry {
+    int app;   // applied-orbital grid (k, j)
+    int occ;   // occupied-orbital grid (q, i); also indexes fillings
+    int kq;    // row of the launch's k-q table
+    int pad;
+};
+
+// One accumulation target (k, j) of a launch: pairs [pair0, pair0+npair).
+struct TargetEntry {
+    int app;
+    int pair0;
+    int npair;
+    int pad;
+};
+
+// One orbital scatter/gather job: coefficients <-> grid.  The grid slot of
+// job i of a launch is i (grids are consecutive from the base pointer).
+struct OrbEntry {
+    long long coef_off;   // offset (complex numbers) into the coefficient array
+    int npw;
+    int gidx_off;         // offset into the concatenated gindex array
+    int kidx;             // k-point (for gradients)
+    int pad;
+};
+
+// Divergence-correction job for one applied band (k, j).
+struct DivEntry {
+    long long app_off;    // coefficient offset of phi_kj (and of hphi_kj)
+    long long occ_off;    // coefficient offset of the occupied block at k
+    int npw;
+    int nocc;
+    int fill_off;         // offset of the fillings at k
+    int kidx;             // k-point: divergence factor dc[kidx]
+};
+
+// ------------------------------------------------------------- launchers
+// prec: TCX_PREC_FP64 (double grids) or TCX_PREC_FP32 (float grids).
+
+void launch_zero(void* p, size_t bytes, hipStream_t s);
+
+// scatter complex128 coefficients to double grids (grids pre-zeroed);
+// dir >= 0 multiplies by the Cartesian component dir of (k+G).
+void launch_scatter(const double* coef, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
+                    double* grids, long long ngrid_pts, int dir, const double* kcart, const Geom& g,
+                    hipStream_t s);
+
+// hphi[job] += factor * W[job.grid][gindex]   (W: double G-space grids)
+void launch_gather_add(const double* W, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
+                       double* out, long long ngrid_pts, double factor, hipStream_t s);
+
+void launch_convert_d2f(const double* in, float* out, long long ncomplex, hipStream_t s);
+
+// in-place 3D FFT of ngrids consecutive grids; sign -1 forward, +1 backward; result scaled by scale
+void launch_fft3d(int prec, int sign, void* grids, int ngrids, const Geom& g, double scale, hipStream_t s);
+
+// hf2x pair pipeline
+void launch_hf2x_A(int prec, const Geom& g, const void* phiR_app, const void* phiR_occ, const PairEntry* pairs,
+                   int npairs, void* buf, hipStream_t s);
+void launch_hf2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const double* kqcart,
+                   hipStream_t s);
+void launch_hf2x_C(int prec, const Geom& g, const void* buf, const void* phiR_occ, const PairEntry* pairs,
+                   const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s);
+
+// divergence correction: hphi += sum_b dc f_b <chi_b|phi_j> phi_b
+void launch_divcorr(const double* phi, const double* occ_right, const double* occ_left, const DivEntry* jobs,
+                    int njobs, const double* fill, const double* dc, double* out, hipStream_t s);
+
+}  // namespace tcx

@@ -1,0 +1,166 @@
+// tcx_terms.hpp -- the context object behind tcx_context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "tcx_internal.hpp"
+
+namespace tcx {
+
+constexpr double kPiHost = 3.14159265358979323846;
+
+// Device buffer that only grows.
+struct DBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            cap = o.cap;
+            o.p = nullptr;
+            o.cap = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void ensure(size_t bytes);
+    void release();
+};
+
+// One launch of the pair pipeline.
+struct Launch {
+    int pair0 = 0, npairs = 0;      // slice of Plan::pairs; slot p of the pair buffer = pair0 + p
+    int target0 = 0, ntargets = 0;  // slice of Plan::targets (pair indices are launch-local)
+    int kq0 = 0;
+};
+
+struct Plan {
+    std::vector<long long> app_first, app_coef;  // per k: first applied orbital, coefficient offset
+    int n_app = 0;
+    std::vector<PairEntry> pairs;
+    std::vector<TargetEntry> targets;
+    std::vector<double> kq;  // Cartesian k - q per (k,q) row
+    std::vector<Launch> launches;
+    int max_launch_pairs = 0;
+    DBuf d_pairs, d_targets, d_kq;
+    DBuf d_appjobs;  // OrbEntry per applied band (band order)
+    DBuf d_divjobs;  // DivEntry per applied band
+};
+
+// synchronous host -> device copy into a growing buffer
+void upload(DBuf& buf, const void* src, size_t bytes);
+
+struct ApplyIO {
+    bool host = true;
+    long long ncoef = 0;
+    const double* phi = nullptr;  // device
+    double* out = nullptr;        // device
+    double* out_host = nullptr;
+};
+
+struct PendingEvent {
+    int cls;
+    long pairs;
+    hipEvent_t a, b;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // geometry and state
+    bool cell_set = false, k_set = false, basis_set = false, states_set = false, jastrow_set = false;
+    Geom geo{};
+    long long N = 0;
+    double volume = 0.0;
+    int nspin = 1, method = TCX_METHOD_HF;
+    int nk = 0, num_kpoints = 0;
+    std::vector<double> kfrac, kcart;
+    std::vector<int> npw;
+    std::vector<long long> pw_off;
+    int max_npw = 0;
+    std::vector<int> nocc;
+    std::vector<long long> occ_first, occ_coef;
+    int n_occ = 0;
+    std::vector<double> fill;
+    double A[4] = {0, 0, 0, 0}, F[4] = {1, 1, 1, 1};
+    bool jastrow_zero = true;
+    bool div_enabled = false;
+    double alpha = 0.0;
+    std::vector<double> sum_vaux;
+
+    // device data
+    DBuf d_gindex, d_kcart, d_phi_occ, d_chi_occ, d_fill;
+    DBuf d_occR[2], d_chiR[2], d_gradR[2];  // per precision
+    bool occR_valid[2] = {false, false}, chiR_valid[2] = {false, false}, gradR_valid[2] = {false, false};
+    DBuf d_density;  // double2 [2][N], real space
+    bool density_valid = false;
+    DBuf d_occjobs;  // OrbEntry per occupied orbital
+    DBuf d_dc;       // divergence factor per k
+    DBuf d_phi_app, d_out, d_appR, d_appGrad, d_tmp64, d_W, d_pairbuf, d_ktab, d_work;
+
+    long max_pairs = 4096;
+    std::map<std::pair<std::vector<int>, long>, Plan> plans;
+
+    // timing
+    bool timing_on = false;
+    double t_ms[4] = {0, 0, 0, 0};
+    long t_launches[4] = {0, 0, 0, 0};
+    long t_pairs[4] = {0, 0, 0, 0};
+    std::vector<PendingEvent> pending;
+
+    ~Context()
+    {
+        for (auto& ev : pending) {
+            (void)hipEventDestroy(ev.a);
+            (void)hipEventDestroy(ev.b);
+        }
+    }
+
+    static void require(bool cond, int code, const char* what);
+    void timed(int cls, long pairs, const std::function<void()>& launch);
+    void collect_timing();
+
+    void set_cell(const double recip_rows[9], double vol, const int grid[3], int ns, int meth);
+    void set_kpoints(int nk, const double* kf, int numk);
+    void set_basis(const int* npw, const int* gindex);
+    void set_states(const int* nocc, const double* filling, const double* phi_occ, const double* chi_occ);
+    void set_jastrow(const double A[4], const double F[4]);
+    void set_div(int enabled, double alpha, const double* svaux);
+    void invalidate_states();
+
+    void orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n, int prec, void* out, int dir);
+    void ensure_occR(int prec);
+    void ensure_density();
+
+    const Plan& get_plan(const std::vector<int>& nb);
+    void check_apply_args(const int* nb, const double* phi, const double* out, int prec);
+    ApplyIO begin_apply(const int* nb, const double* phi, double* out, unsigned flags, const Plan& P);
+    void end_apply(const ApplyIO& io);
+    double div_factor(int k) const;
+    void apply_divcorr(const Plan& P, const ApplyIO& io, const double* occ_left);
+    void finish_W(const Plan& P, const ApplyIO& io, double factor);
+
+    void hf2x(const int* nb, const double* phi, double* out, int prec, unsigned flags);
+    void tc2x(const int* nb, const double* phi, double* out, int prec, unsigned flags);
+    void tc3_local(int which, const int* nb, const double* phi, double* out, int prec, unsigned flags);
+};
+
+}  // namespace tcx
+
+struct tcx_context {
+    tcx::Context impl;
+};

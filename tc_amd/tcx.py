@@ -1,0 +1,229 @@
+"""ctypes binding of libtcx.so (the C ABI declared in include/tcx.h).
+
+The term methods keep the reference's names (calc_hamiltonian::hf2x, tc2x,
+tc3a1, tc3b1; src/include/calc_hamiltonian.hpp:54-108) and semantics: they
+ACCUMULATE into the output array and raise on any non-zero status instead of
+aborting (the reference's error_messages::stop -> MPI_Abort).
+
+There is no fallback: if libtcx.so is missing or no GPU is visible, the
+constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtcx.so")
+
+TCX_OK = 0
+TCX_ERR_ARG = 1
+TCX_ERR_STATE = 2
+TCX_ERR_HIP = 3
+TCX_ERR_UNSUPPORTED = 4
+TCX_ERR_NOMEM = 5
+TCX_ERR_INTERNAL = 6
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+FLAG_DEVICE_PTRS = 1
+
+METHODS = {"HF": 0, "TC": 1, "BITC": 2}
+
+# Every entry point declared in include/tcx.h.
+EXPORTS = [
+    "tcx_create", "tcx_destroy", "tcx_last_error", "tcx_synchronize",
+    "tcx_set_cell", "tcx_set_kpoints", "tcx_set_basis", "tcx_set_states",
+    "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux",
+    "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1",
+    "tcx_set_max_pairs", "tcx_enable_timing", "tcx_get_timing",
+]
+
+
+class TcxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tcx error {code}: {msg}")
+        self.code = code
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+
+def load_library():
+    """Load libtcx.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m tc_amd.build` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.tcx_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.tcx_destroy.argtypes = [vp]
+    L.tcx_last_error.argtypes = [vp]
+    L.tcx_last_error.restype = C.c_char_p
+    L.tcx_synchronize.argtypes = [vp]
+    L.tcx_set_cell.argtypes = [vp, _dp, C.c_double, _ip, C.c_int, C.c_int]
+    L.tcx_set_kpoints.argtypes = [vp, C.c_int, _dp, C.c_int]
+    L.tcx_set_basis.argtypes = [vp, _ip, _ip]
+    L.tcx_set_states.argtypes = [vp, _ip, _dp, _dp, _dp]
+    L.tcx_set_jastrow_rpa.argtypes = [vp, _dp, _dp]
+    L.tcx_set_div_correction.argtypes = [vp, C.c_int, C.c_double, _dp]
+    L.tcx_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
+    for name in ("tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1"):
+        getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_int, C.c_uint]
+    L.tcx_set_max_pairs.argtypes = [vp, C.c_long]
+    L.tcx_enable_timing.argtypes = [vp, C.c_int]
+    L.tcx_get_timing.argtypes = [vp, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+    _lib = L
+    return L
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def sum_of_vaux(recip, volume, kfrac):
+    """alpha and sum_of_Vaux[k] (Potentials::set_sum_of_Vaux_each) from the library's host code."""
+    L = load_library()
+    r = np.ascontiguousarray(recip, dtype=np.float64).reshape(9)
+    kf = np.ascontiguousarray(kfrac, dtype=np.float64).reshape(-1)
+    nk = kf.size // 3
+    out = np.zeros(nk)
+    alpha = np.zeros(1)
+    st = L.tcx_sum_of_vaux(_ptr(r), float(volume), nk, _ptr(kf), _ptr(alpha), _ptr(out))
+    if st != TCX_OK:
+        raise TcxError(st, "tcx_sum_of_vaux")
+    return float(alpha[0]), out
+
+
+class Engine:
+    """One tcx_context on one GPU, loaded with a synthetic or caller-provided system."""
+
+    def __init__(self, device: int = 0):
+        self.L = load_library()
+        h = C.c_void_p()
+        st = self.L.tcx_create(C.byref(h), int(device))
+        if st != TCX_OK:
+            raise TcxError(st, "tcx_create failed (no GPU visible?)")
+        self.h = h
+        self._keep = []
+        self.sys = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tcx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != TCX_OK:
+            msg = self.L.tcx_last_error(self.h)
+            raise TcxError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    # --------------------------------------------------------------- state
+    def set_cell(self, recip, volume, grid, nspin=1, method="HF"):
+        r = np.ascontiguousarray(recip, dtype=np.float64).reshape(9)
+        g = np.ascontiguousarray(grid, dtype=np.int32)
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        self._check(self.L.tcx_set_cell(self.h, _ptr(r), float(volume), _ptr(g, _ip), int(nspin), m), "set_cell")
+
+    def set_kpoints(self, kfrac, num_kpoints=None):
+        kf = np.ascontiguousarray(kfrac, dtype=np.float64).reshape(-1)
+        nk = kf.size // 3
+        self._check(self.L.tcx_set_kpoints(self.h, nk, _ptr(kf), int(num_kpoints or nk)), "set_kpoints")
+
+    def set_basis(self, npw, gindex):
+        a = np.ascontiguousarray(npw, dtype=np.int32)
+        b = np.ascontiguousarray(gindex, dtype=np.int32)
+        self._check(self.L.tcx_set_basis(self.h, _ptr(a, _ip), _ptr(b, _ip)), "set_basis")
+
+    def set_states(self, nocc, filling, phi_occ, chi_occ=None):
+        a = np.ascontiguousarray(nocc, dtype=np.int32)
+        f = np.ascontiguousarray(filling, dtype=np.float64)
+        p = np.ascontiguousarray(phi_occ, dtype=np.complex128)
+        c = None if chi_occ is None else np.ascontiguousarray(chi_occ, dtype=np.complex128)
+        self._check(self.L.tcx_set_states(self.h, _ptr(a, _ip), _ptr(f), p.view(np.float64).ctypes.data_as(_dp),
+                                          None if c is None else c.view(np.float64).ctypes.data_as(_dp)),
+                    "set_states")
+
+    def set_jastrow_rpa(self, A, F):
+        a = np.ascontiguousarray(A, dtype=np.float64).reshape(4)
+        f = np.ascontiguousarray(F, dtype=np.float64).reshape(4)
+        self._check(self.L.tcx_set_jastrow_rpa(self.h, _ptr(a), _ptr(f)), "set_jastrow_rpa")
+
+    def set_div_correction(self, enabled, alpha=0.0, sum_vaux=None):
+        s = None if sum_vaux is None else np.ascontiguousarray(sum_vaux, dtype=np.float64)
+        self._check(self.L.tcx_set_div_correction(self.h, int(bool(enabled)), float(alpha),
+                                                  None if s is None else _ptr(s)), "set_div_correction")
+
+    def load_system(self, sys, div_corr=None):
+        """Push a tc_amd.system.System into the context."""
+        self.sys = sys
+        self.set_cell(sys.recip, sys.volume, sys.n, sys.nspin, sys.method)
+        self.set_kpoints(sys.kfrac, sys.num_kpoints)
+        self.set_basis(sys.npw, sys.gindex)
+        self.set_states(sys.nocc, sys.filling, sys.phi_occ, sys.chi_occ)
+        self.set_jastrow_rpa(sys.A, sys.F)
+        on = sys.div_corr if div_corr is None else div_corr
+        if on:
+            alpha, sv = sum_of_vaux(sys.recip, sys.volume, sys.kfrac)
+            self.set_div_correction(True, alpha, sv)
+        else:
+            self.set_div_correction(False)
+
+    # --------------------------------------------------------------- terms
+    def _term(self, name, nb, phi, out=None, precision=PREC_FP64):
+        nb = np.ascontiguousarray(nb, dtype=np.int32)
+        phi = np.ascontiguousarray(phi, dtype=np.complex128)
+        if out is None:
+            out = np.zeros_like(phi)
+        st = getattr(self.L, "tcx_" + name)(self.h, _ptr(nb, _ip), phi.ctypes.data, out.ctypes.data,
+                                           int(precision), 0)
+        self._check(st, name)
+        return out
+
+    def _term_device(self, name, nb, phi_ptr, out_ptr, precision):
+        nb = np.ascontiguousarray(nb, dtype=np.int32)
+        st = getattr(self.L, "tcx_" + name)(self.h, _ptr(nb, _ip), C.c_void_p(phi_ptr), C.c_void_p(out_ptr),
+                                           int(precision), FLAG_DEVICE_PTRS)
+        self._check(st, name)
+
+    def hf2x(self, nb, phi, out=None, precision=PREC_FP64):
+        return self._term("hf2x", nb, phi, out, precision)
+
+    def tc2x(self, nb, phi, out=None, precision=PREC_FP64):
+        return self._term("tc2x", nb, phi, out, precision)
+
+    def tc3a1(self, nb, phi, out=None, precision=PREC_FP64):
+        return self._term("tc3a1", nb, phi, out, precision)
+
+    def tc3b1(self, nb, phi, out=None, precision=PREC_FP64):
+        return self._term("tc3b1", nb, phi, out, precision)
+
+    # --------------------------------------------------------------- tuning / timing
+    def set_max_pairs(self, n):
+        self._check(self.L.tcx_set_max_pairs(self.h, int(n)), "set_max_pairs")
+
+    def enable_timing(self, on=True):
+        self._check(self.L.tcx_enable_timing(self.h, int(bool(on))), "enable_timing")
+
+    def get_timing(self):
+        ms = np.zeros(4)
+        la = (C.c_long * 4)()
+        pa = (C.c_long * 4)()
+        self._check(self.L.tcx_get_timing(self.h, _ptr(ms), la, pa), "get_timing")
+        return ms, np.array(la[:]), np.array(pa[:])
+
+    def synchronize(self):
+        self._check(self.L.tcx_synchronize(self.h), "synchronize")

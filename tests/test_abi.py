@@ -1,0 +1,60 @@
+"""The C-ABI library: builds, loads, exports what include/tcx.h declares, and
+its host-only pieces behave (no GPU calls here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from tc_amd import tcx
+from oracle import oracle as O
+from tc_amd import system as S
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "tcx.h")).read()
+    return sorted(set(re.findall(r"\b(tcx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(tcx.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = tcx.load_library()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+
+
+def test_library_is_gfx950_only():
+    data = open(tcx.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_sum_of_vaux_host_utility_matches_oracle():
+    for sys_ in (S.tiny_system(), S.heg_sc_config0()):
+        a1, s1 = tcx.sum_of_vaux(sys_.recip, sys_.volume, sys_.kfrac)
+        a2, s2 = O.sum_of_vaux(sys_.recip, sys_.volume, sys_.kfrac)
+        assert abs(a1 - a2) < 1e-15 * a2
+        assert np.abs(s1 - s2).max() < 1e-12 * np.abs(s2).max()
+
+
+def test_sum_of_vaux_rejects_bad_arguments():
+    L = tcx.load_library()
+    r = np.eye(3).reshape(9)
+    kf = np.zeros(3)
+    out = np.zeros(1)
+    alpha = np.zeros(1)
+    dp = ctypes.POINTER(ctypes.c_double)
+    st = L.tcx_sum_of_vaux(r.ctypes.data_as(dp), -1.0, 1, kf.ctypes.data_as(dp), alpha.ctypes.data_as(dp),
+                           out.ctypes.data_as(dp))
+    assert st == tcx.TCX_ERR_ARG
+
+
+def test_null_context_is_an_error_not_a_crash():
+    L = tcx.load_library()
+    assert L.tcx_destroy(None) == tcx.TCX_ERR_ARG
+    assert L.tcx_synchronize(None) == tcx.TCX_ERR_ARG
+    assert L.tcx_last_error(None) == b"null context"

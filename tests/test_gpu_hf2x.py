@@ -1,0 +1,138 @@
+"""GPU parity of hf2x (Fock exchange) through the C ABI against the FP64 oracle.
+
+Tolerances (written here, per result):
+  FP64 path : max |delta| <= 1e-10 * max |ref|  (FFT round-off only)
+  FP32 path : per band ||delta||/||ref|| <= 2e-5 and |delta E_x| <= 1e-6 Ha/cell
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_names
+from oracle import oracle as O
+from tc_amd import energy as E
+from tc_amd import system as S
+from tc_amd import tcx
+import make_golden as MG
+from test_oracle import plane_wave_system
+
+pytestmark = pytest.mark.gpu
+
+HF_GOLDEN = [n for n in golden_names() if n.startswith("hf_")]
+
+
+def per_band_rel(sys_, nb, got, ref):
+    bo = sys_.band_offsets(nb)
+    out = []
+    for ik in range(sys_.nk):
+        g = got[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+        r = ref[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+        out.extend(np.linalg.norm(g - r, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-300))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("name", HF_GOLDEN)
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
+def test_hf2x_golden(engine, golden, name, prec):
+    z = golden[name]
+    sys_ = MG.load_system(z)
+    engine.load_system(sys_)
+    got = engine.hf2x(z["nb"], z["phi"], precision=prec)
+    ref = z["hf2x"]
+    if prec == tcx.PREC_FP64:
+        assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+    else:
+        assert per_band_rel(sys_, z["nb"], got, ref).max() <= 2e-5
+    engine.load_system(sys_, div_corr=False)
+    got = engine.hf2x(z["nb"], z["phi"], precision=prec)
+    tol = 1e-10 if prec == tcx.PREC_FP64 else 2e-5
+    assert np.abs(got - z["hf2x_nodiv"]).max() <= tol * np.abs(z["hf2x_nodiv"]).max()
+
+
+def test_hf2x_plane_wave_closed_form_gpu(engine):
+    sys_ = plane_wave_system()
+    sys_.div_corr = False
+    engine.load_system(sys_)
+    nb = sys_.npw.copy()
+    phi = np.concatenate([np.eye(sys_.npw[k], dtype=np.complex128).reshape(-1) for k in range(sys_.nk)])
+    ref = O.Oracle(sys_).hf2x(nb, phi)
+    for prec, tol in ((tcx.PREC_FP64, 1e-11), (tcx.PREC_FP32, 2e-6)):
+        got = engine.hf2x(nb, phi, precision=prec)
+        assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), prec
+
+
+def test_hf2x_accumulates_and_is_plan_independent(engine):
+    sys_ = S.tiny_system(method="HF", n=(12, 12, 12), kmesh=(2, 2, 1), nbands=4, nocc=3)
+    engine.load_system(sys_)
+    nb, phi = S.applied_bands(sys_, nb=[4, 3, 2, 4], seed=21)
+    a = engine.hf2x(nb, phi)
+    b = engine.hf2x(nb, phi, out=a.copy())
+    assert np.abs(b - 2 * a).max() <= 1e-14 * np.abs(a).max()
+    for mp in (1, 7, 40):
+        engine.set_max_pairs(mp)
+        c = engine.hf2x(nb, phi)
+        assert np.abs(c - a).max() <= 1e-12 * np.abs(a).max(), mp
+    engine.set_max_pairs(4096)
+
+
+def test_hf2x_empty_and_ragged_bands(engine):
+    sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 2, 1), nbands=3, nocc=2)
+    engine.load_system(sys_)
+    nb, phi = S.applied_bands(sys_, nb=[0, 3, 0, 1], seed=22)
+    ref = O.Oracle(sys_).hf2x(nb, phi)
+    got = engine.hf2x(nb, phi)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+    nb0 = np.zeros(sys_.nk, np.int32)
+    empty = engine.hf2x(nb0, np.zeros(0, np.complex128))
+    assert empty.size == 0
+
+
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
+def test_hf2x_config0_heg_sc(engine, prec):
+    """HEG sc rs=4, 2x2x2 k, 8 bands (7 occupied), 24^3: Vx*psi per band and E_x."""
+    sys_ = S.heg_sc_config0()
+    engine.load_system(sys_)
+    nb = np.full(sys_.nk, 8, np.int32)
+    # apply to all 8 bands: the 7 occupied ones plus one more orthonormal band
+    full = S.build_system(sys_.lattice, sys_.n, (2, 2, 2), nbands=8, nocc=8, config=0)
+    phi = full.phi_occ
+    ref = O.Oracle(sys_).hf2x(nb, phi)
+    got = engine.hf2x(nb, phi, precision=prec)
+    rel = per_band_rel(sys_, nb, got, ref)
+    if prec == tcx.PREC_FP64:
+        assert rel.max() < 1e-11
+    else:
+        assert rel.max() < 2e-5
+    occ_got = engine.hf2x(sys_.nocc, sys_.phi_occ, precision=prec)
+    occ_ref = O.Oracle(sys_).hf2x(sys_.nocc, sys_.phi_occ)
+    ex_got = E.term_energy(sys_, occ_got, 2)
+    ex_ref = E.term_energy(sys_, occ_ref, 2)
+    assert abs(ex_got - ex_ref) < (1e-10 if prec == tcx.PREC_FP64 else 1e-6)
+
+
+def test_hf2x_device_pointers(engine):
+    torch = pytest.importorskip("torch")
+    sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2)
+    engine.load_system(sys_)
+    nb, phi = S.applied_bands(sys_, nb=[3, 2], seed=23)
+    ref = engine.hf2x(nb, phi)
+    d_phi = torch.from_numpy(phi.view(np.float64)).cuda()
+    d_out = torch.zeros_like(d_phi)
+    torch.cuda.synchronize()
+    engine._term_device("hf2x", nb, d_phi.data_ptr(), d_out.data_ptr(), tcx.PREC_FP64)
+    got = d_out.cpu().numpy().view(np.complex128)
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_errors_are_status_codes(engine):
+    sys_ = S.tiny_system(method="HF")
+    engine.load_system(sys_)
+    with pytest.raises(tcx.TcxError) as ei:
+        engine.hf2x(np.array([1, 1], np.int32), np.zeros(sys_.npw.sum(), np.complex128), precision=7)
+    assert ei.value.code == tcx.TCX_ERR_ARG
+    with pytest.raises(tcx.TcxError) as ei:
+        engine.set_cell(sys_.recip, sys_.volume, (10, 10, 10))
+    assert ei.value.code == tcx.TCX_ERR_UNSUPPORTED
+    engine.set_kpoints(sys_.kfrac)  # invalidates basis and states
+    with pytest.raises(tcx.TcxError) as ei:
+        engine.hf2x(np.array([1, 1], np.int32), np.zeros(sys_.npw.sum(), np.complex128))
+    assert ei.value.code == tcx.TCX_ERR_STATE
