@@ -81,7 +81,6 @@ def main():
               "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from tc_amd import energy as E
     from tc_amd import system as S
     from tc_amd import tcx
@@ -97,28 +96,28 @@ def main():
     eng.load_system(sys_)
     if args.max_pairs:
         eng.set_max_pairs(args.max_pairs)
-    d_phi = torch.from_numpy(phi.view(np.float64).copy()).cuda()
-    d_out32 = torch.zeros_like(d_phi)
-    d_out64 = torch.zeros_like(d_phi)
-    torch.cuda.synchronize()
+    # inputs and outputs resident in HBM (the library's own allocations)
+    d_phi = eng.to_device(phi)
+    d_out32 = eng.device_buffer(phi.nbytes)
+    d_out64 = eng.device_buffer(phi.nbytes)
+    d_scratch = eng.device_buffer(phi.nbytes)
 
     def step(prec, out):
-        eng._term_device("hf2x", nb, d_phi.data_ptr(), out.data_ptr(), prec)
+        eng._term_device("hf2x", nb, d_phi.ptr, out.ptr, prec)
 
     log(f"bench: warmup {args.warmup} FP32-split step(s)")
     for _ in range(args.warmup):
-        d_out32.zero_()
-        torch.cuda.synchronize()
+        d_out32.zero()
         step(tcx.PREC_FP32, d_out32)
 
     log(f"bench: timing {args.steps} FP32-split step(s)")
     eng.enable_timing(True)
-    torch.cuda.synchronize()
+    eng.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         step(tcx.PREC_FP32, d_out32)
         log(f"bench: step {i + 1}/{args.steps} done")
-    torch.cuda.synchronize()
+    eng.synchronize()
     t1 = time.perf_counter()
     ms, launches, kpairs = eng.get_timing()
     eng.enable_timing(False)
@@ -137,25 +136,20 @@ def main():
 
     # FP64 apply for the ratio, then parity of the two outputs
     log(f"bench: FP64 apply x{args.fp64_steps}")
-    d_out64.zero_()
-    torch.cuda.synchronize()
+    d_out64.zero()
     step(tcx.PREC_FP64, d_out64)  # also the parity reference
-    torch.cuda.synchronize()
-    d_scratch = torch.zeros_like(d_out64)
-    torch.cuda.synchronize()
+    eng.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.fp64_steps):
         step(tcx.PREC_FP64, d_scratch)
-    torch.cuda.synchronize()
+    eng.synchronize()
     fp64_s = (time.perf_counter() - t0) / args.fp64_steps
     value64 = pairs_per_step / fp64_s
 
-    d_out32.zero_()
-    torch.cuda.synchronize()
+    d_out32.zero()
     step(tcx.PREC_FP32, d_out32)
-    torch.cuda.synchronize()
-    h32 = d_out32.cpu().numpy().view(np.complex128)
-    h64 = d_out64.cpu().numpy().view(np.complex128)
+    h32 = d_out32.download(np.complex128)
+    h64 = d_out64.download(np.complex128)
     bo = sys_.band_offsets(nb)
     rel = []
     for ik in range(sys_.nk):
@@ -208,6 +202,8 @@ def main():
         },
     }
     print(json.dumps(out), flush=True)
+    for b in (d_phi, d_out32, d_out64, d_scratch):
+        b.free()
     eng.close()
 
 

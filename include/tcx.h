@@ -130,6 +130,18 @@ int tcx_tc3a1(tcx_context *ctx, const int *nb, const double *phi, double *h3phi,
 int tcx_tc3b1(tcx_context *ctx, const int *nb, const double *phi, double *h3phi,
               int precision, unsigned flags);
 
+/* ---------------------------------------------------------------- device buffers */
+/* No counterpart in the reference (which keeps everything in host memory).
+ * For callers that hold no HIP runtime of their own and want inputs and
+ * outputs resident in HBM for TCX_FLAG_DEVICE_PTRS: allocate on the
+ * context's device, copy (kind 0 host->device, 1 device->host,
+ * 2 device->device, ordered on the context's stream, returning after the
+ * copy completes), and free. */
+int tcx_device_alloc(tcx_context *ctx, unsigned long long bytes, void **ptr);
+int tcx_device_free(tcx_context *ctx, void *ptr);
+int tcx_memcpy(tcx_context *ctx, void *dst, const void *src, unsigned long long bytes, int kind);
+int tcx_memset_zero(tcx_context *ctx, void *ptr, unsigned long long bytes);
+
 /* ---------------------------------------------------------------- tuning / timing */
 /* Upper bound of (k,j,q,i) pairs held in flight per launch (pair-buffer size). */
 int tcx_set_max_pairs(tcx_context *ctx, long max_pairs);

@@ -26,6 +26,8 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     "kernels_fft.hip",
     "kernels_hf2x.hip",
+    "kernels_tc2x.hip",
+    "kernels_local.hip",
     "tcx_api.cpp",
     "tcx_tc.cpp",
 ]

@@ -43,13 +43,14 @@ void launch_zero(void* p, size_t bytes, hipStream_t s)
 // ----------------------------------------------------------------- scatter / gather
 __global__ __launch_bounds__(256) void k_scatter(const double2* __restrict__ coef, const int* __restrict__ gindex,
                                                  const OrbEntry* __restrict__ jobs, double2* __restrict__ grids,
-                                                 long long ngrid, int dir, const double* __restrict__ kcart, Geom g)
+                                                 long long ngrid, const double* __restrict__ kcart, Geom g)
 {
     const OrbEntry e = jobs[blockIdx.y];
     const int l = blockIdx.x * 256 + threadIdx.x;
     if (l >= e.npw) return;
     const int idx = gindex[e.gidx_off + l];
     double2 c = coef[e.coef_off + l];
+    const int dir = e.dir;
     if (dir >= 0) {
         const int i1 = idx % g.n1, i2 = (idx / g.n1) % g.n2, i3 = idx / (g.n1 * g.n2);
         const double kg = kcart[3 * e.kidx + dir] + g.b[dir] * centered(i1, g.n1) +
@@ -61,13 +62,13 @@ __global__ __launch_bounds__(256) void k_scatter(const double2* __restrict__ coe
 }
 
 void launch_scatter(const double* coef, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
-                    double* grids, long long ngrid, int dir, const double* kcart, const Geom& g, hipStream_t s)
+                    double* grids, long long ngrid, const double* kcart, const Geom& g, hipStream_t s)
 {
     if (njobs == 0 || max_npw == 0) return;
     for (int j0 = 0; j0 < njobs; j0 += 65535) {
         const int nj = njobs - j0 < 65535 ? njobs - j0 : 65535;
         hipLaunchKernelGGL(k_scatter, dim3((max_npw + 255) / 256, nj), dim3(256), 0, s, (const double2*)coef,
-                           gindex, jobs + j0, (double2*)grids + (size_t)j0 * ngrid, ngrid, dir, kcart, g);
+                           gindex, jobs + j0, (double2*)grids + (size_t)j0 * ngrid, ngrid, kcart, g);
         TCX_HIP_CHECK(hipGetLastError());
     }
 }

@@ -1,0 +1,269 @@
+// kernels_tc2x.hip -- fused pair pipeline of the TC/BITC two-body exchange (tc2x).
+//
+// Per pair (k,j | q,i) the reference computes (src/calc_hamiltonian_tc2x.cpp:253-401)
+//   rho~      = FFT_forward(conj(chi_qi) phi_kj)                         (:350-351)
+//   sigma~_d  = FFT_forward(conj(chi_qi) g_kj,d),  g_kj,d = FFT_backward((k+G)_d phi_kj) (:372-376)
+//   a_d       = FFT_backward(rho~ du_d),  du_d = (k-q+G)_d u(k-q+G)       (:354-358)
+//   b         = FFT_backward(rho~ V2 + sum_d sigma~_d du_d), V2 = tc_2body(k-q+G) (:379-386)
+//   W_kj     += f_qi [ b phi_qi - sum_d a_d g_qi,d ],  g_qi,d = FFT_backward((q+G)_d phi_qi) (:363-369, :387)
+// (chi = phi for TC).  Eight 3D FFTs per pair, done here as three streaming passes
+// over a 4-channel pair record [rho, sigma_0, sigma_1, sigma_2] -> [b, a_0, a_1, a_2]:
+//   A (z-pencils): form the 4 channels from cached real-space orbitals, forward FFT along z;
+//   B (xy-planes): forward FFT along y, x; combine the channels with the kernel table;
+//                  inverse FFT along x, y (4 padded LDS tiles);
+//   C (z-pencils): inverse FFT along z, multiply by phi_qi / g_qi,d, accumulate per target in FP64.
+#include "jastrow_dev.hpp"
+#include "kernels_common.hpp"
+#include "size_switch.hpp"
+
+namespace tcx {
+
+// ----------------------------------------------------------------- kernel table (V2, u) per k-q row
+template <typename T>
+__global__ __launch_bounds__(256) void k_tc_table(cpx<T>* __restrict__ table, const double* __restrict__ kqcart,
+                                                  int row0, Geom g, JastrowRPA jp, long long N)
+{
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= N) return;
+    const int row = blockIdx.y;
+    const int i1 = (int)(e % g.n1), i2 = (int)((e / g.n1) % g.n2), i3 = (int)(e / ((long long)g.n1 * g.n2));
+    const int m1 = centered(i1, g.n1), m2 = centered(i2, g.n2), m3 = centered(i3, g.n3);
+    const double* kq = kqcart + 3 * (row0 + row);
+    double G2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double c = kq[d] + g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
+        G2 += c * c;
+    }
+    table[(long long)row * N + e] = cpx<T>{static_cast<T>(jastrow_tc2body(jp, G2)), static_cast<T>(jastrow_uk(jp, G2))};
+}
+
+// ----------------------------------------------------------------- pass A
+template <typename T, int NZ>
+__global__ __launch_bounds__(kPencilBlock) void k_tc2x_A(const cpx<T>* __restrict__ appR,
+                                                         const cpx<T>* __restrict__ appGrad,
+                                                         const cpx<T>* __restrict__ chiR,
+                                                         const PairEntry* __restrict__ pairs,
+                                                         cpx<T>* __restrict__ buf, int nxy)
+{
+    const int xy = blockIdx.x * kPencilBlock + threadIdx.x;
+    if (xy >= nxy) return;
+    const PairEntry pe = pairs[blockIdx.y];
+    const size_t ng = (size_t)nxy * NZ;
+    cpx<T> x[NZ];
+    load_pencil<T, NZ>(x, chiR + (size_t)pe.occ * ng + xy, nxy);
+    cpx<T>* out = buf + (size_t)blockIdx.y * 4 * ng + xy;
+    for (int c = 0; c < 4; ++c) {
+        const cpx<T>* src = c == 0 ? appR + (size_t)pe.app * ng + xy : appGrad + ((size_t)pe.app * 3 + (c - 1)) * ng + xy;
+        cpx<T> v[NZ];
+        static_for<0, NZ>([&](auto zc) {
+            constexpr int z = decltype(zc)::value;
+            v[z] = cmulc(x[z], src[(size_t)z * nxy]);
+        });
+        fft<T, NZ, -1>(v);
+        store_pencil<T, NZ>(v, out + (size_t)c * ng, nxy);
+    }
+}
+
+// ----------------------------------------------------------------- pass B
+template <typename T, int NP>
+struct Tiles4 {
+    PlaneTile<T, NP> t[4];
+};
+
+template <typename T, int NP>
+__global__ __launch_bounds__(kPlaneBlock) void k_tc2x_B(cpx<T>* __restrict__ buf, const PairEntry* __restrict__ pairs,
+                                                        const cpx<T>* __restrict__ table, int kq0,
+                                                        const double* __restrict__ kqcart, int nz, Geom g,
+                                                        double invN)
+{
+    __shared__ Tiles4<T, NP> S;
+    const int t = threadIdx.x;
+    const int z = blockIdx.x;
+    const size_t ng = (size_t)NP * NP * nz;
+    cpx<T>* base = buf + (size_t)blockIdx.y * 4 * ng + (size_t)z * NP * NP;
+    cpx<T> v[NP];
+    if (t < NP) {
+        for (int c = 0; c < 4; ++c) {
+            load_pencil<T, NP>(v, base + c * ng + t, NP);
+            fft<T, NP, -1>(v);
+            tile_put_col<T, NP>(S.t[c], v, t);
+        }
+    }
+    __syncthreads();
+    if (t < NP) {
+        for (int c = 0; c < 4; ++c) {
+            tile_get_row<T, NP>(S.t[c], v, t);
+            fft<T, NP, -1>(v);
+            tile_put_row<T, NP>(S.t[c], v, t);
+        }
+        // combine at frequency (x, y=t, z); this thread owns row t of every tile
+        const PairEntry pe = pairs[blockIdx.y];
+        const double* kq = kqcart + 3 * pe.kq;
+        const cpx<T>* tab = table + (size_t)(pe.kq - kq0) * ng + (size_t)z * NP * NP + (size_t)t * NP;
+        const int my = centered(t, NP), mz = centered(z, nz);
+        const double bx = kq[0] + g.b[3] * my + g.b[6] * mz;
+        const double by = kq[1] + g.b[4] * my + g.b[7] * mz;
+        const double bz = kq[2] + g.b[5] * my + g.b[8] * mz;
+        static_for<0, NP>([&](auto xc) {
+            constexpr int x = decltype(xc)::value;
+            constexpr int mx = x <= NP / 2 ? x : x - NP;
+            const cpx<T> vu = tab[x];  // (V2, u)
+            const double us = (double)vu.im * invN;
+            const T du0 = static_cast<T>((bx + g.b[0] * mx) * us);
+            const T du1 = static_cast<T>((by + g.b[1] * mx) * us);
+            const T du2 = static_cast<T>((bz + g.b[2] * mx) * us);
+            const T v2 = static_cast<T>((double)vu.re * invN);
+            const cpx<T> r = S.t[0].a[t][x];
+            const cpx<T> s0 = S.t[1].a[t][x], s1 = S.t[2].a[t][x], s2 = S.t[3].a[t][x];
+            S.t[0].a[t][x] = scale(r, v2) + scale(s0, du0) + scale(s1, du1) + scale(s2, du2);
+            S.t[1].a[t][x] = scale(r, du0);
+            S.t[2].a[t][x] = scale(r, du1);
+            S.t[3].a[t][x] = scale(r, du2);
+        });
+        for (int c = 0; c < 4; ++c) {
+            tile_get_row<T, NP>(S.t[c], v, t);
+            fft<T, NP, 1>(v);
+            tile_put_row<T, NP>(S.t[c], v, t);
+        }
+    }
+    __syncthreads();
+    if (t < NP) {
+        for (int c = 0; c < 4; ++c) {
+            tile_get_col<T, NP>(S.t[c], v, t);
+            fft<T, NP, 1>(v);
+            store_pencil<T, NP>(v, base + c * ng + t, NP);
+        }
+    }
+}
+
+// ----------------------------------------------------------------- pass C
+template <typename T, int NZ>
+__global__ __launch_bounds__(kPencilBlock) void k_tc2x_C(const cpx<T>* __restrict__ buf,
+                                                         const cpx<T>* __restrict__ occR,
+                                                         const cpx<T>* __restrict__ gradR,
+                                                         const PairEntry* __restrict__ pairs,
+                                                         const TargetEntry* __restrict__ targets,
+                                                         const double* __restrict__ fill,
+                                                         cpx<double>* __restrict__ W, int nxy)
+{
+    const int xy = blockIdx.x * kPencilBlock + threadIdx.x;
+    if (xy >= nxy) return;
+    const TargetEntry te = targets[blockIdx.y];
+    const size_t ng = (size_t)nxy * NZ;
+    cpx<double> acc[NZ];
+    static_for<0, NZ>([&](auto zc) { acc[decltype(zc)::value] = cpx<double>{0.0, 0.0}; });
+    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
+        const int occ = pairs[pp].occ;
+        const double f = fill[occ];
+        for (int c = 0; c < 4; ++c) {
+            cpx<T> v[NZ];
+            load_pencil<T, NZ>(v, buf + ((size_t)pp * 4 + c) * ng + xy, nxy);
+            fft<T, NZ, 1>(v);
+            // channel 0: b * phi_qi (+f); channels 1..3: a_d * g_qi,d (-f)   [tc2x.cpp:363-369, :387-388]
+            const cpx<T>* m = c == 0 ? occR + (size_t)occ * ng + xy : gradR + ((size_t)occ * 3 + (c - 1)) * ng + xy;
+            const double s = c == 0 ? f : -f;
+            static_for<0, NZ>([&](auto zc) {
+                constexpr int z = decltype(zc)::value;
+                const cpx<T> prod = v[z] * m[(size_t)z * nxy];
+                acc[z].re += s * static_cast<double>(prod.re);
+                acc[z].im += s * static_cast<double>(prod.im);
+            });
+        }
+    }
+    cpx<double>* w = W + (size_t)te.app * ng + xy;
+    static_for<0, NZ>([&](auto zc) {
+        constexpr int z = decltype(zc)::value;
+        w[(size_t)z * nxy] = w[(size_t)z * nxy] + acc[z];
+    });
+}
+
+// ----------------------------------------------------------------- launchers
+void launch_tc_table(int prec, void* table, const double* kqcart, int row0, int nrows, const Geom& g,
+                     const JastrowRPA& jp, hipStream_t s)
+{
+    if (nrows == 0) return;
+    const long long N = (long long)g.n1 * g.n2 * g.n3;
+    const dim3 grid((unsigned)((N + 255) / 256), nrows);
+    if (prec == TCX_PREC_FP64)
+        hipLaunchKernelGGL(k_tc_table<double>, grid, dim3(256), 0, s, (cpx<double>*)table, kqcart, row0, g, jp, N);
+    else
+        hipLaunchKernelGGL(k_tc_table<float>, grid, dim3(256), 0, s, (cpx<float>*)table, kqcart, row0, g, jp, N);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+static void tc2x_A_impl(const Geom& g, const void* appR, const void* appGrad, const void* chiR, const PairEntry* pairs,
+                        int npairs, void* buf, hipStream_t s)
+{
+    const int nxy = g.n1 * g.n2;
+    const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, npairs);
+#define TCX_A(NZ)                                                                                              \
+    hipLaunchKernelGGL((k_tc2x_A<T, NZ>), grid, dim3(kPencilBlock), 0, s, (const cpx<T>*)appR,                 \
+                       (const cpx<T>*)appGrad, (const cpx<T>*)chiR, pairs, (cpx<T>*)buf, nxy);
+    TCX_SIZE_SWITCH(g.n3, TCX_A)
+#undef TCX_A
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tc2x_A(int prec, const Geom& g, const void* appR, const void* appGrad, const void* chiR,
+                   const PairEntry* pairs, int npairs, void* buf, hipStream_t s)
+{
+    if (npairs == 0) return;
+    if (npairs > 65535) throw Error(TCX_ERR_INTERNAL, "tc2x: too many pairs in one launch");
+    if (prec == TCX_PREC_FP64)
+        tc2x_A_impl<double>(g, appR, appGrad, chiR, pairs, npairs, buf, s);
+    else
+        tc2x_A_impl<float>(g, appR, appGrad, chiR, pairs, npairs, buf, s);
+}
+
+template <typename T>
+static void tc2x_B_impl(const Geom& g, void* buf, const PairEntry* pairs, int npairs, const void* table, int kq0,
+                        const double* kqcart, hipStream_t s)
+{
+    if (g.n1 != g.n2) throw Error(TCX_ERR_UNSUPPORTED, "FFT grid must have n1 == n2");
+    const double invN = 1.0 / ((double)g.n1 * g.n2 * g.n3);
+#define TCX_B(NP)                                                                                        \
+    hipLaunchKernelGGL((k_tc2x_B<T, NP>), dim3(g.n3, npairs), dim3(kPlaneBlock), 0, s, (cpx<T>*)buf,     \
+                       pairs, (const cpx<T>*)table, kq0, kqcart, g.n3, g, invN);
+    TCX_SIZE_SWITCH(g.n1, TCX_B)
+#undef TCX_B
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tc2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const void* table, int kq0,
+                   const double* kqcart, hipStream_t s)
+{
+    if (npairs == 0) return;
+    if (prec == TCX_PREC_FP64)
+        tc2x_B_impl<double>(g, buf, pairs, npairs, table, kq0, kqcart, s);
+    else
+        tc2x_B_impl<float>(g, buf, pairs, npairs, table, kq0, kqcart, s);
+}
+
+template <typename T>
+static void tc2x_C_impl(const Geom& g, const void* buf, const void* occR, const void* gradR, const PairEntry* pairs,
+                        const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s)
+{
+    const int nxy = g.n1 * g.n2;
+    const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, ntargets);
+#define TCX_C(NZ)                                                                                           \
+    hipLaunchKernelGGL((k_tc2x_C<T, NZ>), grid, dim3(kPencilBlock), 0, s, (const cpx<T>*)buf,               \
+                       (const cpx<T>*)occR, (const cpx<T>*)gradR, pairs, targets, fill, (cpx<double>*)W, nxy);
+    TCX_SIZE_SWITCH(g.n3, TCX_C)
+#undef TCX_C
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tc2x_C(int prec, const Geom& g, const void* buf, const void* occR, const void* gradR,
+                   const PairEntry* pairs, const TargetEntry* targets, int ntargets, const double* fill, double* W,
+                   hipStream_t s)
+{
+    if (ntargets == 0) return;
+    if (ntargets > 65535) throw Error(TCX_ERR_INTERNAL, "tc2x: too many targets in one launch");
+    if (prec == TCX_PREC_FP64)
+        tc2x_C_impl<double>(g, buf, occR, gradR, pairs, targets, ntargets, fill, W, s);
+    else
+        tc2x_C_impl<float>(g, buf, occR, gradR, pairs, targets, ntargets, fill, W, s);
+}
+
+}  // namespace tcx

@@ -189,10 +189,15 @@ void Context::set_states(const int* nocc_, const double* filling, const double* 
         TCX_HIP_CHECK(hipMemcpy(d_chi_occ.p, chi_occ, cb, hipMemcpyHostToDevice));
     }
     upload(d_fill, fill.data(), fill.size() * sizeof(double));
-    std::vector<OrbEntry> jobs;
+    std::vector<OrbEntry> jobs, gjobs;
     for (int q = 0; q < nk; ++q)
-        for (int i = 0; i < nocc[q]; ++i) jobs.push_back({occ_coef[q] + (long long)i * npw[q], npw[q], (int)pw_off[q], q, 0});
+        for (int i = 0; i < nocc[q]; ++i) {
+            const OrbEntry e{occ_coef[q] + (long long)i * npw[q], npw[q], (int)pw_off[q], q, -1};
+            jobs.push_back(e);
+            for (int d = 0; d < 3; ++d) gjobs.push_back({e.coef_off, e.npw, e.gidx_off, q, d});
+        }
     upload(d_occjobs, jobs.data(), jobs.size() * sizeof(OrbEntry));
+    upload(d_occgradjobs, gjobs.data(), gjobs.size() * sizeof(OrbEntry));
     states_set = true;
     invalidate_states();
 }
@@ -231,8 +236,8 @@ void Context::set_div(int enabled, double alpha_, const double* svaux)
 // ------------------------------------------------------------------ real-space grids
 // Transforms ncoef-addressed orbitals (complex128, G space) to real space in
 // precision prec: FP64 scatter + FP64 inverse FFT, then rounding to float for
-// FP32.  dir >= 0 multiplies by (k+G)_dir first (gradient components).
-void Context::orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n, int prec, void* out, int dir)
+// FP32.  Jobs with dir >= 0 multiply by (k+G)_dir first (gradient components).
+void Context::orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n, int prec, void* out)
 {
     if (n == 0) return;
     const size_t g64 = (size_t)N * 16;
@@ -247,8 +252,8 @@ void Context::orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n
             dst64 = (double*)d_tmp64.p;
         }
         launch_zero(dst64, g64 * nc, stream);
-        launch_scatter(coef, (const int*)d_gindex.p, d_jobs + c0, nc, max_npw, dst64, N, dir,
-                       (const double*)d_kcart.p, geo, stream);
+        launch_scatter(coef, (const int*)d_gindex.p, d_jobs + c0, nc, max_npw, dst64, N, (const double*)d_kcart.p,
+                       geo, stream);
         launch_fft3d(TCX_PREC_FP64, +1, dst64, nc, geo, 1.0, stream);
         if (prec == TCX_PREC_FP32)
             launch_convert_d2f(dst64, (float*)out + (size_t)c0 * N * 2, (long long)nc * N, stream);
@@ -266,8 +271,26 @@ void Context::ensure_occR(int prec)
     if (occR_valid[prec]) return;
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
     d_occR[prec].ensure(std::max<size_t>((size_t)n_occ * N * es, 16));
-    orbitals_to_real((const double*)d_phi_occ.p, (const OrbEntry*)d_occjobs.p, n_occ, prec, d_occR[prec].p, -1);
+    orbitals_to_real((const double*)d_phi_occ.p, (const OrbEntry*)d_occjobs.p, n_occ, prec, d_occR[prec].p);
     occR_valid[prec] = true;
+}
+
+void Context::ensure_chiR(int prec)
+{
+    if (chiR_valid[prec]) return;
+    const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
+    d_chiR[prec].ensure(std::max<size_t>((size_t)n_occ * N * es, 16));
+    orbitals_to_real((const double*)d_chi_occ.p, (const OrbEntry*)d_occjobs.p, n_occ, prec, d_chiR[prec].p);
+    chiR_valid[prec] = true;
+}
+
+void Context::ensure_gradR(int prec)
+{
+    if (gradR_valid[prec]) return;
+    const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
+    d_gradR[prec].ensure(std::max<size_t>((size_t)3 * n_occ * N * es, 16));
+    orbitals_to_real((const double*)d_phi_occ.p, (const OrbEntry*)d_occgradjobs.p, 3 * n_occ, prec, d_gradR[prec].p);
+    gradR_valid[prec] = true;
 }
 
 // ------------------------------------------------------------------ launch plans
@@ -300,6 +323,7 @@ const Plan& Context::get_plan(const std::vector<int>& nb)
             L.pair0 = (int)P.pairs.size();
             L.target0 = (int)P.targets.size();
             L.kq0 = (int)P.kq.size() / 3;
+            L.nkq = (k1 - k0) * (q1 - q0);
             for (int k = k0; k < k1; ++k)
                 for (int q = q0; q < q1; ++q)
                     for (int d = 0; d < 3; ++d) P.kq.push_back(kcart[3 * k + d] - kcart[3 * q + d]);
@@ -328,14 +352,17 @@ const Plan& Context::get_plan(const std::vector<int>& nb)
     upload(P.d_targets, P.targets.data(), P.targets.size() * sizeof(TargetEntry));
     upload(P.d_kq, P.kq.data(), P.kq.size() * sizeof(double));
     // one scatter/gather job and one divergence job per applied band, in band order
-    std::vector<OrbEntry> appjobs;
+    std::vector<OrbEntry> appjobs, appgradjobs;
     std::vector<DivEntry> divjobs;
     for (int k = 0; k < nk; ++k)
         for (int j = 0; j < nb[k]; ++j) {
-            appjobs.push_back({P.app_coef[k] + (long long)j * npw[k], npw[k], (int)pw_off[k], k, 0});
-            divjobs.push_back({P.app_coef[k] + (long long)j * npw[k], occ_coef[k], npw[k], nocc[k], (int)occ_first[k], k});
+            const OrbEntry e{P.app_coef[k] + (long long)j * npw[k], npw[k], (int)pw_off[k], k, -1};
+            appjobs.push_back(e);
+            for (int d = 0; d < 3; ++d) appgradjobs.push_back({e.coef_off, e.npw, e.gidx_off, k, d});
+            divjobs.push_back({e.coef_off, occ_coef[k], npw[k], nocc[k], (int)occ_first[k], k});
         }
     upload(P.d_appjobs, appjobs.data(), appjobs.size() * sizeof(OrbEntry));
+    upload(P.d_appgradjobs, appgradjobs.data(), appgradjobs.size() * sizeof(OrbEntry));
     upload(P.d_divjobs, divjobs.data(), divjobs.size() * sizeof(DivEntry));
     auto res = plans.emplace(key, std::move(P));
     return res.first->second;
@@ -419,7 +446,7 @@ void Context::hf2x(const int* nb_, const double* phi, double* out, int prec, uns
         ensure_occR(prec);
         const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
         d_appR.ensure((size_t)P.n_app * N * es);
-        timed(3, 0, [&] { orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p, -1); });
+        timed(3, 0, [&] { orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p); });
         d_W.ensure((size_t)P.n_app * N * 16);
         launch_zero(d_W.p, (size_t)P.n_app * N * 16, stream);
         d_pairbuf.ensure((size_t)std::max(1, P.max_launch_pairs) * N * es);
@@ -565,6 +592,48 @@ int tcx_tc3a1(tcx_context* ctx, const int* nb, const double* phi, double* h3phi,
 int tcx_tc3b1(tcx_context* ctx, const int* nb, const double* phi, double* h3phi, int precision, unsigned flags)
 {
     return guarded(ctx, [&] { ctx->impl.tc3_local(1, nb, phi, h3phi, precision, flags); });
+}
+
+int tcx_device_alloc(tcx_context* ctx, unsigned long long bytes, void** ptr)
+{
+    if (!ptr) return TCX_ERR_ARG;
+    *ptr = nullptr;
+    return guarded(ctx, [&] {
+        void* p = nullptr;
+        TCX_HIP_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+        *ptr = p;
+    });
+}
+
+int tcx_device_free(tcx_context* ctx, void* ptr)
+{
+    return guarded(ctx, [&] {
+        TCX_HIP_CHECK(hipStreamSynchronize(ctx->impl.stream));
+        if (ptr) TCX_HIP_CHECK(hipFree(ptr));
+    });
+}
+
+int tcx_memcpy(tcx_context* ctx, void* dst, const void* src, unsigned long long bytes, int kind)
+{
+    return guarded(ctx, [&] {
+        if (kind < 0 || kind > 2) throw Error(TCX_ERR_ARG, "memcpy: kind must be 0, 1 or 2");
+        if (bytes == 0) return;
+        if (!dst || !src) throw Error(TCX_ERR_ARG, "memcpy: null pointer");
+        const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                          : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+        TCX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, ctx->impl.stream));
+        TCX_HIP_CHECK(hipStreamSynchronize(ctx->impl.stream));
+    });
+}
+
+int tcx_memset_zero(tcx_context* ctx, void* ptr, unsigned long long bytes)
+{
+    return guarded(ctx, [&] {
+        if (bytes == 0) return;
+        if (!ptr) throw Error(TCX_ERR_ARG, "memset_zero: null pointer");
+        launch_zero(ptr, bytes, ctx->impl.stream);
+        TCX_HIP_CHECK(hipStreamSynchronize(ctx->impl.stream));
+    });
 }
 
 int tcx_set_max_pairs(tcx_context* ctx, long max_pairs)

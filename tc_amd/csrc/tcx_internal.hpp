@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/tcx.h"
+#include "jastrow_dev.hpp"
 
 namespace tcx {
 
@@ -58,7 +59,7 @@ struct OrbEntry {
     int npw;
     int gidx_off;         // offset into the concatenated gindex array
     int kidx;             // k-point (for gradients)
-    int pad;
+    int dir;              // -1: plain scatter; 0..2: multiply by the Cartesian (k+G)_dir
 };
 
 // Divergence-correction job for one applied band (k, j).
@@ -76,11 +77,10 @@ struct DivEntry {
 
 void launch_zero(void* p, size_t bytes, hipStream_t s);
 
-// scatter complex128 coefficients to double grids (grids pre-zeroed);
-// dir >= 0 multiplies by the Cartesian component dir of (k+G).
+// scatter complex128 coefficients to double grids (grids pre-zeroed), grid i <- job i;
+// a job with dir >= 0 multiplies by the Cartesian component dir of (k+G).
 void launch_scatter(const double* coef, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
-                    double* grids, long long ngrid_pts, int dir, const double* kcart, const Geom& g,
-                    hipStream_t s);
+                    double* grids, long long ngrid_pts, const double* kcart, const Geom& g, hipStream_t s);
 
 // hphi[job] += factor * W[job.grid][gindex]   (W: double G-space grids)
 void launch_gather_add(const double* W, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
@@ -102,5 +102,26 @@ void launch_hf2x_C(int prec, const Geom& g, const void* buf, const void* phiR_oc
 // divergence correction: hphi += sum_b dc f_b <chi_b|phi_j> phi_b
 void launch_divcorr(const double* phi, const double* occ_right, const double* occ_left, const DivEntry* jobs,
                     int njobs, const double* fill, const double* dc, double* out, hipStream_t s);
+
+// tc2x pair pipeline (4-channel pair records)
+void launch_tc_table(int prec, void* table, const double* kqcart, int row0, int nrows, const Geom& g,
+                     const JastrowRPA& jp, hipStream_t s);
+void launch_tc2x_A(int prec, const Geom& g, const void* appR, const void* appGrad, const void* chiR,
+                   const PairEntry* pairs, int npairs, void* buf, hipStream_t s);
+void launch_tc2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const void* table, int kq0,
+                   const double* kqcart, hipStream_t s);
+void launch_tc2x_C(int prec, const Geom& g, const void* buf, const void* occR, const void* gradR,
+                   const PairEntry* pairs, const TargetEntry* targets, int ntargets, const double* fill, double* W,
+                   hipStream_t s);
+
+// density and local three-body potentials (FP64 grids)
+void launch_density(const double* occR, const double* chiR, const double* fill, int nocc, long long N, double* out,
+                    hipStream_t s);
+void launch_dnu(const double* nG, double* out, const Geom& g, const JastrowRPA& j0, const JastrowRPA& j1,
+                hipStream_t s);
+void launch_tc3b1_V(const double* dnu, double* V, const Geom& g, const JastrowRPA& j0, const JastrowRPA& j1,
+                    hipStream_t s);
+void launch_sum_squares(const double* dnu, double* V, long long N, hipStream_t s);
+void launch_mul_shared(double* grids, int ngrids, const double* V, long long N, hipStream_t s);
 
 }  // namespace tcx

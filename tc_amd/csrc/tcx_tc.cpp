@@ -1,18 +1,129 @@
-// tcx_tc.cpp -- TC/BITC terms (tc2x, tc3a1, tc3b1) on the context.
+// tcx_tc.cpp -- TC/BITC terms on the context: tc2x (pair pipeline) and the
+// local three-body terms tc3a1 / tc3b1.
+#include <algorithm>
+#include <cmath>
+
 #include "tcx_terms.hpp"
 
 namespace tcx {
 
-void Context::tc2x(const int*, const double*, double*, int, unsigned)
+void Context::tc2x(const int* nb_, const double* phi, double* out, int prec, unsigned flags)
 {
-    throw Error(TCX_ERR_UNSUPPORTED, "tc2x: not implemented yet");
+    check_apply_args(nb_, phi, out, prec);
+    require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc2x: the cell was set up for HF");
+    require(jastrow_set, TCX_ERR_STATE, "tc2x: call tcx_set_jastrow_rpa first");
+    std::vector<int> nb(nb_, nb_ + nk);
+    const Plan& P = get_plan(nb);
+    ApplyIO io = begin_apply(nb_, phi, out, flags, P);
+    if (P.n_app > 0) {
+        const bool bitc = method == TCX_METHOD_BITC;
+        ensure_occR(prec);
+        ensure_gradR(prec);
+        if (bitc) ensure_chiR(prec);
+        const void* chiR = bitc ? d_chiR[prec].p : d_occR[prec].p;  // bra orbital (tc2x.cpp:306)
+        const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
+        d_appR.ensure((size_t)P.n_app * N * es);
+        d_appGrad.ensure((size_t)3 * P.n_app * N * es);
+        timed(3, 0, [&] {
+            orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p);
+            orbitals_to_real(io.phi, (const OrbEntry*)P.d_appgradjobs.p, 3 * P.n_app, prec, d_appGrad.p);
+        });
+        d_W.ensure((size_t)P.n_app * N * 16);
+        launch_zero(d_W.p, (size_t)P.n_app * N * 16, stream);
+        d_pairbuf.ensure((size_t)std::max(1, P.max_launch_pairs) * 4 * N * es);
+        int max_rows = 1;
+        for (const Launch& L : P.launches) max_rows = std::max(max_rows, L.nkq);
+        d_ktab.ensure((size_t)max_rows * N * es);
+        const JastrowRPA jp = jastrow_pair(0, 0);  // tc2x.cpp:269-271 (ispin, ispin)
+        for (const Launch& L : P.launches) {
+            const PairEntry* pairs = (const PairEntry*)P.d_pairs.p + L.pair0;
+            const TargetEntry* targets = (const TargetEntry*)P.d_targets.p + L.target0;
+            launch_tc_table(prec, d_ktab.p, (const double*)P.d_kq.p, L.kq0, L.nkq, geo, jp, stream);
+            timed(0, L.npairs, [&] {
+                launch_tc2x_A(prec, geo, d_appR.p, d_appGrad.p, chiR, pairs, L.npairs, d_pairbuf.p, stream);
+            });
+            timed(1, L.npairs, [&] {
+                launch_tc2x_B(prec, geo, d_pairbuf.p, pairs, L.npairs, d_ktab.p, L.kq0, (const double*)P.d_kq.p,
+                              stream);
+            });
+            timed(2, L.npairs, [&] {
+                launch_tc2x_C(prec, geo, d_pairbuf.p, d_occR[prec].p, d_gradR[prec].p, pairs, targets, L.ntargets,
+                              (const double*)d_fill.p, (double*)d_W.p, stream);
+            });
+        }
+        // two_body_factor, src/calc_hamiltonian_tc2x.cpp:32-33
+        double factor = -1.0 / (num_kpoints * volume);
+        if (nspin == 1) factor /= 2.0;
+        finish_W(P, io, factor);
+        apply_divcorr(P, io, (const double*)(bitc ? d_chi_occ.p : d_phi_occ.p));
+    }
+    end_apply(io);
 }
 
-void Context::tc3_local(int, const int*, const double*, double*, int, unsigned)
+void Context::ensure_density()
 {
-    throw Error(TCX_ERR_UNSUPPORTED, "tc3a1/tc3b1: not implemented yet");
+    if (density_valid) return;
+    const bool bitc = method == TCX_METHOD_BITC;
+    ensure_occR(TCX_PREC_FP64);
+    if (bitc) ensure_chiR(TCX_PREC_FP64);
+    d_density.ensure((size_t)2 * N * 16);
+    launch_density((const double*)d_occR[TCX_PREC_FP64].p, bitc ? (const double*)d_chiR[TCX_PREC_FP64].p : nullptr,
+                   (const double*)d_fill.p, n_occ, N, (double*)d_density.p, stream);
+    density_valid = true;
 }
 
-void Context::ensure_density() {}
+// which = 0: tc3a1, 1: tc3b1.  Always FP64 (O(Nk Nb) transforms, no pair loop).
+void Context::tc3_local(int which, const int* nb_, const double* phi, double* out, int prec, unsigned flags)
+{
+    check_apply_args(nb_, phi, out, prec);
+    require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc3: the cell was set up for HF");
+    require(jastrow_set, TCX_ERR_STATE, "tc3: call tcx_set_jastrow_rpa first");
+    std::vector<int> nb(nb_, nb_ + nk);
+    const Plan& P = get_plan(nb);
+    ApplyIO io = begin_apply(nb_, phi, out, flags, P);
+    if (P.n_app > 0) {
+        ensure_density();
+        const size_t g = (size_t)N * 16;
+        d_work.ensure(6 * g);
+        double* nG = (double*)d_work.p;             // 2 grids
+        double* dnu = (double*)((char*)d_work.p + 2 * g);  // 3 grids
+        double* V = (double*)((char*)d_work.p + 5 * g);    // 1 grid
+        TCX_HIP_CHECK(hipMemcpyAsync(nG, d_density.p, 2 * g, hipMemcpyDeviceToDevice, stream));
+        launch_fft3d(TCX_PREC_FP64, -1, nG, 2, geo, 1.0 / (double)N, stream);
+        const JastrowRPA j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
+        const double NkV = num_kpoints * volume;
+        double factor;
+        launch_dnu(nG, dnu, geo, j0, j1, stream);
+        launch_fft3d(TCX_PREC_FP64, +1, dnu, 3, geo, 1.0, stream);
+        if (which == 0) {
+            launch_sum_squares(dnu, V, N, stream);         // tc3a1.cpp:78-84
+            factor = 0.5 / (NkV * NkV);                    // tc3a1.cpp:26-28
+        } else {
+            launch_mul_shared(dnu, 3, (const double*)d_density.p, N, stream);  // tc3b1.cpp:77-81 (spin 0)
+            launch_fft3d(TCX_PREC_FP64, -1, dnu, 3, geo, 1.0 / (double)N, stream);
+            launch_tc3b1_V(dnu, V, geo, j0, j1, stream);   // tc3b1.cpp:86-100
+            launch_fft3d(TCX_PREC_FP64, +1, V, 1, geo, 1.0, stream);
+            factor = -1.0 / (NkV * NkV);                   // tc3b1.cpp:26-28
+        }
+        // H3 phi_kj += factor * FFT_forward[V phi_kj] on the sphere of k (tc3a1.cpp:86-116)
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(P.n_app, ((size_t)1 << 30) / g));
+        d_W.ensure((size_t)chunk * g);
+        const OrbEntry* jobs = (const OrbEntry*)P.d_appjobs.p;
+        timed(3, 0, [&] {
+            for (int c0 = 0; c0 < P.n_app; c0 += chunk) {
+                const int nc = std::min(chunk, P.n_app - c0);
+                double* Wc = (double*)d_W.p;
+                launch_zero(Wc, (size_t)nc * g, stream);
+                launch_scatter(io.phi, (const int*)d_gindex.p, jobs + c0, nc, max_npw, Wc, N,
+                               (const double*)d_kcart.p, geo, stream);
+                launch_fft3d(TCX_PREC_FP64, +1, Wc, nc, geo, 1.0, stream);
+                launch_mul_shared(Wc, nc, V, N, stream);
+                launch_fft3d(TCX_PREC_FP64, -1, Wc, nc, geo, 1.0 / (double)N, stream);
+                launch_gather_add(Wc, (const int*)d_gindex.p, jobs + c0, nc, max_npw, io.out, N, factor, stream);
+            }
+        });
+    }
+    end_apply(io);
+}
 
 }  // namespace tcx

@@ -43,7 +43,7 @@ struct DBuf {
 struct Launch {
     int pair0 = 0, npairs = 0;      // slice of Plan::pairs; slot p of the pair buffer = pair0 + p
     int target0 = 0, ntargets = 0;  // slice of Plan::targets (pair indices are launch-local)
-    int kq0 = 0;
+    int kq0 = 0, nkq = 0;           // rows of Plan::kq used by this launch
 };
 
 struct Plan {
@@ -55,8 +55,9 @@ struct Plan {
     std::vector<Launch> launches;
     int max_launch_pairs = 0;
     DBuf d_pairs, d_targets, d_kq;
-    DBuf d_appjobs;  // OrbEntry per applied band (band order)
-    DBuf d_divjobs;  // DivEntry per applied band
+    DBuf d_appjobs;      // OrbEntry per applied band (band order)
+    DBuf d_appgradjobs;  // three OrbEntry (dir 0..2) per applied band
+    DBuf d_divjobs;      // DivEntry per applied band
 };
 
 // synchronous host -> device copy into a growing buffer
@@ -108,7 +109,8 @@ struct Context {
     bool occR_valid[2] = {false, false}, chiR_valid[2] = {false, false}, gradR_valid[2] = {false, false};
     DBuf d_density;  // double2 [2][N], real space
     bool density_valid = false;
-    DBuf d_occjobs;  // OrbEntry per occupied orbital
+    DBuf d_occjobs;      // OrbEntry per occupied orbital
+    DBuf d_occgradjobs;  // three OrbEntry (dir 0..2) per occupied orbital
     DBuf d_dc;       // divergence factor per k
     DBuf d_phi_app, d_out, d_appR, d_appGrad, d_tmp64, d_W, d_pairbuf, d_ktab, d_work;
 
@@ -142,8 +144,11 @@ struct Context {
     void set_div(int enabled, double alpha, const double* svaux);
     void invalidate_states();
 
-    void orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n, int prec, void* out, int dir);
+    void orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n, int prec, void* out);
     void ensure_occR(int prec);
+    void ensure_chiR(int prec);
+    void ensure_gradR(int prec);
+    JastrowRPA jastrow_pair(int s1, int s2) const { return JastrowRPA{A[s1 * 2 + s2], F[s1 * 2 + s2], jastrow_zero ? 1 : 0}; }
     void ensure_density();
 
     const Plan& get_plan(const std::vector<int>& nb);

@@ -38,6 +38,7 @@ EXPORTS = [
     "tcx_set_cell", "tcx_set_kpoints", "tcx_set_basis", "tcx_set_states",
     "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux",
     "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1",
+    "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
     "tcx_set_max_pairs", "tcx_enable_timing", "tcx_get_timing",
 ]
 
@@ -77,6 +78,10 @@ def load_library():
     L.tcx_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
     for name in ("tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1"):
         getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_int, C.c_uint]
+    L.tcx_device_alloc.argtypes = [vp, C.c_ulonglong, C.POINTER(vp)]
+    L.tcx_device_free.argtypes = [vp, vp]
+    L.tcx_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong, C.c_int]
+    L.tcx_memset_zero.argtypes = [vp, vp, C.c_ulonglong]
     L.tcx_set_max_pairs.argtypes = [vp, C.c_long]
     L.tcx_enable_timing.argtypes = [vp, C.c_int]
     L.tcx_get_timing.argtypes = [vp, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
@@ -227,3 +232,53 @@ class Engine:
 
     def synchronize(self):
         self._check(self.L.tcx_synchronize(self.h), "synchronize")
+
+    def device_buffer(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, array):
+        a = np.ascontiguousarray(array)
+        buf = DeviceBuffer(self, a.nbytes)
+        buf.upload(a)
+        return buf
+
+
+class DeviceBuffer:
+    """Device memory owned through the library's own HIP runtime (no torch on
+    the GPU path: torch bundles a second HIP runtime)."""
+
+    def __init__(self, eng: Engine, nbytes: int):
+        self.eng = eng
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        eng._check(eng.L.tcx_device_alloc(eng.h, self.nbytes, C.byref(p)), "device_alloc")
+        self.ptr = p.value
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        assert a.nbytes <= self.nbytes
+        self.eng._check(self.eng.L.tcx_memcpy(self.eng.h, C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data),
+                                              a.nbytes, 0), "memcpy h2d")
+
+    def download(self, dtype, count=None):
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dt)
+        self.eng._check(self.eng.L.tcx_memcpy(self.eng.h, C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr),
+                                              out.nbytes, 1), "memcpy d2h")
+        return out
+
+    def zero(self):
+        self.eng._check(self.eng.L.tcx_memset_zero(self.eng.h, C.c_void_p(self.ptr), self.nbytes), "memset")
+
+    def free(self):
+        if self.ptr:
+            self.eng._check(self.eng.L.tcx_device_free(self.eng.h, C.c_void_p(self.ptr)), "device_free")
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.eng.h:
+                self.free()
+        except Exception:
+            pass

@@ -110,17 +110,21 @@ def test_hf2x_config0_heg_sc(engine, prec):
 
 
 def test_hf2x_device_pointers(engine):
-    torch = pytest.importorskip("torch")
     sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2)
     engine.load_system(sys_)
     nb, phi = S.applied_bands(sys_, nb=[3, 2], seed=23)
     ref = engine.hf2x(nb, phi)
-    d_phi = torch.from_numpy(phi.view(np.float64)).cuda()
-    d_out = torch.zeros_like(d_phi)
-    torch.cuda.synchronize()
-    engine._term_device("hf2x", nb, d_phi.data_ptr(), d_out.data_ptr(), tcx.PREC_FP64)
-    got = d_out.cpu().numpy().view(np.complex128)
+    d_phi = engine.to_device(phi)
+    d_out = engine.device_buffer(phi.nbytes)
+    d_out.zero()
+    engine._term_device("hf2x", nb, d_phi.ptr, d_out.ptr, tcx.PREC_FP64)
+    got = d_out.download(np.complex128)
     assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+    # accumulates in place on the device too
+    engine._term_device("hf2x", nb, d_phi.ptr, d_out.ptr, tcx.PREC_FP64)
+    assert np.abs(d_out.download(np.complex128) - 2 * ref).max() <= 1e-14 * np.abs(ref).max()
+    d_phi.free()
+    d_out.free()
 
 
 def test_errors_are_status_codes(engine):
