@@ -71,9 +71,16 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=1)
-    ap.add_argument("--max-pairs", type=int, default=0)
+    ap.add_argument("--targets", type=int, default=0, help="launch shape: targets per launch (0 = tuned/auto)")
+    ap.add_argument("--sources", type=int, default=0, help="launch shape: sources per launch (0 = tuned/auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--tune", action="store_true", help="run the launch-shape autotuner first")
+    ap.add_argument("--quick", action="store_true",
+                    help="profiling run: one FP32 apply only (no FP64 leg, parity or CPU baseline)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 1 GiB device copy and a 1 GiB fill (counter calibration kernels)")
+    ap.add_argument("--tune-out", default=os.path.join(ROOT, "gpurun_out", "tuned_shapes.json"))
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -81,6 +88,7 @@ def main():
               "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
+    from tc_amd import autotune as AT
     from tc_amd import energy as E
     from tc_amd import system as S
     from tc_amd import tcx
@@ -94,8 +102,11 @@ def main():
 
     eng = tcx.Engine(0)
     eng.load_system(sys_)
-    if args.max_pairs:
-        eng.set_max_pairs(args.max_pairs)
+    def shape_for(prec):
+        if args.targets or args.sources:
+            return (args.targets, args.sources)
+        entry = AT.load_table().get(AT.signature(sys_, nb, "hf2x", prec))
+        return (entry["targets"], entry["sources"]) if entry else (0, 0)
     # inputs and outputs resident in HBM (the library's own allocations)
     d_phi = eng.to_device(phi)
     d_out32 = eng.device_buffer(phi.nbytes)
@@ -104,6 +115,36 @@ def main():
 
     def step(prec, out):
         eng._term_device("hf2x", nb, d_phi.ptr, out.ptr, prec)
+
+    if args.tune:
+        os.makedirs(os.path.dirname(args.tune_out), exist_ok=True)
+        for prec in (tcx.PREC_FP32, tcx.PREC_FP64):
+            best, tim = AT.tune(eng, sys_, "hf2x", nb, d_phi, d_scratch, prec, log=log)
+            AT.save_entry(AT.signature(sys_, nb, "hf2x", prec), best, tim[best], path=args.tune_out)
+            log(f"bench: tuned {'fp32' if prec else 'fp64'} shape {best} {tim[best] * 1e3:.1f} ms")
+        tuned = AT.load_table(args.tune_out)
+        shape32 = tuple(tuned[AT.signature(sys_, nb, "hf2x", tcx.PREC_FP32)][k] for k in ("targets", "sources"))
+        shape64 = tuple(tuned[AT.signature(sys_, nb, "hf2x", tcx.PREC_FP64)][k] for k in ("targets", "sources"))
+    else:
+        shape32, shape64 = shape_for(tcx.PREC_FP32), shape_for(tcx.PREC_FP64)
+    log(f"bench: launch shapes fp32 {shape32} fp64 {shape64} (0 = automatic)")
+    eng.set_launch_shape("hf2x", *shape32)
+
+    if args.calibrate:
+        gib = 1 << 30
+        a, b = eng.device_buffer(gib), eng.device_buffer(gib)
+        for _ in range(3):
+            eng.L.tcx_memcpy(eng.h, b.ptr, a.ptr, gib, 2)  # k_copy: reads and writes 1 GiB
+            a.zero()                                       # k_zero: writes 1 GiB
+        a.free()
+        b.free()
+        log("bench: calibration kernels done (k_copy 1 GiB, k_zero 1 GiB)")
+    if args.quick:
+        step(tcx.PREC_FP32, d_out32)
+        eng.synchronize()
+        log("bench: quick profiling apply done")
+        eng.close()
+        return
 
     log(f"bench: warmup {args.warmup} FP32-split step(s)")
     for _ in range(args.warmup):
@@ -136,6 +177,7 @@ def main():
 
     # FP64 apply for the ratio, then parity of the two outputs
     log(f"bench: FP64 apply x{args.fp64_steps}")
+    eng.set_launch_shape("hf2x", *shape64)
     d_out64.zero()
     step(tcx.PREC_FP64, d_out64)  # also the parity reference
     eng.synchronize()
@@ -146,6 +188,7 @@ def main():
     fp64_s = (time.perf_counter() - t0) / args.fp64_steps
     value64 = pairs_per_step / fp64_s
 
+    eng.set_launch_shape("hf2x", *shape32)
     d_out32.zero()
     step(tcx.PREC_FP32, d_out32)
     h32 = d_out32.download(np.complex128)
@@ -182,6 +225,7 @@ def main():
             "workload": "hf2x fcc a=10.26 bohr, 4x4x4 k, 16 bands/k (all occupied), FFT 40^3, FP32-split",
             "pairs_per_step": pairs_per_step, "grid": list(sys_.n), "nk": sys_.nk, "bands": int(nb[0]),
             "npw_mean": float(np.mean(sys_.npw)), "div_correction": True,
+            "launch_shape_fp32": list(shape32), "launch_shape_fp64": list(shape64),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -143,8 +143,13 @@ int tcx_memcpy(tcx_context *ctx, void *dst, const void *src, unsigned long long 
 int tcx_memset_zero(tcx_context *ctx, void *ptr, unsigned long long bytes);
 
 /* ---------------------------------------------------------------- tuning / timing */
-/* Upper bound of (k,j,q,i) pairs held in flight per launch (pair-buffer size). */
-int tcx_set_max_pairs(tcx_context *ctx, long max_pairs);
+/* Launch shape of a pair pipeline (term 0: hf2x, 1: tc2x): each launch pairs
+ * `targets` consecutive applied bands (k,j) with `sources` consecutive
+ * occupied orbitals (q,i); targets*sources pair records are in flight (the
+ * pair-buffer size).  0 selects the automatic shape.  Set by the autotuner
+ * (tc_amd/autotune.py).  No counterpart in the reference (its OpenMP loop
+ * has no launch shape). */
+int tcx_set_launch_shape(tcx_context *ctx, int term, long targets, long sources);
 
 /* Per-kernel-class timing with HIP events on the context's stream.
  * Classes: 0 pair pass A (z-pencils), 1 pair pass B (xy-planes),

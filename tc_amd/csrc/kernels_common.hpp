@@ -39,6 +39,104 @@ __device__ __forceinline__ void store_pencil(const cpx<T> (&v)[NZ], cpx<T>* __re
     });
 }
 
+// Component-wise select: a ?: on the 16-byte cpx<double> struct is lowered
+// through private memory, two scalar selects stay in registers.
+template <typename T>
+__device__ __forceinline__ cpx<T> csel(bool c, cpx<T> a, cpx<T> b)
+{
+    return cpx<T>{c ? a.re : b.re, c ? a.im : b.im};
+}
+
+// Lane exchange of one value with lane ^ mask (ds_bpermute), doubles as two
+// 32-bit halves kept in registers.
+__device__ __forceinline__ float xshfl(float v, int mask) { return __shfl_xor(v, mask); }
+__device__ __forceinline__ double xshfl(double v, int mask)
+{
+    const int lo = __shfl_xor(__double2loint(v), mask);
+    const int hi = __shfl_xor(__double2hiint(v), mask);
+    return __hiloint2double(hi, lo);
+}
+
+// ---------------------------------------------------------------------------
+// Pencil split over two lanes (lane and lane ^ 32 of a wavefront).
+// Lane half h holds the samples z = 2m + h (m < NZ/2) on entry: a radix-2
+// decimation in time whose length-NZ/2 sub-transforms run in each lane's
+// registers and whose final butterfly crosses the two halves with one
+// __shfl_xor(.., 32) per value.  On exit lane half h holds outputs
+// z = h*NZ/2 + k (k < NZ/2).  Halves the registers a pencil needs, so the
+// FP64 accumulators of the accumulate pass fit at twice the occupancy.
+// ---------------------------------------------------------------------------
+template <typename T, int NZ, int SIGN>
+__device__ __forceinline__ void fft_split2(cpx<T> (&v)[NZ / 2], int h)
+{
+    constexpr int M = NZ / 2;
+    fft<T, M, SIGN>(v);
+    static_for<1, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const cpx<T> t = v[k] * twiddle<T, NZ, SIGN, k>();
+        v[k] = csel(h != 0, t, v[k]);
+    });
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const cpx<T> o{xshfl(v[k].re, 32), xshfl(v[k].im, 32)};
+        v[k] = csel(h != 0, o - v[k], v[k] + o);
+    });
+}
+
+// Pencil split over four lanes (lane ^ 16, ^ 32, ^ 48): lane quarter h holds
+// z = 4m + h on entry; the radix-4 cross-lane DFT runs as two radix-2 stages
+// (shuffle with lane ^ 32, twiddle, shuffle with lane ^ 16).  On exit lane
+// quarter h holds outputs z = ob*NZ/4 + k with ob = (h >> 1) + 2*(h & 1).
+template <typename T, int NZ, int SIGN>
+__device__ __forceinline__ void fft_split4(cpx<T> (&v)[NZ / 4], int h)
+{
+    constexpr int M = NZ / 4;
+    fft<T, M, SIGN>(v);
+    // Y'_h[k] = W_NZ^{h k} Y_h[k] = W^{h0 k} W^{2 h1 k} Y_h[k]  (two predicated
+    // multiplies by constants; a select among constants would become a table)
+    const int h0 = h & 1, h1 = h >> 1;
+    static_for<1, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const cpx<T> t1 = v[k] * twiddle<T, NZ, SIGN, k>();
+        const cpx<T> a = csel(h0 != 0, t1, v[k]);
+        const cpx<T> t2 = a * twiddle<T, NZ, SIGN, (2 * k) % NZ>();
+        v[k] = csel(h1 != 0, t2, a);
+    });
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const cpx<T> o{xshfl(v[k].re, 32), xshfl(v[k].im, 32)};
+        cpx<T> z = csel(h1 != 0, o - v[k], v[k] + o);               // Z_{h0, s0=h1}
+        z = csel((h0 & h1) != 0, mul_isign<T, SIGN>(z, T(1)), z);     // W_4^{h0 s0}
+        const cpx<T> o2{xshfl(z.re, 16), xshfl(z.im, 16)};
+        v[k] = csel(h0 != 0, o2 - z, z + o2);                       // X_{s0 + 2 s1}, s1 = h0
+    });
+}
+
+// Lanes per pencil and the output block of a lane for the split transforms.
+template <int S>
+struct PencilSplit;
+template <>
+struct PencilSplit<2> {
+    static constexpr int pencils = 32;
+    __device__ static int out_block(int h) { return h; }
+    template <typename T, int NZ, int SIGN>
+    __device__ static void fft(cpx<T> (&v)[NZ / 2], int h) { fft_split2<T, NZ, SIGN>(v, h); }
+};
+template <>
+struct PencilSplit<4> {
+    static constexpr int pencils = 16;
+    __device__ static int out_block(int h) { return (h >> 1) + 2 * (h & 1); }
+    template <typename T, int NZ, int SIGN>
+    __device__ static void fft(cpx<T> (&v)[NZ / 4], int h) { fft_split4<T, NZ, SIGN>(v, h); }
+};
+
+// FP32 pair records split a pencil over 2 lanes, FP64 ones over 4 (registers
+// of the FP64 accumulators plus the pencil stay under 256 VGPRs).
+template <typename T, int NZ>
+struct SplitOf {
+    static constexpr int value = (sizeof(T) == 8 && NZ % 4 == 0) ? 4 : 2;
+};
+
 // LDS tile of one plane: [y][x] with one element of padding per row.
 template <typename T, int NP>
 struct PlaneTile {

@@ -40,6 +40,30 @@ void launch_zero(void* p, size_t bytes, hipStream_t s)
     TCX_HIP_CHECK(hipGetLastError());
 }
 
+// ----------------------------------------------------------------- device copy
+// Plain streaming copy (16 B per lane); also the known-size reference kernel
+// against which the HBM byte counters are calibrated.
+__global__ __launch_bounds__(256) void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16,
+                                              unsigned char* dtail, const unsigned char* stail, size_t ntail)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n16; i += stride) dst[i] = src[i];
+    if (blockIdx.x == 0 && threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return;
+    const size_t n16 = bytes / 16, ntail = bytes % 16;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16,
+                       (unsigned char*)dst + n16 * 16, (const unsigned char*)src + n16 * 16, ntail);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
 // ----------------------------------------------------------------- scatter / gather
 __global__ __launch_bounds__(256) void k_scatter(const double2* __restrict__ coef, const int* __restrict__ gindex,
                                                  const OrbEntry* __restrict__ jobs, double2* __restrict__ grids,

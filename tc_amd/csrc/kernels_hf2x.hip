@@ -63,18 +63,23 @@ __global__ __launch_bounds__(kPlaneBlock) void k_hf2x_B(cpx<T>* __restrict__ buf
     if (t < NP) {
         tile_get_row<T, NP>(tile, v, t);
         fft<T, NP, -1>(v);
-        // frequency (x, y=t, z): Cartesian k-q+G, Miller wrapped (plane_wave_basis.cpp:205-216)
+        // frequency (x, y=t, z): Cartesian k-q+G, Miller wrapped (plane_wave_basis.cpp:205-216);
+        // v = 1/|k-q+G|^2, 0 below 1e-8 (hf2x.cpp:91-104), times the 1/N of FFT_forward.
+        // Evaluated in the working precision: the row offset is formed in FP64 and
+        // rounded once, so the FP32 kernel carries one rounding of |k-q+G|^2.
         const int kq = pairs[blockIdx.y].kq;
         const int my = centered(t, NP), mz = centered(z, nz);
-        const double bx = kqcart[3 * kq + 0] + g.b[3] * my + g.b[6] * mz;
-        const double by = kqcart[3 * kq + 1] + g.b[4] * my + g.b[7] * mz;
-        const double bz = kqcart[3 * kq + 2] + g.b[5] * my + g.b[8] * mz;
+        const T bx = static_cast<T>(kqcart[3 * kq + 0] + g.b[3] * my + g.b[6] * mz);
+        const T by = static_cast<T>(kqcart[3 * kq + 1] + g.b[4] * my + g.b[7] * mz);
+        const T bz = static_cast<T>(kqcart[3 * kq + 2] + g.b[5] * my + g.b[8] * mz);
+        const T b0 = static_cast<T>(g.b[0]), b1 = static_cast<T>(g.b[1]), b2 = static_cast<T>(g.b[2]);
+        const T sN = static_cast<T>(invN);
         static_for<0, NP>([&](auto xc) {
             constexpr int x = decltype(xc)::value;
-            constexpr int mx = x <= NP / 2 ? x : x - NP;
-            const double gx = bx + g.b[0] * mx, gy = by + g.b[1] * mx, gz = bz + g.b[2] * mx;
-            const double g2 = gx * gx + gy * gy + gz * gz;
-            const T w = g2 < 1e-8 ? T(0) : static_cast<T>(invN / g2);
+            constexpr T mx = static_cast<T>(x <= NP / 2 ? x : x - NP);
+            const T gx = bx + b0 * mx, gy = by + b1 * mx, gz = bz + b2 * mx;
+            const T g2 = gx * gx + gy * gy + gz * gz;
+            const T w = g2 < T(1e-8) ? T(0) : sN / g2;
             v[x] = scale(v[x], w);
         });
         fft<T, NP, 1>(v);
@@ -88,6 +93,8 @@ __global__ __launch_bounds__(kPlaneBlock) void k_hf2x_B(cpx<T>* __restrict__ buf
     }
 }
 
+// Each pencil is split over S lanes (PencilSplit), so a 64-lane workgroup
+// covers 64/S pencils and each lane keeps NZ/S FP64 accumulators.
 template <typename T, int NZ>
 __global__ __launch_bounds__(kPencilBlock) void k_hf2x_C(const cpx<T>* __restrict__ buf,
                                                          const cpx<T>* __restrict__ phiR_occ,
@@ -96,31 +103,37 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_C(const cpx<T>* __restric
                                                          const double* __restrict__ fill,
                                                          cpx<double>* __restrict__ W, int nxy)
 {
-    const int xy = blockIdx.x * kPencilBlock + threadIdx.x;
-    if (xy >= nxy) return;
+    constexpr int S = SplitOf<T, NZ>::value;
+    using Split = PencilSplit<S>;
+    constexpr int M = NZ / S;
+    const int h = threadIdx.x / Split::pencils;
+    const int xy0 = blockIdx.x * Split::pencils + (threadIdx.x % Split::pencils);
+    const bool valid = xy0 < nxy;
+    const int xy = valid ? xy0 : 0;  // out-of-range lanes still join the shuffles
+    const int ob = Split::out_block(h);
     const TargetEntry te = targets[blockIdx.y];
     const size_t ng = (size_t)nxy * NZ;
-    cpx<double> acc[NZ];
-    static_for<0, NZ>([&](auto zc) { acc[decltype(zc)::value] = cpx<double>{0.0, 0.0}; });
+    cpx<double> acc[M];
+    static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = cpx<double>{0.0, 0.0}; });
     for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
         const int occ = pairs[pp].occ;
-        cpx<T> v[NZ];
-        load_pencil<T, NZ>(v, buf + (size_t)pp * ng + xy, nxy);
-        fft<T, NZ, 1>(v);
-        const cpx<T>* q = phiR_occ + (size_t)occ * ng + xy;
+        cpx<T> v[M];
+        load_pencil<T, M>(v, buf + (size_t)pp * ng + xy + (size_t)h * nxy, S * nxy);  // z = S m + h
+        Split::template fft<T, NZ, 1>(v, h);                                         // -> z = ob*M + k
+        const cpx<T>* q = phiR_occ + (size_t)occ * ng + xy + (size_t)ob * M * nxy;
         const double f = fill[occ];
-        static_for<0, NZ>([&](auto zc) {
-            constexpr int z = decltype(zc)::value;
-            const cpx<T> prod = v[z] * q[(size_t)z * nxy];
-            acc[z].re += f * static_cast<double>(prod.re);
-            acc[z].im += f * static_cast<double>(prod.im);
+        static_for<0, M>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const cpx<T> prod = v[k] * q[(size_t)k * nxy];
+            acc[k].re += f * static_cast<double>(prod.re);
+            acc[k].im += f * static_cast<double>(prod.im);
         });
     }
-    cpx<double>* w = W + (size_t)te.app * ng + xy;
-    static_for<0, NZ>([&](auto zc) {
-        constexpr int z = decltype(zc)::value;
-        cpx<double> cur = w[(size_t)z * nxy];
-        w[(size_t)z * nxy] = cur + acc[z];
+    if (!valid) return;
+    cpx<double>* w = W + (size_t)te.app * ng + xy + (size_t)ob * M * nxy;
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        w[(size_t)k * nxy] = w[(size_t)k * nxy] + acc[k];
     });
 }
 
@@ -179,10 +192,13 @@ static void hf2x_C_impl(const Geom& g, const void* buf, const void* occ, const P
                         const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
-    const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, ntargets);
+    // 64/S split pencils per 64-lane workgroup
 #define TCX_C(NZ)                                                                                             \
-    hipLaunchKernelGGL((k_hf2x_C<T, NZ>), grid, dim3(kPencilBlock), 0, s, (const cpx<T>*)buf,                 \
-                       (const cpx<T>*)occ, pairs, targets, fill, (cpx<double>*)W, nxy);
+    {                                                                                                         \
+        constexpr int P = 64 / SplitOf<T, NZ>::value;                                                         \
+        hipLaunchKernelGGL((k_hf2x_C<T, NZ>), dim3((nxy + P - 1) / P, ntargets), dim3(kPencilBlock), 0, s,    \
+                           (const cpx<T>*)buf, (const cpx<T>*)occ, pairs, targets, fill, (cpx<double>*)W, nxy); \
+    }
     TCX_SIZE_SWITCH(g.n3, TCX_C)
 #undef TCX_C
     TCX_HIP_CHECK(hipGetLastError());

@@ -45,22 +45,28 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_A(const cpx<T>* __restric
                                                          const PairEntry* __restrict__ pairs,
                                                          cpx<T>* __restrict__ buf, int nxy)
 {
-    const int xy = blockIdx.x * kPencilBlock + threadIdx.x;
-    if (xy >= nxy) return;
+    constexpr int S = SplitOf<T, NZ>::value;  // pencil split over S lanes, see PencilSplit
+    using Split = PencilSplit<S>;
+    constexpr int M = NZ / S;
+    const int h = threadIdx.x / Split::pencils;
+    const int xy0 = blockIdx.x * Split::pencils + (threadIdx.x % Split::pencils);
+    const bool valid = xy0 < nxy;
+    const int xy = valid ? xy0 : 0;
     const PairEntry pe = pairs[blockIdx.y];
     const size_t ng = (size_t)nxy * NZ;
-    cpx<T> x[NZ];
-    load_pencil<T, NZ>(x, chiR + (size_t)pe.occ * ng + xy, nxy);
-    cpx<T>* out = buf + (size_t)blockIdx.y * 4 * ng + xy;
+    cpx<T> x[M];
+    load_pencil<T, M>(x, chiR + (size_t)pe.occ * ng + xy + (size_t)h * nxy, S * nxy);           // z = S m + h
+    cpx<T>* out = buf + (size_t)blockIdx.y * 4 * ng + xy + (size_t)Split::out_block(h) * M * nxy;  // z = ob*M + k
     for (int c = 0; c < 4; ++c) {
-        const cpx<T>* src = c == 0 ? appR + (size_t)pe.app * ng + xy : appGrad + ((size_t)pe.app * 3 + (c - 1)) * ng + xy;
-        cpx<T> v[NZ];
-        static_for<0, NZ>([&](auto zc) {
-            constexpr int z = decltype(zc)::value;
-            v[z] = cmulc(x[z], src[(size_t)z * nxy]);
+        const cpx<T>* src = (c == 0 ? appR + (size_t)pe.app * ng : appGrad + ((size_t)pe.app * 3 + (c - 1)) * ng) + xy +
+                            (size_t)h * nxy;
+        cpx<T> v[M];
+        static_for<0, M>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            v[m] = cmulc(x[m], src[(size_t)(S * m) * nxy]);
         });
-        fft<T, NZ, -1>(v);
-        store_pencil<T, NZ>(v, out + (size_t)c * ng, nxy);
+        Split::template fft<T, NZ, -1>(v, h);
+        if (valid) store_pencil<T, M>(v, out + (size_t)c * ng, nxy);
     }
 }
 
@@ -70,69 +76,71 @@ struct Tiles4 {
     PlaneTile<T, NP> t[4];
 };
 
+// One workgroup per (pair, plane z) with four wavefronts: wave c transforms
+// channel c; the channel combination is spread over all 256 lanes.
+constexpr int kTc2xPlaneBlock = 256;
+
 template <typename T, int NP>
-__global__ __launch_bounds__(kPlaneBlock) void k_tc2x_B(cpx<T>* __restrict__ buf, const PairEntry* __restrict__ pairs,
-                                                        const cpx<T>* __restrict__ table, int kq0,
-                                                        const double* __restrict__ kqcart, int nz, Geom g,
-                                                        double invN)
+__global__ __launch_bounds__(kTc2xPlaneBlock) void k_tc2x_B(cpx<T>* __restrict__ buf,
+                                                            const PairEntry* __restrict__ pairs,
+                                                            const cpx<T>* __restrict__ table, int kq0,
+                                                            const double* __restrict__ kqcart, int nz, Geom g,
+                                                            double invN)
 {
     __shared__ Tiles4<T, NP> S;
-    const int t = threadIdx.x;
+    const int c = threadIdx.x >> 6;   // channel of this wavefront
+    const int t = threadIdx.x & 63;
     const int z = blockIdx.x;
     const size_t ng = (size_t)NP * NP * nz;
-    cpx<T>* base = buf + (size_t)blockIdx.y * 4 * ng + (size_t)z * NP * NP;
+    cpx<T>* base = buf + ((size_t)blockIdx.y * 4 + c) * ng + (size_t)z * NP * NP;
     cpx<T> v[NP];
     if (t < NP) {
-        for (int c = 0; c < 4; ++c) {
-            load_pencil<T, NP>(v, base + c * ng + t, NP);
-            fft<T, NP, -1>(v);
-            tile_put_col<T, NP>(S.t[c], v, t);
-        }
+        load_pencil<T, NP>(v, base + t, NP);   // column x = t
+        fft<T, NP, -1>(v);                     // along y
+        tile_put_col<T, NP>(S.t[c], v, t);
     }
     __syncthreads();
     if (t < NP) {
-        for (int c = 0; c < 4; ++c) {
-            tile_get_row<T, NP>(S.t[c], v, t);
-            fft<T, NP, -1>(v);
-            tile_put_row<T, NP>(S.t[c], v, t);
-        }
-        // combine at frequency (x, y=t, z); this thread owns row t of every tile
+        tile_get_row<T, NP>(S.t[c], v, t);     // row y = t
+        fft<T, NP, -1>(v);                     // along x
+        tile_put_row<T, NP>(S.t[c], v, t);
+    }
+    __syncthreads();
+    {
+        // combine at frequency (x, y, z): b = rho V2 + sum_d sigma_d du_d, a_d = rho du_d (1/N folded in)
         const PairEntry pe = pairs[blockIdx.y];
         const double* kq = kqcart + 3 * pe.kq;
-        const cpx<T>* tab = table + (size_t)(pe.kq - kq0) * ng + (size_t)z * NP * NP + (size_t)t * NP;
-        const int my = centered(t, NP), mz = centered(z, nz);
-        const double bx = kq[0] + g.b[3] * my + g.b[6] * mz;
-        const double by = kq[1] + g.b[4] * my + g.b[7] * mz;
-        const double bz = kq[2] + g.b[5] * my + g.b[8] * mz;
-        static_for<0, NP>([&](auto xc) {
-            constexpr int x = decltype(xc)::value;
-            constexpr int mx = x <= NP / 2 ? x : x - NP;
-            const cpx<T> vu = tab[x];  // (V2, u)
+        const cpx<T>* tab = table + (size_t)(pe.kq - kq0) * ng + (size_t)z * NP * NP;
+        const int mz = centered(z, nz);
+        const double kz0 = kq[0] + g.b[6] * mz, kz1 = kq[1] + g.b[7] * mz, kz2 = kq[2] + g.b[8] * mz;
+        for (int e = threadIdx.x; e < NP * NP; e += kTc2xPlaneBlock) {
+            const int x = e % NP, y = e / NP;
+            const int mx = centered(x, NP), my = centered(y, NP);
+            const cpx<T> vu = tab[e];  // (V2, u), coalesced over e
             const double us = (double)vu.im * invN;
-            const T du0 = static_cast<T>((bx + g.b[0] * mx) * us);
-            const T du1 = static_cast<T>((by + g.b[1] * mx) * us);
-            const T du2 = static_cast<T>((bz + g.b[2] * mx) * us);
+            const T du0 = static_cast<T>((kz0 + g.b[0] * mx + g.b[3] * my) * us);
+            const T du1 = static_cast<T>((kz1 + g.b[1] * mx + g.b[4] * my) * us);
+            const T du2 = static_cast<T>((kz2 + g.b[2] * mx + g.b[5] * my) * us);
             const T v2 = static_cast<T>((double)vu.re * invN);
-            const cpx<T> r = S.t[0].a[t][x];
-            const cpx<T> s0 = S.t[1].a[t][x], s1 = S.t[2].a[t][x], s2 = S.t[3].a[t][x];
-            S.t[0].a[t][x] = scale(r, v2) + scale(s0, du0) + scale(s1, du1) + scale(s2, du2);
-            S.t[1].a[t][x] = scale(r, du0);
-            S.t[2].a[t][x] = scale(r, du1);
-            S.t[3].a[t][x] = scale(r, du2);
-        });
-        for (int c = 0; c < 4; ++c) {
-            tile_get_row<T, NP>(S.t[c], v, t);
-            fft<T, NP, 1>(v);
-            tile_put_row<T, NP>(S.t[c], v, t);
+            const cpx<T> r = S.t[0].a[y][x];
+            const cpx<T> s0 = S.t[1].a[y][x], s1 = S.t[2].a[y][x], s2 = S.t[3].a[y][x];
+            S.t[0].a[y][x] = scale(r, v2) + scale(s0, du0) + scale(s1, du1) + scale(s2, du2);
+            S.t[1].a[y][x] = scale(r, du0);
+            S.t[2].a[y][x] = scale(r, du1);
+            S.t[3].a[y][x] = scale(r, du2);
         }
     }
     __syncthreads();
     if (t < NP) {
-        for (int c = 0; c < 4; ++c) {
-            tile_get_col<T, NP>(S.t[c], v, t);
-            fft<T, NP, 1>(v);
-            store_pencil<T, NP>(v, base + c * ng + t, NP);
-        }
+        tile_get_row<T, NP>(S.t[c], v, t);
+        fft<T, NP, 1>(v);
+        tile_put_row<T, NP>(S.t[c], v, t);
+    }
+    __syncthreads();
+    if (t < NP) {
+        tile_get_col<T, NP>(S.t[c], v, t);
+        fft<T, NP, 1>(v);
+        store_pencil<T, NP>(v, base + t, NP);
     }
 }
 
@@ -146,34 +154,42 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_C(const cpx<T>* __restric
                                                          const double* __restrict__ fill,
                                                          cpx<double>* __restrict__ W, int nxy)
 {
-    const int xy = blockIdx.x * kPencilBlock + threadIdx.x;
-    if (xy >= nxy) return;
+    constexpr int S = SplitOf<T, NZ>::value;  // pencil split over S lanes, see PencilSplit
+    using Split = PencilSplit<S>;
+    constexpr int M = NZ / S;
+    const int h = threadIdx.x / Split::pencils;
+    const int xy0 = blockIdx.x * Split::pencils + (threadIdx.x % Split::pencils);
+    const bool valid = xy0 < nxy;
+    const int xy = valid ? xy0 : 0;
+    const int ob = Split::out_block(h);
     const TargetEntry te = targets[blockIdx.y];
     const size_t ng = (size_t)nxy * NZ;
-    cpx<double> acc[NZ];
-    static_for<0, NZ>([&](auto zc) { acc[decltype(zc)::value] = cpx<double>{0.0, 0.0}; });
+    cpx<double> acc[M];
+    static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = cpx<double>{0.0, 0.0}; });
     for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
         const int occ = pairs[pp].occ;
         const double f = fill[occ];
         for (int c = 0; c < 4; ++c) {
-            cpx<T> v[NZ];
-            load_pencil<T, NZ>(v, buf + ((size_t)pp * 4 + c) * ng + xy, nxy);
-            fft<T, NZ, 1>(v);
+            cpx<T> v[M];
+            load_pencil<T, M>(v, buf + ((size_t)pp * 4 + c) * ng + xy + (size_t)h * nxy, S * nxy);
+            Split::template fft<T, NZ, 1>(v, h);
             // channel 0: b * phi_qi (+f); channels 1..3: a_d * g_qi,d (-f)   [tc2x.cpp:363-369, :387-388]
-            const cpx<T>* m = c == 0 ? occR + (size_t)occ * ng + xy : gradR + ((size_t)occ * 3 + (c - 1)) * ng + xy;
+            const cpx<T>* m = (c == 0 ? occR + (size_t)occ * ng : gradR + ((size_t)occ * 3 + (c - 1)) * ng) + xy +
+                              (size_t)ob * M * nxy;
             const double s = c == 0 ? f : -f;
-            static_for<0, NZ>([&](auto zc) {
-                constexpr int z = decltype(zc)::value;
-                const cpx<T> prod = v[z] * m[(size_t)z * nxy];
-                acc[z].re += s * static_cast<double>(prod.re);
-                acc[z].im += s * static_cast<double>(prod.im);
+            static_for<0, M>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const cpx<T> prod = v[k] * m[(size_t)k * nxy];
+                acc[k].re += s * static_cast<double>(prod.re);
+                acc[k].im += s * static_cast<double>(prod.im);
             });
         }
     }
-    cpx<double>* w = W + (size_t)te.app * ng + xy;
-    static_for<0, NZ>([&](auto zc) {
-        constexpr int z = decltype(zc)::value;
-        w[(size_t)z * nxy] = w[(size_t)z * nxy] + acc[z];
+    if (!valid) return;
+    cpx<double>* w = W + (size_t)te.app * ng + xy + (size_t)ob * M * nxy;
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        w[(size_t)k * nxy] = w[(size_t)k * nxy] + acc[k];
     });
 }
 
@@ -196,10 +212,13 @@ static void tc2x_A_impl(const Geom& g, const void* appR, const void* appGrad, co
                         int npairs, void* buf, hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
-    const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, npairs);
 #define TCX_A(NZ)                                                                                              \
-    hipLaunchKernelGGL((k_tc2x_A<T, NZ>), grid, dim3(kPencilBlock), 0, s, (const cpx<T>*)appR,                 \
-                       (const cpx<T>*)appGrad, (const cpx<T>*)chiR, pairs, (cpx<T>*)buf, nxy);
+    {                                                                                                          \
+        constexpr int P = 64 / SplitOf<T, NZ>::value;                                                          \
+        hipLaunchKernelGGL((k_tc2x_A<T, NZ>), dim3((nxy + P - 1) / P, npairs), dim3(kPencilBlock), 0, s,       \
+                           (const cpx<T>*)appR, (const cpx<T>*)appGrad, (const cpx<T>*)chiR, pairs, (cpx<T>*)buf, \
+                           nxy);                                                                               \
+    }
     TCX_SIZE_SWITCH(g.n3, TCX_A)
 #undef TCX_A
     TCX_HIP_CHECK(hipGetLastError());
@@ -223,7 +242,7 @@ static void tc2x_B_impl(const Geom& g, void* buf, const PairEntry* pairs, int np
     if (g.n1 != g.n2) throw Error(TCX_ERR_UNSUPPORTED, "FFT grid must have n1 == n2");
     const double invN = 1.0 / ((double)g.n1 * g.n2 * g.n3);
 #define TCX_B(NP)                                                                                        \
-    hipLaunchKernelGGL((k_tc2x_B<T, NP>), dim3(g.n3, npairs), dim3(kPlaneBlock), 0, s, (cpx<T>*)buf,     \
+    hipLaunchKernelGGL((k_tc2x_B<T, NP>), dim3(g.n3, npairs), dim3(kTc2xPlaneBlock), 0, s, (cpx<T>*)buf, \
                        pairs, (const cpx<T>*)table, kq0, kqcart, g.n3, g, invN);
     TCX_SIZE_SWITCH(g.n1, TCX_B)
 #undef TCX_B
@@ -245,10 +264,13 @@ static void tc2x_C_impl(const Geom& g, const void* buf, const void* occR, const 
                         const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
-    const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, ntargets);
 #define TCX_C(NZ)                                                                                           \
-    hipLaunchKernelGGL((k_tc2x_C<T, NZ>), grid, dim3(kPencilBlock), 0, s, (const cpx<T>*)buf,               \
-                       (const cpx<T>*)occR, (const cpx<T>*)gradR, pairs, targets, fill, (cpx<double>*)W, nxy);
+    {                                                                                                       \
+        constexpr int P = 64 / SplitOf<T, NZ>::value;                                                       \
+        hipLaunchKernelGGL((k_tc2x_C<T, NZ>), dim3((nxy + P - 1) / P, ntargets), dim3(kPencilBlock), 0, s,  \
+                           (const cpx<T>*)buf, (const cpx<T>*)occR, (const cpx<T>*)gradR, pairs, targets, fill, \
+                           (cpx<double>*)W, nxy);                                                           \
+    }
     TCX_SIZE_SWITCH(g.n3, TCX_C)
 #undef TCX_C
     TCX_HIP_CHECK(hipGetLastError());

@@ -294,9 +294,36 @@ void Context::ensure_gradR(int prec)
 }
 
 // ------------------------------------------------------------------ launch plans
-const Plan& Context::get_plan(const std::vector<int>& nb)
+// Launch shape: every launch pairs a block of consecutive targets (k,j) with a
+// block of consecutive sources (q,i).  Targets own their accumulation grid
+// within a launch, so launches of one apply are serialised on the stream and
+// W needs no atomics.  Auto shape: enough targets that pass C issues
+// kAutoTargetWaves wavefronts, then as many sources as kAutoBufferGrids pair
+// grids allow (channels = grids per pair record: 1 for hf2x, 4 for tc2x).
+void Context::resolve_shape(int term, long& tc, long& sc, long ntargets, long nsources) const
 {
-    auto key = std::make_pair(nb, max_pairs);
+    constexpr long kAutoTargetWaves = 8192, kAutoBufferGrids = 8192;
+    const int channels = term == 1 ? 4 : 1;
+    const long waves_per_target = ((long)geo.n1 * geo.n2 + 31) / 32;
+    tc = shape_targets[term] > 0 ? shape_targets[term] : (kAutoTargetWaves + waves_per_target - 1) / waves_per_target;
+    tc = std::max<long>(1, std::min(tc, std::max<long>(1, ntargets)));
+    sc = shape_sources[term] > 0 ? shape_sources[term] : std::max<long>(1, kAutoBufferGrids / channels / tc);
+    sc = std::max<long>(1, std::min(sc, std::max<long>(1, nsources)));
+    while (tc * sc > 65535) {
+        if (sc > 1)
+            sc = std::max<long>(1, 65535 / tc);
+        else
+            tc = 65535;
+    }
+}
+
+const Plan& Context::get_plan(const std::vector<int>& nb, int term)
+{
+    long nt = 0;
+    for (int k = 0; k < nk; ++k) nt += nb[k];
+    long tc, sc;
+    resolve_shape(term, tc, sc, nt, n_occ);
+    auto key = std::make_tuple(nb, tc, sc);
     auto it = plans.find(key);
     if (it != plans.end()) return it->second;
     Plan P;
@@ -307,44 +334,42 @@ const Plan& Context::get_plan(const std::vector<int>& nb)
         P.app_coef[k + 1] = P.app_coef[k] + (long long)nb[k] * npw[k];
     }
     P.n_app = (int)P.app_first[nk];
-    const int nbmax = std::max(1, *std::max_element(nb.begin(), nb.end()));
-    const int noccmax = std::max(1, *std::max_element(nocc.begin(), nocc.end()));
-    // Enough accumulation targets per launch to fill the chip in pass C,
-    // then as many q as the pair budget allows.
-    const long target_goal = std::max<long>(1, (2048L * 64) / std::max<long long>(1, (long long)geo.n1 * geo.n2));
-    int kc = (int)std::min<long>(nk, std::max<long>(1, (target_goal + nbmax - 1) / nbmax));
-    while (kc > 1 && (long)kc * nbmax * noccmax > max_pairs) --kc;
-    int qc = (int)std::max<long>(1, std::min<long>(nk, max_pairs / ((long)kc * nbmax * noccmax)));
-    for (int k0 = 0; k0 < nk; k0 += kc) {
-        const int k1 = std::min(nk, k0 + kc);
-        for (int q0 = 0; q0 < nk; q0 += qc) {
-            const int q1 = std::min(nk, q0 + qc);
+    struct Item {
+        int k, idx;
+    };
+    std::vector<Item> tg, src;  // targets (k, applied band), sources (q, occupied orbital)
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nb[k]; ++j) tg.push_back({k, (int)P.app_first[k] + j});
+    for (int q = 0; q < nk; ++q)
+        for (int i = 0; i < nocc[q]; ++i) src.push_back({q, (int)occ_first[q] + i});
+    for (size_t t0 = 0; t0 < tg.size(); t0 += tc) {
+        const size_t t1 = std::min(tg.size(), t0 + (size_t)tc);
+        const int kf = tg[t0].k, kl = tg[t1 - 1].k;
+        for (size_t s0 = 0; s0 < src.size(); s0 += sc) {
+            const size_t s1 = std::min(src.size(), s0 + (size_t)sc);
+            const int qf = src[s0].k, ql = src[s1 - 1].k;
             Launch L;
             L.pair0 = (int)P.pairs.size();
             L.target0 = (int)P.targets.size();
             L.kq0 = (int)P.kq.size() / 3;
-            L.nkq = (k1 - k0) * (q1 - q0);
-            for (int k = k0; k < k1; ++k)
-                for (int q = q0; q < q1; ++q)
+            L.nkq = (kl - kf + 1) * (ql - qf + 1);
+            for (int k = kf; k <= kl; ++k)
+                for (int q = qf; q <= ql; ++q)
                     for (int d = 0; d < 3; ++d) P.kq.push_back(kcart[3 * k + d] - kcart[3 * q + d]);
             int local = 0;
-            for (int k = k0; k < k1; ++k) {
-                for (int j = 0; j < nb[k]; ++j) {
-                    TargetEntry te{(int)P.app_first[k] + j, local, 0, 0};
-                    for (int q = q0; q < q1; ++q) {
-                        const int kq = L.kq0 + (k - k0) * (q1 - q0) + (q - q0);
-                        for (int i = 0; i < nocc[q]; ++i) {
-                            P.pairs.push_back({(int)P.app_first[k] + j, (int)occ_first[q] + i, kq, 0});
-                            ++local;
-                            ++te.npair;
-                        }
-                    }
-                    if (te.npair > 0) P.targets.push_back(te);
+            for (size_t t = t0; t < t1; ++t) {
+                TargetEntry te{tg[t].idx, local, 0, 0};
+                for (size_t s = s0; s < s1; ++s) {
+                    const int kq = L.kq0 + (tg[t].k - kf) * (ql - qf + 1) + (src[s].k - qf);
+                    P.pairs.push_back({tg[t].idx, src[s].idx, kq, 0});
+                    ++local;
+                    ++te.npair;
                 }
+                P.targets.push_back(te);
             }
             L.npairs = local;
-            L.ntargets = (int)P.targets.size() - L.target0;
-            if (L.npairs > 0) P.launches.push_back(L);
+            L.ntargets = (int)(t1 - t0);
+            P.launches.push_back(L);
         }
     }
     for (const auto& L : P.launches) P.max_launch_pairs = std::max(P.max_launch_pairs, L.npairs);
@@ -440,7 +465,7 @@ void Context::hf2x(const int* nb_, const double* phi, double* out, int prec, uns
 {
     check_apply_args(nb_, phi, out, prec);
     std::vector<int> nb(nb_, nb_ + nk);
-    const Plan& P = get_plan(nb);
+    const Plan& P = get_plan(nb, 0);
     ApplyIO io = begin_apply(nb_, phi, out, flags, P);
     if (P.n_app > 0) {
         ensure_occR(prec);
@@ -619,9 +644,12 @@ int tcx_memcpy(tcx_context* ctx, void* dst, const void* src, unsigned long long 
         if (kind < 0 || kind > 2) throw Error(TCX_ERR_ARG, "memcpy: kind must be 0, 1 or 2");
         if (bytes == 0) return;
         if (!dst || !src) throw Error(TCX_ERR_ARG, "memcpy: null pointer");
-        const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
-                                          : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
-        TCX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, ctx->impl.stream));
+        if (kind == 2) {
+            launch_copy(dst, src, bytes, ctx->impl.stream);  // streaming kernel, not a DMA engine
+        } else {
+            const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+            TCX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, ctx->impl.stream));
+        }
         TCX_HIP_CHECK(hipStreamSynchronize(ctx->impl.stream));
     });
 }
@@ -636,11 +664,14 @@ int tcx_memset_zero(tcx_context* ctx, void* ptr, unsigned long long bytes)
     });
 }
 
-int tcx_set_max_pairs(tcx_context* ctx, long max_pairs)
+int tcx_set_launch_shape(tcx_context* ctx, int term, long targets, long sources)
 {
     return guarded(ctx, [&] {
-        if (max_pairs < 1 || max_pairs > 65535) throw Error(TCX_ERR_ARG, "max_pairs must be in [1, 65535]");
-        ctx->impl.max_pairs = max_pairs;
+        if (term < 0 || term > 1) throw Error(TCX_ERR_ARG, "launch_shape: term must be 0 (hf2x) or 1 (tc2x)");
+        if (targets < 0 || sources < 0 || targets > 65535 || sources > 65535)
+            throw Error(TCX_ERR_ARG, "launch_shape: targets and sources must be in [0, 65535]");
+        ctx->impl.shape_targets[term] = targets;
+        ctx->impl.shape_sources[term] = sources;
     });
 }
 

@@ -76,6 +76,7 @@ struct DivEntry {
 // prec: TCX_PREC_FP64 (double grids) or TCX_PREC_FP32 (float grids).
 
 void launch_zero(void* p, size_t bytes, hipStream_t s);
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // scatter complex128 coefficients to double grids (grids pre-zeroed), grid i <- job i;
 // a job with dir >= 0 multiplies by the Cartesian component dir of (k+G).

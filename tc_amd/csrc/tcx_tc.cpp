@@ -13,7 +13,7 @@ void Context::tc2x(const int* nb_, const double* phi, double* out, int prec, uns
     require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc2x: the cell was set up for HF");
     require(jastrow_set, TCX_ERR_STATE, "tc2x: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);
-    const Plan& P = get_plan(nb);
+    const Plan& P = get_plan(nb, 1);
     ApplyIO io = begin_apply(nb_, phi, out, flags, P);
     if (P.n_app > 0) {
         const bool bitc = method == TCX_METHOD_BITC;
@@ -79,7 +79,7 @@ void Context::tc3_local(int which, const int* nb_, const double* phi, double* ou
     require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc3: the cell was set up for HF");
     require(jastrow_set, TCX_ERR_STATE, "tc3: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);
-    const Plan& P = get_plan(nb);
+    const Plan& P = get_plan(nb, 0);  // only the per-band job tables are used
     ApplyIO io = begin_apply(nb_, phi, out, flags, P);
     if (P.n_app > 0) {
         ensure_density();

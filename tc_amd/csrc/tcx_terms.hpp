@@ -6,6 +6,7 @@
 #include <functional>
 #include <map>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -114,8 +115,10 @@ struct Context {
     DBuf d_dc;       // divergence factor per k
     DBuf d_phi_app, d_out, d_appR, d_appGrad, d_tmp64, d_W, d_pairbuf, d_ktab, d_work;
 
-    long max_pairs = 4096;
-    std::map<std::pair<std::vector<int>, long>, Plan> plans;
+    // launch shape per term (0 hf2x, 1 tc2x): targets x sources per launch, 0 = auto
+    long shape_targets[2] = {0, 0};
+    long shape_sources[2] = {0, 0};
+    std::map<std::tuple<std::vector<int>, long, long>, Plan> plans;
 
     // timing
     bool timing_on = false;
@@ -151,7 +154,8 @@ struct Context {
     JastrowRPA jastrow_pair(int s1, int s2) const { return JastrowRPA{A[s1 * 2 + s2], F[s1 * 2 + s2], jastrow_zero ? 1 : 0}; }
     void ensure_density();
 
-    const Plan& get_plan(const std::vector<int>& nb);
+    void resolve_shape(int term, long& tc, long& sc, long ntargets, long nsources) const;
+    const Plan& get_plan(const std::vector<int>& nb, int term);
     void check_apply_args(const int* nb, const double* phi, const double* out, int prec);
     ApplyIO begin_apply(const int* nb, const double* phi, double* out, unsigned flags, const Plan& P);
     void end_apply(const ApplyIO& io);

@@ -39,7 +39,7 @@ EXPORTS = [
     "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux",
     "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1",
     "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
-    "tcx_set_max_pairs", "tcx_enable_timing", "tcx_get_timing",
+    "tcx_set_launch_shape", "tcx_enable_timing", "tcx_get_timing",
 ]
 
 
@@ -82,7 +82,7 @@ def load_library():
     L.tcx_device_free.argtypes = [vp, vp]
     L.tcx_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong, C.c_int]
     L.tcx_memset_zero.argtypes = [vp, vp, C.c_ulonglong]
-    L.tcx_set_max_pairs.argtypes = [vp, C.c_long]
+    L.tcx_set_launch_shape.argtypes = [vp, C.c_int, C.c_long, C.c_long]
     L.tcx_enable_timing.argtypes = [vp, C.c_int]
     L.tcx_get_timing.argtypes = [vp, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
     _lib = L
@@ -217,8 +217,10 @@ class Engine:
         return self._term("tc3b1", nb, phi, out, precision)
 
     # --------------------------------------------------------------- tuning / timing
-    def set_max_pairs(self, n):
-        self._check(self.L.tcx_set_max_pairs(self.h, int(n)), "set_max_pairs")
+    def set_launch_shape(self, term, targets=0, sources=0):
+        """term: "hf2x" or "tc2x"; 0 = automatic."""
+        t = {"hf2x": 0, "tc2x": 1}[term] if isinstance(term, str) else int(term)
+        self._check(self.L.tcx_set_launch_shape(self.h, t, int(targets), int(sources)), "set_launch_shape")
 
     def enable_timing(self, on=True):
         self._check(self.L.tcx_enable_timing(self.h, int(bool(on))), "enable_timing")

@@ -67,11 +67,11 @@ def test_hf2x_accumulates_and_is_plan_independent(engine):
     a = engine.hf2x(nb, phi)
     b = engine.hf2x(nb, phi, out=a.copy())
     assert np.abs(b - 2 * a).max() <= 1e-14 * np.abs(a).max()
-    for mp in (1, 7, 40):
-        engine.set_max_pairs(mp)
+    for shape in ((1, 1), (3, 5), (40, 2), (7, 40), (0, 3)):
+        engine.set_launch_shape("hf2x", *shape)
         c = engine.hf2x(nb, phi)
-        assert np.abs(c - a).max() <= 1e-12 * np.abs(a).max(), mp
-    engine.set_max_pairs(4096)
+        assert np.abs(c - a).max() <= 1e-12 * np.abs(a).max(), shape
+    engine.set_launch_shape("hf2x", 0, 0)
 
 
 def test_hf2x_empty_and_ragged_bands(engine):
