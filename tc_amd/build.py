@@ -23,19 +23,24 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I", CSRC, "-I", INCLUDE, "-Wall", "-Wno-unused-parameter"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
+# paths relative to tc_amd/
 SOURCES = [
-    "kernels_fft.hip",
-    "kernels_hf2x.hip",
-    "kernels_tc2x.hip",
-    "kernels_local.hip",
-    "tcx_api.cpp",
-    "tcx_tc.cpp",
+    "csrc/kernels_fft.hip",
+    "csrc/kernels_hf2x.hip",
+    "csrc/kernels_tc2x.hip",
+    "csrc/kernels_local.hip",
+    "csrc/tcx_api.cpp",
+    "csrc/tcx_tc.cpp",
+    "host/calc_hamiltonian.cpp",
 ]
+ROOT = os.path.dirname(HERE)
+SHIM_TEST_SRC = os.path.join(ROOT, "tests", "host_shim_test.cpp")
+SHIM_TEST_BIN = os.path.join(ROOT, "tests", "_bin", "host_shim_test")
 
 
 def _compile(src: str) -> str:
-    path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    path = os.path.join(HERE, src)
+    obj = os.path.join(BUILD, os.path.splitext(os.path.basename(src))[0] + ".o")
     if src.endswith(".hip"):
         cmd = [HIPCC, *DEVICE, *COMMON, "-c", path, "-o", obj]
     else:
@@ -63,7 +68,21 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_shim_test(verbose)
     return LIB
+
+
+def build_shim_test(verbose: bool = True) -> str:
+    """C++ program driving the host mirror calc_hamiltonian::* (run by a GPU test)."""
+    os.makedirs(os.path.dirname(SHIM_TEST_BIN), exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", SHIM_TEST_SRC, "-o", SHIM_TEST_BIN, "-L", HERE, "-ltcx",
+           "-Wl,-rpath,$ORIGIN/../../tc_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"shim test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {SHIM_TEST_BIN}")
+    return SHIM_TEST_BIN
 
 
 if __name__ == "__main__":

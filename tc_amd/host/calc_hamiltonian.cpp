@@ -1,0 +1,150 @@
+// calc_hamiltonian.cpp -- host mirror of TC++'s term functions over the C ABI (tcx.h).
+//
+// Each call packs the reference-shaped arguments into the ABI's ragged
+// complex128 arrays, pushes the state the term reads (cell, k-points, basis,
+// occupied orbitals, Jastrow, divergence correction) into a per-device
+// context, runs the term and adds the result into HNphi.
+#include "calc_hamiltonian.hpp"
+
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+namespace {
+
+tcx_context* context_for(int device)
+{
+    static std::map<int, tcx_context*> contexts;  // one per device for the process lifetime
+    auto it = contexts.find(device);
+    if (it != contexts.end()) return it->second;
+    tcx_context* ctx = nullptr;
+    if (tcx_create(&ctx, device) != TCX_OK) throw std::runtime_error("calc_hamiltonian: tcx_create failed");
+    contexts[device] = ctx;
+    return ctx;
+}
+
+void check(tcx_context* ctx, int status, const char* what)
+{
+    if (status != TCX_OK) throw std::runtime_error(std::string("calc_hamiltonian::") + what + ": " + tcx_last_error(ctx));
+}
+
+int method_id(const std::string& m)
+{
+    if (m == "HF") return TCX_METHOD_HF;
+    if (m == "TC") return TCX_METHOD_TC;
+    if (m == "BITC") return TCX_METHOD_BITC;
+    throw std::runtime_error("calc_hamiltonian: unknown calc_method " + m);
+}
+
+// pack [k][band][spinor=0] coefficient vectors of spin 0 into one ragged array
+void pack_bands(const std::vector<std::vector<std::vector<VectorXcd>>>& bands_of_k, const std::vector<int>& count,
+                std::vector<double>& out)
+{
+    out.clear();
+    for (size_t k = 0; k < bands_of_k.size(); ++k)
+        for (int b = 0; b < count[k]; ++b) {
+            const VectorXcd& v = bands_of_k[k][b][0];
+            const double* d = reinterpret_cast<const double*>(v.data());
+            out.insert(out.end(), d, d + 2 * v.size());
+        }
+}
+
+void push_state(tcx_context* ctx, const Method& method, const CrystalStructure& cs, const Potentials& pot,
+                const Spin& spin, const Kpoints& kp, const PlaneWaveBasis& pw, const BlochStates& bs)
+{
+    if (spin.num_independent_spins != 1) throw std::runtime_error("calc_hamiltonian: only one independent spin");
+    if (method.calc_mode != "SCF") throw std::runtime_error("calc_hamiltonian: only SCF mode");
+    const int nk = (int)kp.kvectors_scf.size();
+    double recip[9];
+    for (int i = 0; i < 3; ++i)
+        for (int d = 0; d < 3; ++d) recip[3 * i + d] = cs.reciprocal_vectors[i][d];
+    check(ctx, tcx_set_cell(ctx, recip, cs.unit_cell_volume, pw.size_FFT_grid_vec, 1, method_id(method.calc_method)),
+          "set_cell");
+    std::vector<double> kf;
+    for (const auto& k : kp.kvectors_scf) kf.insert(kf.end(), k.begin(), k.begin() + 3);
+    check(ctx, tcx_set_kpoints(ctx, nk, kf.data(), kp.num_kpoints), "set_kpoints");
+    std::vector<int> npw(nk), gidx;
+    for (int k = 0; k < nk; ++k) {
+        npw[k] = (int)pw.Gindex_at_k_scf[k].size();
+        gidx.insert(gidx.end(), pw.Gindex_at_k_scf[k].begin(), pw.Gindex_at_k_scf[k].end());
+    }
+    check(ctx, tcx_set_basis(ctx, npw.data(), gidx.data()), "set_basis");
+    // occupied orbitals: bands < num_occupied_bands (BlochStates::set_filling, f > 1e-8)
+    std::vector<int> nocc(bs.num_occupied_bands[0].begin(), bs.num_occupied_bands[0].end());
+    std::vector<double> fill, phi_occ, chi_occ;
+    for (int k = 0; k < nk; ++k)
+        for (int b = 0; b < nocc[k]; ++b) fill.push_back(bs.filling[0][k][b]);
+    pack_bands(bs.phik_scf[0], nocc, phi_occ);
+    const bool bitc = method.calc_method == "BITC";
+    if (bitc) pack_bands(bs.phik_left_scf[0], nocc, chi_occ);
+    check(ctx, tcx_set_states(ctx, nocc.data(), fill.data(), phi_occ.data(), bitc ? chi_occ.data() : nullptr),
+          "set_states");
+    double A[4], F[4];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            A[2 * i + j] = pot.jastrow.A_long[i][j];
+            F[2 * i + j] = pot.jastrow.F_long[i][j];
+        }
+    check(ctx, tcx_set_jastrow_rpa(ctx, A, F), "set_jastrow_rpa");
+    check(ctx, tcx_set_div_correction(ctx, pot.includes_div_correction ? 1 : 0, pot.alpha_Vaux,
+                                      pot.sum_of_Vaux_scf.empty() ? nullptr : pot.sum_of_Vaux_scf.data()),
+          "set_div_correction");
+}
+
+using TermFn = int (*)(tcx_context*, const int*, const double*, double*, int, unsigned);
+
+void run_term(TermFn fn, const char* name, const Parallelization& par, const Method& method,
+              const CrystalStructure& cs, const Potentials& pot, const Spin& spin, const Kpoints& kp,
+              const PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi, Phi4& HNphi)
+{
+    tcx_context* ctx = context_for(par.device);
+    push_state(ctx, method, cs, pot, spin, kp, pw, bs);
+    const int nk = (int)kp.kvectors_scf.size();
+    std::vector<int> nb(nk);
+    for (int k = 0; k < nk; ++k) nb[k] = (int)phi[0][k].size();
+    std::vector<double> in, out;
+    pack_bands(phi[0], nb, in);
+    pack_bands(HNphi[0], nb, out);  // the term accumulates into the current H*phi
+    check(ctx, fn(ctx, nb.data(), in.data(), out.data(), method.precision, 0u), name);
+    size_t off = 0;
+    for (int k = 0; k < nk; ++k)
+        for (int b = 0; b < nb[k]; ++b) {
+            VectorXcd& v = HNphi[0][k][b][0];
+            std::memcpy(reinterpret_cast<double*>(v.data()), out.data() + off, sizeof(double) * 2 * v.size());
+            off += 2 * v.size();
+        }
+}
+
+}  // namespace
+
+namespace calc_hamiltonian {
+
+void hf2x(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,
+          const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi,
+          Phi4& H2phi, std::ostream*)
+{
+    run_term(tcx_hf2x, "hf2x", par, m, cs, pot, spin, kp, pw, bs, phi, H2phi);
+}
+
+void tc2x(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,
+          const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi,
+          Phi4& H2phi, std::ostream*)
+{
+    run_term(tcx_tc2x, "tc2x", par, m, cs, pot, spin, kp, pw, bs, phi, H2phi);
+}
+
+void tc3a1(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,
+           const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi,
+           Phi4& H3phi, std::ostream*)
+{
+    run_term(tcx_tc3a1, "tc3a1", par, m, cs, pot, spin, kp, pw, bs, phi, H3phi);
+}
+
+void tc3b1(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,
+           const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi,
+           Phi4& H3phi, std::ostream*)
+{
+    run_term(tcx_tc3b1, "tc3b1", par, m, cs, pot, spin, kp, pw, bs, phi, H3phi);
+}
+
+}  // namespace calc_hamiltonian

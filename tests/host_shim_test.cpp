@@ -1,0 +1,123 @@
+// host_shim_test.cpp -- drives the C++ host mirror (calc_hamiltonian::hf2x / tc2x)
+// exactly as TC++'s calc_hamiltonian::all would, on a plane-wave system whose
+// exchange operator has a closed form:
+//   occupied orbitals = single plane waves e^{i(q+G_b)r}, applied bands = every
+//   plane wave of the sphere at k  =>  Vx is diagonal with
+//   <k+G0|Vx|k+G0> = c_x sum_{q,b} f_qb / |k+G0-q-G_b|^2  (0 below 1e-8),
+//   c_x = -4 pi/(Nk V) / 2   (src/calc_hamiltonian_hf2x.cpp:29-30, 91-147).
+// Also checks tc2x with A = 0 against hf2x (src/jastrow_functions.cpp:30-38).
+// Exit status 0 on success.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../tc_amd/host/calc_hamiltonian.hpp"
+
+int main()
+{
+    const double PI = 3.14159265358979323846, a = 6.0;
+    const int n = 8;
+    CrystalStructure cs;
+    for (int i = 0; i < 3; ++i) cs.reciprocal_vectors[i][i] = 2 * PI / a;
+    cs.unit_cell_volume = a * a * a;
+    Kpoints kp;
+    kp.kvectors_scf = {{0.0, 0.0, 0.0}, {0.5, 0.0, 0.0}, {0.0, 0.5, 0.5}};
+    kp.num_kpoints = 3;
+    PlaneWaveBasis pw;
+    pw.size_FFT_grid_vec[0] = pw.size_FFT_grid_vec[1] = pw.size_FFT_grid_vec[2] = n;
+    const double gw = (n - 1) / 4.0 * 2 * PI / a;
+    std::vector<std::vector<std::vector<int>>> mill(3);  // [k][pw] Miller triple
+    for (int k = 0; k < 3; ++k) {
+        std::vector<int> idx;
+        for (int i3 = 0; i3 < n; ++i3)
+            for (int i2 = 0; i2 < n; ++i2)
+                for (int i1 = 0; i1 < n; ++i1) {
+                    const int m[3] = {i1 <= n / 2 ? i1 : i1 - n, i2 <= n / 2 ? i2 : i2 - n, i3 <= n / 2 ? i3 : i3 - n};
+                    double g2 = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        const double c = 2 * PI / a * (kp.kvectors_scf[k][d] + m[d]);
+                        g2 += c * c;
+                    }
+                    if (g2 <= gw * gw) {
+                        idx.push_back(i1 + n * (i2 + n * i3));
+                        mill[k].push_back({m[0], m[1], m[2]});
+                    }
+                }
+        pw.Gindex_at_k_scf.push_back(idx);
+    }
+    const int nocc = 2;
+    BlochStates bs;
+    bs.phik_scf.resize(1);
+    bs.filling.resize(1);
+    bs.num_occupied_bands.resize(1);
+    Phi4 phi(1), H(1);
+    for (int k = 0; k < 3; ++k) {
+        const int npw = (int)pw.Gindex_at_k_scf[k].size();
+        bs.phik_scf[0].emplace_back();
+        bs.filling[0].emplace_back();
+        for (int b = 0; b < nocc; ++b) {
+            VectorXcd v(npw, 0.0);
+            v[b] = 1.0;
+            bs.phik_scf[0][k].push_back({v});
+            bs.filling[0][k].push_back(2.0);
+        }
+        bs.num_occupied_bands[0].push_back(nocc);
+        phi[0].emplace_back();
+        H[0].emplace_back();
+        for (int j = 0; j < npw; ++j) {
+            VectorXcd v(npw, 0.0);
+            v[j] = 1.0;
+            phi[0][k].push_back({v});
+            H[0][k].push_back({VectorXcd(npw, 0.0)});
+        }
+    }
+    Parallelization par;
+    Method method;
+    method.calc_method = "HF";
+    method.precision = TCX_PREC_FP64;
+    Potentials pot;
+    pot.includes_div_correction = false;
+    Spin spin;
+    calc_hamiltonian::hf2x(par, method, cs, pot, spin, kp, pw, bs, phi, H, nullptr);
+
+    const double cx = -4 * PI / (kp.num_kpoints * cs.unit_cell_volume) / 2.0;
+    double err = 0.0, scale = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const int npw = (int)pw.Gindex_at_k_scf[k].size();
+        for (int j = 0; j < npw; ++j) {
+            double expect = 0.0;
+            for (int q = 0; q < 3; ++q)
+                for (int b = 0; b < nocc; ++b) {
+                    double d2 = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        const double c = 2 * PI / a *
+                                         (kp.kvectors_scf[k][d] + mill[k][j][d] - kp.kvectors_scf[q][d] - mill[q][b][d]);
+                        d2 += c * c;
+                    }
+                    if (d2 >= 1e-8) expect += 2.0 / d2;
+                }
+            expect *= cx;
+            for (int l = 0; l < npw; ++l) {
+                const double want = l == j ? expect : 0.0;
+                err = std::fmax(err, std::abs(H[0][k][j][0][l] - want));
+            }
+            scale = std::fmax(scale, std::fabs(expect));
+        }
+    }
+    std::printf("hf2x closed form: max |err| = %.3e (scale %.3e)\n", err, scale);
+    if (!(err <= 1e-11 * scale)) return 1;
+
+    // tc2x without a Jastrow equals hf2x; accumulation adds onto H
+    Phi4 H2 = H;
+    method.calc_method = "TC";
+    calc_hamiltonian::tc2x(par, method, cs, pot, spin, kp, pw, bs, phi, H2, nullptr);
+    double err2 = 0.0;
+    for (int k = 0; k < 3; ++k)
+        for (size_t j = 0; j < H[0][k].size(); ++j)
+            for (size_t l = 0; l < H[0][k][j][0].size(); ++l)
+                err2 = std::fmax(err2, std::abs(H2[0][k][j][0][l] - 2.0 * H[0][k][j][0][l]));
+    std::printf("tc2x(A=0) accumulated onto hf2x: max |H2 - 2 H| = %.3e\n", err2);
+    if (!(err2 <= 1e-11 * scale)) return 1;
+    std::printf("OK\n");
+    return 0;
+}

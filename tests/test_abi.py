@@ -58,3 +58,12 @@ def test_null_context_is_an_error_not_a_crash():
     assert L.tcx_destroy(None) == tcx.TCX_ERR_ARG
     assert L.tcx_synchronize(None) == tcx.TCX_ERR_ARG
     assert L.tcx_last_error(None) == b"null context"
+
+
+def test_host_shim_exports_reference_names():
+    """The C++ mirror keeps the reference's calc_hamiltonian:: names (mangled C++ symbols)."""
+    import subprocess
+    out = subprocess.run(["nm", "-DC", tcx.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    for name in ("calc_hamiltonian::hf2x(", "calc_hamiltonian::tc2x(", "calc_hamiltonian::tc3a1(",
+                 "calc_hamiltonian::tc3b1("):
+        assert name in out, name
