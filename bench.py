@@ -36,16 +36,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def read_traffic(name):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc summary, if present."""
+def read_traffic(name, pairs_per_launch):
+    """HBM bytes per launch of a kernel: the calibrated FETCH_SIZE + WRITE_SIZE
+    bytes per pair of the committed rocprofv3 --pmc summary times the pairs of
+    one launch (None when no summary is present)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f).get(name)
+            entry = json.load(f).get(name)
     except (OSError, ValueError):
         return None
+    if not entry or "bytes_per_pair" not in entry:
+        return None
+    return float(entry["bytes_per_pair"]) * pairs_per_launch
 
 
 def cpu_baseline(sys_, threads):
@@ -173,7 +178,7 @@ def main():
     bytes_per_launch = 2.0 * N * 8 * pairs_per_launch  # each pass: one read + one write of a FP32 pair grid
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     pipeline_gbs = 6.0 * N * 8 * pairs_per_step / (ms_per_step * 1e-3) / 1e9
-    traffic = read_traffic(CLASS_NAMES[dom])
+    traffic = read_traffic(CLASS_NAMES[dom], pairs_per_launch)
 
     # FP64 apply for the ratio, then parity of the two outputs
     log(f"bench: FP64 apply x{args.fp64_steps}")

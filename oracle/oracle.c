@@ -528,6 +528,141 @@ void orc_tc2x(const orc_problem *p, const int *nb, const double *phi_d, double *
 }
 
 /* ------------------------------------------------------------------ */
+/* kinetic, calc_hamiltonian_kinetic.cpp:6-52: H1 += |k+G|^2/2 phi      */
+/* ------------------------------------------------------------------ */
+
+void orc_kinetic(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *phi = (const cplx *)phi_d;
+    cplx *hphi = (cplx *)hphi_d;
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+    for (int ik = 0; ik < p->nk; ++ik) {
+        const long ao = app_coef_offset(p, nb, ik);
+        const int *gi = p->gindex + pw_offset(p, ik);
+        double kc[3];
+        kcart(p, p->kfrac + 3 * ik, kc);
+        for (int jb = 0; jb < nb[ik]; ++jb)
+            for (int l = 0; l < p->npw[ik]; ++l) {
+                double k2 = 0;
+                for (int d = 0; d < 3; ++d) {
+                    double c = kc[d] + G[3 * (long)gi[l] + d];
+                    k2 += c * c;
+                }
+                long o = ao + (long)jb * p->npw[ik] + l;
+                hphi[o] += 0.5 * k2 * phi[o]; /* :44-45 */
+            }
+    }
+    free(G);
+}
+
+/* ------------------------------------------------------------------ */
+/* local potential applied to phi on the FFT grid                       */
+/* ------------------------------------------------------------------ */
+
+static void apply_local_potential(const orc_problem *p, const int *nb, const double *phi_d,
+                                  double *hphi_d, const cplx *V, double factor);
+
+/* hf2h, calc_hamiltonian_hf2h.cpp:8-85: V_H = FFT_backward[4 pi n(G)/G^2], G=0 dropped */
+void orc_hf2h(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *dens = (const cplx *)p->density;
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+    cplx *V = malloc(sizeof(cplx) * N);
+    for (long r = 0; r < N; ++r) V[r] = dens[r] + dens[N + r];     /* :28 */
+    orc_fft3d((double *)V, p->n, -1);                                /* :29 */
+    V[0] = 0.0;                                                      /* :30 */
+    for (long r = 1; r < N; ++r) {                                   /* :31-38 */
+        double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
+        V[r] *= FOURPI / g2;
+    }
+    orc_fft3d((double *)V, p->n, +1);                                /* :39 */
+    apply_local_potential(p, nb, phi_d, hphi_d, V, 1.0 / (p->num_kpoints * p->volume)); /* :23, :55-82 */
+    free(G); free(V);
+}
+
+/* tc2h, calc_hamiltonian_tc2h.cpp:10-236 (one independent spin) */
+void orc_tc2h(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *dens = (const cplx *)p->density;
+    const cplx *phi = (const cplx *)phi_d;
+    cplx *hphi = (cplx *)hphi_d;
+    const cplx *occ = (const cplx *)p->phi_occ;
+    const int bitc = p->method == 2;
+    const cplx *occl = (bitc && p->chi_occ) ? (const cplx *)p->chi_occ : occ;
+    const double factor = 1.0 / (p->num_kpoints * p->volume);        /* :32 */
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+    cplx *orb = malloc(sizeof(cplx) * N), *gorb = malloc(sizeof(cplx) * 3 * N);
+    cplx *pgp = calloc(3 * N, sizeof(cplx)), *V2 = calloc(N, sizeof(cplx)), *dnu = calloc(3 * N, sizeof(cplx));
+    /* phiq_gphiq_d = sum_{q,i} f conj(chi_qi) FFT_backward((q+G)_d phi_qi)  (:60-127) */
+    for (int iq = 0; iq < p->nk; ++iq) {
+        const long co = occ_coef_offset(p, iq), fo = occ_offset(p, iq);
+        double qc[3];
+        kcart(p, p->kfrac + 3 * iq, qc);
+        for (int ib = 0; ib < p->nocc[iq]; ++ib) {
+            scatter(p, iq, occ + co + (long)ib * p->npw[iq], orb, N);
+            for (int d = 0; d < 3; ++d)
+                for (long r = 0; r < N; ++r) gorb[d * N + r] = (qc[d] + G[3 * r + d]) * orb[r];
+            scatter(p, iq, occl + co + (long)ib * p->npw[iq], orb, N);
+            orc_fft3d((double *)orb, p->n, +1);
+            const double f = p->filling[fo + ib];
+            for (long r = 0; r < N; ++r) orb[r] = f * conj(orb[r]);
+            for (int d = 0; d < 3; ++d) {
+                orc_fft3d((double *)(gorb + d * N), p->n, +1);
+                for (long r = 0; r < N; ++r) pgp[d * N + r] += orb[r] * gorb[d * N + r];
+            }
+        }
+    }
+    for (int d = 0; d < 3; ++d) orc_fft3d((double *)(pgp + d * N), p->n, -1); /* :125 */
+    const double spin_factor = p->nspin == 1 ? 2.0 : 1.0;           /* :145 */
+    for (int js = 0; js < 2; ++js) {                                 /* :146-171 */
+        memcpy(orb, dens + js * N, sizeof(cplx) * N);
+        orc_fft3d((double *)orb, p->n, -1);
+        for (long r = 0; r < N; ++r) {
+            double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
+            V2[r] += orc_tc_2body(p, g2, 0, js) * orb[r];
+            double u = orc_uk(p, g2, 0, js);
+            for (int d = 0; d < 3; ++d) {
+                V2[r] += u * G[3 * r + d] * pgp[d * N + r] / spin_factor;
+                dnu[d * N + r] += u * G[3 * r + d] * orb[r];
+            }
+        }
+    }
+    orc_fft3d((double *)V2, p->n, +1);                                /* :172 */
+    for (int d = 0; d < 3; ++d) orc_fft3d((double *)(dnu + d * N), p->n, +1);
+    cplx *phij = malloc(sizeof(cplx) * N);
+    for (int ik = 0; ik < p->nk; ++ik) {                              /* :178-234 */
+        const long ao = app_coef_offset(p, nb, ik);
+        const int npwk = p->npw[ik];
+        const int *gi = p->gindex + pw_offset(p, ik);
+        double kc[3];
+        kcart(p, p->kfrac + 3 * ik, kc);
+        for (int jb = 0; jb < nb[ik]; ++jb) {
+            scatter(p, ik, phi + ao + (long)jb * npwk, phij, N);
+            for (int d = 0; d < 3; ++d) {
+                for (long r = 0; r < N; ++r) gorb[d * N + r] = (kc[d] + G[3 * r + d]) * phij[r];
+                orc_fft3d((double *)(gorb + d * N), p->n, +1);
+            }
+            orc_fft3d((double *)phij, p->n, +1);
+            for (long r = 0; r < N; ++r) {
+                cplx t = V2[r] * phij[r];
+                for (int d = 0; d < 3; ++d) t -= gorb[d * N + r] * dnu[d * N + r];
+                orb[r] = t;
+            }
+            orc_fft3d((double *)orb, p->n, -1);
+            cplx *out = hphi + ao + (long)jb * npwk;
+            for (int l = 0; l < npwk; ++l) out[l] += factor * orb[gi[l]];
+        }
+    }
+    free(G); free(orb); free(gorb); free(pgp); free(V2); free(dnu); free(phij);
+}
+
+/* ------------------------------------------------------------------ */
 /* tc3a1, calc_hamiltonian_tc3a1.cpp:8-117                              */
 /* ------------------------------------------------------------------ */
 

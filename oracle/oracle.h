@@ -87,6 +87,11 @@ void orc_hf2x(const orc_problem *p, const int *nb, const double *phi, double *hp
 void orc_tc2x(const orc_problem *p, const int *nb, const double *phi, double *hphi, int nthreads);
 void orc_tc3a1(const orc_problem *p, const int *nb, const double *phi, double *hphi);
 void orc_tc3b1(const orc_problem *p, const int *nb, const double *phi, double *hphi);
+/* Assembly terms: kinetic (src/calc_hamiltonian_kinetic.cpp:6-52), Hartree
+ * (src/calc_hamiltonian_hf2h.cpp:8-85), TC Hartree (src/calc_hamiltonian_tc2h.cpp:10-236). */
+void orc_kinetic(const orc_problem *p, const int *nb, const double *phi, double *hphi);
+void orc_hf2h(const orc_problem *p, const int *nb, const double *phi, double *hphi);
+void orc_tc2h(const orc_problem *p, const int *nb, const double *phi, double *hphi);
 
 #ifdef __cplusplus
 }

@@ -70,7 +70,7 @@ def lib():
         L.orc_density.argtypes = [P, _dp]
         for name in ("orc_hf2x", "orc_tc2x"):
             getattr(L, name).argtypes = [P, _ip, _dp, _dp, C.c_int]
-        for name in ("orc_tc3a1", "orc_tc3b1"):
+        for name in ("orc_tc3a1", "orc_tc3b1", "orc_kinetic", "orc_hf2h", "orc_tc2h"):
             getattr(L, name).argtypes = [P, _ip, _dp, _dp]
         _lib = L
     return _lib
@@ -183,6 +183,15 @@ class Oracle:
 
     def tc3b1(self, nb, phi):
         return self._apply("orc_tc3b1", nb, phi)
+
+    def kinetic(self, nb, phi):
+        return self._apply("orc_kinetic", nb, phi)
+
+    def hf2h(self, nb, phi):
+        return self._apply("orc_hf2h", nb, phi)
+
+    def tc2h(self, nb, phi):
+        return self._apply("orc_tc2h", nb, phi)
 
     def uk(self, G2, s1=0, s2=0):
         return lib().orc_uk(C.byref(self.P), float(G2), s1, s2)

@@ -137,6 +137,32 @@ struct SplitOf {
     static constexpr int value = (sizeof(T) == 8 && NZ % 4 == 0) ? 4 : 2;
 };
 
+// Planes per workgroup for the xy-plane passes: one thread per line of a
+// plane, so a workgroup of PL planes has PL*NP busy lanes; choose PL for the
+// best lane utilisation with the padded LDS tiles under kPlaneLdsCap bytes.
+constexpr int kPlaneLdsCap = 80 * 1024;
+template <typename T, int NP>
+struct PlaneGroup {
+    static constexpr int tile_bytes = NP * (NP + 1) * (int)sizeof(cpx<T>);
+    static constexpr int pick()
+    {
+        int best = 1;
+        double best_util = 0.0;
+        for (int p = 1; p <= 64; ++p) {
+            if (p * tile_bytes > kPlaneLdsCap || p * NP > 1024) break;
+            const int lanes = ((p * NP + 63) / 64) * 64;
+            const double util = double(p * NP) / lanes;
+            if (util > best_util + 1e-9) {
+                best_util = util;
+                best = p;
+            }
+        }
+        return best;
+    }
+    static constexpr int planes = pick();
+    static constexpr int threads = ((planes * NP + 63) / 64) * 64;
+};
+
 // LDS tile of one plane: [y][x] with one element of padding per row.
 template <typename T, int NP>
 struct PlaneTile {

@@ -44,49 +44,64 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_A(const cpx<T>* __restric
     store_pencil<T, NZ>(v, buf + (size_t)blockIdx.y * ng + xy, nxy);
 }
 
-template <typename T, int NP>
-__global__ __launch_bounds__(kPlaneBlock) void k_hf2x_B(cpx<T>* __restrict__ buf, const PairEntry* __restrict__ pairs,
-                                                        const double* __restrict__ kqcart, int nz, Geom g,
-                                                        double invN)
+// Coulomb kernel table of a launch: v(G)/N = 1/(N |k-q+G|^2), 0 below 1e-8
+// (hf2x.cpp:91-104; the 1/N is FFT_forward's), one row per k-q of the launch,
+// evaluated in FP64 and rounded once to the working precision.
+template <typename T>
+__global__ __launch_bounds__(256) void k_coulomb_table(T* __restrict__ table, const double* __restrict__ kqcart,
+                                                       int row0, Geom g, long long N, double invN)
 {
-    __shared__ PlaneTile<T, NP> tile;
-    const int t = threadIdx.x;
-    const int z = blockIdx.x;
-    cpx<T>* plane = buf + ((size_t)blockIdx.y * nz + z) * (NP * NP);
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= N) return;
+    const int i1 = (int)(e % g.n1), i2 = (int)((e / g.n1) % g.n2), i3 = (int)(e / ((long long)g.n1 * g.n2));
+    const int m1 = centered(i1, g.n1), m2 = centered(i2, g.n2), m3 = centered(i3, g.n3);
+    const double* kq = kqcart + 3 * (row0 + blockIdx.y);
+    double g2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double c = kq[d] + g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
+        g2 += c * c;
+    }
+    table[(long long)blockIdx.y * N + e] = static_cast<T>(g2 < 1e-8 ? 0.0 : invN / g2);
+}
+
+// One workgroup transforms PL planes of one pair (PlaneGroup): thread t owns
+// line t % NP of plane t / NP.  Column load (coalesced) -> FFT_y -> LDS
+// transpose -> FFT_x -> x v(G)/N -> inverse FFT_x -> LDS transpose -> inverse
+// FFT_y -> column store.
+template <typename T, int NP>
+__global__ __launch_bounds__((PlaneGroup<T, NP>::threads)) void k_hf2x_B(cpx<T>* __restrict__ buf,
+                                                                       const PairEntry* __restrict__ pairs,
+                                                                       const T* __restrict__ table, int kq0, int nz)
+{
+    constexpr int PL = PlaneGroup<T, NP>::planes;
+    __shared__ PlaneTile<T, NP> tiles[PL];
+    const int p = threadIdx.x / NP, t = threadIdx.x % NP;
+    const int z = blockIdx.x * PL + p;
+    const bool active = p < PL && z < nz;
+    const size_t ng = (size_t)NP * NP * nz;
+    cpx<T>* plane = buf + (size_t)blockIdx.y * ng + (size_t)(active ? z : 0) * (NP * NP);
+    PlaneTile<T, NP>& tile = tiles[active ? p : 0];
     cpx<T> v[NP];
-    if (t < NP) {
+    if (active) {
         load_pencil<T, NP>(v, plane + t, NP);
         fft<T, NP, -1>(v);
         tile_put_col<T, NP>(tile, v, t);
     }
     __syncthreads();
-    if (t < NP) {
+    if (active) {
         tile_get_row<T, NP>(tile, v, t);
         fft<T, NP, -1>(v);
-        // frequency (x, y=t, z): Cartesian k-q+G, Miller wrapped (plane_wave_basis.cpp:205-216);
-        // v = 1/|k-q+G|^2, 0 below 1e-8 (hf2x.cpp:91-104), times the 1/N of FFT_forward.
-        // Evaluated in the working precision: the row offset is formed in FP64 and
-        // rounded once, so the FP32 kernel carries one rounding of |k-q+G|^2.
-        const int kq = pairs[blockIdx.y].kq;
-        const int my = centered(t, NP), mz = centered(z, nz);
-        const T bx = static_cast<T>(kqcart[3 * kq + 0] + g.b[3] * my + g.b[6] * mz);
-        const T by = static_cast<T>(kqcart[3 * kq + 1] + g.b[4] * my + g.b[7] * mz);
-        const T bz = static_cast<T>(kqcart[3 * kq + 2] + g.b[5] * my + g.b[8] * mz);
-        const T b0 = static_cast<T>(g.b[0]), b1 = static_cast<T>(g.b[1]), b2 = static_cast<T>(g.b[2]);
-        const T sN = static_cast<T>(invN);
+        // frequency (x, y=t, z)
+        const T* w = table + (size_t)(pairs[blockIdx.y].kq - kq0) * ng + (size_t)z * NP * NP + (size_t)t * NP;
         static_for<0, NP>([&](auto xc) {
             constexpr int x = decltype(xc)::value;
-            constexpr T mx = static_cast<T>(x <= NP / 2 ? x : x - NP);
-            const T gx = bx + b0 * mx, gy = by + b1 * mx, gz = bz + b2 * mx;
-            const T g2 = gx * gx + gy * gy + gz * gz;
-            const T w = g2 < T(1e-8) ? T(0) : sN / g2;
-            v[x] = scale(v[x], w);
+            v[x] = scale(v[x], w[x]);
         });
         fft<T, NP, 1>(v);
         tile_put_row<T, NP>(tile, v, t);
     }
     __syncthreads();
-    if (t < NP) {
+    if (active) {
         tile_get_col<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
         store_pencil<T, NP>(v, plane + t, NP);
@@ -164,27 +179,44 @@ void launch_hf2x_A(int prec, const Geom& g, const void* app, const void* occ, co
 }
 
 template <typename T>
-static void hf2x_B_impl(const Geom& g, void* buf, const PairEntry* pairs, int npairs, const double* kqcart,
+static void hf2x_B_impl(const Geom& g, void* buf, const PairEntry* pairs, int npairs, const void* table, int kq0,
                         hipStream_t s)
 {
     if (g.n1 != g.n2) throw Error(TCX_ERR_UNSUPPORTED, "FFT grid must have n1 == n2");
-    const double invN = 1.0 / ((double)g.n1 * g.n2 * g.n3);
-#define TCX_B(NP)                                                                                       \
-    hipLaunchKernelGGL((k_hf2x_B<T, NP>), dim3(g.n3, npairs), dim3(kPlaneBlock), 0, s, (cpx<T>*)buf,    \
-                       pairs, kqcart, g.n3, g, invN);
+#define TCX_B(NP)                                                                                         \
+    {                                                                                                     \
+        using PG = PlaneGroup<T, NP>;                                                                     \
+        hipLaunchKernelGGL((k_hf2x_B<T, NP>), dim3((g.n3 + PG::planes - 1) / PG::planes, npairs),         \
+                           dim3(PG::threads), 0, s, (cpx<T>*)buf, pairs, (const T*)table, kq0, g.n3);     \
+    }
     TCX_SIZE_SWITCH(g.n1, TCX_B)
 #undef TCX_B
     TCX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_hf2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const double* kqcart,
-                   hipStream_t s)
+void launch_coulomb_table(int prec, void* table, const double* kqcart, int row0, int nrows, const Geom& g,
+                          hipStream_t s)
+{
+    if (nrows == 0) return;
+    const long long N = (long long)g.n1 * g.n2 * g.n3;
+    const dim3 grid((unsigned)((N + 255) / 256), nrows);
+    if (prec == TCX_PREC_FP64)
+        hipLaunchKernelGGL(k_coulomb_table<double>, grid, dim3(256), 0, s, (double*)table, kqcart, row0, g, N,
+                           1.0 / (double)N);
+    else
+        hipLaunchKernelGGL(k_coulomb_table<float>, grid, dim3(256), 0, s, (float*)table, kqcart, row0, g, N,
+                           1.0 / (double)N);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_hf2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const void* table,
+                   int kq0, hipStream_t s)
 {
     if (npairs == 0) return;
     if (prec == TCX_PREC_FP64)
-        hf2x_B_impl<double>(g, buf, pairs, npairs, kqcart, s);
+        hf2x_B_impl<double>(g, buf, pairs, npairs, table, kq0, s);
     else
-        hf2x_B_impl<float>(g, buf, pairs, npairs, kqcart, s);
+        hf2x_B_impl<float>(g, buf, pairs, npairs, table, kq0, s);
 }
 
 template <typename T>

@@ -475,14 +475,18 @@ void Context::hf2x(const int* nb_, const double* phi, double* out, int prec, uns
         d_W.ensure((size_t)P.n_app * N * 16);
         launch_zero(d_W.p, (size_t)P.n_app * N * 16, stream);
         d_pairbuf.ensure((size_t)std::max(1, P.max_launch_pairs) * N * es);
+        int max_rows = 1;
+        for (const Launch& L : P.launches) max_rows = std::max(max_rows, L.nkq);
+        d_ktab.ensure((size_t)max_rows * N * (es / 2));  // one real per k-q row and grid point
         for (const Launch& L : P.launches) {
             const PairEntry* pairs = (const PairEntry*)P.d_pairs.p + L.pair0;
             const TargetEntry* targets = (const TargetEntry*)P.d_targets.p + L.target0;
+            launch_coulomb_table(prec, d_ktab.p, (const double*)P.d_kq.p, L.kq0, L.nkq, geo, stream);
             timed(0, L.npairs, [&] {
                 launch_hf2x_A(prec, geo, d_appR.p, d_occR[prec].p, pairs, L.npairs, d_pairbuf.p, stream);
             });
             timed(1, L.npairs, [&] {
-                launch_hf2x_B(prec, geo, d_pairbuf.p, pairs, L.npairs, (const double*)P.d_kq.p, stream);
+                launch_hf2x_B(prec, geo, d_pairbuf.p, pairs, L.npairs, d_ktab.p, L.kq0, stream);
             });
             timed(2, L.npairs, [&] {
                 launch_hf2x_C(prec, geo, d_pairbuf.p, d_occR[prec].p, pairs, targets, L.ntargets,

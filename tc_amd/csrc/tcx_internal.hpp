@@ -95,8 +95,10 @@ void launch_fft3d(int prec, int sign, void* grids, int ngrids, const Geom& g, do
 // hf2x pair pipeline
 void launch_hf2x_A(int prec, const Geom& g, const void* phiR_app, const void* phiR_occ, const PairEntry* pairs,
                    int npairs, void* buf, hipStream_t s);
-void launch_hf2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const double* kqcart,
-                   hipStream_t s);
+void launch_coulomb_table(int prec, void* table, const double* kqcart, int row0, int nrows, const Geom& g,
+                          hipStream_t s);
+void launch_hf2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const void* table,
+                   int kq0, hipStream_t s);
 void launch_hf2x_C(int prec, const Geom& g, const void* buf, const void* phiR_occ, const PairEntry* pairs,
                    const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s);
 
