@@ -757,3 +757,496 @@ void orc_tc3b1(const orc_problem *p, const int *nb, const double *phi_d, double 
     apply_local_potential(p, nb, phi_d, hphi_d, V, factor);          /* :104-134 */
     free(G); free(dnu); free(nG); free(V);
 }
+
+/* ------------------------------------------------------------------ */
+/* tc3a3b3b4b6, calc_hamiltonian_tc3a3b3b4b6.cpp:11-863                 */
+/* (kVaux branches :537-627, :772-858 vanish for SCF k on the mesh)     */
+/* ------------------------------------------------------------------ */
+
+static void dnu_grids(const orc_problem *p, const double *G, cplx *dnu) /* :128-150 */
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *dens = (const cplx *)p->density;
+    cplx *nG = malloc(sizeof(cplx) * N);
+    memset(dnu, 0, sizeof(cplx) * 3 * N);
+    for (int s3 = 0; s3 < 2; ++s3) {
+        memcpy(nG, dens + s3 * N, sizeof(cplx) * N);
+        orc_fft3d((double *)nG, p->n, -1);
+        for (long r = 0; r < N; ++r) {
+            double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
+            double u = orc_uk(p, g2, 0, s3);
+            for (int d = 0; d < 3; ++d) dnu[d * N + r] += u * G[3 * r + d] * nG[r];
+        }
+    }
+    for (int d = 0; d < 3; ++d) orc_fft3d((double *)(dnu + d * N), p->n, +1);
+    free(nG);
+}
+
+/* du[s][d](r) = (kq + G)_d * uk(kq + G, s, 0) for s = 0, 1 on the grid */
+static void du_grids(const orc_problem *p, const double *G, const double kq[3], int nspin2, int swap, double *du)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    for (long r = 0; r < N; ++r) {
+        double x[3], g2 = 0;
+        for (int d = 0; d < 3; ++d) {
+            x[d] = kq[d] + G[3 * r + d];
+            g2 += x[d] * x[d];
+        }
+        for (int s = 0; s < nspin2; ++s) {
+            double u = swap ? orc_uk(p, g2, 0, s) : orc_uk(p, g2, s, 0);
+            for (int d = 0; d < 3; ++d) du[(s * 3 + d) * N + r] = x[d] * u;
+        }
+    }
+}
+
+static void orbital_grid(const orc_problem *p, int ik, const cplx *c, cplx *grid, long N)
+{
+    scatter(p, ik, c, grid, N);
+    orc_fft3d((double *)grid, p->n, +1);
+}
+
+void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d, int nthreads)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *phi = (const cplx *)phi_d;
+    cplx *hphi = (cplx *)hphi_d;
+    const cplx *dens = (const cplx *)p->density;
+    const cplx *occ = (const cplx *)p->phi_occ;
+    const int bitc = p->method == 2;
+    const cplx *occl = (bitc && p->chi_occ) ? (const cplx *)p->chi_occ : occ;
+    const double NkV = p->num_kpoints * p->volume;
+    const double base = 1.0 / (NkV * NkV);                            /* :33-35 */
+    double fa3 = -base, fb3 = base, fb4 = -base, fb6 = base;          /* :37-47 */
+    if (p->nspin == 1) { fa3 /= 2.0; fb3 /= 2.0; fb4 /= 4.0; fb6 /= 2.0; }
+    const int need_div = p->div_corr && !p->jastrow_A_zero;          /* :99 */
+    const int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+    cplx *dnu = malloc(sizeof(cplx) * 3 * N);
+    dnu_grids(p, G, dnu);                                             /* [3a3,b3] :126-151 */
+
+    for (int ik = 0; ik < p->nk; ++ik) {
+        const long ao = app_coef_offset(p, nb, ik);
+        const int npwk = p->npw[ik];
+        const int *gi = p->gindex + pw_offset(p, ik);
+        double kc[3];
+        kcart(p, p->kfrac + 3 * ik, kc);
+        cplx *qsum = calloc((size_t)nb[ik] * 3 * N, sizeof(cplx));  /* qsum_temp_3a3b4 :162-172 */
+
+        /* phase 1: [3b3], [3a3,b4,b6] (:179-371) */
+#pragma omp parallel num_threads(nth)
+        {
+            double *du = malloc(sizeof(double) * 6 * N);
+            cplx *phij = malloc(sizeof(cplx) * N), *phiq = malloc(sizeof(cplx) * N), *chiq = malloc(sizeof(cplx) * N);
+            cplx *pp = malloc(sizeof(cplx) * N), *orb = malloc(sizeof(cplx) * N), *orb2 = malloc(sizeof(cplx) * N);
+            cplx *dnuphi = malloc(sizeof(cplx) * 3 * N), *hsub = malloc(sizeof(cplx) * N);
+#pragma omp for schedule(dynamic)
+            for (int iq = 0; iq < p->nk; ++iq) {
+                if (p->nocc[iq] == 0) continue;                        /* :201 */
+                double qc[3], kq[3];
+                kcart(p, p->kfrac + 3 * iq, qc);
+                for (int d = 0; d < 3; ++d) kq[d] = kc[d] - qc[d];
+                du_grids(p, G, kq, 2, 0, du);                          /* :207-215 */
+                const long co = occ_coef_offset(p, iq), fo = occ_offset(p, iq);
+                for (int jb = 0; jb < nb[ik]; ++jb) {
+                    orbital_grid(p, ik, phi + ao + (long)jb * npwk, phij, N);  /* :223-228 */
+                    for (int d = 0; d < 3; ++d)                                 /* :231-235 */
+                        for (long r = 0; r < N; ++r) dnuphi[d * N + r] = dnu[d * N + r] * phij[r];
+                    memset(hsub, 0, sizeof(cplx) * N);
+                    for (int ib = 0; ib < p->nocc[iq]; ++ib) {
+                        orbital_grid(p, iq, occ + co + (long)ib * p->npw[iq], phiq, N);
+                        const cplx *chi = phiq;
+                        if (bitc) {
+                            orbital_grid(p, iq, occl + co + (long)ib * p->npw[iq], chiq, N);
+                            chi = chiq;
+                        }
+                        for (long r = 0; r < N; ++r) pp[r] = conj(chi[r]) * phij[r];   /* :280-281 */
+                        orc_fft3d((double *)pp, p->n, -1);
+                        /* [3b3] :284-303 */
+                        memset(orb2, 0, sizeof(cplx) * N);
+                        for (int d = 0; d < 3; ++d) {
+                            for (long r = 0; r < N; ++r) orb[r] = dnuphi[d * N + r] * conj(chi[r]);
+                            orc_fft3d((double *)orb, p->n, -1);
+                            for (long r = 0; r < N; ++r) orb2[r] += du[(0 * 3 + d) * N + r] * orb[r];
+                        }
+                        orc_fft3d((double *)orb2, p->n, +1);
+                        const double f = p->filling[fo + ib];
+                        for (long r = 0; r < N; ++r) hsub[r] += (f * fb3) * orb2[r] * phiq[r];
+                        /* [3a3,b4,b6] :305-349 */
+                        for (long r = 0; r < N; ++r) pp[r] *= f;
+                        memset(orb2, 0, sizeof(cplx) * N);
+                        for (int s2 = 0; s2 < 2; ++s2)
+                            for (int d = 0; d < 3; ++d) {
+                                const double *dus = du + (s2 * 3 + d) * N;
+                                for (long r = 0; r < N; ++r) orb[r] = pp[r] * dus[r];
+                                orc_fft3d((double *)orb, p->n, +1);
+                                if (s2 == 0) {
+#pragma omp critical
+                                    {
+                                        cplx *qs = qsum + ((size_t)jb * 3 + d) * N;
+                                        for (long r = 0; r < N; ++r) qs[r] += orb[r] * phiq[r];
+                                    }
+                                }
+                                for (long r = 0; r < N; ++r) orb[r] *= dens[s2 * N + r];
+                                orc_fft3d((double *)orb, p->n, -1);
+                                for (long r = 0; r < N; ++r) orb2[r] += orb[r] * dus[r];
+                            }
+                        orc_fft3d((double *)orb2, p->n, +1);                  /* :352-358 */
+                        for (long r = 0; r < N; ++r) hsub[r] += fb6 * orb2[r] * phiq[r];
+                    }
+                    orc_fft3d((double *)hsub, p->n, -1);                      /* :363-372 */
+#pragma omp critical
+                    {
+                        cplx *out = hphi + ao + (long)jb * npwk;
+                        for (int l = 0; l < npwk; ++l) out[l] += hsub[gi[l]];
+                    }
+                }
+            }
+            free(du); free(phij); free(phiq); free(chiq); free(pp); free(orb); free(orb2);
+            free(dnuphi); free(hsub);
+        }
+
+        /* per band: [3b6] divergence correction (:393-535), [3a3] (:629-642) */
+        {
+            cplx *phij = malloc(sizeof(cplx) * N), *phib = malloc(sizeof(cplx) * N), *chib = malloc(sizeof(cplx) * N);
+            cplx *pp = malloc(sizeof(cplx) * N), *hsub = malloc(sizeof(cplx) * N);
+            cplx *tG = malloc(sizeof(cplx) * N), *tR = malloc(sizeof(cplx) * N), *nG = malloc(sizeof(cplx) * N);
+            for (int jb = 0; jb < nb[ik]; ++jb) {
+                cplx *out = hphi + ao + (long)jb * npwk;
+                orbital_grid(p, ik, phi + ao + (long)jb * npwk, phij, N);
+                if (need_div) {
+                    /* k is on the SCF mesh, so if_match holds (:396-410) */
+                    const double dcp = 1.0 / sqrt(PI * p->alpha) * NkV - p->sum_vaux[ik]; /* :415-419 */
+                    cplx t1 = 0.0;
+                    memset(tG, 0, sizeof(cplx) * N);
+                    for (int js = 0; js < 2; ++js) {                              /* :427-450 */
+                        memcpy(nG, dens + js * N, sizeof(cplx) * N);
+                        orc_fft3d((double *)nG, p->n, -1);
+                        const double Aj = p->A[0 * 2 + js];
+                        const double Adiv = dcp * Aj + orc_uk(p, 0.0, 0, js);
+                        t1 += FOURPI * Adiv * Aj * nG[0];
+                        for (long r = 1; r < N; ++r) {                            /* AuG :99-112, G=0 excluded */
+                            double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
+                            tG[r] += FOURPI * nG[r] * (Aj * orc_uk(p, g2, 0, js));
+                        }
+                    }
+                    memcpy(tR, tG, sizeof(cplx) * N);
+                    orc_fft3d((double *)tR, p->n, +1);                             /* :451 */
+                    memset(hsub, 0, sizeof(cplx) * N);
+                    const long co = occ_coef_offset(p, ik), fo = occ_offset(p, ik);
+                    for (int b = 0; b < p->nocc[ik]; ++b) {                         /* :456-532 */
+                        const double f = p->filling[fo + b];
+                        if (f < 1e-8) continue;
+                        const cplx *pk = occ + co + (long)b * npwk;
+                        orbital_grid(p, ik, pk, phib, N);
+                        const cplx *chi = phib;
+                        if (bitc) {
+                            orbital_grid(p, ik, occl + co + (long)b * npwk, chib, N);
+                            chi = chib;
+                        }
+                        for (long r = 0; r < N; ++r) pp[r] = chi[r] * conj(phij[r]);   /* :511-512 */
+                        orc_fft3d((double *)pp, p->n, -1);
+                        const cplx Ib = conj(pp[0]);
+                        cplx dot = 0.0;                                                /* phiphi.dot(temp_G) */
+                        for (long r = 0; r < N; ++r) dot += conj(pp[r]) * tG[r];
+                        const cplx c = f * fb6 * t1 * Ib + f * fb6 * dot;                /* contributions 1, 3 */
+                        for (int l = 0; l < npwk; ++l) out[l] += c * pk[l];
+                        for (long r = 0; r < N; ++r) hsub[r] += (f * Ib * fb6) * tR[r] * phib[r]; /* 2 */
+                    }
+                    orc_fft3d((double *)hsub, p->n, -1);
+                    for (int l = 0; l < npwk; ++l) out[l] += hsub[gi[l]];
+                }
+                /* [3a3] */
+                memset(hsub, 0, sizeof(cplx) * N);
+                for (int d = 0; d < 3; ++d) {
+                    const cplx *qs = qsum + ((size_t)jb * 3 + d) * N;
+                    for (long r = 0; r < N; ++r) hsub[r] += qs[r] * dnu[d * N + r];
+                }
+                orc_fft3d((double *)hsub, p->n, -1);
+                for (int l = 0; l < npwk; ++l) out[l] += fa3 * hsub[gi[l]];
+            }
+            free(phij); free(phib); free(chib); free(pp); free(hsub); free(tG); free(tR); free(nG);
+        }
+
+        /* phase 2: [3b4] (:646-770) */
+#pragma omp parallel num_threads(nth)
+        {
+            double *du = malloc(sizeof(double) * 3 * N);
+            cplx *phiq = malloc(sizeof(cplx) * N), *chiq = malloc(sizeof(cplx) * N);
+            cplx *pp = malloc(sizeof(cplx) * N), *orb = malloc(sizeof(cplx) * N), *hsub = malloc(sizeof(cplx) * N);
+#pragma omp for schedule(dynamic)
+            for (int iq = 0; iq < p->nk; ++iq) {
+                if (p->nocc[iq] == 0) continue;
+                double qc[3], kq[3];
+                kcart(p, p->kfrac + 3 * iq, qc);
+                for (int d = 0; d < 3; ++d) kq[d] = kc[d] - qc[d];
+                du_grids(p, G, kq, 1, 0, du);                                  /* :680-687 */
+                const long co = occ_coef_offset(p, iq), fo = occ_offset(p, iq);
+                for (int jb = 0; jb < nb[ik]; ++jb) {
+                    memset(hsub, 0, sizeof(cplx) * N);
+                    for (int ib = 0; ib < p->nocc[iq]; ++ib) {
+                        orbital_grid(p, iq, occ + co + (long)ib * p->npw[iq], phiq, N);
+                        const cplx *chi = phiq;
+                        if (bitc) {
+                            orbital_grid(p, iq, occl + co + (long)ib * p->npw[iq], chiq, N);
+                            chi = chiq;
+                        }
+                        memset(pp, 0, sizeof(cplx) * N);
+                        for (int d = 0; d < 3; ++d) {                                   /* :740-748 */
+                            const cplx *qs = qsum + ((size_t)jb * 3 + d) * N;
+                            for (long r = 0; r < N; ++r) orb[r] = conj(chi[r]) * qs[r];
+                            orc_fft3d((double *)orb, p->n, -1);
+                            for (long r = 0; r < N; ++r) pp[r] += orb[r] * du[d * N + r];
+                        }
+                        orc_fft3d((double *)pp, p->n, +1);
+                        const double f = p->filling[fo + ib];
+                        for (long r = 0; r < N; ++r) hsub[r] += f * pp[r] * phiq[r];    /* :754 */
+                    }
+                    orc_fft3d((double *)hsub, p->n, -1);                                /* :757-766 */
+#pragma omp critical
+                    {
+                        cplx *out = hphi + ao + (long)jb * npwk;
+                        for (int l = 0; l < npwk; ++l) out[l] += fb4 * hsub[gi[l]];
+                    }
+                }
+            }
+            free(du); free(phiq); free(chiq); free(pp); free(orb); free(hsub);
+        }
+        free(qsum);
+    }
+    free(G); free(dnu);
+}
+
+/* ------------------------------------------------------------------ */
+/* tc3a2a4b2b5, calc_hamiltonian_tc3a2a4b2b5.cpp:11-1300                */
+/* Identity symmetry: the U^+ transforms of :576-643 and :682-765 are   */
+/* the identity (phase 1, Udag_cart = 1), so qsum_temp_3a4_trans is     */
+/* FFT_backward(FFT_forward(qsum_temp_3a4)).  The kVaux branches         */
+/* :1036-1299 vanish for SCF k on the mesh.                              */
+/* ------------------------------------------------------------------ */
+
+void orc_tc3a2a4b2b5(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d, int nthreads)
+{
+    const long N = (long)p->n[0] * p->n[1] * p->n[2];
+    const cplx *phi = (const cplx *)phi_d;
+    cplx *hphi = (cplx *)hphi_d;
+    const cplx *occ = (const cplx *)p->phi_occ;
+    const int bitc = p->method == 2;
+    const cplx *occl = (bitc && p->chi_occ) ? (const cplx *)p->chi_occ : occ;
+    const double NkV = p->num_kpoints * p->volume;
+    const double base = 1.0 / (NkV * NkV);                            /* :34-36 */
+    double fa2 = -0.5 * base, fa4 = base, fb2 = base, fb5 = -base;    /* :38-48 */
+    if (p->nspin == 1) { fa2 /= 4.0; fa4 /= 4.0; fb2 /= 4.0; fb5 /= 4.0; }
+    const int need_div = p->div_corr && !p->jastrow_A_zero;
+    const int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+    double *G = malloc(sizeof(double) * 3 * N);
+    grid_gcart(p, G);
+    double *uG = malloc(sizeof(double) * 2 * N);                       /* uG[0][s] :120-131 */
+    for (long r = 0; r < N; ++r) {
+        double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
+        for (int s = 0; s < 2; ++s) uG[s * N + r] = orc_uk(p, g2, 0, s);
+    }
+    cplx *V3a2 = calloc(N, sizeof(cplx));                              /* V_3a2 :113 */
+    cplx *B2 = calloc(3 * N, sizeof(cplx));                            /* qsum_temp_3b2 :160-163 */
+    /* flat (k, j) list for the parallel apply loops */
+    int napp = 0;
+    for (int ik = 0; ik < p->nk; ++ik) napp += nb[ik];
+    int *app_k = malloc(sizeof(int) * (napp + 1)), *app_j = malloc(sizeof(int) * (napp + 1));
+    for (int ik = 0, a = 0; ik < p->nk; ++ik)
+        for (int jb = 0; jb < nb[ik]; ++jb, ++a) { app_k[a] = ik; app_j[a] = jb; }
+
+    for (int iq2 = 0; iq2 < p->nk; ++iq2) {                            /* :165 */
+        const int n2 = p->nocc[iq2];
+        if (n2 == 0) continue;
+        double q2c[3];
+        kcart(p, p->kfrac + 3 * iq2, q2c);
+        const long co2 = occ_coef_offset(p, iq2), fo2 = occ_offset(p, iq2);
+        cplx *Q2 = calloc((size_t)n2 * N, sizeof(cplx));                /* qsum_temp_3a2 */
+        cplx *Q4 = calloc((size_t)n2 * 3 * N, sizeof(cplx));            /* qsum_temp_3a4 */
+        cplx *Q5 = bitc ? calloc((size_t)n2 * 3 * N, sizeof(cplx)) : NULL; /* qsum_temp_3b5 */
+
+        /* (i) q1-q2 phase (:210-443) */
+        double *du = malloc(sizeof(double) * 6 * N);
+        for (int iq1 = 0; iq1 < p->nk; ++iq1) {
+            if (p->nocc[iq1] == 0) continue;
+            double q1c[3], v[3];
+            kcart(p, p->kfrac + 3 * iq1, q1c);
+            for (int d = 0; d < 3; ++d) v[d] = q2c[d] - q1c[d];
+            du_grids(p, G, v, 2, 1, du);                                 /* :239-251 */
+            const long co1 = occ_coef_offset(p, iq1), fo1 = occ_offset(p, iq1);
+#pragma omp parallel for num_threads(nth) schedule(dynamic)
+            for (int ib2 = 0; ib2 < n2; ++ib2) {
+                cplx *phiq2 = malloc(sizeof(cplx) * N), *chiq2 = malloc(sizeof(cplx) * N);
+                cplx *phiq1 = malloc(sizeof(cplx) * N), *chiq1 = malloc(sizeof(cplx) * N);
+                cplx *pp = malloc(sizeof(cplx) * N), *pj = malloc(sizeof(cplx) * N);
+                cplx *orb = malloc(sizeof(cplx) * N), *orb2 = malloc(sizeof(cplx) * N);
+                orbital_grid(p, iq2, occ + co2 + (long)ib2 * p->npw[iq2], phiq2, N);
+                const cplx *c2 = phiq2;
+                if (bitc) {
+                    orbital_grid(p, iq2, occl + co2 + (long)ib2 * p->npw[iq2], chiq2, N);
+                    c2 = chiq2;
+                }
+                cplx *q2s = Q2 + (size_t)ib2 * N;
+                for (int ib1 = 0; ib1 < p->nocc[iq1]; ++ib1) {
+                    orbital_grid(p, iq1, occ + co1 + (long)ib1 * p->npw[iq1], phiq1, N);
+                    const cplx *c1 = phiq1;
+                    if (bitc) {
+                        orbital_grid(p, iq1, occl + co1 + (long)ib1 * p->npw[iq1], chiq1, N);
+                        c1 = chiq1;
+                    }
+                    for (long r = 0; r < N; ++r) pp[r] = conj(c1[r]) * phiq2[r];       /* :356-357 */
+                    orc_fft3d((double *)pp, p->n, -1);
+                    if (bitc) {                                                        /* :358-363 */
+                        for (long r = 0; r < N; ++r) pj[r] = conj(phiq1[r]) * c2[r];
+                        orc_fft3d((double *)pj, p->n, -1);
+                    }
+                    const double f1 = p->filling[fo1 + ib1];
+                    for (int s = 0; s < 2; ++s)
+                        for (int d = 0; d < 3; ++d) {
+                            const double *dus = du + (s * 3 + d) * N;
+                            for (long r = 0; r < N; ++r) orb[r] = dus[r] * pp[r];          /* :373-374 */
+                            orc_fft3d((double *)orb, p->n, +1);
+                            if (bitc) {                                                    /* :376-389 */
+                                for (long r = 0; r < N; ++r) orb2[r] = dus[r] * pj[r];
+                                orc_fft3d((double *)orb2, p->n, +1);
+                                for (long r = 0; r < N; ++r) q2s[r] -= f1 * orb[r] * conj(orb2[r]);
+                            } else {                                                       /* :390-401 */
+                                for (long r = 0; r < N; ++r) {
+                                    const double a = creal(orb[r]), b = cimag(orb[r]);
+                                    q2s[r] -= f1 * (a * a + b * b);
+                                }
+                            }
+                            if (s == 0) {                                                  /* :406-427 */
+                                cplx *q4 = Q4 + ((size_t)ib2 * 3 + d) * N;
+                                for (long r = 0; r < N; ++r) q4[r] += f1 * orb[r] * phiq1[r];
+                                if (bitc) {
+                                    cplx *q5 = Q5 + ((size_t)ib2 * 3 + d) * N;
+                                    for (long r = 0; r < N; ++r) q5[r] -= f1 * conj(orb2[r]) * conj(c1[r]);
+                                }
+                            }
+                        }
+                }
+                free(phiq2); free(chiq2); free(phiq1); free(chiq1); free(pp); free(pj); free(orb); free(orb2);
+            }
+        }
+        free(du);
+
+        /* (ii) per (q2, i2) (:445-915) */
+        for (int ib2 = 0; ib2 < n2; ++ib2) {
+            const double f2 = p->filling[fo2 + ib2];
+            cplx *phiq2 = malloc(sizeof(cplx) * N), *chiq2 = malloc(sizeof(cplx) * N);
+            orbital_grid(p, iq2, occ + co2 + (long)ib2 * p->npw[iq2], phiq2, N);
+            const cplx *c2 = phiq2;
+            if (bitc) {
+                orbital_grid(p, iq2, occl + co2 + (long)ib2 * p->npw[iq2], chiq2, N);
+                c2 = chiq2;
+            }
+            /* [3a2] (:486-573) */
+            cplx *T = malloc(sizeof(cplx) * N);
+            memcpy(T, Q2 + (size_t)ib2 * N, sizeof(cplx) * N);
+            orc_fft3d((double *)T, p->n, -1);
+            if (need_div) {
+                const double dc2 = 1.0 / sqrt(PI * p->alpha) * NkV - p->sum_vaux[iq2];    /* :501-503 */
+                cplx *phiq1 = malloc(sizeof(cplx) * N), *chiq1 = malloc(sizeof(cplx) * N);
+                cplx *pa = malloc(sizeof(cplx) * N), *pb = malloc(sizeof(cplx) * N);
+                for (int ib1 = 0; ib1 < n2; ++ib1) {                                     /* q1 = q2 */
+                    orbital_grid(p, iq2, occ + co2 + (long)ib1 * p->npw[iq2], phiq1, N);
+                    const cplx *c1 = phiq1;
+                    if (bitc) {
+                        orbital_grid(p, iq2, occl + co2 + (long)ib1 * p->npw[iq2], chiq1, N);
+                        c1 = chiq1;
+                    }
+                    for (long r = 0; r < N; ++r) pa[r] = conj(c1[r]) * phiq2[r];          /* :545-547 */
+                    orc_fft3d((double *)pa, p->n, -1);
+                    for (long r = 0; r < N; ++r) pb[r] = conj(c2[r]) * phiq1[r];          /* :548-550 */
+                    orc_fft3d((double *)pb, p->n, -1);
+                    const double f1 = p->filling[fo2 + ib1];
+                    for (int s = 0; s < 2; ++s) {                                         /* :553-569 */
+                        const double As = p->A[0 * 2 + s];
+                        T[0] -= FOURPI * As * (As * dc2 + orc_uk(p, 0.0, 0, s)) * f1 * pa[0] * pb[0];
+                        const double ct = FOURPI * As * f1;
+                        for (long r = 1; r < N; ++r)
+                            T[r] -= ct * uG[s * N + r] * (pa[r] * pb[0] + pa[0] * pb[r]);
+                    }
+                }
+                free(phiq1); free(chiq1); free(pa); free(pb);
+            }
+            for (long r = 0; r < N; ++r) V3a2[r] += f2 * T[r];                             /* :637-641 */
+            free(T);
+
+            /* [3a4,b2,b5] (:644-800) */
+            cplx *Q4t = malloc(sizeof(cplx) * 3 * N), *Q5t = malloc(sizeof(cplx) * 3 * N);
+            for (int d = 0; d < 3; ++d) {
+                cplx *a = Q4t + d * N, *b = Q5t + d * N;
+                const cplx *q4 = Q4 + ((size_t)ib2 * 3 + d) * N;
+                memcpy(a, q4, sizeof(cplx) * N);
+                if (bitc) memcpy(b, Q5 + ((size_t)ib2 * 3 + d) * N, sizeof(cplx) * N);
+                else for (long r = 0; r < N; ++r) b[r] = -conj(q4[r]);                 /* :662-665 */
+                orc_fft3d((double *)a, p->n, -1);
+                orc_fft3d((double *)b, p->n, -1);
+                orc_fft3d((double *)a, p->n, +1);                                          /* :755-759 */
+                orc_fft3d((double *)b, p->n, +1);
+            }
+            for (int d = 0; d < 3; ++d)                                                   /* [3b2] :790-800 */
+                for (long r = 0; r < N; ++r) B2[d * N + r] += f2 * Q5t[d * N + r] * phiq2[r];
+
+#pragma omp parallel num_threads(nth)
+            {
+                double *du0 = malloc(sizeof(double) * 3 * N);
+                cplx *phij = malloc(sizeof(cplx) * N), *pp = malloc(sizeof(cplx) * N);
+                cplx *orb = malloc(sizeof(cplx) * N), *orb2 = malloc(sizeof(cplx) * N);
+                int last_k = -1;
+#pragma omp for schedule(static)
+                for (int a = 0; a < napp; ++a) {                                          /* :801-912 */
+                    const int ik = app_k[a], jb = app_j[a];
+                    if (ik != last_k) {
+                        double kc[3], kq[3];
+                        kcart(p, p->kfrac + 3 * ik, kc);
+                        for (int d = 0; d < 3; ++d) kq[d] = kc[d] - q2c[d];
+                        du_grids(p, G, kq, 1, 0, du0);                                     /* uk_save :194-207, :826-832 */
+                        last_k = ik;
+                    }
+                    const long ao = app_coef_offset(p, nb, ik);
+                    const int npwk = p->npw[ik];
+                    const int *gi = p->gindex + pw_offset(p, ik);
+                    orbital_grid(p, ik, phi + ao + (long)jb * npwk, phij, N);
+                    memset(orb2, 0, sizeof(cplx) * N);
+                    for (long r = 0; r < N; ++r) pp[r] = conj(c2[r]) * phij[r];           /* [3a4] :849-861 */
+                    orc_fft3d((double *)pp, p->n, -1);
+                    for (long r = 0; r < N; ++r) pp[r] *= f2 * fa4;
+                    for (int d = 0; d < 3; ++d) {
+                        for (long r = 0; r < N; ++r) orb[r] = pp[r] * du0[d * N + r];
+                        orc_fft3d((double *)orb, p->n, +1);
+                        for (long r = 0; r < N; ++r) orb2[r] += Q4t[d * N + r] * orb[r];
+                    }
+                    memset(pp, 0, sizeof(cplx) * N);                                       /* [3b5] :863-875 */
+                    for (int d = 0; d < 3; ++d) {
+                        for (long r = 0; r < N; ++r) orb[r] = Q5t[d * N + r] * phij[r];
+                        orc_fft3d((double *)orb, p->n, -1);
+                        for (long r = 0; r < N; ++r) pp[r] += orb[r] * du0[d * N + r];
+                    }
+                    orc_fft3d((double *)pp, p->n, +1);
+                    for (long r = 0; r < N; ++r) orb2[r] += fb5 * f2 * pp[r] * phiq2[r];
+                    orc_fft3d((double *)orb2, p->n, -1);                                   /* :877-889 */
+                    cplx *out = hphi + ao + (long)jb * npwk;
+                    for (int l = 0; l < npwk; ++l) out[l] += orb2[gi[l]];
+                }
+                free(du0); free(phij); free(pp); free(orb); free(orb2);
+            }
+            free(Q4t); free(Q5t); free(phiq2); free(chiq2);
+        }
+        free(Q2); free(Q4); free(Q5);
+    }
+
+    /* [3b2] (:922-992) and [3a2] (:995-1034): two local potentials */
+    cplx *X = calloc(N, sizeof(cplx));
+    for (int d = 0; d < 3; ++d) {
+        orc_fft3d((double *)(B2 + d * N), p->n, -1);
+        for (long r = 0; r < N; ++r) X[r] += G[3 * r + d] * (uG[r] + uG[N + r]) * B2[d * N + r];
+    }
+    orc_fft3d((double *)X, p->n, +1);
+    orc_fft3d((double *)V3a2, p->n, +1);
+    apply_local_potential(p, nb, phi_d, hphi_d, X, fb2);
+    apply_local_potential(p, nb, phi_d, hphi_d, V3a2, fa2);
+    free(G); free(uG); free(V3a2); free(B2); free(X); free(app_k); free(app_j);
+}

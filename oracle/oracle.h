@@ -92,6 +92,10 @@ void orc_tc3b1(const orc_problem *p, const int *nb, const double *phi, double *h
 void orc_kinetic(const orc_problem *p, const int *nb, const double *phi, double *hphi);
 void orc_hf2h(const orc_problem *p, const int *nb, const double *phi, double *hphi);
 void orc_tc2h(const orc_problem *p, const int *nb, const double *phi, double *hphi);
+/* Three-body exchange terms, src/calc_hamiltonian_tc3a3b3b4b6.cpp:11-863 and
+ * src/calc_hamiltonian_tc3a2a4b2b5.cpp:11-1300 (identity symmetry, k on the mesh). */
+void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi, double *hphi, int nthreads);
+void orc_tc3a2a4b2b5(const orc_problem *p, const int *nb, const double *phi, double *hphi, int nthreads);
 
 #ifdef __cplusplus
 }

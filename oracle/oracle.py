@@ -68,7 +68,7 @@ def lib():
         L.orc_tc_2body.restype = C.c_double
         L.orc_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
         L.orc_density.argtypes = [P, _dp]
-        for name in ("orc_hf2x", "orc_tc2x"):
+        for name in ("orc_hf2x", "orc_tc2x", "orc_tc3a3b3b4b6", "orc_tc3a2a4b2b5"):
             getattr(L, name).argtypes = [P, _ip, _dp, _dp, C.c_int]
         for name in ("orc_tc3a1", "orc_tc3b1", "orc_kinetic", "orc_hf2h", "orc_tc2h"):
             getattr(L, name).argtypes = [P, _ip, _dp, _dp]
@@ -145,7 +145,7 @@ class Oracle:
         self.svaux = self._hold(svaux)
         P.sum_vaux = _p(self.svaux)
         self.P = P
-        if density is None and sys.method in ("TC", "BITC"):
+        if density is None:
             density = self.density()
         if density is not None:
             self.dens = self._hold(np.ascontiguousarray(density, dtype=np.complex128))
@@ -167,7 +167,7 @@ class Oracle:
         f = getattr(lib(), fname)
         args = [C.byref(self.P), _p(nb, _ip), phi.view(np.float64).ctypes.data_as(_dp),
                 out.view(np.float64).ctypes.data_as(_dp)]
-        if fname in ("orc_hf2x", "orc_tc2x"):
+        if fname in ("orc_hf2x", "orc_tc2x", "orc_tc3a3b3b4b6", "orc_tc3a2a4b2b5"):
             args.append(int(nthreads))
         f(*args)
         return out
@@ -183,6 +183,12 @@ class Oracle:
 
     def tc3b1(self, nb, phi):
         return self._apply("orc_tc3b1", nb, phi)
+
+    def tc3a3b3b4b6(self, nb, phi, nthreads=0):
+        return self._apply("orc_tc3a3b3b4b6", nb, phi, nthreads=nthreads)
+
+    def tc3a2a4b2b5(self, nb, phi, nthreads=0):
+        return self._apply("orc_tc3a2a4b2b5", nb, phi, nthreads=nthreads)
 
     def kinetic(self, nb, phi):
         return self._apply("orc_kinetic", nb, phi)

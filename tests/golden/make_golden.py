@@ -79,14 +79,19 @@ def make_case(name):
         chi_occ=sys_.chi_occ if sys_.chi_occ is not None else np.zeros(0, np.complex128),
         A=sys_.A, F=sys_.F, method=np.array(method), nb=nb, phi=phi, alpha=alpha, sum_vaux=sv,
     )
+    out["kinetic"] = BF.kinetic(sys_, nb, phi)
     if method == "HF":
         out["hf2x"] = BF.hf2x(sys_, nb, phi, alpha, sv)
+        out["hf2h"] = BF.hf2h(sys_, nb, phi)
         sys_.div_corr = False
         out["hf2x_nodiv"] = BF.hf2x(sys_, nb, phi, alpha, sv)
     else:
         out["tc2x"] = BF.tc2x(sys_, nb, phi, alpha, sv)
+        out["tc2h"] = BF.tc2h(sys_, nb, phi)
         out["tc3a1"] = BF.tc3a1(sys_, nb, phi)
         out["tc3b1"] = BF.tc3b1(sys_, nb, phi)
+        out["tc3a3b3b4b6"] = BF.tc3a3b3b4b6(sys_, nb, phi, alpha, sv)
+        out["tc3a2a4b2b5"] = BF.tc3a2a4b2b5(sys_, nb, phi, alpha, sv)
     np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
     return out
 

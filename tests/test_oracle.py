@@ -48,7 +48,8 @@ def test_oracle_matches_golden(golden, name):
     assert abs(o.P.alpha - float(z["alpha"])) < 1e-14
     assert np.abs(o.svaux - z["sum_vaux"]).max() < 1e-10 * np.abs(z["sum_vaux"]).max()
     nb, phi = z["nb"], z["phi"]
-    terms = ["hf2x"] if str(z["method"]) == "HF" else ["tc2x", "tc3a1", "tc3b1"]
+    terms = (["hf2x", "hf2h", "kinetic"] if str(z["method"]) == "HF" else
+             ["tc2x", "tc2h", "tc3a1", "tc3b1", "tc3a3b3b4b6", "tc3a2a4b2b5", "kinetic"])
     for t in terms:
         got = getattr(o, t)(nb, phi)
         ref = z[t]
@@ -153,13 +154,39 @@ def test_tc2x_reduces_to_hf2x_without_jastrow():
     assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
 
 
+def test_tc_terms_without_jastrow():
+    """A = 0: tc2h is the Hartree term hf2h and every three-body term vanishes."""
+    sys_ = S.tiny_system(method="TC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, A=np.zeros((2, 2)))
+    o = O.Oracle(sys_)
+    nb, phi = _random_bands(sys_, 7)
+    a = o.tc2h(nb, phi)
+    b = o.hf2h(nb, phi)
+    assert np.abs(a - b).max() < 1e-13 * np.abs(b).max()
+    for t in ("tc3a3b3b4b6", "tc3a2a4b2b5"):
+        assert np.abs(getattr(o, t)(nb, phi)).max() == 0.0, t
+
+
+def test_three_body_exchange_needs_divergence_flag():
+    """The [3b6] and [3a2] G=0 corrections are O(A^2) and switch off with the
+    divergence-correction flag: the difference is non-zero and scales as lambda^2."""
+    d = []
+    for lam in (1.0, 2.0):
+        s_on = _scaled(lam)
+        nb, phi = _random_bands(s_on, 5)
+        on = O.Oracle(s_on)
+        off = O.Oracle(s_on, div_corr=False)
+        d.append(sum(getattr(on, t)(nb, phi) - getattr(off, t)(nb, phi) for t in ("tc3a3b3b4b6", "tc3a2a4b2b5")))
+    assert np.abs(d[0]).max() > 1e-6 * np.abs(d[1]).max()
+    assert np.abs(d[1] - 4.0 * d[0]).max() < 1e-12 * np.abs(d[1]).max()
+
+
 def test_bitc_with_equal_orbitals_is_tc():
     base = S.tiny_system(method="TC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2)
     bitc = S.tiny_system(method="TC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2)
     bitc.method = "BITC"
     bitc.chi_occ = bitc.phi_occ.copy()
     nb, phi = _random_bands(base, 8)
-    for t in ("tc2x", "tc3a1", "tc3b1"):
+    for t in ("tc2x", "tc2h", "tc3a1", "tc3b1", "tc3a3b3b4b6", "tc3a2a4b2b5"):
         a = getattr(O.Oracle(base), t)(nb, phi)
         b = getattr(O.Oracle(bitc), t)(nb, phi)
         assert np.abs(a - b).max() < 1e-14 * max(1e-300, np.abs(a).max()), t
@@ -172,7 +199,8 @@ def _scaled(A_scale, method="TC"):
     return sys_
 
 
-@pytest.mark.parametrize("term,degree", [("tc2x", 2), ("tc3a1", 2), ("tc3b1", 2)])
+@pytest.mark.parametrize("term,degree", [("tc2x", 2), ("tc2h", 2), ("tc3a1", 2), ("tc3b1", 2),
+                                         ("tc3a3b3b4b6", 2), ("tc3a2a4b2b5", 2)])
 def test_polynomial_in_jastrow_strength(term, degree):
     """At fixed F, u -> lambda*u makes tc2x a quadratic and the three-body terms
     pure quadratics in lambda: third differences vanish, and tc3 scales as lambda^2."""
@@ -181,7 +209,7 @@ def test_polynomial_in_jastrow_strength(term, degree):
     vals = [getattr(O.Oracle(_scaled(l)), term)(nb, phi) for l in (0.5, 1.0, 1.5, 2.0)]
     third = vals[3] - 3 * vals[2] + 3 * vals[1] - vals[0]
     assert np.abs(third).max() < 1e-10 * np.abs(vals[3]).max()
-    if term != "tc2x":
+    if term.startswith("tc3"):
         assert np.abs(vals[3] - 4.0 * vals[1]).max() < 1e-12 * np.abs(vals[3]).max()
 
 
