@@ -130,6 +130,34 @@ int tcx_tc3a1(tcx_context *ctx, const int *nb, const double *phi, double *h3phi,
 int tcx_tc3b1(tcx_context *ctx, const int *nb, const double *phi, double *h3phi,
               int precision, unsigned flags);
 
+/* Three-body exchange terms [3a3(+3a6), 3b3, 3b4, 3b6] and [3a2, 3a4(+3a5),
+ * 3b2, 3b5]: replace calc_hamiltonian::tc3a3b3b4b6 and tc3a2a4b2b5
+ * (src/include/calc_hamiltonian.hpp:111-130; src/calc_hamiltonian_tc3a3b3b4b6.cpp:11-863,
+ * src/calc_hamiltonian_tc3a2a4b2b5.cpp:11-1300).  Identity symmetry only (the
+ * reference's Symmetry argument is the identity); k-points on the SCF mesh. */
+int tcx_tc3a3b3b4b6(tcx_context *ctx, const int *nb, const double *phi, double *h3phi,
+                    int precision, unsigned flags);
+int tcx_tc3a2a4b2b5(tcx_context *ctx, const int *nb, const double *phi, double *h3phi,
+                    int precision, unsigned flags);
+
+/* One-body kinetic term 0.5|k+G|^2 phi: replaces calc_hamiltonian::kinetic
+ * (src/include/calc_hamiltonian.hpp:23-31; src/calc_hamiltonian_kinetic.cpp:6-52).  FP64. */
+int tcx_kinetic(tcx_context *ctx, const int *nb, const double *phi, double *h1phi, unsigned flags);
+
+/* Hartree terms: replace calc_hamiltonian::hf2h (HF; src/calc_hamiltonian_hf2h.cpp:8-85)
+ * and tc2h (TC/BITC [2a_h, 2b_h1, 2b_h2]; src/calc_hamiltonian_tc2h.cpp:10-236)
+ * (src/include/calc_hamiltonian.hpp:43-52, 66-75).  FP64. */
+int tcx_hf2h(tcx_context *ctx, const int *nb, const double *phi, double *h2phi, unsigned flags);
+int tcx_tc2h(tcx_context *ctx, const int *nb, const double *phi, double *h2phi, unsigned flags);
+
+/* The whole H*phi: replaces calc_hamiltonian::all (src/include/calc_hamiltonian.hpp:10-21;
+ * src/calc_hamiltonian_all.cpp:6-198).  Zeroes h1phi, h2phi and (TC/BITC) h3phi, then
+ * kinetic -> h1phi; HF: hf2h + hf2x -> h2phi; TC/BITC: tc2h + tc2x -> h2phi and
+ * tc3a1 + tc3b1 + tc3a2a4b2b5 + tc3a3b3b4b6 -> h3phi.  h3phi may be NULL for HF.
+ * No pseudopotential term (homogeneous-electron-gas cells, all.cpp:58-66). */
+int tcx_apply_all(tcx_context *ctx, const int *nb, const double *phi, double *h1phi,
+                  double *h2phi, double *h3phi, int precision, unsigned flags);
+
 /* ---------------------------------------------------------------- device buffers */
 /* No counterpart in the reference (which keeps everything in host memory).
  * For callers that hold no HIP runtime of their own and want inputs and

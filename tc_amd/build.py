@@ -29,8 +29,10 @@ SOURCES = [
     "csrc/kernels_hf2x.hip",
     "csrc/kernels_tc2x.hip",
     "csrc/kernels_local.hip",
+    "csrc/kernels_tc3x.hip",
     "csrc/tcx_api.cpp",
     "csrc/tcx_tc.cpp",
+    "csrc/tcx_tc3x.cpp",
     "host/calc_hamiltonian.cpp",
 ]
 ROOT = os.path.dirname(HERE)

@@ -1,0 +1,933 @@
+// kernels_tc3x.hip -- building blocks of the three-body exchange terms
+//   tc3a3b3b4b6 (src/calc_hamiltonian_tc3a3b3b4b6.cpp:11-863) and
+//   tc3a2a4b2b5 (src/calc_hamiltonian_tc3a2a4b2b5.cpp:11-1300)
+// and of the local assembly terms kinetic / hf2h / tc2h.
+//
+// Every (target, source) pair of a launch owns a record of C grids (complex
+// FP32 or FP64, consecutive in HBM) that moves through
+//   products   record[c] = A_c(r) * B_c(r), orbital grids chosen by the pair;
+//   3D FFT     batched over all records of the launch;
+//   combine    a per-term G-space map that multiplies by the Jastrow gradient
+//              du_d(G) = (kq+G)_d u(kq+G), evaluated in FP64 on the fly;
+//   3D FFT     batched;
+//   accumulate one thread owns one point of one target and loops over the
+//              target's pairs, summing in FP64 registers, then adds once into
+//              the FP64 real-space grid of the target (no atomics).
+// All point-wise kernels are one thread per grid point, 256 threads, with the
+// record / target / band index in blockIdx.y (< 65536, the launchers split).
+#include "jastrow_dev.hpp"
+#include "kernels_common.hpp"
+
+namespace tcx {
+
+namespace {
+
+constexpr double kFourPi = 4.0 * 3.14159265358979323846;
+
+__device__ __forceinline__ void miller_of(const Geom& g, long long e, int& m1, int& m2, int& m3)
+{
+    const int i1 = (int)(e % g.n1), i2 = (int)((e / g.n1) % g.n2), i3 = (int)(e / ((long long)g.n1 * g.n2));
+    m1 = centered(i1, g.n1);
+    m2 = centered(i2, g.n2);
+    m3 = centered(i3, g.n3);
+}
+
+// x = kq + G of grid point e (kq may be null: G alone); returns |x|^2
+__device__ __forceinline__ double kq_plus_G(const Geom& g, long long e, const double* kq, double x[3])
+{
+    int m1, m2, m3;
+    miller_of(g, e, m1, m2, m3);
+    double x2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        x[d] = (kq ? kq[d] : 0.0) + g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
+        x2 += x[d] * x[d];
+    }
+    return x2;
+}
+
+__device__ __forceinline__ double2 dmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 dconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 dadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 dscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+template <typename T>
+__device__ __forceinline__ double2 up(cpx<T> a)
+{
+    return make_double2((double)a.re, (double)a.im);
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> down(double2 a)
+{
+    return cpx<T>{(T)a.x, (T)a.y};
+}
+
+inline dim3 grid_pts(long long N, unsigned y) { return dim3((unsigned)((N + 255) / 256), y); }
+
+// Run f(first, count) over [0, n) in slices that fit blockIdx.y.
+template <typename F>
+void for_slices(long long n, F&& f)
+{
+    for (long long a = 0; a < n; a += 65535) f((int)a, (unsigned)(n - a < 65535 ? n - a : 65535));
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- products
+template <typename T>
+__global__ __launch_bounds__(256) void k_products(const PairEntry* __restrict__ pairs,
+                                                  const ProdChan* __restrict__ chans, int nchan, int rec0,
+                                                  cpx<T>* __restrict__ out, long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int rec = rec0 + (int)blockIdx.y;
+    const int p = rec / nchan, c = rec - p * nchan;
+    const PairEntry e = pairs[p];
+    const ProdChan ch = chans[c];
+    const long long ia = (long long)(ch.a_sel ? e.occ : e.app) * ch.a_mul + ch.a_off;
+    const long long ib = (long long)(ch.b_sel ? e.occ : e.app) * ch.b_mul + ch.b_off;
+    cpx<T> a = ((const cpx<T>*)ch.a)[ia * N + r];
+    cpx<T> b = ((const cpx<T>*)ch.b)[ib * N + r];
+    if (ch.a_conj) a = conjg(a);
+    if (ch.b_conj) b = conjg(b);
+    out[(long long)rec * N + r] = a * b;
+}
+
+void launch_products(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
+                     long long N, hipStream_t s)
+{
+    for_slices((long long)npairs * nchan, [&](int r0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_products<double>, grid_pts(N, ny), dim3(256), 0, s, pairs, chans, nchan, r0,
+                               (cpx<double>*)out, N);
+        else
+            hipLaunchKernelGGL(k_products<float>, grid_pts(N, ny), dim3(256), 0, s, pairs, chans, nchan, r0,
+                               (cpx<float>*)out, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// ----------------------------------------------------------------- generic accumulate
+// W[t.app](r) += factor * sum_{p in t} rec[p*stride + ch](r) * M[pair.occ](r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_acc_occ(const cpx<T>* __restrict__ rec, int stride, int ch,
+                                                 const cpx<T>* __restrict__ M, const PairEntry* __restrict__ pairs,
+                                                 const TargetEntry* __restrict__ targets, int t0,
+                                                 double2* __restrict__ W, long long N, double factor)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const TargetEntry t = targets[t0 + blockIdx.y];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = 0; i < t.npair; ++i) {
+        const int p = t.pair0 + i;
+        const double2 v = up(rec[((long long)p * stride + ch) * N + r]);
+        const double2 m = up(M[(long long)pairs[p].occ * N + r]);
+        acc = dadd(acc, dmul(v, m));
+    }
+    double2* w = W + (long long)t.app * N + r;
+    *w = dadd(*w, dscale(acc, factor));
+}
+
+void launch_acc_occ(int prec, const void* rec, int stride, int ch, const void* M, const PairEntry* pairs,
+                    const TargetEntry* targets, int ntargets, double* W, long long N, double factor, hipStream_t s)
+{
+    for_slices(ntargets, [&](int t0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_acc_occ<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec, stride,
+                               ch, (const cpx<double>*)M, pairs, targets, t0, (double2*)W, N, factor);
+        else
+            hipLaunchKernelGGL(k_acc_occ<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec, stride, ch,
+                               (const cpx<float>*)M, pairs, targets, t0, (double2*)W, N, factor);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// ----------------------------------------------------------------- tc3a3b3b4b6
+// Phase-1 combine (tc3a3b3b4b6.cpp:280-349).  rec[p] = FFT_forward of
+// (conj(chi_qi) phi_j, conj(chi_qi) dnu_d phi_j).  With du_s,d = (k-q+G)_d u(k-q+G, s, 0):
+//   Y[p]        = f b3 sum_d du_0,d rec[p][1+d]          ([3b3], before FFT_backward)
+//   P[p][3s+d]  = f rec[p][0] du_s,d                      ([3a3,b4,b6])
+template <typename T>
+__global__ __launch_bounds__(256) void k_a7_comb1(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Y,
+                                                  cpx<T>* __restrict__ P, const PairEntry* __restrict__ pairs, int p0,
+                                                  const double* __restrict__ kqcart, const double* __restrict__ fill,
+                                                  Geom g, long long N, JastrowRPA j0, JastrowRPA j1, double fb3)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int p = p0 + (int)blockIdx.y;
+    const PairEntry e = pairs[p];
+    double x[3];
+    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
+    const double u0 = jastrow_uk(j0, x2), u1 = jastrow_uk(j1, x2);
+    const double f = fill[e.occ];
+    const cpx<T>* in = rec + (long long)p * 4 * N + r;
+    const double2 rho = up(in[0]);
+    double2 y = make_double2(0.0, 0.0);
+    for (int d = 0; d < 3; ++d) y = dadd(y, dscale(up(in[(long long)(d + 1) * N]), x[d] * u0));
+    Y[(long long)p * N + r] = down<T>(dscale(y, f * fb3));
+    cpx<T>* po = P + (long long)p * 6 * N + r;
+    for (int d = 0; d < 3; ++d) {
+        po[(long long)d * N] = down<T>(dscale(rho, f * x[d] * u0));
+        po[(long long)(3 + d) * N] = down<T>(dscale(rho, f * x[d] * u1));
+    }
+}
+
+// Phase-1 real-space step (tc3a3b3b4b6.cpp:316-335): per target (k,j)
+//   qsum[j][d] += sum_p P[p][d] phi_qi  (s2 = 0 channels, before the density)
+//   P[p][3s+d] *= density_s(r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_a7_mid(cpx<T>* __restrict__ P, const cpx<T>* __restrict__ occR,
+                                                const PairEntry* __restrict__ pairs,
+                                                const TargetEntry* __restrict__ targets, int t0,
+                                                const double2* __restrict__ dens, double2* __restrict__ qsum,
+                                                long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const TargetEntry t = targets[t0 + blockIdx.y];
+    const double2 n0 = dens[r], n1 = dens[N + r];
+    double2 acc0 = make_double2(0.0, 0.0), acc1 = acc0, acc2 = acc0;
+    for (int i = 0; i < t.npair; ++i) {
+        const int p = t.pair0 + i;
+        const double2 ph = up(occR[(long long)pairs[p].occ * N + r]);
+        cpx<T>* pp = P + (long long)p * 6 * N + r;
+        const double2 v0 = up(pp[0]), v1 = up(pp[N]), v2 = up(pp[2 * N]);
+        acc0 = dadd(acc0, dmul(v0, ph));
+        acc1 = dadd(acc1, dmul(v1, ph));
+        acc2 = dadd(acc2, dmul(v2, ph));
+        pp[0] = down<T>(dmul(v0, n0));
+        pp[N] = down<T>(dmul(v1, n0));
+        pp[2 * N] = down<T>(dmul(v2, n0));
+        for (int c = 3; c < 6; ++c) pp[(long long)c * N] = down<T>(dmul(up(pp[(long long)c * N]), n1));
+    }
+    double2* q = qsum + (long long)t.app * 3 * N + r;
+    q[0] = dadd(q[0], acc0);
+    q[N] = dadd(q[N], acc1);
+    q[2 * N] = dadd(q[2 * N], acc2);
+}
+
+// Phase-1 second combine (tc3a3b3b4b6.cpp:344-358, the [3b3] and [3b6] inverse
+// transforms merged by linearity): Y[p] += b6 sum_{s,d} du_s,d P[p][3s+d]
+template <typename T>
+__global__ __launch_bounds__(256) void k_a7_comb2(cpx<T>* __restrict__ Y, const cpx<T>* __restrict__ P,
+                                                  const PairEntry* __restrict__ pairs, int p0,
+                                                  const double* __restrict__ kqcart, Geom g, long long N,
+                                                  JastrowRPA j0, JastrowRPA j1, double fb6)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int p = p0 + (int)blockIdx.y;
+    const PairEntry e = pairs[p];
+    double x[3];
+    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
+    const double u0 = jastrow_uk(j0, x2), u1 = jastrow_uk(j1, x2);
+    const cpx<T>* pp = P + (long long)p * 6 * N + r;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int d = 0; d < 3; ++d) {
+        acc = dadd(acc, dscale(up(pp[(long long)d * N]), x[d] * u0));
+        acc = dadd(acc, dscale(up(pp[(long long)(3 + d) * N]), x[d] * u1));
+    }
+    cpx<T>* y = Y + (long long)p * N + r;
+    *y = down<T>(dadd(up(*y), dscale(acc, fb6)));
+}
+
+// Phase-2 combine ([3b4], tc3a3b3b4b6.cpp:740-754): Z[p] = b4 f sum_d du_0,d rec[p][d],
+// rec[p][d] = FFT_forward(conj(chi_qi) qsum_j,d)
+template <typename T>
+__global__ __launch_bounds__(256) void k_b4_comb(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Z,
+                                                 const PairEntry* __restrict__ pairs, int p0,
+                                                 const double* __restrict__ kqcart, const double* __restrict__ fill,
+                                                 Geom g, long long N, JastrowRPA j0, double fb4)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int p = p0 + (int)blockIdx.y;
+    const PairEntry e = pairs[p];
+    double x[3];
+    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
+    const double u0 = jastrow_uk(j0, x2);
+    const cpx<T>* in = rec + (long long)p * 3 * N + r;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int d = 0; d < 3; ++d) acc = dadd(acc, dscale(up(in[(long long)d * N]), x[d] * u0));
+    Z[(long long)p * N + r] = down<T>(dscale(acc, fb4 * fill[e.occ]));
+}
+
+void launch_a7_comb1(int prec, const void* rec, void* Y, void* P, const PairEntry* pairs, int npairs,
+                     const double* kqcart, const double* fill, const Geom& g, long long N, const JastrowRPA& j0,
+                     const JastrowRPA& j1, double fb3, hipStream_t s)
+{
+    for_slices(npairs, [&](int p0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_a7_comb1<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec,
+                               (cpx<double>*)Y, (cpx<double>*)P, pairs, p0, kqcart, fill, g, N, j0, j1, fb3);
+        else
+            hipLaunchKernelGGL(k_a7_comb1<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec,
+                               (cpx<float>*)Y, (cpx<float>*)P, pairs, p0, kqcart, fill, g, N, j0, j1, fb3);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+void launch_a7_mid(int prec, void* P, const void* occR, const PairEntry* pairs, const TargetEntry* targets,
+                   int ntargets, const double* dens, double* qsum, long long N, hipStream_t s)
+{
+    for_slices(ntargets, [&](int t0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_a7_mid<double>, grid_pts(N, ny), dim3(256), 0, s, (cpx<double>*)P,
+                               (const cpx<double>*)occR, pairs, targets, t0, (const double2*)dens, (double2*)qsum, N);
+        else
+            hipLaunchKernelGGL(k_a7_mid<float>, grid_pts(N, ny), dim3(256), 0, s, (cpx<float>*)P,
+                               (const cpx<float>*)occR, pairs, targets, t0, (const double2*)dens, (double2*)qsum, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+void launch_a7_comb2(int prec, void* Y, const void* P, const PairEntry* pairs, int npairs, const double* kqcart,
+                     const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1, double fb6,
+                     hipStream_t s)
+{
+    for_slices(npairs, [&](int p0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_a7_comb2<double>, grid_pts(N, ny), dim3(256), 0, s, (cpx<double>*)Y,
+                               (const cpx<double>*)P, pairs, p0, kqcart, g, N, j0, j1, fb6);
+        else
+            hipLaunchKernelGGL(k_a7_comb2<float>, grid_pts(N, ny), dim3(256), 0, s, (cpx<float>*)Y,
+                               (const cpx<float>*)P, pairs, p0, kqcart, g, N, j0, j1, fb6);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+void launch_b4_comb(int prec, const void* rec, void* Z, const PairEntry* pairs, int npairs, const double* kqcart,
+                    const double* fill, const Geom& g, long long N, const JastrowRPA& j0, double fb4, hipStream_t s)
+{
+    for_slices(npairs, [&](int p0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_b4_comb<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec,
+                               (cpx<double>*)Z, pairs, p0, kqcart, fill, g, N, j0, fb4);
+        else
+            hipLaunchKernelGGL(k_b4_comb<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec,
+                               (cpx<float>*)Z, pairs, p0, kqcart, fill, g, N, j0, fb4);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// out[j][d](r) = dnu_d(r) * phi_j(r) in the working precision (dnu FP64, 3 grids)
+template <typename T>
+__global__ __launch_bounds__(256) void k_mul_dnu(const cpx<T>* __restrict__ appR, const double2* __restrict__ dnu,
+                                                 cpx<T>* __restrict__ out, int j0, long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int j = j0 + (int)blockIdx.y;
+    const double2 ph = up(appR[(long long)j * N + r]);
+    for (int d = 0; d < 3; ++d) out[((long long)j * 3 + d) * N + r] = down<T>(dmul(dnu[(long long)d * N + r], ph));
+}
+
+void launch_mul_dnu(int prec, const void* appR, const double* dnu, void* out, int napp, long long N, hipStream_t s)
+{
+    for_slices(napp, [&](int j0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_mul_dnu<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)appR,
+                               (const double2*)dnu, (cpx<double>*)out, j0, N);
+        else
+            hipLaunchKernelGGL(k_mul_dnu<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)appR,
+                               (const double2*)dnu, (cpx<float>*)out, j0, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// [3a3] (tc3a3b3b4b6.cpp:629-642): W[j] += fa3 sum_d qsum[j][d] dnu_d
+__global__ __launch_bounds__(256) void k_a3(double2* __restrict__ W, const double2* __restrict__ qsum,
+                                            const double2* __restrict__ dnu, int j0, long long N, double fa3)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int j = j0 + (int)blockIdx.y;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int d = 0; d < 3; ++d) acc = dadd(acc, dmul(qsum[((long long)j * 3 + d) * N + r], dnu[(long long)d * N + r]));
+    double2* w = W + (long long)j * N + r;
+    *w = dadd(*w, dscale(acc, fa3));
+}
+
+void launch_a3(double* W, const double* qsum, const double* dnu, int napp, long long N, double fa3, hipStream_t s)
+{
+    for_slices(napp, [&](int j0, unsigned ny) {
+        hipLaunchKernelGGL(k_a3, grid_pts(N, ny), dim3(256), 0, s, (double2*)W, (const double2*)qsum,
+                           (const double2*)dnu, j0, N, fa3);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// [3b6] divergence correction (tc3a3b3b4b6.cpp:424-451):
+//   tG(G) = sum_js 4 pi n_js(G) A_js u(G, 0, js) (G != 0),  tG(0) = 0
+__global__ __launch_bounds__(256) void k_b6_tG(const double2* __restrict__ nG, double2* __restrict__ tG, Geom g,
+                                               long long N, JastrowRPA j0, JastrowRPA j1)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    double x[3];
+    const double x2 = kq_plus_G(g, r, nullptr, x);
+    if (r == 0) {
+        tG[0] = make_double2(0.0, 0.0);
+        return;
+    }
+    const double c0 = kFourPi * j0.A * jastrow_uk(j0, x2), c1 = kFourPi * j1.A * jastrow_uk(j1, x2);
+    tG[r] = dadd(dscale(nG[r], c0), dscale(nG[N + r], c1));
+}
+
+//   t1[k] = sum_js 4 pi (dcp[k] A_js + u(0, 0, js)) A_js n_js(G=0)
+__global__ void k_b6_t1(const double2* __restrict__ nG, const double* __restrict__ dcp, int nk, long long N,
+                        JastrowRPA j0, JastrowRPA j1, double2* __restrict__ t1)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    const double a0 = kFourPi * (dcp[k] * j0.A + jastrow_uk(j0, 0.0)) * j0.A;
+    const double a1 = kFourPi * (dcp[k] * j1.A + jastrow_uk(j1, 0.0)) * j1.A;
+    t1[k] = dadd(dscale(nG[0], a0), dscale(nG[N], a1));
+}
+
+// One workgroup per applied band j at k (tc3a3b3b4b6.cpp:456-532), for every
+// occupied band b at k with f_b >= 1e-8:
+//   Ib    = <chi_b|phi_j>  (= conj(FFT_forward(chi_b conj(phi_j))(G=0)))
+//   dot_b = sum_G conj(FFT_forward(chi_b conj(phi_j))) tG
+//         = (1/N) sum_r conj(chi_b(r)) phi_j(r) tR(r)        (Parseval, tR = FFT_backward(tG))
+//   out_j += f b6 (t1[k] Ib + dot_b) phi_b                   (contributions 1 and 3)
+//   cR[j][b] = f Ib b6                                       (contribution 2, real space)
+constexpr int kB6MaxOcc = 512;
+
+__global__ __launch_bounds__(256) void k_b6_coef(const double2* __restrict__ phi, const double2* __restrict__ occ_r,
+                                                 const double2* __restrict__ occ_l, const DivEntry* __restrict__ jobs,
+                                                 const double* __restrict__ fill, const double2* __restrict__ t1,
+                                                 const double2* __restrict__ appR, const double2* __restrict__ chiR,
+                                                 const double2* __restrict__ tR, double fb6, long long N,
+                                                 double2* __restrict__ out, double2* __restrict__ cR, int maxocc)
+{
+    const int j = blockIdx.x;
+    const DivEntry e = jobs[j];
+    __shared__ double2 coef[kB6MaxOcc];
+    __shared__ double red[2][256];
+    const int t = threadIdx.x;
+    const double2* pj = phi + e.app_off;
+    const double2* aj = appR + (long long)j * N;
+    const double invN = 1.0 / (double)N;
+    for (int b = 0; b < e.nocc; ++b) {
+        const double f = fill[e.fill_off + b];
+        const double2* cb = occ_l + e.occ_off + (long long)b * e.npw;
+        const double2* xb = chiR + (long long)(e.fill_off + b) * N;
+        double2 ib = make_double2(0.0, 0.0), db = make_double2(0.0, 0.0);
+        for (int l = t; l < e.npw; l += 256) ib = dadd(ib, dmul(dconj(cb[l]), pj[l]));
+        for (long long r = t; r < N; r += 256) db = dadd(db, dmul(dmul(dconj(xb[r]), aj[r]), tR[r]));
+        // block reduction of the two complex sums, one at a time through LDS
+        double2 sums[2] = {ib, db};
+        for (int q = 0; q < 2; ++q) {
+            red[0][t] = sums[q].x;
+            red[1][t] = sums[q].y;
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if (t < w) {
+                    red[0][t] += red[0][t + w];
+                    red[1][t] += red[1][t + w];
+                }
+                __syncthreads();
+            }
+            sums[q] = make_double2(red[0][0], red[1][0]);
+            __syncthreads();
+        }
+        if (t == 0) {
+            double2 c = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
+            if (f >= 1e-8) {
+                const double2 Ib = sums[0], dot = dscale(sums[1], invN);
+                c = dscale(dadd(dmul(t1[e.kidx], Ib), dot), f * fb6);
+                cr = dscale(Ib, f * fb6);
+            }
+            coef[b] = c;
+            cR[(long long)j * maxocc + b] = cr;
+        }
+        __syncthreads();
+    }
+    double2* o = out + e.app_off;
+    for (int l = t; l < e.npw; l += 256) {
+        double2 acc = o[l];
+        for (int b = 0; b < e.nocc; ++b) acc = dadd(acc, dmul(coef[b], occ_r[e.occ_off + (long long)b * e.npw + l]));
+        o[l] = acc;
+    }
+}
+
+// W[j](r) += tR(r) sum_b cR[j][b] phi_b(r)
+__global__ __launch_bounds__(256) void k_b6_W(const double2* __restrict__ cR, int maxocc,
+                                              const DivEntry* __restrict__ jobs, const double2* __restrict__ occR,
+                                              const double2* __restrict__ tR, double2* __restrict__ W, int j0,
+                                              long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int j = j0 + (int)blockIdx.y;
+    const DivEntry e = jobs[j];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int b = 0; b < e.nocc; ++b)
+        acc = dadd(acc, dmul(cR[(long long)j * maxocc + b], occR[(long long)(e.fill_off + b) * N + r]));
+    double2* w = W + (long long)j * N + r;
+    *w = dadd(*w, dmul(acc, tR[r]));
+}
+
+void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, int nk, const Geom& g, long long N,
+               const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_b6_tG, grid_pts(N, 1), dim3(256), 0, s, (const double2*)nG, (double2*)tG, g, N, j0, j1);
+    TCX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_b6_t1, dim3((nk + 63) / 64), dim3(64), 0, s, (const double2*)nG, dcp, nk, N, j0, j1,
+                       (double2*)t1);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l, const DivEntry* jobs, int njobs,
+                    int maxocc, const double* fill, const double* t1, const double* appR, const double* chiR,
+                    const double* occR, const double* tR, double fb6, long long N, double* out, double* cR,
+                    double* W, hipStream_t s)
+{
+    if (njobs == 0) return;
+    if (maxocc > kB6MaxOcc) throw Error(TCX_ERR_UNSUPPORTED, "tc3a3b3b4b6: more than 512 occupied bands at one k");
+    hipLaunchKernelGGL(k_b6_coef, dim3(njobs), dim3(256), 0, s, (const double2*)phi, (const double2*)occ_r,
+                       (const double2*)occ_l, jobs, fill, (const double2*)t1, (const double2*)appR,
+                       (const double2*)chiR, (const double2*)tR, fb6, N, (double2*)out, (double2*)cR, maxocc);
+    TCX_HIP_CHECK(hipGetLastError());
+    for_slices(njobs, [&](int j0, unsigned ny) {
+        hipLaunchKernelGGL(k_b6_W, grid_pts(N, ny), dim3(256), 0, s, (const double2*)cR, maxocc, jobs,
+                           (const double2*)occR, (const double2*)tR, (double2*)W, j0, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// ----------------------------------------------------------------- tc3a2a4b2b5
+// q1-q2 phase combine (tc3a2a4b2b5.cpp:371-375): O[p][6h + 3s + d] = du12_s,d rec[p][h],
+// du12_s,d = (q2-q1+G)_d u(q2-q1+G, 0, s); h = 0 (conj(chi_q1) phi_q2) and, for BITC,
+// h = 1 (conj(phi_q1) chi_q2).
+template <typename T>
+__global__ __launch_bounds__(256) void k_a8_comb1(const cpx<T>* __restrict__ rec, int nin, cpx<T>* __restrict__ O,
+                                                  const PairEntry* __restrict__ pairs, int p0,
+                                                  const double* __restrict__ kqcart, Geom g, long long N,
+                                                  JastrowRPA j0, JastrowRPA j1)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int p = p0 + (int)blockIdx.y;
+    const PairEntry e = pairs[p];
+    double x[3];
+    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
+    const double u[2] = {jastrow_uk(j0, x2), jastrow_uk(j1, x2)};
+    for (int h = 0; h < nin; ++h) {
+        const double2 v = up(rec[((long long)p * nin + h) * N + r]);
+        cpx<T>* o = O + ((long long)p * 6 * nin + 6 * h) * N + r;
+        for (int s = 0; s < 2; ++s)
+            for (int d = 0; d < 3; ++d) o[(long long)(3 * s + d) * N] = down<T>(dscale(v, x[d] * u[s]));
+    }
+}
+
+// q1-q2 phase accumulation (tc3a2a4b2b5.cpp:377-427), per target (q2,i2):
+//   Q2 -= f1 sum_{s,d} O conj(O2)            (O2 = O for TC: -f1 |O|^2)
+//   Q4[d] += f1 O_0,d phi_q1                  Q5[d] -= f1 conj(O2_0,d) conj(chi_q1)   (BITC)
+template <typename T>
+__global__ __launch_bounds__(256) void k_a8_acc1(const cpx<T>* __restrict__ O, int bitc,
+                                                 const cpx<T>* __restrict__ occR, const cpx<T>* __restrict__ chiR,
+                                                 const PairEntry* __restrict__ pairs,
+                                                 const TargetEntry* __restrict__ targets, int t0,
+                                                 const double* __restrict__ fill, double2* __restrict__ Q2,
+                                                 double2* __restrict__ Q4, double2* __restrict__ Q5, long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const TargetEntry t = targets[t0 + blockIdx.y];
+    const int nch = bitc ? 12 : 6;
+    double2 a2 = make_double2(0.0, 0.0);
+    double2 a4[3] = {a2, a2, a2}, a5[3] = {a2, a2, a2};
+    for (int i = 0; i < t.npair; ++i) {
+        const int p = t.pair0 + i;
+        const PairEntry e = pairs[p];
+        const double f1 = fill[e.occ];
+        const double2 ph = up(occR[(long long)e.occ * N + r]);
+        const double2 ch = bitc ? up(chiR[(long long)e.occ * N + r]) : ph;
+        const cpx<T>* base = O + (long long)p * nch * N + r;
+        for (int s = 0; s < 2; ++s)
+            for (int d = 0; d < 3; ++d) {
+                const double2 o = up(base[(long long)(3 * s + d) * N]);
+                const double2 o2 = bitc ? up(base[(long long)(6 + 3 * s + d) * N]) : o;
+                a2 = dadd(a2, dscale(dmul(o, dconj(o2)), -f1));
+                if (s == 0) {
+                    a4[d] = dadd(a4[d], dscale(dmul(o, ph), f1));
+                    if (bitc) a5[d] = dadd(a5[d], dscale(dconj(dmul(o2, ch)), -f1));
+                }
+            }
+    }
+    double2* q2 = Q2 + (long long)t.app * N + r;
+    *q2 = dadd(*q2, a2);
+    for (int d = 0; d < 3; ++d) {
+        double2* q4 = Q4 + ((long long)t.app * 3 + d) * N + r;
+        *q4 = dadd(*q4, a4[d]);
+        if (bitc) {
+            double2* q5 = Q5 + ((long long)t.app * 3 + d) * N + r;
+            *q5 = dadd(*q5, a5[d]);
+        }
+    }
+}
+
+// [3a2] divergence correction (tc3a2a4b2b5.cpp:496-573), per target (q2,i2), over
+// the occupied i1 at q2 (rec[p] = FFT_forward(conj(chi_q2,i1) phi_q2,i2),
+// FFT_forward(conj(chi_q2,i2) phi_q2,i1)):
+//   T(0)  -= sum_s 4 pi A_s (A_s dc2[q2] + u_s(0)) f1 pa(0) pb(0)
+//   T(G)  -= sum_s 4 pi A_s f1 u_s(G) (pa(G) pb(0) + pa(0) pb(G))
+__global__ __launch_bounds__(256) void k_a8_div(double2* __restrict__ Tg, const double2* __restrict__ rec,
+                                                const PairEntry* __restrict__ pairs,
+                                                const TargetEntry* __restrict__ targets, int t0,
+                                                const double* __restrict__ fill, const double* __restrict__ dc2,
+                                                Geom g, long long N, JastrowRPA j0, JastrowRPA j1)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const TargetEntry t = targets[t0 + blockIdx.y];
+    double x[3];
+    const double x2 = kq_plus_G(g, r, nullptr, x);
+    const double dc = dc2[t.pad];
+    const double w0 = r == 0 ? kFourPi * j0.A * (j0.A * dc + jastrow_uk(j0, 0.0)) : kFourPi * j0.A * jastrow_uk(j0, x2);
+    const double w1 = r == 0 ? kFourPi * j1.A * (j1.A * dc + jastrow_uk(j1, 0.0)) : kFourPi * j1.A * jastrow_uk(j1, x2);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = 0; i < t.npair; ++i) {
+        const int p = t.pair0 + i;
+        const double f1 = fill[pairs[p].occ];
+        const double2* pa = rec + (long long)p * 2 * N;
+        const double2* pb = pa + N;
+        const double2 pa0 = pa[0], pb0 = pb[0];
+        const double2 v = r == 0 ? dmul(pa0, pb0) : dadd(dmul(pa[r], pb0), dmul(pa0, pb[r]));
+        acc = dadd(acc, dscale(v, f1 * (w0 + w1)));
+    }
+    double2* o = Tg + (long long)t.app * N + r;
+    *o = make_double2(o->x - acc.x, o->y - acc.y);
+}
+
+// V3a2(G) = sum_t f_t T[t](G);  B2[d](r) = sum_t f_t Q5'[t][d](r) phi_t(r) with
+// Q5' = Q5 (BITC) or -conj(Q4) (TC, tc3a2a4b2b5.cpp:662-665)  ([3a2] :637-641, [3b2] :790-800)
+__global__ __launch_bounds__(256) void k_a8_reduce(const double2* __restrict__ Tg, const double2* __restrict__ Q4,
+                                                   const double2* __restrict__ Q5, int bitc,
+                                                   const double2* __restrict__ occR, const double* __restrict__ fill,
+                                                   int nocc, double2* __restrict__ V3a2, double2* __restrict__ B2,
+                                                   long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    double2 v = make_double2(0.0, 0.0), b0 = v, b1 = v, b2 = v;
+    for (int t = 0; t < nocc; ++t) {
+        const double f = fill[t];
+        v = dadd(v, dscale(Tg[(long long)t * N + r], f));
+        const double2 ph = dscale(occR[(long long)t * N + r], f);
+        double2 q[3];
+        for (int d = 0; d < 3; ++d) {
+            const long long o = ((long long)t * 3 + d) * N + r;
+            q[d] = bitc ? Q5[o] : make_double2(-Q4[o].x, Q4[o].y);
+        }
+        b0 = dadd(b0, dmul(q[0], ph));
+        b1 = dadd(b1, dmul(q[1], ph));
+        b2 = dadd(b2, dmul(q[2], ph));
+    }
+    V3a2[r] = v;
+    B2[r] = b0;
+    B2[N + r] = b1;
+    B2[2 * N + r] = b2;
+}
+
+// Q4T = Q4, Q5T = Q5' in the working precision (3 grids per occupied orbital)
+template <typename T>
+__global__ __launch_bounds__(256) void k_a8_q45(const double2* __restrict__ Q4, const double2* __restrict__ Q5,
+                                                int bitc, cpx<T>* __restrict__ Q4T, cpx<T>* __restrict__ Q5T, int g0,
+                                                long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const long long o = (long long)(g0 + blockIdx.y) * N + r;
+    const double2 a = Q4[o];
+    Q4T[o] = down<T>(a);
+    Q5T[o] = down<T>(bitc ? Q5[o] : make_double2(-a.x, a.y));
+}
+
+// Apply-phase combine (tc3a2a4b2b5.cpp:849-876), in place on rec[p][0..3]:
+//   in : FFT_forward(conj(chi_t) phi_kj), FFT_forward(Q5'_t,d phi_kj)
+//   out: b5 f2 sum_d du0_d in[1+d],  f2 a4 du0_d in[0]   (du0 = (k-q2+G)_d u(k-q2+G, 0, 0))
+template <typename T>
+__global__ __launch_bounds__(256) void k_a8_comb2(cpx<T>* __restrict__ rec, const PairEntry* __restrict__ pairs,
+                                                  int p0, const double* __restrict__ kqcart,
+                                                  const double* __restrict__ fill, Geom g, long long N, JastrowRPA j0,
+                                                  double fa4, double fb5)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int p = p0 + (int)blockIdx.y;
+    const PairEntry e = pairs[p];
+    double x[3];
+    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
+    const double u = jastrow_uk(j0, x2);
+    const double f2 = fill[e.occ];
+    cpx<T>* io = rec + (long long)p * 4 * N + r;
+    const double2 r0 = up(io[0]);
+    double2 b = make_double2(0.0, 0.0);
+    for (int d = 0; d < 3; ++d) b = dadd(b, dscale(up(io[(long long)(1 + d) * N]), x[d] * u));
+    io[0] = down<T>(dscale(b, fb5 * f2));
+    for (int d = 0; d < 3; ++d) io[(long long)(1 + d) * N] = down<T>(dscale(r0, f2 * fa4 * x[d] * u));
+}
+
+// W[t.app] += sum_p rec[p][0] phi_occ(p) + sum_d rec[p][1+d] Q4T[occ(p)][d]
+template <typename T>
+__global__ __launch_bounds__(256) void k_a8_acc2(const cpx<T>* __restrict__ rec, const cpx<T>* __restrict__ occR,
+                                                 const cpx<T>* __restrict__ Q4T, const PairEntry* __restrict__ pairs,
+                                                 const TargetEntry* __restrict__ targets, int t0,
+                                                 double2* __restrict__ W, long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const TargetEntry t = targets[t0 + blockIdx.y];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = 0; i < t.npair; ++i) {
+        const int p = t.pair0 + i;
+        const int o = pairs[p].occ;
+        const cpx<T>* in = rec + (long long)p * 4 * N + r;
+        acc = dadd(acc, dmul(up(in[0]), up(occR[(long long)o * N + r])));
+        for (int d = 0; d < 3; ++d)
+            acc = dadd(acc, dmul(up(in[(long long)(1 + d) * N]), up(Q4T[((long long)o * 3 + d) * N + r])));
+    }
+    double2* w = W + (long long)t.app * N + r;
+    *w = dadd(*w, acc);
+}
+
+void launch_a8_comb1(int prec, const void* rec, int nin, void* O, const PairEntry* pairs, int npairs,
+                     const double* kqcart, const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1,
+                     hipStream_t s)
+{
+    for_slices(npairs, [&](int p0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_a8_comb1<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec, nin,
+                               (cpx<double>*)O, pairs, p0, kqcart, g, N, j0, j1);
+        else
+            hipLaunchKernelGGL(k_a8_comb1<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec, nin,
+                               (cpx<float>*)O, pairs, p0, kqcart, g, N, j0, j1);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+void launch_a8_acc1(int prec, const void* O, int bitc, const void* occR, const void* chiR, const PairEntry* pairs,
+                    const TargetEntry* targets, int ntargets, const double* fill, double* Q2, double* Q4, double* Q5,
+                    long long N, hipStream_t s)
+{
+    for_slices(ntargets, [&](int t0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_a8_acc1<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)O, bitc,
+                               (const cpx<double>*)occR, (const cpx<double>*)chiR, pairs, targets, t0, fill,
+                               (double2*)Q2, (double2*)Q4, (double2*)Q5, N);
+        else
+            hipLaunchKernelGGL(k_a8_acc1<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)O, bitc,
+                               (const cpx<float>*)occR, (const cpx<float>*)chiR, pairs, targets, t0, fill,
+                               (double2*)Q2, (double2*)Q4, (double2*)Q5, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+void launch_a8_div(double* Tg, const double* rec, const PairEntry* pairs, const TargetEntry* targets, int ntargets,
+                   const double* fill, const double* dc2, const Geom& g, long long N, const JastrowRPA& j0,
+                   const JastrowRPA& j1, hipStream_t s)
+{
+    for_slices(ntargets, [&](int t0, unsigned ny) {
+        hipLaunchKernelGGL(k_a8_div, grid_pts(N, ny), dim3(256), 0, s, (double2*)Tg, (const double2*)rec, pairs,
+                           targets, t0, fill, dc2, g, N, j0, j1);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+void launch_a8_reduce(const double* Tg, const double* Q4, const double* Q5, int bitc, const double* occR,
+                      const double* fill, int nocc, double* V3a2, double* B2, long long N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_a8_reduce, grid_pts(N, 1), dim3(256), 0, s, (const double2*)Tg, (const double2*)Q4,
+                       (const double2*)Q5, bitc, (const double2*)occR, fill, nocc, (double2*)V3a2, (double2*)B2, N);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_a8_q45(int prec, const double* Q4, const double* Q5, int bitc, void* Q4T, void* Q5T, int ngrids,
+                   long long N, hipStream_t s)
+{
+    for_slices(ngrids, [&](int g0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_a8_q45<double>, grid_pts(N, ny), dim3(256), 0, s, (const double2*)Q4,
+                               (const double2*)Q5, bitc, (cpx<double>*)Q4T, (cpx<double>*)Q5T, g0, N);
+        else
+            hipLaunchKernelGGL(k_a8_q45<float>, grid_pts(N, ny), dim3(256), 0, s, (const double2*)Q4,
+                               (const double2*)Q5, bitc, (cpx<float>*)Q4T, (cpx<float>*)Q5T, g0, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+void launch_a8_comb2(int prec, void* rec, const PairEntry* pairs, int npairs, const double* kqcart,
+                     const double* fill, const Geom& g, long long N, const JastrowRPA& j0, double fa4, double fb5,
+                     hipStream_t s)
+{
+    for_slices(npairs, [&](int p0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_a8_comb2<double>, grid_pts(N, ny), dim3(256), 0, s, (cpx<double>*)rec, pairs, p0,
+                               kqcart, fill, g, N, j0, fa4, fb5);
+        else
+            hipLaunchKernelGGL(k_a8_comb2<float>, grid_pts(N, ny), dim3(256), 0, s, (cpx<float>*)rec, pairs, p0,
+                               kqcart, fill, g, N, j0, fa4, fb5);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+void launch_a8_acc2(int prec, const void* rec, const void* occR, const void* Q4T, const PairEntry* pairs,
+                    const TargetEntry* targets, int ntargets, double* W, long long N, hipStream_t s)
+{
+    for_slices(ntargets, [&](int t0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_a8_acc2<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec,
+                               (const cpx<double>*)occR, (const cpx<double>*)Q4T, pairs, targets, t0, (double2*)W, N);
+        else
+            hipLaunchKernelGGL(k_a8_acc2<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec,
+                               (const cpx<float>*)occR, (const cpx<float>*)Q4T, pairs, targets, t0, (double2*)W, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// ----------------------------------------------------------------- assembly terms
+// kinetic (src/calc_hamiltonian_kinetic.cpp:38-46): out += 0.5 |k+G|^2 phi, one job per band
+__global__ __launch_bounds__(256) void k_kinetic(const double2* __restrict__ phi, const int* __restrict__ gindex,
+                                                 const OrbEntry* __restrict__ jobs, const double* __restrict__ kcart,
+                                                 Geom g, double2* __restrict__ out)
+{
+    const OrbEntry e = jobs[blockIdx.y];
+    const int l = blockIdx.x * 256 + threadIdx.x;
+    if (l >= e.npw) return;
+    double x[3];
+    const double x2 = kq_plus_G(g, gindex[e.gidx_off + l], kcart + 3 * e.kidx, x);
+    const long long o = e.coef_off + l;
+    out[o] = dadd(out[o], dscale(phi[o], 0.5 * x2));
+}
+
+void launch_kinetic(const double* phi, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
+                    const double* kcart, const Geom& g, double* out, hipStream_t s)
+{
+    for_slices(njobs, [&](int j0, unsigned ny) {
+        hipLaunchKernelGGL(k_kinetic, dim3((max_npw + 255) / 256, ny), dim3(256), 0, s, (const double2*)phi, gindex,
+                           jobs + j0, kcart, g, (double2*)out);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// hf2h (src/calc_hamiltonian_hf2h.cpp:28-38): V(G) = 4 pi (n_0 + n_1)(G) / G^2, V(0) = 0
+__global__ __launch_bounds__(256) void k_hartree(const double2* __restrict__ nG, double2* __restrict__ V, Geom g,
+                                                 long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    double x[3];
+    const double x2 = kq_plus_G(g, r, nullptr, x);
+    V[r] = r == 0 ? make_double2(0.0, 0.0) : dscale(dadd(nG[r], nG[N + r]), kFourPi / x2);
+}
+
+void launch_hartree(const double* nG, double* V, const Geom& g, long long N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_hartree, grid_pts(N, 1), dim3(256), 0, s, (const double2*)nG, (double2*)V, g, N);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+// tc2h (src/calc_hamiltonian_tc2h.cpp:60-124): pgp_d(r) = sum_{q,i} f conj(chi_qi(r)) gphi_qi,d(r)
+__global__ __launch_bounds__(256) void k_pgp(const double2* __restrict__ chiR, const double2* __restrict__ gradR,
+                                             const double* __restrict__ fill, int nocc, double2* __restrict__ pgp,
+                                             long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    double2 a0 = make_double2(0.0, 0.0), a1 = a0, a2 = a0;
+    for (int b = 0; b < nocc; ++b) {
+        const double2 c = dscale(dconj(chiR[(long long)b * N + r]), fill[b]);
+        a0 = dadd(a0, dmul(c, gradR[((long long)b * 3 + 0) * N + r]));
+        a1 = dadd(a1, dmul(c, gradR[((long long)b * 3 + 1) * N + r]));
+        a2 = dadd(a2, dmul(c, gradR[((long long)b * 3 + 2) * N + r]));
+    }
+    pgp[r] = a0;
+    pgp[N + r] = a1;
+    pgp[2 * N + r] = a2;
+}
+
+// tc2h.cpp:146-171 (one independent spin, spin_factor 2):
+//   V2(G) = sum_js [tc_2body(G,0,js) n_js(G) + u(G,0,js) sum_d G_d pgp_d(G) / 2]
+__global__ __launch_bounds__(256) void k_tc2h_V(const double2* __restrict__ nG, const double2* __restrict__ pgpG,
+                                                double2* __restrict__ V, Geom g, long long N, JastrowRPA j0,
+                                                JastrowRPA j1)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    double x[3];
+    const double x2 = kq_plus_G(g, r, nullptr, x);
+    const double2 n0 = nG[r], n1 = nG[N + r];
+    double2 v = dadd(dscale(n0, jastrow_tc2body(j0, x2)), dscale(n1, jastrow_tc2body(j1, x2)));
+    const double us = jastrow_uk(j0, x2) + jastrow_uk(j1, x2);
+    for (int d = 0; d < 3; ++d) v = dadd(v, dscale(pgpG[(long long)d * N + r], 0.5 * us * x[d]));
+    V[r] = v;
+}
+
+void launch_tc2h_V(const double* chiR, const double* gradR, const double* fill, int nocc, double* pgp,
+                   const Geom& g, long long N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pgp, grid_pts(N, 1), dim3(256), 0, s, (const double2*)chiR, (const double2*)gradR, fill,
+                       nocc, (double2*)pgp, N);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tc2h_combine(const double* nG, const double* pgpG, double* V, const Geom& g, long long N,
+                         const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tc2h_V, grid_pts(N, 1), dim3(256), 0, s, (const double2*)nG, (const double2*)pgpG,
+                       (double2*)V, g, N, j0, j1);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+// tc2h.cpp:212-217: out[b] = V2 phi_b - sum_d gphi_b,d dnu_d (bands of a chunk; grads 3 per band)
+__global__ __launch_bounds__(256) void k_tc2h_apply(double2* __restrict__ orb, const double2* __restrict__ grad,
+                                                    const double2* __restrict__ V2, const double2* __restrict__ dnu,
+                                                    int b0, long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const int b = b0 + (int)blockIdx.y;
+    double2* o = orb + (long long)b * N + r;
+    double2 v = dmul(V2[r], *o);
+    for (int d = 0; d < 3; ++d) {
+        const double2 t = dmul(grad[((long long)b * 3 + d) * N + r], dnu[(long long)d * N + r]);
+        v = make_double2(v.x - t.x, v.y - t.y);
+    }
+    *o = v;
+}
+
+void launch_tc2h_apply(double* orb, const double* grad, const double* V2, const double* dnu, int nb, long long N,
+                       hipStream_t s)
+{
+    for_slices(nb, [&](int b0, unsigned ny) {
+        hipLaunchKernelGGL(k_tc2h_apply, grid_pts(N, ny), dim3(256), 0, s, (double2*)orb, (const double2*)grad,
+                           (const double2*)V2, (const double2*)dnu, b0, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+// out = a X + b Y (FP64 grids)
+__global__ __launch_bounds__(256) void k_axpby(const double2* __restrict__ X, const double2* __restrict__ Y,
+                                               double2* __restrict__ out, double a, double b, long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    out[r] = dadd(dscale(X[r], a), dscale(Y[r], b));
+}
+
+void launch_axpby(const double* X, const double* Y, double* out, double a, double b, long long N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpby, grid_pts(N, 1), dim3(256), 0, s, (const double2*)X, (const double2*)Y,
+                       (double2*)out, a, b, N);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace tcx

@@ -45,6 +45,8 @@ void DBuf::ensure(size_t bytes)
 
 void DBuf::release()
 {
+    // kernels queued on any stream may still read the old allocation
+    if (p) (void)hipDeviceSynchronize();
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -317,6 +319,69 @@ void Context::resolve_shape(int term, long& tc, long& sc, long ntargets, long ns
     }
 }
 
+void build_pairs(PairSet& S, const std::vector<PairItem>& tg, const std::vector<PairItem>& src, long tc, long sc,
+                 const std::vector<double>& kcart, bool same_k)
+{
+    for (size_t t0 = 0; t0 < tg.size(); t0 += tc) {
+        const size_t t1 = std::min(tg.size(), t0 + (size_t)tc);
+        const int kf = tg[t0].k, kl = tg[t1 - 1].k;
+        for (size_t s0 = 0; s0 < src.size(); s0 += sc) {
+            const size_t s1 = std::min(src.size(), s0 + (size_t)sc);
+            const int qf = src[s0].k, ql = src[s1 - 1].k;
+            if (same_k && (ql < kf || qf > kl)) continue;
+            Launch L;
+            L.pair0 = (int)S.pairs.size();
+            L.target0 = (int)S.targets.size();
+            L.kq0 = (int)S.kq.size() / 3;
+            L.nkq = (kl - kf + 1) * (ql - qf + 1);
+            for (int k = kf; k <= kl; ++k)
+                for (int q = qf; q <= ql; ++q)
+                    for (int d = 0; d < 3; ++d) S.kq.push_back(kcart[3 * k + d] - kcart[3 * q + d]);
+            int local = 0;
+            for (size_t t = t0; t < t1; ++t) {
+                TargetEntry te{tg[t].idx, local, 0, tg[t].k};
+                for (size_t s = s0; s < s1; ++s) {
+                    if (same_k && src[s].k != tg[t].k) continue;
+                    const int kq = L.kq0 + (tg[t].k - kf) * (ql - qf + 1) + (src[s].k - qf);
+                    S.pairs.push_back({tg[t].idx, src[s].idx, kq, 0});
+                    ++local;
+                    ++te.npair;
+                }
+                S.targets.push_back(te);
+            }
+            L.npairs = local;
+            L.ntargets = (int)(t1 - t0);
+            if (local == 0) {
+                S.targets.resize(L.target0);
+                S.kq.resize(3 * (size_t)L.kq0);
+                continue;
+            }
+            S.launches.push_back(L);
+        }
+    }
+    for (const auto& L : S.launches) S.max_launch_pairs = std::max(S.max_launch_pairs, L.npairs);
+    upload(S.d_pairs, S.pairs.data(), S.pairs.size() * sizeof(PairEntry));
+    upload(S.d_targets, S.targets.data(), S.targets.size() * sizeof(TargetEntry));
+    upload(S.d_kq, S.kq.data(), S.kq.size() * sizeof(double));
+}
+
+std::vector<PairItem> Context::applied_items(const std::vector<int>& nb) const
+{
+    std::vector<PairItem> v;
+    int idx = 0;
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nb[k]; ++j) v.push_back({k, idx++});
+    return v;
+}
+
+std::vector<PairItem> Context::occupied_items() const
+{
+    std::vector<PairItem> v;
+    for (int q = 0; q < nk; ++q)
+        for (int i = 0; i < nocc[q]; ++i) v.push_back({q, (int)occ_first[q] + i});
+    return v;
+}
+
 const Plan& Context::get_plan(const std::vector<int>& nb, int term)
 {
     long nt = 0;
@@ -334,48 +399,7 @@ const Plan& Context::get_plan(const std::vector<int>& nb, int term)
         P.app_coef[k + 1] = P.app_coef[k] + (long long)nb[k] * npw[k];
     }
     P.n_app = (int)P.app_first[nk];
-    struct Item {
-        int k, idx;
-    };
-    std::vector<Item> tg, src;  // targets (k, applied band), sources (q, occupied orbital)
-    for (int k = 0; k < nk; ++k)
-        for (int j = 0; j < nb[k]; ++j) tg.push_back({k, (int)P.app_first[k] + j});
-    for (int q = 0; q < nk; ++q)
-        for (int i = 0; i < nocc[q]; ++i) src.push_back({q, (int)occ_first[q] + i});
-    for (size_t t0 = 0; t0 < tg.size(); t0 += tc) {
-        const size_t t1 = std::min(tg.size(), t0 + (size_t)tc);
-        const int kf = tg[t0].k, kl = tg[t1 - 1].k;
-        for (size_t s0 = 0; s0 < src.size(); s0 += sc) {
-            const size_t s1 = std::min(src.size(), s0 + (size_t)sc);
-            const int qf = src[s0].k, ql = src[s1 - 1].k;
-            Launch L;
-            L.pair0 = (int)P.pairs.size();
-            L.target0 = (int)P.targets.size();
-            L.kq0 = (int)P.kq.size() / 3;
-            L.nkq = (kl - kf + 1) * (ql - qf + 1);
-            for (int k = kf; k <= kl; ++k)
-                for (int q = qf; q <= ql; ++q)
-                    for (int d = 0; d < 3; ++d) P.kq.push_back(kcart[3 * k + d] - kcart[3 * q + d]);
-            int local = 0;
-            for (size_t t = t0; t < t1; ++t) {
-                TargetEntry te{tg[t].idx, local, 0, 0};
-                for (size_t s = s0; s < s1; ++s) {
-                    const int kq = L.kq0 + (tg[t].k - kf) * (ql - qf + 1) + (src[s].k - qf);
-                    P.pairs.push_back({tg[t].idx, src[s].idx, kq, 0});
-                    ++local;
-                    ++te.npair;
-                }
-                P.targets.push_back(te);
-            }
-            L.npairs = local;
-            L.ntargets = (int)(t1 - t0);
-            P.launches.push_back(L);
-        }
-    }
-    for (const auto& L : P.launches) P.max_launch_pairs = std::max(P.max_launch_pairs, L.npairs);
-    upload(P.d_pairs, P.pairs.data(), P.pairs.size() * sizeof(PairEntry));
-    upload(P.d_targets, P.targets.data(), P.targets.size() * sizeof(TargetEntry));
-    upload(P.d_kq, P.kq.data(), P.kq.size() * sizeof(double));
+    build_pairs(P, applied_items(nb), occupied_items(), tc, sc, kcart, false);
     // one scatter/gather job and one divergence job per applied band, in band order
     std::vector<OrbEntry> appjobs, appgradjobs;
     std::vector<DivEntry> divjobs;
@@ -621,6 +645,39 @@ int tcx_tc3a1(tcx_context* ctx, const int* nb, const double* phi, double* h3phi,
 int tcx_tc3b1(tcx_context* ctx, const int* nb, const double* phi, double* h3phi, int precision, unsigned flags)
 {
     return guarded(ctx, [&] { ctx->impl.tc3_local(1, nb, phi, h3phi, precision, flags); });
+}
+
+int tcx_tc3a3b3b4b6(tcx_context* ctx, const int* nb, const double* phi, double* h3phi, int precision,
+                    unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.tc3a3b3b4b6(nb, phi, h3phi, precision, flags); });
+}
+
+int tcx_tc3a2a4b2b5(tcx_context* ctx, const int* nb, const double* phi, double* h3phi, int precision,
+                    unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.tc3a2a4b2b5(nb, phi, h3phi, precision, flags); });
+}
+
+int tcx_kinetic(tcx_context* ctx, const int* nb, const double* phi, double* h1phi, unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.kinetic(nb, phi, h1phi, flags); });
+}
+
+int tcx_hf2h(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.hf2h(nb, phi, h2phi, flags); });
+}
+
+int tcx_tc2h(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.tc2h(nb, phi, h2phi, flags); });
+}
+
+int tcx_apply_all(tcx_context* ctx, const int* nb, const double* phi, double* h1phi, double* h2phi, double* h3phi,
+                  int precision, unsigned flags)
+{
+    return guarded(ctx, [&] { ctx->impl.apply_all(nb, phi, h1phi, h2phi, h3phi, precision, flags); });
 }
 
 int tcx_device_alloc(tcx_context* ctx, unsigned long long bytes, void** ptr)

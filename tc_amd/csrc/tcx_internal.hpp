@@ -72,6 +72,15 @@ struct DivEntry {
     int kidx;             // k-point: divergence factor dc[kidx]
 };
 
+// One channel of a pair record: record = A * B with A, B grids of the working
+// precision; grid index of A = (sel ? pair.occ : pair.app) * mul + off.
+struct ProdChan {
+    const void* a;
+    const void* b;
+    int a_sel, a_mul, a_off, a_conj;
+    int b_sel, b_mul, b_off, b_conj;
+};
+
 // ------------------------------------------------------------- launchers
 // prec: TCX_PREC_FP64 (double grids) or TCX_PREC_FP32 (float grids).
 
@@ -126,5 +135,59 @@ void launch_tc3b1_V(const double* dnu, double* V, const Geom& g, const JastrowRP
                     hipStream_t s);
 void launch_sum_squares(const double* dnu, double* V, long long N, hipStream_t s);
 void launch_mul_shared(double* grids, int ngrids, const double* V, long long N, hipStream_t s);
+
+// three-body exchange building blocks (records of the working precision, FP64 accumulators)
+void launch_products(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
+                     long long N, hipStream_t s);
+void launch_acc_occ(int prec, const void* rec, int stride, int ch, const void* M, const PairEntry* pairs,
+                    const TargetEntry* targets, int ntargets, double* W, long long N, double factor, hipStream_t s);
+void launch_a7_comb1(int prec, const void* rec, void* Y, void* P, const PairEntry* pairs, int npairs,
+                     const double* kqcart, const double* fill, const Geom& g, long long N, const JastrowRPA& j0,
+                     const JastrowRPA& j1, double fb3, hipStream_t s);
+void launch_a7_mid(int prec, void* P, const void* occR, const PairEntry* pairs, const TargetEntry* targets,
+                   int ntargets, const double* dens, double* qsum, long long N, hipStream_t s);
+void launch_a7_comb2(int prec, void* Y, const void* P, const PairEntry* pairs, int npairs, const double* kqcart,
+                     const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1, double fb6,
+                     hipStream_t s);
+void launch_b4_comb(int prec, const void* rec, void* Z, const PairEntry* pairs, int npairs, const double* kqcart,
+                    const double* fill, const Geom& g, long long N, const JastrowRPA& j0, double fb4, hipStream_t s);
+void launch_mul_dnu(int prec, const void* appR, const double* dnu, void* out, int napp, long long N, hipStream_t s);
+void launch_a3(double* W, const double* qsum, const double* dnu, int napp, long long N, double fa3, hipStream_t s);
+void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, int nk, const Geom& g, long long N,
+               const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
+void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l, const DivEntry* jobs, int njobs,
+                    int maxocc, const double* fill, const double* t1, const double* appR, const double* chiR,
+                    const double* occR, const double* tR, double fb6, long long N, double* out, double* cR,
+                    double* W, hipStream_t s);
+void launch_a8_comb1(int prec, const void* rec, int nin, void* O, const PairEntry* pairs, int npairs,
+                     const double* kqcart, const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1,
+                     hipStream_t s);
+void launch_a8_acc1(int prec, const void* O, int bitc, const void* occR, const void* chiR, const PairEntry* pairs,
+                    const TargetEntry* targets, int ntargets, const double* fill, double* Q2, double* Q4, double* Q5,
+                    long long N, hipStream_t s);
+void launch_a8_div(double* Tg, const double* rec, const PairEntry* pairs, const TargetEntry* targets, int ntargets,
+                   const double* fill, const double* dc2, const Geom& g, long long N, const JastrowRPA& j0,
+                   const JastrowRPA& j1, hipStream_t s);
+void launch_a8_reduce(const double* Tg, const double* Q4, const double* Q5, int bitc, const double* occR,
+                      const double* fill, int nocc, double* V3a2, double* B2, long long N, hipStream_t s);
+void launch_a8_q45(int prec, const double* Q4, const double* Q5, int bitc, void* Q4T, void* Q5T, int ngrids,
+                   long long N, hipStream_t s);
+void launch_a8_comb2(int prec, void* rec, const PairEntry* pairs, int npairs, const double* kqcart,
+                     const double* fill, const Geom& g, long long N, const JastrowRPA& j0, double fa4, double fb5,
+                     hipStream_t s);
+void launch_a8_acc2(int prec, const void* rec, const void* occR, const void* Q4T, const PairEntry* pairs,
+                    const TargetEntry* targets, int ntargets, double* W, long long N, hipStream_t s);
+
+// assembly terms (FP64)
+void launch_kinetic(const double* phi, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
+                    const double* kcart, const Geom& g, double* out, hipStream_t s);
+void launch_hartree(const double* nG, double* V, const Geom& g, long long N, hipStream_t s);
+void launch_tc2h_V(const double* chiR, const double* gradR, const double* fill, int nocc, double* pgp,
+                   const Geom& g, long long N, hipStream_t s);
+void launch_tc2h_combine(const double* nG, const double* pgpG, double* V, const Geom& g, long long N,
+                         const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
+void launch_tc2h_apply(double* orb, const double* grad, const double* V2, const double* dnu, int nb, long long N,
+                       hipStream_t s);
+void launch_axpby(const double* X, const double* Y, double* out, double a, double b, long long N, hipStream_t s);
 
 }  // namespace tcx

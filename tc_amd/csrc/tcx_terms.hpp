@@ -47,15 +47,30 @@ struct Launch {
     int kq0 = 0, nkq = 0;           // rows of Plan::kq used by this launch
 };
 
-struct Plan {
-    std::vector<long long> app_first, app_coef;  // per k: first applied orbital, coefficient offset
-    int n_app = 0;
+// (target, source) pairs grouped into launches; TargetEntry::pad holds the
+// k-point of the target.
+struct PairSet {
     std::vector<PairEntry> pairs;
     std::vector<TargetEntry> targets;
     std::vector<double> kq;  // Cartesian k - q per (k,q) row
     std::vector<Launch> launches;
     int max_launch_pairs = 0;
     DBuf d_pairs, d_targets, d_kq;
+};
+
+// A grid of a pair set: k-point and grid index (applied band or occupied orbital).
+struct PairItem {
+    int k, idx;
+};
+
+// Every target paired with every source (same_k: only sources at the target's
+// k), in launches of tc consecutive targets x sc consecutive sources.
+void build_pairs(PairSet& S, const std::vector<PairItem>& tg, const std::vector<PairItem>& src, long tc, long sc,
+                 const std::vector<double>& kcart, bool same_k);
+
+struct Plan : PairSet {
+    std::vector<long long> app_first, app_coef;  // per k: first applied orbital, coefficient offset
+    int n_app = 0;
     DBuf d_appjobs;      // OrbEntry per applied band (band order)
     DBuf d_appgradjobs;  // three OrbEntry (dir 0..2) per applied band
     DBuf d_divjobs;      // DivEntry per applied band
@@ -119,6 +134,9 @@ struct Context {
     long shape_targets[2] = {0, 0};
     long shape_sources[2] = {0, 0};
     std::map<std::tuple<std::vector<int>, long, long>, Plan> plans;
+    // pair sets of the three-body exchange terms, keyed by (applied bands, kind)
+    std::map<std::tuple<std::vector<int>, int>, PairSet> pairsets;
+    DBuf d_qsum, d_qsumT, d_rec, d_Q2, d_Q4, d_Q5, d_Q4T, d_Q5T, d_appR64, d_chans, d_aux, d_local;
 
     // timing
     bool timing_on = false;
@@ -166,6 +184,19 @@ struct Context {
     void hf2x(const int* nb, const double* phi, double* out, int prec, unsigned flags);
     void tc2x(const int* nb, const double* phi, double* out, int prec, unsigned flags);
     void tc3_local(int which, const int* nb, const double* phi, double* out, int prec, unsigned flags);
+
+    // three-body exchange terms and the assembly terms
+    const PairSet& pair_set(const std::vector<int>& nb, int kind, int channels, int prec);
+    std::vector<PairItem> applied_items(const std::vector<int>& nb) const;
+    std::vector<PairItem> occupied_items() const;
+    void apply_local(const Plan& P, const ApplyIO& io, const double* V, double factor);
+    void density_G(double* nG);
+    void tc3a3b3b4b6(const int* nb, const double* phi, double* out, int prec, unsigned flags);
+    void tc3a2a4b2b5(const int* nb, const double* phi, double* out, int prec, unsigned flags);
+    void kinetic(const int* nb, const double* phi, double* out, unsigned flags);
+    void hf2h(const int* nb, const double* phi, double* out, unsigned flags);
+    void tc2h(const int* nb, const double* phi, double* out, unsigned flags);
+    void apply_all(const int* nb, const double* phi, double* h1, double* h2, double* h3, int prec, unsigned flags);
 };
 
 }  // namespace tcx

@@ -37,7 +37,8 @@ EXPORTS = [
     "tcx_create", "tcx_destroy", "tcx_last_error", "tcx_synchronize",
     "tcx_set_cell", "tcx_set_kpoints", "tcx_set_basis", "tcx_set_states",
     "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux",
-    "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1",
+    "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
+    "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
     "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
     "tcx_set_launch_shape", "tcx_enable_timing", "tcx_get_timing",
 ]
@@ -76,8 +77,11 @@ def load_library():
     L.tcx_set_jastrow_rpa.argtypes = [vp, _dp, _dp]
     L.tcx_set_div_correction.argtypes = [vp, C.c_int, C.c_double, _dp]
     L.tcx_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
-    for name in ("tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1"):
+    for name in ("tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5"):
         getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_int, C.c_uint]
+    for name in ("tcx_kinetic", "tcx_hf2h", "tcx_tc2h"):
+        getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_uint]
+    L.tcx_apply_all.argtypes = [vp, _ip, vp, vp, vp, vp, C.c_int, C.c_uint]
     L.tcx_device_alloc.argtypes = [vp, C.c_ulonglong, C.POINTER(vp)]
     L.tcx_device_free.argtypes = [vp, vp]
     L.tcx_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong, C.c_int]
@@ -119,6 +123,7 @@ class Engine:
         self.h = h
         self._keep = []
         self.sys = None
+        self.method = "HF"
 
     def close(self):
         if getattr(self, "h", None):
@@ -141,6 +146,7 @@ class Engine:
         r = np.ascontiguousarray(recip, dtype=np.float64).reshape(9)
         g = np.ascontiguousarray(grid, dtype=np.int32)
         m = METHODS[method] if isinstance(method, str) else int(method)
+        self.method = {v: k for k, v in METHODS.items()}[m]
         self._check(self.L.tcx_set_cell(self.h, _ptr(r), float(volume), _ptr(g, _ip), int(nspin), m), "set_cell")
 
     def set_kpoints(self, kfrac, num_kpoints=None):
@@ -215,6 +221,41 @@ class Engine:
 
     def tc3b1(self, nb, phi, out=None, precision=PREC_FP64):
         return self._term("tc3b1", nb, phi, out, precision)
+
+    def tc3a3b3b4b6(self, nb, phi, out=None, precision=PREC_FP64):
+        return self._term("tc3a3b3b4b6", nb, phi, out, precision)
+
+    def tc3a2a4b2b5(self, nb, phi, out=None, precision=PREC_FP64):
+        return self._term("tc3a2a4b2b5", nb, phi, out, precision)
+
+    def _term_fp64(self, name, nb, phi, out=None):
+        nb = np.ascontiguousarray(nb, dtype=np.int32)
+        phi = np.ascontiguousarray(phi, dtype=np.complex128)
+        if out is None:
+            out = np.zeros_like(phi)
+        st = getattr(self.L, "tcx_" + name)(self.h, _ptr(nb, _ip), phi.ctypes.data, out.ctypes.data, 0)
+        self._check(st, name)
+        return out
+
+    def kinetic(self, nb, phi, out=None):
+        return self._term_fp64("kinetic", nb, phi, out)
+
+    def hf2h(self, nb, phi, out=None):
+        return self._term_fp64("hf2h", nb, phi, out)
+
+    def tc2h(self, nb, phi, out=None):
+        return self._term_fp64("tc2h", nb, phi, out)
+
+    def apply_all(self, nb, phi, precision=PREC_FP64, method=None):
+        """H1 phi, H2 phi, H3 phi (H3 is None for HF), each zeroed first (calc_hamiltonian::all)."""
+        nb = np.ascontiguousarray(nb, dtype=np.int32)
+        phi = np.ascontiguousarray(phi, dtype=np.complex128)
+        h1, h2 = np.empty_like(phi), np.empty_like(phi)
+        h3 = np.empty_like(phi) if (method or self.method) != "HF" else None
+        st = self.L.tcx_apply_all(self.h, _ptr(nb, _ip), phi.ctypes.data, h1.ctypes.data, h2.ctypes.data,
+                                  h3.ctypes.data if h3 is not None else None, int(precision), 0)
+        self._check(st, "apply_all")
+        return h1, h2, h3
 
     # --------------------------------------------------------------- tuning / timing
     def set_launch_shape(self, term, targets=0, sources=0):
