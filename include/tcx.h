@@ -158,6 +158,35 @@ int tcx_tc2h(tcx_context *ctx, const int *nb, const double *phi, double *h2phi, 
 int tcx_apply_all(tcx_context *ctx, const int *nb, const double *phi, double *h1phi,
                   double *h2phi, double *h3phi, int precision, unsigned flags);
 
+/* ---------------------------------------------------------------- consumer */
+/* Block Davidson at fixed occupied orbitals (one Fock operator): replaces
+ * Diagonalization::block_davidson (src/include/diagonalization.hpp; src/diagonalization_davidson.cpp:7-689)
+ * with make_a_new_trial_vector, Gram_Schmidt and sort_eigen
+ * (src/diagonalization_utils.cpp:7-29, 79-118, 182-210).  H*phi is tcx_apply_all on
+ * the GPU in `precision`; the subspace algebra is FP64 on the host.  Host pointers:
+ *   phi      in/out complex128 ragged [k][nb[k]][npw_k]: trial bands in, eigenvectors out;
+ *   phi_left out (BITC only, else may be NULL): left eigenvectors V (X^-1)^H;
+ *   eig      in/out complex128 [sum nb]: residual shifts in (the previous eigenvalues),
+ *            eigenvalues out, ascending real part;
+ *   energies out complex128 [sum nb][3]: e_n = diag(X_L^H H_n X), n = 1, 2, 3.
+ * max_blocks >= 2 and num_refresh >= 1 are the reference's max_num_blocks_david and
+ * num_refresh_david (defaults 2 and 1); biortho must be 0 (biortho_david = false). */
+int tcx_davidson(tcx_context *ctx, const int *nb, double *phi, double *phi_left, double *eig,
+                 double *energies, int max_blocks, int num_refresh, int biortho, int precision);
+
+/* Total energies from per-band e_n: replaces TotalEnergy::calc_total_energy
+ * (src/total_energy.cpp:98-163; identity symmetry, no Ewald term):
+ * out complex128 [4] = E_1 = sum f e_1 / Nk, E_2 = sum f e_2 / (2 Nk),
+ * E_3 = sum f e_3 / (3 Nk) (0 unless uses_3body), and E_1 + E_2 + E_3.  No GPU. */
+int tcx_total_energy(int nk, const int *nb, const double *filling, const double *energies,
+                     int num_kpoints, int uses_3body, double *out);
+
+/* Dense complex eigen-decomposition A X = X diag(w), n <= 4096: replaces
+ * Eigen::ComplexEigenSolver in the Davidson subspace step
+ * (src/diagonalization_davidson.cpp:480-482).  A, X complex128 n x n row-major;
+ * unit-norm eigenvector columns; eigenvalues unsorted.  No GPU. */
+int tcx_eig_dense(int n, const double *A, double *w, double *X);
+
 /* ---------------------------------------------------------------- device buffers */
 /* No counterpart in the reference (which keeps everything in host memory).
  * For callers that hold no HIP runtime of their own and want inputs and

@@ -33,6 +33,7 @@ SOURCES = [
     "csrc/tcx_api.cpp",
     "csrc/tcx_tc.cpp",
     "csrc/tcx_tc3x.cpp",
+    "csrc/tcx_davidson.cpp",
     "host/calc_hamiltonian.cpp",
 ]
 ROOT = os.path.dirname(HERE)

@@ -147,6 +147,7 @@ void Context::set_basis(const int* npw_, const int* gindex)
     for (long long l = 0; l < pw_off[nk]; ++l)
         require(gindex[l] >= 0 && gindex[l] < N, TCX_ERR_ARG, "set_basis: gindex outside the FFT grid");
     max_npw = *std::max_element(npw.begin(), npw.end());
+    gindex_h.assign(gindex, gindex + pw_off[nk]);
     d_gindex.ensure(pw_off[nk] * sizeof(int));
     TCX_HIP_CHECK(hipMemcpy(d_gindex.p, gindex, pw_off[nk] * sizeof(int), hipMemcpyHostToDevice));
     basis_set = true;
@@ -161,6 +162,7 @@ void Context::invalidate_states()
     gradR_valid[0] = gradR_valid[1] = false;
     density_valid = false;
     plans.clear();
+    pairsets.clear();  // pair lists index the occupied orbitals and k-points
 }
 
 void Context::set_states(const int* nocc_, const double* filling, const double* phi_occ, const double* chi_occ)
@@ -678,6 +680,50 @@ int tcx_apply_all(tcx_context* ctx, const int* nb, const double* phi, double* h1
                   int precision, unsigned flags)
 {
     return guarded(ctx, [&] { ctx->impl.apply_all(nb, phi, h1phi, h2phi, h3phi, precision, flags); });
+}
+
+int tcx_davidson(tcx_context* ctx, const int* nb, double* phi, double* phi_left, double* eig, double* energies,
+                 int max_blocks, int num_refresh, int biortho, int precision)
+{
+    return guarded(ctx, [&] {
+        ctx->impl.davidson(nb, phi, phi_left, eig, energies, max_blocks, num_refresh, biortho, precision);
+    });
+}
+
+int tcx_total_energy(int nk, const int* nb, const double* filling, const double* energies, int num_kpoints,
+                     int uses_3body, double* out)
+{
+    if (nk < 0 || !nb || !filling || !energies || !out || num_kpoints <= 0) return TCX_ERR_ARG;
+    // TotalEnergy::calc_total_energy (src/total_energy.cpp:98-163), identity symmetry, no Ewald term
+    double e[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    long long b = 0;
+    for (int k = 0; k < nk; ++k) {
+        if (nb[k] < 0) return TCX_ERR_ARG;
+        for (int j = 0; j < nb[k]; ++j, ++b) {
+            const double f = filling[b];
+            for (int n = 0; n < 3; ++n) {
+                if (n == 2 && !uses_3body) continue;
+                e[n][0] += f * energies[(b * 3 + n) * 2] / (n + 1);
+                e[n][1] += f * energies[(b * 3 + n) * 2 + 1] / (n + 1);
+            }
+        }
+    }
+    for (int n = 0; n < 3; ++n) {
+        out[2 * n] = e[n][0] / num_kpoints;
+        out[2 * n + 1] = e[n][1] / num_kpoints;
+    }
+    out[6] = out[0] + out[2] + out[4];
+    out[7] = out[1] + out[3] + out[5];
+    return TCX_OK;
+}
+
+int tcx_eig_dense(int n, const double* A, double* w, double* X)
+{
+    if (n < 0 || n > 4096 || (n > 0 && (!A || !w || !X))) return TCX_ERR_ARG;
+    using cd = std::complex<double>;
+    return eig_dense(n, reinterpret_cast<const cd*>(A), reinterpret_cast<cd*>(w), reinterpret_cast<cd*>(X))
+               ? TCX_OK
+               : TCX_ERR_INTERNAL;
 }
 
 int tcx_device_alloc(tcx_context* ctx, unsigned long long bytes, void** ptr)

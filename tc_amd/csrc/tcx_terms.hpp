@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <complex>
 #include <functional>
 #include <map>
 #include <string>
@@ -108,6 +109,7 @@ struct Context {
     std::vector<double> kfrac, kcart;
     std::vector<int> npw;
     std::vector<long long> pw_off;
+    std::vector<int> gindex_h;  // host copy of the plane-wave grid indices
     int max_npw = 0;
     std::vector<int> nocc;
     std::vector<long long> occ_first, occ_coef;
@@ -197,7 +199,13 @@ struct Context {
     void hf2h(const int* nb, const double* phi, double* out, unsigned flags);
     void tc2h(const int* nb, const double* phi, double* out, unsigned flags);
     void apply_all(const int* nb, const double* phi, double* h1, double* h2, double* h3, int prec, unsigned flags);
+    void davidson(const int* nb, double* phi, double* phi_left, double* eig, double* energies, int max_blocks,
+                  int num_refresh, int biortho, int prec);
 };
+
+// Dense complex eigen-decomposition A X = X diag(w), A and X n x n row-major,
+// unit-norm eigenvector columns.  Returns false if the QR iteration does not converge.
+bool eig_dense(int n, const std::complex<double>* A, std::complex<double>* w, std::complex<double>* X);
 
 }  // namespace tcx
 

@@ -39,6 +39,7 @@ EXPORTS = [
     "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux",
     "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
+    "tcx_davidson", "tcx_total_energy", "tcx_eig_dense",
     "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
     "tcx_set_launch_shape", "tcx_enable_timing", "tcx_get_timing",
 ]
@@ -82,6 +83,9 @@ def load_library():
     for name in ("tcx_kinetic", "tcx_hf2h", "tcx_tc2h"):
         getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_uint]
     L.tcx_apply_all.argtypes = [vp, _ip, vp, vp, vp, vp, C.c_int, C.c_uint]
+    L.tcx_davidson.argtypes = [vp, _ip, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.tcx_total_energy.argtypes = [C.c_int, _ip, _dp, vp, C.c_int, C.c_int, vp]
+    L.tcx_eig_dense.argtypes = [C.c_int, vp, vp, vp]
     L.tcx_device_alloc.argtypes = [vp, C.c_ulonglong, C.POINTER(vp)]
     L.tcx_device_free.argtypes = [vp, vp]
     L.tcx_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong, C.c_int]
@@ -95,6 +99,31 @@ def load_library():
 
 def _ptr(a, t=_dp):
     return a.ctypes.data_as(t)
+
+
+def eig_dense(A):
+    """Eigenvalues and unit-norm right eigenvectors (columns) of a small complex matrix."""
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    n = A.shape[0]
+    w = np.zeros(n, np.complex128)
+    X = np.zeros((n, n), np.complex128)
+    st = load_library().tcx_eig_dense(n, A.ctypes.data, w.ctypes.data, X.ctypes.data)
+    if st != TCX_OK:
+        raise TcxError(st, "eig_dense failed")
+    return w, X
+
+
+def total_energy(nb, filling, energies, num_kpoints, uses_3body):
+    """(E_1, E_2, E_3, E_total) from per-band e_n (TotalEnergy::calc_total_energy)."""
+    nb = np.ascontiguousarray(nb, dtype=np.int32)
+    f = np.ascontiguousarray(filling, dtype=np.float64)
+    e = np.ascontiguousarray(energies, dtype=np.complex128)
+    out = np.zeros(4, np.complex128)
+    st = load_library().tcx_total_energy(len(nb), _ptr(nb, _ip), _ptr(f), e.ctypes.data, int(num_kpoints),
+                                         int(bool(uses_3body)), out.ctypes.data)
+    if st != TCX_OK:
+        raise TcxError(st, "total_energy failed")
+    return out
 
 
 def sum_of_vaux(recip, volume, kfrac):
@@ -245,6 +274,19 @@ class Engine:
 
     def tc2h(self, nb, phi, out=None):
         return self._term_fp64("tc2h", nb, phi, out)
+
+    def davidson(self, nb, phi, eig, max_blocks=2, num_refresh=1, precision=PREC_FP64):
+        """One Diagonalization::block_davidson call: returns (phi, phi_left or None, eig, energies[:,3])."""
+        nb = np.ascontiguousarray(nb, dtype=np.int32)
+        phi = np.array(phi, dtype=np.complex128, copy=True)
+        eig = np.array(eig, dtype=np.complex128, copy=True)
+        left = np.zeros_like(phi) if self.method == "BITC" else None
+        en = np.zeros((int(nb.sum()), 3), np.complex128)
+        st = self.L.tcx_davidson(self.h, _ptr(nb, _ip), phi.ctypes.data, left.ctypes.data if left is not None else None,
+                                 eig.ctypes.data, en.ctypes.data, int(max_blocks), int(num_refresh), 0,
+                                 int(precision))
+        self._check(st, "davidson")
+        return phi, left, eig, en
 
     def apply_all(self, nb, phi, precision=PREC_FP64, method=None):
         """H1 phi, H2 phi, H3 phi (H3 is None for HF), each zeroed first (calc_hamiltonian::all)."""

@@ -131,3 +131,15 @@ def test_three_body_exchange_device_pointers(engine):
     engine._term_device("tc3a2a4b2b5", nb, d_phi.ptr, d_out.ptr, tcx.PREC_FP64)
     got = d_out.download(np.complex128)
     assert _close(got, ref, 1e-13)
+
+
+def test_pair_lists_follow_state_changes(engine):
+    """Two systems with the same applied-band counts but different occupied sets
+    in a row: the cached three-body pair lists must be rebuilt for the second."""
+    for nocc, cfg in ((2, 51), (3, 52)):
+        sys_ = S.tiny_system(method="TC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=nocc, config=cfg)
+        engine.load_system(sys_)
+        nb, phi = S.applied_bands(sys_, nb=[2, 2], seed=53)
+        o = O.Oracle(sys_)
+        for term in THREE_BODY_X:
+            assert _close(getattr(engine, term)(nb, phi), getattr(o, term)(nb, phi)), (nocc, term)
