@@ -181,6 +181,17 @@ int tcx_davidson(tcx_context *ctx, const int *nb, double *phi, double *phi_left,
 int tcx_total_energy(int nk, const int *nb, const double *filling, const double *energies,
                      int num_kpoints, int uses_3body, double *out);
 
+/* Band fillings and Fermi energy: replaces BlochStates::set_filling and
+ * Kpoints::return_band_filling (src/bloch_states_scfloop_filling_density.cpp:9-194,
+ * src/kpoints.cpp:349-387) for one independent spin.  eig: real parts of the
+ * eigenvalues [sum nb]; kweight[nk] as Kpoints::kweight_scf (2/Nk per k with identity
+ * symmetry, 0 for zero-weight k); smearing 0 = fixed (lowest round(N_e/2) bands get 2),
+ * 1 = gaussian of the given width (f = 1 - erf((e - mu)/width), mu by 60 bisections
+ * of [-1e5, 1e5] Ha).  Out: filling [sum nb], *fermi_energy, num_occupied[nk] (f > 1e-8).
+ * Returns TCX_ERR_ARG when the electron count cannot be met.  No GPU. */
+int tcx_set_filling(int nk, const int *nb, const double *eig, const double *kweight, double num_electrons,
+                    int smearing, double width, double *filling, double *fermi_energy, int *num_occupied);
+
 /* Dense complex eigen-decomposition A X = X diag(w), n <= 4096: replaces
  * Eigen::ComplexEigenSolver in the Davidson subspace step
  * (src/diagonalization_davidson.cpp:480-482).  A, X complex128 n x n row-major;

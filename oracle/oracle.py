@@ -204,3 +204,48 @@ class Oracle:
 
     def tc_2body(self, G2, s1=0, s2=0):
         return lib().orc_tc_2body(C.byref(self.P), float(G2), s1, s2)
+
+
+def fillings(nb, eig, kweight, num_electrons, smearing="fixed", width=0.01):
+    """Band fillings and Fermi energy, one independent spin: BlochStates::set_filling
+    (src/bloch_states_scfloop_filling_density.cpp:35-194) with Kpoints::return_band_filling
+    (src/kpoints.cpp:349-387).  Returns (filling, fermi_energy, num_occupied) or raises
+    ValueError where the reference stops."""
+    import math
+    nb = np.asarray(nb, dtype=int)
+    eig = np.real(np.asarray(eig, dtype=np.complex128))
+    off = np.concatenate([[0], np.cumsum(nb)])
+    f = np.zeros(int(off[-1]))
+    spin = 2.0
+    if smearing == "fixed":                                   # :37-70
+        nocc = int(round(num_electrons / 2.0))
+        if abs(nocc - num_electrons / 2.0) > 1e-5:
+            raise ValueError("fixed smearing needs an even electron count")
+        mu = -1e5
+        for k in range(len(nb)):
+            if nocc > nb[k]:
+                raise ValueError("num_bands_tc too small")
+            if kweight[k] < 1e-8:
+                continue
+            for b in range(nocc):
+                f[off[k] + b] = spin
+                mu = max(mu, eig[off[k] + b])
+    else:                                                     # :91-130
+        lo, hi = -1e5, 1e5
+        for _ in range(60):
+            mu = 0.5 * (hi + lo)
+            ne = 0.0
+            for k in range(len(nb)):
+                for b in range(nb[k]):
+                    v = 0.0 if kweight[k] < 1e-8 else spin * 0.5 * (1.0 - math.erf((eig[off[k] + b] - mu) / width))
+                    f[off[k] + b] = v
+                    ne += v * kweight[k] / spin
+            if ne > num_electrons:
+                hi = mu
+            else:
+                lo = mu
+    ne = sum(f[off[k]:off[k + 1]].sum() * kweight[k] / spin for k in range(len(nb)))
+    if abs(ne - num_electrons) > 1e-8:                        # :174
+        raise ValueError("set Fermi energy failed")
+    nocc_k = np.array([int((f[off[k]:off[k + 1]] > 1e-8).sum()) for k in range(len(nb))])  # :182-194
+    return f, mu, nocc_k

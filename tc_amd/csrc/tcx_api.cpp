@@ -131,6 +131,12 @@ void Context::set_kpoints(int nk_, const double* kf, int numk)
     basis_set = false;
     states_set = false;
     div_enabled = false;
+    // no occupied orbitals until tcx_set_states (the one-body term needs none)
+    nocc.assign(nk, 0);
+    occ_first.assign(nk + 1, 0);
+    occ_coef.assign(nk + 1, 0);
+    n_occ = 0;
+    fill.clear();
     invalidate_states();
 }
 
@@ -715,6 +721,63 @@ int tcx_total_energy(int nk, const int* nb, const double* filling, const double*
     out[6] = out[0] + out[2] + out[4];
     out[7] = out[1] + out[3] + out[5];
     return TCX_OK;
+}
+
+int tcx_set_filling(int nk, const int* nb, const double* eig, const double* kweight, double num_electrons,
+                    int smearing, double width, double* filling, double* fermi_energy, int* num_occupied)
+{
+    // BlochStates::set_filling + Kpoints::return_band_filling
+    // (bloch_states_scfloop_filling_density.cpp:9-194, kpoints.cpp:349-387), one independent spin
+    if (nk <= 0 || !nb || !eig || !kweight || !filling || !fermi_energy || !num_occupied) return TCX_ERR_ARG;
+    if (smearing != 0 && smearing != 1) return TCX_ERR_ARG;
+    if (smearing == 1 && !(width > 0.0)) return TCX_ERR_ARG;
+    const double spin_factor = 2.0;
+    std::vector<long long> off(nk + 1, 0);
+    for (int k = 0; k < nk; ++k) {
+        if (nb[k] < 0) return TCX_ERR_ARG;
+        off[k + 1] = off[k] + nb[k];
+    }
+    double mu = 0.0;
+    if (smearing == 0) {  // fixed: the lowest round(N_e / 2) bands are doubly occupied
+        const int nocc = (int)std::lround(num_electrons / 2.0);
+        if (std::fabs(nocc - num_electrons / 2.0) > 1e-5) return TCX_ERR_ARG;
+        mu = -1e5;
+        for (int k = 0; k < nk; ++k) {
+            if (nocc > nb[k]) return TCX_ERR_ARG;  // num_bands_tc too small
+            for (int b = 0; b < nb[k]; ++b) {
+                const bool occ = kweight[k] >= 1e-8 && b < nocc;
+                filling[off[k] + b] = occ ? spin_factor : 0.0;
+                if (occ) mu = std::max(mu, eig[off[k] + b]);
+            }
+        }
+    } else {  // gaussian: bisection for the Fermi energy, 60 halvings of [-1e5, 1e5]
+        double lo = -1e5, hi = 1e5;
+        for (int it = 0; it < 60; ++it) {
+            mu = 0.5 * (hi + lo);
+            double ne = 0.0;
+            for (int k = 0; k < nk; ++k)
+                for (int b = 0; b < nb[k]; ++b) {
+                    double f = 0.0;
+                    if (kweight[k] >= 1e-8) f = spin_factor * 0.5 * (1.0 - std::erf((eig[off[k] + b] - mu) / width));
+                    filling[off[k] + b] = f;
+                    ne += f * (kweight[k] / spin_factor);
+                }
+            if (ne > num_electrons)
+                hi = mu;
+            else
+                lo = mu;
+        }
+    }
+    double ne = 0.0;
+    for (int k = 0; k < nk; ++k) {
+        num_occupied[k] = 0;
+        for (int b = 0; b < nb[k]; ++b) {
+            ne += filling[off[k] + b] * (kweight[k] / spin_factor);
+            if (filling[off[k] + b] > 1e-8) ++num_occupied[k];
+        }
+    }
+    *fermi_energy = mu;
+    return std::fabs(ne - num_electrons) > 1e-8 ? TCX_ERR_ARG : TCX_OK;  // "set Fermi energy failed"
 }
 
 int tcx_eig_dense(int n, const double* A, double* w, double* X)

@@ -394,7 +394,10 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
 // ------------------------------------------------------------------ assembly terms
 void Context::kinetic(const int* nb_, const double* phi, double* out, unsigned flags)
 {
-    check_apply_args(nb_, phi, out, TCX_PREC_FP64);
+    // reads no orbital state: the cell, k-points and basis suffice
+    require(basis_set, TCX_ERR_STATE, "kinetic: call tcx_set_basis first");
+    require(nb_ && phi && out, TCX_ERR_ARG, "kinetic: null pointer");
+    for (int k = 0; k < nk; ++k) require(nb_[k] >= 0, TCX_ERR_ARG, "kinetic: negative band count");
     std::vector<int> nb(nb_, nb_ + nk);
     const Plan& P = get_plan(nb, 0);
     ApplyIO io = begin_apply(nb_, phi, out, flags, P);

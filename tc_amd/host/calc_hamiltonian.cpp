@@ -49,8 +49,8 @@ void pack_bands(const std::vector<std::vector<std::vector<VectorXcd>>>& bands_of
         }
 }
 
-void push_state(tcx_context* ctx, const Method& method, const CrystalStructure& cs, const Potentials& pot,
-                const Spin& spin, const Kpoints& kp, const PlaneWaveBasis& pw, const BlochStates& bs)
+void push_geometry(tcx_context* ctx, const Method& method, const CrystalStructure& cs, const Spin& spin,
+                   const Kpoints& kp, const PlaneWaveBasis& pw)
 {
     if (spin.num_independent_spins != 1) throw std::runtime_error("calc_hamiltonian: only one independent spin");
     if (method.calc_mode != "SCF") throw std::runtime_error("calc_hamiltonian: only SCF mode");
@@ -69,6 +69,13 @@ void push_state(tcx_context* ctx, const Method& method, const CrystalStructure& 
         gidx.insert(gidx.end(), pw.Gindex_at_k_scf[k].begin(), pw.Gindex_at_k_scf[k].end());
     }
     check(ctx, tcx_set_basis(ctx, npw.data(), gidx.data()), "set_basis");
+}
+
+void push_state(tcx_context* ctx, const Method& method, const CrystalStructure& cs, const Potentials& pot,
+                const Spin& spin, const Kpoints& kp, const PlaneWaveBasis& pw, const BlochStates& bs)
+{
+    push_geometry(ctx, method, cs, spin, kp, pw);
+    const int nk = (int)kp.kvectors_scf.size();
     // occupied orbitals: bands < num_occupied_bands (BlochStates::set_filling, f > 1e-8)
     std::vector<int> nocc(bs.num_occupied_bands[0].begin(), bs.num_occupied_bands[0].end());
     std::vector<double> fill, phi_occ, chi_occ;
@@ -115,9 +122,105 @@ void run_term(TermFn fn, const char* name, const Parallelization& par, const Met
         }
 }
 
+int hf2h_term(tcx_context* c, const int* nb, const double* phi, double* out, int, unsigned f)
+{
+    return tcx_hf2h(c, nb, phi, out, f);
+}
+int tc2h_term(tcx_context* c, const int* nb, const double* phi, double* out, int, unsigned f)
+{
+    return tcx_tc2h(c, nb, phi, out, f);
+}
+
+void unpack_bands(const std::vector<double>& buf, Phi4& H)
+{
+    size_t off = 0;
+    for (auto& k : H[0])
+        for (auto& b : k) {
+            VectorXcd& v = b[0];
+            std::memcpy(reinterpret_cast<double*>(v.data()), buf.data() + off, sizeof(double) * 2 * v.size());
+            off += 2 * v.size();
+        }
+}
+
 }  // namespace
 
 namespace calc_hamiltonian {
+
+void all(const Parallelization& par, MyClock&, const Method& m, const CrystalStructure& cs, const Symmetry&,
+         const Potentials& pot, const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs,
+         const Phi4& phi, Phi4& H1phi, Phi4& H2phi, Phi4& H3phi, std::ostream*)
+{
+    tcx_context* ctx = context_for(par.device);
+    push_state(ctx, m, cs, pot, spin, kp, pw, bs);
+    const int nk = (int)kp.kvectors_scf.size();
+    std::vector<int> nb(nk);
+    for (int k = 0; k < nk; ++k) nb[k] = (int)phi[0][k].size();
+    const bool tc = m.calc_method != "HF";
+    std::vector<double> in, h1, h2, h3;
+    pack_bands(phi[0], nb, in);
+    h1.assign(in.size(), 0.0);
+    h2.assign(in.size(), 0.0);
+    if (tc) h3.assign(in.size(), 0.0);
+    check(ctx,
+          tcx_apply_all(ctx, nb.data(), in.data(), h1.data(), h2.data(), tc ? h3.data() : nullptr, m.precision, 0u),
+          "all");
+    // H*phi arrays take the shape of phi (calc_hamiltonian_all.cpp:28-49)
+    for (Phi4* H : {&H1phi, &H2phi, &H3phi}) {
+        if (H == &H3phi && !tc) continue;
+        H->assign(1, std::vector<std::vector<std::vector<VectorXcd>>>(nk));
+        for (int k = 0; k < nk; ++k) {
+            (*H)[0][k].resize(nb[k]);
+            for (int b = 0; b < nb[k]; ++b) (*H)[0][k][b].assign(1, VectorXcd(phi[0][k][b][0].size()));
+        }
+    }
+    unpack_bands(h1, H1phi);
+    unpack_bands(h2, H2phi);
+    if (tc) unpack_bands(h3, H3phi);
+}
+
+void kinetic(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Spin& spin,
+             const Kpoints& kp, PlaneWaveBasis& pw, const Phi4& phi, Phi4& H1phi, std::ostream*)
+{
+    // the one-body term reads no orbital state: only the cell, k-points and basis are pushed
+    tcx_context* ctx = context_for(par.device);
+    push_geometry(ctx, m, cs, spin, kp, pw);
+    const int nk = (int)kp.kvectors_scf.size();
+    std::vector<int> nb(nk);
+    for (int k = 0; k < nk; ++k) nb[k] = (int)phi[0][k].size();
+    std::vector<double> in, out;
+    pack_bands(phi[0], nb, in);
+    pack_bands(H1phi[0], nb, out);
+    check(ctx, tcx_kinetic(ctx, nb.data(), in.data(), out.data(), 0u), "kinetic");
+    unpack_bands(out, H1phi);
+}
+
+void hf2h(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,
+          const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi,
+          Phi4& H2phi, std::ostream*)
+{
+    run_term(hf2h_term, "hf2h", par, m, cs, pot, spin, kp, pw, bs, phi, H2phi);
+}
+
+void tc2h(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,
+          const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi,
+          Phi4& H2phi, std::ostream*)
+{
+    run_term(tc2h_term, "tc2h", par, m, cs, pot, spin, kp, pw, bs, phi, H2phi);
+}
+
+void tc3a3b3b4b6(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,
+                 const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi,
+                 Phi4& H3phi, std::ostream*)
+{
+    run_term(tcx_tc3a3b3b4b6, "tc3a3b3b4b6", par, m, cs, pot, spin, kp, pw, bs, phi, H3phi);
+}
+
+void tc3a2a4b2b5(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Symmetry&,
+                 const Potentials& pot, const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw,
+                 const BlochStates& bs, const Phi4& phi, Phi4& H3phi, std::ostream*)
+{
+    run_term(tcx_tc3a2a4b2b5, "tc3a2a4b2b5", par, m, cs, pot, spin, kp, pw, bs, phi, H3phi);
+}
 
 void hf2x(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,
           const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi,

@@ -78,7 +78,27 @@ struct BlochStates {
     std::vector<std::vector<int>> num_occupied_bands;        // [spin][k]
 };
 
+struct Symmetry {};  // identity only: the one rotation of an identity-symmetry k-mesh
+struct MyClock {};   // timing printouts of the reference are not mirrored
+
 namespace calc_hamiltonian {
+
+// src/include/calc_hamiltonian.hpp:10-21: zeroes H1/H2/H3 phi, then every term
+void all(const Parallelization&, MyClock&, const Method&, const CrystalStructure&, const Symmetry&,
+         const Potentials&, const Spin&, const Kpoints&, PlaneWaveBasis&, const BlochStates&, const Phi4& phi,
+         Phi4& H1phi, Phi4& H2phi, Phi4& H3phi, std::ostream* ost);
+void kinetic(const Parallelization&, const Method&, const CrystalStructure&, const Spin&, const Kpoints&,
+             PlaneWaveBasis&, const Phi4& phi, Phi4& H1phi, std::ostream* ost);
+void hf2h(const Parallelization&, const Method&, const CrystalStructure&, const Potentials&, const Spin&,
+          const Kpoints&, PlaneWaveBasis&, const BlochStates&, const Phi4& phi, Phi4& H2phi, std::ostream* ost);
+void tc2h(const Parallelization&, const Method&, const CrystalStructure&, const Potentials&, const Spin&,
+          const Kpoints&, PlaneWaveBasis&, const BlochStates&, const Phi4& phi, Phi4& H2phi, std::ostream* ost);
+void tc3a3b3b4b6(const Parallelization&, const Method&, const CrystalStructure&, const Potentials&, const Spin&,
+                 const Kpoints&, PlaneWaveBasis&, const BlochStates&, const Phi4& phi, Phi4& H3phi,
+                 std::ostream* ost);
+void tc3a2a4b2b5(const Parallelization&, const Method&, const CrystalStructure&, const Symmetry&,
+                 const Potentials&, const Spin&, const Kpoints&, PlaneWaveBasis&, const BlochStates&,
+                 const Phi4& phi, Phi4& H3phi, std::ostream* ost);
 
 void hf2x(const Parallelization&, const Method&, const CrystalStructure&, const Potentials&, const Spin&,
           const Kpoints&, PlaneWaveBasis&, const BlochStates&, const Phi4& phi, Phi4& H2phi, std::ostream* ost);

@@ -39,7 +39,7 @@ EXPORTS = [
     "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux",
     "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
-    "tcx_davidson", "tcx_total_energy", "tcx_eig_dense",
+    "tcx_davidson", "tcx_total_energy", "tcx_eig_dense", "tcx_set_filling",
     "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
     "tcx_set_launch_shape", "tcx_enable_timing", "tcx_get_timing",
 ]
@@ -86,6 +86,7 @@ def load_library():
     L.tcx_davidson.argtypes = [vp, _ip, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.tcx_total_energy.argtypes = [C.c_int, _ip, _dp, vp, C.c_int, C.c_int, vp]
     L.tcx_eig_dense.argtypes = [C.c_int, vp, vp, vp]
+    L.tcx_set_filling.argtypes = [C.c_int, _ip, _dp, _dp, C.c_double, C.c_int, C.c_double, _dp, _dp, _ip]
     L.tcx_device_alloc.argtypes = [vp, C.c_ulonglong, C.POINTER(vp)]
     L.tcx_device_free.argtypes = [vp, vp]
     L.tcx_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong, C.c_int]
@@ -111,6 +112,22 @@ def eig_dense(A):
     if st != TCX_OK:
         raise TcxError(st, "eig_dense failed")
     return w, X
+
+
+def set_filling(nb, eig, kweight, num_electrons, smearing="fixed", width=0.01):
+    """(filling, fermi_energy, num_occupied) as BlochStates::set_filling."""
+    nb = np.ascontiguousarray(nb, dtype=np.int32)
+    e = np.ascontiguousarray(np.real(eig), dtype=np.float64)
+    kw = np.ascontiguousarray(kweight, dtype=np.float64)
+    f = np.zeros(int(nb.sum()))
+    mu = np.zeros(1)
+    nocc = np.zeros(len(nb), np.int32)
+    st = load_library().tcx_set_filling(len(nb), _ptr(nb, _ip), _ptr(e), _ptr(kw), float(num_electrons),
+                                        {"fixed": 0, "gaussian": 1}[smearing], float(width), _ptr(f), _ptr(mu),
+                                        _ptr(nocc, _ip))
+    if st != TCX_OK:
+        raise TcxError(st, "set_filling failed")
+    return f, float(mu[0]), nocc
 
 
 def total_energy(nb, filling, energies, num_kpoints, uses_3body):

@@ -5,7 +5,8 @@
 //   plane wave of the sphere at k  =>  Vx is diagonal with
 //   <k+G0|Vx|k+G0> = c_x sum_{q,b} f_qb / |k+G0-q-G_b|^2  (0 below 1e-8),
 //   c_x = -4 pi/(Nk V) / 2   (src/calc_hamiltonian_hf2x.cpp:29-30, 91-147).
-// Also checks tc2x with A = 0 against hf2x (src/jastrow_functions.cpp:30-38).
+// Also checks tc2x with A = 0 against hf2x (src/jastrow_functions.cpp:30-38) and
+// calc_hamiltonian::all (kinetic closed form, Hartree = 0 for a uniform density).
 // Exit status 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -118,6 +119,33 @@ int main()
                 err2 = std::fmax(err2, std::abs(H2[0][k][j][0][l] - 2.0 * H[0][k][j][0][l]));
     std::printf("tc2x(A=0) accumulated onto hf2x: max |H2 - 2 H| = %.3e\n", err2);
     if (!(err2 <= 1e-11 * scale)) return 1;
+
+    // calc_hamiltonian::all (calc_hamiltonian_all.cpp:28-159): H1 = 0.5 |k+G|^2 (diagonal),
+    // H2 = hf2h + hf2x = hf2x (plane-wave occupied orbitals give a uniform density, so the
+    // Hartree potential vanishes); with TC and A = 0, H2 is the same and H3 = 0.
+    for (const char* m : {"HF", "TC"}) {
+        method.calc_method = m;
+        MyClock clock;
+        Symmetry sym;
+        Phi4 H1, H2a, H3;
+        calc_hamiltonian::all(par, clock, method, cs, sym, pot, spin, kp, pw, bs, phi, H1, H2a, H3, nullptr);
+        double e1 = 0.0, e2 = 0.0, e3 = 0.0;
+        for (int k = 0; k < 3; ++k)
+            for (size_t j = 0; j < H[0][k].size(); ++j) {
+                double kg2 = 0.0;
+                for (int d = 0; d < 3; ++d) {
+                    const double c = 2 * PI / a * (kp.kvectors_scf[k][d] + mill[k][j][d]);
+                    kg2 += c * c;
+                }
+                for (size_t l = 0; l < H[0][k][j][0].size(); ++l) {
+                    e1 = std::fmax(e1, std::abs(H1[0][k][j][0][l] - (l == j ? 0.5 * kg2 : 0.0)));
+                    e2 = std::fmax(e2, std::abs(H2a[0][k][j][0][l] - H[0][k][j][0][l]));
+                    if (method.calc_method == "TC") e3 = std::fmax(e3, std::abs(H3[0][k][j][0][l]));
+                }
+            }
+        std::printf("all(%s): kinetic |err| = %.3e, H2 |err| = %.3e, |H3| = %.3e\n", m, e1, e2, e3);
+        if (!(e1 <= 1e-13 && e2 <= 1e-11 * scale && e3 <= 1e-14)) return 1;
+    }
     std::printf("OK\n");
     return 0;
 }

@@ -47,3 +47,43 @@ def test_total_energy_sum():
     assert np.allclose(out, [e1, e2, e3, e1 + e2 + e3], atol=1e-15)
     out = tcx.total_energy(nb, f, e, num_kpoints=2, uses_3body=False)
     assert out[2] == 0 and np.isclose(out[3], e1 + e2)
+
+
+# ---------------------------------------------------------------- fillings (no GPU)
+from oracle import oracle as O
+
+
+def _spectrum(seed, nk=4, nb=6, gap=0.3):
+    rng = np.random.default_rng(seed)
+    e = np.sort(rng.uniform(-1.0, 0.0, (nk, nb)), axis=1)
+    e[:, 3:] += gap  # an insulating gap above band 3
+    return np.full(nk, nb, np.int32), e.reshape(-1), np.full(nk, 2.0 / nk)
+
+
+@pytest.mark.parametrize("smearing,width", [("fixed", 0.0), ("gaussian", 0.05), ("gaussian", 0.3)])
+def test_fillings_match_oracle(smearing, width):
+    nb, e, kw = _spectrum(7)
+    f, mu, nocc = tcx.set_filling(nb, e, kw, 6.0, smearing, width if width else 0.01)
+    fo, muo, nocco = O.fillings(nb, e, kw, 6.0, smearing, width if width else 0.01)
+    assert np.abs(f - fo).max() < 1e-12 and abs(mu - muo) < 1e-12 and (nocc == nocco).all()
+    # electrons per cell: sum_k sum_b f kweight / 2
+    assert abs(sum(f.reshape(len(nb), -1).sum(axis=1) * kw / 2) - 6.0) < 1e-8
+
+
+def test_fillings_fixed_and_narrow_gaussian_agree_for_a_gap():
+    nb, e, kw = _spectrum(8)
+    f0, mu0, n0 = tcx.set_filling(nb, e, kw, 6.0, "fixed")
+    f1, mu1, n1 = tcx.set_filling(nb, e, kw, 6.0, "gaussian", 1e-3)
+    assert (n0 == 3).all() and np.abs(f0 - f1).max() < 1e-12
+    assert mu0 == e.reshape(len(nb), -1)[:, 2].max()          # valence-band top (:56-59)
+    assert e.reshape(len(nb), -1)[:, 2].max() < mu1 < e.reshape(len(nb), -1)[:, 3].min()
+
+
+def test_fillings_errors_like_the_reference():
+    nb, e, kw = _spectrum(9)
+    with pytest.raises(tcx.TcxError):
+        tcx.set_filling(nb, e, kw, 14.0, "fixed")             # more occupied than bands
+    with pytest.raises(tcx.TcxError):
+        tcx.set_filling(nb, e, kw, 5.0, "fixed")              # odd electron count
+    with pytest.raises(ValueError):
+        O.fillings(nb, e, kw, 14.0, "fixed")
