@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB device copy and a 1 GiB fill (counter calibration kernels)")
     ap.add_argument("--tune-out", default=os.path.join(ROOT, "gpurun_out", "tuned_shapes.json"))
+    ap.add_argument("--c2", action="store_true",
+                    help="also time the BITC workload (tc2x + three-body terms, fcc 4x4x4k 16b 40^3) in "
+                         "FP32-split and FP64 and report its energy errors under 'c2'")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -250,10 +253,58 @@ def main():
             "E_x_fp64_Ha": float(ex64.real), "dE_x_Ha": float(abs(ex32 - ex64)),
         },
     }
+    if args.c2:
+        out["c2"] = bench_c2(eng, log)
     print(json.dumps(out), flush=True)
     for b in (d_phi, d_out32, d_out64, d_scratch):
         b.free()
     eng.close()
+
+
+def bench_c2(eng, log):
+    """BITC two- and three-body terms on the fcc 4x4x4 k, 16-band, 40^3 cell (A = 0.5, F from
+    the cusp): one FP32-split and one FP64 apply per term on HBM-resident buffers, with the
+    per-band error and the term energy (left orbitals) of FP32 against FP64."""
+    from tc_amd import energy as E
+    from tc_amd import system as S
+    from tc_amd import tcx
+    sys_ = S.fcc_config2()
+    nb = sys_.nocc.copy()
+    phi = sys_.phi_occ
+    eng.load_system(sys_)
+    d_phi = eng.to_device(phi)
+    d_out = eng.device_buffer(phi.nbytes)
+    bo = sys_.band_offsets(nb)
+    res = {"workload": "BITC fcc a=10.26, 4x4x4 k, 16 bands/k (all occupied), 40^3, RPA Jastrow A=0.5",
+           "pairs_per_apply": int(np.sum(nb) * np.sum(sys_.nocc)), "terms": {}}
+    for term, nbody in (("tc2x", 2), ("tc3a1", 3), ("tc3b1", 3), ("tc3a2a4b2b5", 3), ("tc3a3b3b4b6", 3)):
+        outs, secs = {}, {}
+        for prec in (tcx.PREC_FP32, tcx.PREC_FP64):
+            d_out.zero()
+            eng.synchronize()
+            t0 = time.perf_counter()
+            eng._term_device(term, nb, d_phi.ptr, d_out.ptr, prec)
+            eng.synchronize()
+            secs[prec] = time.perf_counter() - t0
+            outs[prec] = d_out.download(np.complex128)
+            log(f"bench c2: {term} {'fp32' if prec else 'fp64'} {secs[prec]:.2f} s")
+        a, b = outs[tcx.PREC_FP32], outs[tcx.PREC_FP64]
+        rel = []
+        for ik in range(sys_.nk):
+            x = a[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+            y = b[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+            rel.extend(np.linalg.norm(x - y, axis=1) / np.maximum(np.linalg.norm(y, axis=1), 1e-300))
+        e32 = E.term_energy(sys_, a, nbody, left=True)
+        e64 = E.term_energy(sys_, b, nbody, left=True)
+        res["terms"][term] = {
+            "fp32_s": secs[tcx.PREC_FP32], "fp64_s": secs[tcx.PREC_FP64],
+            "fp32_pairs_per_s": res["pairs_per_apply"] / secs[tcx.PREC_FP32] if term not in ("tc3a1", "tc3b1") else None,
+            "E_fp64_Ha": [float(e64.real), float(e64.imag)], "dE_Ha": float(abs(e32 - e64)),
+            "per_band_rel_max": float(np.max(rel)),
+        }
+    d_phi.free()
+    d_out.free()
+    return res
 
 
 if __name__ == "__main__":

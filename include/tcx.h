@@ -158,6 +158,24 @@ int tcx_tc2h(tcx_context *ctx, const int *nb, const double *phi, double *h2phi, 
 int tcx_apply_all(tcx_context *ctx, const int *nb, const double *phi, double *h1phi,
                   double *h2phi, double *h3phi, int precision, unsigned flags);
 
+/* ---------------------------------------------------------------- HIP graphs */
+/* No counterpart in the reference (which launches no device work).  For callers that
+ * replay the same applies many times (a Davidson block, batched per-k workloads):
+ * tcx_graph_begin starts capturing this context's stream; the term calls that follow
+ * must pass TCX_FLAG_DEVICE_PTRS and repeat a call already made once (so that its
+ * launch plan and workspaces exist; otherwise TCX_ERR_STATE and the capture is
+ * dropped); tcx_graph_end instantiates the graph and returns its id; tcx_graph_launch
+ * replays it asynchronously (tcx_synchronize waits).  Terms accumulate, so clear the
+ * outputs (tcx_memset_zero, a fill kernel) between replays when needed.  The
+ * three-body exchange terms upload small tables and cannot be captured.  A graph
+ * addresses the caller's buffers and the context's workspaces as they were at
+ * capture: workspaces that later calls outgrow stay allocated until the last graph is
+ * destroyed; the caller keeps its buffers alive; capture again after tcx_set_states. */
+int tcx_graph_begin(tcx_context *ctx);
+int tcx_graph_end(tcx_context *ctx, int *graph_id);
+int tcx_graph_launch(tcx_context *ctx, int graph_id);
+int tcx_graph_destroy(tcx_context *ctx, int graph_id);
+
 /* ---------------------------------------------------------------- consumer */
 /* Block Davidson at fixed occupied orbitals (one Fock operator): replaces
  * Diagonalization::block_davidson (src/include/diagonalization.hpp; src/diagonalization_davidson.cpp:7-689)

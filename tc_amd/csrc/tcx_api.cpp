@@ -18,6 +18,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,9 +29,34 @@ using namespace tcx;
 
 namespace tcx {
 
+thread_local bool g_capturing = false;
+
+namespace {
+std::mutex g_graph_mu;
+int g_live_graphs = 0;
+std::vector<void*> g_retired;
+}  // namespace
+
+void graph_created()
+{
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    ++g_live_graphs;
+}
+
+void graph_destroyed()
+{
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    if (--g_live_graphs > 0) return;
+    if (!g_retired.empty()) (void)hipDeviceSynchronize();
+    for (void* p : g_retired) (void)hipFree(p);
+    g_retired.clear();
+}
+
 void DBuf::ensure(size_t bytes)
 {
     if (bytes <= cap) return;
+    if (g_capturing)
+        throw Error(TCX_ERR_STATE, "graph capture: a workspace would grow; run the same call once before capturing");
     release();
     if (bytes == 0) return;
     hipError_t e = hipMalloc(&p, bytes);
@@ -45,9 +71,15 @@ void DBuf::ensure(size_t bytes)
 
 void DBuf::release()
 {
-    // kernels queued on any stream may still read the old allocation
-    if (p) (void)hipDeviceSynchronize();
-    if (p) (void)hipFree(p);
+    if (p) {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        if (g_live_graphs > 0) {
+            g_retired.push_back(p);  // a captured graph may still address it
+        } else {
+            (void)hipDeviceSynchronize();  // kernels queued on any stream may still read it
+            (void)hipFree(p);
+        }
+    }
     p = nullptr;
     cap = 0;
 }
@@ -59,7 +91,7 @@ void Context::require(bool cond, int code, const char* what)
 
 void Context::timed(int cls, long pairs, const std::function<void()>& launch)
 {
-    if (!timing_on) {
+    if (!timing_on || g_capturing) {
         launch();
         return;
     }
@@ -272,6 +304,8 @@ void Context::orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n
 
 void upload(DBuf& buf, const void* src, size_t bytes)
 {
+    if (g_capturing)
+        throw Error(TCX_ERR_STATE, "graph capture: a launch plan would be built; run the same call once before capturing");
     buf.ensure(std::max<size_t>(bytes, 16));
     if (bytes) TCX_HIP_CHECK(hipMemcpy(buf.p, src, bytes, hipMemcpyHostToDevice));
 }
@@ -438,6 +472,7 @@ ApplyIO Context::begin_apply(const int* nb, const double* phi, double* out, unsi
 {
     ApplyIO io;
     io.host = !(flags & TCX_FLAG_DEVICE_PTRS);
+    require(!(io.host && g_capturing), TCX_ERR_STATE, "graph capture: term calls need TCX_FLAG_DEVICE_PTRS");
     io.ncoef = P.app_coef[nk];
     io.out_host = out;
     if (io.host) {
@@ -458,6 +493,7 @@ ApplyIO Context::begin_apply(const int* nb, const double* phi, double* out, unsi
 
 void Context::end_apply(const ApplyIO& io)
 {
+    if (g_capturing) return;  // the graph replays asynchronously; tcx_synchronize waits
     if (io.host && io.ncoef > 0) {
         std::vector<double> tmp((size_t)io.ncoef * 2);
         TCX_HIP_CHECK(hipMemcpyAsync(tmp.data(), io.out, (size_t)io.ncoef * 16, hipMemcpyDeviceToHost, stream));
@@ -539,6 +575,17 @@ void Context::hf2x(const int* nb_, const double* phi, double* out, int prec, uns
 // ======================================================================= C ABI
 namespace {
 
+// an error inside a captured region ends the capture and drops the partial graph
+void abort_capture(tcx_context* ctx)
+{
+    if (!g_capturing) return;
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(ctx->impl.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    g_capturing = false;
+}
+
 template <typename F>
 int guarded(tcx_context* ctx, F&& f)
 {
@@ -549,13 +596,16 @@ int guarded(tcx_context* ctx, F&& f)
         ctx->impl.err.clear();
         return TCX_OK;
     } catch (const Error& e) {
+        abort_capture(ctx);
         ctx->impl.err = e.what();
         ctx->impl.pending.clear();
         return e.code;
     } catch (const std::exception& e) {
+        abort_capture(ctx);
         ctx->impl.err = e.what();
         return TCX_ERR_INTERNAL;
     } catch (...) {
+        abort_capture(ctx);
         ctx->impl.err = "unknown error";
         return TCX_ERR_INTERNAL;
     }
@@ -789,6 +839,55 @@ int tcx_eig_dense(int n, const double* A, double* w, double* X)
                : TCX_ERR_INTERNAL;
 }
 
+int tcx_graph_begin(tcx_context* ctx)
+{
+    return guarded(ctx, [&] {
+        Context::require(!g_capturing, TCX_ERR_STATE, "graph_begin: a capture is already open on this thread");
+        Context::require(!ctx->impl.timing_on, TCX_ERR_STATE, "graph_begin: switch timing off first");
+        TCX_HIP_CHECK(hipStreamBeginCapture(ctx->impl.stream, hipStreamCaptureModeThreadLocal));
+        g_capturing = true;
+    });
+}
+
+int tcx_graph_end(tcx_context* ctx, int* graph_id)
+{
+    return guarded(ctx, [&] {
+        Context::require(g_capturing, TCX_ERR_STATE, "graph_end: no open capture");
+        Context::require(graph_id != nullptr, TCX_ERR_ARG, "graph_end: null id");
+        hipGraph_t g = nullptr;
+        g_capturing = false;
+        TCX_HIP_CHECK(hipStreamEndCapture(ctx->impl.stream, &g));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        TCX_HIP_CHECK(e);
+        ctx->impl.graphs.push_back(exec);
+        graph_created();
+        *graph_id = (int)ctx->impl.graphs.size() - 1;
+    });
+}
+
+int tcx_graph_launch(tcx_context* ctx, int graph_id)
+{
+    return guarded(ctx, [&] {
+        Context::require(graph_id >= 0 && graph_id < (int)ctx->impl.graphs.size() && ctx->impl.graphs[graph_id],
+                         TCX_ERR_ARG, "graph_launch: unknown graph");
+        TCX_HIP_CHECK(hipGraphLaunch(ctx->impl.graphs[graph_id], ctx->impl.stream));
+    });
+}
+
+int tcx_graph_destroy(tcx_context* ctx, int graph_id)
+{
+    return guarded(ctx, [&] {
+        Context::require(graph_id >= 0 && graph_id < (int)ctx->impl.graphs.size() && ctx->impl.graphs[graph_id],
+                         TCX_ERR_ARG, "graph_destroy: unknown graph");
+        TCX_HIP_CHECK(hipStreamSynchronize(ctx->impl.stream));
+        TCX_HIP_CHECK(hipGraphExecDestroy(ctx->impl.graphs[graph_id]));
+        ctx->impl.graphs[graph_id] = nullptr;
+        graph_destroyed();
+    });
+}
+
 int tcx_device_alloc(tcx_context* ctx, unsigned long long bytes, void** ptr)
 {
     if (!ptr) return TCX_ERR_ARG;
@@ -829,8 +928,8 @@ int tcx_memset_zero(tcx_context* ctx, void* ptr, unsigned long long bytes)
     return guarded(ctx, [&] {
         if (bytes == 0) return;
         if (!ptr) throw Error(TCX_ERR_ARG, "memset_zero: null pointer");
-        launch_zero(ptr, bytes, ctx->impl.stream);
-        TCX_HIP_CHECK(hipStreamSynchronize(ctx->impl.stream));
+        launch_zero(ptr, bytes, ctx->impl.stream);  // a fill kernel, also inside a captured graph
+        if (!g_capturing) TCX_HIP_CHECK(hipStreamSynchronize(ctx->impl.stream));
     });
 }
 

@@ -17,6 +17,15 @@ namespace tcx {
 
 constexpr double kPiHost = 3.14159265358979323846;
 
+// Set while a context's stream is being captured into a HIP graph: nothing may
+// allocate, copy synchronously or wait then.
+extern thread_local bool g_capturing;
+
+// Live captured graphs hold raw pointers into workspaces: while any exists, a
+// released workspace is retired (kept allocated) and freed with the last graph.
+void graph_created();
+void graph_destroyed();
+
 // Device buffer that only grows.
 struct DBuf {
     void* p = nullptr;
@@ -140,6 +149,9 @@ struct Context {
     std::map<std::tuple<std::vector<int>, int>, PairSet> pairsets;
     DBuf d_qsum, d_qsumT, d_rec, d_Q2, d_Q4, d_Q5, d_Q4T, d_Q5T, d_appR64, d_chans, d_aux, d_local;
 
+    // HIP graphs captured from this context's stream (tcx_graph_*)
+    std::vector<hipGraphExec_t> graphs;
+
     // timing
     bool timing_on = false;
     double t_ms[4] = {0, 0, 0, 0};
@@ -149,6 +161,12 @@ struct Context {
 
     ~Context()
     {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& g : graphs)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                graph_destroyed();
+            }
         for (auto& ev : pending) {
             (void)hipEventDestroy(ev.a);
             (void)hipEventDestroy(ev.b);

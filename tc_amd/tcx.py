@@ -40,6 +40,7 @@ EXPORTS = [
     "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
     "tcx_davidson", "tcx_total_energy", "tcx_eig_dense", "tcx_set_filling",
+    "tcx_graph_begin", "tcx_graph_end", "tcx_graph_launch", "tcx_graph_destroy",
     "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
     "tcx_set_launch_shape", "tcx_enable_timing", "tcx_get_timing",
 ]
@@ -86,6 +87,10 @@ def load_library():
     L.tcx_davidson.argtypes = [vp, _ip, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.tcx_total_energy.argtypes = [C.c_int, _ip, _dp, vp, C.c_int, C.c_int, vp]
     L.tcx_eig_dense.argtypes = [C.c_int, vp, vp, vp]
+    L.tcx_graph_begin.argtypes = [vp]
+    L.tcx_graph_end.argtypes = [vp, _ip]
+    L.tcx_graph_launch.argtypes = [vp, C.c_int]
+    L.tcx_graph_destroy.argtypes = [vp, C.c_int]
     L.tcx_set_filling.argtypes = [C.c_int, _ip, _dp, _dp, C.c_double, C.c_int, C.c_double, _dp, _dp, _ip]
     L.tcx_device_alloc.argtypes = [vp, C.c_ulonglong, C.POINTER(vp)]
     L.tcx_device_free.argtypes = [vp, vp]
@@ -304,6 +309,21 @@ class Engine:
                                  int(precision))
         self._check(st, "davidson")
         return phi, left, eig, en
+
+    # --------------------------------------------------------------- HIP graphs
+    def graph_begin(self):
+        self._check(self.L.tcx_graph_begin(self.h), "graph_begin")
+
+    def graph_end(self):
+        gid = np.zeros(1, np.int32)
+        self._check(self.L.tcx_graph_end(self.h, _ptr(gid, _ip)), "graph_end")
+        return int(gid[0])
+
+    def graph_launch(self, gid):
+        self._check(self.L.tcx_graph_launch(self.h, int(gid)), "graph_launch")
+
+    def graph_destroy(self, gid):
+        self._check(self.L.tcx_graph_destroy(self.h, int(gid)), "graph_destroy")
 
     def apply_all(self, nb, phi, precision=PREC_FP64, method=None):
         """H1 phi, H2 phi, H3 phi (H3 is None for HF), each zeroed first (calc_hamiltonian::all)."""
