@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB device copy and a 1 GiB fill (counter calibration kernels)")
     ap.add_argument("--tune-out", default=os.path.join(ROOT, "gpurun_out", "tuned_shapes.json"))
+    ap.add_argument("--c3", type=int, default=0, metavar="REPLAYS",
+                    help="also run the batched uneven workload (3 grids x 32 k, 8-48 bands per k, hf2x + tc2x) "
+                         "as captured HIP graphs for REPLAYS replays vs sequential per-k launches; report 'c3'")
     ap.add_argument("--c2", action="store_true",
                     help="also time the BITC workload (tc2x + three-body terms, fcc 4x4x4k 16b 40^3) in "
                          "FP32-split and FP64 and report its energy errors under 'c2'")
@@ -255,10 +258,101 @@ def main():
     }
     if args.c2:
         out["c2"] = bench_c2(eng, log)
+    if args.c3:
+        out["c3"] = bench_c3(log, args.c3)
     print(json.dumps(out), flush=True)
     for b in (d_phi, d_out32, d_out64, d_scratch):
         b.free()
     eng.close()
+
+
+def bench_c3(log, replays):
+    """Batched uneven workload: three fcc cells (32^3, 36^3, 48^3), 32 k each with 8-48 bands
+    per k (all occupied), hf2x + tc2x in FP32-split.  Graph mode: one context per cell, each
+    apply captured once into a HIP graph, the three graphs launched together on their own
+    streams per replay.  Sequential mode: the same work as separate per-k calls (one k's
+    bands at a time), each call synchronous.  Reports p50 / p99 wall time per apply."""
+    from tc_amd import system as S
+    from tc_amd import tcx
+    systems = S.batched_uneven_systems()
+    ctxs = []
+    pairs = 0
+    for sys_ in systems:
+        eng = tcx.Engine(0)
+        eng.load_system(sys_)
+        nb = sys_.nocc.copy()
+        d_phi = eng.to_device(sys_.phi_occ)
+        d_out = eng.device_buffer(sys_.phi_occ.nbytes)
+        ctxs.append((sys_, eng, nb, d_phi, d_out))
+        pairs += 2 * int(nb.sum()) * int(sys_.nocc.sum())  # hf2x + tc2x pairs
+
+    def apply(eng, nb, d_phi, d_out):
+        d_out.zero()
+        eng._term_device("hf2x", nb, d_phi.ptr, d_out.ptr, tcx.PREC_FP32)
+        eng._term_device("tc2x", nb, d_phi.ptr, d_out.ptr, tcx.PREC_FP32)
+
+    log("bench c3: eager warm-up of the batched applies")
+    ref = []
+    for sys_, eng, nb, d_phi, d_out in ctxs:
+        apply(eng, nb, d_phi, d_out)
+        eng.synchronize()
+        ref.append(d_out.download(np.complex128))
+    gids = []
+    for sys_, eng, nb, d_phi, d_out in ctxs:
+        eng.graph_begin()
+        apply(eng, nb, d_phi, d_out)
+        gids.append(eng.graph_end())
+    graph_t = []
+    for r in range(replays):
+        t0 = time.perf_counter()
+        for (sys_, eng, nb, d_phi, d_out), gid in zip(ctxs, gids):
+            eng.graph_launch(gid)
+        for sys_, eng, nb, d_phi, d_out in ctxs:
+            eng.synchronize()
+        graph_t.append(time.perf_counter() - t0)
+        log(f"bench c3: graph replay {r + 1}/{replays} {graph_t[-1]:.3f} s")
+    same = all(np.array_equal(d_out.download(np.complex128), rf) for (_, _, _, _, d_out), rf in zip(ctxs, ref))
+
+    def per_k(sys_, eng, nb, d_phi, d_out):
+        d_out.zero()
+        for ik in range(sys_.nk):
+            one = np.zeros(sys_.nk, np.int32)
+            one[ik] = nb[ik]
+            off = int(sys_.band_offsets(nb)[ik])
+            for term in ("hf2x", "tc2x"):
+                # one k's bands: phi and out pointers advanced to its block
+                eng._term_device(term, one, d_phi.ptr + 16 * off, d_out.ptr + 16 * off, tcx.PREC_FP32)
+
+    log("bench c3: sequential per-k warm-up")
+    for c in ctxs:
+        per_k(*c)
+    seq_t = []
+    for r in range(max(2, replays // 2)):
+        t0 = time.perf_counter()
+        for c in ctxs:
+            per_k(*c)
+        for c in ctxs:
+            c[1].synchronize()
+        seq_t.append(time.perf_counter() - t0)
+        log(f"bench c3: sequential per-k apply {r + 1} {seq_t[-1]:.3f} s")
+    seq_same = all(np.allclose(d_out.download(np.complex128), rf, rtol=0, atol=1e-6 * np.abs(rf).max())
+                   for (_, _, _, _, d_out), rf in zip(ctxs, ref))
+    for (sys_, eng, nb, d_phi, d_out), gid in zip(ctxs, gids):
+        eng.graph_destroy(gid)
+        d_phi.free()
+        d_out.free()
+        eng.close()
+    q = lambda a, p: float(np.percentile(np.asarray(a), p))
+    return {
+        "workload": "3 fcc cells (32^3, 36^3, 48^3) x 32 k, bands per k uniform in [8,48] (seeded, all occupied), "
+                    "hf2x + tc2x FP32-split",
+        "bands_per_cell": [int(c[2].sum()) for c in ctxs], "pairs_per_apply": pairs,
+        "graph": {"replays": replays, "p50_s": q(graph_t, 50), "p99_s": q(graph_t, 99),
+                  "pairs_per_s": pairs / q(graph_t, 50), "replays_bit_identical_to_eager": bool(same)},
+        "sequential_per_k": {"runs": len(seq_t), "p50_s": q(seq_t, 50), "p99_s": q(seq_t, 99),
+                             "pairs_per_s": pairs / q(seq_t, 50), "matches_batched": bool(seq_same)},
+        "speedup_p50": q(seq_t, 50) / q(graph_t, 50),
+    }
 
 
 def bench_c2(eng, log):

@@ -158,6 +158,22 @@ int tcx_tc2h(tcx_context *ctx, const int *nb, const double *phi, double *h2phi, 
 int tcx_apply_all(tcx_context *ctx, const int *nb, const double *phi, double *h1phi,
                   double *h2phi, double *h3phi, int precision, unsigned flags);
 
+/* ---------------------------------------------------------------- gradients */
+/* No single counterpart in the reference (which has no derivatives); the energies
+ * are those of TotalEnergy::calc_total_energy (src/total_energy.cpp:118-163) for the
+ * occupied set, and H_2^dagger is the reference's own Hermitian conjugate
+ * (Jastrow::take_Hermitian_conj + BlochStates::switch_left_right_orbitals,
+ * src/diagonalization_davidson.cpp:234-251).  Host pointers, complex128 in the
+ * occupied layout of tcx_set_states:
+ *   tcx_grad_exchange: energy[2] = E_x = (1/(2Nk)) sum f <phi|V_x phi>,
+ *                      grad = dE_x/d conj(phi) = (f/Nk) V_x phi;
+ *   tcx_grad_tc2:      energy[2] = E_2 = (1/(2Nk)) sum f <chi|(tc2h + tc2x) phi>,
+ *                      grad_left = dE_2/d conj(chi), grad_right = dE_2/d phi,
+ *                      dE_dA[2] = dE_2/dA at fixed F (all four spin-pair A scaled together). */
+int tcx_grad_exchange(tcx_context *ctx, double *grad, double *energy, int precision);
+int tcx_grad_tc2(tcx_context *ctx, double *grad_left, double *grad_right, double *energy,
+                 double *dE_dA, int precision);
+
 /* ---------------------------------------------------------------- HIP graphs */
 /* No counterpart in the reference (which launches no device work).  For callers that
  * replay the same applies many times (a Davidson block, batched per-k workloads):

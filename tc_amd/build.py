@@ -34,6 +34,7 @@ SOURCES = [
     "csrc/tcx_tc.cpp",
     "csrc/tcx_tc3x.cpp",
     "csrc/tcx_davidson.cpp",
+    "csrc/tcx_adjoint.cpp",
     "host/calc_hamiltonian.cpp",
 ]
 ROOT = os.path.dirname(HERE)

@@ -839,6 +839,17 @@ int tcx_eig_dense(int n, const double* A, double* w, double* X)
                : TCX_ERR_INTERNAL;
 }
 
+int tcx_grad_exchange(tcx_context* ctx, double* grad, double* energy, int precision)
+{
+    return guarded(ctx, [&] { ctx->impl.grad_x(grad, energy, precision); });
+}
+
+int tcx_grad_tc2(tcx_context* ctx, double* grad_left, double* grad_right, double* energy, double* dE_dA,
+                 int precision)
+{
+    return guarded(ctx, [&] { ctx->impl.grad_tc2(grad_left, grad_right, energy, dE_dA, precision); });
+}
+
 int tcx_graph_begin(tcx_context* ctx)
 {
     return guarded(ctx, [&] {

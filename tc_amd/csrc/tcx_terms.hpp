@@ -189,7 +189,11 @@ struct Context {
     void ensure_occR(int prec);
     void ensure_chiR(int prec);
     void ensure_gradR(int prec);
-    JastrowRPA jastrow_pair(int s1, int s2) const { return JastrowRPA{A[s1 * 2 + s2], F[s1 * 2 + s2], jastrow_zero ? 1 : 0}; }
+    bool jastrow_dA = false;  // evaluate d/dA of the Jastrow kernels (adjoint of the A-dependence)
+    JastrowRPA jastrow_pair(int s1, int s2) const
+    {
+        return JastrowRPA{A[s1 * 2 + s2], F[s1 * 2 + s2], jastrow_zero ? 1 : 0, jastrow_dA ? 1 : 0};
+    }
     void ensure_density();
 
     void resolve_shape(int term, long& tc, long& sc, long ntargets, long nsources) const;
@@ -219,6 +223,11 @@ struct Context {
     void apply_all(const int* nb, const double* phi, double* h1, double* h2, double* h3, int prec, unsigned flags);
     void davidson(const int* nb, double* phi, double* phi_left, double* eig, double* energies, int max_blocks,
                   int num_refresh, int biortho, int prec);
+
+    // gradients of the occupied-set energies (tcx_adjoint.cpp)
+    void swap_left_right();
+    void grad_x(double* grad, double* energy, int prec);
+    void grad_tc2(double* grad_left, double* grad_right, double* energy, double* dE_dA, int prec);
 };
 
 // Dense complex eigen-decomposition A X = X diag(w), A and X n x n row-major,

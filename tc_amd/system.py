@@ -260,6 +260,37 @@ def fcc_config2(**kw) -> System:
                         method="BITC", A=A, name="bitc fcc a10.26 4x4x4k 16b 40^3 A0.5", **kw)
 
 
+def uneven_bands(sys_: System, counts) -> System:
+    """The same cell with only the first counts[k] occupied orbitals at k."""
+    counts = np.asarray(counts, dtype=np.int32)
+    po, oo = sys_.pw_offsets(), sys_.occ_offsets()
+    phis, chis, fill = [], [], []
+    for ik in range(sys_.nk):
+        assert counts[ik] <= sys_.nocc[ik]
+        blk = sys_.phi_occ[oo[ik]:oo[ik + 1]].reshape(sys_.nocc[ik], -1)[:counts[ik]]
+        phis.append(blk.reshape(-1))
+        if sys_.chi_occ is not None:
+            chis.append(sys_.chi_occ[oo[ik]:oo[ik + 1]].reshape(sys_.nocc[ik], -1)[:counts[ik]].reshape(-1))
+        fo = int(np.sum(sys_.nocc[:ik]))
+        fill.append(sys_.filling[fo:fo + counts[ik]])
+    return System(**{**sys_.__dict__, "nocc": counts, "filling": np.concatenate(fill),
+                     "phi_occ": np.concatenate(phis),
+                     "chi_occ": np.concatenate(chis) if chis else None})
+
+
+def batched_uneven_systems(seed=3):
+    """Three fcc a=10.26 cells, one per FFT grid 32^3 / 36^3 / 48^3, 4x4x2 k (32 k each,
+    96 in all), a TC Jastrow (A = 0.5), and a seeded uniform band count in [8, 48] per k,
+    all occupied."""
+    rng = np.random.default_rng(1234 + 1000 * seed)
+    out = []
+    for g, grid in enumerate((32, 36, 48)):
+        base = build_system(fcc_lattice(10.26), (grid,) * 3, (4, 4, 2), nbands=48, nocc=48, config=3 + 10 * g,
+                            method="TC", A=np.full((2, 2), 0.5), name=f"tc fcc a10.26 4x4x2k {grid}^3 uneven")
+        out.append(uneven_bands(base, rng.integers(8, 49, size=base.nk)))
+    return out
+
+
 def tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, lattice=None,
                 A=None, config=90, **kw) -> System:
     """Small systems for golden vectors and brute-force checks."""

@@ -41,6 +41,7 @@ EXPORTS = [
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
     "tcx_davidson", "tcx_total_energy", "tcx_eig_dense", "tcx_set_filling",
     "tcx_graph_begin", "tcx_graph_end", "tcx_graph_launch", "tcx_graph_destroy",
+    "tcx_grad_exchange", "tcx_grad_tc2",
     "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
     "tcx_set_launch_shape", "tcx_enable_timing", "tcx_get_timing",
 ]
@@ -87,6 +88,8 @@ def load_library():
     L.tcx_davidson.argtypes = [vp, _ip, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.tcx_total_energy.argtypes = [C.c_int, _ip, _dp, vp, C.c_int, C.c_int, vp]
     L.tcx_eig_dense.argtypes = [C.c_int, vp, vp, vp]
+    L.tcx_grad_exchange.argtypes = [vp, vp, _dp, C.c_int]
+    L.tcx_grad_tc2.argtypes = [vp, vp, vp, _dp, _dp, C.c_int]
     L.tcx_graph_begin.argtypes = [vp]
     L.tcx_graph_end.argtypes = [vp, _ip]
     L.tcx_graph_launch.argtypes = [vp, C.c_int]
@@ -309,6 +312,26 @@ class Engine:
                                  int(precision))
         self._check(st, "davidson")
         return phi, left, eig, en
+
+    # --------------------------------------------------------------- gradients
+    def grad_exchange(self, precision=PREC_FP64):
+        """(E_x, dE_x/d conj(phi_occ)) for the loaded occupied set."""
+        n = int(self.sys.phi_occ.size)
+        g = np.zeros(n, np.complex128)
+        e = np.zeros(2)
+        self._check(self.L.tcx_grad_exchange(self.h, g.ctypes.data, _ptr(e), int(precision)), "grad_exchange")
+        return complex(e[0], e[1]), g
+
+    def grad_tc2(self, precision=PREC_FP64):
+        """(E_2, dE_2/d conj(chi), dE_2/d phi, dE_2/dA) for the loaded occupied set."""
+        n = int(self.sys.phi_occ.size)
+        gl = np.zeros(n, np.complex128)
+        gr = np.zeros(n, np.complex128)
+        e = np.zeros(2)
+        da = np.zeros(2)
+        self._check(self.L.tcx_grad_tc2(self.h, gl.ctypes.data, gr.ctypes.data, _ptr(e), _ptr(da), int(precision)),
+                    "grad_tc2")
+        return complex(e[0], e[1]), gl, gr, complex(da[0], da[1])
 
     # --------------------------------------------------------------- HIP graphs
     def graph_begin(self):

@@ -254,3 +254,20 @@ def test_jastrow_kernels():
         assert abs(o.tc_2body(G * G) - expect) < 1e-4 * abs(expect)
     assert abs(o.uk(0.0) - (-4 * np.pi * A * F * F)) < 1e-15
     assert abs(o.tc_2body(0.0) - (-2 * np.pi * A * A / F)) < 1e-14
+
+
+@pytest.mark.parametrize("method", ["TC", "BITC"])
+def test_two_body_adjoint_is_minus_u_with_swapped_orbitals(method):
+    """<a|H[u; phi, chi] b> = conj <b|H[-u; chi, phi] a> for tc2x + tc2h
+    (src/diagonalization_davidson.cpp:234-251: take_Hermitian_conj + switch_left_right)."""
+    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=81)
+    adj = S.System(**{**sys_.__dict__, "A": -sys_.A})
+    if method == "BITC":
+        adj.phi_occ, adj.chi_occ = sys_.chi_occ.copy(), sys_.phi_occ.copy()
+    nb, a = _random_bands(sys_, 82)
+    _, b = _random_bands(sys_, 83)
+    o, oa = O.Oracle(sys_), O.Oracle(adj)
+    Hb = o.tc2x(nb, b) + o.tc2h(nb, b)
+    Ha = oa.tc2x(nb, a) + oa.tc2h(nb, a)
+    lhs, rhs = np.vdot(a, Hb), np.conj(np.vdot(b, Ha))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
