@@ -174,6 +174,18 @@ int tcx_grad_exchange(tcx_context *ctx, double *grad, double *energy, int precis
 int tcx_grad_tc2(tcx_context *ctx, double *grad_left, double *grad_right, double *energy,
                  double *dE_dA, int precision);
 
+/* Backward pass with respect to the two-body kernels, radial bins of |k-q+G| of width
+ * `width` (1 <= nbins <= 1024, last bin open).  Pair densities are recomputed, not stored.
+ * term 0: E_x and its Coulomb kernel v = 1/|k-q+G|^2 (src/calc_hamiltonian_hf2x.cpp:91-104):
+ *         dE_dK complex128[nbins] = dE_x/dv per bin; dE_du unused (may be NULL);
+ * term 1: the exchange part of E_2 (tc2x without the divergence term,
+ *         src/calc_hamiltonian_tc2x.cpp:253-401) and its kernels:
+ *         dE_dK complex128[nbins] = dE/d tc_2body, dE_du complex128[nbins] = dE/du.
+ * sums[4]: sum over all points of K dE/dK (+ u dE/du): the kernel-linear energy (re, im),
+ *          and of dK/dA dE/dK + du/dA dE/du: dE/dA at fixed F (re, im; term 1). */
+int tcx_kernel_gradient(tcx_context *ctx, int term, int nbins, double width, double *dE_dK,
+                        double *dE_du, double *sums, int precision);
+
 /* ---------------------------------------------------------------- HIP graphs */
 /* No counterpart in the reference (which launches no device work).  For callers that
  * replay the same applies many times (a Davidson block, batched per-k workloads):

@@ -931,3 +931,113 @@ void launch_axpby(const double* X, const double* Y, double* out, double a, doubl
 }
 
 }  // namespace tcx
+
+namespace tcx {
+
+// ----------------------------------------------------------------- kernel-gradient reductions
+// Backward pass of the occupied-set energies with respect to the two-body kernels.  For
+// every (target kj, source qi) pair of occupied orbitals the forward pair densities are
+// recomputed (FFT_forward of the products, in the working precision) and, with
+// x = k - q + G and w = c f_kj f_qi / (2 Nk):
+//   E_x : dE/dv(x)   = w |rho(G)|^2,                         rho = FFT(conj(phi_qi) phi_kj)
+//   E_2x: dE/dV2(x)  = w conj(rho'(G)) rho(G)
+//         dE/du(x)   = w sum_d x_d [conj(rho'(G)) sig_d(G) - conj(tau_d(G)) rho(G)]
+//         rho = FFT(conj(chi_qi) phi_kj), sig_d = FFT(conj(chi_qi) g_kj,d),
+//         rho' = FFT(conj(phi_qi) chi_kj), tau_d = FFT(conj(g_qi,d) chi_kj)
+// (the G-space form of <chi_kj|tc2x phi_kj> in src/calc_hamiltonian_tc2x.cpp:253-401).  The
+// values are summed into radial bins of |x| (width `width`, last bin open) through an LDS
+// histogram per workgroup and FP64 global atomics (so the bin sums are not bit-reproducible
+// between runs), together with two checks: sum_x K(x) dE/dK(x) (the kernel-linear part of
+// the energy) and sum_x dK/dA(x) dE/dK(x) (dE/dA at fixed F).
+constexpr int kMaxKernelBins = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_kgrad(const cpx<T>* __restrict__ rec, int tc2, const PairEntry* __restrict__ pairs,
+                                               int p0, const double* __restrict__ kqcart, const double* __restrict__ fill,
+                                               Geom g, long long N, JastrowRPA j0, JastrowRPA j1, double w0,
+                                               int nbins, double width, double* __restrict__ bins_v,
+                                               double* __restrict__ bins_u, double* __restrict__ sums)
+{
+    __shared__ double hv[2 * kMaxKernelBins], hu[2 * kMaxKernelBins];
+    __shared__ double red[4][256];
+    const int t = threadIdx.x;
+    for (int b = t; b < 2 * nbins; b += 256) {
+        hv[b] = 0.0;
+        hu[b] = 0.0;
+    }
+    __syncthreads();
+    const long long r = (long long)blockIdx.x * 256 + t;
+    const int p = p0 + (int)blockIdx.y;
+    double s_lin[2] = {0.0, 0.0}, s_dA[2] = {0.0, 0.0};
+    if (r < N) {
+        const PairEntry e = pairs[p];
+        double x[3];
+        const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
+        const int b = min(nbins - 1, (int)(sqrt(x2) / width));
+        const double w = w0 * fill[e.app] * fill[e.occ];
+        if (!tc2) {
+            const double2 rho = up(rec[(long long)p * N + r]);
+            const double sv = w * (rho.x * rho.x + rho.y * rho.y);
+            atomicAdd(&hv[2 * b], sv);
+            const double v = x2 < 1e-8 ? 0.0 : 1.0 / x2;  // hf2x.cpp:91-104
+            s_lin[0] = v * sv;
+        } else {
+            const cpx<T>* in = rec + (long long)p * 8 * N + r;
+            const double2 rho = up(in[0]), rhop = up(in[4 * N]);
+            const double2 sV = dscale(dmul(dconj(rhop), rho), w);
+            double2 sU = make_double2(0.0, 0.0);
+            for (int d = 0; d < 3; ++d) {
+                const double2 sig = up(in[(long long)(1 + d) * N]), tau = up(in[(long long)(5 + d) * N]);
+                const double2 a = dmul(dconj(rhop), sig), c = dmul(dconj(tau), rho);
+                sU = dadd(sU, dscale(make_double2(a.x - c.x, a.y - c.y), x[d]));
+            }
+            sU = dscale(sU, w);
+            atomicAdd(&hv[2 * b], sV.x);
+            atomicAdd(&hv[2 * b + 1], sV.y);
+            atomicAdd(&hu[2 * b], sU.x);
+            atomicAdd(&hu[2 * b + 1], sU.y);
+            // spin pair (ispin, ispin) of tc2x.cpp:269-271
+            JastrowRPA jd = j0;
+            jd.dA = 1;
+            const double V2 = jastrow_tc2body(j0, x2), u = jastrow_uk(j0, x2);
+            const double dV2 = jastrow_tc2body(jd, x2), du = jastrow_uk(jd, x2);
+            s_lin[0] = V2 * sV.x + u * sU.x;
+            s_lin[1] = V2 * sV.y + u * sU.y;
+            s_dA[0] = dV2 * sV.x + du * sU.x;
+            s_dA[1] = dV2 * sV.y + du * sU.y;
+        }
+    }
+    red[0][t] = s_lin[0];
+    red[1][t] = s_lin[1];
+    red[2][t] = s_dA[0];
+    red[3][t] = s_dA[1];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st)
+            for (int q = 0; q < 4; ++q) red[q][t] += red[q][t + st];
+        __syncthreads();
+    }
+    if (t < 4) atomicAdd(&sums[t], red[t][0]);
+    for (int b = t; b < 2 * nbins; b += 256) {
+        if (hv[b] != 0.0) atomicAdd(&bins_v[b], hv[b]);
+        if (tc2 && hu[b] != 0.0) atomicAdd(&bins_u[b], hu[b]);
+    }
+}
+
+void launch_kgrad(int prec, const void* rec, int tc2, const PairEntry* pairs, int npairs, const double* kqcart,
+                  const double* fill, const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1, double w0,
+                  int nbins, double width, double* bins_v, double* bins_u, double* sums, hipStream_t s)
+{
+    if (nbins < 1 || nbins > kMaxKernelBins) throw Error(TCX_ERR_ARG, "kernel gradient: 1 <= nbins <= 1024");
+    for_slices(npairs, [&](int p0, unsigned ny) {
+        if (prec == TCX_PREC_FP64)
+            hipLaunchKernelGGL(k_kgrad<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec, tc2, pairs,
+                               p0, kqcart, fill, g, N, j0, j1, w0, nbins, width, bins_v, bins_u, sums);
+        else
+            hipLaunchKernelGGL(k_kgrad<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec, tc2, pairs,
+                               p0, kqcart, fill, g, N, j0, j1, w0, nbins, width, bins_v, bins_u, sums);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
+}  // namespace tcx

@@ -160,4 +160,62 @@ void Context::grad_tc2(double* grad_left, double* grad_right, double* energy, do
     dE_dA[1] = dA.imag();
 }
 
+// Kernel gradients (the backward pair reduction of kernels_tc3x.hip, k_kgrad): pair densities
+// of every occupied (target, source) pair are recomputed and reduced into radial bins of
+// |k - q + G|.  term 0: E_x and its Coulomb kernel v; term 1: the exchange part of E_2
+// (tc2x without the divergence term) and its kernels tc_2body (dE_dK) and u (dE_du).
+void Context::kernel_grad(int term, int nbins, double width, double* dE_dK, double* dE_du, double* sums, int prec)
+{
+    require(states_set, TCX_ERR_STATE, "kernel_grad: call tcx_set_states first");
+    require(term == 0 || term == 1, TCX_ERR_ARG, "kernel_grad: term is 0 (exchange) or 1 (TC two-body)");
+    require(dE_dK && sums && (term == 0 || dE_du), TCX_ERR_ARG, "kernel_grad: null pointer");
+    require(nbins >= 1 && nbins <= 1024 && width > 0.0, TCX_ERR_ARG, "kernel_grad: 1 <= nbins <= 1024, width > 0");
+    const bool tc2 = term == 1;
+    if (tc2) require(method != TCX_METHOD_HF && jastrow_set, TCX_ERR_STATE, "kernel_grad: needs a TC/BITC cell");
+    const bool bitc = method == TCX_METHOD_BITC;
+    const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gT = (size_t)N * es;
+    const int C = tc2 ? 8 : 1;
+    d_aux.ensure((4 * (size_t)nbins + 4) * sizeof(double));
+    launch_zero(d_aux.p, (4 * (size_t)nbins + 4) * sizeof(double), stream);
+    double* bins_v = (double*)d_aux.p;
+    double* bins_u = bins_v + 2 * nbins;
+    double* d_sums = bins_u + 2 * nbins;
+    if (n_occ > 0) {
+        ensure_occR(prec);
+        if (bitc) ensure_chiR(prec);
+        if (tc2) ensure_gradR(prec);
+        const void* occR = d_occR[prec].p;
+        const void* chiR = bitc ? d_chiR[prec].p : occR;
+        const void* gradR = tc2 ? d_gradR[prec].p : nullptr;
+        std::vector<ProdChan> ch;
+        if (!tc2) {
+            ch.push_back(ProdChan{occR, occR, 1, 1, 0, 1, 0, 1, 0, 0});  // conj(phi_qi) phi_kj
+        } else {
+            ch.push_back(ProdChan{chiR, occR, 1, 1, 0, 1, 0, 1, 0, 0});  // rho = conj(chi_qi) phi_kj
+            for (int d = 0; d < 3; ++d) ch.push_back(ProdChan{chiR, gradR, 1, 1, 0, 1, 0, 3, d, 0});  // sig_d
+            ch.push_back(ProdChan{occR, chiR, 1, 1, 0, 1, 0, 1, 0, 0});  // rho' = conj(phi_qi) chi_kj
+            for (int d = 0; d < 3; ++d) ch.push_back(ProdChan{gradR, chiR, 1, 3, d, 1, 0, 1, 0, 0});  // tau_d
+        }
+        upload(d_chans, ch.data(), ch.size() * sizeof(ProdChan));
+        const PairSet& S = pair_set(nocc, 1, C, prec);  // occupied x occupied
+        d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * C * gT);
+        const double NkV = num_kpoints * volume;
+        double c = tc2 ? -1.0 / NkV : -4.0 * kPiHost / NkV;  // two_body_factor of tc2x / hf2x
+        if (nspin == 1) c /= 2.0;
+        const double w0 = c / (2.0 * num_kpoints);
+        const JastrowRPA j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
+        for (const Launch& L : S.launches) {
+            const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
+            launch_products(prec, pairs, L.npairs, (const ProdChan*)d_chans.p, C, d_rec.p, N, stream);
+            launch_fft3d(prec, -1, d_rec.p, C * L.npairs, geo, 1.0 / (double)N, stream);
+            launch_kgrad(prec, d_rec.p, tc2 ? 1 : 0, pairs, L.npairs, (const double*)S.d_kq.p, (const double*)d_fill.p,
+                         geo, N, j0, j1, w0, nbins, width, bins_v, bins_u, d_sums, stream);
+        }
+    }
+    TCX_HIP_CHECK(hipStreamSynchronize(stream));
+    TCX_HIP_CHECK(hipMemcpy(dE_dK, bins_v, 2 * (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost));
+    if (tc2) TCX_HIP_CHECK(hipMemcpy(dE_du, bins_u, 2 * (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost));
+    TCX_HIP_CHECK(hipMemcpy(sums, d_sums, 4 * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 }  // namespace tcx

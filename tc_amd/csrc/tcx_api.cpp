@@ -850,6 +850,12 @@ int tcx_grad_tc2(tcx_context* ctx, double* grad_left, double* grad_right, double
     return guarded(ctx, [&] { ctx->impl.grad_tc2(grad_left, grad_right, energy, dE_dA, precision); });
 }
 
+int tcx_kernel_gradient(tcx_context* ctx, int term, int nbins, double width, double* dE_dK, double* dE_du,
+                        double* sums, int precision)
+{
+    return guarded(ctx, [&] { ctx->impl.kernel_grad(term, nbins, width, dE_dK, dE_du, sums, precision); });
+}
+
 int tcx_graph_begin(tcx_context* ctx)
 {
     return guarded(ctx, [&] {

@@ -178,6 +178,11 @@ void launch_a8_comb2(int prec, void* rec, const PairEntry* pairs, int npairs, co
 void launch_a8_acc2(int prec, const void* rec, const void* occR, const void* Q4T, const PairEntry* pairs,
                     const TargetEntry* targets, int ntargets, double* W, long long N, hipStream_t s);
 
+// kernel-gradient reduction of the occupied-set energies (rec: 1 channel E_x, 8 channels E_2x)
+void launch_kgrad(int prec, const void* rec, int tc2, const PairEntry* pairs, int npairs, const double* kqcart,
+                  const double* fill, const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1, double w0,
+                  int nbins, double width, double* bins_v, double* bins_u, double* sums, hipStream_t s);
+
 // assembly terms (FP64)
 void launch_kinetic(const double* phi, const int* gindex, const OrbEntry* jobs, int njobs, int max_npw,
                     const double* kcart, const Geom& g, double* out, hipStream_t s);

@@ -228,6 +228,7 @@ struct Context {
     void swap_left_right();
     void grad_x(double* grad, double* energy, int prec);
     void grad_tc2(double* grad_left, double* grad_right, double* energy, double* dE_dA, int prec);
+    void kernel_grad(int term, int nbins, double width, double* dE_dK, double* dE_du, double* sums, int prec);
 };
 
 // Dense complex eigen-decomposition A X = X diag(w), A and X n x n row-major,

@@ -41,7 +41,7 @@ EXPORTS = [
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
     "tcx_davidson", "tcx_total_energy", "tcx_eig_dense", "tcx_set_filling",
     "tcx_graph_begin", "tcx_graph_end", "tcx_graph_launch", "tcx_graph_destroy",
-    "tcx_grad_exchange", "tcx_grad_tc2",
+    "tcx_grad_exchange", "tcx_grad_tc2", "tcx_kernel_gradient",
     "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
     "tcx_set_launch_shape", "tcx_enable_timing", "tcx_get_timing",
 ]
@@ -90,6 +90,7 @@ def load_library():
     L.tcx_eig_dense.argtypes = [C.c_int, vp, vp, vp]
     L.tcx_grad_exchange.argtypes = [vp, vp, _dp, C.c_int]
     L.tcx_grad_tc2.argtypes = [vp, vp, vp, _dp, _dp, C.c_int]
+    L.tcx_kernel_gradient.argtypes = [vp, C.c_int, C.c_int, C.c_double, vp, vp, _dp, C.c_int]
     L.tcx_graph_begin.argtypes = [vp]
     L.tcx_graph_end.argtypes = [vp, _ip]
     L.tcx_graph_launch.argtypes = [vp, C.c_int]
@@ -332,6 +333,16 @@ class Engine:
         self._check(self.L.tcx_grad_tc2(self.h, gl.ctypes.data, gr.ctypes.data, _ptr(e), _ptr(da), int(precision)),
                     "grad_tc2")
         return complex(e[0], e[1]), gl, gr, complex(da[0], da[1])
+
+    def kernel_gradient(self, term, nbins, width, precision=PREC_FP64):
+        """(dE/dK bins, dE/du bins or None, sums[4]) -- term "x" (E_x, Coulomb) or "tc2" (E_2 exchange)."""
+        t = {"x": 0, "tc2": 1}[term]
+        gk = np.zeros(nbins, np.complex128)
+        gu = np.zeros(nbins, np.complex128)
+        sums = np.zeros(4)
+        self._check(self.L.tcx_kernel_gradient(self.h, t, int(nbins), float(width), gk.ctypes.data,
+                                               gu.ctypes.data, _ptr(sums), int(precision)), "kernel_gradient")
+        return gk, (gu if t == 1 else None), sums
 
     # --------------------------------------------------------------- HIP graphs
     def graph_begin(self):

@@ -96,3 +96,77 @@ def test_exchange_energy_autograd_function(engine):
     im = torch.tensor(sys_.phi_occ.imag.copy(), dtype=torch.float64, requires_grad=True)
     assert torch.autograd.gradcheck(ExchangeEnergy.apply, (re, im), eps=1e-6, atol=1e-7, rtol=1e-5)
     engine.load_system(sys_)
+
+
+def _half_energy(sys_, bra, h):
+    e, off, b = 0.0, 0, 0
+    for ik in range(sys_.nk):
+        n = sys_.npw[ik]
+        for i in range(sys_.nocc[ik]):
+            e += sys_.filling[b] * np.vdot(bra[off:off + n], h[off:off + n])
+            off += n
+            b += 1
+    return e / (2 * sys_.num_kpoints)
+
+
+def _real_space(sys_, coef):
+    """phi(r) = FFT_backward (unscaled) of the scattered coefficients, per occupied orbital."""
+    n1, n2, n3 = sys_.n
+    po, out, off = sys_.pw_offsets(), [], 0
+    for ik in range(sys_.nk):
+        for i in range(sys_.nocc[ik]):
+            g = np.zeros(sys_.ngrid, np.complex128)
+            g[sys_.gindex[po[ik]:po[ik + 1]]] = coef[off:off + sys_.npw[ik]]
+            off += sys_.npw[ik]
+            out.append((ik, np.fft.ifftn(g.reshape(n3, n2, n1)) * sys_.ngrid))
+    return out
+
+
+def test_exchange_kernel_gradient_bins_and_euler(engine):
+    sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=94)
+    engine.load_system(sys_, div_corr=False)
+    e, _ = engine.grad_exchange()
+    nbins, width = 48, 0.3
+    gk, _, sums = engine.kernel_gradient("x", nbins, width)
+    assert abs(sums[0] - e.real) < 1e-11 * abs(e)              # sum_x v dE/dv = E_x (linear in v)
+    # the same bins from numpy FFTs of the orbitals
+    orbs = _real_space(sys_, sys_.phi_occ)
+    f = sys_.filling
+    cx = -4 * np.pi / (sys_.num_kpoints * sys_.volume) / 2 / (2 * sys_.num_kpoints)
+    n1, n2, n3 = sys_.n
+    m = [np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n) for n in (n1, n2, n3)]
+    M3, M2, M1 = np.meshgrid(m[2], m[1], m[0], indexing="ij")
+    ref = np.zeros(nbins)
+    for a, (ka, pa) in enumerate(orbs):
+        for b, (kb, pb) in enumerate(orbs):
+            rho = np.fft.fftn(np.conj(pb) * pa) / sys_.ngrid
+            kq = (sys_.kfrac[ka] - sys_.kfrac[kb])
+            x = (kq[0] + M1)[..., None] * sys_.recip[0] + (kq[1] + M2)[..., None] * sys_.recip[1] + \
+                (kq[2] + M3)[..., None] * sys_.recip[2]
+            idx = np.minimum(nbins - 1, (np.sqrt(np.sum(x * x, axis=-1)) / width).astype(int))
+            np.add.at(ref, idx.reshape(-1), (cx * f[a] * f[b] * np.abs(rho) ** 2).reshape(-1))
+    assert np.abs(gk.real - ref).max() < 1e-11 * np.abs(ref).max()
+    assert np.abs(gk.imag).max() == 0.0
+
+
+@pytest.mark.parametrize("method", ["TC", "BITC"])
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
+def test_tc2_kernel_gradient_euler_and_dA(engine, method, prec):
+    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=95)
+    chi = sys_.chi_occ if method == "BITC" else sys_.phi_occ
+
+    def e2x(s):
+        engine.load_system(s, div_corr=False)
+        return _half_energy(s, chi, engine.tc2x(s.nocc, s.phi_occ))
+
+    e = e2x(sys_)
+    gk, gu, sums = engine.kernel_gradient("tc2", 64, 0.25, precision=prec)
+    lin, dA = complex(sums[0], sums[1]), complex(sums[2], sums[3])
+    tol = 1e-11 if prec == tcx.PREC_FP64 else 2e-6
+    print(method, prec, e, lin - e, flush=True)
+    assert abs(lin - e) < tol * abs(e)                          # sum_x (V2 dE/dV2 + u dE/du) = E_2x
+    assert abs(gk.sum() + gu.sum()) > 0
+    fdA = (e2x(S.System(**{**sys_.__dict__, "A": sys_.A + H})) -
+           e2x(S.System(**{**sys_.__dict__, "A": sys_.A - H}))) / (2 * H)
+    print("dE/dA", fdA, dA, flush=True)
+    assert abs(fdA - dA) < (1e-8 if prec == tcx.PREC_FP64 else 2e-6) * abs(fdA)
