@@ -89,6 +89,11 @@ def main():
     ap.add_argument("--c3", type=int, default=0, metavar="REPLAYS",
                     help="also run the batched uneven workload (3 grids x 32 k, 8-48 bands per k, hf2x + tc2x) "
                          "as captured HIP graphs for REPLAYS replays vs sequential per-k launches; report 'c3'")
+    ap.add_argument("--c4", action="store_true",
+                    help="also time the gradients of E_x and of the BITC two-body energy on fcc 3x3x3 k, "
+                         "32 bands, 48^3 (FP32-split forward) with a central-difference check; report 'c4'")
+    ap.add_argument("--skip-c1", action="store_true",
+                    help="run only the requested extra legs (--c2/--c3/--c4), e.g. under a profiler")
     ap.add_argument("--c2", action="store_true",
                     help="also time the BITC workload (tc2x + three-body terms, fcc 4x4x4k 16b 40^3) in "
                          "FP32-split and FP64 and report its energy errors under 'c2'")
@@ -103,6 +108,19 @@ def main():
     from tc_amd import energy as E
     from tc_amd import system as S
     from tc_amd import tcx
+
+    if args.skip_c1:
+        eng = tcx.Engine(0)
+        extra = {}
+        if args.c2:
+            extra["c2"] = bench_c2(eng, log)
+        if args.c3:
+            extra["c3"] = bench_c3(log, args.c3)
+        if args.c4:
+            extra["c4"] = bench_c4(eng, log)
+        print(json.dumps(extra), flush=True)
+        eng.close()
+        return
 
     log("bench: building workload fcc 4x4x4k 16b 40^3")
     sys_ = S.fcc_config1()
@@ -260,10 +278,64 @@ def main():
         out["c2"] = bench_c2(eng, log)
     if args.c3:
         out["c3"] = bench_c3(log, args.c3)
+    if args.c4:
+        out["c4"] = bench_c4(eng, log)
     print(json.dumps(out), flush=True)
     for b in (d_phi, d_out32, d_out64, d_scratch):
         b.free()
     eng.close()
+
+
+def bench_c4(eng, log, h=1e-4):
+    """Gradients on the 3x3x3 k, 32-band, 48^3 fcc cell (746,496 pairs per apply): dE_x/d conj(phi)
+    (HF cell) and dE_2/d conj(chi), dE_2/d phi, dE_2/dA (BITC cell), FP32-split forward with FP64
+    accumulation; each checked by a central difference (step h) along one seeded direction, with
+    the energies of the difference in the same precision; plus the kernel-gradient backward pass."""
+    from tc_amd import system as S
+    from tc_amd import tcx
+    res = {}
+    rng = np.random.default_rng(4)
+    for method in ("HF", "BITC"):
+        sys_ = S.fcc_adjoint_config(method)
+        d = rng.standard_normal(sys_.phi_occ.size) + 1j * rng.standard_normal(sys_.phi_occ.size)
+        d /= np.linalg.norm(d)
+        eng.load_system(sys_)
+        prec = tcx.PREC_FP32
+        t0 = time.perf_counter()
+        if method == "HF":
+            e, g = eng.grad_exchange(prec)
+            t_grad = time.perf_counter() - t0
+            an = 2.0 * np.real(np.vdot(g, d))
+            energy = lambda s: (eng.load_system(s), eng.grad_exchange(prec)[0])[1].real
+        else:
+            e, gl, gr, dA = eng.grad_tc2(prec)
+            t_grad = time.perf_counter() - t0
+            an = np.sum(d * gr)  # along phi
+            energy = lambda s: (eng.load_system(s), eng.grad_tc2(prec)[0])[1]
+        log(f"bench c4: {method} gradient {t_grad:.2f} s")
+        ep = energy(S.System(**{**sys_.__dict__, "phi_occ": sys_.phi_occ + h * d}))
+        em = energy(S.System(**{**sys_.__dict__, "phi_occ": sys_.phi_occ - h * d}))
+        fd = (ep - em) / (2 * h)
+        entry = {"pairs_per_apply": int(sys_.nocc.sum()) ** 2, "gradient_s": t_grad,
+                 "energy_Ha": [float(np.real(e)), float(np.imag(e))],
+                 "directional_derivative": {"analytic": [float(np.real(an)), float(np.imag(an))],
+                                            "central_difference": [float(np.real(fd)), float(np.imag(fd))],
+                                            "rel_diff": float(abs(fd - an) / max(abs(an), 1e-300))}}
+        if method == "BITC":
+            eng.load_system(sys_)
+            ea = energy(S.System(**{**sys_.__dict__, "A": sys_.A + h}))
+            eb = energy(S.System(**{**sys_.__dict__, "A": sys_.A - h}))
+            fdA = (ea - eb) / (2 * h)
+            entry["dE_dA"] = {"analytic": [dA.real, dA.imag], "central_difference": [float(np.real(fdA)), float(np.imag(fdA))],
+                              "rel_diff": float(abs(fdA - dA) / max(abs(dA), 1e-300))}
+        eng.load_system(sys_)
+        t0 = time.perf_counter()
+        gk, gu, sums = eng.kernel_gradient("x" if method == "HF" else "tc2", 256, 0.05, prec)
+        entry["kernel_gradient_s"] = time.perf_counter() - t0
+        entry["kernel_gradient_linear_sum"] = [float(sums[0]), float(sums[1])]
+        log(f"bench c4: {method} kernel gradient {entry['kernel_gradient_s']:.2f} s")
+        res[method] = entry
+    return res
 
 
 def bench_c3(log, replays):

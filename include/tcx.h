@@ -210,7 +210,9 @@ int tcx_graph_destroy(tcx_context *ctx, int graph_id);
  * with make_a_new_trial_vector, Gram_Schmidt and sort_eigen
  * (src/diagonalization_utils.cpp:7-29, 79-118, 182-210).  H*phi is tcx_apply_all on
  * the GPU in `precision`; the subspace algebra is FP64 on the host.  Host pointers:
- *   phi      in/out complex128 ragged [k][nb[k]][npw_k]: trial bands in, eigenvectors out;
+ *   phi      in/out complex128 ragged [k][nb[k]][npw_k]: trial bands in (orthonormal per k,
+ *            as the reference's bands are; BITC bands are orthonormalised on entry),
+ *            eigenvectors out;
  *   phi_left out (BITC only, else may be NULL): left eigenvectors V (X^-1)^H;
  *   eig      in/out complex128 [sum nb]: residual shifts in (the previous eigenvalues),
  *            eigenvalues out, ascending real part;

@@ -186,15 +186,20 @@ double norm2(const Vec& a)
 }
 
 // Diagonalization::Gram_Schmidt (diagonalization_utils.cpp:7-29): orthonormalise
-// t against vs[0..target) by classical Gram-Schmidt; false if |t| < 1e-8.
+// t against vs[0..target) by classical Gram-Schmidt; false if |t| < 1e-8 after the
+// first projection (the reference's criterion).  A second projection pass follows
+// ("twice is enough"): identical in exact arithmetic, it keeps the basis orthonormal
+// when a residual is nearly inside the span (a subspace that fills the basis).
 bool gram_schmidt(const std::vector<Vec>& vs, int target, Vec& t)
 {
-    for (int b = 0; b < target; ++b) {
-        const cd c = dotc(vs[b], t);
-        for (size_t l = 0; l < t.size(); ++l) t[l] -= c * vs[b][l];
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int b = 0; b < target; ++b) {
+            const cd c = dotc(vs[b], t);
+            for (size_t l = 0; l < t.size(); ++l) t[l] -= c * vs[b][l];
+        }
+        if (pass == 0 && norm2(t) < 1e-8) return false;
     }
     const double nrm = norm2(t);
-    if (nrm < 1e-8) return false;
     for (cd& x : t) x /= nrm;
     return true;
 }

@@ -260,6 +260,14 @@ def fcc_config2(**kw) -> System:
                         method="BITC", A=A, name="bitc fcc a10.26 4x4x4k 16b 40^3 A0.5", **kw)
 
 
+def fcc_adjoint_config(method="BITC", **kw) -> System:
+    """fcc a=10.26, 3x3x3 k (27 k), 32 bands/k all occupied, 48^3; BITC with A = 0.5 by
+    default (the cell of the gradient workload)."""
+    A = np.full((2, 2), 0.5) if method != "HF" else None
+    return build_system(fcc_lattice(10.26), (48, 48, 48), (3, 3, 3), nbands=32, nocc=32, config=4,
+                        method=method, A=A, name=f"{method.lower()} fcc a10.26 3x3x3k 32b 48^3", **kw)
+
+
 def uneven_bands(sys_: System, counts) -> System:
     """The same cell with only the first counts[k] occupied orbitals at k."""
     counts = np.asarray(counts, dtype=np.int32)

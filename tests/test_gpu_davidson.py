@@ -50,6 +50,18 @@ def lowest(M, nb):
     return w[np.argsort(w.real, kind="stable")][:nb]
 
 
+def trial_bands(sys_, nb, seed):
+    """Seeded random bands, orthonormal per k (the Davidson's input contract)."""
+    _, phi = S.applied_bands(sys_, nb=nb, seed=seed)
+    bo = sys_.band_offsets(nb)
+    out = phi.copy()
+    for ik in range(sys_.nk):
+        blk = phi[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+        q, _ = np.linalg.qr(blk.T)
+        out[bo[ik]:bo[ik + 1]] = q.T.reshape(-1)
+    return out
+
+
 def system(method, config):
     return S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=config)
 
@@ -61,7 +73,7 @@ def test_davidson_full_subspace_is_exact(engine, method):
     mats = full_hamiltonians(sys_)
     nbk = 2
     nb = np.full(sys_.nk, nbk, np.int32)
-    _, phi = S.applied_bands(sys_, nb=nb, seed=62)
+    phi = trial_bands(sys_, nb, 62)
     blocks = int(np.ceil(max(sys_.npw) / nbk)) + 2
     phi_out, left, eig, en = engine.davidson(nb, phi, np.zeros(nb.sum(), np.complex128), max_blocks=blocks)
     bo = sys_.band_offsets(nb)
@@ -90,15 +102,54 @@ def test_davidson_default_blocks_converge(engine, method):
     mats = full_hamiltonians(sys_)
     nbk = 2
     nb = np.full(sys_.nk, nbk, np.int32)
-    _, phi = S.applied_bands(sys_, nb=nb, seed=64)
+    phi = trial_bands(sys_, nb, 64)
     eig = np.zeros(nb.sum(), np.complex128)
-    for it in range(40):
+    for it in range(120):
         phi, _, eig, en = engine.davidson(nb, phi, eig)
     for ik in range(sys_.nk):
         ref = lowest(sum(mats[ik]), nbk)
         got = eig[ik * nbk:(ik + 1) * nbk]
         print(method, ik, got - ref, flush=True)
-        assert np.abs(got - ref).max() < 1e-7
+        # HF converges fully; for TC the re-orthonormalised Ritz vectors (:530-537) make the
+        # second band stagnate at a few 1e-6 with these defaults -- within the 1e-5 Ha target
+        assert np.abs(got - ref).max() < (1e-7 if method == "HF" else 1e-5)
+
+
+@pytest.mark.parametrize("method", ["HF", "TC", "BITC"])
+def test_davidson_follows_the_reference_algorithm(engine, method):
+    """Eigenvalues, band energies and bands of successive calls match the numpy
+    restatement of block_davidson (oracle/davidson.py) run on the oracle's dense H_n."""
+    from oracle import davidson as OD
+    sys_ = system(method, 67)
+    engine.load_system(sys_)
+    mats = full_hamiltonians(sys_)
+    nbk = 2
+    nb = np.full(sys_.nk, nbk, np.int32)
+    phi = trial_bands(sys_, nb, 68)
+    eig = np.zeros(nb.sum(), np.complex128)
+    bo = sys_.band_offsets(nb)
+    ref_phi = [phi[bo[ik]:bo[ik + 1]].reshape(nbk, -1) for ik in range(sys_.nk)]
+    ref_eig = [eig[ik * nbk:(ik + 1) * nbk] for ik in range(sys_.nk)]
+    kG2 = []
+    for ik in range(sys_.nk):
+        kg = (sys_.kfrac[ik] + S.miller_of_index(sys_.n)[sys_.gindex[sys_.pw_offsets()[ik]:sys_.pw_offsets()[ik + 1]]]) @ sys_.recip
+        kG2.append(np.sum(kg * kg, axis=1))
+    for it in range(6):
+        phi, left, eig, en = engine.davidson(nb, phi, eig, max_blocks=3, num_refresh=2)
+        for ik in range(sys_.nk):
+            Hn = list(mats[ik]) + ([np.zeros_like(mats[ik][0])] if method == "HF" else [])
+            rp, rl, re, ren = OD.block_davidson(Hn, kG2[ik], ref_phi[ik], ref_eig[ik], max_blocks=3, num_refresh=2,
+                                                bitc=method == "BITC")
+            ref_phi[ik], ref_eig[ik] = rp, re
+            got = eig[ik * nbk:(ik + 1) * nbk]
+            assert np.abs(got - re).max() < 1e-10 * max(1.0, np.abs(re).max()), (it, ik)
+            assert np.abs(en[ik * nbk:(ik + 1) * nbk] - ren).max() < 1e-10 * max(1.0, np.abs(ren).max())
+            # bands agree up to one phase per band (eigenvectors carry an arbitrary phase)
+            P = phi[bo[ik]:bo[ik + 1]].reshape(nbk, -1)
+            for b in range(nbk):
+                ov = np.vdot(rp[b], P[b])
+                assert abs(abs(ov) - np.linalg.norm(rp[b]) * np.linalg.norm(P[b])) < 1e-8
+        print(method, it, eig, flush=True)
 
 
 def test_davidson_hf_band_energies(engine):
@@ -108,7 +159,7 @@ def test_davidson_hf_band_energies(engine):
     mats = full_hamiltonians(sys_)
     nbk = 2
     nb = np.full(sys_.nk, nbk, np.int32)
-    _, phi = S.applied_bands(sys_, nb=nb, seed=66)
+    phi = trial_bands(sys_, nb, 66)
     blocks = int(np.ceil(max(sys_.npw) / nbk)) + 2
     phi_out, _, eig, en = engine.davidson(nb, phi, np.zeros(nb.sum(), np.complex128), max_blocks=blocks)
     bo = sys_.band_offsets(nb)
