@@ -2,15 +2,19 @@
 // eigensolver.
 //
 // Diagonalization::block_davidson (src/diagonalization_davidson.cpp:7-689)
-// at fixed occupied orbitals: H*phi comes from calc_hamiltonian::all on the
-// GPU (Context::apply_all); the subspace algebra (Gram-Schmidt, the Payne
-// preconditioner, V^H H V, the rotation V X) is O(dim^2 npw) per k and runs on
-// the host, as the reference runs it on MPI rank 0.  The complex non-Hermitian
-// subspace problem (Eigen::ComplexEigenSolver in the reference, :480-482) is
-// solved by eig_dense: Householder reduction to Hessenberg form, single-shift
-// QR with Wilkinson shifts to the complex Schur form (Golub & Van Loan,
-// Matrix Computations, Alg. 7.4.2 / 7.5), and back-substitution for the
-// eigenvectors of the triangular factor.
+// at fixed occupied orbitals.  Everything stays in HBM: H*phi comes from
+// calc_hamiltonian::all (Context::apply_all on device pointers), and the FP64
+// subspace algebra -- the Payne preconditioner, Gram-Schmidt, V^H H_n V, the
+// batched non-Hermitian eigensolve of every k, the rotations V X and V X_L --
+// runs in the kernels of kernels_davidson.hip, batched over k.  The host only
+// sequences the launches and reads back one norm per k per Gram-Schmidt step
+// (the reference's 1e-8 drop decides which candidates enter the subspace).
+//
+// eig_dense is the host version of the subspace eigensolver behind
+// tcx_eig_dense: Householder reduction to Hessenberg form, single-shift QR with
+// Wilkinson shifts to the complex Schur form (Golub & Van Loan, Matrix
+// Computations, Alg. 7.4.2 / 7.5), and back-substitution for the eigenvectors
+// of the triangular factor; the device kernel follows the same steps.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -167,117 +171,19 @@ bool eig_dense(int n, const cd* A_in, cd* w, cd* X)
     return true;
 }
 
+
 namespace {
 
-using Vec = std::vector<cd>;
-
-cd dotc(const Vec& a, const Vec& b)
-{
-    cd s = 0.0;
-    for (size_t l = 0; l < a.size(); ++l) s += std::conj(a[l]) * b[l];
-    return s;
-}
-
-double norm2(const Vec& a)
-{
-    double s = 0.0;
-    for (const cd& x : a) s += std::norm(x);
-    return std::sqrt(s);
-}
-
-// Diagonalization::Gram_Schmidt (diagonalization_utils.cpp:7-29): orthonormalise
-// t against vs[0..target) by classical Gram-Schmidt; false if |t| < 1e-8 after the
-// first projection (the reference's criterion).  A second projection pass follows
-// ("twice is enough"): identical in exact arithmetic, it keeps the basis orthonormal
-// when a residual is nearly inside the span (a subspace that fills the basis).
-bool gram_schmidt(const std::vector<Vec>& vs, int target, Vec& t)
-{
-    for (int pass = 0; pass < 2; ++pass) {
-        for (int b = 0; b < target; ++b) {
-            const cd c = dotc(vs[b], t);
-            for (size_t l = 0; l < t.size(); ++l) t[l] -= c * vs[b][l];
-        }
-        if (pass == 0 && norm2(t) < 1e-8) return false;
-    }
-    const double nrm = norm2(t);
-    for (cd& x : t) x /= nrm;
-    return true;
-}
-
-// make_a_new_trial_vector (diagonalization_utils.cpp:182-210): Payne preconditioner on
-// the residual H phi - eps phi, normalised (zero if its norm is below 1e-8).
-Vec new_trial_vector(const std::vector<double>& kG2, const Vec& phi, const cd* Hphi, cd eps)
-{
-    double kin = 0.0, pn = 0.0;
-    for (size_t l = 0; l < phi.size(); ++l) {
-        kin += kG2[l] * std::norm(phi[l]);
-        pn += std::norm(phi[l]);
-    }
-    kin /= pn;
-    Vec t(phi.size());
-    for (size_t l = 0; l < phi.size(); ++l) {
-        const double x = kG2[l] / kin;
-        const double c = (27 + x * (18 + x * (12 + x * 8))) / (27 + x * (18 + x * (12 + x * (8 + x * 16))));
-        t[l] = -c * (Hphi[l] - eps * phi[l]);
-    }
-    const double nrm = norm2(t);
-    if (nrm < 1e-8)
-        std::fill(t.begin(), t.end(), cd(0.0));
-    else
-        for (cd& x : t) x /= nrm;
-    return t;
-}
-
-// sort_eigen (diagonalization_utils.cpp:79-118): ascending real part, same swap order
-void sort_eigen(int n, std::vector<cd>& w, std::vector<cd>& X)
-{
-    for (int i = 0; i < n; ++i)
-        for (int j = i + 1; j < n; ++j)
-            if (w[i].real() > w[j].real()) {
-                std::swap(w[i], w[j]);
-                for (int r = 0; r < n; ++r) std::swap(X[(size_t)r * n + i], X[(size_t)r * n + j]);
-            }
-}
-
-// inverse of a small complex matrix (Gauss-Jordan, partial pivoting), row-major
-bool invert(int n, std::vector<cd>& A)
-{
-    std::vector<cd> I((size_t)n * n, cd(0.0));
-    for (int i = 0; i < n; ++i) I[(size_t)i * n + i] = 1.0;
-    for (int c = 0; c < n; ++c) {
-        int p = c;
-        for (int r = c + 1; r < n; ++r)
-            if (std::abs(A[(size_t)r * n + c]) > std::abs(A[(size_t)p * n + c])) p = r;
-        if (std::abs(A[(size_t)p * n + c]) == 0.0) return false;
-        if (p != c)
-            for (int j = 0; j < n; ++j) {
-                std::swap(A[(size_t)p * n + j], A[(size_t)c * n + j]);
-                std::swap(I[(size_t)p * n + j], I[(size_t)c * n + j]);
-            }
-        const cd piv = A[(size_t)c * n + c];
-        for (int j = 0; j < n; ++j) {
-            A[(size_t)c * n + j] /= piv;
-            I[(size_t)c * n + j] /= piv;
-        }
-        for (int r = 0; r < n; ++r) {
-            if (r == c) continue;
-            const cd f = A[(size_t)r * n + c];
-            if (f == cd(0.0)) continue;
-            for (int j = 0; j < n; ++j) {
-                A[(size_t)r * n + j] -= f * A[(size_t)c * n + j];
-                I[(size_t)r * n + j] -= f * I[(size_t)c * n + j];
-            }
-        }
-    }
-    A.swap(I);
-    return true;
-}
+// Device layout of one Davidson call: int and long long tables of nk entries each.
+enum IntTab { kNpw, kBand0, kNb, kFirst, kCnt, kSize, kNA, kBidx, kOdst, kStatus, kNumIntTabs };
+enum OffTab { kVoff, kPoff, kGoff, kCoff, kNumOffTabs };
 
 }  // namespace
 
 void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double* eig_d, double* energies_d,
                        int max_blocks, int num_refresh, int biortho, int prec)
 {
+    require(!g_capturing, TCX_ERR_STATE, "graph capture: davidson is not capturable");
     require(states_set, TCX_ERR_STATE, "davidson: call tcx_set_states first");
     require(nb_ && phi_d && eig_d && energies_d, TCX_ERR_ARG, "davidson: null pointer");
     require(max_blocks >= 2 && num_refresh >= 1, TCX_ERR_ARG, "davidson: need max_blocks >= 2, num_refresh >= 1");
@@ -286,20 +192,36 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
     require(!bitc || phi_left_d, TCX_ERR_ARG, "davidson: BITC needs phi_left");
     if (uses3) require(jastrow_set, TCX_ERR_STATE, "davidson: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);
-    for (int k = 0; k < nk; ++k) require(nb[k] >= 0, TCX_ERR_ARG, "davidson: negative band count");
-    cd* phi = reinterpret_cast<cd*>(phi_d);
-    cd* phi_left = reinterpret_cast<cd*>(phi_left_d);
-    cd* eig = reinterpret_cast<cd*>(eig_d);
-    cd* energies = reinterpret_cast<cd*>(energies_d);
-    std::vector<long long> coef(nk + 1, 0), band(nk + 1, 0);
+    int max_nb = 0, max_npw = 0;
     for (int k = 0; k < nk; ++k) {
-        coef[k + 1] = coef[k] + (long long)nb[k] * npw[k];
-        band[k + 1] = band[k] + nb[k];
+        require(nb[k] >= 0 && nb[k] <= npw[k], TCX_ERR_ARG, "davidson: band count outside [0, npw]");
+        max_nb = std::max(max_nb, nb[k]);
+        max_npw = std::max(max_npw, npw[k]);
     }
-    // |k+G|^2 on the sphere of every k
-    std::vector<std::vector<double>> kG2(nk);
+    if (max_nb == 0) return;
+    const int gs = max_blocks * max_nb;  // subspace capacity per k (rows of the small matrices)
+    const int nH = uses3 ? 3 : 2;
+
+    // tables
+    std::vector<int> I((size_t)kNumIntTabs * nk, 0);
+    std::vector<long long> L((size_t)kNumOffTabs * nk, 0);
+    auto it = [&](IntTab t) { return I.data() + (size_t)t * nk; };
+    long long vtot = 0, ptot = 0;
     for (int k = 0; k < nk; ++k) {
-        kG2[k].resize(npw[k]);
+        it(kNpw)[k] = npw[k];
+        it(kBand0)[k] = k ? it(kBand0)[k - 1] + nb[k - 1] : 0;
+        it(kNb)[k] = nb[k];
+        L[kVoff * nk + k] = vtot;
+        L[kPoff * nk + k] = ptot;
+        L[kGoff * nk + k] = pw_off[k];
+        vtot += (long long)max_blocks * nb[k] * npw[k];
+        ptot += (long long)nb[k] * npw[k];
+    }
+    const int nbands = it(kBand0)[nk - 1] + nb[nk - 1];
+
+    // |k+G|^2 on the sphere of every k
+    std::vector<double> kG2(std::max<long long>(1, pw_off[nk]));
+    for (int k = 0; k < nk; ++k)
         for (int l = 0; l < npw[k]; ++l) {
             const int e = gindex_h[pw_off[k] + l];
             const int i1 = e % geo.n1, i2 = (e / geo.n1) % geo.n2, i3 = e / (geo.n1 * geo.n2);
@@ -310,125 +232,187 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
                 const double x = kcart[3 * k + d] + geo.b[d] * m[0] + geo.b[3 + d] * m[1] + geo.b[6 + d] * m[2];
                 s += x * x;
             }
-            kG2[k][l] = s;
-        }
-    }
-    const std::vector<cd> eig_ref(eig, eig + band[nk]);  // bloch_states.eigenvalues (residual shifts)
-
-    // trial vectors V (initialised by phi, :48-65; orthonormalised for BITC)
-    std::vector<std::vector<Vec>> V(nk);
-    for (int k = 0; k < nk; ++k)
-        for (int b = 0; b < nb[k]; ++b) {
-            Vec v(phi + coef[k] + (long long)b * npw[k], phi + coef[k] + (long long)(b + 1) * npw[k]);
-            if (bitc) require(gram_schmidt(V[k], b, v), TCX_ERR_INTERNAL, "davidson: linear independence breaks");
-            V[k].push_back(std::move(v));
+            kG2[pw_off[k] + l] = s;
         }
 
-    // H_n applied to the last `cnt[k]` vectors of V[k]; appends to HnV
-    std::vector<std::vector<Vec>> HV[3];
-    auto apply_block = [&](const std::vector<int>& cnt) {
-        std::vector<long long> off(nk + 1, 0);
-        for (int k = 0; k < nk; ++k) off[k + 1] = off[k] + (long long)cnt[k] * npw[k];
-        std::vector<cd> in(std::max<long long>(1, off[nk])), h[3];
-        for (int n = 0; n < 3; ++n) h[n].assign(std::max<long long>(1, off[nk]), cd(0.0));
-        for (int k = 0; k < nk; ++k) {
-            const int first = (int)V[k].size() - cnt[k];
-            for (int b = 0; b < cnt[k]; ++b)
-                std::copy(V[k][first + b].begin(), V[k][first + b].end(), in.begin() + off[k] + (long long)b * npw[k]);
-        }
-        apply_all(cnt.data(), reinterpret_cast<const double*>(in.data()), reinterpret_cast<double*>(h[0].data()),
-                  reinterpret_cast<double*>(h[1].data()), uses3 ? reinterpret_cast<double*>(h[2].data()) : nullptr,
-                  prec, 0);
-        for (int n = 0; n < 3; ++n)
+    // device memory of this call (vtot, ptot coefficients of 16 bytes)
+    DBuf dV[2], dHV[3], dP, dT, dIn, dO[3], dkG2, deig, dHs, dX, dXL, dw, den, dc, dnrm, dI, dL, dwork;
+    const size_t vb = (size_t)std::max<long long>(1, vtot) * 16, pb = (size_t)std::max<long long>(1, ptot) * 16;
+    const size_t mb = (size_t)nk * gs * gs * 16;
+    for (auto& b : dV) b.ensure(vb);
+    for (int n = 0; n < nH; ++n) dHV[n].ensure(vb);
+    for (DBuf* b : {&dP, &dT, &dIn, &dO[0], &dO[1]}) b->ensure(pb);
+    if (uses3) dO[2].ensure(pb);
+    dkG2.ensure(kG2.size() * 8);
+    deig.ensure((size_t)nbands * 16);
+    dHs.ensure(mb * nH);
+    dX.ensure(mb);
+    if (bitc) dXL.ensure(mb);
+    dw.ensure((size_t)nk * gs * 16);
+    den.ensure((size_t)nbands * 3 * 16);
+    dc.ensure((size_t)nk * gs * 16);
+    dnrm.ensure((size_t)nk * 8);
+    dI.ensure(I.size() * 4);
+    dL.ensure(L.size() * 8);
+    if (!dv_eig_fits_lds(gs)) dwork.ensure(dv_eig_work_bytes(gs, nk));
+
+    // copies on the context stream, waited for (host buffers are reused right after)
+    auto h2d = [&](void* dst, const void* src, size_t bytes) {
+        TCX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+        TCX_HIP_CHECK(hipStreamSynchronize(stream));
+    };
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        TCX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+        TCX_HIP_CHECK(hipStreamSynchronize(stream));
+    };
+    int* dIp = (int*)dI.p;
+    const long long* dLp = (const long long*)dL.p;
+    auto di = [&](IntTab t) { return dIp + (size_t)t * nk; };
+    auto dl = [&](OffTab t) { return dLp + (size_t)t * nk; };
+    auto push_ints = [&](IntTab t0, int ntabs) { h2d(di(t0), it(t0), (size_t)ntabs * nk * 4); };
+    auto d64 = [](DBuf& b) { return (double*)b.p; };
+
+    h2d(dI.p, I.data(), I.size() * 4);
+    h2d(dL.p, L.data(), L.size() * 8);
+    h2d(dkG2.p, kG2.data(), kG2.size() * 8);
+    h2d(deig.p, eig_d, (size_t)nbands * 16);  // bloch_states.eigenvalues shift the residuals
+    h2d(dP.p, phi_d, (size_t)ptot * 16);
+
+    // Gram-Schmidt of slots 0..nb[k] of B in place, in band order; a vector whose first
+    // projection leaves a norm below 1e-8 is an error (diagonalization_davidson.cpp:527-545)
+    auto orthonormalise = [&](double* B) {
+        for (int b = 0; b < max_nb; ++b) {
+            for (int k = 0; k < nk; ++k) {
+                const bool on = b < nb[k];
+                it(kNA)[k] = on ? b : 0;
+                it(kBidx)[k] = on ? b : -1;
+                it(kOdst)[k] = on ? b : -1;
+            }
+            push_ints(kNA, 3);
+            launch_dv_project(B, dl(kVoff), di(kNA), b, B, dl(kVoff), di(kBidx), di(kNpw), max_npw, nk, d64(dc),
+                              d64(dnrm), stream);
+            std::vector<double> nrm(nk);
+            d2h(nrm.data(), dnrm.p, (size_t)nk * 8);
             for (int k = 0; k < nk; ++k)
-                for (int b = 0; b < cnt[k]; ++b) {
-                    const cd* src = h[n].data() + off[k] + (long long)b * npw[k];
-                    HV[n][k].emplace_back(src, src + npw[k]);
-                }
+                require(it(kBidx)[k] < 0 || nrm[k] >= 1e-8, TCX_ERR_INTERNAL, "davidson: linear independence breaks");
+            launch_dv_project(B, dl(kVoff), di(kNA), b, B, dl(kVoff), di(kBidx), di(kNpw), max_npw, nk, d64(dc),
+                              d64(dnrm), stream);
+            launch_dv_store(B, dl(kVoff), di(kBidx), d64(dnrm), B, dl(kVoff), di(kOdst), di(kNpw), max_npw, nk,
+                            stream);
+        }
     };
 
-    for (int it = 0; it < num_refresh; ++it) {
-        const bool final_loop = it == num_refresh - 1;
-        for (int n = 0; n < 3; ++n) HV[n].assign(nk, {});
-        std::vector<int> cnt = nb;
-        for (int blk = 0; blk < max_blocks - 1; ++blk) {  // grow the subspace (:190-365)
-            apply_block(cnt);
-            for (int k = 0; k < nk; ++k) {
-                const int nold = (int)V[k].size();
-                const int first = nold - cnt[k];
-                for (int b = 0; b < cnt[k]; ++b) {
-                    std::vector<cd> hsum(npw[k]);
-                    for (int l = 0; l < npw[k]; ++l)
-                        hsum[l] = HV[0][k][first + b][l] + HV[1][k][first + b][l] + HV[2][k][first + b][l];
-                    Vec t = new_trial_vector(kG2[k], V[k][first + b], hsum.data(), eig_ref[band[k] + b]);
-                    if (gram_schmidt(V[k], (int)V[k].size(), t)) V[k].push_back(std::move(t));
-                }
-                cnt[k] = (int)V[k].size() - nold;
-            }
-        }
-        apply_block(cnt);  // remaining matrix elements of H V (:408-437)
+    int cur = 0;
+    std::vector<int> size(nk);
+    // trial vectors V (initialised by phi, :48-65; orthonormalised for BITC)
+    launch_dv_copy(d64(dP), dl(kPoff), nullptr, d64(dV[cur]), dl(kVoff), nullptr, di(kNb), max_nb, di(kNpw), max_npw,
+                   nk, stream);
+    if (bitc) orthonormalise(d64(dV[cur]));
+    size = nb;
 
-        for (int k = 0; k < nk; ++k) {  // subspace problem (:438-640)
-            const int dim = (int)V[k].size(), nbk = nb[k];
-            if (nbk == 0) continue;
-            std::vector<cd> Hs[3], H((size_t)dim * dim, cd(0.0));
-            for (int n = 0; n < 3; ++n) {
-                Hs[n].assign((size_t)dim * dim, cd(0.0));
-                if (n == 2 && !uses3) continue;
-                for (int i = 0; i < dim; ++i)
-                    for (int j = 0; j < dim; ++j) Hs[n][(size_t)i * dim + j] = dotc(V[k][i], HV[n][k][j]);
+    // H_n applied to V slots first[k] .. + cnt[k], into the same slots of the H_n V sets
+    auto apply_block = [&]() {
+        long long c = 0;
+        int max_cnt = 0;
+        for (int k = 0; k < nk; ++k) {
+            L[kCoff * nk + k] = c;
+            c += (long long)it(kCnt)[k] * npw[k];
+            max_cnt = std::max(max_cnt, it(kCnt)[k]);
+        }
+        h2d((void*)dl(kCoff), L.data() + (size_t)kCoff * nk, (size_t)nk * 8);
+        push_ints(kFirst, 2);
+        launch_dv_copy(d64(dV[cur]), dl(kVoff), di(kFirst), d64(dIn), dl(kCoff), nullptr, di(kCnt), max_cnt, di(kNpw),
+                       max_npw, nk, stream);
+        apply_all(it(kCnt), d64(dIn), d64(dO[0]), d64(dO[1]), uses3 ? d64(dO[2]) : nullptr, prec,
+                  TCX_FLAG_DEVICE_PTRS);
+        for (int n = 0; n < nH; ++n)
+            launch_dv_copy(d64(dO[n]), dl(kCoff), nullptr, d64(dHV[n]), dl(kVoff), di(kFirst), di(kCnt), max_cnt,
+                           di(kNpw), max_npw, nk, stream);
+    };
+
+    for (int iter = 0; iter < num_refresh; ++iter) {
+        const bool final_loop = iter == num_refresh - 1;
+        for (int k = 0; k < nk; ++k) it(kCnt)[k] = nb[k];
+        for (int blk = 0; blk < max_blocks - 1; ++blk) {  // grow the subspace (:190-365)
+            int max_cnt = 0;
+            for (int k = 0; k < nk; ++k) {
+                it(kFirst)[k] = size[k] - it(kCnt)[k];
+                max_cnt = std::max(max_cnt, it(kCnt)[k]);
             }
-            for (size_t e = 0; e < H.size(); ++e) H[e] = Hs[0][e] + Hs[1][e] + Hs[2][e];
-            std::vector<cd> w(dim), X((size_t)dim * dim), XL;
-            require(eig_dense(dim, H.data(), w.data(), X.data()), TCX_ERR_INTERNAL,
-                    "davidson: diagonalization of H_subspace failed");
-            sort_eigen(dim, w, X);
-            if (bitc) {  // X_L = (X^-1)^H (:498)
-                std::vector<cd> Xi = X;
-                require(invert(dim, Xi), TCX_ERR_INTERNAL, "davidson: singular eigenvector matrix");
-                XL.assign((size_t)dim * dim, cd(0.0));
-                for (int i = 0; i < dim; ++i)
-                    for (int j = 0; j < dim; ++j) XL[(size_t)i * dim + j] = std::conj(Xi[(size_t)j * dim + i]);
-            }
-            auto rotate = [&](const std::vector<cd>& C) {
-                std::vector<Vec> out(nbk, Vec(npw[k], cd(0.0)));
-                for (int b = 0; b < nbk; ++b)
-                    for (int i = 0; i < dim; ++i) {
-                        const cd c = C[(size_t)i * dim + b];
-                        for (int l = 0; l < npw[k]; ++l) out[b][l] += c * V[k][i][l];
-                    }
-                return out;
-            };
-            std::vector<Vec> Vt = rotate(X);
-            if (!(bitc && final_loop)) {
-                for (int b = 0; b < nbk; ++b)
-                    require(gram_schmidt(Vt, b, Vt[b]), TCX_ERR_INTERNAL, "davidson: linear independence breaks");
-            }
-            if (final_loop) {
-                for (int b = 0; b < nbk; ++b) std::copy(Vt[b].begin(), Vt[b].end(), phi + coef[k] + (long long)b * npw[k]);
-                if (bitc) {
-                    std::vector<Vec> Vl = rotate(XL);
-                    for (int b = 0; b < nbk; ++b)
-                        std::copy(Vl[b].begin(), Vl[b].end(), phi_left + coef[k] + (long long)b * npw[k]);
+            apply_block();
+            launch_dv_precond(d64(dV[cur]), dl(kVoff), di(kFirst), di(kCnt), max_cnt, d64(dHV[0]), d64(dHV[1]),
+                              uses3 ? d64(dHV[2]) : nullptr, d64(dkG2), dl(kGoff), d64(deig), di(kBand0), d64(dT),
+                              dl(kPoff), di(kNpw), nk, stream);
+            const std::vector<int> nold = size;
+            for (int b = 0; b < max_cnt; ++b) {  // candidates in band order, each against the grown V
+                int max_nA = 0;
+                for (int k = 0; k < nk; ++k) {
+                    it(kNA)[k] = size[k];
+                    it(kBidx)[k] = b < it(kCnt)[k] ? b : -1;
+                    it(kOdst)[k] = size[k];
+                    max_nA = std::max(max_nA, size[k]);
                 }
-                // eigenvalues and e_n = diag(X_L^H H_n X) (X_L = X unless BITC, :571-585)
-                const std::vector<cd>& L = bitc ? XL : X;
-                for (int b = 0; b < nbk; ++b) {
-                    eig[band[k] + b] = w[b];
-                    for (int n = 0; n < 3; ++n) {
-                        cd e = 0.0;
-                        for (int i = 0; i < dim; ++i) {
-                            cd s = 0.0;
-                            for (int j = 0; j < dim; ++j) s += Hs[n][(size_t)i * dim + j] * X[(size_t)j * dim + b];
-                            e += std::conj(L[(size_t)i * dim + b]) * s;
-                        }
-                        energies[(band[k] + b) * 3 + n] = e;
-                    }
+                push_ints(kNA, 3);
+                launch_dv_project(d64(dV[cur]), dl(kVoff), di(kNA), max_nA, d64(dT), dl(kPoff), di(kBidx), di(kNpw),
+                                  max_npw, nk, d64(dc), d64(dnrm), stream);
+                std::vector<double> nrm(nk);
+                d2h(nrm.data(), dnrm.p, (size_t)nk * 8);
+                bool any = false;
+                for (int k = 0; k < nk; ++k) {  // Gram_Schmidt's 1e-8 drop (diagonalization_utils.cpp:7-29)
+                    if (it(kBidx)[k] >= 0 && !(nrm[k] >= 1e-8)) it(kBidx)[k] = -1;
+                    any = any || it(kBidx)[k] >= 0;
                 }
-            } else {
-                V[k].resize(nbk);
-                for (int b = 0; b < nbk; ++b) V[k][b] = Vt[b];
+                if (!any) continue;
+                push_ints(kBidx, 1);
+                launch_dv_project(d64(dV[cur]), dl(kVoff), di(kNA), max_nA, d64(dT), dl(kPoff), di(kBidx), di(kNpw),
+                                  max_npw, nk, d64(dc), d64(dnrm), stream);
+                launch_dv_store(d64(dT), dl(kPoff), di(kBidx), d64(dnrm), d64(dV[cur]), dl(kVoff), di(kOdst),
+                                di(kNpw), max_npw, nk, stream);
+                for (int k = 0; k < nk; ++k)
+                    if (it(kBidx)[k] >= 0) ++size[k];
             }
+            for (int k = 0; k < nk; ++k) it(kCnt)[k] = size[k] - nold[k];
+        }
+        for (int k = 0; k < nk; ++k) it(kFirst)[k] = size[k] - it(kCnt)[k];
+        apply_block();  // remaining matrix elements of H V (:408-437)
+
+        // subspace problem (:438-640)
+        for (int k = 0; k < nk; ++k) it(kSize)[k] = size[k];
+        push_ints(kSize, 1);
+        launch_dv_gram(d64(dV[cur]), dl(kVoff), di(kSize), d64(dHV[0]), d64(dHV[1]), uses3 ? d64(dHV[2]) : nullptr,
+                       nH, di(kNpw), nk, d64(dHs), gs, stream);
+        launch_dv_eig(d64(dHs), nH, di(kSize), gs, nk, bitc && final_loop, d64(dw), d64(dX), bitc ? d64(dXL) : nullptr,
+                      dwork.p ? d64(dwork) : nullptr, di(kNb), di(kBand0), final_loop ? d64(den) : nullptr,
+                      di(kStatus), stream);
+        d2h(it(kStatus), di(kStatus), (size_t)nk * 4);
+        for (int k = 0; k < nk; ++k) {
+            require(it(kStatus)[k] != 1, TCX_ERR_INTERNAL, "davidson: diagonalization of H_subspace failed");
+            require(it(kStatus)[k] != 2, TCX_ERR_INTERNAL, "davidson: singular eigenvector matrix");
+        }
+        const int alt = 1 - cur;
+        launch_dv_rotate(d64(dV[cur]), dl(kVoff), di(kSize), d64(dX), gs, d64(dV[alt]), dl(kVoff), di(kNb), di(kNpw),
+                         max_npw, nk, stream);
+        if (!(bitc && final_loop)) orthonormalise(d64(dV[alt]));
+        if (final_loop) {
+            launch_dv_copy(d64(dV[alt]), dl(kVoff), nullptr, d64(dP), dl(kPoff), nullptr, di(kNb), max_nb, di(kNpw),
+                           max_npw, nk, stream);
+            d2h(phi_d, dP.p, (size_t)ptot * 16);
+            if (bitc) {
+                launch_dv_rotate(d64(dV[cur]), dl(kVoff), di(kSize), d64(dXL), gs, d64(dT), dl(kPoff), di(kNb),
+                                 di(kNpw), max_npw, nk, stream);
+                d2h(phi_left_d, dT.p, (size_t)ptot * 16);
+            }
+            std::vector<double> w((size_t)nk * gs * 2);
+            d2h(w.data(), dw.p, w.size() * 8);
+            for (int k = 0; k < nk; ++k)
+                for (int b = 0; b < nb[k]; ++b) {
+                    eig_d[2 * (it(kBand0)[k] + b)] = w[2 * ((size_t)k * gs + b)];
+                    eig_d[2 * (it(kBand0)[k] + b) + 1] = w[2 * ((size_t)k * gs + b) + 1];
+                }
+            d2h(energies_d, den.p, (size_t)nbands * 3 * 16);
+        } else {
+            cur = alt;
+            size = nb;
         }
     }
 }

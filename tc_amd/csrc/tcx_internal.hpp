@@ -195,4 +195,25 @@ void launch_tc2h_apply(double* orb, const double* grad, const double* V2, const 
                        hipStream_t s);
 void launch_axpby(const double* X, const double* Y, double* out, double a, double b, long long N, hipStream_t s);
 
+// block Davidson algebra (FP64, batched over k; vector i of k at base + 2 (off[k] + i npw[k]))
+void launch_dv_precond(const double* V, const long long* voff, const int* first, const int* cnt, int max_cnt,
+                       const double* H0, const double* H1, const double* H2, const double* kG2, const long long* goff,
+                       const double* eig, const int* band0, double* T, const long long* toff, const int* npw, int nk,
+                       hipStream_t s);
+void launch_dv_project(const double* A, const long long* aoff, const int* nA, int max_nA, double* B,
+                       const long long* boff, const int* bidx, const int* npw, int max_npw, int nk, double* c,
+                       double* nrm, hipStream_t s);
+void launch_dv_store(const double* B, const long long* boff, const int* bidx, const double* nrm, double* O,
+                     const long long* ooff, const int* odst, const int* npw, int max_npw, int nk, hipStream_t s);
+void launch_dv_copy(const double* S, const long long* soff, const int* sfirst, double* D, const long long* doff,
+                    const int* dfirst, const int* cnt, int max_cnt, const int* npw, int max_npw, int nk, hipStream_t s);
+void launch_dv_gram(const double* A, const long long* aoff, const int* nA, const double* B0, const double* B1,
+                    const double* B2, int nB, const int* npw, int nk, double* G, int gs, hipStream_t s);
+void launch_dv_rotate(const double* A, const long long* aoff, const int* nA, const double* C, int gs, double* O,
+                      const long long* ooff, const int* nout, const int* npw, int max_npw, int nk, hipStream_t s);
+size_t dv_eig_work_bytes(int gs, int nk);
+bool dv_eig_fits_lds(int gs);
+void launch_dv_eig(const double* Hs, int nH, const int* dim, int gs, int nk, int want_left, double* w, double* X,
+                   double* XL, double* gwork, const int* nb, const int* band0, double* en, int* status, hipStream_t s);
+
 }  // namespace tcx
