@@ -94,6 +94,9 @@ def main():
                          "32 bands, 48^3 (FP32-split forward) with a central-difference check; report 'c4'")
     ap.add_argument("--skip-c1", action="store_true",
                     help="run only the requested extra legs (--c2/--c3/--c4), e.g. under a profiler")
+    ap.add_argument("--davidson", action="store_true",
+                    help="also time tcx_davidson (2 blocks, 1 refresh, FP32-split pair terms) on the HF "
+                         "4x4x4 k, 16-band, 40^3 cell against one calc_hamiltonian::all apply; report 'davidson'")
     ap.add_argument("--c2", action="store_true",
                     help="also time the BITC workload (tc2x + three-body terms, fcc 4x4x4k 16b 40^3) in "
                          "FP32-split and FP64 and report its energy errors under 'c2'")
@@ -118,6 +121,8 @@ def main():
             extra["c3"] = bench_c3(log, args.c3)
         if args.c4:
             extra["c4"] = bench_c4(eng, log)
+        if args.davidson:
+            extra["davidson"] = bench_davidson(eng, log)
         print(json.dumps(extra), flush=True)
         eng.close()
         return
@@ -280,17 +285,20 @@ def main():
         out["c3"] = bench_c3(log, args.c3)
     if args.c4:
         out["c4"] = bench_c4(eng, log)
+    if args.davidson:
+        out["davidson"] = bench_davidson(eng, log)
     print(json.dumps(out), flush=True)
     for b in (d_phi, d_out32, d_out64, d_scratch):
         b.free()
     eng.close()
 
 
-def bench_c4(eng, log, h=1e-4):
+def bench_c4(eng, log, h=1e-3):
     """Gradients on the 3x3x3 k, 32-band, 48^3 fcc cell (746,496 pairs per apply): dE_x/d conj(phi)
     (HF cell) and dE_2/d conj(chi), dE_2/d phi, dE_2/dA (BITC cell), FP32-split forward with FP64
-    accumulation; each checked by a central difference (step h) along one seeded direction, with
-    the energies of the difference in the same precision; plus the kernel-gradient backward pass."""
+    accumulation (timed); each checked by a central difference (step h) along one seeded direction
+    with FP64 energies (an FP32-split energy difference over 2h would be rounding-limited at
+    ~1e-7 Ha / 2h); plus the kernel-gradient backward pass."""
     from tc_amd import system as S
     from tc_amd import tcx
     res = {}
@@ -306,12 +314,12 @@ def bench_c4(eng, log, h=1e-4):
             e, g = eng.grad_exchange(prec)
             t_grad = time.perf_counter() - t0
             an = 2.0 * np.real(np.vdot(g, d))
-            energy = lambda s: (eng.load_system(s), eng.grad_exchange(prec)[0])[1].real
+            energy = lambda s: (eng.load_system(s), eng.grad_exchange(tcx.PREC_FP64)[0])[1].real
         else:
             e, gl, gr, dA = eng.grad_tc2(prec)
             t_grad = time.perf_counter() - t0
             an = np.sum(d * gr)  # along phi
-            energy = lambda s: (eng.load_system(s), eng.grad_tc2(prec)[0])[1]
+            energy = lambda s: (eng.load_system(s), eng.grad_tc2(tcx.PREC_FP64)[0])[1]
         log(f"bench c4: {method} gradient {t_grad:.2f} s")
         ep = energy(S.System(**{**sys_.__dict__, "phi_occ": sys_.phi_occ + h * d}))
         em = energy(S.System(**{**sys_.__dict__, "phi_occ": sys_.phi_occ - h * d}))
@@ -424,6 +432,48 @@ def bench_c3(log, replays):
         "sequential_per_k": {"runs": len(seq_t), "p50_s": q(seq_t, 50), "p99_s": q(seq_t, 99),
                              "pairs_per_s": pairs / q(seq_t, 50), "matches_batched": bool(seq_same)},
         "speedup_p50": q(seq_t, 50) / q(graph_t, 50),
+    }
+
+
+def bench_davidson(eng, log):
+    """One block-Davidson update of the 16 occupied bands of the HF 4x4x4 k, 40^3 cell
+    (host pointers in and out, as TC++ calls it): wall time against one apply of
+    calc_hamiltonian::all on the same bands; the difference over the two applies the
+    Davidson makes is the cost of its device-resident FP64 subspace algebra."""
+    from tc_amd import system as S
+    from tc_amd import tcx
+
+    sys_ = S.fcc_config1()
+    eng.load_system(sys_)
+    nb, phi = sys_.nocc.copy(), sys_.phi_occ
+    h1, h2, _ = eng.apply_all(nb, phi, precision=tcx.PREC_FP32)  # warm-up (plans, workspaces)
+    d_phi = eng.to_device(phi)
+    d_h = [eng.device_buffer(phi.nbytes) for _ in range(2)]
+    nb32 = np.ascontiguousarray(nb, np.int32)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    st = eng.L.tcx_apply_all(eng.h, tcx._ptr(nb32, tcx._ip), d_phi.ptr, d_h[0].ptr, d_h[1].ptr, None, tcx.PREC_FP32,
+                             tcx.FLAG_DEVICE_PTRS)
+    eng.synchronize()
+    t_apply = time.perf_counter() - t0
+    eng._check(st, "apply_all")
+    for b in [d_phi] + d_h:
+        b.free()
+    bo = sys_.band_offsets(nb)
+    eig = np.concatenate([np.einsum("bl,bl->b", phi[bo[k]:bo[k + 1]].reshape(nb[k], -1).conj(),
+                                    (h1 + h2)[bo[k]:bo[k + 1]].reshape(nb[k], -1)) for k in range(sys_.nk)])
+    log("bench: davidson warm-up call")
+    phi1, _, eig1, en1 = eng.davidson(nb, phi, eig, precision=tcx.PREC_FP32)
+    t0 = time.perf_counter()
+    phi2, _, eig2, en2 = eng.davidson(nb, phi1, eig1, precision=tcx.PREC_FP32)
+    t_dav = time.perf_counter() - t0
+    log(f"bench: davidson {t_dav * 1e3:.1f} ms, apply_all {t_apply * 1e3:.1f} ms")
+    return {
+        "workload": "HF fcc 4x4x4 k, 16 bands/k, 40^3, max_blocks 2, num_refresh 1, FP32-split hf2x",
+        "davidson_ms": t_dav * 1e3, "apply_all_device_ptrs_ms": t_apply * 1e3,
+        "algebra_ms_est": (t_dav - 2 * t_apply) * 1e3,
+        "sum_eig_Ha": float(np.sum(eig2.real)), "max_abs_eig_change_Ha": float(np.max(np.abs(eig2 - eig1))),
+        "max_abs_eig_minus_energy_sum": float(np.max(np.abs(eig2 - en2.sum(axis=1)))),
     }
 
 

@@ -209,7 +209,8 @@ int tcx_graph_destroy(tcx_context *ctx, int graph_id);
  * Diagonalization::block_davidson (src/include/diagonalization.hpp; src/diagonalization_davidson.cpp:7-689)
  * with make_a_new_trial_vector, Gram_Schmidt and sort_eigen
  * (src/diagonalization_utils.cpp:7-29, 79-118, 182-210).  H*phi is tcx_apply_all on
- * the GPU in `precision`; the subspace algebra is FP64 on the host.  Host pointers:
+ * the GPU in `precision`; the subspace algebra (Gram-Schmidt, preconditioner, V^H H V,
+ * the per-k eigensolve, rotations) runs in FP64 on the GPU too.  Host pointers:
  *   phi      in/out complex128 ragged [k][nb[k]][npw_k]: trial bands in (orthonormal per k,
  *            as the reference's bands are; BITC bands are orthonormalised on entry),
  *            eigenvectors out;

@@ -94,25 +94,45 @@ def test_davidson_full_subspace_is_exact(engine, method):
         assert np.abs(en[ik * nbk:(ik + 1) * nbk].sum(axis=1) - got).max() < 1e-9 * max(1.0, np.abs(got).max())
 
 
+def _kG2(sys_, ik):
+    sl = slice(sys_.pw_offsets()[ik], sys_.pw_offsets()[ik + 1])
+    kg = (sys_.kfrac[ik] + S.miller_of_index(sys_.n)[sys_.gindex[sl]]) @ sys_.recip
+    return np.sum(kg * kg, axis=1)
+
+
 @pytest.mark.parametrize("method", ["HF", "TC"])
 def test_davidson_default_blocks_converge(engine, method):
-    """Repeated calls with the reference defaults (2 blocks, 1 refresh), eigenvalues fed back."""
+    """Repeated calls with the reference defaults (2 blocks, 1 refresh), eigenvalues fed back.
+    HF converges to the exact eigenvalues.  For TC the reference re-orthonormalises the
+    Ritz vectors of the non-Hermitian H (diagonalization_davidson.cpp:530-537), so with
+    these defaults only the lowest band converges; the second stops at a fixed point of
+    the algorithm (5.8e-5 Ha off at k=1 here).  The numpy restatement of block_davidson
+    run for the same number of calls stops at the same point, which the GPU must match."""
+    from oracle import davidson as OD
     sys_ = system(method, 63)
     engine.load_system(sys_)
     mats = full_hamiltonians(sys_)
     nbk = 2
     nb = np.full(sys_.nk, nbk, np.int32)
-    phi = trial_bands(sys_, nb, 64)
+    phi0 = trial_bands(sys_, nb, 64)
+    phi = phi0
+    bo = sys_.band_offsets(nb)
     eig = np.zeros(nb.sum(), np.complex128)
-    for it in range(120):
+    calls = 120
+    for it in range(calls):
         phi, _, eig, en = engine.davidson(nb, phi, eig)
     for ik in range(sys_.nk):
         ref = lowest(sum(mats[ik]), nbk)
         got = eig[ik * nbk:(ik + 1) * nbk]
         print(method, ik, got - ref, flush=True)
-        # HF converges fully; for TC the re-orthonormalised Ritz vectors (:530-537) make the
-        # second band stagnate at a few 1e-6 with these defaults -- within the 1e-5 Ha target
-        assert np.abs(got - ref).max() < (1e-7 if method == "HF" else 1e-5)
+        if method == "HF":
+            assert np.abs(got - ref).max() < 1e-7
+            continue
+        assert abs(got[0] - ref[0]) < 1e-7
+        P, e = phi0[bo[ik]:bo[ik + 1]].reshape(nbk, -1), np.zeros(nbk, complex)
+        for it in range(calls):
+            P, _, e, _ = OD.block_davidson(list(mats[ik]), _kG2(sys_, ik), P, e)
+        assert np.abs(got - e).max() < 1e-9, (got - e)
 
 
 @pytest.mark.parametrize("method", ["HF", "TC", "BITC"])
@@ -130,10 +150,7 @@ def test_davidson_follows_the_reference_algorithm(engine, method):
     bo = sys_.band_offsets(nb)
     ref_phi = [phi[bo[ik]:bo[ik + 1]].reshape(nbk, -1) for ik in range(sys_.nk)]
     ref_eig = [eig[ik * nbk:(ik + 1) * nbk] for ik in range(sys_.nk)]
-    kG2 = []
-    for ik in range(sys_.nk):
-        kg = (sys_.kfrac[ik] + S.miller_of_index(sys_.n)[sys_.gindex[sys_.pw_offsets()[ik]:sys_.pw_offsets()[ik + 1]]]) @ sys_.recip
-        kG2.append(np.sum(kg * kg, axis=1))
+    kG2 = [_kG2(sys_, ik) for ik in range(sys_.nk)]
     for it in range(6):
         phi, left, eig, en = engine.davidson(nb, phi, eig, max_blocks=3, num_refresh=2)
         for ik in range(sys_.nk):
