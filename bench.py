@@ -97,6 +97,11 @@ def main():
     ap.add_argument("--davidson", action="store_true",
                     help="also time tcx_davidson (2 blocks, 1 refresh, FP32-split pair terms) on the HF "
                          "4x4x4 k, 16-band, 40^3 cell against one calc_hamiltonian::all apply; report 'davidson'")
+    ap.add_argument("--c2-terms", default="tc2x,tc3a1,tc3b1,tc3a2a4b2b5,tc3a3b3b4b6",
+                    help="comma-separated terms of the --c2 leg")
+    ap.add_argument("--c2-precs", default="fp32,fp64", help="precisions of the --c2 leg (fp32, fp64)")
+    ap.add_argument("--c2-repeat", type=int, default=1,
+                    help="applies per term and precision in --c2; the last one is timed (2 = steady state)")
     ap.add_argument("--c2", action="store_true",
                     help="also time the BITC workload (tc2x + three-body terms, fcc 4x4x4k 16b 40^3) in "
                          "FP32-split and FP64 and report its energy errors under 'c2'")
@@ -116,7 +121,7 @@ def main():
         eng = tcx.Engine(0)
         extra = {}
         if args.c2:
-            extra["c2"] = bench_c2(eng, log)
+            extra["c2"] = bench_c2(eng, log, args.c2_terms.split(","), args.c2_precs.split(","), args.c2_repeat)
         if args.c3:
             extra["c3"] = bench_c3(log, args.c3)
         if args.c4:
@@ -280,7 +285,7 @@ def main():
         },
     }
     if args.c2:
-        out["c2"] = bench_c2(eng, log)
+        out["c2"] = bench_c2(eng, log, args.c2_terms.split(","), args.c2_precs.split(","), args.c2_repeat)
     if args.c3:
         out["c3"] = bench_c3(log, args.c3)
     if args.c4:
@@ -477,13 +482,16 @@ def bench_davidson(eng, log):
     }
 
 
-def bench_c2(eng, log):
+def bench_c2(eng, log, terms, precs, repeat=1):
     """BITC two- and three-body terms on the fcc 4x4x4 k, 16-band, 40^3 cell (A = 0.5, F from
-    the cusp): one FP32-split and one FP64 apply per term on HBM-resident buffers, with the
-    per-band error and the term energy (left orbitals) of FP32 against FP64."""
+    the cusp): FP32-split and FP64 applies per term on HBM-resident buffers (the last of `repeat`
+    applies is timed), with the per-band error and the term energy (left orbitals) of FP32
+    against FP64 when both ran."""
     from tc_amd import energy as E
     from tc_amd import system as S
     from tc_amd import tcx
+    nbody_of = {"tc2x": 2, "tc3a1": 3, "tc3b1": 3, "tc3a2a4b2b5": 3, "tc3a3b3b4b6": 3}
+    prec_of = {"fp32": tcx.PREC_FP32, "fp64": tcx.PREC_FP64}
     sys_ = S.fcc_config2()
     nb = sys_.nocc.copy()
     phi = sys_.phi_occ
@@ -492,32 +500,36 @@ def bench_c2(eng, log):
     d_out = eng.device_buffer(phi.nbytes)
     bo = sys_.band_offsets(nb)
     res = {"workload": "BITC fcc a=10.26, 4x4x4 k, 16 bands/k (all occupied), 40^3, RPA Jastrow A=0.5",
-           "pairs_per_apply": int(np.sum(nb) * np.sum(sys_.nocc)), "terms": {}}
-    for term, nbody in (("tc2x", 2), ("tc3a1", 3), ("tc3b1", 3), ("tc3a2a4b2b5", 3), ("tc3a3b3b4b6", 3)):
+           "pairs_per_apply": int(np.sum(nb) * np.sum(sys_.nocc)), "timed_apply": repeat, "terms": {}}
+    for term in terms:
         outs, secs = {}, {}
-        for prec in (tcx.PREC_FP32, tcx.PREC_FP64):
-            d_out.zero()
-            eng.synchronize()
-            t0 = time.perf_counter()
-            eng._term_device(term, nb, d_phi.ptr, d_out.ptr, prec)
-            eng.synchronize()
-            secs[prec] = time.perf_counter() - t0
-            outs[prec] = d_out.download(np.complex128)
-            log(f"bench c2: {term} {'fp32' if prec else 'fp64'} {secs[prec]:.2f} s")
-        a, b = outs[tcx.PREC_FP32], outs[tcx.PREC_FP64]
-        rel = []
-        for ik in range(sys_.nk):
-            x = a[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
-            y = b[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
-            rel.extend(np.linalg.norm(x - y, axis=1) / np.maximum(np.linalg.norm(y, axis=1), 1e-300))
-        e32 = E.term_energy(sys_, a, nbody, left=True)
-        e64 = E.term_energy(sys_, b, nbody, left=True)
-        res["terms"][term] = {
-            "fp32_s": secs[tcx.PREC_FP32], "fp64_s": secs[tcx.PREC_FP64],
-            "fp32_pairs_per_s": res["pairs_per_apply"] / secs[tcx.PREC_FP32] if term not in ("tc3a1", "tc3b1") else None,
-            "E_fp64_Ha": [float(e64.real), float(e64.imag)], "dE_Ha": float(abs(e32 - e64)),
-            "per_band_rel_max": float(np.max(rel)),
-        }
+        for pname in precs:
+            prec = prec_of[pname]
+            for _ in range(repeat):
+                d_out.zero()
+                eng.synchronize()
+                t0 = time.perf_counter()
+                eng._term_device(term, nb, d_phi.ptr, d_out.ptr, prec)
+                eng.synchronize()
+                secs[pname] = time.perf_counter() - t0
+            outs[pname] = d_out.download(np.complex128)
+            log(f"bench c2: {term} {pname} {secs[pname]:.2f} s")
+        entry = {f"{p}_s": secs[p] for p in precs}
+        if "fp32" in secs and term not in ("tc3a1", "tc3b1"):
+            entry["fp32_pairs_per_s"] = res["pairs_per_apply"] / secs["fp32"]
+        if "fp32" in outs and "fp64" in outs:
+            a, b = outs["fp32"], outs["fp64"]
+            rel = []
+            for ik in range(sys_.nk):
+                x = a[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+                y = b[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+                rel.extend(np.linalg.norm(x - y, axis=1) / np.maximum(np.linalg.norm(y, axis=1), 1e-300))
+            e32 = E.term_energy(sys_, a, nbody_of[term], left=True)
+            e64 = E.term_energy(sys_, b, nbody_of[term], left=True)
+            entry.update({"fp32_over_fp64": secs["fp64"] / secs["fp32"],
+                          "E_fp64_Ha": [float(e64.real), float(e64.imag)], "dE_Ha": float(abs(e32 - e64)),
+                          "per_band_rel_max": float(np.max(rel))})
+        res["terms"][term] = entry
     d_phi.free()
     d_out.free()
     return res

@@ -212,6 +212,34 @@ static void fft3d_impl(int sign, T* grids, int ngrids, const Geom& g, double sc,
     }
 }
 
+template <typename T>
+static void fft_xy_impl(int sign, T* grids, int ngrids, const Geom& g, hipStream_t s)
+{
+    if (g.n1 != g.n2) throw Error(TCX_ERR_UNSUPPORTED, "FFT grid must have n1 == n2");
+    cpx<T>* cg = reinterpret_cast<cpx<T>*>(grids);
+    for (int b0 = 0; b0 < ngrids; b0 += 65535) {
+        const int nb = ngrids - b0 < 65535 ? ngrids - b0 : 65535;
+        cpx<T>* base = cg + (size_t)b0 * g.n1 * g.n2 * g.n3;
+#define TCX_XY(NP)                                                                                     \
+    if (sign < 0)                                                                                      \
+        hipLaunchKernelGGL((k_fft_xy<T, NP, -1>), dim3(g.n3, nb), dim3(kPlaneBlock), 0, s, base, g.n3); \
+    else                                                                                               \
+        hipLaunchKernelGGL((k_fft_xy<T, NP, 1>), dim3(g.n3, nb), dim3(kPlaneBlock), 0, s, base, g.n3);
+        TCX_SIZE_SWITCH(g.n1, TCX_XY)
+#undef TCX_XY
+        TCX_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void launch_fft_xy(int prec, int sign, void* grids, int ngrids, const Geom& g, hipStream_t s)
+{
+    if (ngrids == 0) return;
+    if (prec == TCX_PREC_FP64)
+        fft_xy_impl<double>(sign, (double*)grids, ngrids, g, s);
+    else
+        fft_xy_impl<float>(sign, (float*)grids, ngrids, g, s);
+}
+
 void launch_fft3d(int prec, int sign, void* grids, int ngrids, const Geom& g, double sc, hipStream_t s)
 {
     if (ngrids == 0) return;

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_C(const cpx<T>* __restric
         load_pencil<T, M>(v, buf + (size_t)pp * ng + xy + (size_t)h * nxy, S * nxy);  // z = S m + h
         Split::template fft<T, NZ, 1>(v, h);                                         // -> z = ob*M + k
         const cpx<T>* q = phiR_occ + (size_t)occ * ng + xy + (size_t)ob * M * nxy;
-        const double f = fill[occ];
+        const double f = fill ? fill[occ] : 1.0;
         static_for<0, M>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             const cpx<T> prod = v[k] * q[(size_t)k * nxy];

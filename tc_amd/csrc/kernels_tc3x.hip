@@ -5,18 +5,21 @@
 //
 // Every (target, source) pair of a launch owns a record of C grids (complex
 // FP32 or FP64, consecutive in HBM) that moves through
-//   products   record[c] = A_c(r) * B_c(r), orbital grids chosen by the pair;
-//   3D FFT     batched over all records of the launch;
+//   products+z record[c] = A_c(r) * B_c(r) formed in registers from the two
+//              orbital grids chosen by the pair and transformed along z in the
+//              same pass, then the xy planes: FFT_forward;
 //   combine    a per-term G-space map that multiplies by the Jastrow gradient
 //              du_d(G) = (kq+G)_d u(kq+G), evaluated in FP64 on the fly;
-//   3D FFT     batched;
-//   accumulate one thread owns one point of one target and loops over the
+//   3D FFT     batched FFT_backward (xy planes, then z pencils);
+//   accumulate one thread owns one point (or one z pencil, when the z pass of
+//              FFT_backward is fused in) of one target and loops over the
 //              target's pairs, summing in FP64 registers, then adds once into
 //              the FP64 real-space grid of the target (no atomics).
 // All point-wise kernels are one thread per grid point, 256 threads, with the
 // record / target / band index in blockIdx.y (< 65536, the launchers split).
 #include "jastrow_dev.hpp"
 #include "kernels_common.hpp"
+#include "size_switch.hpp"
 
 namespace tcx {
 
@@ -76,74 +79,64 @@ void for_slices(long long n, F&& f)
 }  // namespace
 
 // ----------------------------------------------------------------- products
-template <typename T>
-__global__ __launch_bounds__(256) void k_products(const PairEntry* __restrict__ pairs,
-                                                  const ProdChan* __restrict__ chans, int nchan, int rec0,
-                                                  cpx<T>* __restrict__ out, long long N)
+// Products fused with the forward z pass: one thread owns pencil (x, y) of one record
+// channel, forms A_c * B_c along z from the two orbital grids (shared by many records, so
+// mostly cache hits), transforms along z, scales, and stores; launch_fft_xy(-1) over the
+// records completes FFT_forward.  Same records as k_products + launch_fft3d(-1).
+template <typename T, int NZ>
+__global__ __launch_bounds__(kPencilBlock) void k_products_z(const PairEntry* __restrict__ pairs,
+                                                             const ProdChan* __restrict__ chans, int nchan, int rec0,
+                                                             cpx<T>* __restrict__ out, int nxy, T sc)
 {
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
+    const int xy = blockIdx.x * kPencilBlock + threadIdx.x;
+    if (xy >= nxy) return;
     const int rec = rec0 + (int)blockIdx.y;
     const int p = rec / nchan, c = rec - p * nchan;
     const PairEntry e = pairs[p];
     const ProdChan ch = chans[c];
+    const size_t ng = (size_t)nxy * NZ;
     const long long ia = (long long)(ch.a_sel ? e.occ : e.app) * ch.a_mul + ch.a_off;
     const long long ib = (long long)(ch.b_sel ? e.occ : e.app) * ch.b_mul + ch.b_off;
-    cpx<T> a = ((const cpx<T>*)ch.a)[ia * N + r];
-    cpx<T> b = ((const cpx<T>*)ch.b)[ib * N + r];
-    if (ch.a_conj) a = conjg(a);
-    if (ch.b_conj) b = conjg(b);
-    out[(long long)rec * N + r] = a * b;
-}
-
-void launch_products(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
-                     long long N, hipStream_t s)
-{
-    for_slices((long long)npairs * nchan, [&](int r0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_products<double>, grid_pts(N, ny), dim3(256), 0, s, pairs, chans, nchan, r0,
-                               (cpx<double>*)out, N);
-        else
-            hipLaunchKernelGGL(k_products<float>, grid_pts(N, ny), dim3(256), 0, s, pairs, chans, nchan, r0,
-                               (cpx<float>*)out, N);
-        TCX_HIP_CHECK(hipGetLastError());
+    const cpx<T>* a = (const cpx<T>*)ch.a + (size_t)ia * ng + xy;
+    const cpx<T>* b = (const cpx<T>*)ch.b + (size_t)ib * ng + xy;
+    cpx<T> v[NZ];
+    static_for<0, NZ>([&](auto zc) {
+        constexpr int z = decltype(zc)::value;
+        cpx<T> x = a[(size_t)z * nxy], y = b[(size_t)z * nxy];
+        if (ch.a_conj) x = conjg(x);
+        if (ch.b_conj) y = conjg(y);
+        v[z] = x * y;
     });
+    fft<T, NZ, -1>(v);
+    static_for<0, NZ>([&](auto zc) { v[decltype(zc)::value] = scale(v[decltype(zc)::value], sc); });
+    store_pencil<T, NZ>(v, out + (size_t)rec * ng + xy, nxy);
 }
 
-// ----------------------------------------------------------------- generic accumulate
-// W[t.app](r) += factor * sum_{p in t} rec[p*stride + ch](r) * M[pair.occ](r)
 template <typename T>
-__global__ __launch_bounds__(256) void k_acc_occ(const cpx<T>* __restrict__ rec, int stride, int ch,
-                                                 const cpx<T>* __restrict__ M, const PairEntry* __restrict__ pairs,
-                                                 const TargetEntry* __restrict__ targets, int t0,
-                                                 double2* __restrict__ W, long long N, double factor)
+static void products_fft_impl(const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, T* out,
+                              const Geom& g, T sc, hipStream_t s)
 {
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const TargetEntry t = targets[t0 + blockIdx.y];
-    double2 acc = make_double2(0.0, 0.0);
-    for (int i = 0; i < t.npair; ++i) {
-        const int p = t.pair0 + i;
-        const double2 v = up(rec[((long long)p * stride + ch) * N + r]);
-        const double2 m = up(M[(long long)pairs[p].occ * N + r]);
-        acc = dadd(acc, dmul(v, m));
-    }
-    double2* w = W + (long long)t.app * N + r;
-    *w = dadd(*w, dscale(acc, factor));
-}
-
-void launch_acc_occ(int prec, const void* rec, int stride, int ch, const void* M, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, double* W, long long N, double factor, hipStream_t s)
-{
-    for_slices(ntargets, [&](int t0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_acc_occ<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec, stride,
-                               ch, (const cpx<double>*)M, pairs, targets, t0, (double2*)W, N, factor);
-        else
-            hipLaunchKernelGGL(k_acc_occ<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec, stride, ch,
-                               (const cpx<float>*)M, pairs, targets, t0, (double2*)W, N, factor);
+    const int nxy = g.n1 * g.n2;
+    for_slices((long long)npairs * nchan, [&](int r0, unsigned ny) {
+        const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, ny);
+#define TCX_PZ(NZ)                                                                                       \
+    hipLaunchKernelGGL((k_products_z<T, NZ>), grid, dim3(kPencilBlock), 0, s, pairs, chans, nchan, r0, \
+                       (cpx<T>*)out, nxy, sc);
+        TCX_SIZE_SWITCH(g.n3, TCX_PZ)
+#undef TCX_PZ
         TCX_HIP_CHECK(hipGetLastError());
     });
+}
+
+void launch_products_fft(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
+                         const Geom& g, double sc, hipStream_t s)
+{
+    if (npairs == 0 || nchan == 0) return;
+    if (prec == TCX_PREC_FP64)
+        products_fft_impl<double>(pairs, npairs, chans, nchan, (double*)out, g, sc, s);
+    else
+        products_fft_impl<float>(pairs, npairs, chans, nchan, (float*)out, g, (float)sc, s);
+    launch_fft_xy(prec, -1, out, npairs * nchan, g, s);
 }
 
 // ----------------------------------------------------------------- tc3a3b3b4b6

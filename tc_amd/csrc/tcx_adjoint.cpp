@@ -206,8 +206,8 @@ void Context::kernel_grad(int term, int nbins, double width, double* dE_dK, doub
         const JastrowRPA j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
         for (const Launch& L : S.launches) {
             const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
-            launch_products(prec, pairs, L.npairs, (const ProdChan*)d_chans.p, C, d_rec.p, N, stream);
-            launch_fft3d(prec, -1, d_rec.p, C * L.npairs, geo, 1.0 / (double)N, stream);
+            launch_products_fft(prec, pairs, L.npairs, (const ProdChan*)d_chans.p, C, d_rec.p, geo, 1.0 / (double)N,
+                                stream);
             launch_kgrad(prec, d_rec.p, tc2 ? 1 : 0, pairs, L.npairs, (const double*)S.d_kq.p, (const double*)d_fill.p,
                          geo, N, j0, j1, w0, nbins, width, bins_v, bins_u, d_sums, stream);
         }

@@ -100,6 +100,9 @@ void launch_convert_d2f(const double* in, float* out, long long ncomplex, hipStr
 
 // in-place 3D FFT of ngrids consecutive grids; sign -1 forward, +1 backward; result scaled by scale
 void launch_fft3d(int prec, int sign, void* grids, int ngrids, const Geom& g, double scale, hipStream_t s);
+// the xy-plane half of a 3D transform (no scaling): with a z-pencil pass before it, a
+// complete 3D FFT
+void launch_fft_xy(int prec, int sign, void* grids, int ngrids, const Geom& g, hipStream_t s);
 
 // hf2x pair pipeline
 void launch_hf2x_A(int prec, const Geom& g, const void* phiR_app, const void* phiR_occ, const PairEntry* pairs,
@@ -137,10 +140,9 @@ void launch_sum_squares(const double* dnu, double* V, long long N, hipStream_t s
 void launch_mul_shared(double* grids, int ngrids, const double* V, long long N, hipStream_t s);
 
 // three-body exchange building blocks (records of the working precision, FP64 accumulators)
-void launch_products(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
-                     long long N, hipStream_t s);
-void launch_acc_occ(int prec, const void* rec, int stride, int ch, const void* M, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, double* W, long long N, double factor, hipStream_t s);
+// records = FFT_forward(A * B) scaled by sc (the products fused with the z pass)
+void launch_products_fft(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
+                         const Geom& g, double sc, hipStream_t s);
 void launch_a7_comb1(int prec, const void* rec, void* Y, void* P, const PairEntry* pairs, int npairs,
                      const double* kqcart, const double* fill, const Geom& g, long long N, const JastrowRPA& j0,
                      const JastrowRPA& j1, double fb3, hipStream_t s);

@@ -155,8 +155,8 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             char* Pb = rec4 + (size_t)4 * np * gT;
             char* Y = Pb + (size_t)6 * np * gT;
             timed(0, np, [&] {
-                launch_products(prec, pairs, np, (const ProdChan*)d_chans.p, 4, rec4, N, stream);
-                launch_fft3d(prec, -1, rec4, 4 * np, geo, 1.0 / (double)N, stream);
+                launch_products_fft(prec, pairs, np, (const ProdChan*)d_chans.p, 4, rec4, geo, 1.0 / (double)N,
+                                    stream);
             });
             timed(1, np, [&] {
                 launch_a7_comb1(prec, rec4, Y, Pb, pairs, np, kq, fill, geo, N, j00, j10, fb3, stream);
@@ -165,10 +165,10 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
                               stream);
                 launch_fft3d(prec, -1, Pb, 6 * np, geo, 1.0 / (double)N, stream);
                 launch_a7_comb2(prec, Y, Pb, pairs, np, kq, geo, N, j00, j10, fb6, stream);
-                launch_fft3d(prec, +1, Y, np, geo, 1.0, stream);
+                launch_fft_xy(prec, +1, Y, np, geo, stream);
             });
-            timed(2, np, [&] {
-                launch_acc_occ(prec, Y, 1, 0, occR, pairs, targets, L.ntargets, W, N, 1.0, stream);
+            timed(2, np, [&] {  // z pass of FFT_backward fused with the accumulation
+                launch_hf2x_C(prec, geo, Y, occR, pairs, targets, L.ntargets, nullptr, W, stream);
             });
         }
 
@@ -224,15 +224,15 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             char* rec3 = (char*)d_rec.p;
             char* Z = rec3 + (size_t)3 * np * gT;
             timed(0, np, [&] {
-                launch_products(prec, pairs, np, (const ProdChan*)d_chans.p, 3, rec3, N, stream);
-                launch_fft3d(prec, -1, rec3, 3 * np, geo, 1.0 / (double)N, stream);
+                launch_products_fft(prec, pairs, np, (const ProdChan*)d_chans.p, 3, rec3, geo, 1.0 / (double)N,
+                                    stream);
             });
             timed(1, np, [&] {
                 launch_b4_comb(prec, rec3, Z, pairs, np, kq, fill, geo, N, j00, fb4, stream);
-                launch_fft3d(prec, +1, Z, np, geo, 1.0, stream);
+                launch_fft_xy(prec, +1, Z, np, geo, stream);
             });
             timed(2, np, [&] {
-                launch_acc_occ(prec, Z, 1, 0, occR, pairs, targets, L.ntargets, W, N, 1.0, stream);
+                launch_hf2x_C(prec, geo, Z, occR, pairs, targets, L.ntargets, nullptr, W, stream);
             });
         }
         finish_W(P, io, 1.0);
@@ -299,8 +299,8 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
                 char* rec = (char*)d_rec.p;
                 char* O = rec + (size_t)nin * np * gT;
                 timed(0, np, [&] {
-                    launch_products(prec, pairs, np, (const ProdChan*)d_chans.p, nin, rec, N, stream);
-                    launch_fft3d(prec, -1, rec, nin * np, geo, 1.0 / (double)N, stream);
+                    launch_products_fft(prec, pairs, np, (const ProdChan*)d_chans.p, nin, rec, geo,
+                                        1.0 / (double)N, stream);
                 });
                 timed(1, np, [&] {
                     launch_a8_comb1(prec, rec, nin, O, pairs, np, (const double*)S.d_kq.p, geo, N, j00, j01, stream);
@@ -331,8 +331,8 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
             for (const Launch& L : S.launches) {
                 const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
                 const TargetEntry* targets = (const TargetEntry*)S.d_targets.p + L.target0;
-                launch_products(TCX_PREC_FP64, pairs, L.npairs, (const ProdChan*)d_chans.p, 2, d_rec.p, N, stream);
-                launch_fft3d(TCX_PREC_FP64, -1, d_rec.p, 2 * L.npairs, geo, 1.0 / (double)N, stream);
+                launch_products_fft(TCX_PREC_FP64, pairs, L.npairs, (const ProdChan*)d_chans.p, 2, d_rec.p, geo,
+                                    1.0 / (double)N, stream);
                 launch_a8_div(Q2, (const double*)d_rec.p, pairs, targets, L.ntargets, fill, (const double*)d_aux.p,
                               geo, N, j00, j01, stream);
             }
@@ -363,8 +363,8 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
                 const TargetEntry* targets = (const TargetEntry*)S.d_targets.p + L.target0;
                 const int np = L.npairs;
                 timed(0, np, [&] {
-                    launch_products(prec, pairs, np, (const ProdChan*)d_chans.p, 4, d_rec.p, N, stream);
-                    launch_fft3d(prec, -1, d_rec.p, 4 * np, geo, 1.0 / (double)N, stream);
+                    launch_products_fft(prec, pairs, np, (const ProdChan*)d_chans.p, 4, d_rec.p, geo,
+                                        1.0 / (double)N, stream);
                 });
                 timed(1, np, [&] {
                     launch_a8_comb2(prec, d_rec.p, pairs, np, (const double*)S.d_kq.p, fill, geo, N, j00, fa4, fb5,
