@@ -240,6 +240,35 @@ void launch_fft_xy(int prec, int sign, void* grids, int ngrids, const Geom& g, h
         fft_xy_impl<float>(sign, (float*)grids, ngrids, g, s);
 }
 
+template <typename T>
+static void fft_z_impl(int sign, T* grids, int ngrids, const Geom& g, double sc, hipStream_t s)
+{
+    const int nxy = g.n1 * g.n2;
+    cpx<T>* cg = reinterpret_cast<cpx<T>*>(grids);
+    for (int b0 = 0; b0 < ngrids; b0 += 65535) {
+        const int nb = ngrids - b0 < 65535 ? ngrids - b0 : 65535;
+        cpx<T>* base = cg + (size_t)b0 * nxy * g.n3;
+        const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, nb);
+#define TCX_Z(NZ)                                                                                            \
+    if (sign < 0)                                                                                            \
+        hipLaunchKernelGGL((k_fft_z<T, NZ, -1>), grid, dim3(kPencilBlock), 0, s, base, nxy, (T)sc);          \
+    else                                                                                                     \
+        hipLaunchKernelGGL((k_fft_z<T, NZ, 1>), grid, dim3(kPencilBlock), 0, s, base, nxy, (T)sc);
+        TCX_SIZE_SWITCH(g.n3, TCX_Z)
+#undef TCX_Z
+        TCX_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void launch_fft_z(int prec, int sign, void* grids, int ngrids, const Geom& g, double sc, hipStream_t s)
+{
+    if (ngrids == 0) return;
+    if (prec == TCX_PREC_FP64)
+        fft_z_impl<double>(sign, (double*)grids, ngrids, g, sc, s);
+    else
+        fft_z_impl<float>(sign, (float*)grids, ngrids, g, sc, s);
+}
+
 void launch_fft3d(int prec, int sign, void* grids, int ngrids, const Geom& g, double sc, hipStream_t s)
 {
     if (ngrids == 0) return;

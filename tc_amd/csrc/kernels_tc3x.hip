@@ -128,48 +128,24 @@ static void products_fft_impl(const PairEntry* pairs, int npairs, const ProdChan
     });
 }
 
-void launch_products_fft(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
-                         const Geom& g, double sc, hipStream_t s)
+void launch_products_z(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
+                       const Geom& g, double sc, hipStream_t s)
 {
     if (npairs == 0 || nchan == 0) return;
     if (prec == TCX_PREC_FP64)
         products_fft_impl<double>(pairs, npairs, chans, nchan, (double*)out, g, sc, s);
     else
         products_fft_impl<float>(pairs, npairs, chans, nchan, (float*)out, g, (float)sc, s);
+}
+
+void launch_products_fft(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
+                         const Geom& g, double sc, hipStream_t s)
+{
+    launch_products_z(prec, pairs, npairs, chans, nchan, out, g, sc, s);
     launch_fft_xy(prec, -1, out, npairs * nchan, g, s);
 }
 
 // ----------------------------------------------------------------- tc3a3b3b4b6
-// Phase-1 combine (tc3a3b3b4b6.cpp:280-349).  rec[p] = FFT_forward of
-// (conj(chi_qi) phi_j, conj(chi_qi) dnu_d phi_j).  With du_s,d = (k-q+G)_d u(k-q+G, s, 0):
-//   Y[p]        = f b3 sum_d du_0,d rec[p][1+d]          ([3b3], before FFT_backward)
-//   P[p][3s+d]  = f rec[p][0] du_s,d                      ([3a3,b4,b6])
-template <typename T>
-__global__ __launch_bounds__(256) void k_a7_comb1(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Y,
-                                                  cpx<T>* __restrict__ P, const PairEntry* __restrict__ pairs, int p0,
-                                                  const double* __restrict__ kqcart, const double* __restrict__ fill,
-                                                  Geom g, long long N, JastrowRPA j0, JastrowRPA j1, double fb3)
-{
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const int p = p0 + (int)blockIdx.y;
-    const PairEntry e = pairs[p];
-    double x[3];
-    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
-    const double u0 = jastrow_uk(j0, x2), u1 = jastrow_uk(j1, x2);
-    const double f = fill[e.occ];
-    const cpx<T>* in = rec + (long long)p * 4 * N + r;
-    const double2 rho = up(in[0]);
-    double2 y = make_double2(0.0, 0.0);
-    for (int d = 0; d < 3; ++d) y = dadd(y, dscale(up(in[(long long)(d + 1) * N]), x[d] * u0));
-    Y[(long long)p * N + r] = down<T>(dscale(y, f * fb3));
-    cpx<T>* po = P + (long long)p * 6 * N + r;
-    for (int d = 0; d < 3; ++d) {
-        po[(long long)d * N] = down<T>(dscale(rho, f * x[d] * u0));
-        po[(long long)(3 + d) * N] = down<T>(dscale(rho, f * x[d] * u1));
-    }
-}
-
 // Phase-1 real-space step (tc3a3b3b4b6.cpp:316-335): per target (k,j)
 //   qsum[j][d] += sum_p P[p][d] phi_qi  (s2 = 0 channels, before the density)
 //   P[p][3s+d] *= density_s(r)
@@ -204,67 +180,6 @@ __global__ __launch_bounds__(256) void k_a7_mid(cpx<T>* __restrict__ P, const cp
     q[2 * N] = dadd(q[2 * N], acc2);
 }
 
-// Phase-1 second combine (tc3a3b3b4b6.cpp:344-358, the [3b3] and [3b6] inverse
-// transforms merged by linearity): Y[p] += b6 sum_{s,d} du_s,d P[p][3s+d]
-template <typename T>
-__global__ __launch_bounds__(256) void k_a7_comb2(cpx<T>* __restrict__ Y, const cpx<T>* __restrict__ P,
-                                                  const PairEntry* __restrict__ pairs, int p0,
-                                                  const double* __restrict__ kqcart, Geom g, long long N,
-                                                  JastrowRPA j0, JastrowRPA j1, double fb6)
-{
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const int p = p0 + (int)blockIdx.y;
-    const PairEntry e = pairs[p];
-    double x[3];
-    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
-    const double u0 = jastrow_uk(j0, x2), u1 = jastrow_uk(j1, x2);
-    const cpx<T>* pp = P + (long long)p * 6 * N + r;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int d = 0; d < 3; ++d) {
-        acc = dadd(acc, dscale(up(pp[(long long)d * N]), x[d] * u0));
-        acc = dadd(acc, dscale(up(pp[(long long)(3 + d) * N]), x[d] * u1));
-    }
-    cpx<T>* y = Y + (long long)p * N + r;
-    *y = down<T>(dadd(up(*y), dscale(acc, fb6)));
-}
-
-// Phase-2 combine ([3b4], tc3a3b3b4b6.cpp:740-754): Z[p] = b4 f sum_d du_0,d rec[p][d],
-// rec[p][d] = FFT_forward(conj(chi_qi) qsum_j,d)
-template <typename T>
-__global__ __launch_bounds__(256) void k_b4_comb(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Z,
-                                                 const PairEntry* __restrict__ pairs, int p0,
-                                                 const double* __restrict__ kqcart, const double* __restrict__ fill,
-                                                 Geom g, long long N, JastrowRPA j0, double fb4)
-{
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const int p = p0 + (int)blockIdx.y;
-    const PairEntry e = pairs[p];
-    double x[3];
-    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
-    const double u0 = jastrow_uk(j0, x2);
-    const cpx<T>* in = rec + (long long)p * 3 * N + r;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int d = 0; d < 3; ++d) acc = dadd(acc, dscale(up(in[(long long)d * N]), x[d] * u0));
-    Z[(long long)p * N + r] = down<T>(dscale(acc, fb4 * fill[e.occ]));
-}
-
-void launch_a7_comb1(int prec, const void* rec, void* Y, void* P, const PairEntry* pairs, int npairs,
-                     const double* kqcart, const double* fill, const Geom& g, long long N, const JastrowRPA& j0,
-                     const JastrowRPA& j1, double fb3, hipStream_t s)
-{
-    for_slices(npairs, [&](int p0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_a7_comb1<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec,
-                               (cpx<double>*)Y, (cpx<double>*)P, pairs, p0, kqcart, fill, g, N, j0, j1, fb3);
-        else
-            hipLaunchKernelGGL(k_a7_comb1<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec,
-                               (cpx<float>*)Y, (cpx<float>*)P, pairs, p0, kqcart, fill, g, N, j0, j1, fb3);
-        TCX_HIP_CHECK(hipGetLastError());
-    });
-}
-
 void launch_a7_mid(int prec, void* P, const void* occR, const PairEntry* pairs, const TargetEntry* targets,
                    int ntargets, const double* dens, double* qsum, long long N, hipStream_t s)
 {
@@ -275,35 +190,6 @@ void launch_a7_mid(int prec, void* P, const void* occR, const PairEntry* pairs, 
         else
             hipLaunchKernelGGL(k_a7_mid<float>, grid_pts(N, ny), dim3(256), 0, s, (cpx<float>*)P,
                                (const cpx<float>*)occR, pairs, targets, t0, (const double2*)dens, (double2*)qsum, N);
-        TCX_HIP_CHECK(hipGetLastError());
-    });
-}
-
-void launch_a7_comb2(int prec, void* Y, const void* P, const PairEntry* pairs, int npairs, const double* kqcart,
-                     const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1, double fb6,
-                     hipStream_t s)
-{
-    for_slices(npairs, [&](int p0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_a7_comb2<double>, grid_pts(N, ny), dim3(256), 0, s, (cpx<double>*)Y,
-                               (const cpx<double>*)P, pairs, p0, kqcart, g, N, j0, j1, fb6);
-        else
-            hipLaunchKernelGGL(k_a7_comb2<float>, grid_pts(N, ny), dim3(256), 0, s, (cpx<float>*)Y,
-                               (const cpx<float>*)P, pairs, p0, kqcart, g, N, j0, j1, fb6);
-        TCX_HIP_CHECK(hipGetLastError());
-    });
-}
-
-void launch_b4_comb(int prec, const void* rec, void* Z, const PairEntry* pairs, int npairs, const double* kqcart,
-                    const double* fill, const Geom& g, long long N, const JastrowRPA& j0, double fb4, hipStream_t s)
-{
-    for_slices(npairs, [&](int p0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_b4_comb<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec,
-                               (cpx<double>*)Z, pairs, p0, kqcart, fill, g, N, j0, fb4);
-        else
-            hipLaunchKernelGGL(k_b4_comb<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec,
-                               (cpx<float>*)Z, pairs, p0, kqcart, fill, g, N, j0, fb4);
         TCX_HIP_CHECK(hipGetLastError());
     });
 }
@@ -496,30 +382,6 @@ void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l,
 }
 
 // ----------------------------------------------------------------- tc3a2a4b2b5
-// q1-q2 phase combine (tc3a2a4b2b5.cpp:371-375): O[p][6h + 3s + d] = du12_s,d rec[p][h],
-// du12_s,d = (q2-q1+G)_d u(q2-q1+G, 0, s); h = 0 (conj(chi_q1) phi_q2) and, for BITC,
-// h = 1 (conj(phi_q1) chi_q2).
-template <typename T>
-__global__ __launch_bounds__(256) void k_a8_comb1(const cpx<T>* __restrict__ rec, int nin, cpx<T>* __restrict__ O,
-                                                  const PairEntry* __restrict__ pairs, int p0,
-                                                  const double* __restrict__ kqcart, Geom g, long long N,
-                                                  JastrowRPA j0, JastrowRPA j1)
-{
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const int p = p0 + (int)blockIdx.y;
-    const PairEntry e = pairs[p];
-    double x[3];
-    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
-    const double u[2] = {jastrow_uk(j0, x2), jastrow_uk(j1, x2)};
-    for (int h = 0; h < nin; ++h) {
-        const double2 v = up(rec[((long long)p * nin + h) * N + r]);
-        cpx<T>* o = O + ((long long)p * 6 * nin + 6 * h) * N + r;
-        for (int s = 0; s < 2; ++s)
-            for (int d = 0; d < 3; ++d) o[(long long)(3 * s + d) * N] = down<T>(dscale(v, x[d] * u[s]));
-    }
-}
-
 // q1-q2 phase accumulation (tc3a2a4b2b5.cpp:377-427), per target (q2,i2):
 //   Q2 -= f1 sum_{s,d} O conj(O2)            (O2 = O for TC: -f1 |O|^2)
 //   Q4[d] += f1 O_0,d phi_q1                  Q5[d] -= f1 conj(O2_0,d) conj(chi_q1)   (BITC)
@@ -644,31 +506,6 @@ __global__ __launch_bounds__(256) void k_a8_q45(const double2* __restrict__ Q4, 
     Q5T[o] = down<T>(bitc ? Q5[o] : make_double2(-a.x, a.y));
 }
 
-// Apply-phase combine (tc3a2a4b2b5.cpp:849-876), in place on rec[p][0..3]:
-//   in : FFT_forward(conj(chi_t) phi_kj), FFT_forward(Q5'_t,d phi_kj)
-//   out: b5 f2 sum_d du0_d in[1+d],  f2 a4 du0_d in[0]   (du0 = (k-q2+G)_d u(k-q2+G, 0, 0))
-template <typename T>
-__global__ __launch_bounds__(256) void k_a8_comb2(cpx<T>* __restrict__ rec, const PairEntry* __restrict__ pairs,
-                                                  int p0, const double* __restrict__ kqcart,
-                                                  const double* __restrict__ fill, Geom g, long long N, JastrowRPA j0,
-                                                  double fa4, double fb5)
-{
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const int p = p0 + (int)blockIdx.y;
-    const PairEntry e = pairs[p];
-    double x[3];
-    const double x2 = kq_plus_G(g, r, kqcart + 3 * e.kq, x);
-    const double u = jastrow_uk(j0, x2);
-    const double f2 = fill[e.occ];
-    cpx<T>* io = rec + (long long)p * 4 * N + r;
-    const double2 r0 = up(io[0]);
-    double2 b = make_double2(0.0, 0.0);
-    for (int d = 0; d < 3; ++d) b = dadd(b, dscale(up(io[(long long)(1 + d) * N]), x[d] * u));
-    io[0] = down<T>(dscale(b, fb5 * f2));
-    for (int d = 0; d < 3; ++d) io[(long long)(1 + d) * N] = down<T>(dscale(r0, f2 * fa4 * x[d] * u));
-}
-
 // W[t.app] += sum_p rec[p][0] phi_occ(p) + sum_d rec[p][1+d] Q4T[occ(p)][d]
 template <typename T>
 __global__ __launch_bounds__(256) void k_a8_acc2(const cpx<T>* __restrict__ rec, const cpx<T>* __restrict__ occR,
@@ -690,21 +527,6 @@ __global__ __launch_bounds__(256) void k_a8_acc2(const cpx<T>* __restrict__ rec,
     }
     double2* w = W + (long long)t.app * N + r;
     *w = dadd(*w, acc);
-}
-
-void launch_a8_comb1(int prec, const void* rec, int nin, void* O, const PairEntry* pairs, int npairs,
-                     const double* kqcart, const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1,
-                     hipStream_t s)
-{
-    for_slices(npairs, [&](int p0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_a8_comb1<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec, nin,
-                               (cpx<double>*)O, pairs, p0, kqcart, g, N, j0, j1);
-        else
-            hipLaunchKernelGGL(k_a8_comb1<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec, nin,
-                               (cpx<float>*)O, pairs, p0, kqcart, g, N, j0, j1);
-        TCX_HIP_CHECK(hipGetLastError());
-    });
 }
 
 void launch_a8_acc1(int prec, const void* O, int bitc, const void* occR, const void* chiR, const PairEntry* pairs,
@@ -753,21 +575,6 @@ void launch_a8_q45(int prec, const double* Q4, const double* Q5, int bitc, void*
         else
             hipLaunchKernelGGL(k_a8_q45<float>, grid_pts(N, ny), dim3(256), 0, s, (const double2*)Q4,
                                (const double2*)Q5, bitc, (cpx<float>*)Q4T, (cpx<float>*)Q5T, g0, N);
-        TCX_HIP_CHECK(hipGetLastError());
-    });
-}
-
-void launch_a8_comb2(int prec, void* rec, const PairEntry* pairs, int npairs, const double* kqcart,
-                     const double* fill, const Geom& g, long long N, const JastrowRPA& j0, double fa4, double fb5,
-                     hipStream_t s)
-{
-    for_slices(npairs, [&](int p0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_a8_comb2<double>, grid_pts(N, ny), dim3(256), 0, s, (cpx<double>*)rec, pairs, p0,
-                               kqcart, fill, g, N, j0, fa4, fb5);
-        else
-            hipLaunchKernelGGL(k_a8_comb2<float>, grid_pts(N, ny), dim3(256), 0, s, (cpx<float>*)rec, pairs, p0,
-                               kqcart, fill, g, N, j0, fa4, fb5);
         TCX_HIP_CHECK(hipGetLastError());
     });
 }

@@ -103,6 +103,8 @@ void launch_fft3d(int prec, int sign, void* grids, int ngrids, const Geom& g, do
 // the xy-plane half of a 3D transform (no scaling): with a z-pencil pass before it, a
 // complete 3D FFT
 void launch_fft_xy(int prec, int sign, void* grids, int ngrids, const Geom& g, hipStream_t s);
+// the z-pencil half, scaled by scale
+void launch_fft_z(int prec, int sign, void* grids, int ngrids, const Geom& g, double scale, hipStream_t s);
 
 // hf2x pair pipeline
 void launch_hf2x_A(int prec, const Geom& g, const void* phiR_app, const void* phiR_occ, const PairEntry* pairs,
@@ -143,16 +145,26 @@ void launch_mul_shared(double* grids, int ngrids, const double* V, long long N, 
 // records = FFT_forward(A * B) scaled by sc (the products fused with the z pass)
 void launch_products_fft(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
                          const Geom& g, double sc, hipStream_t s);
-void launch_a7_comb1(int prec, const void* rec, void* Y, void* P, const PairEntry* pairs, int npairs,
-                     const double* kqcart, const double* fill, const Geom& g, long long N, const JastrowRPA& j0,
-                     const JastrowRPA& j1, double fb3, hipStream_t s);
+// the same records transformed along z only (an xy pass with a fused combine completes them)
+void launch_products_z(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
+                       const Geom& g, double sc, hipStream_t s);
+// xy-plane passes with the three-body combines fused in (kernels_tc3xy.hip); tab holds
+// (u(kq+G, j0), u(kq+G, j1)) per k-q row of the launch (launch_u2_table)
+void launch_u2_table(int prec, void* tab, const double* kqcart, int row0, int nrows, const Geom& g,
+                     const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
+void launch_xy_a7c1(int prec, const Geom& g, const void* rec, void* P, void* Y, const PairEntry* pairs, int npairs,
+                    const void* tab, int kq0, const double* kqcart, const double* fill, double fb3, hipStream_t s);
+void launch_xy_a7c2(int prec, const Geom& g, const void* P, void* Y, const PairEntry* pairs, int npairs,
+                    const void* tab, int kq0, const double* kqcart, double fb6, hipStream_t s);
+void launch_xy_b4(int prec, const Geom& g, const void* rec, void* Z, const PairEntry* pairs, int npairs,
+                  const void* tab, int kq0, const double* kqcart, const double* fill, double fb4, hipStream_t s);
+void launch_xy_a8c1(int prec, const Geom& g, const void* rec, int nin, void* O, const PairEntry* pairs, int npairs,
+                    const void* tab, int kq0, const double* kqcart, hipStream_t s);
+void launch_xy_a8c2(int prec, const Geom& g, const void* rec, void* Z, const PairEntry* pairs, int npairs,
+                    const void* tab, int kq0, const double* kqcart, const double* fill, double fa4, double fb5,
+                    hipStream_t s);
 void launch_a7_mid(int prec, void* P, const void* occR, const PairEntry* pairs, const TargetEntry* targets,
                    int ntargets, const double* dens, double* qsum, long long N, hipStream_t s);
-void launch_a7_comb2(int prec, void* Y, const void* P, const PairEntry* pairs, int npairs, const double* kqcart,
-                     const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1, double fb6,
-                     hipStream_t s);
-void launch_b4_comb(int prec, const void* rec, void* Z, const PairEntry* pairs, int npairs, const double* kqcart,
-                    const double* fill, const Geom& g, long long N, const JastrowRPA& j0, double fb4, hipStream_t s);
 void launch_mul_dnu(int prec, const void* appR, const double* dnu, void* out, int napp, long long N, hipStream_t s);
 void launch_a3(double* W, const double* qsum, const double* dnu, int napp, long long N, double fa3, hipStream_t s);
 void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, int nk, const Geom& g, long long N,
@@ -161,9 +173,6 @@ void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l,
                     int maxocc, const double* fill, const double* t1, const double* appR, const double* chiR,
                     const double* occR, const double* tR, double fb6, long long N, double* out, double* cR,
                     double* W, hipStream_t s);
-void launch_a8_comb1(int prec, const void* rec, int nin, void* O, const PairEntry* pairs, int npairs,
-                     const double* kqcart, const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1,
-                     hipStream_t s);
 void launch_a8_acc1(int prec, const void* O, int bitc, const void* occR, const void* chiR, const PairEntry* pairs,
                     const TargetEntry* targets, int ntargets, const double* fill, double* Q2, double* Q4, double* Q5,
                     long long N, hipStream_t s);
@@ -174,9 +183,6 @@ void launch_a8_reduce(const double* Tg, const double* Q4, const double* Q5, int 
                       const double* fill, int nocc, double* V3a2, double* B2, long long N, hipStream_t s);
 void launch_a8_q45(int prec, const double* Q4, const double* Q5, int bitc, void* Q4T, void* Q5T, int ngrids,
                    long long N, hipStream_t s);
-void launch_a8_comb2(int prec, void* rec, const PairEntry* pairs, int npairs, const double* kqcart,
-                     const double* fill, const Geom& g, long long N, const JastrowRPA& j0, double fa4, double fb5,
-                     hipStream_t s);
 void launch_a8_acc2(int prec, const void* rec, const void* occR, const void* Q4T, const PairEntry* pairs,
                     const TargetEntry* targets, int ntargets, double* W, long long N, hipStream_t s);
 

@@ -138,6 +138,9 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
         const PairSet& S = pair_set(nb, kPairsAppOcc, 11, prec);
         d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 11 * gT);
         const double* kq = (const double*)S.d_kq.p;
+        int max_rows = 1;
+        for (const Launch& L : S.launches) max_rows = std::max(max_rows, L.nkq);
+        d_ktab.ensure((size_t)max_rows * gT);  // (u(kq+G, 00), u(kq+G, 10)) per k-q row
         const double* fill = (const double*)d_fill.p;
 
         // phase 1: [3b3], [3a3,b4,b6] (:179-371)
@@ -154,18 +157,18 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             char* rec4 = (char*)d_rec.p;
             char* Pb = rec4 + (size_t)4 * np * gT;
             char* Y = Pb + (size_t)6 * np * gT;
+            launch_u2_table(prec, d_ktab.p, kq, L.kq0, L.nkq, geo, j00, j10, stream);
             timed(0, np, [&] {
-                launch_products_fft(prec, pairs, np, (const ProdChan*)d_chans.p, 4, rec4, geo, 1.0 / (double)N,
-                                    stream);
+                launch_products_z(prec, pairs, np, (const ProdChan*)d_chans.p, 4, rec4, geo, 1.0 / (double)N,
+                                  stream);
             });
             timed(1, np, [&] {
-                launch_a7_comb1(prec, rec4, Y, Pb, pairs, np, kq, fill, geo, N, j00, j10, fb3, stream);
-                launch_fft3d(prec, +1, Pb, 6 * np, geo, 1.0, stream);
+                launch_xy_a7c1(prec, geo, rec4, Pb, Y, pairs, np, d_ktab.p, L.kq0, kq, fill, fb3, stream);
+                launch_fft_z(prec, +1, Pb, 6 * np, geo, 1.0, stream);
                 launch_a7_mid(prec, Pb, occR, pairs, targets, L.ntargets, (const double*)d_density.p, qsum, N,
                               stream);
-                launch_fft3d(prec, -1, Pb, 6 * np, geo, 1.0 / (double)N, stream);
-                launch_a7_comb2(prec, Y, Pb, pairs, np, kq, geo, N, j00, j10, fb6, stream);
-                launch_fft_xy(prec, +1, Y, np, geo, stream);
+                launch_fft_z(prec, -1, Pb, 6 * np, geo, 1.0 / (double)N, stream);
+                launch_xy_a7c2(prec, geo, Pb, Y, pairs, np, d_ktab.p, L.kq0, kq, fb6, stream);
             });
             timed(2, np, [&] {  // z pass of FFT_backward fused with the accumulation
                 launch_hf2x_C(prec, geo, Y, occR, pairs, targets, L.ntargets, nullptr, W, stream);
@@ -223,13 +226,13 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             const int np = L.npairs;
             char* rec3 = (char*)d_rec.p;
             char* Z = rec3 + (size_t)3 * np * gT;
+            launch_u2_table(prec, d_ktab.p, kq, L.kq0, L.nkq, geo, j00, j10, stream);
             timed(0, np, [&] {
-                launch_products_fft(prec, pairs, np, (const ProdChan*)d_chans.p, 3, rec3, geo, 1.0 / (double)N,
-                                    stream);
+                launch_products_z(prec, pairs, np, (const ProdChan*)d_chans.p, 3, rec3, geo, 1.0 / (double)N,
+                                  stream);
             });
             timed(1, np, [&] {
-                launch_b4_comb(prec, rec3, Z, pairs, np, kq, fill, geo, N, j00, fb4, stream);
-                launch_fft_xy(prec, +1, Z, np, geo, stream);
+                launch_xy_b4(prec, geo, rec3, Z, pairs, np, d_ktab.p, L.kq0, kq, fill, fb4, stream);
             });
             timed(2, np, [&] {
                 launch_hf2x_C(prec, geo, Z, occR, pairs, targets, L.ntargets, nullptr, W, stream);
@@ -288,6 +291,9 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
         {
             const PairSet& S = pair_set(nb, kPairsOccOcc, 7 * nin, prec);
             d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 7 * nin * gT);
+            int max_rows = 1;
+            for (const Launch& L : S.launches) max_rows = std::max(max_rows, L.nkq);
+            d_ktab.ensure((size_t)max_rows * gT);
             ProdChan ch[2];
             ch[0] = chan(chiR, 1, 1, 0, 1, occR, 0, 1, 0, 0);  // conj(chi_q1) phi_q2 (:356-357)
             ch[1] = chan(occR, 1, 1, 0, 1, chiR, 0, 1, 0, 0);  // conj(phi_q1) chi_q2 (:358-363)
@@ -298,13 +304,15 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
                 const int np = L.npairs;
                 char* rec = (char*)d_rec.p;
                 char* O = rec + (size_t)nin * np * gT;
+                launch_u2_table(prec, d_ktab.p, (const double*)S.d_kq.p, L.kq0, L.nkq, geo, j00, j01, stream);
                 timed(0, np, [&] {
-                    launch_products_fft(prec, pairs, np, (const ProdChan*)d_chans.p, nin, rec, geo,
-                                        1.0 / (double)N, stream);
+                    launch_products_z(prec, pairs, np, (const ProdChan*)d_chans.p, nin, rec, geo, 1.0 / (double)N,
+                                      stream);
                 });
                 timed(1, np, [&] {
-                    launch_a8_comb1(prec, rec, nin, O, pairs, np, (const double*)S.d_kq.p, geo, N, j00, j01, stream);
-                    launch_fft3d(prec, +1, O, 6 * nin * np, geo, 1.0, stream);
+                    launch_xy_a8c1(prec, geo, rec, nin, O, pairs, np, d_ktab.p, L.kq0, (const double*)S.d_kq.p,
+                                   stream);
+                    launch_fft_z(prec, +1, O, 6 * nin * np, geo, 1.0, stream);
                 });
                 timed(2, np, [&] {
                     launch_a8_acc1(prec, O, bitc ? 1 : 0, occR, chiR, pairs, targets, L.ntargets, fill, Q2, Q4,
@@ -352,8 +360,11 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
         d_W.ensure((size_t)P.n_app * g64);
         launch_zero(d_W.p, (size_t)P.n_app * g64, stream);
         {
-            const PairSet& S = pair_set(nb, kPairsAppOcc, 4, prec);
-            d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 4 * gT);
+            const PairSet& S = pair_set(nb, kPairsAppOcc, 8, prec);  // 4 input + 4 output channels
+            d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 8 * gT);
+            int max_rows = 1;
+            for (const Launch& L : S.launches) max_rows = std::max(max_rows, L.nkq);
+            d_ktab.ensure((size_t)max_rows * gT);
             ProdChan ch[4];
             ch[0] = chan(chiR, 1, 1, 0, 1, d_appR.p, 0, 1, 0, 0);  // conj(chi_q2) phi_j (:849-850)
             for (int d = 0; d < 3; ++d) ch[1 + d] = chan(d_Q5T.p, 1, 3, d, 0, d_appR.p, 0, 1, 0, 0);  // (:867-868)
@@ -362,18 +373,20 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
                 const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
                 const TargetEntry* targets = (const TargetEntry*)S.d_targets.p + L.target0;
                 const int np = L.npairs;
+                char* recA = (char*)d_rec.p;
+                char* Zo = recA + (size_t)4 * np * gT;
+                launch_u2_table(prec, d_ktab.p, (const double*)S.d_kq.p, L.kq0, L.nkq, geo, j00, j01, stream);
                 timed(0, np, [&] {
-                    launch_products_fft(prec, pairs, np, (const ProdChan*)d_chans.p, 4, d_rec.p, geo,
-                                        1.0 / (double)N, stream);
+                    launch_products_z(prec, pairs, np, (const ProdChan*)d_chans.p, 4, recA, geo, 1.0 / (double)N,
+                                      stream);
                 });
                 timed(1, np, [&] {
-                    launch_a8_comb2(prec, d_rec.p, pairs, np, (const double*)S.d_kq.p, fill, geo, N, j00, fa4, fb5,
-                                    stream);
-                    launch_fft3d(prec, +1, d_rec.p, 4 * np, geo, 1.0, stream);
+                    launch_xy_a8c2(prec, geo, recA, Zo, pairs, np, d_ktab.p, L.kq0, (const double*)S.d_kq.p, fill, fa4,
+                                   fb5, stream);
+                    launch_fft_z(prec, +1, Zo, 4 * np, geo, 1.0, stream);
                 });
                 timed(2, np, [&] {
-                    launch_a8_acc2(prec, d_rec.p, occR, d_Q4T.p, pairs, targets, L.ntargets, (double*)d_W.p, N,
-                                   stream);
+                    launch_a8_acc2(prec, Zo, occR, d_Q4T.p, pairs, targets, L.ntargets, (double*)d_W.p, N, stream);
                 });
             }
         }
