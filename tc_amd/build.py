@@ -31,6 +31,7 @@ SOURCES = [
     "csrc/kernels_local.hip",
     "csrc/kernels_tc3x.hip",
     "csrc/kernels_tc3xy.hip",
+    "csrc/kernels_tc3z.hip",
     "csrc/kernels_davidson.hip",
     "csrc/tcx_api.cpp",
     "csrc/tcx_tc.cpp",

@@ -146,54 +146,6 @@ void launch_products_fft(int prec, const PairEntry* pairs, int npairs, const Pro
 }
 
 // ----------------------------------------------------------------- tc3a3b3b4b6
-// Phase-1 real-space step (tc3a3b3b4b6.cpp:316-335): per target (k,j)
-//   qsum[j][d] += sum_p P[p][d] phi_qi  (s2 = 0 channels, before the density)
-//   P[p][3s+d] *= density_s(r)
-template <typename T>
-__global__ __launch_bounds__(256) void k_a7_mid(cpx<T>* __restrict__ P, const cpx<T>* __restrict__ occR,
-                                                const PairEntry* __restrict__ pairs,
-                                                const TargetEntry* __restrict__ targets, int t0,
-                                                const double2* __restrict__ dens, double2* __restrict__ qsum,
-                                                long long N)
-{
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const TargetEntry t = targets[t0 + blockIdx.y];
-    const double2 n0 = dens[r], n1 = dens[N + r];
-    double2 acc0 = make_double2(0.0, 0.0), acc1 = acc0, acc2 = acc0;
-    for (int i = 0; i < t.npair; ++i) {
-        const int p = t.pair0 + i;
-        const double2 ph = up(occR[(long long)pairs[p].occ * N + r]);
-        cpx<T>* pp = P + (long long)p * 6 * N + r;
-        const double2 v0 = up(pp[0]), v1 = up(pp[N]), v2 = up(pp[2 * N]);
-        acc0 = dadd(acc0, dmul(v0, ph));
-        acc1 = dadd(acc1, dmul(v1, ph));
-        acc2 = dadd(acc2, dmul(v2, ph));
-        pp[0] = down<T>(dmul(v0, n0));
-        pp[N] = down<T>(dmul(v1, n0));
-        pp[2 * N] = down<T>(dmul(v2, n0));
-        for (int c = 3; c < 6; ++c) pp[(long long)c * N] = down<T>(dmul(up(pp[(long long)c * N]), n1));
-    }
-    double2* q = qsum + (long long)t.app * 3 * N + r;
-    q[0] = dadd(q[0], acc0);
-    q[N] = dadd(q[N], acc1);
-    q[2 * N] = dadd(q[2 * N], acc2);
-}
-
-void launch_a7_mid(int prec, void* P, const void* occR, const PairEntry* pairs, const TargetEntry* targets,
-                   int ntargets, const double* dens, double* qsum, long long N, hipStream_t s)
-{
-    for_slices(ntargets, [&](int t0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_a7_mid<double>, grid_pts(N, ny), dim3(256), 0, s, (cpx<double>*)P,
-                               (const cpx<double>*)occR, pairs, targets, t0, (const double2*)dens, (double2*)qsum, N);
-        else
-            hipLaunchKernelGGL(k_a7_mid<float>, grid_pts(N, ny), dim3(256), 0, s, (cpx<float>*)P,
-                               (const cpx<float>*)occR, pairs, targets, t0, (const double2*)dens, (double2*)qsum, N);
-        TCX_HIP_CHECK(hipGetLastError());
-    });
-}
-
 // out[j][d](r) = dnu_d(r) * phi_j(r) in the working precision (dnu FP64, 3 grids)
 template <typename T>
 __global__ __launch_bounds__(256) void k_mul_dnu(const cpx<T>* __restrict__ appR, const double2* __restrict__ dnu,
@@ -382,53 +334,6 @@ void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l,
 }
 
 // ----------------------------------------------------------------- tc3a2a4b2b5
-// q1-q2 phase accumulation (tc3a2a4b2b5.cpp:377-427), per target (q2,i2):
-//   Q2 -= f1 sum_{s,d} O conj(O2)            (O2 = O for TC: -f1 |O|^2)
-//   Q4[d] += f1 O_0,d phi_q1                  Q5[d] -= f1 conj(O2_0,d) conj(chi_q1)   (BITC)
-template <typename T>
-__global__ __launch_bounds__(256) void k_a8_acc1(const cpx<T>* __restrict__ O, int bitc,
-                                                 const cpx<T>* __restrict__ occR, const cpx<T>* __restrict__ chiR,
-                                                 const PairEntry* __restrict__ pairs,
-                                                 const TargetEntry* __restrict__ targets, int t0,
-                                                 const double* __restrict__ fill, double2* __restrict__ Q2,
-                                                 double2* __restrict__ Q4, double2* __restrict__ Q5, long long N)
-{
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const TargetEntry t = targets[t0 + blockIdx.y];
-    const int nch = bitc ? 12 : 6;
-    double2 a2 = make_double2(0.0, 0.0);
-    double2 a4[3] = {a2, a2, a2}, a5[3] = {a2, a2, a2};
-    for (int i = 0; i < t.npair; ++i) {
-        const int p = t.pair0 + i;
-        const PairEntry e = pairs[p];
-        const double f1 = fill[e.occ];
-        const double2 ph = up(occR[(long long)e.occ * N + r]);
-        const double2 ch = bitc ? up(chiR[(long long)e.occ * N + r]) : ph;
-        const cpx<T>* base = O + (long long)p * nch * N + r;
-        for (int s = 0; s < 2; ++s)
-            for (int d = 0; d < 3; ++d) {
-                const double2 o = up(base[(long long)(3 * s + d) * N]);
-                const double2 o2 = bitc ? up(base[(long long)(6 + 3 * s + d) * N]) : o;
-                a2 = dadd(a2, dscale(dmul(o, dconj(o2)), -f1));
-                if (s == 0) {
-                    a4[d] = dadd(a4[d], dscale(dmul(o, ph), f1));
-                    if (bitc) a5[d] = dadd(a5[d], dscale(dconj(dmul(o2, ch)), -f1));
-                }
-            }
-    }
-    double2* q2 = Q2 + (long long)t.app * N + r;
-    *q2 = dadd(*q2, a2);
-    for (int d = 0; d < 3; ++d) {
-        double2* q4 = Q4 + ((long long)t.app * 3 + d) * N + r;
-        *q4 = dadd(*q4, a4[d]);
-        if (bitc) {
-            double2* q5 = Q5 + ((long long)t.app * 3 + d) * N + r;
-            *q5 = dadd(*q5, a5[d]);
-        }
-    }
-}
-
 // [3a2] divergence correction (tc3a2a4b2b5.cpp:496-573), per target (q2,i2), over
 // the occupied i1 at q2 (rec[p] = FFT_forward(conj(chi_q2,i1) phi_q2,i2),
 // FFT_forward(conj(chi_q2,i2) phi_q2,i1)):
@@ -506,46 +411,6 @@ __global__ __launch_bounds__(256) void k_a8_q45(const double2* __restrict__ Q4, 
     Q5T[o] = down<T>(bitc ? Q5[o] : make_double2(-a.x, a.y));
 }
 
-// W[t.app] += sum_p rec[p][0] phi_occ(p) + sum_d rec[p][1+d] Q4T[occ(p)][d]
-template <typename T>
-__global__ __launch_bounds__(256) void k_a8_acc2(const cpx<T>* __restrict__ rec, const cpx<T>* __restrict__ occR,
-                                                 const cpx<T>* __restrict__ Q4T, const PairEntry* __restrict__ pairs,
-                                                 const TargetEntry* __restrict__ targets, int t0,
-                                                 double2* __restrict__ W, long long N)
-{
-    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (r >= N) return;
-    const TargetEntry t = targets[t0 + blockIdx.y];
-    double2 acc = make_double2(0.0, 0.0);
-    for (int i = 0; i < t.npair; ++i) {
-        const int p = t.pair0 + i;
-        const int o = pairs[p].occ;
-        const cpx<T>* in = rec + (long long)p * 4 * N + r;
-        acc = dadd(acc, dmul(up(in[0]), up(occR[(long long)o * N + r])));
-        for (int d = 0; d < 3; ++d)
-            acc = dadd(acc, dmul(up(in[(long long)(1 + d) * N]), up(Q4T[((long long)o * 3 + d) * N + r])));
-    }
-    double2* w = W + (long long)t.app * N + r;
-    *w = dadd(*w, acc);
-}
-
-void launch_a8_acc1(int prec, const void* O, int bitc, const void* occR, const void* chiR, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, const double* fill, double* Q2, double* Q4, double* Q5,
-                    long long N, hipStream_t s)
-{
-    for_slices(ntargets, [&](int t0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_a8_acc1<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)O, bitc,
-                               (const cpx<double>*)occR, (const cpx<double>*)chiR, pairs, targets, t0, fill,
-                               (double2*)Q2, (double2*)Q4, (double2*)Q5, N);
-        else
-            hipLaunchKernelGGL(k_a8_acc1<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)O, bitc,
-                               (const cpx<float>*)occR, (const cpx<float>*)chiR, pairs, targets, t0, fill,
-                               (double2*)Q2, (double2*)Q4, (double2*)Q5, N);
-        TCX_HIP_CHECK(hipGetLastError());
-    });
-}
-
 void launch_a8_div(double* Tg, const double* rec, const PairEntry* pairs, const TargetEntry* targets, int ntargets,
                    const double* fill, const double* dc2, const Geom& g, long long N, const JastrowRPA& j0,
                    const JastrowRPA& j1, hipStream_t s)
@@ -575,20 +440,6 @@ void launch_a8_q45(int prec, const double* Q4, const double* Q5, int bitc, void*
         else
             hipLaunchKernelGGL(k_a8_q45<float>, grid_pts(N, ny), dim3(256), 0, s, (const double2*)Q4,
                                (const double2*)Q5, bitc, (cpx<float>*)Q4T, (cpx<float>*)Q5T, g0, N);
-        TCX_HIP_CHECK(hipGetLastError());
-    });
-}
-
-void launch_a8_acc2(int prec, const void* rec, const void* occR, const void* Q4T, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, double* W, long long N, hipStream_t s)
-{
-    for_slices(ntargets, [&](int t0, unsigned ny) {
-        if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_a8_acc2<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec,
-                               (const cpx<double>*)occR, (const cpx<double>*)Q4T, pairs, targets, t0, (double2*)W, N);
-        else
-            hipLaunchKernelGGL(k_a8_acc2<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec,
-                               (const cpx<float>*)occR, (const cpx<float>*)Q4T, pairs, targets, t0, (double2*)W, N);
         TCX_HIP_CHECK(hipGetLastError());
     });
 }

@@ -1,0 +1,330 @@
+// kernels_tc3z.hip -- z passes of FFT_backward fused with the real-space consumers of the
+// three-body record pipelines.
+//
+// A record channel arrives inverse-transformed along x and y (the fused combine pass);
+// one thread group owns a z pencil of one target and loops over the target's pairs: it
+// finishes FFT_backward along z with the pencil split over S lanes (PencilSplit), then
+// applies the consumer at the lane's output points, summing in FP64 registers, and adds
+// once into the FP64 grids of the target (no atomics, deterministic).  The consumers are
+// the reference's real-space steps:
+//   tc3a3b3b4b6 (src/calc_hamiltonian_tc3a3b3b4b6.cpp:316-335): qsum_j,d += P_0,d phi_qi,
+//     then P_s,d *= density_s, written back in real space for the forward z pass;
+//   tc3a2a4b2b5 (src/calc_hamiltonian_tc3a2a4b2b5.cpp:377-427): the q-sums Q2, Q4, Q5;
+//     (:848-913) the apply-phase accumulation into W.
+#include "kernels_common.hpp"
+#include "size_switch.hpp"
+
+namespace tcx {
+
+namespace {
+
+// lanes per pencil: Want when it divides NZ, else 2
+template <int NZ, int Want>
+struct SplitFor {
+    static constexpr int value = (NZ % Want == 0) ? Want : 2;
+};
+
+template <typename T>
+__device__ __forceinline__ double2 upd(cpx<T> a)
+{
+    return make_double2((double)a.re, (double)a.im);
+}
+__device__ __forceinline__ double2 cmadd(double2 acc, double2 a, double2 b, double s)  // acc + s a b
+{
+    return make_double2(acc.x + s * (a.x * b.x - a.y * b.y), acc.y + s * (a.x * b.y + a.y * b.x));
+}
+
+// Lane geometry of a split pencil.
+template <int S>
+struct LaneOf {
+    int h, xy, ob;
+    bool valid;
+    __device__ LaneOf(int nxy)
+    {
+        using Split = PencilSplit<S>;
+        h = threadIdx.x / Split::pencils;
+        const int xy0 = blockIdx.x * Split::pencils + (threadIdx.x % Split::pencils);
+        valid = xy0 < nxy;
+        xy = valid ? xy0 : 0;  // out-of-range lanes still join the shuffles
+        ob = Split::out_block(h);
+    }
+};
+
+// v = FFT_backward along z of the pencil of grid g (split layout in, blocked layout out)
+template <typename T, int NZ, int S>
+__device__ __forceinline__ void zinv(cpx<T> (&v)[NZ / S], const cpx<T>* grid, const LaneOf<S>& L, int nxy)
+{
+    load_pencil<T, NZ / S>(v, grid + L.xy + (size_t)L.h * nxy, S * nxy);  // z = S m + h
+    PencilSplit<S>::template fft<T, NZ, 1>(v, L.h);                      // -> z = ob M + k
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- tc3a3b3b4b6 real-space step
+// Per target (k,j) and direction d = blockIdx.z: qsum[j][d] += sum_p P[p][d] phi_qi (the
+// s2 = 0 channel, before the density), and P[p][3s+d] <- density_s * P[p][3s+d] in real
+// space (in place) for s = 0, 1.
+template <typename T, int NZ, int S>
+__global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P, const cpx<T>* __restrict__ occR,
+                                                          const PairEntry* __restrict__ pairs,
+                                                          const TargetEntry* __restrict__ targets,
+                                                          const double2* __restrict__ dens,
+                                                          double2* __restrict__ qsum, int nxy)
+{
+    constexpr int M = NZ / S;
+    const LaneOf<S> L(nxy);
+    const TargetEntry te = targets[blockIdx.y];
+    const int d = blockIdx.z;
+    const size_t ng = (size_t)nxy * NZ;
+    const size_t o0 = L.xy + (size_t)L.ob * M * nxy;  // first output point of the lane
+    double2 acc[M];
+    static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = make_double2(0.0, 0.0); });
+    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
+        const cpx<T>* ph = occR + (size_t)pairs[pp].occ * ng + o0;
+#pragma unroll 1
+        for (int s = 0; s < 2; ++s) {
+            cpx<T>* g = P + ((size_t)pp * 6 + 3 * s + d) * ng;
+            cpx<T> v[M];
+            zinv<T, NZ, S>(v, g, L, nxy);
+            const double2* n = dens + (size_t)s * ng + o0;
+            static_for<0, M>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const double2 x = upd(v[k]);
+                if (s == 0) acc[k] = cmadd(acc[k], x, upd(ph[(size_t)k * nxy]), 1.0);
+                const double2 dn = n[(size_t)k * nxy];
+                v[k] = cpx<T>{(T)(x.x * dn.x - x.y * dn.y), (T)(x.x * dn.y + x.y * dn.x)};
+            });
+            if (L.valid)
+                static_for<0, M>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    g[o0 + (size_t)k * nxy] = v[k];
+                });
+        }
+    }
+    if (!L.valid) return;
+    double2* q = qsum + ((size_t)te.app * 3 + d) * ng + o0;
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const double2 a = q[(size_t)k * nxy];
+        q[(size_t)k * nxy] = make_double2(a.x + acc[k].x, a.y + acc[k].y);
+    });
+}
+
+// ----------------------------------------------------------------- tc3a2a4b2b5 q-sums
+// Per target (q2,i2) over pairs (q1,i1), O[p][6h + 3s + d] (h = 1 only for BITC):
+//   Q2 -= f1 sum_{s,d} O_0,s,d conj(O_h2,s,d)     (h2 = 1 for BITC, 0 for TC)
+template <typename T, int NZ, int S>
+__global__ __launch_bounds__(kPencilBlock) void k_za8_q2(const cpx<T>* __restrict__ O, int bitc,
+                                                         const PairEntry* __restrict__ pairs,
+                                                         const TargetEntry* __restrict__ targets,
+                                                         const double* __restrict__ fill, double2* __restrict__ Q2,
+                                                         int nxy)
+{
+    constexpr int M = NZ / S;
+    const LaneOf<S> L(nxy);
+    const TargetEntry te = targets[blockIdx.y];
+    const size_t ng = (size_t)nxy * NZ;
+    const int nch = bitc ? 12 : 6;
+    double2 acc[M];
+    static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = make_double2(0.0, 0.0); });
+    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
+        const double f1 = fill[pairs[pp].occ];
+        const cpx<T>* base = O + (size_t)pp * nch * ng;
+        for (int c = 0; c < 6; ++c) {
+            cpx<T> v[M];
+            zinv<T, NZ, S>(v, base + (size_t)c * ng, L, nxy);
+            if (bitc) {
+                cpx<T> w[M];
+                zinv<T, NZ, S>(w, base + (size_t)(6 + c) * ng, L, nxy);
+                static_for<0, M>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    const double2 a = upd(v[k]), b = upd(w[k]);
+                    acc[k] = cmadd(acc[k], a, make_double2(b.x, -b.y), -f1);
+                });
+            } else {
+                static_for<0, M>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    const double2 a = upd(v[k]);
+                    acc[k].x -= f1 * (a.x * a.x + a.y * a.y);
+                });
+            }
+        }
+    }
+    if (!L.valid) return;
+    double2* q = Q2 + (size_t)te.app * ng + L.xy + (size_t)L.ob * M * nxy;
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const double2 a = q[(size_t)k * nxy];
+        q[(size_t)k * nxy] = make_double2(a.x + acc[k].x, a.y + acc[k].y);
+    });
+}
+
+// Q4[d] += f1 O_0,0,d phi_q1     (conj5 = 0, channel d, X = occupied right orbitals)
+// Q5[d] -= f1 conj(O_1,0,d chi_q1)  (conj5 = 1, channel 6 + d, X = occupied left orbitals; BITC)
+// with d = blockIdx.z.
+template <typename T, int NZ, int S>
+__global__ __launch_bounds__(kPencilBlock) void k_za8_q45(const cpx<T>* __restrict__ O, int nch, int ch0, int conj5,
+                                                          const cpx<T>* __restrict__ X,
+                                                          const PairEntry* __restrict__ pairs,
+                                                          const TargetEntry* __restrict__ targets,
+                                                          const double* __restrict__ fill, double2* __restrict__ Q,
+                                                          int nxy)
+{
+    constexpr int M = NZ / S;
+    const LaneOf<S> L(nxy);
+    const TargetEntry te = targets[blockIdx.y];
+    const int d = blockIdx.z;
+    const size_t ng = (size_t)nxy * NZ;
+    const size_t o0 = L.xy + (size_t)L.ob * M * nxy;
+    const double sy = conj5 ? -1.0 : 1.0;  // conj flips the imaginary part; Q5 also flips the sign
+    double2 acc[M];
+    static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = make_double2(0.0, 0.0); });
+    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
+        const PairEntry e = pairs[pp];
+        const double f1 = fill[e.occ];
+        const cpx<T>* xg = X + (size_t)e.occ * ng + o0;
+        cpx<T> v[M];
+        zinv<T, NZ, S>(v, O + ((size_t)pp * nch + ch0 + d) * ng, L, nxy);
+        static_for<0, M>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const double2 a = upd(v[k]), b = upd(xg[(size_t)k * nxy]);
+            acc[k].x += (sy * f1) * (a.x * b.x - a.y * b.y);
+            acc[k].y += f1 * (a.x * b.y + a.y * b.x);
+        });
+    }
+    if (!L.valid) return;
+    double2* q = Q + ((size_t)te.app * 3 + d) * ng + o0;
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const double2 a = q[(size_t)k * nxy];
+        q[(size_t)k * nxy] = make_double2(a.x + acc[k].x, a.y + acc[k].y);
+    });
+}
+
+// Apply phase: W[t.app] += sum_p Z[p][0] phi_occ(p) + sum_d Z[p][1+d] Q4T[occ(p)][d]
+template <typename T, int NZ, int S>
+__global__ __launch_bounds__(kPencilBlock) void k_za8_apply(const cpx<T>* __restrict__ Z,
+                                                            const cpx<T>* __restrict__ occR,
+                                                            const cpx<T>* __restrict__ Q4T,
+                                                            const PairEntry* __restrict__ pairs,
+                                                            const TargetEntry* __restrict__ targets,
+                                                            double2* __restrict__ W, int nxy)
+{
+    constexpr int M = NZ / S;
+    const LaneOf<S> L(nxy);
+    const TargetEntry te = targets[blockIdx.y];
+    const size_t ng = (size_t)nxy * NZ;
+    const size_t o0 = L.xy + (size_t)L.ob * M * nxy;
+    double2 acc[M];
+    static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = make_double2(0.0, 0.0); });
+    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
+        const int o = pairs[pp].occ;
+        for (int c = 0; c < 4; ++c) {
+            cpx<T> v[M];
+            zinv<T, NZ, S>(v, Z + ((size_t)pp * 4 + c) * ng, L, nxy);
+            const cpx<T>* m = (c == 0 ? occR + (size_t)o * ng : Q4T + ((size_t)o * 3 + (c - 1)) * ng) + o0;
+            static_for<0, M>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                acc[k] = cmadd(acc[k], upd(v[k]), upd(m[(size_t)k * nxy]), 1.0);
+            });
+        }
+    }
+    if (!L.valid) return;
+    double2* w = W + (size_t)te.app * ng + o0;
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const double2 a = w[(size_t)k * nxy];
+        w[(size_t)k * nxy] = make_double2(a.x + acc[k].x, a.y + acc[k].y);
+    });
+}
+
+// ----------------------------------------------------------------- launchers
+namespace {
+
+template <typename F>
+void for_target_slices(int n, F&& f)
+{
+    for (int a = 0; a < n; a += 65535) f(a, n - a < 65535 ? n - a : 65535);
+}
+
+}  // namespace
+
+#define TCX_Z_DISPATCH(prec, g, LAUNCH)                        \
+    do {                                                       \
+        if ((prec) == TCX_PREC_FP64) {                         \
+            using T = double;                                  \
+            TCX_SIZE_SWITCH((g).n3, LAUNCH)                    \
+        } else {                                               \
+            using T = float;                                   \
+            TCX_SIZE_SWITCH((g).n3, LAUNCH)                    \
+        }                                                      \
+        TCX_HIP_CHECK(hipGetLastError());                      \
+    } while (0)
+
+void launch_za7_mid(int prec, const Geom& g, void* P, const void* occR, const PairEntry* pairs,
+                    const TargetEntry* targets, int ntargets, const double* dens, double* qsum, hipStream_t s)
+{
+    const int nxy = g.n1 * g.n2;
+    for_target_slices(ntargets, [&](int a, int n) {
+#define TCX_L(NZ)                                                                                            \
+    {                                                                                                        \
+        constexpr int S = SplitFor<NZ, 4>::value;                                                            \
+        constexpr int PN = PencilSplit<S>::pencils;                                                          \
+        hipLaunchKernelGGL((k_za7_mid<T, NZ, S>), dim3((nxy + PN - 1) / PN, n, 3), dim3(kPencilBlock), 0, s,\
+                           (cpx<T>*)P, (const cpx<T>*)occR, pairs, targets + a, (const double2*)dens,        \
+                           (double2*)qsum, nxy);                                                             \
+    }
+        TCX_Z_DISPATCH(prec, g, TCX_L);
+#undef TCX_L
+    });
+}
+
+void launch_za8_q(int prec, const Geom& g, const void* O, int bitc, const void* occR, const void* chiR,
+                  const PairEntry* pairs, const TargetEntry* targets, int ntargets, const double* fill, double* Q2,
+                  double* Q4, double* Q5, hipStream_t s)
+{
+    const int nxy = g.n1 * g.n2;
+    const int nch = bitc ? 12 : 6;
+    for_target_slices(ntargets, [&](int a, int n) {
+#define TCX_L(NZ)                                                                                              \
+    {                                                                                                          \
+        constexpr int S2 = SplitFor<NZ, 4>::value;                                                             \
+        constexpr int S4 = SplitFor<NZ, 4>::value;                                                             \
+        constexpr int P2 = PencilSplit<S2>::pencils, P4 = PencilSplit<S4>::pencils;                            \
+        hipLaunchKernelGGL((k_za8_q2<T, NZ, S2>), dim3((nxy + P2 - 1) / P2, n), dim3(kPencilBlock), 0, s,     \
+                           (const cpx<T>*)O, bitc, pairs, targets + a, fill, (double2*)Q2, nxy);               \
+        hipLaunchKernelGGL((k_za8_q45<T, NZ, S4>), dim3((nxy + P4 - 1) / P4, n, 3), dim3(kPencilBlock), 0, s, \
+                           (const cpx<T>*)O, nch, 0, 0, (const cpx<T>*)occR, pairs, targets + a, fill,        \
+                           (double2*)Q4, nxy);                                                                 \
+        if (bitc)                                                                                              \
+            hipLaunchKernelGGL((k_za8_q45<T, NZ, S4>), dim3((nxy + P4 - 1) / P4, n, 3), dim3(kPencilBlock), 0, \
+                               s,                                                                              \
+                               (const cpx<T>*)O, nch, 6, 1, (const cpx<T>*)chiR, pairs, targets + a, fill,    \
+                               (double2*)Q5, nxy);                                                             \
+    }
+        TCX_Z_DISPATCH(prec, g, TCX_L);
+#undef TCX_L
+    });
+}
+
+void launch_za8_apply(int prec, const Geom& g, const void* Z, const void* occR, const void* Q4T,
+                      const PairEntry* pairs, const TargetEntry* targets, int ntargets, double* W, hipStream_t s)
+{
+    const int nxy = g.n1 * g.n2;
+    for_target_slices(ntargets, [&](int a, int n) {
+#define TCX_L(NZ)                                                                                              \
+    {                                                                                                          \
+        constexpr int S = SplitFor<NZ, 4>::value;                                                              \
+        constexpr int PN = PencilSplit<S>::pencils;                                                            \
+        hipLaunchKernelGGL((k_za8_apply<T, NZ, S>), dim3((nxy + PN - 1) / PN, n), dim3(kPencilBlock), 0, s,   \
+                           (const cpx<T>*)Z, (const cpx<T>*)occR, (const cpx<T>*)Q4T, pairs, targets + a,      \
+                           (double2*)W, nxy);                                                                  \
+    }
+        TCX_Z_DISPATCH(prec, g, TCX_L);
+#undef TCX_L
+    });
+}
+
+#undef TCX_Z_DISPATCH
+
+}  // namespace tcx

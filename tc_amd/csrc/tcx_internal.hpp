@@ -163,8 +163,14 @@ void launch_xy_a8c1(int prec, const Geom& g, const void* rec, int nin, void* O, 
 void launch_xy_a8c2(int prec, const Geom& g, const void* rec, void* Z, const PairEntry* pairs, int npairs,
                     const void* tab, int kq0, const double* kqcart, const double* fill, double fa4, double fb5,
                     hipStream_t s);
-void launch_a7_mid(int prec, void* P, const void* occR, const PairEntry* pairs, const TargetEntry* targets,
-                   int ntargets, const double* dens, double* qsum, long long N, hipStream_t s);
+// z passes of FFT_backward fused with the real-space consumers (kernels_tc3z.hip)
+void launch_za7_mid(int prec, const Geom& g, void* P, const void* occR, const PairEntry* pairs,
+                    const TargetEntry* targets, int ntargets, const double* dens, double* qsum, hipStream_t s);
+void launch_za8_q(int prec, const Geom& g, const void* O, int bitc, const void* occR, const void* chiR,
+                  const PairEntry* pairs, const TargetEntry* targets, int ntargets, const double* fill, double* Q2,
+                  double* Q4, double* Q5, hipStream_t s);
+void launch_za8_apply(int prec, const Geom& g, const void* Z, const void* occR, const void* Q4T,
+                      const PairEntry* pairs, const TargetEntry* targets, int ntargets, double* W, hipStream_t s);
 void launch_mul_dnu(int prec, const void* appR, const double* dnu, void* out, int napp, long long N, hipStream_t s);
 void launch_a3(double* W, const double* qsum, const double* dnu, int napp, long long N, double fa3, hipStream_t s);
 void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, int nk, const Geom& g, long long N,
@@ -173,9 +179,6 @@ void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l,
                     int maxocc, const double* fill, const double* t1, const double* appR, const double* chiR,
                     const double* occR, const double* tR, double fb6, long long N, double* out, double* cR,
                     double* W, hipStream_t s);
-void launch_a8_acc1(int prec, const void* O, int bitc, const void* occR, const void* chiR, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, const double* fill, double* Q2, double* Q4, double* Q5,
-                    long long N, hipStream_t s);
 void launch_a8_div(double* Tg, const double* rec, const PairEntry* pairs, const TargetEntry* targets, int ntargets,
                    const double* fill, const double* dc2, const Geom& g, long long N, const JastrowRPA& j0,
                    const JastrowRPA& j1, hipStream_t s);
@@ -183,8 +186,6 @@ void launch_a8_reduce(const double* Tg, const double* Q4, const double* Q5, int 
                       const double* fill, int nocc, double* V3a2, double* B2, long long N, hipStream_t s);
 void launch_a8_q45(int prec, const double* Q4, const double* Q5, int bitc, void* Q4T, void* Q5T, int ngrids,
                    long long N, hipStream_t s);
-void launch_a8_acc2(int prec, const void* rec, const void* occR, const void* Q4T, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, double* W, long long N, hipStream_t s);
 
 // kernel-gradient reduction of the occupied-set energies (rec: 1 channel E_x, 8 channels E_2x)
 void launch_kgrad(int prec, const void* rec, int tc2, const PairEntry* pairs, int npairs, const double* kqcart,

@@ -164,9 +164,8 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             });
             timed(1, np, [&] {
                 launch_xy_a7c1(prec, geo, rec4, Pb, Y, pairs, np, d_ktab.p, L.kq0, kq, fill, fb3, stream);
-                launch_fft_z(prec, +1, Pb, 6 * np, geo, 1.0, stream);
-                launch_a7_mid(prec, Pb, occR, pairs, targets, L.ntargets, (const double*)d_density.p, qsum, N,
-                              stream);
+                launch_za7_mid(prec, geo, Pb, occR, pairs, targets, L.ntargets, (const double*)d_density.p, qsum,
+                               stream);
                 launch_fft_z(prec, -1, Pb, 6 * np, geo, 1.0 / (double)N, stream);
                 launch_xy_a7c2(prec, geo, Pb, Y, pairs, np, d_ktab.p, L.kq0, kq, fb6, stream);
             });
@@ -312,11 +311,10 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
                 timed(1, np, [&] {
                     launch_xy_a8c1(prec, geo, rec, nin, O, pairs, np, d_ktab.p, L.kq0, (const double*)S.d_kq.p,
                                    stream);
-                    launch_fft_z(prec, +1, O, 6 * nin * np, geo, 1.0, stream);
                 });
-                timed(2, np, [&] {
-                    launch_a8_acc1(prec, O, bitc ? 1 : 0, occR, chiR, pairs, targets, L.ntargets, fill, Q2, Q4,
-                                   bitc ? Q5 : Q4, N, stream);
+                timed(2, np, [&] {  // z pass of FFT_backward fused with the q-sums
+                    launch_za8_q(prec, geo, O, bitc ? 1 : 0, occR, chiR, pairs, targets, L.ntargets, fill, Q2, Q4,
+                                 bitc ? Q5 : Q4, stream);
                 });
             }
         }
@@ -383,10 +381,9 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
                 timed(1, np, [&] {
                     launch_xy_a8c2(prec, geo, recA, Zo, pairs, np, d_ktab.p, L.kq0, (const double*)S.d_kq.p, fill, fa4,
                                    fb5, stream);
-                    launch_fft_z(prec, +1, Zo, 4 * np, geo, 1.0, stream);
                 });
                 timed(2, np, [&] {
-                    launch_a8_acc2(prec, Zo, occR, d_Q4T.p, pairs, targets, L.ntargets, (double*)d_W.p, N, stream);
+                    launch_za8_apply(prec, geo, Zo, occR, d_Q4T.p, pairs, targets, L.ntargets, (double*)d_W.p, stream);
                 });
             }
         }
