@@ -43,6 +43,8 @@ SOURCES = [
 ROOT = os.path.dirname(HERE)
 SHIM_TEST_SRC = os.path.join(ROOT, "tests", "host_shim_test.cpp")
 SHIM_TEST_BIN = os.path.join(ROOT, "tests", "_bin", "host_shim_test")
+FFT_TEST_SRC = os.path.join(ROOT, "tests", "fft_reg_test.cpp")
+FFT_TEST_BIN = os.path.join(ROOT, "tests", "_bin", "fft_reg_test")
 
 
 def _compile(src: str) -> str:
@@ -76,6 +78,7 @@ def build(verbose: bool = True) -> str:
     if verbose:
         print(f"built {LIB}")
     build_shim_test(verbose)
+    build_fft_test(verbose)
     return LIB
 
 
@@ -90,6 +93,19 @@ def build_shim_test(verbose: bool = True) -> str:
     if verbose:
         print(f"built {SHIM_TEST_BIN}")
     return SHIM_TEST_BIN
+
+
+def build_fft_test(verbose: bool = True) -> str:
+    """Host program checking the register FFT templates against a direct DFT (a CPU test)."""
+    os.makedirs(os.path.dirname(FFT_TEST_BIN), exist_ok=True)
+    cmd = [HIPCC, "-x", "c++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-I", CSRC,
+           FFT_TEST_SRC, "-o", FFT_TEST_BIN]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"fft test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {FFT_TEST_BIN}")
+    return FFT_TEST_BIN
 
 
 if __name__ == "__main__":

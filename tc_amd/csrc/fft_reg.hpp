@@ -1,8 +1,9 @@
 // fft_reg.hpp -- register-resident mixed-radix FFTs for gfx950.
 //
 // One thread owns one whole 1D line (a z-pencil, an x-row or a y-column of
-// the FFT grid) in VGPRs and transforms it with a fully unrolled Stockham
-// autosort sequence of radix-8/5/4/3/2 butterflies.  Every index and twiddle
+// the FFT grid) in VGPRs and transforms it with a fully unrolled prime-factor
+// split (power-of-two part x odd part, no twiddles between them) over Stockham
+// autosort sequences of radix-8/5/4/3/2 butterflies.  Every index and twiddle
 // factor is a compile-time constant: the twiddles are evaluated in double by
 // constexpr code (exact integer range reduction + Taylor series) and rounded
 // once to the working precision, so the FP32 path carries correctly rounded
@@ -286,12 +287,78 @@ __host__ __device__ __forceinline__ void fft_stages(cpx<T> (&x)[N])
     }
 }
 
-// In-place length-N DFT of a register array (unscaled, natural order in and out).
+// ---------------------------------------------------------------------------
+// Prime-factor (Good-Thomas) split N = N1 N2 with gcd(N1, N2) = 1: with the input
+// index n = (N2 n1 + N1 n2) mod N and the output index k (k = k1 mod N1, k = k2 mod N2,
+// Chinese remainder), exp(+-2 pi i n k / N) = W_N1^(n1 k1) W_N2^(n2 k2), so the length-N
+// DFT is an N1 x N2 two-dimensional DFT with no twiddle factors between the stages.
+// Both index maps are compile-time register renamings.  N1 is the power-of-two part.
+// ---------------------------------------------------------------------------
+constexpr int pow2_part(int n)
+{
+    int p = 1;
+    while (n % 2 == 0) {
+        n /= 2;
+        p *= 2;
+    }
+    return p;
+}
+constexpr int crt_index(int k1, int k2, int n1, int n2)
+{
+    for (int k = 0; k < n1 * n2; ++k)
+        if (k % n1 == k1 && k % n2 == k2) return k;
+    return -1;
+}
+
+template <typename T, int N, int SIGN>
+__host__ __device__ __forceinline__ void fft(cpx<T> (&x)[N]);
+
+template <typename T, int N, int N1, int SIGN>
+__host__ __device__ __forceinline__ void fft_pfa(cpx<T> (&x)[N])
+{
+    constexpr int N2 = N / N1;
+    cpx<T> A[N];  // A[k1 * N2 + n2]
+    static_for<0, N2>([&](auto n2c) {
+        constexpr int n2 = decltype(n2c)::value;
+        cpx<T> a[N1];
+        static_for<0, N1>([&](auto n1c) {
+            constexpr int n1 = decltype(n1c)::value;
+            constexpr int n = (N2 * n1 + N1 * n2) % N;  // constant: keeps x in registers
+            a[n1] = x[n];
+        });
+        fft<T, N1, SIGN>(a);
+        static_for<0, N1>([&](auto k1c) {
+            constexpr int k1 = decltype(k1c)::value;
+            A[k1 * N2 + n2] = a[k1];
+        });
+    });
+    static_for<0, N1>([&](auto k1c) {
+        constexpr int k1 = decltype(k1c)::value;
+        cpx<T> b[N2];
+        static_for<0, N2>([&](auto n2c) {
+            constexpr int n2 = decltype(n2c)::value;
+            b[n2] = A[k1 * N2 + n2];
+        });
+        fft<T, N2, SIGN>(b);
+        static_for<0, N2>([&](auto k2c) {
+            constexpr int k2 = decltype(k2c)::value;
+            constexpr int k = crt_index(k1, k2, N1, N2);
+            x[k] = b[k2];
+        });
+    });
+}
+
+// In-place length-N DFT of a register array (unscaled, natural order in and out):
+// prime-factor split when N has a power-of-two part and an odd part, else Stockham.
 template <typename T, int N, int SIGN>
 __host__ __device__ __forceinline__ void fft(cpx<T> (&x)[N])
 {
     static_assert(SIGN == 1 || SIGN == -1, "sign");
-    if constexpr (N > 1) fft_stages<T, N, 1, SIGN>(x);
+    constexpr int N1 = pow2_part(N);
+    if constexpr (N1 > 1 && N1 < N)
+        fft_pfa<T, N, N1, SIGN>(x);
+    else if constexpr (N > 1)
+        fft_stages<T, N, 1, SIGN>(x);
 }
 
 }  // namespace tcx
