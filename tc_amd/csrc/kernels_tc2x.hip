@@ -71,74 +71,100 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_A(const cpx<T>* __restric
 }
 
 // ----------------------------------------------------------------- pass B
+// A workgroup owns P planes z of one pair, all four channels: P*4 plane-channels of NP
+// lines each, packed onto consecutive lanes (lane l -> plane-channel l / NP, line l % NP)
+// so that almost every lane transforms a line.  The channel combination at every G of
+// the P planes is spread over all lanes of the workgroup.
+// LDS per workgroup capped so that two or more workgroups share a CU (one 160 KB
+// workgroup per CU measured 40% slower at 40^3 than 52 KB ones)
+constexpr int kTc2xLdsCap = 80 * 1024;
 template <typename T, int NP>
-struct Tiles4 {
-    PlaneTile<T, NP> t[4];
+struct Tc2xPlanes {
+    static constexpr int tile_bytes = NP * (NP + 1) * (int)sizeof(cpx<T>);
+    static constexpr int pick()
+    {
+        int best = 1;
+        double best_util = 0.0;
+        for (int p = 1; p <= 8; ++p) {
+            if (4 * p * tile_bytes > kTc2xLdsCap || 4 * p * NP > 1024) break;
+            const int lanes = ((4 * p * NP + 63) / 64) * 64;
+            const double util = double(4 * p * NP) / lanes;
+            if (util > best_util + 1e-9) {
+                best_util = util;
+                best = p;
+            }
+        }
+        return best;
+    }
+    static constexpr int planes = pick();
+    static constexpr int threads = ((4 * planes * NP + 63) / 64) * 64;
 };
 
-// One workgroup per (pair, plane z) with four wavefronts: wave c transforms
-// channel c; the channel combination is spread over all 256 lanes.
-constexpr int kTc2xPlaneBlock = 256;
-
 template <typename T, int NP>
-__global__ __launch_bounds__(kTc2xPlaneBlock) void k_tc2x_B(cpx<T>* __restrict__ buf,
-                                                            const PairEntry* __restrict__ pairs,
-                                                            const cpx<T>* __restrict__ table, int kq0,
-                                                            const double* __restrict__ kqcart, int nz, Geom g,
-                                                            double invN)
+__global__ __launch_bounds__((Tc2xPlanes<T, NP>::threads)) void k_tc2x_B(cpx<T>* __restrict__ buf,
+                                                                       const PairEntry* __restrict__ pairs,
+                                                                       const cpx<T>* __restrict__ table, int kq0,
+                                                                       const double* __restrict__ kqcart, int nz,
+                                                                       Geom g, double invN)
 {
-    __shared__ Tiles4<T, NP> S;
-    const int c = threadIdx.x >> 6;   // channel of this wavefront
-    const int t = threadIdx.x & 63;
-    const int z = blockIdx.x;
+    constexpr int P = Tc2xPlanes<T, NP>::planes;
+    constexpr int NT = Tc2xPlanes<T, NP>::threads;
+    __shared__ PlaneTile<T, NP> S[P * 4];
+    const int pc = threadIdx.x / NP, t = threadIdx.x % NP;
+    const int p = pc >> 2, c = pc & 3;  // plane slot and channel of this lane's line
+    const int z = blockIdx.x * P + p;
+    const bool on = pc < 4 * P && z < nz;
     const size_t ng = (size_t)NP * NP * nz;
-    cpx<T>* base = buf + ((size_t)blockIdx.y * 4 + c) * ng + (size_t)z * NP * NP;
+    PlaneTile<T, NP>& tile = S[on ? pc : 0];
+    cpx<T>* base = buf + ((size_t)blockIdx.y * 4 + (on ? c : 0)) * ng + (size_t)(on ? z : 0) * NP * NP;
     cpx<T> v[NP];
-    if (t < NP) {
+    if (on) {
         load_pencil<T, NP>(v, base + t, NP);   // column x = t
         fft<T, NP, -1>(v);                     // along y
-        tile_put_col<T, NP>(S.t[c], v, t);
+        tile_put_col<T, NP>(tile, v, t);
     }
     __syncthreads();
-    if (t < NP) {
-        tile_get_row<T, NP>(S.t[c], v, t);     // row y = t
+    if (on) {
+        tile_get_row<T, NP>(tile, v, t);       // row y = t
         fft<T, NP, -1>(v);                     // along x
-        tile_put_row<T, NP>(S.t[c], v, t);
+        tile_put_row<T, NP>(tile, v, t);
     }
     __syncthreads();
     {
         // combine at frequency (x, y, z): b = rho V2 + sum_d sigma_d du_d, a_d = rho du_d (1/N folded in)
         const PairEntry pe = pairs[blockIdx.y];
         const double* kq = kqcart + 3 * pe.kq;
-        const cpx<T>* tab = table + (size_t)(pe.kq - kq0) * ng + (size_t)z * NP * NP;
-        const int mz = centered(z, nz);
-        const double kz0 = kq[0] + g.b[6] * mz, kz1 = kq[1] + g.b[7] * mz, kz2 = kq[2] + g.b[8] * mz;
-        for (int e = threadIdx.x; e < NP * NP; e += kTc2xPlaneBlock) {
-            const int x = e % NP, y = e / NP;
-            const int mx = centered(x, NP), my = centered(y, NP);
-            const cpx<T> vu = tab[e];  // (V2, u), coalesced over e
+        const cpx<T>* tab = table + (size_t)(pe.kq - kq0) * ng;
+        for (int e = threadIdx.x; e < P * NP * NP; e += NT) {
+            const int q = e / (NP * NP), xy = e % (NP * NP);
+            const int zz = blockIdx.x * P + q;
+            if (zz >= nz) continue;
+            const int x = xy % NP, y = xy / NP;
+            const int mx = centered(x, NP), my = centered(y, NP), mz = centered(zz, nz);
+            const cpx<T> vu = tab[(size_t)zz * NP * NP + xy];  // (V2, u), coalesced over e
             const double us = (double)vu.im * invN;
-            const T du0 = static_cast<T>((kz0 + g.b[0] * mx + g.b[3] * my) * us);
-            const T du1 = static_cast<T>((kz1 + g.b[1] * mx + g.b[4] * my) * us);
-            const T du2 = static_cast<T>((kz2 + g.b[2] * mx + g.b[5] * my) * us);
+            const T du0 = static_cast<T>((kq[0] + g.b[0] * mx + g.b[3] * my + g.b[6] * mz) * us);
+            const T du1 = static_cast<T>((kq[1] + g.b[1] * mx + g.b[4] * my + g.b[7] * mz) * us);
+            const T du2 = static_cast<T>((kq[2] + g.b[2] * mx + g.b[5] * my + g.b[8] * mz) * us);
             const T v2 = static_cast<T>((double)vu.re * invN);
-            const cpx<T> r = S.t[0].a[y][x];
-            const cpx<T> s0 = S.t[1].a[y][x], s1 = S.t[2].a[y][x], s2 = S.t[3].a[y][x];
-            S.t[0].a[y][x] = scale(r, v2) + scale(s0, du0) + scale(s1, du1) + scale(s2, du2);
-            S.t[1].a[y][x] = scale(r, du0);
-            S.t[2].a[y][x] = scale(r, du1);
-            S.t[3].a[y][x] = scale(r, du2);
+            PlaneTile<T, NP>* Q = S + 4 * q;
+            const cpx<T> r = Q[0].a[y][x];
+            const cpx<T> s0 = Q[1].a[y][x], s1 = Q[2].a[y][x], s2 = Q[3].a[y][x];
+            Q[0].a[y][x] = scale(r, v2) + scale(s0, du0) + scale(s1, du1) + scale(s2, du2);
+            Q[1].a[y][x] = scale(r, du0);
+            Q[2].a[y][x] = scale(r, du1);
+            Q[3].a[y][x] = scale(r, du2);
         }
     }
     __syncthreads();
-    if (t < NP) {
-        tile_get_row<T, NP>(S.t[c], v, t);
+    if (on) {
+        tile_get_row<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
-        tile_put_row<T, NP>(S.t[c], v, t);
+        tile_put_row<T, NP>(tile, v, t);
     }
     __syncthreads();
-    if (t < NP) {
-        tile_get_col<T, NP>(S.t[c], v, t);
+    if (on) {
+        tile_get_col<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
         store_pencil<T, NP>(v, base + t, NP);
     }
@@ -241,9 +267,13 @@ static void tc2x_B_impl(const Geom& g, void* buf, const PairEntry* pairs, int np
 {
     if (g.n1 != g.n2) throw Error(TCX_ERR_UNSUPPORTED, "FFT grid must have n1 == n2");
     const double invN = 1.0 / ((double)g.n1 * g.n2 * g.n3);
-#define TCX_B(NP)                                                                                        \
-    hipLaunchKernelGGL((k_tc2x_B<T, NP>), dim3(g.n3, npairs), dim3(kTc2xPlaneBlock), 0, s, (cpx<T>*)buf, \
-                       pairs, (const cpx<T>*)table, kq0, kqcart, g.n3, g, invN);
+#define TCX_B(NP)                                                                                         \
+    {                                                                                                     \
+        using TP = Tc2xPlanes<T, NP>;                                                                     \
+        hipLaunchKernelGGL((k_tc2x_B<T, NP>), dim3((g.n3 + TP::planes - 1) / TP::planes, npairs),         \
+                           dim3(TP::threads), 0, s, (cpx<T>*)buf, pairs, (const cpx<T>*)table, kq0, kqcart, \
+                           g.n3, g, invN);                                                                \
+    }
     TCX_SIZE_SWITCH(g.n1, TCX_B)
 #undef TCX_B
     TCX_HIP_CHECK(hipGetLastError());
