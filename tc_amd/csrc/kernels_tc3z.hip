@@ -8,7 +8,7 @@
 // once into the FP64 grids of the target (no atomics, deterministic).  The consumers are
 // the reference's real-space steps:
 //   tc3a3b3b4b6 (src/calc_hamiltonian_tc3a3b3b4b6.cpp:316-335): qsum_j,d += P_0,d phi_qi,
-//     then P_s,d *= density_s, written back in real space for the forward z pass;
+//     then P_s,d *= density_s, transformed along z again for the next FFT_forward;
 //   tc3a2a4b2b5 (src/calc_hamiltonian_tc3a2a4b2b5.cpp:377-427): the q-sums Q2, Q4, Q5;
 //     (:848-913) the apply-phase accumulation into W.
 #include "kernels_common.hpp"
@@ -62,17 +62,22 @@ __device__ __forceinline__ void zinv(cpx<T> (&v)[NZ / S], const cpx<T>* grid, co
 
 // ----------------------------------------------------------------- tc3a3b3b4b6 real-space step
 // Per target (k,j) and direction d = blockIdx.z: qsum[j][d] += sum_p P[p][d] phi_qi (the
-// s2 = 0 channel, before the density), and P[p][3s+d] <- density_s * P[p][3s+d] in real
-// space (in place) for s = 0, 1.
+// s2 = 0 channel, before the density), and P[p][3s+d] <- FFT_z(density_s * P[p][3s+d]) * sc
+// for s = 0, 1 (in place): the real-space step between the z pass of FFT_backward and the
+// z pass of the next FFT_forward.  The lanes of a pencil move from the blocked output layout
+// of one split transform to the strided input layout of the next through LDS.
 template <typename T, int NZ, int S>
 __global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P, const cpx<T>* __restrict__ occR,
                                                           const PairEntry* __restrict__ pairs,
                                                           const TargetEntry* __restrict__ targets,
                                                           const double2* __restrict__ dens,
-                                                          double2* __restrict__ qsum, int nxy)
+                                                          double2* __restrict__ qsum, int nxy, T sc)
 {
     constexpr int M = NZ / S;
+    constexpr int PN = PencilSplit<S>::pencils;
+    __shared__ cpx<T> xs[PN][NZ + 1];  // one pencil per row (+1 against bank conflicts)
     const LaneOf<S> L(nxy);
+    const int pi = threadIdx.x % PN;
     const TargetEntry te = targets[blockIdx.y];
     const int d = blockIdx.z;
     const size_t ng = (size_t)nxy * NZ;
@@ -92,12 +97,19 @@ __global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P
                 const double2 x = upd(v[k]);
                 if (s == 0) acc[k] = cmadd(acc[k], x, upd(ph[(size_t)k * nxy]), 1.0);
                 const double2 dn = n[(size_t)k * nxy];
-                v[k] = cpx<T>{(T)(x.x * dn.x - x.y * dn.y), (T)(x.x * dn.y + x.y * dn.x)};
+                xs[pi][L.ob * M + k] = cpx<T>{(T)(x.x * dn.x - x.y * dn.y), (T)(x.x * dn.y + x.y * dn.x)};
             });
+            __syncthreads();
+            static_for<0, M>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                v[m] = xs[pi][S * m + L.h];  // z = S m + h
+            });
+            __syncthreads();
+            PencilSplit<S>::template fft<T, NZ, -1>(v, L.h);  // -> z = ob M + k
             if (L.valid)
                 static_for<0, M>([&](auto kc) {
                     constexpr int k = decltype(kc)::value;
-                    g[o0 + (size_t)k * nxy] = v[k];
+                    g[o0 + (size_t)k * nxy] = scale(v[k], sc);
                 });
         }
     }
@@ -262,7 +274,8 @@ void for_target_slices(int n, F&& f)
     } while (0)
 
 void launch_za7_mid(int prec, const Geom& g, void* P, const void* occR, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, const double* dens, double* qsum, hipStream_t s)
+                    const TargetEntry* targets, int ntargets, const double* dens, double* qsum, double sc,
+                    hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
     for_target_slices(ntargets, [&](int a, int n) {
@@ -272,7 +285,7 @@ void launch_za7_mid(int prec, const Geom& g, void* P, const void* occR, const Pa
         constexpr int PN = PencilSplit<S>::pencils;                                                          \
         hipLaunchKernelGGL((k_za7_mid<T, NZ, S>), dim3((nxy + PN - 1) / PN, n, 3), dim3(kPencilBlock), 0, s,\
                            (cpx<T>*)P, (const cpx<T>*)occR, pairs, targets + a, (const double2*)dens,        \
-                           (double2*)qsum, nxy);                                                             \
+                           (double2*)qsum, nxy, (T)sc);                                                      \
     }
         TCX_Z_DISPATCH(prec, g, TCX_L);
 #undef TCX_L
