@@ -750,8 +750,14 @@ int tcx_total_energy(int nk, const int* nb, const double* filling, const double*
                      int uses_3body, double* out)
 {
     if (nk < 0 || !nb || !filling || !energies || !out || num_kpoints <= 0) return TCX_ERR_ARG;
-    // TotalEnergy::calc_total_energy (src/total_energy.cpp:98-163), identity symmetry, no Ewald term
-    double e[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    // TotalEnergy::calc_total_energy (src/total_energy.cpp:98-163), identity symmetry, no Ewald term;
+    // band sums compensated (Neumaier)
+    double e[3][2] = {{0, 0}, {0, 0}, {0, 0}}, c[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    auto add = [](double& sum, double& comp, double x) {
+        const double t = sum + x;
+        comp += std::fabs(sum) >= std::fabs(x) ? (sum - t) + x : (x - t) + sum;
+        sum = t;
+    };
     long long b = 0;
     for (int k = 0; k < nk; ++k) {
         if (nb[k] < 0) return TCX_ERR_ARG;
@@ -759,14 +765,14 @@ int tcx_total_energy(int nk, const int* nb, const double* filling, const double*
             const double f = filling[b];
             for (int n = 0; n < 3; ++n) {
                 if (n == 2 && !uses_3body) continue;
-                e[n][0] += f * energies[(b * 3 + n) * 2] / (n + 1);
-                e[n][1] += f * energies[(b * 3 + n) * 2 + 1] / (n + 1);
+                add(e[n][0], c[n][0], f * energies[(b * 3 + n) * 2] / (n + 1));
+                add(e[n][1], c[n][1], f * energies[(b * 3 + n) * 2 + 1] / (n + 1));
             }
         }
     }
     for (int n = 0; n < 3; ++n) {
-        out[2 * n] = e[n][0] / num_kpoints;
-        out[2 * n + 1] = e[n][1] / num_kpoints;
+        out[2 * n] = (e[n][0] + c[n][0]) / num_kpoints;
+        out[2 * n + 1] = (e[n][1] + c[n][1]) / num_kpoints;
     }
     out[6] = out[0] + out[2] + out[4];
     out[7] = out[1] + out[3] + out[5];
