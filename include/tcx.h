@@ -102,6 +102,25 @@ int tcx_set_jastrow_rpa(tcx_context *ctx, const double A[4], const double F[4]);
  * alpha_Vaux and sum_of_Vaux_scf[nk] (src/potentials_coulomb.cpp:9-161). */
 int tcx_set_div_correction(tcx_context *ctx, int enabled, double alpha, const double *sum_of_vaux);
 
+/* Symmetry of the SCF k-mesh: k is built from the orbitals of the
+ * irreducible k_src[k] (k_src[q] == q for an irreducible q, which needs the
+ * identity below).  rot[9k..9k+8]: the TC++ rotation U of k (integer, crystal
+ * coordinates, row-major; the inverse of QE's rotation), trans[3k..3k+2]: its
+ * fractional translation, trev[k]: time reversal used.  k = +-U^T k_src in
+ * crystal coordinates.  Replaces Kpoints::index_of_rotation_at_k /
+ * is_time_reversal_used_at_k (src/kpoints.cpp:34-110) and Symmetry::rotation /
+ * translation (src/symmetry.cpp:17-33).  The occupied orbitals of
+ * tcx_set_states at k must be those of k_src[k] mapped as in
+ * src/plane_wave_basis.cpp:115-140 (c(G) placed at G' = +-U^T G, conjugated
+ * under time reversal, times exp(2 pi i G'.tau)), with the same fillings.
+ * With a symmetry set, tcx_tc3a2a4b2b5 forms its q2-resolved sums at the
+ * irreducible q2 only and rotates them to the other q2, as the reference does
+ * (src/calc_hamiltonian_tc3a2a4b2b5.cpp:143-765); the other terms read the
+ * full mesh as given.  k_src == NULL restores the identity symmetry.  Cleared
+ * by tcx_set_cell / tcx_set_kpoints. */
+int tcx_set_kpoint_symmetry(tcx_context *ctx, const int *k_src, const int *rot, const double *trans,
+                            const int *trev);
+
 /* Host utility computing alpha and sum_of_Vaux for a k-set, restating
  * Potentials::set_sum_of_Vaux_each (src/potentials_coulomb.cpp:60-161). */
 int tcx_sum_of_vaux(const double recip_rows[9], double volume, int nk, const double *kfrac,

@@ -411,6 +411,65 @@ __global__ __launch_bounds__(256) void k_a8_q45(const double2* __restrict__ Q4, 
     Q5T[o] = down<T>(bitc ? Q5[o] : make_double2(-a.x, a.y));
 }
 
+// Symmetry rotation of q2-resolved sums from an irreducible q2 to a replica
+// (tc3a2a4b2b5.cpp:576-640 scalar [3a2], :667-745 Cartesian vectors [3a4] [3b5]), in G space:
+//   G' = s U^+ G wrapped into the grid, s = -1 with time reversal,
+//   dst(G') = e^{2 pi i tau.G'} c(src(G))                           (scalar)
+//   dst_d(G') = s e^{2 pi i tau.G'} sum_e U^+_cart(d,e) c(src_e(G))  (vector)
+// with c = conj under time reversal.  One thread per source point; the map
+// G -> G' is a bijection of the grid (checked when the symmetry is set), so the
+// scattered stores of one job never collide, and jobs write distinct grids.
+template <int NC>
+__global__ __launch_bounds__(256) void k_sym_rotate(double2* __restrict__ Q, const SymRotJob* __restrict__ jobs,
+                                                    int j0, Geom g, long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const SymRotJob& J = jobs[j0 + blockIdx.y];
+    int G[3];
+    miller_of(g, r, G[0], G[1], G[2]);
+    const int s = J.trev ? -1 : 1;
+    const int n[3] = {g.n1, g.n2, g.n3};
+    int w[3];
+    for (int i = 0; i < 3; ++i) {
+        int v = s * (J.ud[3 * i] * G[0] + J.ud[3 * i + 1] * G[1] + J.ud[3 * i + 2] * G[2]);
+        v %= n[i];
+        w[i] = v < 0 ? v + n[i] : v;
+    }
+    double sn, cs;
+    sincos(2.0 * 3.14159265358979323846 * (J.tau[0] * w[0] + J.tau[1] * w[1] + J.tau[2] * w[2]), &sn, &cs);
+    const double2 ph = make_double2(cs, sn);
+    const long long rd = w[0] + (long long)g.n1 * (w[1] + (long long)g.n2 * w[2]);
+    if (NC == 1) {
+        double2 v = Q[(long long)J.src * N + r];
+        if (J.trev) v = dconj(v);
+        Q[(long long)J.dst * N + rd] = dmul(ph, v);
+    } else {
+        double2 v[3];
+        for (int e = 0; e < 3; ++e) {
+            v[e] = Q[((long long)J.src * 3 + e) * N + r];
+            if (J.trev) v[e] = dconj(v[e]);
+        }
+        for (int d = 0; d < 3; ++d) {
+            double2 a = make_double2(0.0, 0.0);
+            for (int e = 0; e < 3; ++e) a = dadd(a, dscale(v[e], J.udc[3 * d + e]));
+            Q[((long long)J.dst * 3 + d) * N + rd] = dscale(dmul(ph, a), (double)s);
+        }
+    }
+}
+
+void launch_sym_rotate(double* Q, int ncomp, const SymRotJob* jobs, int njobs, const Geom& g, long long N,
+                       hipStream_t s)
+{
+    for_slices(njobs, [&](int j0, unsigned ny) {
+        if (ncomp == 1)
+            hipLaunchKernelGGL(k_sym_rotate<1>, grid_pts(N, ny), dim3(256), 0, s, (double2*)Q, jobs, j0, g, N);
+        else
+            hipLaunchKernelGGL(k_sym_rotate<3>, grid_pts(N, ny), dim3(256), 0, s, (double2*)Q, jobs, j0, g, N);
+        TCX_HIP_CHECK(hipGetLastError());
+    });
+}
+
 void launch_a8_div(double* Tg, const double* rec, const PairEntry* pairs, const TargetEntry* targets, int ntargets,
                    const double* fill, const double* dc2, const Geom& g, long long N, const JastrowRPA& j0,
                    const JastrowRPA& j1, hipStream_t s)

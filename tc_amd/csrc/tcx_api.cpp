@@ -140,6 +140,7 @@ void Context::set_cell(const double recip_rows[9], double vol, const int grid[3]
     nspin = ns;
     method = meth;
     cell_set = true;
+    sym_set = false;
     basis_set = false;
     states_set = false;
     invalidate_states();
@@ -163,6 +164,7 @@ void Context::set_kpoints(int nk_, const double* kf, int numk)
     basis_set = false;
     states_set = false;
     div_enabled = false;
+    sym_set = false;
     // no occupied orbitals until tcx_set_states (the one-body term needs none)
     nocc.assign(nk, 0);
     occ_first.assign(nk + 1, 0);
@@ -199,6 +201,7 @@ void Context::invalidate_states()
     chiR_valid[0] = chiR_valid[1] = false;
     gradR_valid[0] = gradR_valid[1] = false;
     density_valid = false;
+    sym_jobs_valid = false;
     plans.clear();
     pairsets.clear();  // pair lists index the occupied orbitals and k-points
 }
@@ -273,6 +276,108 @@ void Context::set_div(int enabled, double alpha_, const double* svaux)
         upload(d_dc, dc.data(), dc.size() * sizeof(double));
     }
     div_enabled = enabled != 0;
+}
+
+// k-mesh symmetry: each k is built from the orbitals of k_src[k] by the TC++
+// rotation U (symmetry.cpp:17-33), translation tau and optional time reversal,
+// as in Kpoints::set_kvectors_by_symmetry (kpoints.cpp:34-110) and
+// PlaneWaveBasis (plane_wave_basis.cpp:115-140).
+void Context::set_symmetry(const int* k_src, const int* rot, const double* trans, const int* trev)
+{
+    require(k_set, TCX_ERR_STATE, "set_kpoint_symmetry: call tcx_set_kpoints first");
+    sym_set = false;
+    invalidate_states();
+    if (k_src == nullptr) return;  // identity symmetry
+    require(rot && trans && trev, TCX_ERR_ARG, "set_kpoint_symmetry: null pointer");
+    // lattice rows a_i = 2 pi (B^-1)^T, B = reciprocal rows
+    const double* B = geo.b;
+    const double det = B[0] * (B[4] * B[8] - B[5] * B[7]) - B[1] * (B[3] * B[8] - B[5] * B[6]) +
+                       B[2] * (B[3] * B[7] - B[4] * B[6]);
+    double Binv[9];  // B^-1
+    Binv[0] = (B[4] * B[8] - B[5] * B[7]) / det;
+    Binv[1] = (B[2] * B[7] - B[1] * B[8]) / det;
+    Binv[2] = (B[1] * B[5] - B[2] * B[4]) / det;
+    Binv[3] = (B[5] * B[6] - B[3] * B[8]) / det;
+    Binv[4] = (B[0] * B[8] - B[2] * B[6]) / det;
+    Binv[5] = (B[2] * B[3] - B[0] * B[5]) / det;
+    Binv[6] = (B[3] * B[7] - B[4] * B[6]) / det;
+    Binv[7] = (B[1] * B[6] - B[0] * B[7]) / det;
+    Binv[8] = (B[0] * B[4] - B[1] * B[3]) / det;
+    double Alat[9];  // rows a_i
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Alat[3 * i + j] = 2.0 * kPiHost * Binv[3 * j + i];
+    const int n[3] = {geo.n1, geo.n2, geo.n3};
+    std::vector<SymRotJob> ops(nk);
+    std::vector<char> seen((size_t)N);
+    for (int k = 0; k < nk; ++k) {
+        const int s = k_src[k];
+        require(s >= 0 && s < nk, TCX_ERR_ARG, "set_kpoint_symmetry: k_src out of range");
+        require(k_src[s] == s, TCX_ERR_ARG, "set_kpoint_symmetry: k_src must name an irreducible k (k_src[q] == q)");
+        const int* U = rot + 9 * k;
+        const double* t = trans + 3 * k;
+        require(trev[k] == 0 || trev[k] == 1, TCX_ERR_ARG, "set_kpoint_symmetry: trev must be 0 or 1");
+        const long dU = (long)U[0] * (U[4] * U[8] - U[5] * U[7]) - (long)U[1] * (U[3] * U[8] - U[5] * U[6]) +
+                        (long)U[2] * (U[3] * U[7] - U[4] * U[6]);
+        require(dU == 1 || dU == -1, TCX_ERR_ARG, "set_kpoint_symmetry: rotation is not unimodular");
+        if (s == k) {
+            bool ident = trev[k] == 0;
+            for (int i = 0; i < 9; ++i) ident = ident && U[i] == (i % 4 == 0 ? 1 : 0);
+            for (int d = 0; d < 3; ++d) ident = ident && std::fabs(t[d]) < 1e-12;
+            require(ident, TCX_ERR_ARG,
+                    "set_kpoint_symmetry: an irreducible k needs the identity (no translation, no time reversal)");
+        }
+        for (int d = 0; d < 3; ++d) {
+            const double tn = t[d] * n[d];
+            require(std::isfinite(tn) && std::fabs(tn - std::round(tn)) < 1e-6, TCX_ERR_ARG,
+                    "set_kpoint_symmetry: fractional translation not commensurate with the FFT grid");
+        }
+        SymRotJob& J = ops[k];
+        J.dst = J.src = 0;
+        J.trev = trev[k];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) J.ud[3 * i + j] = U[3 * j + i];  // U^+ = U^T
+        for (int d = 0; d < 3; ++d) J.tau[d] = t[d];
+        // k = s U^+ k_src modulo reciprocal-lattice vectors (kpoints.cpp:10-26)
+        const int sg = J.trev ? -1 : 1;
+        double d2 = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            double v = 0.0;
+            for (int j = 0; j < 3; ++j) v += J.ud[3 * i + j] * kfrac[3 * s + j];
+            double dd = kfrac[3 * k + i] - sg * v;
+            dd -= std::round(dd);
+            d2 += dd * dd;
+        }
+        require(d2 < 1e-8, TCX_ERR_ARG, "set_kpoint_symmetry: k is not the rotated k_src");
+        // U^+ in Cartesian coordinates: B^T U^+ A / (2 pi) (tc3a2a4b2b5.cpp:677-679)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double a = 0.0;
+                for (int p = 0; p < 3; ++p)
+                    for (int q = 0; q < 3; ++q) a += B[3 * p + i] * J.ud[3 * p + q] * Alat[3 * q + j];
+                J.udc[3 * i + j] = a / (2.0 * kPiHost);
+            }
+        if (s == k) continue;
+        // the wrapped map G -> s U^+ G must be a bijection of the FFT grid
+        std::fill(seen.begin(), seen.end(), 0);
+        for (long long r = 0; r < N; ++r) {
+            const int i1 = (int)(r % n[0]), i2 = (int)((r / n[0]) % n[1]), i3 = (int)(r / ((long long)n[0] * n[1]));
+            const int G[3] = {i1 <= n[0] / 2 ? i1 : i1 - n[0], i2 <= n[1] / 2 ? i2 : i2 - n[1],
+                              i3 <= n[2] / 2 ? i3 : i3 - n[2]};
+            long long rd = 0, mul = 1;
+            for (int i = 0; i < 3; ++i) {
+                int v = sg * (J.ud[3 * i] * G[0] + J.ud[3 * i + 1] * G[1] + J.ud[3 * i + 2] * G[2]);
+                v %= n[i];
+                if (v < 0) v += n[i];
+                rd += mul * v;
+                mul *= n[i];
+            }
+            require(!seen[rd], TCX_ERR_ARG, "set_kpoint_symmetry: rotation does not map the FFT grid onto itself");
+            seen[rd] = 1;
+        }
+    }
+    sym_src.assign(k_src, k_src + nk);
+    sym_ops = std::move(ops);
+    sym_set = true;
 }
 
 // ------------------------------------------------------------------ real-space grids
@@ -416,11 +521,13 @@ std::vector<PairItem> Context::applied_items(const std::vector<int>& nb) const
     return v;
 }
 
-std::vector<PairItem> Context::occupied_items() const
+std::vector<PairItem> Context::occupied_items(bool irreducible_only) const
 {
     std::vector<PairItem> v;
-    for (int q = 0; q < nk; ++q)
+    for (int q = 0; q < nk; ++q) {
+        if (irreducible_only && sym_set && sym_src[q] != q) continue;
         for (int i = 0; i < nocc[q]; ++i) v.push_back({q, (int)occ_first[q] + i});
+    }
     return v;
 }
 
@@ -683,6 +790,11 @@ int tcx_set_jastrow_rpa(tcx_context* ctx, const double A[4], const double F[4])
 int tcx_set_div_correction(tcx_context* ctx, int enabled, double alpha, const double* sum_of_vaux)
 {
     return guarded(ctx, [&] { ctx->impl.set_div(enabled, alpha, sum_of_vaux); });
+}
+
+int tcx_set_kpoint_symmetry(tcx_context* ctx, const int* k_src, const int* rot, const double* trans, const int* trev)
+{
+    return guarded(ctx, [&] { ctx->impl.set_symmetry(k_src, rot, trans, trev); });
 }
 
 int tcx_hf2x(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision, unsigned flags)

@@ -52,6 +52,16 @@ struct TargetEntry {
     int pad;
 };
 
+// One symmetry rotation of a q2-resolved sum: the grids of occupied orbital
+// `src` (an irreducible q2) mapped to those of `dst` (a replica of it).
+struct SymRotJob {
+    int dst, src;
+    int trev;       // time reversal used at the replica
+    int ud[9];      // U^+ = U^T (integer, crystal), row-major
+    double udc[9];  // U^+ in Cartesian coordinates
+    double tau[3];  // fractional translation (crystal)
+};
+
 // One orbital scatter/gather job: coefficients <-> grid.  The grid slot of
 // job i of a launch is i (grids are consecutive from the base pointer).
 struct OrbEntry {
@@ -187,6 +197,9 @@ void launch_a8_reduce(const double* Tg, const double* Q4, const double* Q5, int 
                       const double* fill, int nocc, double* V3a2, double* B2, long long N, hipStream_t s);
 void launch_a8_q45(int prec, const double* Q4, const double* Q5, int bitc, void* Q4T, void* Q5T, int ngrids,
                    long long N, hipStream_t s);
+// G-space rotation of ncomp-component sums (ncomp 1: scalar, 3: Cartesian vector) from src to dst grids
+void launch_sym_rotate(double* Q, int ncomp, const SymRotJob* jobs, int njobs, const Geom& g, long long N,
+                       hipStream_t s);
 
 // kernel-gradient reduction of the occupied-set energies (rec: 1 channel E_x, 8 channels E_2x)
 void launch_kgrad(int prec, const void* rec, int tc2, const PairEntry* pairs, int npairs, const double* kqcart,

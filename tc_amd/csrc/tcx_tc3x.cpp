@@ -4,10 +4,11 @@
 // The pair work of the three-body terms runs in the working precision
 // (FP32-split or FP64) through batched record pipelines
 // (kernels_tc3x.hip); all accumulators, the divergence corrections and the
-// local potentials are FP64.  Identity symmetry: the reference's U^+ transforms
-// of the q2-resolved sums (tc3a2a4b2b5.cpp:576-765) are the identity and their
-// FFT_forward / FFT_backward round trip is omitted.  The kVaux branches vanish
-// for k-points on the SCF mesh and are not implemented.
+// local potentials are FP64.  Without a k-mesh symmetry the q2-resolved sums of
+// tc3a2a4b2b5 are formed at every q2 directly; with one (tcx_set_kpoint_symmetry)
+// they are formed at the irreducible q2 only and rotated to the other q2 by the
+// U^+ transforms of tc3a2a4b2b5.cpp:576-765, as the reference does.  The kVaux
+// branches vanish for k-points on the SCF mesh and are not implemented.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,7 +23,8 @@ namespace {
 constexpr size_t kRecordBudget = (size_t)3 << 30;
 constexpr long kTc3Targets = 32;
 
-enum PairKind { kPairsAppOcc = 0, kPairsOccOcc = 1, kPairsOccSameK = 2 };
+// kPairsIrr*: as kPairsOcc*, with targets at the irreducible k of the k-mesh symmetry only
+enum PairKind { kPairsAppOcc = 0, kPairsOccOcc = 1, kPairsOccSameK = 2, kPairsIrrOcc = 3, kPairsIrrSameK = 4 };
 
 ProdChan chan(const void* a, int a_sel, int a_mul, int a_off, int a_conj, const void* b, int b_sel, int b_mul,
               int b_off, int b_conj)
@@ -40,14 +42,15 @@ const PairSet& Context::pair_set(const std::vector<int>& nb, int kind, int chann
     auto key = std::make_tuple(nb, kind * 1000 + channels * 2 + prec);
     auto it = pairsets.find(key);
     if (it != pairsets.end()) return it->second;
-    const std::vector<PairItem> tg = kind == kPairsAppOcc ? applied_items(nb) : occupied_items();
+    const bool irr = kind == kPairsIrrOcc || kind == kPairsIrrSameK;
+    const std::vector<PairItem> tg = kind == kPairsAppOcc ? applied_items(nb) : occupied_items(irr);
     const std::vector<PairItem> src = occupied_items();
     const long maxpairs = std::max<long>(1, (long)(kRecordBudget / ((size_t)channels * N * es)));
     const long tc = std::max<long>(1, std::min<long>({kTc3Targets, maxpairs, (long)std::max<size_t>(1, tg.size())}));
     long sc = std::max<long>(1, maxpairs / tc);
     sc = std::min<long>(sc, (long)std::max<size_t>(1, src.size()));
     PairSet S;
-    build_pairs(S, tg, src, tc, sc, kcart, kind == kPairsOccSameK);
+    build_pairs(S, tg, src, tc, sc, kcart, kind == kPairsOccSameK || kind == kPairsIrrSameK);
     auto res = pairsets.emplace(key, std::move(S));
     return res.first->second;
 }
@@ -287,7 +290,7 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
             Q5 = (double*)d_Q5.p;
         }
         {
-            const PairSet& S = pair_set(nb, kPairsOccOcc, 7 * nin, prec);
+            const PairSet& S = pair_set(nb, sym_set ? kPairsIrrOcc : kPairsOccOcc, 7 * nin, prec);
             d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 7 * nin * gT);
             int max_rows = 1;
             for (const Launch& L : S.launches) max_rows = std::max(max_rows, L.nkq);
@@ -327,7 +330,7 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
             for (int k = 0; k < nk; ++k) dc2[k] = 1.0 / std::sqrt(kPiHost * alpha) * NkV - sum_vaux[k];
             d_aux.ensure((size_t)nk * 8);
             TCX_HIP_CHECK(hipMemcpy(d_aux.p, dc2.data(), nk * sizeof(double), hipMemcpyHostToDevice));
-            const PairSet& S = pair_set(nb, kPairsOccSameK, 2, TCX_PREC_FP64);
+            const PairSet& S = pair_set(nb, sym_set ? kPairsIrrSameK : kPairsOccSameK, 2, TCX_PREC_FP64);
             d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 2 * g64);
             ProdChan ch[2];
             ch[0] = chan(chiR64, 1, 1, 0, 1, occR64, 0, 1, 0, 0);  // conj(chi_q2,i1) phi_q2,i2 (:545-547)
@@ -342,6 +345,8 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
                               geo, N, j00, j01, stream);
             }
         }
+        // replicas of the irreducible q2: rotated sums (:576-765)
+        if (sym_set) rotate_qsums(Q2, Q4, bitc ? Q5 : nullptr);
         // V_3a2 (G) and qsum_3b2 (r) (:637-641, :790-800)
         d_local.ensure(5 * g64);
         double* V3a2 = (double*)d_local.p;
@@ -398,6 +403,46 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
         apply_local(P, io, X, 1.0);
     }
     end_apply(io);
+}
+
+// Q2 (G space, one grid per occupied orbital) and Q4, Q5 (real space, three
+// Cartesian grids each) at every replica q2 from those of its irreducible q2:
+// FFT_forward, the U^+ map with the translation phase, FFT_backward for the
+// vector sums (tc3a2a4b2b5.cpp:576-640, :644-745).  For TC, Q5 is not stored:
+// the consumers form -conj(Q4) of the rotated Q4, which equals the rotated
+// -conj(Q4) because the translations are commensurate with the grid.
+void Context::rotate_qsums(double* Q2, double* Q4, double* Q5)
+{
+    if (!sym_jobs_valid) {
+        std::vector<SymRotJob> jobs;
+        for (int k = 0; k < nk; ++k) {
+            const int s = sym_src[k];
+            if (s == k) continue;
+            require(nocc[k] == nocc[s], TCX_ERR_STATE,
+                    "tc3a2a4b2b5: a symmetry replica holds a different number of occupied orbitals than its "
+                    "irreducible k");
+            for (int i = 0; i < nocc[k]; ++i) {
+                SymRotJob J = sym_ops[k];
+                J.dst = (int)occ_first[k] + i;
+                J.src = (int)occ_first[s] + i;
+                require(std::fabs(fill[J.dst] - fill[J.src]) <= 1e-12 * std::max(1.0, std::fabs(fill[J.src])),
+                        TCX_ERR_STATE, "tc3a2a4b2b5: a symmetry replica has other fillings than its irreducible k");
+                jobs.push_back(J);
+            }
+        }
+        upload(d_symjobs, jobs.data(), jobs.size() * sizeof(SymRotJob));
+        n_sym_jobs = (int)jobs.size();
+        sym_jobs_valid = true;
+    }
+    if (n_sym_jobs == 0) return;
+    const SymRotJob* jobs = (const SymRotJob*)d_symjobs.p;
+    launch_sym_rotate(Q2, 1, jobs, n_sym_jobs, geo, N, stream);
+    for (double* Q : {Q4, Q5}) {
+        if (!Q) continue;
+        launch_fft3d(TCX_PREC_FP64, -1, Q, 3 * n_occ, geo, 1.0 / (double)N, stream);
+        launch_sym_rotate(Q, 3, jobs, n_sym_jobs, geo, N, stream);
+        launch_fft3d(TCX_PREC_FP64, +1, Q, 3 * n_occ, geo, 1.0, stream);
+    }
 }
 
 // ------------------------------------------------------------------ assembly terms

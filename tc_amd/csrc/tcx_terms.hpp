@@ -129,6 +129,13 @@ struct Context {
     bool div_enabled = false;
     double alpha = 0.0;
     std::vector<double> sum_vaux;
+    // k-mesh symmetry (tcx_set_kpoint_symmetry): sym_src[k] = irreducible k of k,
+    // sym_ops[k] = its rotation (dst/src unused); off = identity symmetry
+    bool sym_set = false;
+    std::vector<int> sym_src;
+    std::vector<SymRotJob> sym_ops;
+    bool sym_jobs_valid = false;
+    int n_sym_jobs = 0;
 
     // device data
     DBuf d_gindex, d_kcart, d_phi_occ, d_chi_occ, d_fill;
@@ -139,6 +146,7 @@ struct Context {
     DBuf d_occjobs;      // OrbEntry per occupied orbital
     DBuf d_occgradjobs;  // three OrbEntry (dir 0..2) per occupied orbital
     DBuf d_dc;       // divergence factor per k
+    DBuf d_symjobs;  // SymRotJob per occupied orbital at a replica k
     DBuf d_phi_app, d_out, d_appR, d_appGrad, d_tmp64, d_W, d_pairbuf, d_ktab, d_work;
 
     // launch shape per term (0 hf2x, 1 tc2x): targets x sources per launch, 0 = auto
@@ -183,6 +191,8 @@ struct Context {
     void set_states(const int* nocc, const double* filling, const double* phi_occ, const double* chi_occ);
     void set_jastrow(const double A[4], const double F[4]);
     void set_div(int enabled, double alpha, const double* svaux);
+    void set_symmetry(const int* k_src, const int* rot, const double* trans, const int* trev);
+    void rotate_qsums(double* Q2, double* Q4, double* Q5);
     void invalidate_states();
 
     void orbitals_to_real(const double* coef, const OrbEntry* d_jobs, int n, int prec, void* out);
@@ -212,7 +222,7 @@ struct Context {
     // three-body exchange terms and the assembly terms
     const PairSet& pair_set(const std::vector<int>& nb, int kind, int channels, int prec);
     std::vector<PairItem> applied_items(const std::vector<int>& nb) const;
-    std::vector<PairItem> occupied_items() const;
+    std::vector<PairItem> occupied_items(bool irreducible_only = false) const;
     void apply_local(const Plan& P, const ApplyIO& io, const double* V, double factor);
     void density_G(double* nG);
     void tc3a3b3b4b6(const int* nb, const double* phi, double* out, int prec, unsigned flags);

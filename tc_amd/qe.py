@@ -1,0 +1,196 @@
+"""Quantum ESPRESSO inputs on the reference's symmetry-replicated k-mesh.
+
+TC++ starts an SCF run from a QE calculation: the irreducible k-points with their
+weights and eigenvalues, the symmetry operations, and the plane-wave coefficients of
+each irreducible k (io_qe_files_read_xml.cpp:150-241, io_qe_files_read_wfc.f90:69-140).
+It then works on the full k-mesh without storing it: every k of the mesh is an
+irreducible k with a rotation (and possibly time reversal), and each orbital at that k
+is produced on the fly by re-indexing the irreducible coefficients
+(plane_wave_basis.cpp:81-164, 250-270).  This module builds that full mesh once,
+explicitly, as a `tc_amd.system.System` with identity symmetry, so that the engine's
+operators see exactly the orbitals the reference's loops see:
+
+  * TC++ rotation U = (QE rotation)^-1, integer, crystal coordinates (symmetry.cpp:17-33);
+  * k-replicas k' = U^T k (crystal), kept when not equal to an earlier one modulo the
+    reciprocal lattice, and -k' by time reversal unless QE's noinv is true
+    (kpoints.cpp:34-110);
+  * orbital at k': coefficient c(G) placed at G' = U^T G (G' -> -G' and c -> conj(c)
+    under time reversal), times exp(2 pi i G'.t) with t the fractional translation
+    (plane_wave_basis.cpp:81-164, 250-270).
+
+The fixtures (`tests/golden/qe_example*.npz`) hold the same data as the QE files of the
+reference's examples.
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import math
+
+import numpy as np
+
+from . import system as S
+
+HT_IN_EV = 27.21138505  # src/include/header.hpp:47
+
+
+@dataclasses.dataclass
+class QEInput:
+    alat: float
+    cell: np.ndarray          # rows a_i (bohr)
+    grid: tuple
+    b_norm: np.ndarray        # rows b_i in 2 pi / alat
+    rot_qe: np.ndarray        # (nsym, 3, 3) integer, crystal coordinates
+    trans: np.ndarray         # (nsym, 3) fractional translations (crystal)
+    time_reversal: bool       # noinv != true (symmetry.cpp:124)
+    nelec: float
+    kcart: np.ndarray         # (nirr, 3) in 2 pi / alat
+    kweight: np.ndarray       # (nirr,) QE weights (sum 2 for nspin = 1)
+    eig_qe: np.ndarray        # (nirr, nb) Hartree
+    npw: np.ndarray           # (nirr,)
+    mill: list                # per irreducible k: (npw, 3) Miller indices
+    coef: list                # per irreducible k: (nb, npw) complex128
+    setup: dict
+    expected: dict
+
+    @property
+    def nirr(self) -> int:
+        return len(self.kcart)
+
+    @property
+    def nbands(self) -> int:
+        return self.eig_qe.shape[1]
+
+
+def load_fixture(path: str) -> QEInput:
+    z = np.load(path, allow_pickle=False)
+    npw = z["npw"]
+    nb = z["eig_qe"].shape[1]
+    mill, coef = [], []
+    m0 = c0 = 0
+    for n in npw:
+        mill.append(z["mill"][m0:m0 + n])
+        coef.append(z["coef"][c0:c0 + nb * n].reshape(nb, n))
+        m0 += n
+        c0 += nb * n
+    return QEInput(alat=float(z["alat"]), cell=z["cell"], grid=tuple(int(v) for v in z["grid"]),
+                   b_norm=z["b_norm"], rot_qe=z["rot_qe"], trans=z["trans"],
+                   time_reversal=str(z["noinv"]) != "true", nelec=float(z["nelec"]), kcart=z["kcart"],
+                   kweight=z["kweight"], eig_qe=z["eig_qe"], npw=npw, mill=mill, coef=coef,
+                   setup=json.loads(str(z["setup"])), expected=json.loads(str(z["expected"])))
+
+
+def tc_rotations(qe: QEInput) -> np.ndarray:
+    """U = inverse of the QE rotation, rounded to integers (symmetry.cpp:17-33)."""
+    return np.array([np.rint(np.linalg.inv(r.astype(np.float64))).astype(np.int64) for r in qe.rot_qe])
+
+
+def _matched(v, lst):
+    """kpoints.cpp:10-26: equal modulo reciprocal-lattice vectors."""
+    for w in lst:
+        d = v - w
+        d = d - np.round(d)  # components folded to [-0.5, 0.5]
+        if float(np.sum(d * d)) < 1e-8:
+            return True
+    return False
+
+
+@dataclasses.dataclass
+class KReplica:
+    ik: int        # irreducible k
+    isym: int      # symmetry operation (TC++ index)
+    trev: bool     # time reversal used
+    kcrys: np.ndarray
+
+
+def replicate_kpoints(qe: QEInput) -> list:
+    """Kpoints::set_kvectors_by_symmetry (kpoints.cpp:34-110), in index_all_kscf order."""
+    U = tc_rotations(qe)
+    kcrys = [np.linalg.solve(qe.b_norm.T, k) for k in qe.kcart]  # kcart = (b1 b2 b3) kcrys
+    per_k = [[KReplica(ik, 0, False, kcrys[ik])] for ik in range(qe.nirr)]
+    vec_list = [kcrys[ik] for ik in range(qe.nirr) if qe.kweight[ik] >= 1e-8]
+    for ik in range(qe.nirr):
+        if qe.kweight[ik] < 1e-8:
+            continue
+        for isym in range(len(U)):
+            v = U[isym].T.astype(np.float64) @ kcrys[ik]
+            v[np.abs(v) < 1e-4] = 0.0
+            if not _matched(v, vec_list):
+                vec_list.append(v)
+                per_k[ik].append(KReplica(ik, isym, False, v))
+            if qe.time_reversal and not _matched(-v, vec_list):
+                vec_list.append(-v)
+                per_k[ik].append(KReplica(ik, isym, True, -v))
+    return [r for lst in per_k for r in lst]
+
+
+def replica_basis(qe: QEInput, rep: KReplica):
+    """(FFT-grid linear indices, phase factors) of the irreducible plane waves at a replica
+    (plane_wave_basis.cpp:81-164)."""
+    U = tc_rotations(qe)[rep.isym]
+    G = qe.mill[rep.ik].astype(np.int64)
+    Gs = G @ U  # rows: (U^T G)^T = G^T U
+    if rep.trev:
+        Gs = -Gs
+    phase = np.exp(2j * math.pi * (Gs.astype(np.float64) @ qe.trans[rep.isym]))
+    n1, n2, n3 = qe.grid
+    w = Gs % np.array([n1, n2, n3])
+    gidx = (w[:, 0] + n1 * (w[:, 1] + n2 * w[:, 2])).astype(np.int32)
+    return gidx, phase
+
+
+def jastrow_rpa(qe: QEInput, volume: float):
+    """Default RPA Jastrow: A_long = A sqrt(V / (4 pi N_e)) (jastrow_initialize.cpp:28-50),
+    F from the cusp condition (:62-75): F_parallel = sqrt(2A), F_antiparallel = sqrt(A)."""
+    a = float(qe.setup.get("A_input", 1.0)) * math.sqrt(volume / (S.FOURPI * qe.nelec))
+    A = np.full((2, 2), a)
+    F = np.array([[math.sqrt(2 * a), math.sqrt(a)], [math.sqrt(a), math.sqrt(2 * a)]])
+    return A, F
+
+
+def build_system(qe: QEInput, phi_irr: list, filling_irr: np.ndarray, nocc_irr: np.ndarray,
+                 chi_irr: list | None = None, method: str | None = None, symmetric: bool = True) -> tuple:
+    """Full-mesh System whose occupied set at every replica is the irreducible set
+    re-indexed; returns (System, replicas).  phi_irr[ik]: (nb, npw) coefficients,
+    filling_irr (nirr, nb), nocc_irr (nirr,) occupied bands (f > 1e-8) per irreducible k.
+    symmetric: carry the mesh symmetry, so that tcx_tc3a2a4b2b5 rotates its q2 sums from
+    the irreducible q2 as the reference does; False treats every replica as independent."""
+    reps = replicate_kpoints(qe)
+    recip = S.reciprocal_rows(qe.cell)
+    volume = abs(float(np.linalg.det(qe.cell)))
+    method = method or qe.setup["method"]
+    npw, gidx_all, kfrac, nocc, fill, occ, left = [], [], [], [], [], [], []
+    for r in reps:
+        gidx, ph = replica_basis(qe, r)
+        npw.append(len(gidx))
+        gidx_all.append(gidx)
+        kfrac.append(r.kcrys)
+        n = int(nocc_irr[r.ik])
+        nocc.append(n)
+        fill.append(filling_irr[r.ik][:n])
+        for src, dst in ((phi_irr, occ), (chi_irr, left)):
+            if src is None:
+                continue
+            c = np.asarray(src[r.ik])[:n]
+            dst.append(((np.conj(c) if r.trev else c) * ph[None, :]).reshape(-1))
+    A, F = jastrow_rpa(qe, volume) if method != "HF" else (np.zeros((2, 2)), np.ones((2, 2)))
+    sys_ = S.System(lattice=np.asarray(qe.cell, dtype=np.float64), recip=recip, volume=volume, n=qe.grid,
+                    kfrac=np.array(kfrac), npw=np.array(npw, np.int32), gindex=np.concatenate(gidx_all),
+                    nocc=np.array(nocc, np.int32), filling=np.concatenate(fill), phi_occ=np.concatenate(occ),
+                    chi_occ=np.concatenate(left) if chi_irr is not None else None, method=method, A=A, F=F,
+                    div_corr=True, name=qe.setup.get("source", "qe"),
+                    symmetry=mesh_symmetry(qe, reps) if symmetric else None)
+    return sys_, reps
+
+
+def mesh_symmetry(qe: QEInput, reps: list) -> dict:
+    """The symmetry relation of the replicated mesh for tcx_set_kpoint_symmetry: each
+    replica built from the first replica (isym 0, the irreducible k itself) of its ik."""
+    U = tc_rotations(qe)
+    first = {}
+    for i, r in enumerate(reps):
+        first.setdefault(r.ik, i)
+    return {"k_src": np.array([first[r.ik] for r in reps], np.int32),
+            "rot": np.array([U[r.isym] for r in reps], np.int32),
+            "trans": np.array([qe.trans[r.isym] for r in reps], np.float64),
+            "trev": np.array([int(r.trev) for r in reps], np.int32)}

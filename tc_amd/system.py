@@ -107,7 +107,8 @@ def gram_schmidt_biortho(right: np.ndarray, left: np.ndarray):
 
 @dataclasses.dataclass
 class System:
-    """One spin-unpolarised plane-wave system with identity symmetry.
+    """One spin-unpolarised plane-wave system on an explicit k-mesh (identity
+    symmetry unless `symmetry` relates its k-points).
 
     Ragged per-k arrays are stored concatenated over k (k-major, band-major,
     then plane wave), complex128.
@@ -129,6 +130,9 @@ class System:
     F: np.ndarray = dataclasses.field(default_factory=lambda: np.ones((2, 2)))
     div_corr: bool = True
     name: str = ""
+    # k-mesh symmetry (tcx_set_kpoint_symmetry) or None for identity symmetry:
+    # {"k_src": (nk,), "rot": (nk, 3, 3), "trans": (nk, 3), "trev": (nk,)}
+    symmetry: dict | None = None
 
     @property
     def nk(self) -> int:

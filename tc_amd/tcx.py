@@ -36,7 +36,7 @@ METHODS = {"HF": 0, "TC": 1, "BITC": 2}
 EXPORTS = [
     "tcx_create", "tcx_destroy", "tcx_last_error", "tcx_synchronize",
     "tcx_set_cell", "tcx_set_kpoints", "tcx_set_basis", "tcx_set_states",
-    "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux",
+    "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux", "tcx_set_kpoint_symmetry",
     "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
     "tcx_davidson", "tcx_total_energy", "tcx_eig_dense", "tcx_set_filling",
@@ -80,6 +80,7 @@ def load_library():
     L.tcx_set_jastrow_rpa.argtypes = [vp, _dp, _dp]
     L.tcx_set_div_correction.argtypes = [vp, C.c_int, C.c_double, _dp]
     L.tcx_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
+    L.tcx_set_kpoint_symmetry.argtypes = [vp, _ip, _ip, _dp, _ip]
     for name in ("tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5"):
         getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_int, C.c_uint]
     for name in ("tcx_kinetic", "tcx_hf2h", "tcx_tc2h"):
@@ -233,6 +234,19 @@ class Engine:
         self._check(self.L.tcx_set_div_correction(self.h, int(bool(enabled)), float(alpha),
                                                   None if s is None else _ptr(s)), "set_div_correction")
 
+    def set_kpoint_symmetry(self, k_src=None, rot=None, trans=None, trev=None):
+        """k-mesh symmetry (None: identity).  k_src (nk,), rot (nk, 3, 3) TC++ rotations,
+        trans (nk, 3) fractional translations, trev (nk,) time-reversal flags."""
+        if k_src is None:
+            self._check(self.L.tcx_set_kpoint_symmetry(self.h, None, None, None, None), "set_kpoint_symmetry")
+            return
+        ks = np.ascontiguousarray(k_src, dtype=np.int32)
+        r = np.ascontiguousarray(rot, dtype=np.int32).reshape(-1)
+        t = np.ascontiguousarray(trans, dtype=np.float64).reshape(-1)
+        tr = np.ascontiguousarray(trev, dtype=np.int32)
+        self._check(self.L.tcx_set_kpoint_symmetry(self.h, _ptr(ks, _ip), _ptr(r, _ip), _ptr(t), _ptr(tr, _ip)),
+                    "set_kpoint_symmetry")
+
     def load_system(self, sys, div_corr=None):
         """Push a tc_amd.system.System into the context."""
         self.sys = sys
@@ -240,6 +254,9 @@ class Engine:
         self.set_kpoints(sys.kfrac, sys.num_kpoints)
         self.set_basis(sys.npw, sys.gindex)
         self.set_states(sys.nocc, sys.filling, sys.phi_occ, sys.chi_occ)
+        sym = getattr(sys, "symmetry", None)
+        if sym is not None:
+            self.set_kpoint_symmetry(sym["k_src"], sym["rot"], sym["trans"], sym["trev"])
         self.set_jastrow_rpa(sys.A, sys.F)
         on = sys.div_corr if div_corr is None else div_corr
         if on:

@@ -1,0 +1,152 @@
+"""The first SCF iteration of the reference's end-to-end examples on the GPU engine
+(test/example14 HF, example15 TC, example16 BITC; fixtures tests/golden/qe_example*.npz),
+against the numbers the reference printed after "Iteration 1":
+  * the eigenvalues at every irreducible k;
+  * the Fermi energy from those eigenvalues;
+  * the 1-, 2- and 3-body energies and the total energy;
+to 1e-8 Ha in FP64 (the printed digits agree to a few 1e-10 Ha) and 1e-6 Ha in the
+FP32-split precision.  This runs the whole path: QE orbitals replicated over the
+symmetry-expanded k-mesh, density, kinetic + Hartree + exchange (HF) or the TC/BITC two-
+and three-body terms with tc3a2a4b2b5's q2 sums rotated from the irreducible q2, block
+Davidson on the device, fillings and total energy (tc_amd.scf.first_iteration).
+
+The occupied sets of example15/16 are not symmetric (six bands with Gaussian fillings cut
+degenerate multiplets), so the rotated q2 sums differ from direct full-mesh sums; the
+reference's numbers are only met with the rotation, which these tests therefore pin."""
+import os
+
+import numpy as np
+import pytest
+
+from tc_amd import qe as Q
+from tc_amd import scf
+from tc_amd import tcx
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL_HA = {tcx.PREC_FP64: 1e-8, tcx.PREC_FP32: 1e-6}
+
+
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
+@pytest.mark.parametrize("ex", [14, 15, 16])
+def test_first_scf_iteration_matches_reference(engine, ex, prec):
+    qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
+    exp = qe.expected
+    r = scf.first_iteration(engine, qe, precision=prec)
+    eig_ev = r.eigenvalues.real * Q.HT_IN_EV
+    ref_ev = np.array(exp["eigenvalues_iter1_eV"])
+    d_eig = np.abs(eig_ev - ref_ev).max() / Q.HT_IN_EV
+    e = exp["energies_iter1_Ha"]
+    ref = np.array([e["e1"][0], e["e2"][0], e.get("e3", [0.0])[0], e["etot"][0]])
+    d_e = np.abs(r.energies.real - ref)
+    d_ef = abs(r.fermi * Q.HT_IN_EV - exp["fermi_energy_iter1_eV"]) / Q.HT_IN_EV
+    print(f"example{ex} {qe.setup['method']} prec={prec}: max|d eig| {d_eig:.2e} Ha, d E_1,2,3,tot "
+          f"{d_e[0]:.2e} {d_e[1]:.2e} {d_e[2]:.2e} {d_e[3]:.2e} Ha, d E_F {d_ef:.2e} Ha; "
+          f"E_tot {r.energies[3].real:.10f} (reference {ref[3]:.10f})", flush=True)
+    tol = TOL_HA[prec]
+    assert d_eig < tol
+    assert d_e.max() < tol
+    assert d_ef < tol
+
+
+def _tc_system_from(ex, symmetric):
+    """The QE orbitals of a reference example with the QE fillings, as a TC system."""
+    qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
+    f0, _, nocc0 = scf._fillings(qe, qe.eig_qe, qe.setup["smearing_mode"], float(qe.setup["smearing_width"]))
+    sys_, reps = Q.build_system(qe, [np.asarray(c) for c in qe.coef], f0, nocc0, None, "TC", symmetric=symmetric)
+    return qe, sys_, reps
+
+
+def _generic_k_system(method, symmetric, seed=5):
+    """Random orbitals at two generic k-points of example14's fcc cell (20^3 grid),
+    replicated by the group {E, (S|1/4 1/4 1/4)} (S a two-fold rotation of the QE set)
+    and by time reversal.  Every k has a trivial little group and all four images of
+    each k are present, so the occupied set is exactly symmetric: the rotated q2 sums
+    must equal the direct ones to rounding."""
+    import dataclasses
+    qe = Q.load_fixture(os.path.join(HERE, "golden", "qe_example14.npz"))
+    pick = [0, 4]
+    assert np.array_equal(qe.rot_qe[0], np.eye(3, dtype=qe.rot_qe.dtype))
+    assert np.array_equal(qe.rot_qe[4] @ qe.rot_qe[4], np.eye(3, dtype=qe.rot_qe.dtype))
+    assert np.abs(qe.trans[4]).max() > 0
+    kcrys = np.array([[0.1, 0.23, 0.37], [0.31, -0.17, 0.05]])
+    kcart = kcrys @ qe.b_norm  # (b1 b2 b3) k
+    rng = np.random.default_rng(seed)
+    m = np.array(np.meshgrid(*[np.arange(-6, 7)] * 3, indexing="ij")).reshape(3, -1).T
+    mill, coef, left = [], [], []
+    for k in kcart:
+        keep = np.sum((k + m @ qe.b_norm) ** 2, axis=1) < 25.0
+        mk = m[keep].astype(np.int32)
+        mill.append(mk)
+        for dst in (coef, left):
+            c = rng.standard_normal((2, len(mk))) + 1j * rng.standard_normal((2, len(mk)))
+            dst.append(c / np.linalg.norm(c, axis=1, keepdims=True))
+    q = dataclasses.replace(qe, rot_qe=qe.rot_qe[pick], trans=qe.trans[pick], time_reversal=True, kcart=kcart,
+                            kweight=np.array([1.0, 1.0]), eig_qe=np.zeros((2, 2)),
+                            npw=np.array([len(x) for x in mill]), mill=mill, coef=coef)
+    fill = np.array([[2.0, 1.3], [2.0, 0.7]])
+    sys_, reps = Q.build_system(q, coef, fill, np.array([2, 2]), left if method == "BITC" else None, method,
+                                symmetric=symmetric)
+    assert len(reps) == 8 and sum(r.trev for r in reps) == 4
+    return sys_
+
+
+@pytest.mark.parametrize("method", ["TC", "BITC"])
+def test_rotated_q2_sums_exact_on_a_symmetric_set(engine, method):
+    """Scalar [3a2] and Cartesian-vector [3a4]/[3b5] maps, with the fractional-translation
+    phase, with and without time reversal, for TC (Q5 = -conj(Q4)) and BITC (Q5 rotated
+    separately): tc3a2a4b2b5 with rotated q2 sums equals the direct full-mesh sums."""
+    out = []
+    for symmetric in (False, True):
+        sys_ = _generic_k_system(method, symmetric)
+        engine.load_system(sys_)
+        out.append(engine.tc3a2a4b2b5(sys_.nocc, sys_.phi_occ, precision=tcx.PREC_FP64))
+    rel = np.abs(out[1] - out[0]).max() / np.abs(out[0]).max()
+    print(f"{method} tc3a2a4b2b5 rotated vs direct q2 sums, generic k + time reversal: rel {rel:.2e}", flush=True)
+    assert rel < 1e-12
+
+
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
+def test_rotated_q2_sums_equal_direct_sums_on_example14(engine, prec):
+    """Example14's occupied set (QE orbitals, eight bands) is symmetric to QE's precision
+    (its HF operators are covariant to ~1e-7), so tc3a2a4b2b5 with the q2 sums rotated
+    from the irreducible q2 must equal the direct full-mesh sums to that level: a wrong
+    sign, transpose or phase in the rotation moves the result at O(1)."""
+    _, sys_d, reps = _tc_system_from(14, symmetric=False)
+    assert any(not r.trev and r.isym > 0 for r in reps)
+    engine.load_system(sys_d)
+    direct = engine.tc3a2a4b2b5(sys_d.nocc, sys_d.phi_occ, precision=prec)
+    _, sys_s, _ = _tc_system_from(14, symmetric=True)
+    engine.load_system(sys_s)
+    rotated = engine.tc3a2a4b2b5(sys_s.nocc, sys_s.phi_occ, precision=prec)
+    rel = np.abs(rotated - direct).max() / np.abs(direct).max()
+    print(f"tc3a2a4b2b5 rotated vs direct q2 sums (prec {prec}): rel {rel:.2e}", flush=True)
+    assert rel < 2e-6
+
+
+def test_kpoint_symmetry_rejects_inconsistent_relations(engine):
+    sys_ = _generic_k_system("TC", symmetric=True)
+    engine.load_system(sys_)
+    sym = sys_.symmetry
+    i = 2  # replica of k 0 by the two-fold rotation, no time reversal
+    assert sym["k_src"][i] == 0 and sym["trev"][i] == 0
+
+    def bad(**change):
+        b = {k: v.copy() for k, v in sym.items()}
+        for key, (idx, val) in change.items():
+            b[key][idx] = val
+        return b
+
+    with pytest.raises(tcx.TcxError, match="unimodular"):
+        engine.set_kpoint_symmetry(**bad(rot=(i, np.eye(3, dtype=np.int32) * 2)))
+    with pytest.raises(tcx.TcxError, match="irreducible"):
+        engine.set_kpoint_symmetry(**bad(k_src=(0, i)))  # an irreducible k must name itself
+    with pytest.raises(tcx.TcxError, match="rotated k_src"):
+        engine.set_kpoint_symmetry(**bad(trev=(i, 1)))  # -S k is another k-point
+    with pytest.raises(tcx.TcxError, match="commensurate"):
+        engine.set_kpoint_symmetry(**bad(trans=(i, [0.03, 0.0, 0.0])))  # 0.03 * 20 is not an integer
+    with pytest.raises(tcx.TcxError, match="identity"):
+        engine.set_kpoint_symmetry(**bad(trev=(0, 1)))
+    engine.set_kpoint_symmetry(**sym)
+    engine.set_kpoint_symmetry(None)  # identity symmetry again
