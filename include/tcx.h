@@ -121,6 +121,18 @@ int tcx_set_div_correction(tcx_context *ctx, int enabled, double alpha, const do
 int tcx_set_kpoint_symmetry(tcx_context *ctx, const int *k_src, const int *rot, const double *trans,
                             const int *trev);
 
+/* Real-space density the terms read (BlochStates::density_;
+ * src/bloch_states_scfloop_filling_density.cpp:256-358): complex128 [2][N]
+ * (spin up, spin down), normalised so that sum_r rho_s / (N * num_kpoints) is
+ * the electron count of spin s.  It is formed from the occupied orbitals and
+ * fillings of tcx_set_states on first use.  tcx_set_density replaces it, e.g.
+ * by the mixed density (1-beta) rho_old + beta rho_new of the reference's
+ * density mixing (:345-349), until the next tcx_set_states or
+ * tcx_set_kpoint_symmetry.  The gradient entry points always use the density
+ * of the orbitals. */
+int tcx_get_density(tcx_context *ctx, double *density);
+int tcx_set_density(tcx_context *ctx, const double *density);
+
 /* Host utility computing alpha and sum_of_Vaux for a k-set, restating
  * Potentials::set_sum_of_Vaux_each (src/potentials_coulomb.cpp:60-161). */
 int tcx_sum_of_vaux(const double recip_rows[9], double volume, int nk, const double *kfrac,

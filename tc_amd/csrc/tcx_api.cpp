@@ -380,6 +380,28 @@ void Context::set_symmetry(const int* k_src, const int* rot, const double* trans
     sym_set = true;
 }
 
+// The real-space density the terms read (BlochStates::density_): formed from
+// the occupied orbitals on first use, or replaced by the caller (density mixing).
+void Context::get_density(double* out)
+{
+    require(states_set, TCX_ERR_STATE, "get_density: call tcx_set_states first");
+    require(out != nullptr, TCX_ERR_ARG, "get_density: null pointer");
+    ensure_density();
+    TCX_HIP_CHECK(hipStreamSynchronize(stream));
+    TCX_HIP_CHECK(hipMemcpy(out, d_density.p, (size_t)2 * N * 16, hipMemcpyDeviceToHost));
+}
+
+void Context::set_density(const double* in)
+{
+    require(states_set, TCX_ERR_STATE, "set_density: call tcx_set_states first");
+    require(in != nullptr, TCX_ERR_ARG, "set_density: null pointer");
+    for (long long i = 0; i < 4 * N; ++i) require(std::isfinite(in[i]), TCX_ERR_ARG, "set_density: non-finite value");
+    TCX_HIP_CHECK(hipStreamSynchronize(stream));
+    d_density.ensure((size_t)2 * N * 16);
+    TCX_HIP_CHECK(hipMemcpy(d_density.p, in, (size_t)2 * N * 16, hipMemcpyHostToDevice));
+    density_valid = true;
+}
+
 // ------------------------------------------------------------------ real-space grids
 // Transforms ncoef-addressed orbitals (complex128, G space) to real space in
 // precision prec: FP64 scatter + FP64 inverse FFT, then rounding to float for
@@ -795,6 +817,16 @@ int tcx_set_div_correction(tcx_context* ctx, int enabled, double alpha, const do
 int tcx_set_kpoint_symmetry(tcx_context* ctx, const int* k_src, const int* rot, const double* trans, const int* trev)
 {
     return guarded(ctx, [&] { ctx->impl.set_symmetry(k_src, rot, trans, trev); });
+}
+
+int tcx_get_density(tcx_context* ctx, double* density)
+{
+    return guarded(ctx, [&] { ctx->impl.get_density(density); });
+}
+
+int tcx_set_density(tcx_context* ctx, const double* density)
+{
+    return guarded(ctx, [&] { ctx->impl.set_density(density); });
 }
 
 int tcx_hf2x(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision, unsigned flags)

@@ -192,6 +192,8 @@ struct Context {
     void set_jastrow(const double A[4], const double F[4]);
     void set_div(int enabled, double alpha, const double* svaux);
     void set_symmetry(const int* k_src, const int* rot, const double* trans, const int* trev);
+    void get_density(double* out);
+    void set_density(const double* in);
     void rotate_qsums(double* Q2, double* Q4, double* Q5);
     void invalidate_states();
 

@@ -37,6 +37,7 @@ EXPORTS = [
     "tcx_create", "tcx_destroy", "tcx_last_error", "tcx_synchronize",
     "tcx_set_cell", "tcx_set_kpoints", "tcx_set_basis", "tcx_set_states",
     "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux", "tcx_set_kpoint_symmetry",
+    "tcx_get_density", "tcx_set_density",
     "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
     "tcx_davidson", "tcx_total_energy", "tcx_eig_dense", "tcx_set_filling",
@@ -81,6 +82,8 @@ def load_library():
     L.tcx_set_div_correction.argtypes = [vp, C.c_int, C.c_double, _dp]
     L.tcx_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
     L.tcx_set_kpoint_symmetry.argtypes = [vp, _ip, _ip, _dp, _ip]
+    L.tcx_get_density.argtypes = [vp, vp]
+    L.tcx_set_density.argtypes = [vp, vp]
     for name in ("tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5"):
         getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_int, C.c_uint]
     for name in ("tcx_kinetic", "tcx_hf2h", "tcx_tc2h"):
@@ -246,6 +249,20 @@ class Engine:
         tr = np.ascontiguousarray(trev, dtype=np.int32)
         self._check(self.L.tcx_set_kpoint_symmetry(self.h, _ptr(ks, _ip), _ptr(r, _ip), _ptr(t), _ptr(tr, _ip)),
                     "set_kpoint_symmetry")
+
+    def get_density(self):
+        """The real-space density the terms read, complex128 (2, N)."""
+        if getattr(self, "sys", None) is None:
+            raise TcxError(TCX_ERR_STATE, "get_density: no system loaded")
+        n = int(np.prod(self.sys.n))
+        out = np.zeros((2, n), np.complex128)
+        self._check(self.L.tcx_get_density(self.h, out.ctypes.data), "get_density")
+        return out
+
+    def set_density(self, rho):
+        """Replace the density (complex128 (2, N)) until the next set_states."""
+        r = np.ascontiguousarray(rho, dtype=np.complex128)
+        self._check(self.L.tcx_set_density(self.h, r.ctypes.data), "set_density")
 
     def load_system(self, sys, div_corr=None):
         """Push a tc_amd.system.System into the context."""

@@ -13,9 +13,10 @@ It reads the examples' data files as data (no reference code is run or imported)
   * output.out            -- the printed results of the reference run (expected values).
 and writes tests/golden/qe_example{14,15,16}.npz: the inputs (cell, symmetry, k-points,
 Miller indices and the first num_bands_tc coefficient vectors per irreducible k) and the
-expected values of the first SCF iteration: the symmetry-replicated k-vectors, the Fermi
-energy from the QE eigenvalues, and after "Iteration 1" the Fermi energy, the eigenvalues
-and fillings at every irreducible k, and the 1/2/3-body and total energies.
+expected values: the symmetry-replicated k-vectors, the Fermi energy from the QE
+eigenvalues, after "Iteration 1" the Fermi energy, the eigenvalues and fillings at every
+irreducible k and the 1/2/3-body and total energies, and for the whole SCF run every
+printed Fermi energy / eigenvalue / filling block, total energy and density difference.
 """
 from __future__ import annotations
 
@@ -140,6 +141,28 @@ def read_output(path, nirr):
 
     start = lines.index(" Start SCF calculation!")
     ef0, _, _ = filling_block(start)
+    # the whole run: every set_filling print, every total energy, every density difference
+    scf = {"fermi_eV": [], "eigenvalues_eV": [], "fillings": [], "energies_Ha": [], "density_difference": []}
+    for i, ln in enumerate(lines):
+        if i < start:
+            continue
+        if "Fermi energy:" in ln:
+            ef, blocks, _ = filling_block(i)
+            scf["fermi_eV"].append(ef)
+            scf["eigenvalues_eV"].append([[r[0] for r in b] for b in blocks])
+            scf["fillings"].append([[r[1] for r in b] for b in blocks])
+        if ln.strip().startswith("Total energy ="):
+            e = {}
+            for back in lines[max(0, i - 3):i + 1]:
+                for key, tag in (("e1", "(1-body terms)"), ("e2", "(2-body terms)"), ("e3", "(3-body terms)"),
+                                 ("etot", "Total energy = ")):
+                    if tag in back and (key != "etot" or back.strip().startswith("Total energy =")):
+                        m = re.search(r"\( (\S+) , (\S+) \)", back)
+                        e[key] = [float(m.group(1)), float(m.group(2))]
+            scf["energies_Ha"].append(e)
+        m = re.match(r"\s+density difference \(L1 norm\) from the previous loop = (\S+)", ln)
+        if m:
+            scf["density_difference"].append(float(m.group(1)))
     it1 = lines.index(" Iteration 1")
     ef1, blocks1, j = filling_block(it1)
     energies = {}
@@ -153,7 +176,7 @@ def read_output(path, nirr):
             if key == "e3" and lines[j].strip().startswith("Total energy ="):
                 break
             j += 1
-    return equiv, ef0, ef1, blocks1, energies
+    return equiv, ef0, ef1, blocks1, energies, scf
 
 
 def main():
@@ -171,7 +194,7 @@ def main():
             assert len(m) == x["npw"][ik]
             mills.append(m)
             coefs.append(c)
-        equiv, ef0, ef1, blocks1, energies = read_output(os.path.join(d, "output.out"), nirr)
+        equiv, ef0, ef1, blocks1, energies, scf = read_output(os.path.join(d, "output.out"), nirr)
         expected = {
             "fermi_energy_initial_eV": ef0,
             "fermi_energy_iter1_eV": ef1,
@@ -179,6 +202,7 @@ def main():
             "fillings_iter1": [[r[1] for r in b] for b in blocks1],
             "energies_iter1_Ha": energies,
             "equivalent_kvectors": equiv,
+            "scf": scf,
         }
         setup = {
             "method": method,
@@ -187,6 +211,12 @@ def main():
             "smearing_width": float(params.get("smearing_width", 0.01)),
             "max_num_blocks_david": int(params.get("max_num_blocks_david", 2)),
             "num_refresh_david": int(params.get("num_refresh_david", 1)),
+            # Diagonalization defaults (include/diagonalization.hpp:75-89, diagonalization.cpp:48-64)
+            "max_num_iterations": int(params.get("max_num_iterations", 30)),
+            "mixing_beta": float(params.get("mixing_beta", 0.7)),
+            "mixes_density_matrix": params.get("mixes_density_matrix", "false").lower() == "true",
+            "energy_tolerance": float(params.get("energy_tolerance", 1e-6)),
+            "charge_tolerance": float(params.get("charge_tolerance", 1e-5)),
             "A_input": 1.0,  # default RPA Jastrow (no parameters.casl in the examples)
             "source": f"reference test/example{ex}",
         }

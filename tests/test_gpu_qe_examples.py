@@ -150,3 +150,53 @@ def test_kpoint_symmetry_rejects_inconsistent_relations(engine):
         engine.set_kpoint_symmetry(**bad(trev=(0, 1)))
     engine.set_kpoint_symmetry(**sym)
     engine.set_kpoint_symmetry(None)  # identity symmetry again
+
+
+# Example14's HF run does not converge: a degenerate pair at the second k (bands 3, 4,
+# fillings 1.25 / 1.25) splits from 0.001 eV at iteration 4 to 4.5 eV at iteration 8 under
+# the 0.01 eV Gaussian smearing, an unstable mode that amplifies any perturbation.  FP64
+# follows the reference through it (all 50 energies within ~1e-8 Ha); the FP32-split
+# trajectory leaves it there (its eigenvalues move by 2e-6 Ha after iteration 5), so FP32
+# is compared over the first four iterations only.
+FP32_EX14_STEPS = 4
+
+
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
+@pytest.mark.parametrize("ex", [15, 16, 14])
+def test_scf_run_matches_reference(engine, ex, prec):
+    """The whole SCF run of the example (max_num_iterations Davidson calls, density mixing
+    for example14/15, density-matrix mixing with old bands for example16): every printed
+    total energy (1-, 2-, 3-body and total), Fermi energy, eigenvalue block and density
+    difference; the reference's own check (test/test.py:33-74) is 1e-5 Ha on the last two
+    total energies and on the band-5 energy."""
+    qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
+    s = qe.expected["scf"]
+    r = scf.run(engine, qe, precision=prec)
+    assert r.iterations == qe.setup["max_num_iterations"] and not r.converged
+    ref_e = np.array([[e.get(k, [0.0])[0] for k in ("e1", "e2", "e3", "etot")] for e in s["energies_Ha"]])
+    got_e = np.array([e.real for e in r.energies])
+    assert got_e.shape == ref_e.shape
+    d_e = np.abs(got_e - ref_e).max(axis=1)
+    ref_ef = np.array(s["fermi_eV"]) / Q.HT_IN_EV
+    got_ef = np.array([f.fermi for f in r.fillings])
+    assert len(got_ef) == len(ref_ef)
+    d_ef = np.abs(got_ef - ref_ef)
+    d_eig = np.array([np.abs(f.eigenvalues.real * Q.HT_IN_EV - np.array(ev)).max() / Q.HT_IN_EV
+                      for f, ev in zip(r.fillings, s["eigenvalues_eV"])])
+    nd = len(s["density_difference"])
+    dd = np.abs(np.array(r.density_difference[:nd]) / np.array(s["density_difference"]) - 1.0)
+    print(f"example{ex} {qe.setup['method']} prec={prec} {r.iterations} iterations: max|dE| per step "
+          f"{np.array2string(d_e, precision=1, max_line_width=400)}; max|dE_F| {d_ef.max():.2e}; "
+          f"max|d eig| {d_eig.max():.2e} (first steps {np.array2string(d_eig[:8], precision=1)}, last {d_eig[-1]:.2e}); density difference rel {dd.max():.2e}; "
+          f"E_tot last two {got_e[-2:, 3]} (reference {ref_e[-2:, 3]})", flush=True)
+    n = len(d_e)
+    if ex == 14 and prec == tcx.PREC_FP32:
+        n = FP32_EX14_STEPS
+    tol_e, tol_eig, tol_dd = (5e-8, 1e-7, 1e-5) if prec == tcx.PREC_FP64 else (1e-6, 1e-6, 1e-4)
+    assert d_e[:n].max() < tol_e
+    assert d_ef[:n + 1].max() < tol_eig
+    assert d_eig[:n + 1].max() < tol_eig
+    assert dd[:n - 1].max() < tol_dd
+    if n == len(d_e):  # the reference's own end-to-end check
+        assert np.abs(got_e[-2:, 3] - ref_e[-2:, 3]).max() < 1e-5
+        assert d_eig[-1] < 1e-5
