@@ -21,12 +21,37 @@ constexpr int kPlaneBlock = 64;   // threads per xy-plane workgroup (plane side 
 
 __device__ __forceinline__ int centered(int i, int n) { return i <= n / 2 ? i : i - n; }
 
+
 template <typename T, int NZ>
 __device__ __forceinline__ void load_pencil(cpx<T> (&v)[NZ], const cpx<T>* __restrict__ p, int stride)
 {
     static_for<0, NZ>([&](auto zc) {
         constexpr int z = decltype(zc)::value;
         v[z] = p[(size_t)z * stride];
+    });
+}
+
+// Streaming (non-temporal) variants for grids that a pass touches once, so that
+// they do not displace the orbital grids that many workgroups re-read from cache.
+template <typename T, int NZ>
+__device__ __forceinline__ void store_pencil_nt(const cpx<T> (&v)[NZ], cpx<T>* __restrict__ p, int stride)
+{
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    static_for<0, NZ>([&](auto zc) {
+        constexpr int z = decltype(zc)::value;
+        const v2 x = {v[z].re, v[z].im};
+        __builtin_nontemporal_store(x, reinterpret_cast<v2*>(p + (size_t)z * stride));
+    });
+}
+
+template <typename T, int NZ>
+__device__ __forceinline__ void load_pencil_nt(cpx<T> (&v)[NZ], const cpx<T>* __restrict__ p, int stride)
+{
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    static_for<0, NZ>([&](auto zc) {
+        constexpr int z = decltype(zc)::value;
+        const v2 x = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p + (size_t)z * stride));
+        v[z] = cpx<T>{x.x, x.y};
     });
 }
 

@@ -23,6 +23,8 @@
 
 namespace tcx {
 
+// The pair record is written once and read by the next pass only: streaming
+// stores keep it from displacing the orbital grids in L2 / MALL.
 template <typename T, int NZ>
 __global__ __launch_bounds__(kPencilBlock) void k_hf2x_A(const cpx<T>* __restrict__ phiR_app,
                                                          const cpx<T>* __restrict__ phiR_occ,
@@ -41,7 +43,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_A(const cpx<T>* __restric
         v[z] = cmulc(b[(size_t)z * nxy], a[(size_t)z * nxy]);
     });
     fft<T, NZ, -1>(v);
-    store_pencil<T, NZ>(v, buf + (size_t)blockIdx.y * ng + xy, nxy);
+    store_pencil_nt<T, NZ>(v, buf + (size_t)blockIdx.y * ng + xy, nxy);
 }
 
 // Coulomb kernel table of a launch: v(G)/N = 1/(N |k-q+G|^2), 0 below 1e-8
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(256) void k_coulomb_table(T* __restrict__ table, co
 // One workgroup transforms PL planes of one pair (PlaneGroup): thread t owns
 // line t % NP of plane t / NP.  Column load (coalesced) -> FFT_y -> LDS
 // transpose -> FFT_x -> x v(G)/N -> inverse FFT_x -> LDS transpose -> inverse
-// FFT_y -> column store.
+// FFT_y -> column store.  Each plane is read and written once (streaming).
 template <typename T, int NP>
 __global__ __launch_bounds__((PlaneGroup<T, NP>::threads)) void k_hf2x_B(cpx<T>* __restrict__ buf,
                                                                        const PairEntry* __restrict__ pairs,
@@ -83,7 +85,7 @@ __global__ __launch_bounds__((PlaneGroup<T, NP>::threads)) void k_hf2x_B(cpx<T>*
     PlaneTile<T, NP>& tile = tiles[active ? p : 0];
     cpx<T> v[NP];
     if (active) {
-        load_pencil<T, NP>(v, plane + t, NP);
+        load_pencil_nt<T, NP>(v, plane + t, NP);
         fft<T, NP, -1>(v);
         tile_put_col<T, NP>(tile, v, t);
     }
@@ -104,7 +106,7 @@ __global__ __launch_bounds__((PlaneGroup<T, NP>::threads)) void k_hf2x_B(cpx<T>*
     if (active) {
         tile_get_col<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
-        store_pencil<T, NP>(v, plane + t, NP);
+        store_pencil_nt<T, NP>(v, plane + t, NP);
     }
 }
 

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_A(const cpx<T>* __restric
             v[m] = cmulc(x[m], src[(size_t)(S * m) * nxy]);
         });
         Split::template fft<T, NZ, -1>(v, h);
-        if (valid) store_pencil<T, M>(v, out + (size_t)c * ng, nxy);
+        if (valid) store_pencil_nt<T, M>(v, out + (size_t)c * ng, nxy);  // streaming: read once by pass B
     }
 }
 
@@ -119,7 +119,7 @@ __global__ __launch_bounds__((Tc2xPlanes<T, NP>::threads)) void k_tc2x_B(cpx<T>*
     cpx<T>* base = buf + ((size_t)blockIdx.y * 4 + (on ? c : 0)) * ng + (size_t)(on ? z : 0) * NP * NP;
     cpx<T> v[NP];
     if (on) {
-        load_pencil<T, NP>(v, base + t, NP);   // column x = t
+        load_pencil_nt<T, NP>(v, base + t, NP);  // column x = t; planes are streamed once
         fft<T, NP, -1>(v);                     // along y
         tile_put_col<T, NP>(tile, v, t);
     }
@@ -166,7 +166,7 @@ __global__ __launch_bounds__((Tc2xPlanes<T, NP>::threads)) void k_tc2x_B(cpx<T>*
     if (on) {
         tile_get_col<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
-        store_pencil<T, NP>(v, base + t, NP);
+        store_pencil_nt<T, NP>(v, base + t, NP);
     }
 }
 

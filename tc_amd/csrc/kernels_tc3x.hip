@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_products_z(const PairEntry* __
     });
     fft<T, NZ, -1>(v);
     static_for<0, NZ>([&](auto zc) { v[decltype(zc)::value] = scale(v[decltype(zc)::value], sc); });
-    store_pencil<T, NZ>(v, out + (size_t)rec * ng + xy, nxy);
+    store_pencil_nt<T, NZ>(v, out + (size_t)rec * ng + xy, nxy);  // streaming: read once by the xy pass
 }
 
 template <typename T>

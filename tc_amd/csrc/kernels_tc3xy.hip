@@ -55,7 +55,7 @@ __device__ __forceinline__ void xy_forward(PlaneTile<T, NP>& tile, const cpx<T>*
     cpx<T> v[NP];
     const bool on = job && t < NP;
     if (on) {
-        load_pencil<T, NP>(v, plane + t, NP);
+        load_pencil_nt<T, NP>(v, plane + t, NP);  // record planes are streamed once
         fft<T, NP, -1>(v);
         tile_put_col<T, NP>(tile, v, t);
     }
@@ -83,7 +83,7 @@ __device__ __forceinline__ void xy_inverse_store(PlaneTile<T, NP>& tile, cpx<T>*
     if (on) {
         tile_get_col<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
-        store_pencil<T, NP>(v, plane + t, NP);
+        store_pencil_nt<T, NP>(v, plane + t, NP);
     }
     __syncthreads();
 }
