@@ -39,10 +39,13 @@ SOURCES = [
     "csrc/tcx_davidson.cpp",
     "csrc/tcx_adjoint.cpp",
     "host/calc_hamiltonian.cpp",
+    "host/scf.cpp",
 ]
 ROOT = os.path.dirname(HERE)
 SHIM_TEST_SRC = os.path.join(ROOT, "tests", "host_shim_test.cpp")
 SHIM_TEST_BIN = os.path.join(ROOT, "tests", "_bin", "host_shim_test")
+SCF_TEST_SRC = os.path.join(ROOT, "tests", "scf_host_test.cpp")
+SCF_TEST_BIN = os.path.join(ROOT, "tests", "_bin", "scf_host_test")
 FFT_TEST_SRC = os.path.join(ROOT, "tests", "fft_reg_test.cpp")
 FFT_TEST_BIN = os.path.join(ROOT, "tests", "_bin", "fft_reg_test")
 
@@ -83,15 +86,16 @@ def build(verbose: bool = True) -> str:
 
 
 def build_shim_test(verbose: bool = True) -> str:
-    """C++ program driving the host mirror calc_hamiltonian::* (run by a GPU test)."""
+    """C++ programs driving the host side (calc_hamiltonian::* and the SCF loop), run by GPU tests."""
     os.makedirs(os.path.dirname(SHIM_TEST_BIN), exist_ok=True)
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", SHIM_TEST_SRC, "-o", SHIM_TEST_BIN, "-L", HERE, "-ltcx",
-           "-Wl,-rpath,$ORIGIN/../../tc_amd"]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"shim test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    if verbose:
-        print(f"built {SHIM_TEST_BIN}")
+    for src, out in ((SHIM_TEST_SRC, SHIM_TEST_BIN), (SCF_TEST_SRC, SCF_TEST_BIN)):
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", src, "-o", out, "-L", HERE, "-ltcx",
+               "-Wl,-rpath,$ORIGIN/../../tc_amd"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"host test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+        if verbose:
+            print(f"built {out}")
     return SHIM_TEST_BIN
 
 

@@ -194,3 +194,41 @@ def mesh_symmetry(qe: QEInput, reps: list) -> dict:
             "rot": np.array([U[r.isym] for r in reps], np.int32),
             "trans": np.array([qe.trans[r.isym] for r in reps], np.float64),
             "trev": np.array([int(r.trev) for r in reps], np.int32)}
+
+
+def write_flat(qe: QEInput, path: str) -> None:
+    """The QE inputs and SCF settings as named little-endian records for the C++ host
+    (tc_amd/host/scf.hpp; read by tests/scf_host_test.cpp): after the 8-byte magic
+    "TCXQE001", each record is a 32-byte name, an 8-byte numpy type tag ("<f8", "<i4",
+    "<c16"), an int64 element count and the data."""
+    st = qe.setup
+    methods = {"HF": 0, "TC": 1, "BITC": 2}
+    recs = [
+        ("cell", np.asarray(qe.cell, "<f8")), ("b_norm", np.asarray(qe.b_norm, "<f8")),
+        ("grid", np.asarray(qe.grid, "<i4")), ("rot_qe", np.asarray(qe.rot_qe, "<i4")),
+        ("trans", np.asarray(qe.trans, "<f8")), ("time_reversal", np.asarray([int(qe.time_reversal)], "<i4")),
+        ("nelec", np.asarray([qe.nelec], "<f8")), ("kcart", np.asarray(qe.kcart, "<f8")),
+        ("kweight", np.asarray(qe.kweight, "<f8")), ("nb", np.asarray([qe.nbands], "<i4")),
+        ("eig", np.asarray(qe.eig_qe, "<f8")), ("npw", np.asarray([len(m) for m in qe.mill], "<i4")),
+        ("mill", np.concatenate([np.asarray(m, "<i4").reshape(-1) for m in qe.mill])),
+        ("coef", np.concatenate([np.asarray(c, "<c16").reshape(-1) for c in qe.coef])),
+        ("method", np.asarray([methods[st["method"]]], "<i4")),
+        ("smearing", np.asarray([{"fixed": 0, "gaussian": 1}[st["smearing_mode"]]], "<i4")),
+        ("smearing_width", np.asarray([float(st["smearing_width"])], "<f8")),
+        ("mixing_beta", np.asarray([float(st.get("mixing_beta", 0.7))], "<f8")),
+        ("mixes_density_matrix", np.asarray([int(bool(st.get("mixes_density_matrix", False)))], "<i4")),
+        ("max_num_iterations", np.asarray([int(st.get("max_num_iterations", 30))], "<i4")),
+        ("energy_tolerance", np.asarray([float(st.get("energy_tolerance", 1e-6))], "<f8")),
+        ("charge_tolerance", np.asarray([float(st.get("charge_tolerance", 1e-5))], "<f8")),
+        ("max_num_blocks_david", np.asarray([int(st["max_num_blocks_david"])], "<i4")),
+        ("num_refresh_david", np.asarray([int(st["num_refresh_david"])], "<i4")),
+        ("A_input", np.asarray([float(st.get("A_input", 1.0))], "<f8")),
+    ]
+    with open(path, "wb") as f:
+        f.write(b"TCXQE001")
+        for name, arr in recs:
+            a = np.ascontiguousarray(arr)
+            f.write(name.encode().ljust(32, b"\0"))
+            f.write(a.dtype.str.encode().ljust(8, b"\0"))
+            f.write(np.int64(a.size).tobytes())
+            f.write(a.tobytes())

@@ -161,33 +161,28 @@ def test_kpoint_symmetry_rejects_inconsistent_relations(engine):
 FP32_EX14_STEPS = 4
 
 
-@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
-@pytest.mark.parametrize("ex", [15, 16, 14])
-def test_scf_run_matches_reference(engine, ex, prec):
-    """The whole SCF run of the example (max_num_iterations Davidson calls, density mixing
-    for example14/15, density-matrix mixing with old bands for example16): every printed
-    total energy (1-, 2-, 3-body and total), Fermi energy, eigenvalue block and density
-    difference; the reference's own check (test/test.py:33-74) is 1e-5 Ha on the last two
-    total energies and on the band-5 energy."""
-    qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
+def _check_run(qe, ex, prec, label, fermi, eigs, energies, ddiff, iterations, converged):
+    """Every printed total energy (1-, 2-, 3-body and total), Fermi energy, eigenvalue
+    block and density difference of the reference's run; the reference's own check
+    (test/test.py:33-74) is 1e-5 Ha on the last two total energies and the band-5 energy."""
     s = qe.expected["scf"]
-    r = scf.run(engine, qe, precision=prec)
-    assert r.iterations == qe.setup["max_num_iterations"] and not r.converged
+    assert iterations == qe.setup["max_num_iterations"] and not converged
     ref_e = np.array([[e.get(k, [0.0])[0] for k in ("e1", "e2", "e3", "etot")] for e in s["energies_Ha"]])
-    got_e = np.array([e.real for e in r.energies])
+    got_e = np.array([np.real(e) for e in energies])
     assert got_e.shape == ref_e.shape
     d_e = np.abs(got_e - ref_e).max(axis=1)
     ref_ef = np.array(s["fermi_eV"]) / Q.HT_IN_EV
-    got_ef = np.array([f.fermi for f in r.fillings])
+    got_ef = np.array(fermi)
     assert len(got_ef) == len(ref_ef)
     d_ef = np.abs(got_ef - ref_ef)
-    d_eig = np.array([np.abs(f.eigenvalues.real * Q.HT_IN_EV - np.array(ev)).max() / Q.HT_IN_EV
-                      for f, ev in zip(r.fillings, s["eigenvalues_eV"])])
+    d_eig = np.array([np.abs(np.real(e) * Q.HT_IN_EV - np.array(ev)).max() / Q.HT_IN_EV
+                      for e, ev in zip(eigs, s["eigenvalues_eV"])])
     nd = len(s["density_difference"])
-    dd = np.abs(np.array(r.density_difference[:nd]) / np.array(s["density_difference"]) - 1.0)
-    print(f"example{ex} {qe.setup['method']} prec={prec} {r.iterations} iterations: max|dE| per step "
+    dd = np.abs(np.array(ddiff[:nd]) / np.array(s["density_difference"]) - 1.0)
+    print(f"{label} example{ex} {qe.setup['method']} prec={prec} {iterations} iterations: max|dE| per step "
           f"{np.array2string(d_e, precision=1, max_line_width=400)}; max|dE_F| {d_ef.max():.2e}; "
-          f"max|d eig| {d_eig.max():.2e} (first steps {np.array2string(d_eig[:8], precision=1)}, last {d_eig[-1]:.2e}); density difference rel {dd.max():.2e}; "
+          f"max|d eig| {d_eig.max():.2e} (first steps {np.array2string(d_eig[:8], precision=1)}, last "
+          f"{d_eig[-1]:.2e}); density difference rel {dd.max():.2e}; "
           f"E_tot last two {got_e[-2:, 3]} (reference {ref_e[-2:, 3]})", flush=True)
     n = len(d_e)
     if ex == 14 and prec == tcx.PREC_FP32:
@@ -200,3 +195,46 @@ def test_scf_run_matches_reference(engine, ex, prec):
     if n == len(d_e):  # the reference's own end-to-end check
         assert np.abs(got_e[-2:, 3] - ref_e[-2:, 3]).max() < 1e-5
         assert d_eig[-1] < 1e-5
+
+
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
+@pytest.mark.parametrize("ex", [15, 16, 14])
+def test_scf_run_matches_reference(engine, ex, prec):
+    """The whole SCF run of the example (max_num_iterations Davidson calls, density mixing
+    for example14/15, density-matrix mixing with old bands for example16) through the
+    Python harness (tc_amd.scf)."""
+    qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
+    r = scf.run(engine, qe, precision=prec)
+    _check_run(qe, ex, prec, "python", [f.fermi for f in r.fillings], [f.eigenvalues for f in r.fillings],
+               r.energies, r.density_difference, r.iterations, r.converged)
+
+
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
+@pytest.mark.parametrize("ex", [15, 16, 14])
+def test_cpp_host_scf_matches_reference(ex, prec, tmp_path):
+    """The same runs through the C++ host (tc_amd/host/scf.cpp: Diagonalization::scf with
+    the symmetry expansion in C++), driven by tests/scf_host_test.cpp in a child process."""
+    import subprocess
+    binp = os.path.join(os.path.dirname(HERE), "tests", "_bin", "scf_host_test")
+    if not os.path.exists(binp):
+        pytest.fail(f"{binp} is not built (run __graft_entry__.build())")
+    qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
+    path = str(tmp_path / "in.tcxqe")
+    Q.write_flat(qe, path)
+    res = subprocess.run([binp, path, str(prec)], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+    fermi, eigs, energies, ddiff, done = [], [], [], [], None
+    for ln in res.stdout.splitlines():
+        t = ln.split()
+        if t[0] == "FILL":
+            fermi.append(float(t[1]))
+            eigs.append(np.array([float(x) for x in t[2:]]).reshape(qe.nirr, qe.nbands))
+        elif t[0] == "ENERGY":
+            v = [float(x) for x in t[1:]]
+            energies.append(np.array([complex(v[2 * i], v[2 * i + 1]) for i in range(4)]))
+        elif t[0] == "DDIFF":
+            ddiff.append(float(t[1]))
+        elif t[0] == "DONE":
+            done = (int(t[1]), bool(int(t[2])))
+    assert done is not None
+    _check_run(qe, ex, prec, "c++", fermi, eigs, energies, ddiff, done[0], done[1])

@@ -68,3 +68,27 @@ def test_initial_fermi_energy_matches_reference(ex):
     assert abs(mu * Q.HT_IN_EV - qe.expected["fermi_energy_initial_eV"]) < 1e-8
     # electron count with the QE weights
     assert abs(np.sum(qe.kweight[:, None] * f.reshape(qe.nirr, -1)) / 2.0 - qe.nelec) < 1e-9
+
+
+@pytest.mark.parametrize("ex", [14, 15, 16])
+def test_cpp_host_replication_matches(ex, tmp_path):
+    """The C++ host (tc_amd/host/scf.cpp) builds the same symmetry-expanded k-mesh and
+    replica plane-wave indices / phases as tc_amd.qe (no GPU: the program's replicas mode)."""
+    import subprocess
+    binp = os.path.join(os.path.dirname(HERE), "tests", "_bin", "scf_host_test")
+    if not os.path.exists(binp):
+        pytest.skip("scf_host_test not built (run tc_amd/build.py)")
+    qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
+    path = str(tmp_path / "in.tcxqe")
+    Q.write_flat(qe, path)
+    res = subprocess.run([binp, path, "replicas"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    rows = [ln.split()[1:] for ln in res.stdout.splitlines() if ln.startswith("REP")]
+    reps = Q.replicate_kpoints(qe)
+    assert len(rows) == len(reps)
+    for row, r in zip(rows, reps):
+        gidx, ph = Q.replica_basis(qe, r)
+        assert (int(row[0]), int(row[1]), int(row[2])) == (r.ik, r.isym, int(r.trev))
+        assert np.allclose([float(x) for x in row[3:6]], r.kcrys, atol=1e-12)
+        assert int(row[6]) == len(gidx) and int(row[7]) == int(gidx.astype(np.int64).sum())
+        assert abs(complex(float(row[8]), float(row[9])) - ph.sum()) < 1e-9
