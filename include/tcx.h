@@ -125,7 +125,9 @@ int tcx_set_kpoint_symmetry(tcx_context *ctx, const int *k_src, const int *rot, 
  * src/bloch_states_scfloop_filling_density.cpp:256-358): complex128 [2][N]
  * (spin up, spin down), normalised so that sum_r rho_s / (N * num_kpoints) is
  * the electron count of spin s.  It is formed from the occupied orbitals and
- * fillings of tcx_set_states on first use.  tcx_set_density replaces it, e.g.
+ * fillings of tcx_set_states on first use.  With one independent spin both
+ * spins take the first grid, as the reference sets density_[1] = density_[0]
+ * (:355-358); tcx_set_density then ignores the second.  tcx_set_density replaces it, e.g.
  * by the mixed density (1-beta) rho_old + beta rho_new of the reference's
  * density mixing (:345-349), until the next tcx_set_states or
  * tcx_set_kpoint_symmetry.  The gradient entry points always use the density

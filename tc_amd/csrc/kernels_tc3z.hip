@@ -70,7 +70,7 @@ template <typename T, int NZ, int S>
 __global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P, const cpx<T>* __restrict__ occR,
                                                           const PairEntry* __restrict__ pairs,
                                                           const TargetEntry* __restrict__ targets,
-                                                          const double2* __restrict__ dens,
+                                                          const double2* __restrict__ dens, size_t dstride,
                                                           double2* __restrict__ qsum, int nxy, T sc)
 {
     constexpr int M = NZ / S;
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P
             cpx<T>* g = P + ((size_t)pp * 6 + 3 * s + d) * ng;
             cpx<T> v[M];
             zinv<T, NZ, S>(v, g, L, nxy);
-            const double2* n = dens + (size_t)s * ng + o0;
+            const double2* n = dens + s * dstride + o0;  // dstride 0: one independent spin
             static_for<0, M>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
                 const double2 x = upd(v[k]);
@@ -274,8 +274,8 @@ void for_target_slices(int n, F&& f)
     } while (0)
 
 void launch_za7_mid(int prec, const Geom& g, void* P, const void* occR, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, const double* dens, double* qsum, double sc,
-                    hipStream_t s)
+                    const TargetEntry* targets, int ntargets, const double* dens, long long dens_stride,
+                    double* qsum, double sc, hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
     for_target_slices(ntargets, [&](int a, int n) {
@@ -285,7 +285,7 @@ void launch_za7_mid(int prec, const Geom& g, void* P, const void* occR, const Pa
         constexpr int PN = PencilSplit<S>::pencils;                                                          \
         hipLaunchKernelGGL((k_za7_mid<T, NZ, S>), dim3((nxy + PN - 1) / PN, n, 3), dim3(kPencilBlock), 0, s,\
                            (cpx<T>*)P, (const cpx<T>*)occR, pairs, targets + a, (const double2*)dens,        \
-                           (double2*)qsum, nxy, (T)sc);                                                      \
+                           (size_t)dens_stride, (double2*)qsum, nxy, (T)sc);                                 \
     }
         TCX_Z_DISPATCH(prec, g, TCX_L);
 #undef TCX_L

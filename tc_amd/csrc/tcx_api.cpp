@@ -398,7 +398,10 @@ void Context::set_density(const double* in)
     for (long long i = 0; i < 4 * N; ++i) require(std::isfinite(in[i]), TCX_ERR_ARG, "set_density: non-finite value");
     TCX_HIP_CHECK(hipStreamSynchronize(stream));
     d_density.ensure((size_t)2 * N * 16);
-    TCX_HIP_CHECK(hipMemcpy(d_density.p, in, (size_t)2 * N * 16, hipMemcpyHostToDevice));
+    // one independent spin: density_[1] = density_[0] (bloch_states_scfloop_filling_density.cpp:355-358)
+    const double* in1 = nspin == 1 ? in : in + 2 * N;
+    TCX_HIP_CHECK(hipMemcpy(d_density.p, in, (size_t)N * 16, hipMemcpyHostToDevice));
+    TCX_HIP_CHECK(hipMemcpy((char*)d_density.p + (size_t)N * 16, in1, (size_t)N * 16, hipMemcpyHostToDevice));
     density_valid = true;
 }
 

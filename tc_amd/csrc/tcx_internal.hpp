@@ -174,9 +174,10 @@ void launch_xy_a8c2(int prec, const Geom& g, const void* rec, void* Z, const Pai
                     const void* tab, int kq0, const double* kqcart, const double* fill, double fa4, double fb5,
                     hipStream_t s);
 // z passes of FFT_backward fused with the real-space consumers (kernels_tc3z.hip)
+// dens_stride: complex elements between the two spin densities (0: one independent spin)
 void launch_za7_mid(int prec, const Geom& g, void* P, const void* occR, const PairEntry* pairs,
-                    const TargetEntry* targets, int ntargets, const double* dens, double* qsum, double sc,
-                    hipStream_t s);
+                    const TargetEntry* targets, int ntargets, const double* dens, long long dens_stride,
+                    double* qsum, double sc, hipStream_t s);
 void launch_za8_q(int prec, const Geom& g, const void* O, int bitc, const void* occR, const void* chiR,
                   const PairEntry* pairs, const TargetEntry* targets, int ntargets, const double* fill, double* Q2,
                   double* Q4, double* Q5, hipStream_t s);

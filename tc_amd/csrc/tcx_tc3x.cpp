@@ -167,8 +167,9 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             });
             timed(1, np, [&] {
                 launch_xy_a7c1(prec, geo, rec4, Pb, Y, pairs, np, d_ktab.p, L.kq0, kq, fill, fb3, stream);
-                launch_za7_mid(prec, geo, Pb, occR, pairs, targets, L.ntargets, (const double*)d_density.p, qsum,
-                               1.0 / (double)N, stream);
+                // one independent spin: both spin channels read the one density grid
+                launch_za7_mid(prec, geo, Pb, occR, pairs, targets, L.ntargets, (const double*)d_density.p,
+                               nspin == 1 ? 0 : N, qsum, 1.0 / (double)N, stream);
                 launch_xy_a7c2(prec, geo, Pb, Y, pairs, np, d_ktab.p, L.kq0, kq, fb6, stream);
             });
             timed(2, np, [&] {  // z pass of FFT_backward fused with the accumulation
