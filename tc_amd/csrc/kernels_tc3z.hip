@@ -84,13 +84,27 @@ __global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P
     const size_t o0 = L.xy + (size_t)L.ob * M * nxy;  // first output point of the lane
     double2 acc[M];
     static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = make_double2(0.0, 0.0); });
-    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
-        const cpx<T>* ph = occR + (size_t)pairs[pp].occ * ng + o0;
+    // steps it = 2 (pair) + s over the target's pairs; the pencil of step it + 1 is
+    // loaded before step it is transformed (steps touch distinct grids)
+    const int nsteps = 2 * te.npair;
+    auto grid_of = [&](int it) { return P + ((size_t)(te.pair0 + (it >> 1)) * 6 + 3 * (it & 1) + d) * ng; };
+    constexpr bool kPrefetch = sizeof(T) == 4;  // FP64 instances have no VGPRs to spare
+    cpx<T> vn[M];
+    if (kPrefetch && nsteps > 0) load_pencil<T, M>(vn, grid_of(0) + L.xy + (size_t)L.h * nxy, S * nxy);
 #pragma unroll 1
-        for (int s = 0; s < 2; ++s) {
-            cpx<T>* g = P + ((size_t)pp * 6 + 3 * s + d) * ng;
-            cpx<T> v[M];
-            zinv<T, NZ, S>(v, g, L, nxy);
+    for (int it = 0; it < nsteps; ++it) {
+        const int pp = te.pair0 + (it >> 1), s = it & 1;
+        const cpx<T>* ph = occR + (size_t)pairs[pp].occ * ng + o0;
+        cpx<T>* g = grid_of(it);
+        cpx<T> v[M];
+        if (kPrefetch) {
+            static_for<0, M>([&](auto mc) { v[decltype(mc)::value] = vn[decltype(mc)::value]; });
+            if (it + 1 < nsteps) load_pencil<T, M>(vn, grid_of(it + 1) + L.xy + (size_t)L.h * nxy, S * nxy);
+        } else {
+            load_pencil<T, M>(v, g + L.xy + (size_t)L.h * nxy, S * nxy);  // z = S m + h
+        }
+        PencilSplit<S>::template fft<T, NZ, 1>(v, L.h);  // -> z = ob M + k
+        {
             const double2* n = dens + s * dstride + o0;  // dstride 0: one independent spin
             static_for<0, M>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
@@ -139,15 +153,39 @@ __global__ __launch_bounds__(kPencilBlock) void k_za8_q2(const cpx<T>* __restric
     const int nch = bitc ? 12 : 6;
     double2 acc[M];
     static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = make_double2(0.0, 0.0); });
-    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
-        const double f1 = fill[pairs[pp].occ];
-        const cpx<T>* base = O + (size_t)pp * nch * ng;
-        for (int c = 0; c < 6; ++c) {
+    // steps it = 6 (pair) + c; the pencil(s) of step it + 1 are loaded before step it is
+    // transformed
+    const int nsteps = 6 * te.npair;
+    auto grid_of = [&](int it, int half) {
+        return O + ((size_t)(te.pair0 + it / 6) * nch + 6 * half + it % 6) * ng + L.xy + (size_t)L.h * nxy;
+    };
+    constexpr bool kPrefetch = sizeof(T) == 4;  // FP64 instances have no VGPRs to spare
+    cpx<T> vn[M], wn[M];
+    if (kPrefetch && nsteps > 0) {
+        load_pencil<T, M>(vn, grid_of(0, 0), S * nxy);  // z = S m + h
+        if (bitc) load_pencil<T, M>(wn, grid_of(0, 1), S * nxy);
+    }
+#pragma unroll 1
+    for (int it = 0; it < nsteps; ++it) {
+        const double f1 = fill[pairs[te.pair0 + it / 6].occ];
+        {
             cpx<T> v[M];
-            zinv<T, NZ, S>(v, base + (size_t)c * ng, L, nxy);
+            if (kPrefetch) {
+                static_for<0, M>([&](auto mc) { v[decltype(mc)::value] = vn[decltype(mc)::value]; });
+                if (it + 1 < nsteps) load_pencil<T, M>(vn, grid_of(it + 1, 0), S * nxy);
+            } else {
+                load_pencil<T, M>(v, grid_of(it, 0), S * nxy);
+            }
+            PencilSplit<S>::template fft<T, NZ, 1>(v, L.h);  // -> z = ob M + k
             if (bitc) {
                 cpx<T> w[M];
-                zinv<T, NZ, S>(w, base + (size_t)(6 + c) * ng, L, nxy);
+                if (kPrefetch) {
+                    static_for<0, M>([&](auto mc) { w[decltype(mc)::value] = wn[decltype(mc)::value]; });
+                    if (it + 1 < nsteps) load_pencil<T, M>(wn, grid_of(it + 1, 1), S * nxy);
+                } else {
+                    load_pencil<T, M>(w, grid_of(it, 1), S * nxy);
+                }
+                PencilSplit<S>::template fft<T, NZ, 1>(w, L.h);
                 static_for<0, M>([&](auto kc) {
                     constexpr int k = decltype(kc)::value;
                     const double2 a = upd(v[k]), b = upd(w[k]);
