@@ -132,11 +132,22 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_C(const cpx<T>* __restric
     const size_t ng = (size_t)nxy * NZ;
     cpx<double> acc[M];
     static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = cpx<double>{0.0, 0.0}; });
+    // the pencil of pair pp + 1 is loaded before pair pp is transformed (FP32)
+    constexpr bool kPrefetch = sizeof(T) == 4 && M <= 20;  // 48^3 pencils would drop to one wave
+    auto grid_of = [&](int pp) { return buf + (size_t)pp * ng + xy + (size_t)h * nxy; };
+    cpx<T> vn[M];
+    if (kPrefetch && te.npair > 0) load_pencil<T, M>(vn, grid_of(te.pair0), S * nxy);
+#pragma unroll 1
     for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
         const int occ = pairs[pp].occ;
         cpx<T> v[M];
-        load_pencil<T, M>(v, buf + (size_t)pp * ng + xy + (size_t)h * nxy, S * nxy);  // z = S m + h
-        Split::template fft<T, NZ, 1>(v, h);                                         // -> z = ob*M + k
+        if (kPrefetch) {
+            static_for<0, M>([&](auto mc) { v[decltype(mc)::value] = vn[decltype(mc)::value]; });
+            if (pp + 1 < te.pair0 + te.npair) load_pencil<T, M>(vn, grid_of(pp + 1), S * nxy);
+        } else {
+            load_pencil<T, M>(v, grid_of(pp), S * nxy);  // z = S m + h
+        }
+        Split::template fft<T, NZ, 1>(v, h);  // -> z = ob*M + k
         const cpx<T>* q = phiR_occ + (size_t)occ * ng + xy + (size_t)ob * M * nxy;
         const double f = fill ? fill[occ] : 1.0;
         static_for<0, M>([&](auto kc) {

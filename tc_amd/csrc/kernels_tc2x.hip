@@ -192,12 +192,26 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_C(const cpx<T>* __restric
     const size_t ng = (size_t)nxy * NZ;
     cpx<double> acc[M];
     static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = cpx<double>{0.0, 0.0}; });
-    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
+    // steps it = 4 (pair) + c; the pencil of step it + 1 is loaded before step it is
+    // transformed (FP32)
+    constexpr bool kPrefetch = sizeof(T) == 4 && M <= 20;  // 48^3 pencils would drop to one wave
+    const int nsteps = 4 * te.npair;
+    auto grid_of = [&](int it) { return buf + ((size_t)te.pair0 * 4 + it) * ng + xy + (size_t)h * nxy; };
+    cpx<T> vn[M];
+    if (kPrefetch && nsteps > 0) load_pencil<T, M>(vn, grid_of(0), S * nxy);
+#pragma unroll 1
+    for (int it = 0; it < nsteps; ++it) {
+        const int pp = te.pair0 + it / 4, c = it % 4;
         const int occ = pairs[pp].occ;
         const double f = fill[occ];
-        for (int c = 0; c < 4; ++c) {
+        {
             cpx<T> v[M];
-            load_pencil<T, M>(v, buf + ((size_t)pp * 4 + c) * ng + xy + (size_t)h * nxy, S * nxy);
+            if (kPrefetch) {
+                static_for<0, M>([&](auto mc) { v[decltype(mc)::value] = vn[decltype(mc)::value]; });
+                if (it + 1 < nsteps) load_pencil<T, M>(vn, grid_of(it + 1), S * nxy);
+            } else {
+                load_pencil<T, M>(v, grid_of(it), S * nxy);  // z = S m + h
+            }
             Split::template fft<T, NZ, 1>(v, h);
             // channel 0: b * phi_qi (+f); channels 1..3: a_d * g_qi,d (-f)   [tc2x.cpp:363-369, :387-388]
             const cpx<T>* m = (c == 0 ? occR + (size_t)occ * ng : gradR + ((size_t)occ * 3 + (c - 1)) * ng) + xy +

@@ -229,12 +229,24 @@ __global__ __launch_bounds__(kPencilBlock) void k_za8_q45(const cpx<T>* __restri
     const double sy = conj5 ? -1.0 : 1.0;  // conj flips the imaginary part; Q5 also flips the sign
     double2 acc[M];
     static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = make_double2(0.0, 0.0); });
+    // the pencil of pair pp + 1 is loaded before pair pp is transformed (FP32)
+    constexpr bool kPrefetch = sizeof(T) == 4;
+    auto grid_of = [&](int pp) { return O + ((size_t)pp * nch + ch0 + d) * ng + L.xy + (size_t)L.h * nxy; };
+    cpx<T> vn[M];
+    if (kPrefetch && te.npair > 0) load_pencil<T, M>(vn, grid_of(te.pair0), S * nxy);
+#pragma unroll 1
     for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
         const PairEntry e = pairs[pp];
         const double f1 = fill[e.occ];
         const cpx<T>* xg = X + (size_t)e.occ * ng + o0;
         cpx<T> v[M];
-        zinv<T, NZ, S>(v, O + ((size_t)pp * nch + ch0 + d) * ng, L, nxy);
+        if (kPrefetch) {
+            static_for<0, M>([&](auto mc) { v[decltype(mc)::value] = vn[decltype(mc)::value]; });
+            if (pp + 1 < te.pair0 + te.npair) load_pencil<T, M>(vn, grid_of(pp + 1), S * nxy);
+        } else {
+            load_pencil<T, M>(v, grid_of(pp), S * nxy);  // z = S m + h
+        }
+        PencilSplit<S>::template fft<T, NZ, 1>(v, L.h);  // -> z = ob M + k
         static_for<0, M>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             const double2 a = upd(v[k]), b = upd(xg[(size_t)k * nxy]);
@@ -267,11 +279,26 @@ __global__ __launch_bounds__(kPencilBlock) void k_za8_apply(const cpx<T>* __rest
     const size_t o0 = L.xy + (size_t)L.ob * M * nxy;
     double2 acc[M];
     static_for<0, M>([&](auto kc) { acc[decltype(kc)::value] = make_double2(0.0, 0.0); });
-    for (int pp = te.pair0; pp < te.pair0 + te.npair; ++pp) {
+    // steps it = 4 (pair) + c; the pencil of step it + 1 is loaded before step it is
+    // transformed (FP32)
+    constexpr bool kPrefetch = sizeof(T) == 4;
+    const int nsteps = 4 * te.npair;
+    auto grid_of = [&](int it) { return Z + ((size_t)te.pair0 * 4 + it) * ng + L.xy + (size_t)L.h * nxy; };
+    cpx<T> vn[M];
+    if (kPrefetch && nsteps > 0) load_pencil<T, M>(vn, grid_of(0), S * nxy);
+#pragma unroll 1
+    for (int it = 0; it < nsteps; ++it) {
+        const int pp = te.pair0 + it / 4, c = it % 4;
         const int o = pairs[pp].occ;
-        for (int c = 0; c < 4; ++c) {
+        {
             cpx<T> v[M];
-            zinv<T, NZ, S>(v, Z + ((size_t)pp * 4 + c) * ng, L, nxy);
+            if (kPrefetch) {
+                static_for<0, M>([&](auto mc) { v[decltype(mc)::value] = vn[decltype(mc)::value]; });
+                if (it + 1 < nsteps) load_pencil<T, M>(vn, grid_of(it + 1), S * nxy);
+            } else {
+                load_pencil<T, M>(v, grid_of(it), S * nxy);  // z = S m + h
+            }
+            PencilSplit<S>::template fft<T, NZ, 1>(v, L.h);  // -> z = ob M + k
             const cpx<T>* m = (c == 0 ? occR + (size_t)o * ng : Q4T + ((size_t)o * 3 + (c - 1)) * ng) + o0;
             static_for<0, M>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
