@@ -85,31 +85,32 @@ def main():
                     help="profiling run: one FP32 apply only (no FP64 leg, parity or CPU baseline)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 1 GiB device copy and a 1 GiB fill (counter calibration kernels)")
-    ap.add_argument("--tune-out", default=os.path.join(ROOT, "gpurun_out", "tuned_shapes.json"))
-    ap.add_argument("--c3", type=int, default=0, metavar="REPLAYS",
+    ap.add_argument("--tune-out", default=os.path.join(ROOT, "tuned_shapes.new.json"),
+                    help="where --tune writes its table (tc_amd/tuned_shapes.json is the one the library reads)")
+    ap.add_argument("--batched", type=int, default=0, metavar="REPLAYS",
                     help="also run the batched uneven workload (3 grids x 32 k, 8-48 bands per k, hf2x + tc2x) "
-                         "as captured HIP graphs for REPLAYS replays vs sequential per-k launches; report 'c3'")
-    ap.add_argument("--c4", action="store_true",
+                         "as captured HIP graphs for REPLAYS replays vs sequential per-k launches; report 'batched'")
+    ap.add_argument("--adjoint", action="store_true",
                     help="also time the gradients of E_x and of the BITC two-body energy on fcc 3x3x3 k, "
-                         "32 bands, 48^3 (FP32-split forward) with a central-difference check; report 'c4'")
-    ap.add_argument("--skip-c1", action="store_true",
-                    help="run only the requested extra legs (--c2/--c3/--c4), e.g. under a profiler")
+                         "32 bands, 48^3 (FP32-split forward) with a central-difference check; report 'adjoint'")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the requested extra legs (--bitc/--batched/--adjoint), e.g. under a profiler")
     ap.add_argument("--davidson", action="store_true",
                     help="also time tcx_davidson (2 blocks, 1 refresh, FP32-split pair terms) on the HF "
                          "4x4x4 k, 16-band, 40^3 cell against one calc_hamiltonian::all apply; report 'davidson'")
-    ap.add_argument("--c2-terms", default="tc2x,tc3a1,tc3b1,tc3a2a4b2b5,tc3a3b3b4b6",
-                    help="comma-separated terms of the --c2 leg")
-    ap.add_argument("--c2-precs", default="fp32,fp64", help="precisions of the --c2 leg (fp32, fp64)")
-    ap.add_argument("--c2-repeat", type=int, default=1,
-                    help="applies per term and precision in --c2; the last one is timed (2 = steady state)")
-    ap.add_argument("--c2", action="store_true",
+    ap.add_argument("--bitc-terms", default="tc2x,tc3a1,tc3b1,tc3a2a4b2b5,tc3a3b3b4b6",
+                    help="comma-separated terms of the --bitc leg")
+    ap.add_argument("--bitc-precs", default="fp32,fp64", help="precisions of the --bitc leg (fp32, fp64)")
+    ap.add_argument("--bitc-repeat", type=int, default=1,
+                    help="applies per term and precision in --bitc; the last one is timed (2 = steady state)")
+    ap.add_argument("--bitc", action="store_true",
                     help="also time the BITC workload (tc2x + three-body terms, fcc 4x4x4k 16b 40^3) in "
-                         "FP32-split and FP64 and report its energy errors under 'c2'")
+                         "FP32-split and FP64 and report its energy errors under 'bitc_terms'")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
+              "sharding over k across GPUs is a design, not part of this build.", file=sys.stderr)
         sys.exit(2)
 
     from tc_amd import autotune as AT
@@ -117,15 +118,15 @@ def main():
     from tc_amd import system as S
     from tc_amd import tcx
 
-    if args.skip_c1:
+    if args.skip_headline:
         eng = tcx.Engine(0)
         extra = {}
-        if args.c2:
-            extra["c2"] = bench_c2(eng, log, args.c2_terms.split(","), args.c2_precs.split(","), args.c2_repeat)
-        if args.c3:
-            extra["c3"] = bench_c3(log, args.c3)
-        if args.c4:
-            extra["c4"] = bench_c4(eng, log)
+        if args.bitc:
+            extra["bitc_terms"] = bench_bitc_terms(eng, log, args.bitc_terms.split(","), args.bitc_precs.split(","), args.bitc_repeat)
+        if args.batched:
+            extra["batched"] = bench_batched(log, args.batched)
+        if args.adjoint:
+            extra["adjoint"] = bench_adjoint(eng, log)
         if args.davidson:
             extra["davidson"] = bench_davidson(eng, log)
         print(json.dumps(extra), flush=True)
@@ -284,12 +285,12 @@ def main():
             "E_x_fp64_Ha": float(ex64.real), "dE_x_Ha": float(abs(ex32 - ex64)),
         },
     }
-    if args.c2:
-        out["c2"] = bench_c2(eng, log, args.c2_terms.split(","), args.c2_precs.split(","), args.c2_repeat)
-    if args.c3:
-        out["c3"] = bench_c3(log, args.c3)
-    if args.c4:
-        out["c4"] = bench_c4(eng, log)
+    if args.bitc:
+        out["bitc_terms"] = bench_bitc_terms(eng, log, args.bitc_terms.split(","), args.bitc_precs.split(","), args.bitc_repeat)
+    if args.batched:
+        out["batched"] = bench_batched(log, args.batched)
+    if args.adjoint:
+        out["adjoint"] = bench_adjoint(eng, log)
     if args.davidson:
         out["davidson"] = bench_davidson(eng, log)
     print(json.dumps(out), flush=True)
@@ -298,7 +299,7 @@ def main():
     eng.close()
 
 
-def bench_c4(eng, log, h=1e-3):
+def bench_adjoint(eng, log, h=1e-3):
     """Gradients on the 3x3x3 k, 32-band, 48^3 fcc cell (746,496 pairs per apply): dE_x/d conj(phi)
     (HF cell) and dE_2/d conj(chi), dE_2/d phi, dE_2/dA (BITC cell), FP32-split forward with FP64
     accumulation (timed); each checked by a central difference (step h) along one seeded direction
@@ -325,7 +326,7 @@ def bench_c4(eng, log, h=1e-3):
             t_grad = time.perf_counter() - t0
             an = np.sum(d * gr)  # along phi
             energy = lambda s: (eng.load_system(s), eng.grad_tc2(tcx.PREC_FP64)[0])[1]
-        log(f"bench c4: {method} gradient {t_grad:.2f} s")
+        log(f"bench adjoint: {method} gradient {t_grad:.2f} s")
         ep = energy(S.System(**{**sys_.__dict__, "phi_occ": sys_.phi_occ + h * d}))
         em = energy(S.System(**{**sys_.__dict__, "phi_occ": sys_.phi_occ - h * d}))
         fd = (ep - em) / (2 * h)
@@ -346,12 +347,12 @@ def bench_c4(eng, log, h=1e-3):
         gk, gu, sums = eng.kernel_gradient("x" if method == "HF" else "tc2", 256, 0.05, prec)
         entry["kernel_gradient_s"] = time.perf_counter() - t0
         entry["kernel_gradient_linear_sum"] = [float(sums[0]), float(sums[1])]
-        log(f"bench c4: {method} kernel gradient {entry['kernel_gradient_s']:.2f} s")
+        log(f"bench adjoint: {method} kernel gradient {entry['kernel_gradient_s']:.2f} s")
         res[method] = entry
     return res
 
 
-def bench_c3(log, replays):
+def bench_batched(log, replays):
     """Batched uneven workload: three fcc cells (32^3, 36^3, 48^3), 32 k each with 8-48 bands
     per k (all occupied), hf2x + tc2x in FP32-split.  Graph mode: one context per cell, each
     apply captured once into a HIP graph, the three graphs launched together on their own
@@ -376,7 +377,7 @@ def bench_c3(log, replays):
         eng._term_device("hf2x", nb, d_phi.ptr, d_out.ptr, tcx.PREC_FP32)
         eng._term_device("tc2x", nb, d_phi.ptr, d_out.ptr, tcx.PREC_FP32)
 
-    log("bench c3: eager warm-up of the batched applies")
+    log("bench batched: eager warm-up of the batched applies")
     ref = []
     for sys_, eng, nb, d_phi, d_out in ctxs:
         apply(eng, nb, d_phi, d_out)
@@ -395,7 +396,7 @@ def bench_c3(log, replays):
         for sys_, eng, nb, d_phi, d_out in ctxs:
             eng.synchronize()
         graph_t.append(time.perf_counter() - t0)
-        log(f"bench c3: graph replay {r + 1}/{replays} {graph_t[-1]:.3f} s")
+        log(f"bench batched: graph replay {r + 1}/{replays} {graph_t[-1]:.3f} s")
     same = all(np.array_equal(d_out.download(np.complex128), rf) for (_, _, _, _, d_out), rf in zip(ctxs, ref))
 
     def per_k(sys_, eng, nb, d_phi, d_out):
@@ -408,7 +409,7 @@ def bench_c3(log, replays):
                 # one k's bands: phi and out pointers advanced to its block
                 eng._term_device(term, one, d_phi.ptr + 16 * off, d_out.ptr + 16 * off, tcx.PREC_FP32)
 
-    log("bench c3: sequential per-k warm-up")
+    log("bench batched: sequential per-k warm-up")
     for c in ctxs:
         per_k(*c)
     seq_t = []
@@ -419,7 +420,7 @@ def bench_c3(log, replays):
         for c in ctxs:
             c[1].synchronize()
         seq_t.append(time.perf_counter() - t0)
-        log(f"bench c3: sequential per-k apply {r + 1} {seq_t[-1]:.3f} s")
+        log(f"bench batched: sequential per-k apply {r + 1} {seq_t[-1]:.3f} s")
     seq_same = all(np.allclose(d_out.download(np.complex128), rf, rtol=0, atol=1e-6 * np.abs(rf).max())
                    for (_, _, _, _, d_out), rf in zip(ctxs, ref))
     for (sys_, eng, nb, d_phi, d_out), gid in zip(ctxs, gids):
@@ -482,7 +483,7 @@ def bench_davidson(eng, log):
     }
 
 
-def bench_c2(eng, log, terms, precs, repeat=1):
+def bench_bitc_terms(eng, log, terms, precs, repeat=1):
     """BITC two- and three-body terms on the fcc 4x4x4 k, 16-band, 40^3 cell (A = 0.5, F from
     the cusp): FP32-split and FP64 applies per term on HBM-resident buffers (the last of `repeat`
     applies is timed), with the per-band error and the term energy (left orbitals) of FP32
@@ -513,7 +514,7 @@ def bench_c2(eng, log, terms, precs, repeat=1):
                 eng.synchronize()
                 secs[pname] = time.perf_counter() - t0
             outs[pname] = d_out.download(np.complex128)
-            log(f"bench c2: {term} {pname} {secs[pname]:.2f} s")
+            log(f"bench bitc: {term} {pname} {secs[pname]:.2f} s")
         entry = {f"{p}_s": secs[p] for p in precs}
         if "fp32" in secs and term not in ("tc3a1", "tc3b1"):
             entry["fp32_pairs_per_s"] = res["pairs_per_apply"] / secs["fp32"]

@@ -8,7 +8,7 @@
 //          HNphi[spin][k][band][spinor] (accumulated, +=), std::ostream*)
 // The state classes below are minimal, builder-owned stand-ins carrying only
 // what these terms read from the reference's classes; a maintainer wiring the
-// engine into TC++ maps the reference objects onto them (INTEGRATION.md).
+// engine into TC++ maps the reference objects onto them.
 // Eigen::VectorXcd becomes std::vector<std::complex<double>>.
 //
 // Error behaviour: the reference stops the run (error_messages::stop ->
