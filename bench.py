@@ -134,7 +134,7 @@ def main():
         return
 
     log("bench: building workload fcc 4x4x4k 16b 40^3")
-    sys_ = S.fcc_config1()
+    sys_ = S.fcc_hf_40()
     nb = sys_.nocc.copy()
     phi = sys_.phi_occ
     pairs_per_step = int(np.sum(nb) * np.sum(sys_.nocc))
@@ -310,7 +310,7 @@ def bench_adjoint(eng, log, h=1e-3):
     res = {}
     rng = np.random.default_rng(4)
     for method in ("HF", "BITC"):
-        sys_ = S.fcc_adjoint_config(method)
+        sys_ = S.fcc_adjoint_48(method)
         d = rng.standard_normal(sys_.phi_occ.size) + 1j * rng.standard_normal(sys_.phi_occ.size)
         d /= np.linalg.norm(d)
         eng.load_system(sys_)
@@ -449,7 +449,7 @@ def bench_davidson(eng, log):
     from tc_amd import system as S
     from tc_amd import tcx
 
-    sys_ = S.fcc_config1()
+    sys_ = S.fcc_hf_40()
     eng.load_system(sys_)
     nb, phi = sys_.nocc.copy(), sys_.phi_occ
     h1, h2, _ = eng.apply_all(nb, phi, precision=tcx.PREC_FP32)  # warm-up (plans, workspaces)
@@ -493,7 +493,7 @@ def bench_bitc_terms(eng, log, terms, precs, repeat=1):
     from tc_amd import tcx
     nbody_of = {"tc2x": 2, "tc3a1": 3, "tc3b1": 3, "tc3a2a4b2b5": 3, "tc3a3b3b4b6": 3}
     prec_of = {"fp32": tcx.PREC_FP32, "fp64": tcx.PREC_FP64}
-    sys_ = S.fcc_config2()
+    sys_ = S.fcc_bitc_40()
     nb = sys_.nocc.copy()
     phi = sys_.phi_occ
     eng.load_system(sys_)

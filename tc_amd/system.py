@@ -179,11 +179,11 @@ class System:
         return float(self.filling.sum()) / self.num_kpoints
 
 
-def build_system(lattice, n, kmesh, nbands, nocc, *, seed0=1234, config=0, method="HF",
+def build_system(lattice, n, kmesh, nbands, nocc, *, seed0=1234, seed_stream=0, method="HF",
                  A=None, F=None, div_corr=True, left_seed_offset=500, name="",
                  filling_value=2.0, gw=None) -> System:
     """Synthetic system: orbitals c(G) = complex Gaussian * exp(-|k+G|^2/2),
-    Gram-Schmidt orthonormalised per k; seed = seed0 + 1000*config + ik.
+    Gram-Schmidt orthonormalised per k; seed = seed0 + 1000*seed_stream + ik.
 
     ``nbands`` orbitals are generated per k and the first ``nocc`` of them are
     occupied with ``filling_value`` (spin factor 2 included).  For BITC, an
@@ -209,11 +209,11 @@ def build_system(lattice, n, kmesh, nbands, nocc, *, seed0=1234, config=0, metho
         sel = sel[order]
         npw.append(len(sel))
         gidx.append(sel.astype(np.int32))
-        rng = np.random.default_rng(seed0 + 1000 * config + ik)
+        rng = np.random.default_rng(seed0 + 1000 * seed_stream + ik)
         env = np.exp(-kg2[sel] / 2.0)
         c = (rng.standard_normal((nbands, len(sel))) + 1j * rng.standard_normal((nbands, len(sel)))) * env
         if method == "BITC":
-            rngl = np.random.default_rng(seed0 + 1000 * config + ik + left_seed_offset)
+            rngl = np.random.default_rng(seed0 + 1000 * seed_stream + ik + left_seed_offset)
             cl = (rngl.standard_normal((nbands, len(sel))) + 1j * rngl.standard_normal((nbands, len(sel)))) * env
             # left = right + perturbation keeps the pair well conditioned
             R, L = gram_schmidt_biortho(gram_schmidt(c), gram_schmidt(c) + 0.3 * gram_schmidt(cl))
@@ -243,32 +243,32 @@ def build_system(lattice, n, kmesh, nbands, nocc, *, seed0=1234, config=0, metho
 # Named workloads
 # ---------------------------------------------------------------------------
 
-def heg_sc_config0(**kw) -> System:
+def heg_sc_rs4_24(**kw) -> System:
     """HEG, simple cubic, rs=4 bohr, 14 electrons/cell, 2x2x2 k, 8 bands (7 occupied), 24^3."""
     vol = 14 * 4.0 / 3.0 * math.pi * 4.0 ** 3
     a = vol ** (1.0 / 3.0)
-    return build_system(sc_lattice(a), (24, 24, 24), (2, 2, 2), nbands=8, nocc=7, config=0,
+    return build_system(sc_lattice(a), (24, 24, 24), (2, 2, 2), nbands=8, nocc=7, seed_stream=0,
                         name="hf2x heg-sc rs4 2x2x2k 8b(7occ) 24^3", **kw)
 
 
-def fcc_config1(**kw) -> System:
+def fcc_hf_40(**kw) -> System:
     """fcc a=10.26 bohr, 4x4x4 k, 16 bands all occupied (f=2), 40^3."""
-    return build_system(fcc_lattice(10.26), (40, 40, 40), (4, 4, 4), nbands=16, nocc=16, config=1,
+    return build_system(fcc_lattice(10.26), (40, 40, 40), (4, 4, 4), nbands=16, nocc=16, seed_stream=1,
                         name="hf2x fcc a10.26 4x4x4k 16b 40^3", **kw)
 
 
-def fcc_config2(**kw) -> System:
-    """Config 1 cell with BITC, RPA Jastrow A=0.5 (F=1 from the cusp), independent left set."""
+def fcc_bitc_40(**kw) -> System:
+    """The fcc a=10.26, 4x4x4 k, 16-band, 40^3 cell with BITC, RPA Jastrow A=0.5 (F=1 from the cusp), independent left set."""
     A = np.full((2, 2), 0.5)
-    return build_system(fcc_lattice(10.26), (40, 40, 40), (4, 4, 4), nbands=16, nocc=16, config=2,
+    return build_system(fcc_lattice(10.26), (40, 40, 40), (4, 4, 4), nbands=16, nocc=16, seed_stream=2,
                         method="BITC", A=A, name="bitc fcc a10.26 4x4x4k 16b 40^3 A0.5", **kw)
 
 
-def fcc_adjoint_config(method="BITC", **kw) -> System:
+def fcc_adjoint_48(method="BITC", **kw) -> System:
     """fcc a=10.26, 3x3x3 k (27 k), 32 bands/k all occupied, 48^3; BITC with A = 0.5 by
     default (the cell of the gradient workload)."""
     A = np.full((2, 2), 0.5) if method != "HF" else None
-    return build_system(fcc_lattice(10.26), (48, 48, 48), (3, 3, 3), nbands=32, nocc=32, config=4,
+    return build_system(fcc_lattice(10.26), (48, 48, 48), (3, 3, 3), nbands=32, nocc=32, seed_stream=4,
                         method=method, A=A, name=f"{method.lower()} fcc a10.26 3x3x3k 32b 48^3", **kw)
 
 
@@ -297,20 +297,20 @@ def batched_uneven_systems(seed=3):
     rng = np.random.default_rng(1234 + 1000 * seed)
     out = []
     for g, grid in enumerate((32, 36, 48)):
-        base = build_system(fcc_lattice(10.26), (grid,) * 3, (4, 4, 2), nbands=48, nocc=48, config=3 + 10 * g,
+        base = build_system(fcc_lattice(10.26), (grid,) * 3, (4, 4, 2), nbands=48, nocc=48, seed_stream=3 + 10 * g,
                             method="TC", A=np.full((2, 2), 0.5), name=f"tc fcc a10.26 4x4x2k {grid}^3 uneven")
         out.append(uneven_bands(base, rng.integers(8, 49, size=base.nk)))
     return out
 
 
 def tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, lattice=None,
-                A=None, config=90, **kw) -> System:
+                A=None, seed_stream=90, **kw) -> System:
     """Small systems for golden vectors and brute-force checks."""
     if lattice is None:
         lattice = fcc_lattice(6.0)
     if method in ("TC", "BITC") and A is None:
         A = np.full((2, 2), 0.5)
-    return build_system(lattice, n, kmesh, nbands=nbands, nocc=nocc, config=config, method=method,
+    return build_system(lattice, n, kmesh, nbands=nbands, nocc=nocc, seed_stream=seed_stream, method=method,
                         A=A, name=f"tiny {method} {n}", **kw)
 
 

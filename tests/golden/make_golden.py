@@ -70,7 +70,7 @@ CASES = {
 def make_case(name):
     method, grid, kmesh, nbands, nocc, lattice, nb, seed = CASES[name]
     sys_ = S.tiny_system(method=method, n=grid, kmesh=kmesh, nbands=nbands, nocc=nocc, lattice=lattice,
-                         config=seed)
+                         seed_stream=seed)
     nb, phi = S.applied_bands(sys_, nb=nb, seed=seed)
     alpha, sv = _sum_of_vaux(sys_)
     out = dict(

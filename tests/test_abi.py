@@ -34,7 +34,7 @@ def test_library_is_gfx950_only():
 
 
 def test_sum_of_vaux_host_utility_matches_oracle():
-    for sys_ in (S.tiny_system(), S.heg_sc_config0()):
+    for sys_ in (S.tiny_system(), S.heg_sc_rs4_24()):
         a1, s1 = tcx.sum_of_vaux(sys_.recip, sys_.volume, sys_.kfrac)
         a2, s2 = O.sum_of_vaux(sys_.recip, sys_.volume, sys_.kfrac)
         assert abs(a1 - a2) < 1e-15 * a2

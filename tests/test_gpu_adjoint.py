@@ -25,7 +25,7 @@ def _with(sys_, **kw):
 
 
 def test_exchange_gradient_finite_differences(engine):
-    sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=91)
+    sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=91)
     engine.load_system(sys_)
     e0, g = engine.grad_exchange()
     assert abs(e0.imag) < 1e-14 * abs(e0.real) and e0.real < 0
@@ -42,7 +42,7 @@ def test_exchange_gradient_finite_differences(engine):
 
 @pytest.mark.parametrize("method", ["TC", "BITC"])
 def test_tc2_gradients_finite_differences(engine, method):
-    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=92)
+    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=92)
     engine.load_system(sys_)
     e0, gl, gr, dA = engine.grad_tc2()
     chi = sys_.chi_occ if method == "BITC" else sys_.phi_occ
@@ -75,7 +75,7 @@ def test_exchange_energy_autograd_function(engine):
     """torch.autograd.Function around the C-ABI energy and gradient, checked by gradcheck."""
     import torch
 
-    sys_ = S.tiny_system(method="HF", n=(6, 6, 6), kmesh=(2, 1, 1), nbands=2, nocc=1, config=93,
+    sys_ = S.tiny_system(method="HF", n=(6, 6, 6), kmesh=(2, 1, 1), nbands=2, nocc=1, seed_stream=93,
                          lattice=S.sc_lattice(5.0))
 
     class ExchangeEnergy(torch.autograd.Function):
@@ -123,7 +123,7 @@ def _real_space(sys_, coef):
 
 
 def test_exchange_kernel_gradient_bins_and_euler(engine):
-    sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=94)
+    sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=94)
     engine.load_system(sys_, div_corr=False)
     e, _ = engine.grad_exchange()
     nbins, width = 48, 0.3
@@ -152,7 +152,7 @@ def test_exchange_kernel_gradient_bins_and_euler(engine):
 @pytest.mark.parametrize("method", ["TC", "BITC"])
 @pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
 def test_tc2_kernel_gradient_euler_and_dA(engine, method, prec):
-    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=95)
+    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=95)
     chi = sys_.chi_occ if method == "BITC" else sys_.phi_occ
 
     def e2x(s):

@@ -62,8 +62,8 @@ def trial_bands(sys_, nb, seed):
     return out
 
 
-def system(method, config):
-    return S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=config)
+def system(method, stream):
+    return S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=stream)
 
 
 @pytest.mark.parametrize("method", ["HF", "TC", "BITC"])

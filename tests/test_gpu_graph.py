@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _setup(engine):
-    sys_ = S.tiny_system(method="TC", n=(12, 12, 12), kmesh=(2, 2, 1), nbands=4, nocc=3, config=71)
+    sys_ = S.tiny_system(method="TC", n=(12, 12, 12), kmesh=(2, 2, 1), nbands=4, nocc=3, seed_stream=71)
     engine.load_system(sys_)
     nb, phi = S.applied_bands(sys_, nb=[4, 2, 3, 1], seed=72)
     return sys_, nb, phi

@@ -87,13 +87,13 @@ def test_hf2x_empty_and_ragged_bands(engine):
 
 
 @pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
-def test_hf2x_config0_heg_sc(engine, prec):
+def test_hf2x_heg_sc_rs4_24(engine, prec):
     """HEG sc rs=4, 2x2x2 k, 8 bands (7 occupied), 24^3: Vx*psi per band and E_x."""
-    sys_ = S.heg_sc_config0()
+    sys_ = S.heg_sc_rs4_24()
     engine.load_system(sys_)
     nb = np.full(sys_.nk, 8, np.int32)
     # apply to all 8 bands: the 7 occupied ones plus one more orthonormal band
-    full = S.build_system(sys_.lattice, sys_.n, (2, 2, 2), nbands=8, nocc=8, config=0)
+    full = S.build_system(sys_.lattice, sys_.n, (2, 2, 2), nbands=8, nocc=8, seed_stream=0)
     phi = full.phi_occ
     ref = O.Oracle(sys_).hf2x(nb, phi)
     got = engine.hf2x(nb, phi, precision=prec)

@@ -38,7 +38,7 @@ def test_tc_terms_golden(engine, golden, name, prec):
 
 @pytest.mark.parametrize("method", ["TC", "BITC"])
 def test_tc2x_against_oracle_12(engine, method):
-    sys_ = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 2, 1), nbands=4, nocc=3, config=31)
+    sys_ = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 2, 1), nbands=4, nocc=3, seed_stream=31)
     engine.load_system(sys_)
     nb, phi = S.applied_bands(sys_, nb=[4, 3, 2, 4], seed=32)
     o = O.Oracle(sys_)
@@ -63,7 +63,7 @@ def test_tc2x_without_jastrow_equals_hf2x(engine):
 
 def test_tc_energies_fp32_vs_oracle(engine):
     """Two- and three-body energies of the occupied bands, FP32-split vs FP64 oracle."""
-    sys_ = S.tiny_system(method="BITC", n=(12, 12, 12), kmesh=(2, 2, 1), nbands=3, nocc=3, config=34)
+    sys_ = S.tiny_system(method="BITC", n=(12, 12, 12), kmesh=(2, 2, 1), nbands=3, nocc=3, seed_stream=34)
     engine.load_system(sys_)
     o = O.Oracle(sys_)
     for term, nbody in (("tc2x", 2), ("tc3a1", 3), ("tc3b1", 3)):

@@ -57,7 +57,7 @@ def test_assembly_terms_golden(engine, golden, name):
 
 @pytest.mark.parametrize("method", ["TC", "BITC"])
 def test_three_body_exchange_against_oracle_12(engine, method):
-    sys_ = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 2, 1), nbands=4, nocc=3, config=41)
+    sys_ = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 2, 1), nbands=4, nocc=3, seed_stream=41)
     engine.load_system(sys_)
     nb, phi = S.applied_bands(sys_, nb=[3, 2, 1, 3], seed=42)
     o = O.Oracle(sys_)
@@ -72,7 +72,7 @@ def test_three_body_exchange_against_oracle_12(engine, method):
 
 
 def test_three_body_exchange_without_divergence_correction(engine):
-    sys_ = S.tiny_system(method="BITC", n=(12, 12, 12), kmesh=(2, 1, 1), nbands=3, nocc=2, config=43)
+    sys_ = S.tiny_system(method="BITC", n=(12, 12, 12), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=43)
     engine.load_system(sys_, div_corr=False)
     nb, phi = S.applied_bands(sys_, nb=[3, 2], seed=44)
     o = O.Oracle(sys_, div_corr=False)
@@ -90,7 +90,7 @@ def test_three_body_exchange_zero_without_jastrow(engine):
 
 def test_three_body_energy_fp32_vs_oracle(engine):
     """E_3 of the occupied bands from the two exchange terms, FP32-split vs the FP64 oracle."""
-    sys_ = S.tiny_system(method="BITC", n=(12, 12, 12), kmesh=(2, 2, 1), nbands=3, nocc=3, config=46)
+    sys_ = S.tiny_system(method="BITC", n=(12, 12, 12), kmesh=(2, 2, 1), nbands=3, nocc=3, seed_stream=46)
     engine.load_system(sys_)
     o = O.Oracle(sys_)
     for term in THREE_BODY_X:
@@ -105,7 +105,7 @@ def test_three_body_energy_fp32_vs_oracle(engine):
 @pytest.mark.parametrize("method", ["HF", "TC", "BITC"])
 def test_apply_all_matches_sum_of_terms(engine, method):
     """calc_hamiltonian::all: H1 = kinetic, H2 = Hartree + exchange, H3 = the four three-body terms."""
-    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=47)
+    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=47)
     engine.load_system(sys_)
     nb, phi = S.applied_bands(sys_, nb=[3, 2], seed=48)
     o = O.Oracle(sys_)
@@ -121,7 +121,7 @@ def test_apply_all_matches_sum_of_terms(engine, method):
 
 
 def test_three_body_exchange_device_pointers(engine):
-    sys_ = S.tiny_system(method="TC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=49)
+    sys_ = S.tiny_system(method="TC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=49)
     engine.load_system(sys_)
     nb, phi = S.applied_bands(sys_, nb=[3, 1], seed=50)
     ref = engine.tc3a2a4b2b5(nb, phi)
@@ -137,7 +137,7 @@ def test_pair_lists_follow_state_changes(engine):
     """Two systems with the same applied-band counts but different occupied sets
     in a row: the cached three-body pair lists must be rebuilt for the second."""
     for nocc, cfg in ((2, 51), (3, 52)):
-        sys_ = S.tiny_system(method="TC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=nocc, config=cfg)
+        sys_ = S.tiny_system(method="TC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=nocc, seed_stream=cfg)
         engine.load_system(sys_)
         nb, phi = S.applied_bands(sys_, nb=[2, 2], seed=53)
         o = O.Oracle(sys_)

@@ -65,7 +65,7 @@ def plane_wave_system(method="HF", n=(8, 8, 8), kmesh=(2, 2, 1), nocc=3, lattice
     """Occupied orbitals are single plane waves (the lowest |q+G|), so the
     exchange operator is diagonal with a closed form and the density is uniform."""
     sys_ = S.tiny_system(method=method, n=n, kmesh=kmesh, nbands=nocc, nocc=nocc,
-                         lattice=S.sc_lattice(6.0) if lattice is None else lattice, A=A, config=77)
+                         lattice=S.sc_lattice(6.0) if lattice is None else lattice, A=A, seed_stream=77)
     po = sys_.pw_offsets()
     blocks = []
     for ik in range(sys_.nk):
@@ -260,7 +260,7 @@ def test_jastrow_kernels():
 def test_two_body_adjoint_is_minus_u_with_swapped_orbitals(method):
     """<a|H[u; phi, chi] b> = conj <b|H[-u; chi, phi] a> for tc2x + tc2h
     (src/diagonalization_davidson.cpp:234-251: take_Hermitian_conj + switch_left_right)."""
-    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, config=81)
+    sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=81)
     adj = S.System(**{**sys_.__dict__, "A": -sys_.A})
     if method == "BITC":
         adj.phi_occ, adj.chi_occ = sys_.chi_occ.copy(), sys_.phi_occ.copy()
