@@ -98,14 +98,41 @@ void push_state(tcx_context* ctx, const Method& method, const CrystalStructure& 
           "set_div_correction");
 }
 
+// The k-mesh symmetry relation for tc3a2a4b2b5 (identity when Kpoints carries none).
+void push_symmetry(tcx_context* ctx, const Symmetry& sym, const Kpoints& kp)
+{
+    if (kp.irreducible_of.empty()) {
+        check(ctx, tcx_set_kpoint_symmetry(ctx, nullptr, nullptr, nullptr, nullptr), "set_kpoint_symmetry");
+        return;
+    }
+    const size_t nk = kp.kvectors_scf.size();
+    if (kp.irreducible_of.size() != nk || kp.index_of_rotation.size() != nk || kp.time_reversal_used.size() != nk)
+        throw std::runtime_error("calc_hamiltonian: Kpoints symmetry relation does not cover the mesh");
+    std::vector<int> rot(9 * nk);
+    std::vector<double> trans(3 * nk);
+    for (size_t k = 0; k < nk; ++k) {
+        const int s = kp.index_of_rotation[k];
+        if (s < 0 || s >= (int)sym.rotation.size() || s >= (int)sym.translation.size())
+            throw std::runtime_error("calc_hamiltonian: rotation index outside Symmetry");
+        for (int i = 0; i < 9; ++i) rot[9 * k + i] = sym.rotation[s][i];
+        for (int d = 0; d < 3; ++d) trans[3 * k + d] = sym.translation[s][d];
+    }
+    check(ctx,
+          tcx_set_kpoint_symmetry(ctx, kp.irreducible_of.data(), rot.data(), trans.data(),
+                                  kp.time_reversal_used.data()),
+          "set_kpoint_symmetry");
+}
+
 using TermFn = int (*)(tcx_context*, const int*, const double*, double*, int, unsigned);
 
 void run_term(TermFn fn, const char* name, const Parallelization& par, const Method& method,
               const CrystalStructure& cs, const Potentials& pot, const Spin& spin, const Kpoints& kp,
-              const PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi, Phi4& HNphi)
+              const PlaneWaveBasis& pw, const BlochStates& bs, const Phi4& phi, Phi4& HNphi,
+              const Symmetry* sym = nullptr)
 {
     tcx_context* ctx = context_for(par.device);
     push_state(ctx, method, cs, pot, spin, kp, pw, bs);
+    if (sym) push_symmetry(ctx, *sym, kp);
     const int nk = (int)kp.kvectors_scf.size();
     std::vector<int> nb(nk);
     for (int k = 0; k < nk; ++k) nb[k] = (int)phi[0][k].size();
@@ -146,12 +173,13 @@ void unpack_bands(const std::vector<double>& buf, Phi4& H)
 
 namespace calc_hamiltonian {
 
-void all(const Parallelization& par, MyClock&, const Method& m, const CrystalStructure& cs, const Symmetry&,
+void all(const Parallelization& par, MyClock&, const Method& m, const CrystalStructure& cs, const Symmetry& sym,
          const Potentials& pot, const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw, const BlochStates& bs,
          const Phi4& phi, Phi4& H1phi, Phi4& H2phi, Phi4& H3phi, std::ostream*)
 {
     tcx_context* ctx = context_for(par.device);
     push_state(ctx, m, cs, pot, spin, kp, pw, bs);
+    push_symmetry(ctx, sym, kp);
     const int nk = (int)kp.kvectors_scf.size();
     std::vector<int> nb(nk);
     for (int k = 0; k < nk; ++k) nb[k] = (int)phi[0][k].size();
@@ -215,11 +243,11 @@ void tc3a3b3b4b6(const Parallelization& par, const Method& m, const CrystalStruc
     run_term(tcx_tc3a3b3b4b6, "tc3a3b3b4b6", par, m, cs, pot, spin, kp, pw, bs, phi, H3phi);
 }
 
-void tc3a2a4b2b5(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Symmetry&,
+void tc3a2a4b2b5(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Symmetry& sym,
                  const Potentials& pot, const Spin& spin, const Kpoints& kp, PlaneWaveBasis& pw,
                  const BlochStates& bs, const Phi4& phi, Phi4& H3phi, std::ostream*)
 {
-    run_term(tcx_tc3a2a4b2b5, "tc3a2a4b2b5", par, m, cs, pot, spin, kp, pw, bs, phi, H3phi);
+    run_term(tcx_tc3a2a4b2b5, "tc3a2a4b2b5", par, m, cs, pot, spin, kp, pw, bs, phi, H3phi, &sym);
 }
 
 void hf2x(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,

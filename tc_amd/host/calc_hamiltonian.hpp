@@ -13,10 +13,12 @@
 //
 // Error behaviour: the reference stops the run (error_messages::stop ->
 // MPI_Abort, src/error_messages.cpp:60-64); here a failing call throws
-// std::runtime_error carrying tcx_last_error().  Scope: identity symmetry,
-// SCF mode, num_independent_spins == 1, no spinors.
+// std::runtime_error carrying tcx_last_error().  Scope: SCF mode,
+// num_independent_spins == 1, no spinors; the k-mesh is passed flattened (every
+// symmetry replica a k of its own), with its symmetry relation for tc3a2a4b2b5.
 #pragma once
 
+#include <array>
 #include <complex>
 #include <memory>
 #include <ostream>
@@ -50,8 +52,13 @@ struct Spin {
 };
 
 struct Kpoints {
-    std::vector<std::vector<double>> kvectors_scf;  // [k][3] crystal coordinates (isym = 0 only)
+    std::vector<std::vector<double>> kvectors_scf;  // [k][3] crystal coordinates, every k of the mesh
     int num_kpoints = 0;                             // Nk of the 1/(Nk V) prefactors
+    // Symmetry relation of the flattened mesh (Kpoints::index_of_rotation_at_k,
+    // is_time_reversal_used_at_k; src/kpoints.cpp:34-110); empty: every k irreducible.
+    std::vector<int> irreducible_of;      // [k] mesh index of the irreducible k whose orbitals k's are
+    std::vector<int> index_of_rotation;   // [k] index into Symmetry::rotation
+    std::vector<int> time_reversal_used;  // [k] 0 or 1
 };
 
 struct PlaneWaveBasis {
@@ -78,7 +85,13 @@ struct BlochStates {
     std::vector<std::vector<int>> num_occupied_bands;        // [spin][k]
 };
 
-struct Symmetry {};  // identity only: the one rotation of an identity-symmetry k-mesh
+// Symmetry::rotation / translation (src/symmetry.cpp:17-33): TC++ rotations U
+// (integer, crystal, row-major) and fractional translations; read by tc3a2a4b2b5
+// (and by all) together with the Kpoints relation.
+struct Symmetry {
+    std::vector<std::array<int, 9>> rotation;
+    std::vector<std::array<double, 3>> translation;
+};
 struct MyClock {};   // timing printouts of the reference are not mirrored
 
 namespace calc_hamiltonian {

@@ -238,3 +238,35 @@ def test_cpp_host_scf_matches_reference(ex, prec, tmp_path):
             done = (int(t[1]), bool(int(t[2])))
     assert done is not None
     _check_run(qe, ex, prec, "c++", fermi, eigs, energies, ddiff, done[0], done[1])
+
+
+def test_cpp_host_mirror_tc3a2a4b2b5_with_symmetry(engine, tmp_path):
+    """calc_hamiltonian::tc3a2a4b2b5 of the C++ host mirror (Symmetry + Kpoints relation
+    pushed through tcx_set_kpoint_symmetry) equals the Python path on example15's expanded
+    mesh, with the symmetry and without it; and the two differ, since example15's
+    occupied set is not symmetric, so the mirror does use the relation."""
+    import subprocess
+    binp = os.path.join(os.path.dirname(HERE), "tests", "_bin", "scf_host_test")
+    if not os.path.exists(binp):
+        pytest.fail(f"{binp} is not built (run __graft_entry__.build())")
+    qe = Q.load_fixture(os.path.join(HERE, "golden", "qe_example15.npz"))
+    path, out = str(tmp_path / "in.tcxqe"), str(tmp_path / "h3.bin")
+    Q.write_flat(qe, path)
+    res = subprocess.run([binp, path, "mirror", out], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+    got = np.fromfile(out, dtype=np.complex128)
+    ref = []
+    for symmetric in (True, False):
+        _, sys_, _ = _tc_system_from(15, symmetric=symmetric)
+        engine.load_system(sys_)
+        ref.append(engine.tc3a2a4b2b5(sys_.nocc, sys_.phi_occ, precision=tcx.PREC_FP64))
+    assert got.size == ref[0].size + ref[1].size
+    g_sym, g_nosym = got[:ref[0].size], got[ref[0].size:]
+    scale = np.abs(ref[0]).max()
+    d_sym = np.abs(g_sym - ref[0]).max() / scale
+    d_nosym = np.abs(g_nosym - ref[1]).max() / scale
+    gap = np.abs(ref[0] - ref[1]).max() / scale
+    print(f"mirror vs python: with symmetry {d_sym:.2e}, without {d_nosym:.2e}; symmetric vs direct sums {gap:.2e}",
+          flush=True)
+    assert d_sym < 1e-12 and d_nosym < 1e-12
+    assert gap > 1e-6
