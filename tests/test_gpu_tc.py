@@ -81,3 +81,34 @@ def test_tc_requires_jastrow_and_method(engine):
     with pytest.raises(tcx.TcxError) as ei:
         engine.tc2x(nb, phi)
     assert ei.value.code == tcx.TCX_ERR_STATE
+
+
+def test_density_get_set(engine):
+    """tcx_get_density / tcx_set_density (BlochStates::density_): the density formed from
+    the occupied set integrates to the electron count; the terms read a replaced density
+    (the Hartree term hf2h is linear in it, tc3a1 is not: both follow the replacement);
+    with one independent spin the second grid is ignored and mirrors the first;
+    tcx_set_states forms the density from the orbitals again."""
+    sys_ = S.tiny_system(method="TC", n=(12, 12, 12), kmesh=(2, 2, 1), nbands=4, nocc=3, seed_stream=35)
+    nb, phi = S.applied_bands(sys_, nb=[2, 3, 1, 2], seed=7)
+    engine.load_system(sys_)
+    rho = engine.get_density()
+    N = sys_.ngrid
+    assert rho.shape == (2, N)
+    assert abs(rho[0].sum().real / (N * sys_.num_kpoints) - sys_.num_electrons() / 2) < 1e-10
+    assert np.array_equal(rho[0], rho[1])
+    h1 = engine.hf2h(nb, phi)
+    t1 = engine.tc3a1(nb, phi)
+    two = np.stack([2.0 * rho[0], 5.0 * rho[0]])  # second spin grid differs: ignored
+    engine.set_density(two)
+    back = engine.get_density()
+    assert np.array_equal(back[0], 2.0 * rho[0]) and np.array_equal(back[1], 2.0 * rho[0])
+    h2 = engine.hf2h(nb, phi)
+    t2 = engine.tc3a1(nb, phi)
+    assert np.abs(h2 - 2.0 * h1).max() <= 1e-12 * np.abs(h1).max()
+    assert np.abs(t2 - t1).max() > 1e-3 * np.abs(t1).max()  # tc3a1 reads the replaced density too
+    engine.set_states(sys_.nocc, sys_.filling, sys_.phi_occ, sys_.chi_occ)
+    assert np.array_equal(engine.get_density(), rho)
+    assert np.abs(engine.hf2h(nb, phi) - h1).max() <= 1e-13 * np.abs(h1).max()
+    with pytest.raises(tcx.TcxError):
+        engine.set_density(np.full((2, N), np.nan, np.complex128))
