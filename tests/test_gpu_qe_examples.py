@@ -270,3 +270,48 @@ def test_cpp_host_mirror_tc3a2a4b2b5_with_symmetry(engine, tmp_path):
           flush=True)
     assert d_sym < 1e-12 and d_nosym < 1e-12
     assert gap > 1e-6
+
+
+def _run_cpp(qe, prec, tmp_path):
+    import subprocess
+    binp = os.path.join(os.path.dirname(HERE), "tests", "_bin", "scf_host_test")
+    if not os.path.exists(binp):
+        pytest.fail(f"{binp} is not built (run __graft_entry__.build())")
+    path = str(tmp_path / "in.tcxqe")
+    Q.write_flat(qe, path)
+    res = subprocess.run([binp, path, str(prec)], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
+    energies, ddiff, done = [], [], None
+    for ln in res.stdout.splitlines():
+        t = ln.split()
+        if t[0] == "ENERGY":
+            v = [float(x) for x in t[1:]]
+            energies.append(np.array([complex(v[2 * i], v[2 * i + 1]) for i in range(4)]))
+        elif t[0] == "DDIFF":
+            ddiff.append(float(t[1]))
+        elif t[0] == "DONE":
+            done = (int(t[1]), bool(int(t[2])))
+    return energies, ddiff, done
+
+
+def test_scf_drivers_converge_alike(engine, tmp_path):
+    """The convergence branch of Diagonalization::scf (diagonalization.cpp:186-227: energy and
+    charge converged in two consecutive iterations ends the loop without the final
+    set_filling), which none of the reference's examples reach: example15 with loose
+    tolerances converges, and the C++ host and the Python harness take the same
+    trajectory and stop at the same iteration."""
+    qe = Q.load_fixture(os.path.join(HERE, "golden", "qe_example15.npz"))
+    qe.setup.update({"max_num_iterations": 12, "energy_tolerance": 5e-3, "charge_tolerance": 0.8})
+    r = scf.run(engine, qe, precision=tcx.PREC_FP64)
+    energies, ddiff, done = _run_cpp(qe, tcx.PREC_FP64, tmp_path)
+    print(f"python: {r.iterations} iterations, converged {r.converged}, dE "
+          f"{[f'{abs(b[3] - a[3]):.1e}' for a, b in zip(r.energies, r.energies[1:])]}, ddiff "
+          f"{[f'{x:.3f}' for x in r.density_difference]}; c++: {done}", flush=True)
+    assert r.converged and done == (r.iterations, True)
+    assert r.iterations < 12
+    # converged in iteration it: energies of iterations 1..it-1, no final one
+    assert len(r.energies) == r.iterations and len(energies) == len(r.energies)
+    assert len(ddiff) == len(r.density_difference) == r.iterations
+    assert np.abs(np.array(energies) - np.array(r.energies)).max() < 1e-9
+    assert np.abs(np.array(ddiff) - np.array(r.density_difference)).max() < 1e-9
+    assert abs(r.energies[-1][3] - r.energies[-2][3]) < 5e-3 and r.density_difference[-1] < 0.8
