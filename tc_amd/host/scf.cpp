@@ -188,6 +188,8 @@ SCFResult Diagonalization::scf(tcx_context* ctx, const QEInputs& qe) const
         for (int d = 0; d < 3; ++d) trans[3 * k + d] = qe.trans[reps[k].isym][d];
         trev[k] = reps[k].trev ? 1 : 0;
     }
+    check(ctx, tcx_set_kpoint_symmetry(ctx, k_src.data(), rot.data(), trans.data(), trev.data()),
+          "set_kpoint_symmetry");  // kept across tcx_set_states
     const long long N = (long long)qe.grid[0] * qe.grid[1] * qe.grid[2];
 
     // ---- state of the loop (BlochStates at the irreducible k)
@@ -275,8 +277,6 @@ SCFResult Diagonalization::scf(tcx_context* ctx, const QEInputs& qe) const
               tcx_set_states(ctx, nocc_k.data(), f_cat.data(), (const double*)occ_cat.data(),
                              bitc ? (const double*)left_cat.data() : nullptr),
               "set_states");
-        check(ctx, tcx_set_kpoint_symmetry(ctx, k_src.data(), rot.data(), trans.data(), trev.data()),
-              "set_kpoint_symmetry");
         std::vector<Complex> rnew((size_t)2 * N);
         check(ctx, tcx_get_density(ctx, (double*)rnew.data()), "get_density");
         if (first_iter) {
