@@ -48,14 +48,16 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_A(const cpx<T>* __restric
 
 // Coulomb kernel table of a launch: v(G)/N = 1/(N |k-q+G|^2), 0 below 1e-8
 // (hf2x.cpp:91-104; the 1/N is FFT_forward's), one row per k-q of the launch,
-// evaluated in FP64 and rounded once to the working precision.
+// evaluated in FP64 and rounded once to the working precision.  Within a plane
+// the table is stored y-fastest, e = i2 + n2 (i1 + n1 i3): pass B holds one
+// y line per lane when it multiplies, so its table loads are coalesced.
 template <typename T>
 __global__ __launch_bounds__(256) void k_coulomb_table(T* __restrict__ table, const double* __restrict__ kqcart,
                                                        int row0, Geom g, long long N, double invN)
 {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= N) return;
-    const int i1 = (int)(e % g.n1), i2 = (int)((e / g.n1) % g.n2), i3 = (int)(e / ((long long)g.n1 * g.n2));
+    const int i2 = (int)(e % g.n2), i1 = (int)((e / g.n2) % g.n1), i3 = (int)(e / ((long long)g.n1 * g.n2));
     const int m1 = centered(i1, g.n1), m2 = centered(i2, g.n2), m3 = centered(i3, g.n3);
     const double* kq = kqcart + 3 * (row0 + blockIdx.y);
     double g2 = 0.0;
@@ -64,6 +66,61 @@ __global__ __launch_bounds__(256) void k_coulomb_table(T* __restrict__ table, co
         g2 += c * c;
     }
     table[(long long)blockIdx.y * N + e] = static_cast<T>(g2 < 1e-8 ? 0.0 : invN / g2);
+}
+
+// LDS tile of one component (real or imaginary part) of a plane, [y][x] with one
+// element of padding per row.  The FP64 pass B transposes the two parts one after
+// the other: that halves its LDS per plane and doubles its waves per SIMD (1 -> 2 at
+// 40^3).  FP32 keeps whole complex tiles, which take half the LDS instructions and
+// a third of the barriers; its occupancy is set by the LDS either way.
+template <typename T, int NP>
+struct PartTile {
+    T a[NP][NP + 1];
+};
+template <int C, typename T>
+__device__ __forceinline__ T& part(cpx<T>& z)
+{
+    if constexpr (C == 0) return z.re; else return z.im;
+}
+
+// Transposes of the PL planes of a workgroup: thread t (active) holds column x=t
+// (values along y) -> row y=t (values along x), and back.  Every thread of the
+// workgroup calls them (they contain the barriers).
+template <typename T, int NP>
+__device__ __forceinline__ void cols_to_rows(PlaneTile<T, NP>& s, cpx<T> (&v)[NP], int t, bool active)
+{
+    if (active) tile_put_col<T, NP>(s, v, t);
+    __syncthreads();
+    if (active) tile_get_row<T, NP>(s, v, t);
+}
+template <typename T, int NP>
+__device__ __forceinline__ void rows_to_cols(PlaneTile<T, NP>& s, cpx<T> (&v)[NP], int t, bool active)
+{
+    if (active) tile_put_row<T, NP>(s, v, t);  // row t was read by thread t alone
+    __syncthreads();
+    if (active) tile_get_col<T, NP>(s, v, t);
+}
+template <typename T, int NP>
+__device__ __forceinline__ void cols_to_rows(PartTile<T, NP>& s, cpx<T> (&v)[NP], int t, bool active)
+{
+    static_for<0, 2>([&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        if constexpr (C == 1) __syncthreads();  // every thread has read the real parts
+        if (active) static_for<0, NP>([&](auto yc) { s.a[decltype(yc)::value][t] = part<C>(v[decltype(yc)::value]); });
+        __syncthreads();
+        if (active) static_for<0, NP>([&](auto xc) { part<C>(v[decltype(xc)::value]) = s.a[t][decltype(xc)::value]; });
+    });
+}
+template <typename T, int NP>
+__device__ __forceinline__ void rows_to_cols(PartTile<T, NP>& s, cpx<T> (&v)[NP], int t, bool active)
+{
+    static_for<0, 2>([&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        if constexpr (C == 1) __syncthreads();
+        if (active) static_for<0, NP>([&](auto xc) { s.a[t][decltype(xc)::value] = part<C>(v[decltype(xc)::value]); });
+        __syncthreads();
+        if (active) static_for<0, NP>([&](auto yc) { part<C>(v[decltype(yc)::value]) = s.a[decltype(yc)::value][t]; });
+    });
 }
 
 // One workgroup transforms PL planes of one pair (PlaneGroup): thread t owns
@@ -76,35 +133,32 @@ __global__ __launch_bounds__((PlaneGroup<T, NP>::threads)) void k_hf2x_B(cpx<T>*
                                                                        const T* __restrict__ table, int kq0, int nz)
 {
     constexpr int PL = PlaneGroup<T, NP>::planes;
-    __shared__ PlaneTile<T, NP> tiles[PL];
+    using Tile = std::conditional_t<sizeof(T) == 8, PartTile<T, NP>, PlaneTile<T, NP>>;
+    __shared__ Tile tiles[PL];
     const int p = threadIdx.x / NP, t = threadIdx.x % NP;
     const int z = blockIdx.x * PL + p;
     const bool active = p < PL && z < nz;
     const size_t ng = (size_t)NP * NP * nz;
     cpx<T>* plane = buf + (size_t)blockIdx.y * ng + (size_t)(active ? z : 0) * (NP * NP);
-    PlaneTile<T, NP>& tile = tiles[active ? p : 0];
+    Tile& tile = tiles[active ? p : 0];
     cpx<T> v[NP];
     if (active) {
         load_pencil_nt<T, NP>(v, plane + t, NP);
         fft<T, NP, -1>(v);
-        tile_put_col<T, NP>(tile, v, t);
     }
-    __syncthreads();
+    cols_to_rows<T, NP>(tile, v, t, active);
     if (active) {
-        tile_get_row<T, NP>(tile, v, t);
         fft<T, NP, -1>(v);
-        // frequency (x, y=t, z)
-        const T* w = table + (size_t)(pairs[blockIdx.y].kq - kq0) * ng + (size_t)z * NP * NP + (size_t)t * NP;
+        // frequency (x, y=t, z); the table is y-fastest within the plane
+        const T* w = table + (size_t)(pairs[blockIdx.y].kq - kq0) * ng + (size_t)z * NP * NP + t;
         static_for<0, NP>([&](auto xc) {
             constexpr int x = decltype(xc)::value;
-            v[x] = scale(v[x], w[x]);
+            v[x] = scale(v[x], w[x * NP]);
         });
         fft<T, NP, 1>(v);
-        tile_put_row<T, NP>(tile, v, t);
     }
-    __syncthreads();
+    rows_to_cols<T, NP>(tile, v, t, active);
     if (active) {
-        tile_get_col<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
         store_pencil_nt<T, NP>(v, plane + t, NP);
     }
