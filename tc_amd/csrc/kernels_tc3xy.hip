@@ -261,13 +261,17 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_b4(const cpx<T>* __restrict__ r
 // ----------------------------------------------------------------- tc3a2a4b2b5
 // q1-q2 combine.  In: rec[p][h] z-transformed (h = 0: conj(chi_q1) phi_q2; BITC h = 1:
 // conj(phi_q1) chi_q2).  Out: O[p][6h + 3s + d] = du12_s,d rec[p][h], inverse-transformed
-// along x and y.
-template <typename T, int NP>
-__global__ __launch_bounds__(kXyBlock) void k_xy_a8c1(const cpx<T>* __restrict__ rec, int nin,
-                                                      cpx<T>* __restrict__ O, const PairEntry* __restrict__ pairs,
+// along x and y.  The NIN inputs are transformed side by side (waves 0..NIN-1); each
+// thread then keeps its points of them in registers, so all four tiles take outputs and
+// the 6 NIN outputs leave in rounds of four (three full rounds for BITC).
+template <typename T, int NP, int NIN>
+__global__ __launch_bounds__(kXyBlock) void k_xy_a8c1(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ O,
+                                                      const PairEntry* __restrict__ pairs,
                                                       const cpx<T>* __restrict__ tab, int kq0,
                                                       const double* __restrict__ kqcart, int nz, Geom g)
 {
+    constexpr int NOUT = 6 * NIN;
+    constexpr int EPT = (NP * NP + kXyBlock - 1) / kXyBlock;  // plane points per thread
     __shared__ XyTiles<T, NP> S;
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
     const int z = blockIdx.x, p = blockIdx.y;
@@ -275,21 +279,47 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a8c1(const cpx<T>* __restrict__
     const PairEntry pe = pairs[p];
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
-    for (int h = 0; h < nin; ++h) {
-        xy_forward<T, NP>(S.t[0], rec + ((size_t)p * nin + h) * ng + pz, w == 0, t);
-        for (int s = 0; s < 2; ++s) {
-            for (int e = threadIdx.x; e < NP * NP; e += kXyBlock) {
+    xy_forward<T, NP>(S.t[w], rec + ((size_t)p * NIN + (w < NIN ? w : 0)) * ng + pz, w < NIN, t);
+    cpx<T> r[NIN][EPT];
+    static_for<0, EPT>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const int e = threadIdx.x + i * kXyBlock;
+        if (e < NP * NP) {
+            const int x = e % NP, y = e / NP;
+            static_for<0, NIN>([&](auto hc) { r[decltype(hc)::value][i] = S.t[decltype(hc)::value].a[y][x]; });
+        }
+    });
+    __syncthreads();  // the inputs have been read before outputs overwrite the tiles
+#pragma unroll 1
+    for (int o0 = 0; o0 < NOUT; o0 += kXyWaves) {
+        // the per-point terms are recomputed each round rather than held across rounds:
+        // the empty asm hides that the thread index does not change
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        static_for<0, EPT>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            const int e = tid + i * kXyBlock;
+            if (e < NP * NP) {
                 const int x = e % NP, y = e / NP;
                 double xd[3];
                 pp.at(g, x, y, NP, xd);
-                const double u = s == 0 ? (double)tp[e].re : (double)tp[e].im;
-                const cpx<T> r = S.t[0].a[y][x];
-                for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r, xd[d] * u);
+                const cpx<T> u = tp[e];
+                static_for<0, kXyWaves>([&](auto wc) {
+                    constexpr int wi = decltype(wc)::value;
+                    const int o = o0 + wi;
+                    if (o < NOUT) {
+                        const int sd = o % 6, d = sd % 3;
+                        const cpx<T> rh = (NIN == 1 || o < 6) ? r[0][i] : r[NIN - 1][i];
+                        const double us = sd < 3 ? (double)u.re : (double)u.im;
+                        const double xs = d == 0 ? xd[0] : (d == 1 ? xd[1] : xd[2]);
+                        S.t[wi].a[y][x] = mulr(rh, xs * us);
+                    }
+                });
             }
-            __syncthreads();
-            const int ow = w >= 1 ? w - 1 : 0;
-            xy_inverse_store<T, NP>(S.t[w], O + ((size_t)p * 6 * nin + 6 * h + 3 * s + ow) * ng + pz, w >= 1, t);
-        }
+        });
+        __syncthreads();
+        const int o = o0 + w;
+        xy_inverse_store<T, NP>(S.t[w], O + ((size_t)p * NOUT + (o < NOUT ? o : 0)) * ng + pz, o < NOUT, t);
     }
 }
 
@@ -423,10 +453,16 @@ void launch_xy_a8c1(int prec, const Geom& g, const void* rec, int nin, void* O, 
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
 #define TCX_L(NP)                                                                                           \
-    hipLaunchKernelGGL((k_xy_a8c1<T, NP>), dim3(g.n3, n), dim3(kXyBlock), 0, s,                           \
-                       (const cpx<T>*)((const char*)rec + (size_t)a * nin * gb), nin,                      \
-                       (cpx<T>*)((char*)O + (size_t)a * 6 * nin * gb), pairs + a, (const cpx<T>*)tab, kq0, kqcart, \
-                       g.n3, g);
+    {                                                                                                       \
+        const cpx<T>* in = (const cpx<T>*)((const char*)rec + (size_t)a * nin * gb);                        \
+        cpx<T>* out = (cpx<T>*)((char*)O + (size_t)a * 6 * nin * gb);                                       \
+        if (nin == 2)                                                                                       \
+            hipLaunchKernelGGL((k_xy_a8c1<T, NP, 2>), dim3(g.n3, n), dim3(kXyBlock), 0, s, in, out, pairs + a, \
+                               (const cpx<T>*)tab, kq0, kqcart, g.n3, g);                                   \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_xy_a8c1<T, NP, 1>), dim3(g.n3, n), dim3(kXyBlock), 0, s, in, out, pairs + a, \
+                               (const cpx<T>*)tab, kq0, kqcart, g.n3, g);                                   \
+    }
         TCX_XY_DISPATCH(prec, g, TCX_L);
 #undef TCX_L
     });
