@@ -367,7 +367,7 @@ void launch_za8_q(int prec, const Geom& g, const void* O, int bitc, const void* 
 #define TCX_L(NZ)                                                                                              \
     {                                                                                                          \
         constexpr int S2 = SplitFor<NZ, 4>::value;                                                             \
-        constexpr int S4 = SplitFor<NZ, 4>::value;                                                             \
+        constexpr int S4 = SplitFor<NZ, sizeof(T) == 8 ? 4 : 2>::value;                                        \
         constexpr int P2 = PencilSplit<S2>::pencils, P4 = PencilSplit<S4>::pencils;                            \
         hipLaunchKernelGGL((k_za8_q2<T, NZ, S2>), dim3((nxy + P2 - 1) / P2, n), dim3(kPencilBlock), 0, s,     \
                            (const cpx<T>*)O, bitc, pairs, targets + a, fill, (double2*)Q2, nxy);               \
@@ -392,7 +392,7 @@ void launch_za8_apply(int prec, const Geom& g, const void* Z, const void* occR, 
     for_target_slices(ntargets, [&](int a, int n) {
 #define TCX_L(NZ)                                                                                              \
     {                                                                                                          \
-        constexpr int S = SplitFor<NZ, 4>::value;                                                              \
+        constexpr int S = SplitFor<NZ, sizeof(T) == 8 ? 4 : 2>::value;                                         \
         constexpr int PN = PencilSplit<S>::pencils;                                                            \
         hipLaunchKernelGGL((k_za8_apply<T, NZ, S>), dim3((nxy + PN - 1) / PN, n), dim3(kPencilBlock), 0, s,   \
                            (const cpx<T>*)Z, (const cpx<T>*)occR, (const cpx<T>*)Q4T, pairs, targets + a,      \
