@@ -82,6 +82,53 @@ __device__ __forceinline__ double xshfl(double v, int mask)
     return __hiloint2double(hi, lo);
 }
 
+// Radix-2 butterfly between lane and lane ^ MASK (MASK = 32 or 16) through gfx950's
+// lane-swap instructions, which exchange values in the VALU rather than through LDS.
+// With both operands holding x, v_permlane32_swap returns (own x, partner's x) in
+// lanes 0-31 and (partner's x, own x) in lanes 32-63; v_permlane16_swap does the same
+// for the even and odd 16-lane rows.  So a lane whose MASK bit is clear gets
+// r0 + r1 = own + partner, and a lane whose MASK bit is set gets r0 - r1 =
+// partner - own, with no select of the partner's value.  `upper` must be
+// (lane & MASK) != 0.  A double is exchanged as its two 32-bit halves.
+// The swap is in-order VALU work, where ds_bpermute (xshfl) can have several
+// exchanges in flight; which one is faster depends on the kernel around it.
+template <int MASK>
+__device__ __forceinline__ void lane_swap(unsigned x, unsigned& r0, unsigned& r1)
+{
+    static_assert(MASK == 32 || MASK == 16, "lane swaps exchange lane ^ 32 or lane ^ 16");
+    if constexpr (MASK == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        r0 = r[0];
+        r1 = r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        r0 = r[0];
+        r1 = r[1];
+    }
+}
+template <int MASK>
+__device__ __forceinline__ float xbfly(float x, bool upper)
+{
+    unsigned a, b;
+    lane_swap<MASK>(__float_as_uint(x), a, b);
+    const float r0 = __uint_as_float(a), r1 = __uint_as_float(b);
+    return upper ? r0 - r1 : r0 + r1;
+}
+template <int MASK>
+__device__ __forceinline__ double xbfly(double x, bool upper)
+{
+    unsigned lo0, lo1, hi0, hi1;
+    lane_swap<MASK>((unsigned)__double2loint(x), lo0, lo1);
+    lane_swap<MASK>((unsigned)__double2hiint(x), hi0, hi1);
+    const double r0 = __hiloint2double((int)hi0, (int)lo0), r1 = __hiloint2double((int)hi1, (int)lo1);
+    return upper ? r0 - r1 : r0 + r1;
+}
+template <int MASK, typename T>
+__device__ __forceinline__ cpx<T> xbfly(cpx<T> x, bool upper)
+{
+    return cpx<T>{xbfly<MASK>(x.re, upper), xbfly<MASK>(x.im, upper)};
+}
+
 // ---------------------------------------------------------------------------
 // Pencil split over two lanes (lane and lane ^ 32 of a wavefront).
 // Lane half h holds the samples z = 2m + h (m < NZ/2) on entry: a radix-2
@@ -91,7 +138,7 @@ __device__ __forceinline__ double xshfl(double v, int mask)
 // z = h*NZ/2 + k (k < NZ/2).  Halves the registers a pencil needs, so the
 // FP64 accumulators of the accumulate pass fit at twice the occupancy.
 // ---------------------------------------------------------------------------
-template <typename T, int NZ, int SIGN>
+template <typename T, int NZ, int SIGN, bool kLaneSwap = false>
 __device__ __forceinline__ void fft_split2(cpx<T> (&v)[NZ / 2], int h)
 {
     constexpr int M = NZ / 2;
@@ -103,8 +150,12 @@ __device__ __forceinline__ void fft_split2(cpx<T> (&v)[NZ / 2], int h)
     });
     static_for<0, M>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const cpx<T> o{xshfl(v[k].re, 32), xshfl(v[k].im, 32)};
-        v[k] = csel(h != 0, o - v[k], v[k] + o);
+        if constexpr (kLaneSwap) {
+            v[k] = xbfly<32>(v[k], h != 0);  // h = lane >> 5
+        } else {
+            const cpx<T> o{xshfl(v[k].re, 32), xshfl(v[k].im, 32)};
+            v[k] = csel(h != 0, o - v[k], v[k] + o);
+        }
     });
 }
 
@@ -112,7 +163,7 @@ __device__ __forceinline__ void fft_split2(cpx<T> (&v)[NZ / 2], int h)
 // z = 4m + h on entry; the radix-4 cross-lane DFT runs as two radix-2 stages
 // (shuffle with lane ^ 32, twiddle, shuffle with lane ^ 16).  On exit lane
 // quarter h holds outputs z = ob*NZ/4 + k with ob = (h >> 1) + 2*(h & 1).
-template <typename T, int NZ, int SIGN>
+template <typename T, int NZ, int SIGN, bool kLaneSwap = false>
 __device__ __forceinline__ void fft_split4(cpx<T> (&v)[NZ / 4], int h)
 {
     constexpr int M = NZ / 4;
@@ -129,11 +180,17 @@ __device__ __forceinline__ void fft_split4(cpx<T> (&v)[NZ / 4], int h)
     });
     static_for<0, M>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const cpx<T> o{xshfl(v[k].re, 32), xshfl(v[k].im, 32)};
-        cpx<T> z = csel(h1 != 0, o - v[k], v[k] + o);               // Z_{h0, s0=h1}
-        z = csel((h0 & h1) != 0, mul_isign<T, SIGN>(z, T(1)), z);     // W_4^{h0 s0}
-        const cpx<T> o2{xshfl(z.re, 16), xshfl(z.im, 16)};
-        v[k] = csel(h0 != 0, o2 - z, z + o2);                       // X_{s0 + 2 s1}, s1 = h0
+        if constexpr (kLaneSwap) {
+            cpx<T> z = xbfly<32>(v[k], h1 != 0);                        // h1 = (lane >> 5) & 1
+            z = csel((h0 & h1) != 0, mul_isign<T, SIGN>(z, T(1)), z);
+            v[k] = xbfly<16>(z, h0 != 0);                               // h0 = (lane >> 4) & 1
+        } else {
+            const cpx<T> o{xshfl(v[k].re, 32), xshfl(v[k].im, 32)};
+            cpx<T> z = csel(h1 != 0, o - v[k], v[k] + o);               // Z_{h0, s0=h1}
+            z = csel((h0 & h1) != 0, mul_isign<T, SIGN>(z, T(1)), z);     // W_4^{h0 s0}
+            const cpx<T> o2{xshfl(z.re, 16), xshfl(z.im, 16)};
+            v[k] = csel(h0 != 0, o2 - z, z + o2);                       // X_{s0 + 2 s1}, s1 = h0
+        }
     });
 }
 
@@ -144,15 +201,15 @@ template <>
 struct PencilSplit<2> {
     static constexpr int pencils = 32;
     __device__ static int out_block(int h) { return h; }
-    template <typename T, int NZ, int SIGN>
-    __device__ static void fft(cpx<T> (&v)[NZ / 2], int h) { fft_split2<T, NZ, SIGN>(v, h); }
+    template <typename T, int NZ, int SIGN, bool kLaneSwap = false>
+    __device__ static void fft(cpx<T> (&v)[NZ / 2], int h) { fft_split2<T, NZ, SIGN, kLaneSwap>(v, h); }
 };
 template <>
 struct PencilSplit<4> {
     static constexpr int pencils = 16;
     __device__ static int out_block(int h) { return (h >> 1) + 2 * (h & 1); }
-    template <typename T, int NZ, int SIGN>
-    __device__ static void fft(cpx<T> (&v)[NZ / 4], int h) { fft_split4<T, NZ, SIGN>(v, h); }
+    template <typename T, int NZ, int SIGN, bool kLaneSwap = false>
+    __device__ static void fft(cpx<T> (&v)[NZ / 4], int h) { fft_split4<T, NZ, SIGN, kLaneSwap>(v, h); }
 };
 
 // FP32 pair records split a pencil over 2 lanes, FP64 ones over 4 (registers

@@ -103,7 +103,8 @@ __global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P
         } else {
             load_pencil<T, M>(v, g + L.xy + (size_t)L.h * nxy, S * nxy);  // z = S m + h
         }
-        PencilSplit<S>::template fft<T, NZ, 1>(v, L.h);  // -> z = ob M + k
+        // the exchanges of both transforms use lane swaps here (3.5% faster than ds_bpermute)
+        PencilSplit<S>::template fft<T, NZ, 1, true>(v, L.h);  // -> z = ob M + k
         {
             const double2* n = dens + s * dstride + o0;  // dstride 0: one independent spin
             static_for<0, M>([&](auto kc) {
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P
                 v[m] = xs[pi][S * m + L.h];  // z = S m + h
             });
             __syncthreads();
-            PencilSplit<S>::template fft<T, NZ, -1>(v, L.h);  // -> z = ob M + k
+            PencilSplit<S>::template fft<T, NZ, -1, true>(v, L.h);  // -> z = ob M + k
             if (L.valid)
                 static_for<0, M>([&](auto kc) {
                     constexpr int k = decltype(kc)::value;
