@@ -71,8 +71,9 @@ __global__ __launch_bounds__(256) void k_coulomb_table(T* __restrict__ table, co
 // LDS tile of one component (real or imaginary part) of a plane, [y][x] with one
 // element of padding per row.  The FP64 pass B transposes the two parts one after
 // the other: that halves its LDS per plane and doubles its waves per SIMD (1 -> 2 at
-// 40^3).  FP32 keeps whole complex tiles, which take half the LDS instructions and
-// a third of the barriers; its occupancy is set by the LDS either way.
+// 40^3).  FP32 keeps whole complex tiles: split tiles would double its waves too
+// (2 -> 4), but with twice the LDS instructions and three times the barriers the
+// pass ran 5% slower.
 template <typename T, int NP>
 struct PartTile {
     T a[NP][NP + 1];

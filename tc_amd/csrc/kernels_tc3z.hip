@@ -367,16 +367,17 @@ void launch_za8_q(int prec, const Geom& g, const void* O, int bitc, const void* 
     for_target_slices(ntargets, [&](int a, int n) {
 #define TCX_L(NZ)                                                                                              \
     {                                                                                                          \
-        constexpr int S2 = SplitFor<NZ, 4>::value;                                                             \
-        constexpr int S4 = SplitFor<NZ, sizeof(T) == 8 ? 4 : 2>::value;                                        \
-        constexpr int P2 = PencilSplit<S2>::pencils, P4 = PencilSplit<S4>::pencils;                            \
-        hipLaunchKernelGGL((k_za8_q2<T, NZ, S2>), dim3((nxy + P2 - 1) / P2, n), dim3(kPencilBlock), 0, s,     \
+        /* 4 lanes per pencil for Q2 (2 would need 256 VGPRs); Q4/Q5 take 2 in FP32 (longer load runs) */  \
+        constexpr int SQ2 = SplitFor<NZ, 4>::value;                                                            \
+        constexpr int SQ45 = SplitFor<NZ, sizeof(T) == 8 ? 4 : 2>::value;                                     \
+        constexpr int P2 = PencilSplit<SQ2>::pencils, P4 = PencilSplit<SQ45>::pencils;                         \
+        hipLaunchKernelGGL((k_za8_q2<T, NZ, SQ2>), dim3((nxy + P2 - 1) / P2, n), dim3(kPencilBlock), 0, s,    \
                            (const cpx<T>*)O, bitc, pairs, targets + a, fill, (double2*)Q2, nxy);               \
-        hipLaunchKernelGGL((k_za8_q45<T, NZ, S4>), dim3((nxy + P4 - 1) / P4, n, 3), dim3(kPencilBlock), 0, s, \
+        hipLaunchKernelGGL((k_za8_q45<T, NZ, SQ45>), dim3((nxy + P4 - 1) / P4, n, 3), dim3(kPencilBlock), 0, s, \
                            (const cpx<T>*)O, nch, 0, 0, (const cpx<T>*)occR, pairs, targets + a, fill,        \
                            (double2*)Q4, nxy);                                                                 \
         if (bitc)                                                                                              \
-            hipLaunchKernelGGL((k_za8_q45<T, NZ, S4>), dim3((nxy + P4 - 1) / P4, n, 3), dim3(kPencilBlock), 0, \
+            hipLaunchKernelGGL((k_za8_q45<T, NZ, SQ45>), dim3((nxy + P4 - 1) / P4, n, 3), dim3(kPencilBlock), 0, \
                                s,                                                                              \
                                (const cpx<T>*)O, nch, 6, 1, (const cpx<T>*)chiR, pairs, targets + a, fill,    \
                                (double2*)Q5, nxy);                                                             \
