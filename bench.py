@@ -70,6 +70,19 @@ def cpu_baseline(sys_, threads):
                       f"{dt:.2f} s wall on {threads} threads; full apply extrapolates x{sys_.nk * sys_.nocc[0]}"}
 
 
+def calibration_kernels(eng, log):
+    """Known-size kernels that scale the HBM counters of a profiled run: k_copy reads
+    and writes 1 GiB, k_zero writes 1 GiB."""
+    gib = 1 << 30
+    a, b = eng.device_buffer(gib), eng.device_buffer(gib)
+    for _ in range(3):
+        eng.L.tcx_memcpy(eng.h, b.ptr, a.ptr, gib, 2)  # k_copy: reads and writes 1 GiB
+        a.zero()                                       # k_zero: writes 1 GiB
+    a.free()
+    b.free()
+    log("bench: calibration kernels done (k_copy 1 GiB, k_zero 1 GiB)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +133,8 @@ def main():
 
     if args.skip_headline:
         eng = tcx.Engine(0)
+        if args.calibrate:
+            calibration_kernels(eng, log)
         extra = {}
         if args.bitc:
             extra["bitc_terms"] = bench_bitc_terms(eng, log, args.bitc_terms.split(","), args.bitc_precs.split(","), args.bitc_repeat)
@@ -171,14 +186,7 @@ def main():
     eng.set_launch_shape("hf2x", *shape32)
 
     if args.calibrate:
-        gib = 1 << 30
-        a, b = eng.device_buffer(gib), eng.device_buffer(gib)
-        for _ in range(3):
-            eng.L.tcx_memcpy(eng.h, b.ptr, a.ptr, gib, 2)  # k_copy: reads and writes 1 GiB
-            a.zero()                                       # k_zero: writes 1 GiB
-        a.free()
-        b.free()
-        log("bench: calibration kernels done (k_copy 1 GiB, k_zero 1 GiB)")
+        calibration_kernels(eng, log)
     if args.quick:
         step(tcx.PREC_FP32, d_out32)
         eng.synchronize()
