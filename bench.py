@@ -120,6 +120,8 @@ def main():
                     help="also time the BITC workload (tc2x + three-body terms, fcc 4x4x4k 16b 40^3) in "
                          "FP32-split and FP64 and report its energy errors under 'bitc_terms'")
     args = ap.parse_args()
+    if args.steps < 1 or args.fp64_steps < 1 or args.warmup < 0:
+        ap.error("--steps and --fp64-steps must be at least 1, --warmup at least 0")
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
