@@ -194,6 +194,12 @@ __device__ __forceinline__ void fft_split4(cpx<T> (&v)[NZ / 4], int h)
     });
 }
 
+// Exchange choice of the hf2x and tc2x pair passes: lane swaps in FP64 (tc2x passes A
+// and C then fit one more wave per SIMD; hf2x and tc2x FP64 applies ran 1% faster),
+// ds_bpermute in FP32 (no gain there).
+template <typename T>
+constexpr bool kSwapF64 = sizeof(T) == 8;
+
 // Lanes per pencil and the output block of a lane for the split transforms.
 template <int S>
 struct PencilSplit;

@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_C(const cpx<T>* __restric
         } else {
             load_pencil<T, M>(v, grid_of(pp), S * nxy);  // z = S m + h
         }
-        Split::template fft<T, NZ, 1>(v, h);  // -> z = ob*M + k
+        Split::template fft<T, NZ, 1, kSwapF64<T>>(v, h);  // -> z = ob*M + k
         const cpx<T>* q = phiR_occ + (size_t)occ * ng + xy + (size_t)ob * M * nxy;
         const double f = fill ? fill[occ] : 1.0;
         static_for<0, M>([&](auto kc) {

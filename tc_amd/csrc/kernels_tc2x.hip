@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_A(const cpx<T>* __restric
             constexpr int m = decltype(mc)::value;
             v[m] = cmulc(x[m], src[(size_t)(S * m) * nxy]);
         });
-        Split::template fft<T, NZ, -1>(v, h);
+        Split::template fft<T, NZ, -1, kSwapF64<T>>(v, h);
         if (valid) store_pencil_nt<T, M>(v, out + (size_t)c * ng, nxy);  // streaming: read once by pass B
     }
 }
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_C(const cpx<T>* __restric
             } else {
                 load_pencil<T, M>(v, grid_of(it), S * nxy);  // z = S m + h
             }
-            Split::template fft<T, NZ, 1>(v, h);
+            Split::template fft<T, NZ, 1, kSwapF64<T>>(v, h);
             // channel 0: b * phi_qi (+f); channels 1..3: a_d * g_qi,d (-f)   [tc2x.cpp:363-369, :387-388]
             const cpx<T>* m = (c == 0 ? occR + (size_t)occ * ng : gradR + ((size_t)occ * 3 + (c - 1)) * ng) + xy +
                               (size_t)ob * M * nxy;
