@@ -53,21 +53,46 @@ def read_traffic(name, pairs_per_launch):
     return float(entry["bytes_per_pair"]) * pairs_per_launch
 
 
-def cpu_baseline(sys_, threads):
-    """FP64 oracle (reference loop nest, OpenMP over q) on one applied band at k=0:
-    all 64 q x 16 i = 1024 pairs, about 40 core-seconds."""
+def cpu_model():
+    """Model name of the host CPU (first /proc/cpuinfo entry), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_cores():
+    """CPUs this process may run on (the box's share), not the machine's total."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def oracle_bands(sys_, nbands, threads):
+    """FP64 oracle (reference loop nest, OpenMP over q) on the first `nbands` occupied bands at
+    k=0 against all occupied (q,i); returns (output, seconds, pairs)."""
     from oracle import oracle as O
     o = O.Oracle(sys_)
     nb = np.zeros(sys_.nk, np.int32)
-    nb[0] = 1
-    phi = sys_.occ_block(0)[:1].reshape(-1).copy()
-    pairs = int(sys_.nocc.sum())
+    nb[0] = nbands
+    phi = sys_.occ_block(0)[:nbands].reshape(-1).copy()
     t0 = time.perf_counter()
-    o.hf2x(nb, phi, nthreads=threads)
+    out = o.hf2x(nb, phi, nthreads=threads)
     dt = time.perf_counter() - t0
+    return out, dt, nbands * int(sys_.nocc.sum())
+
+
+def cpu_baseline_entry(dt, pairs, nbands, threads):
     return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"hf2x FP64 oracle, 1 applied band at k=0 x {pairs} occupied (q,i) = {pairs} pairs, "
-                      f"{dt:.2f} s wall on {threads} threads; full apply extrapolates x{sys_.nk * sys_.nocc[0]}"}
+            "cpu_model": cpu_model(), "machine_cpus": os.cpu_count(),
+            "sample": f"hf2x FP64 oracle (recursive radix-2/3/5 FFT, OpenMP over q), {nbands} applied bands at k=0 "
+                      f"x {pairs // nbands} occupied (q,i) = {pairs} pairs, {dt:.2f} s wall on {threads} threads; "
+                      "pairs/s is per pair, the same unit as value"}
 
 
 def calibration_kernels(eng, log):
@@ -92,7 +117,11 @@ def main():
     ap.add_argument("--targets", type=int, default=0, help="launch shape: targets per launch (0 = tuned/auto)")
     ap.add_argument("--sources", type=int, default=0, help="launch shape: sources per launch (0 = tuned/auto)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: every CPU this process may use)")
+    ap.add_argument("--cpu-bands", type=int, default=4, help="applied bands at k=0 in the CPU-baseline sample")
+    ap.add_argument("--no-tc", action="store_true",
+                    help="skip the default TC legs (tc2x and the three-body terms on the BITC cell)")
+    ap.add_argument("--tc-steps", type=int, default=3, help="timed FP32-split tc2x applies in the TC leg")
     ap.add_argument("--tune", action="store_true", help="run the launch-shape autotuner first")
     ap.add_argument("--quick", action="store_true",
                     help="profiling run: one FP32 apply only (no FP64 leg, parity or CPU baseline)")
@@ -252,11 +281,22 @@ def main():
     ex32 = E.term_energy(sys_, h32, 2)
     ex64 = E.term_energy(sys_, h64, 2)
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"bench: CPU baseline on {threads} threads")
-        cpu = cpu_baseline(sys_, threads)
+    # parity against the FP64 CPU oracle on band (k=0, j=0) (and the CPU baseline on the same run)
+    threads = args.cpu_threads or host_cores()
+    nsample = 1 if args.no_cpu_baseline else max(1, min(args.cpu_bands, int(nb[0])))
+    log(f"bench: FP64 oracle on {nsample} band(s) at k=0, {threads} threads")
+    ref, dt_cpu, cpu_pairs = oracle_bands(sys_, nsample, threads)
+    npw0 = int(sys_.npw[0])
+    ref0 = ref[:npw0]
+    oracle_rel = {p: float(np.linalg.norm(h[:npw0] - ref0) / np.linalg.norm(ref0)) for p, h in (("fp64", h64), ("fp32", h32))}
+    cpu = None if args.no_cpu_baseline else cpu_baseline_entry(dt_cpu, cpu_pairs, nsample, threads)
+    failures = []
+    if not oracle_rel["fp64"] <= 1e-10:
+        failures.append(f"FP64 band (0,0) differs from the oracle: {oracle_rel['fp64']:.3e}")
+    if not oracle_rel["fp32"] <= 1e-6:
+        failures.append(f"FP32-split band (0,0) differs from the oracle: {oracle_rel['fp32']:.3e}")
+    if not abs(ex32 - ex64) < 1e-6:
+        failures.append(f"|dE_x| FP32-split vs FP64 = {abs(ex32 - ex64):.3e} Ha >= 1e-6")
 
     out = {
         "metric": METRIC,
@@ -294,7 +334,22 @@ def main():
             "per_band_rel_max": float(np.max(rel)), "per_band_rel_median": float(np.median(rel)),
             "E_x_fp64_Ha": float(ex64.real), "dE_x_Ha": float(abs(ex32 - ex64)),
         },
+        "parity_vs_oracle": {"band": [0, 0], "rel_fp64": oracle_rel["fp64"], "rel_fp32": oracle_rel["fp32"],
+                             "tolerance": {"fp64": 1e-10, "fp32": 1e-6}},
     }
+    if not args.no_tc:
+        out["tc"] = bench_tc_legs(eng, log, args.tc_steps)
+        tc2 = out["tc"]["tc2x"]
+        if not tc2["dE_Ha"] < 1e-6:
+            failures.append(f"tc2x |dE| FP32-split vs FP64 = {tc2['dE_Ha']:.3e} Ha >= 1e-6")
+        for term, e in out["tc"]["three_body"].items():
+            if not e["dE_Ha"] < 1e-6:
+                failures.append(f"{term} |dE| FP32-split vs FP64 = {e['dE_Ha']:.3e} Ha >= 1e-6")
+        # the north-star ratio: exchange + TC two-body, FP64 GPU time over FP32-split
+        out["exchange_plus_tc2x"] = {
+            "fp32_ms": ms_per_step + tc2["fp32_ms"], "fp64_ms": fp64_s * 1e3 + tc2["fp64_ms"],
+            "fp32_over_fp64": (fp64_s * 1e3 + tc2["fp64_ms"]) / (ms_per_step + tc2["fp32_ms"])}
+    out["checks_failed"] = failures
     if args.bitc:
         out["bitc_terms"] = bench_bitc_terms(eng, log, args.bitc_terms.split(","), args.bitc_precs.split(","), args.bitc_repeat)
     if args.batched:
@@ -307,6 +362,10 @@ def main():
     for b in (d_phi, d_out32, d_out64, d_scratch):
         b.free()
     eng.close()
+    if failures:
+        for f in failures:
+            log(f"bench: CHECK FAILED: {f}")
+        sys.exit(1)
 
 
 def bench_adjoint(eng, log, h=1e-3):
@@ -541,6 +600,105 @@ def bench_bitc_terms(eng, log, terms, precs, repeat=1):
                           "E_fp64_Ha": [float(e64.real), float(e64.imag)], "dE_Ha": float(abs(e32 - e64)),
                           "per_band_rel_max": float(np.max(rel))})
         res["terms"][term] = entry
+    d_phi.free()
+    d_out.free()
+    return res
+
+
+def bench_tc_legs(eng, log, tc_steps):
+    """BITC cell of the TC workload (fcc a=10.26, 4x4x4 k, 16 bands, 40^3, RPA Jastrow A=0.5),
+    inputs resident in HBM:
+      tc2x: `tc_steps` timed FP32-split applies after one warm-up, with the HIP-event roofline
+            of its dominant pass; one FP64 apply after a warm-up; energy and per-band errors;
+      tc3a3b3b4b6, tc3a2a4b2b5: FP32-split first apply (occupied-set work included), repeated
+            apply (occupied-set q-sums reused), FP64 first apply; energy errors.
+    Algorithmic bytes per tc2x pair and pass (N points, 8 B per FP32 complex value): pass A
+    reads chi_qi and writes 4 record channels (5 N 8 B), pass B reads and writes 4 channels
+    (8 N 8 B), pass C reads 4 channels and the 4 grids of phi_qi and grad phi_qi (8 N 8 B)."""
+    from tc_amd import energy as E
+    from tc_amd import system as S
+    from tc_amd import tcx
+    sys_ = S.fcc_bitc_40()
+    nb = sys_.nocc.copy()
+    phi = sys_.phi_occ
+    N = sys_.ngrid
+    pairs = int(np.sum(nb) * np.sum(sys_.nocc))
+    eng.load_system(sys_)
+    d_phi = eng.to_device(phi)
+    d_out = eng.device_buffer(phi.nbytes)
+    bo = sys_.band_offsets(nb)
+
+    def per_band(a, b):
+        rel = []
+        for ik in range(sys_.nk):
+            x = a[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+            y = b[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
+            rel.extend(np.linalg.norm(x - y, axis=1) / np.maximum(np.linalg.norm(y, axis=1), 1e-300))
+        return float(np.max(rel))
+
+    def apply(term, prec):
+        d_out.zero()
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng._term_device(term, nb, d_phi.ptr, d_out.ptr, prec)
+        eng.synchronize()
+        return time.perf_counter() - t0
+
+    res = {"workload": "BITC fcc a=10.26 bohr, 4x4x4 k, 16 bands/k (all occupied), FFT 40^3, RPA Jastrow A=0.5",
+           "pairs_per_apply": pairs}
+    # ---- tc2x
+    log("bench tc: tc2x FP32-split warm-up")
+    apply("tc2x", tcx.PREC_FP32)
+    eng.enable_timing(True)
+    t32 = []
+    for i in range(tc_steps):
+        t32.append(apply("tc2x", tcx.PREC_FP32))
+        log(f"bench tc: tc2x FP32-split apply {i + 1}/{tc_steps} {t32[-1]:.3f} s")
+    ms, launches, kpairs = eng.get_timing()
+    eng.enable_timing(False)
+    h32 = d_out.download(np.complex128)
+    apply("tc2x", tcx.PREC_FP64)
+    t64 = apply("tc2x", tcx.PREC_FP64)
+    log(f"bench tc: tc2x FP64 apply {t64:.3f} s")
+    h64 = d_out.download(np.complex128)
+    e32 = E.term_energy(sys_, h32, 2, left=True)
+    e64 = E.term_energy(sys_, h64, 2, left=True)
+    words = [5, 8, 8]   # grid values moved per pair in passes A, B, C
+    dom = int(np.argmax(ms[:3]))
+    avg_ms = ms[dom] / max(1, launches[dom])
+    ppl = kpairs[dom] / max(1, launches[dom])
+    bpl = words[dom] * N * 8 * ppl
+    fp32_s = float(np.median(t32))
+    res["tc2x"] = {
+        "fp32_ms": fp32_s * 1e3, "fp64_ms": t64 * 1e3, "fp32_over_fp64": t64 / fp32_s,
+        "fp32_pairs_per_s": pairs / fp32_s, "E_fp64_Ha": [float(e64.real), float(e64.imag)],
+        "dE_Ha": float(abs(e32 - e64)), "per_band_rel_max": per_band(h32, h64),
+        "pipeline_algorithmic_GBps_21NBc": 21 * N * 8 * pairs / fp32_s / 1e9,
+        "roofline": {"bound": "hbm", "achieved": bpl / (avg_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bpl / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": read_traffic("tc2x_" + CLASS_NAMES[dom], ppl),
+                     "kernel": "tc2x_" + CLASS_NAMES[dom], "avg_launch_ms": avg_ms, "pairs_per_launch": ppl,
+                     "algorithmic_bytes_per_launch": bpl},
+        "class_ms_per_apply": {CLASS_NAMES[c]: float(ms[c]) / tc_steps for c in range(4)},
+    }
+    # ---- three-body exchange terms
+    res["three_body"] = {}
+    for term in ("tc3a3b3b4b6", "tc3a2a4b2b5"):
+        eng.load_system(sys_)             # drops the occupied-set caches: the first apply does all the work
+        first = apply(term, tcx.PREC_FP32)
+        h32 = d_out.download(np.complex128)
+        again = apply(term, tcx.PREC_FP32)
+        same = bool(np.array_equal(d_out.download(np.complex128), h32))
+        t64 = apply(term, tcx.PREC_FP64)
+        h64 = d_out.download(np.complex128)
+        e32 = E.term_energy(sys_, h32, 3, left=True)
+        e64 = E.term_energy(sys_, h64, 3, left=True)
+        log(f"bench tc: {term} FP32 first {first:.2f} s, repeated {again:.2f} s, FP64 first {t64:.2f} s")
+        res["three_body"][term] = {
+            "fp32_first_s": first, "fp32_repeated_s": again, "repeated_bit_identical": same,
+            "fp64_first_s": t64, "fp32_over_fp64_first": t64 / first,
+            "E_fp64_Ha": [float(e64.real), float(e64.imag)], "dE_Ha": float(abs(e32 - e64)),
+            "per_band_rel_max": per_band(h32, h64)}
     d_phi.free()
     d_out.free()
     return res

@@ -64,6 +64,7 @@ void Context::swap_left_right()
         gradR_valid[p] = false;  // gradients are of the right orbitals
     }
     density_valid = false;       // rebuilt from the swapped pair: conj of the density
+    invalidate_local();
 }
 
 void Context::grad_x(double* grad, double* energy, int prec)
@@ -113,33 +114,40 @@ void Context::grad_tc2(double* grad_left, double* grad_right, double* energy, do
     const cd e = half_energy(chi, Hphi, occ_coef, nocc, npw, fill, num_kpoints);
 
     // H_2^dagger chi: u -> -u, right <-> left (TC: the orbitals are their own left set)
+    // (every change of the kernels or orbitals drops the cached local potentials)
     for (double& a : A) a = -a;
+    invalidate_local();
     if (bitc) swap_left_right();  // TC: phi = chi, nothing to swap
     std::vector<cd> Hdchi;
     try {
         Hdchi = H2(chi);
     } catch (...) {
         for (double& a : A) a = -a;
+        invalidate_local();
         if (bitc) swap_left_right();
         throw;
     }
     for (double& a : A) a = -a;
+    invalidate_local();
     if (bitc) swap_left_right();
 
     // dE_2/dA: the same energy with the d/dA kernels and no divergence term
     const bool div_saved = div_enabled;
     jastrow_dA = true;
     div_enabled = false;
+    invalidate_local();
     std::vector<cd> Hd;
     try {
         Hd = H2(phi);
     } catch (...) {
         jastrow_dA = false;
         div_enabled = div_saved;
+        invalidate_local();
         throw;
     }
     jastrow_dA = false;
     div_enabled = div_saved;
+    invalidate_local();
     const cd dA = half_energy(chi, Hd, occ_coef, nocc, npw, fill, num_kpoints);
 
     cd* gl = reinterpret_cast<cd*>(grad_left);

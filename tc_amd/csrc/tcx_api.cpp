@@ -174,29 +174,45 @@ void Context::set_kpoints(int nk_, const double* kf, int numk)
     invalidate_states();
 }
 
+// The setters below drop the state they replace on entry (so a rejected call
+// leaves the context without it: later applies return TCX_ERR_STATE instead of
+// reading buffers sized for the old counts), validate into locals, and commit
+// only when every check has passed.
 void Context::set_basis(const int* npw_, const int* gindex)
 {
     require(k_set, TCX_ERR_STATE, "set_basis: call tcx_set_kpoints first");
-    require(npw_ && gindex, TCX_ERR_ARG, "set_basis: null pointer");
-    npw.assign(npw_, npw_ + nk);
-    pw_off.assign(nk + 1, 0);
-    for (int k = 0; k < nk; ++k) {
-        require(npw[k] > 0, TCX_ERR_ARG, "set_basis: npw must be positive");
-        pw_off[k + 1] = pw_off[k] + npw[k];
-    }
-    for (long long l = 0; l < pw_off[nk]; ++l)
-        require(gindex[l] >= 0 && gindex[l] < N, TCX_ERR_ARG, "set_basis: gindex outside the FFT grid");
-    max_npw = *std::max_element(npw.begin(), npw.end());
-    gindex_h.assign(gindex, gindex + pw_off[nk]);
-    d_gindex.ensure(pw_off[nk] * sizeof(int));
-    TCX_HIP_CHECK(hipMemcpy(d_gindex.p, gindex, pw_off[nk] * sizeof(int), hipMemcpyHostToDevice));
-    basis_set = true;
+    basis_set = false;
     states_set = false;
     invalidate_states();
+    require(npw_ && gindex, TCX_ERR_ARG, "set_basis: null pointer");
+    std::vector<int> n_(npw_, npw_ + nk);
+    std::vector<long long> off(nk + 1, 0);
+    for (int k = 0; k < nk; ++k) {
+        require(n_[k] > 0, TCX_ERR_ARG, "set_basis: npw must be positive");
+        off[k + 1] = off[k] + n_[k];
+    }
+    for (long long l = 0; l < off[nk]; ++l)
+        require(gindex[l] >= 0 && gindex[l] < N, TCX_ERR_ARG, "set_basis: gindex outside the FFT grid");
+    d_gindex.ensure(off[nk] * sizeof(int));
+    TCX_HIP_CHECK(hipMemcpy(d_gindex.p, gindex, off[nk] * sizeof(int), hipMemcpyHostToDevice));
+    npw = std::move(n_);
+    pw_off = std::move(off);
+    max_npw = *std::max_element(npw.begin(), npw.end());
+    gindex_h.assign(gindex, gindex + pw_off[nk]);
+    basis_set = true;
+}
+
+void Context::invalidate_local()
+{
+    for (LocalCache* c : {&c_a8, &c_hartree, &c_tc2h, &c_tc3a1, &c_tc3b1}) {
+        c->valid = false;
+        c->prec = -1;
+    }
 }
 
 void Context::invalidate_states()
 {
+    invalidate_local();
     occR_valid[0] = occR_valid[1] = false;
     chiR_valid[0] = chiR_valid[1] = false;
     gradR_valid[0] = gradR_valid[1] = false;
@@ -209,24 +225,33 @@ void Context::invalidate_states()
 void Context::set_states(const int* nocc_, const double* filling, const double* phi_occ, const double* chi_occ)
 {
     require(basis_set, TCX_ERR_STATE, "set_states: call tcx_set_basis first");
+    states_set = false;
+    invalidate_states();
     require(nocc_ && filling && phi_occ, TCX_ERR_ARG, "set_states: null pointer");
     require(method == TCX_METHOD_BITC || chi_occ == nullptr, TCX_ERR_ARG,
             "set_states: left orbitals are only meaningful for BITC");
     require(method != TCX_METHOD_BITC || chi_occ != nullptr, TCX_ERR_ARG, "set_states: BITC needs left orbitals");
-    nocc.assign(nocc_, nocc_ + nk);
-    occ_first.assign(nk + 1, 0);
-    occ_coef.assign(nk + 1, 0);
+    std::vector<int> no(nocc_, nocc_ + nk);
+    std::vector<long long> first(nk + 1, 0), coef(nk + 1, 0);
     for (int k = 0; k < nk; ++k) {
-        require(nocc[k] >= 0 && nocc[k] <= 512, TCX_ERR_ARG, "set_states: nocc must be in [0, 512]");
-        occ_first[k + 1] = occ_first[k] + nocc[k];
-        occ_coef[k + 1] = occ_coef[k] + (long long)nocc[k] * npw[k];
+        require(no[k] >= 0 && no[k] <= 512, TCX_ERR_ARG, "set_states: nocc must be in [0, 512]");
+        first[k + 1] = first[k] + no[k];
+        coef[k + 1] = coef[k] + (long long)no[k] * npw[k];
     }
-    n_occ = (int)occ_first[nk];
-    fill.assign(filling, filling + n_occ);
-    for (double f : fill) require(std::isfinite(f) && f >= 0.0, TCX_ERR_ARG, "set_states: filling must be >= 0");
-    const size_t cb = (size_t)occ_coef[nk] * 16;
-    for (size_t i = 0; i < (size_t)occ_coef[nk] * 2; ++i)
+    const int nall = (int)first[nk];
+    std::vector<double> f_(filling, filling + nall);
+    for (double f : f_) require(std::isfinite(f) && f >= 0.0, TCX_ERR_ARG, "set_states: filling must be >= 0");
+    const size_t cb = (size_t)coef[nk] * 16;
+    for (size_t i = 0; i < (size_t)coef[nk] * 2; ++i)
         require(std::isfinite(phi_occ[i]), TCX_ERR_ARG, "set_states: non-finite orbital coefficient");
+    if (chi_occ)
+        for (size_t i = 0; i < (size_t)coef[nk] * 2; ++i)
+            require(std::isfinite(chi_occ[i]), TCX_ERR_ARG, "set_states: non-finite left orbital coefficient");
+    nocc = std::move(no);
+    occ_first = std::move(first);
+    occ_coef = std::move(coef);
+    n_occ = nall;
+    fill = std::move(f_);
     d_phi_occ.ensure(std::max<size_t>(cb, 16));
     TCX_HIP_CHECK(hipMemcpy(d_phi_occ.p, phi_occ, cb, hipMemcpyHostToDevice));
     if (chi_occ) {
@@ -243,32 +268,42 @@ void Context::set_states(const int* nocc_, const double* filling, const double* 
         }
     upload(d_occjobs, jobs.data(), jobs.size() * sizeof(OrbEntry));
     upload(d_occgradjobs, gjobs.data(), gjobs.size() * sizeof(OrbEntry));
+    ++states_version;
     states_set = true;
-    invalidate_states();
 }
 
 void Context::set_jastrow(const double A_[4], const double F_[4])
 {
+    jastrow_set = false;
+    plans.clear();
+    invalidate_local();
     require(A_ && F_, TCX_ERR_ARG, "set_jastrow: null pointer");
-    jastrow_zero = true;
+    bool zero = true;
+    for (int i = 0; i < 4; ++i) {
+        require(std::isfinite(A_[i]) && std::isfinite(F_[i]), TCX_ERR_ARG, "set_jastrow: non-finite parameter");
+        if (std::fabs(A_[i]) > 1e-8) zero = false;
+    }
+    if (!zero)
+        for (int i = 0; i < 4; ++i)
+            require(std::fabs(F_[i]) > 1e-8, TCX_ERR_ARG, "set_jastrow: F must be non-zero");
     for (int i = 0; i < 4; ++i) {
         A[i] = A_[i];
         F[i] = F_[i];
-        if (std::fabs(A[i]) > 1e-8) jastrow_zero = false;
     }
-    if (!jastrow_zero)
-        for (int i = 0; i < 4; ++i)
-            require(std::fabs(F[i]) > 1e-8, TCX_ERR_ARG, "set_jastrow: F must be non-zero");
+    jastrow_zero = zero;
     jastrow_set = true;
-    plans.clear();
 }
 
 void Context::set_div(int enabled, double alpha_, const double* svaux)
 {
     require(k_set, TCX_ERR_STATE, "set_div_correction: call tcx_set_kpoints first");
+    invalidate_local();
+    div_enabled = false;
     if (enabled) {
         require(svaux != nullptr, TCX_ERR_ARG, "set_div_correction: sum_of_vaux is null");
-        require(alpha_ > 0.0, TCX_ERR_ARG, "set_div_correction: alpha must be positive");
+        require(std::isfinite(alpha_) && alpha_ > 0.0, TCX_ERR_ARG, "set_div_correction: alpha must be positive");
+        for (int k = 0; k < nk; ++k)
+            require(std::isfinite(svaux[k]), TCX_ERR_ARG, "set_div_correction: non-finite sum_of_vaux");
         alpha = alpha_;
         sum_vaux.assign(svaux, svaux + nk);
         std::vector<double> dc(nk);
@@ -395,7 +430,9 @@ void Context::set_density(const double* in)
 {
     require(states_set, TCX_ERR_STATE, "set_density: call tcx_set_states first");
     require(in != nullptr, TCX_ERR_ARG, "set_density: null pointer");
-    for (long long i = 0; i < 4 * N; ++i) require(std::isfinite(in[i]), TCX_ERR_ARG, "set_density: non-finite value");
+    for (long long i = 0; i < (nspin == 1 ? 2 : 4) * N; ++i)
+        require(std::isfinite(in[i]), TCX_ERR_ARG, "set_density: non-finite value");
+    invalidate_local();
     TCX_HIP_CHECK(hipStreamSynchronize(stream));
     d_density.ensure((size_t)2 * N * 16);
     // one independent spin: density_[1] = density_[0] (bloch_states_scfloop_filling_density.cpp:355-358)
@@ -769,9 +806,7 @@ int tcx_destroy(tcx_context* ctx)
 {
     if (!ctx) return TCX_ERR_ARG;
     (void)hipSetDevice(ctx->impl.device);
-    (void)hipStreamSynchronize(ctx->impl.stream);
-    (void)hipStreamDestroy(ctx->impl.stream);
-    delete ctx;
+    delete ctx;  // ~Context tears down graphs and events, then destroys the stream
     return TCX_OK;
 }
 

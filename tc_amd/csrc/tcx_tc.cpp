@@ -82,29 +82,34 @@ void Context::tc3_local(int which, const int* nb_, const double* phi, double* ou
     const Plan& P = get_plan(nb, 0);  // only the per-band job tables are used
     ApplyIO io = begin_apply(nb_, phi, out, flags, P);
     if (P.n_app > 0) {
-        ensure_density();
         const size_t g = (size_t)N * 16;
-        d_work.ensure(6 * g);
-        double* nG = (double*)d_work.p;             // 2 grids
-        double* dnu = (double*)((char*)d_work.p + 2 * g);  // 3 grids
-        double* V = (double*)((char*)d_work.p + 5 * g);    // 1 grid
-        TCX_HIP_CHECK(hipMemcpyAsync(nG, d_density.p, 2 * g, hipMemcpyDeviceToDevice, stream));
-        launch_fft3d(TCX_PREC_FP64, -1, nG, 2, geo, 1.0 / (double)N, stream);
-        const JastrowRPA j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
         const double NkV = num_kpoints * volume;
-        double factor;
-        launch_dnu(nG, dnu, geo, j0, j1, stream);
-        launch_fft3d(TCX_PREC_FP64, +1, dnu, 3, geo, 1.0, stream);
-        if (which == 0) {
-            launch_sum_squares(dnu, V, N, stream);         // tc3a1.cpp:78-84
-            factor = 0.5 / (NkV * NkV);                    // tc3a1.cpp:26-28
-        } else {
-            launch_mul_shared(dnu, 3, (const double*)d_density.p, N, stream);  // tc3b1.cpp:77-81 (spin 0)
-            launch_fft3d(TCX_PREC_FP64, -1, dnu, 3, geo, 1.0 / (double)N, stream);
-            launch_tc3b1_V(dnu, V, geo, j0, j1, stream);   // tc3b1.cpp:86-100
-            launch_fft3d(TCX_PREC_FP64, +1, V, 1, geo, 1.0, stream);
-            factor = -1.0 / (NkV * NkV);                   // tc3b1.cpp:26-28
+        // tc3a1.cpp:26-28, tc3b1.cpp:26-28
+        const double factor = which == 0 ? 0.5 / (NkV * NkV) : -1.0 / (NkV * NkV);
+        LocalCache& C = which == 0 ? c_tc3a1 : c_tc3b1;
+        if (!C.valid) {
+            ensure_density();
+            d_work.ensure(5 * g);
+            double* nG = (double*)d_work.p;                    // 2 grids
+            double* dnu = (double*)((char*)d_work.p + 2 * g);  // 3 grids
+            C.buf.ensure(g);
+            double* Vc = (double*)C.buf.p;                     // the local potential, real space
+            TCX_HIP_CHECK(hipMemcpyAsync(nG, d_density.p, 2 * g, hipMemcpyDeviceToDevice, stream));
+            launch_fft3d(TCX_PREC_FP64, -1, nG, 2, geo, 1.0 / (double)N, stream);
+            const JastrowRPA j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
+            launch_dnu(nG, dnu, geo, j0, j1, stream);
+            launch_fft3d(TCX_PREC_FP64, +1, dnu, 3, geo, 1.0, stream);
+            if (which == 0) {
+                launch_sum_squares(dnu, Vc, N, stream);        // tc3a1.cpp:78-84
+            } else {
+                launch_mul_shared(dnu, 3, (const double*)d_density.p, N, stream);  // tc3b1.cpp:77-81 (spin 0)
+                launch_fft3d(TCX_PREC_FP64, -1, dnu, 3, geo, 1.0 / (double)N, stream);
+                launch_tc3b1_V(dnu, Vc, geo, j0, j1, stream);  // tc3b1.cpp:86-100
+                launch_fft3d(TCX_PREC_FP64, +1, Vc, 1, geo, 1.0, stream);
+            }
+            C.valid = true;
         }
+        const double* V = (const double*)C.buf.p;
         // H3 phi_kj += factor * FFT_forward[V phi_kj] on the sphere of k (tc3a1.cpp:86-116)
         const int chunk = (int)std::max<size_t>(1, std::min<size_t>(P.n_app, ((size_t)1 << 30) / g));
         d_W.ensure((size_t)chunk * g);

@@ -39,7 +39,8 @@ ProdChan chan(const void* a, int a_sel, int a_mul, int a_off, int a_conj, const 
 const PairSet& Context::pair_set(const std::vector<int>& nb, int kind, int channels, int prec)
 {
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
-    auto key = std::make_tuple(nb, kind * 1000 + channels * 2 + prec);
+    // only the (k,j) x (q,i) kind depends on the applied bands
+    auto key = std::make_tuple(kind == kPairsAppOcc ? nb : std::vector<int>(), kind * 1000 + channels * 2 + prec);
     auto it = pairsets.find(key);
     if (it != pairsets.end()) return it->second;
     const bool irr = kind == kPairsIrrOcc || kind == kPairsIrrSameK;
@@ -246,6 +247,119 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
 }
 
 // ------------------------------------------------------------------ tc3a2a4b2b5
+// The occupied-set phase of tc3a2a4b2b5 (:210-800): the q1-q2 sums Q2, Q4, Q5 in FP64,
+// the [3a2] G=0 correction, the symmetry rotation, the working-precision copies Q4T/Q5T
+// read by the apply phase, and the [3b2]+[3a2] local potential (real space, FP64) into
+// c_a8.buf.
+void Context::a8_occupied_phase(const std::vector<int>& nb, int prec)
+{
+    const bool bitc = method == TCX_METHOD_BITC;
+    const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
+    const size_t g64 = (size_t)N * 16, gT = (size_t)N * es;
+    const double NkV = num_kpoints * volume;
+    const double base = 1.0 / (NkV * NkV);  // :34-48
+    double fa2 = -0.5 * base, fb2 = base;
+    if (nspin == 1) {
+        fa2 /= 4.0;
+        fb2 /= 4.0;
+    }
+    const JastrowRPA j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
+    const void* occR = d_occR[prec].p;
+    const void* chiR = bitc ? d_chiR[prec].p : occR;
+    ensure_occR(TCX_PREC_FP64);
+    const double* occR64 = (const double*)d_occR[TCX_PREC_FP64].p;
+    const double* fill = (const double*)d_fill.p;
+
+    // (i) q1-q2 phase (:210-443): targets (q2,i2) x sources (q1,i1)
+    const int nin = bitc ? 2 : 1;
+    d_Q2.ensure((size_t)n_occ * g64);
+    d_Q4.ensure((size_t)3 * n_occ * g64);
+    launch_zero(d_Q2.p, (size_t)n_occ * g64, stream);
+    launch_zero(d_Q4.p, (size_t)3 * n_occ * g64, stream);
+    double* Q2 = (double*)d_Q2.p;
+    double* Q4 = (double*)d_Q4.p;
+    double* Q5 = nullptr;
+    if (bitc) {
+        d_Q5.ensure((size_t)3 * n_occ * g64);
+        launch_zero(d_Q5.p, (size_t)3 * n_occ * g64, stream);
+        Q5 = (double*)d_Q5.p;
+    }
+    {
+        const PairSet& S = pair_set(nb, sym_set ? kPairsIrrOcc : kPairsOccOcc, 7 * nin, prec);
+        d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 7 * nin * gT);
+        int max_rows = 1;
+        for (const Launch& L : S.launches) max_rows = std::max(max_rows, L.nkq);
+        d_ktab.ensure((size_t)max_rows * gT);
+        ProdChan ch[2];
+        ch[0] = chan(chiR, 1, 1, 0, 1, occR, 0, 1, 0, 0);  // conj(chi_q1) phi_q2 (:356-357)
+        ch[1] = chan(occR, 1, 1, 0, 1, chiR, 0, 1, 0, 0);  // conj(phi_q1) chi_q2 (:358-363)
+        upload(d_chans, ch, sizeof(ch));
+        for (const Launch& L : S.launches) {
+            const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
+            const TargetEntry* targets = (const TargetEntry*)S.d_targets.p + L.target0;
+            const int np = L.npairs;
+            char* rec = (char*)d_rec.p;
+            char* O = rec + (size_t)nin * np * gT;
+            launch_u2_table(prec, d_ktab.p, (const double*)S.d_kq.p, L.kq0, L.nkq, geo, j00, j01, stream);
+            timed(0, np, [&] {
+                launch_products_z(prec, pairs, np, (const ProdChan*)d_chans.p, nin, rec, geo, 1.0 / (double)N,
+                                  stream);
+            });
+            timed(1, np, [&] {
+                launch_xy_a8c1(prec, geo, rec, nin, O, pairs, np, d_ktab.p, L.kq0, (const double*)S.d_kq.p,
+                               stream);
+            });
+            timed(2, np, [&] {  // z pass of FFT_backward fused with the q-sums
+                launch_za8_q(prec, geo, O, bitc ? 1 : 0, occR, chiR, pairs, targets, L.ntargets, fill, Q2, Q4,
+                             bitc ? Q5 : Q4, stream);
+            });
+        }
+    }
+
+    // (ii) [3a2]: T = FFT_forward(qsum_3a2) (:486-494), G=0 correction (:496-573)
+    launch_fft3d(TCX_PREC_FP64, -1, Q2, n_occ, geo, 1.0 / (double)N, stream);
+    if (div_enabled) {
+        if (bitc) ensure_chiR(TCX_PREC_FP64);
+        const double* chiR64 = bitc ? (const double*)d_chiR[TCX_PREC_FP64].p : occR64;
+        std::vector<double> dc2(nk);
+        for (int k = 0; k < nk; ++k) dc2[k] = 1.0 / std::sqrt(kPiHost * alpha) * NkV - sum_vaux[k];
+        d_aux.ensure((size_t)nk * 8);
+        TCX_HIP_CHECK(hipMemcpy(d_aux.p, dc2.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+        const PairSet& S = pair_set(nb, sym_set ? kPairsIrrSameK : kPairsOccSameK, 2, TCX_PREC_FP64);
+        d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 2 * g64);
+        ProdChan ch[2];
+        ch[0] = chan(chiR64, 1, 1, 0, 1, occR64, 0, 1, 0, 0);  // conj(chi_q2,i1) phi_q2,i2 (:545-547)
+        ch[1] = chan(chiR64, 0, 1, 0, 1, occR64, 1, 1, 0, 0);  // conj(chi_q2,i2) phi_q2,i1 (:548-550)
+        upload(d_chans, ch, sizeof(ch));
+        for (const Launch& L : S.launches) {
+            const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
+            const TargetEntry* targets = (const TargetEntry*)S.d_targets.p + L.target0;
+            launch_products_fft(TCX_PREC_FP64, pairs, L.npairs, (const ProdChan*)d_chans.p, 2, d_rec.p, geo,
+                                1.0 / (double)N, stream);
+            launch_a8_div(Q2, (const double*)d_rec.p, pairs, targets, L.ntargets, fill, (const double*)d_aux.p,
+                          geo, N, j00, j01, stream);
+        }
+    }
+    // replicas of the irreducible q2: rotated sums (:576-765)
+    if (sym_set) rotate_qsums(Q2, Q4, bitc ? Q5 : nullptr);
+    // V_3a2 (G) and qsum_3b2 (r) (:637-641, :790-800)
+    d_local.ensure(5 * g64);
+    double* V3a2 = (double*)d_local.p;
+    double* B2 = V3a2 + 2 * N;
+    launch_a8_reduce(Q2, Q4, bitc ? Q5 : Q4, bitc ? 1 : 0, occR64, fill, n_occ, V3a2, B2, N, stream);
+    d_Q4T.ensure((size_t)3 * n_occ * gT);
+    d_Q5T.ensure((size_t)3 * n_occ * gT);
+    launch_a8_q45(prec, Q4, bitc ? Q5 : Q4, bitc ? 1 : 0, d_Q4T.p, d_Q5T.p, 3 * n_occ, N, stream);
+    // [3b2] (:922-992) and [3a2] (:995-1034) as one local potential X (real space)
+    c_a8.buf.ensure(g64);
+    double* X = (double*)c_a8.buf.p;
+    launch_fft3d(TCX_PREC_FP64, -1, B2, 3, geo, 1.0 / (double)N, stream);
+    launch_tc3b1_V(B2, X, geo, j00, j01, stream);  // sum_d G_d (u(G,0,0) + u(G,0,1)) B2_d(G)
+    launch_fft3d(TCX_PREC_FP64, +1, X, 1, geo, 1.0, stream);
+    launch_fft3d(TCX_PREC_FP64, +1, V3a2, 1, geo, 1.0, stream);
+    launch_axpby(X, V3a2, X, fb2, fa2, N, stream);
+}
+
 void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int prec, unsigned flags)
 {
     check_apply_args(nb_, phi, out, prec);
@@ -260,11 +374,9 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
         const size_t g64 = (size_t)N * 16, gT = (size_t)N * es;
         const double NkV = num_kpoints * volume;
         const double base = 1.0 / (NkV * NkV);  // :34-48
-        double fa2 = -0.5 * base, fa4 = base, fb2 = base, fb5 = -base;
+        double fa4 = base, fb5 = -base;
         if (nspin == 1) {
-            fa2 /= 4.0;
             fa4 /= 4.0;
-            fb2 /= 4.0;
             fb5 /= 4.0;
         }
         const JastrowRPA j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
@@ -272,91 +384,17 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
         if (bitc) ensure_chiR(prec);
         const void* occR = d_occR[prec].p;
         const void* chiR = bitc ? d_chiR[prec].p : occR;
-        ensure_occR(TCX_PREC_FP64);
-        const double* occR64 = (const double*)d_occR[TCX_PREC_FP64].p;
         const double* fill = (const double*)d_fill.p;
-
-        // (i) q1-q2 phase (:210-443): targets (q2,i2) x sources (q1,i1)
-        const int nin = bitc ? 2 : 1;
-        d_Q2.ensure((size_t)n_occ * g64);
-        d_Q4.ensure((size_t)3 * n_occ * g64);
-        launch_zero(d_Q2.p, (size_t)n_occ * g64, stream);
-        launch_zero(d_Q4.p, (size_t)3 * n_occ * g64, stream);
-        double* Q2 = (double*)d_Q2.p;
-        double* Q4 = (double*)d_Q4.p;
-        double* Q5 = nullptr;
-        if (bitc) {
-            d_Q5.ensure((size_t)3 * n_occ * g64);
-            launch_zero(d_Q5.p, (size_t)3 * n_occ * g64, stream);
-            Q5 = (double*)d_Q5.p;
+        // Everything up to the (k,j) x (q2,i2) apply depends on the occupied set only: the
+        // q-sums in the working precision (Q4T, Q5T) and the [3b2]+[3a2] local potential are
+        // kept for later applies until the states, Jastrow, divergence correction or
+        // symmetry change.
+        if (!(c_a8.valid && c_a8.prec == prec)) {
+            c_a8.valid = false;
+            a8_occupied_phase(nb, prec);
+            c_a8.valid = true;
+            c_a8.prec = prec;
         }
-        {
-            const PairSet& S = pair_set(nb, sym_set ? kPairsIrrOcc : kPairsOccOcc, 7 * nin, prec);
-            d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 7 * nin * gT);
-            int max_rows = 1;
-            for (const Launch& L : S.launches) max_rows = std::max(max_rows, L.nkq);
-            d_ktab.ensure((size_t)max_rows * gT);
-            ProdChan ch[2];
-            ch[0] = chan(chiR, 1, 1, 0, 1, occR, 0, 1, 0, 0);  // conj(chi_q1) phi_q2 (:356-357)
-            ch[1] = chan(occR, 1, 1, 0, 1, chiR, 0, 1, 0, 0);  // conj(phi_q1) chi_q2 (:358-363)
-            upload(d_chans, ch, sizeof(ch));
-            for (const Launch& L : S.launches) {
-                const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
-                const TargetEntry* targets = (const TargetEntry*)S.d_targets.p + L.target0;
-                const int np = L.npairs;
-                char* rec = (char*)d_rec.p;
-                char* O = rec + (size_t)nin * np * gT;
-                launch_u2_table(prec, d_ktab.p, (const double*)S.d_kq.p, L.kq0, L.nkq, geo, j00, j01, stream);
-                timed(0, np, [&] {
-                    launch_products_z(prec, pairs, np, (const ProdChan*)d_chans.p, nin, rec, geo, 1.0 / (double)N,
-                                      stream);
-                });
-                timed(1, np, [&] {
-                    launch_xy_a8c1(prec, geo, rec, nin, O, pairs, np, d_ktab.p, L.kq0, (const double*)S.d_kq.p,
-                                   stream);
-                });
-                timed(2, np, [&] {  // z pass of FFT_backward fused with the q-sums
-                    launch_za8_q(prec, geo, O, bitc ? 1 : 0, occR, chiR, pairs, targets, L.ntargets, fill, Q2, Q4,
-                                 bitc ? Q5 : Q4, stream);
-                });
-            }
-        }
-
-        // (ii) [3a2]: T = FFT_forward(qsum_3a2) (:486-494), G=0 correction (:496-573)
-        launch_fft3d(TCX_PREC_FP64, -1, Q2, n_occ, geo, 1.0 / (double)N, stream);
-        if (div_enabled) {
-            if (bitc) ensure_chiR(TCX_PREC_FP64);
-            const double* chiR64 = bitc ? (const double*)d_chiR[TCX_PREC_FP64].p : occR64;
-            std::vector<double> dc2(nk);
-            for (int k = 0; k < nk; ++k) dc2[k] = 1.0 / std::sqrt(kPiHost * alpha) * NkV - sum_vaux[k];
-            d_aux.ensure((size_t)nk * 8);
-            TCX_HIP_CHECK(hipMemcpy(d_aux.p, dc2.data(), nk * sizeof(double), hipMemcpyHostToDevice));
-            const PairSet& S = pair_set(nb, sym_set ? kPairsIrrSameK : kPairsOccSameK, 2, TCX_PREC_FP64);
-            d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 2 * g64);
-            ProdChan ch[2];
-            ch[0] = chan(chiR64, 1, 1, 0, 1, occR64, 0, 1, 0, 0);  // conj(chi_q2,i1) phi_q2,i2 (:545-547)
-            ch[1] = chan(chiR64, 0, 1, 0, 1, occR64, 1, 1, 0, 0);  // conj(chi_q2,i2) phi_q2,i1 (:548-550)
-            upload(d_chans, ch, sizeof(ch));
-            for (const Launch& L : S.launches) {
-                const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
-                const TargetEntry* targets = (const TargetEntry*)S.d_targets.p + L.target0;
-                launch_products_fft(TCX_PREC_FP64, pairs, L.npairs, (const ProdChan*)d_chans.p, 2, d_rec.p, geo,
-                                    1.0 / (double)N, stream);
-                launch_a8_div(Q2, (const double*)d_rec.p, pairs, targets, L.ntargets, fill, (const double*)d_aux.p,
-                              geo, N, j00, j01, stream);
-            }
-        }
-        // replicas of the irreducible q2: rotated sums (:576-765)
-        if (sym_set) rotate_qsums(Q2, Q4, bitc ? Q5 : nullptr);
-        // V_3a2 (G) and qsum_3b2 (r) (:637-641, :790-800)
-        d_local.ensure(5 * g64);
-        double* V3a2 = (double*)d_local.p;
-        double* B2 = V3a2 + 2 * N;
-        launch_a8_reduce(Q2, Q4, bitc ? Q5 : Q4, bitc ? 1 : 0, occR64, fill, n_occ, V3a2, B2, N, stream);
-        d_Q4T.ensure((size_t)3 * n_occ * gT);
-        d_Q5T.ensure((size_t)3 * n_occ * gT);
-        launch_a8_q45(prec, Q4, bitc ? Q5 : Q4, bitc ? 1 : 0, d_Q4T.p, d_Q5T.p, 3 * n_occ, N, stream);
-
         // [3a4], [3b5] applied to (k,j) (:801-913): targets (k,j) x sources (q2,i2)
         d_appR.ensure((size_t)P.n_app * gT);
         orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p);
@@ -393,15 +431,8 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
             }
         }
         finish_W(P, io, 1.0);
-
         // [3b2] (:922-992) and [3a2] (:995-1034) as one local potential
-        double* X = B2 + 6 * N;
-        launch_fft3d(TCX_PREC_FP64, -1, B2, 3, geo, 1.0 / (double)N, stream);
-        launch_tc3b1_V(B2, X, geo, j00, j01, stream);  // sum_d G_d (u(G,0,0) + u(G,0,1)) B2_d(G)
-        launch_fft3d(TCX_PREC_FP64, +1, X, 1, geo, 1.0, stream);
-        launch_fft3d(TCX_PREC_FP64, +1, V3a2, 1, geo, 1.0, stream);
-        launch_axpby(X, V3a2, X, fb2, fa2, N, stream);
-        apply_local(P, io, X, 1.0);
+        apply_local(P, io, (const double*)c_a8.buf.p, 1.0);
     }
     end_apply(io);
 }
@@ -470,12 +501,16 @@ void Context::hf2h(const int* nb_, const double* phi, double* out, unsigned flag
     ApplyIO io = begin_apply(nb_, phi, out, flags, P);
     if (P.n_app > 0 && n_occ > 0) {
         const size_t g = (size_t)N * 16;
-        d_local.ensure(3 * g);
-        double* nG = (double*)d_local.p;
-        double* V = nG + 4 * N;
-        density_G(nG);
-        launch_hartree(nG, V, geo, N, stream);  // hf2h.cpp:28-38
-        launch_fft3d(TCX_PREC_FP64, +1, V, 1, geo, 1.0, stream);
+        if (!c_hartree.valid) {
+            d_local.ensure(2 * g);
+            double* nG = (double*)d_local.p;
+            c_hartree.buf.ensure(g);
+            density_G(nG);
+            launch_hartree(nG, (double*)c_hartree.buf.p, geo, N, stream);  // hf2h.cpp:28-38
+            launch_fft3d(TCX_PREC_FP64, +1, c_hartree.buf.p, 1, geo, 1.0, stream);
+            c_hartree.valid = true;
+        }
+        const double* V = (const double*)c_hartree.buf.p;
         apply_local(P, io, V, 1.0 / (num_kpoints * volume));  // two_body_factor, hf2h.cpp:23
     }
     end_apply(io);
@@ -497,20 +532,24 @@ void Context::tc2h(const int* nb_, const double* phi, double* out, unsigned flag
         ensure_gradR(TCX_PREC_FP64);
         if (bitc) ensure_chiR(TCX_PREC_FP64);
         const double* chiR = (const double*)(bitc ? d_chiR[TCX_PREC_FP64].p : d_occR[TCX_PREC_FP64].p);
-        d_local.ensure(9 * g);
-        double* nG = (double*)d_local.p;   // 2 grids
-        double* pgp = nG + 4 * N;          // 3 grids
-        double* V2 = nG + 10 * N;          // 1 grid
-        double* dnu = nG + 12 * N;         // 3 grids
-        // phiq_gphiq (tc2h.cpp:60-127)
-        launch_tc2h_V(chiR, (const double*)d_gradR[TCX_PREC_FP64].p, (const double*)d_fill.p, n_occ, pgp, geo, N,
-                      stream);
-        launch_fft3d(TCX_PREC_FP64, -1, pgp, 3, geo, 1.0 / (double)N, stream);
-        density_G(nG);
-        launch_tc2h_combine(nG, pgp, V2, geo, N, j00, j01, stream);  // tc2h.cpp:146-171
-        launch_dnu(nG, dnu, geo, j00, j01, stream);
-        launch_fft3d(TCX_PREC_FP64, +1, V2, 1, geo, 1.0, stream);
-        launch_fft3d(TCX_PREC_FP64, +1, dnu, 3, geo, 1.0, stream);
+        c_tc2h.buf.ensure(4 * g);
+        double* V2 = (double*)c_tc2h.buf.p;  // 1 grid
+        double* dnu = V2 + 2 * N;            // 3 grids
+        if (!c_tc2h.valid) {
+            d_local.ensure(5 * g);
+            double* nG = (double*)d_local.p;   // 2 grids
+            double* pgp = nG + 4 * N;          // 3 grids
+            // phiq_gphiq (tc2h.cpp:60-127)
+            launch_tc2h_V(chiR, (const double*)d_gradR[TCX_PREC_FP64].p, (const double*)d_fill.p, n_occ, pgp, geo,
+                          N, stream);
+            launch_fft3d(TCX_PREC_FP64, -1, pgp, 3, geo, 1.0 / (double)N, stream);
+            density_G(nG);
+            launch_tc2h_combine(nG, pgp, V2, geo, N, j00, j01, stream);  // tc2h.cpp:146-171
+            launch_dnu(nG, dnu, geo, j00, j01, stream);
+            launch_fft3d(TCX_PREC_FP64, +1, V2, 1, geo, 1.0, stream);
+            launch_fft3d(TCX_PREC_FP64, +1, dnu, 3, geo, 1.0, stream);
+            c_tc2h.valid = true;
+        }
         // per band: FFT_forward[V2 phi_j - sum_d gphi_j,d dnu_d] (tc2h.cpp:178-234)
         const double factor = 1.0 / (num_kpoints * volume);  // tc2h.cpp:32
         const int chunk = (int)std::max<size_t>(1, std::min<size_t>(P.n_app, ((size_t)1 << 30) / (4 * g)));

@@ -129,6 +129,7 @@ struct Context {
     bool div_enabled = false;
     double alpha = 0.0;
     std::vector<double> sum_vaux;
+    long long states_version = 0;  // counts tcx_set_states calls
     // k-mesh symmetry (tcx_set_kpoint_symmetry): sym_src[k] = irreducible k of k,
     // sym_ops[k] = its rotation (dst/src unused); off = identity symmetry
     bool sym_set = false;
@@ -157,6 +158,19 @@ struct Context {
     std::map<std::tuple<std::vector<int>, int>, PairSet> pairsets;
     DBuf d_qsum, d_qsumT, d_rec, d_Q2, d_Q4, d_Q5, d_Q4T, d_Q5T, d_appR64, d_chans, d_aux, d_local;
 
+    // Intermediates that depend on the occupied set (and density, Jastrow, divergence
+    // correction, k-mesh symmetry) but not on the applied bands, kept across applies:
+    // tc3a2a4b2b5's q-sum products Q4T/Q5T in the working precision `prec` plus its
+    // [3b2]+[3a2] local potential (one FP64 grid in d_a8X), and the FP64 real-space local
+    // potentials of hf2h (1 grid), tc2h (V2 and dnu: 4 grids), tc3a1 and tc3b1 (1 each).
+    struct LocalCache {
+        DBuf buf;
+        bool valid = false;
+        int prec = -1;
+    };
+    LocalCache c_a8, c_hartree, c_tc2h, c_tc3a1, c_tc3b1;
+    void invalidate_local();
+
     // HIP graphs captured from this context's stream (tcx_graph_*)
     std::vector<hipGraphExec_t> graphs;
 
@@ -167,6 +181,9 @@ struct Context {
     long t_pairs[4] = {0, 0, 0, 0};
     std::vector<PendingEvent> pending;
 
+    // Graphs and events go first, then the stream they were recorded on; the
+    // workspaces (DBuf members) are released after this body, with a device-wide
+    // synchronisation each.
     ~Context()
     {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -175,10 +192,14 @@ struct Context {
                 (void)hipGraphExecDestroy(g);
                 graph_destroyed();
             }
+        graphs.clear();
         for (auto& ev : pending) {
             (void)hipEventDestroy(ev.a);
             (void)hipEventDestroy(ev.b);
         }
+        pending.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
     }
 
     static void require(bool cond, int code, const char* what);
@@ -229,6 +250,7 @@ struct Context {
     void density_G(double* nG);
     void tc3a3b3b4b6(const int* nb, const double* phi, double* out, int prec, unsigned flags);
     void tc3a2a4b2b5(const int* nb, const double* phi, double* out, int prec, unsigned flags);
+    void a8_occupied_phase(const std::vector<int>& nb, int prec);
     void kinetic(const int* nb, const double* phi, double* out, unsigned flags);
     void hf2h(const int* nb, const double* phi, double* out, unsigned flags);
     void tc2h(const int* nb, const double* phi, double* out, unsigned flags);

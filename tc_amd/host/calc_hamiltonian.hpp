@@ -83,6 +83,10 @@ struct BlochStates {
     Phi4 phik_left_scf;  // BITC left orbitals (same shape) or empty
     std::vector<std::vector<std::vector<double>>> filling;  // [spin][k][band]
     std::vector<std::vector<int>> num_occupied_bands;        // [spin][k]
+    // BlochStates::density_ (src/bloch_states_scfloop_filling_density.cpp:256-358): the
+    // real-space density [spin][N] the terms read, e.g. after density mixing.  Empty: the
+    // density of the occupied orbitals above.
+    std::vector<VectorXcd> density;
 };
 
 // Symmetry::rotation / translation (src/symmetry.cpp:17-33): TC++ rotations U

@@ -8,6 +8,7 @@
 // Also checks tc2x with A = 0 against hf2x (src/jastrow_functions.cpp:30-38) and
 // calc_hamiltonian::all (kinetic closed form, Hartree = 0 for a uniform density).
 // Exit status 0 on success.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -107,6 +108,31 @@ int main()
     }
     std::printf("hf2x closed form: max |err| = %.3e (scale %.3e)\n", err, scale);
     if (!(err <= 1e-11 * scale)) return 1;
+
+    // The mirror pushes state only when it changed: the same call again (state
+    // reused) gives the same H, and halving every filling (state re-pushed) halves it,
+    // since Vx is linear in the fillings.
+    {
+        Phi4 Ha = H, Hb = H;
+        for (auto& k : Ha[0])
+            for (auto& b : k) std::fill(b[0].begin(), b[0].end(), Complex(0.0));
+        for (auto& k : Hb[0])
+            for (auto& b : k) std::fill(b[0].begin(), b[0].end(), Complex(0.0));
+        calc_hamiltonian::hf2x(par, method, cs, pot, spin, kp, pw, bs, phi, Ha, nullptr);
+        BlochStates half = bs;
+        for (auto& k : half.filling[0])
+            for (double& f : k) f *= 0.5;
+        calc_hamiltonian::hf2x(par, method, cs, pot, spin, kp, pw, half, phi, Hb, nullptr);
+        double e_same = 0.0, e_half = 0.0;
+        for (int k = 0; k < 3; ++k)
+            for (size_t j = 0; j < H[0][k].size(); ++j)
+                for (size_t l = 0; l < H[0][k][j][0].size(); ++l) {
+                    e_same = std::fmax(e_same, std::abs(Ha[0][k][j][0][l] - H[0][k][j][0][l]));
+                    e_half = std::fmax(e_half, std::abs(Hb[0][k][j][0][l] - 0.5 * H[0][k][j][0][l]));
+                }
+        std::printf("repeated call: max |dH| = %.3e; halved fillings: max |H' - H/2| = %.3e\n", e_same, e_half);
+        if (!(e_same == 0.0 && e_half <= 1e-12 * scale)) return 1;
+    }
 
     // tc2x without a Jastrow equals hf2x; accumulation adds onto H
     Phi4 H2 = H;

@@ -74,18 +74,24 @@ def test_three_body_exchange_against_oracle_12(engine, method):
 @pytest.mark.parametrize("method", ["TC", "BITC"])
 def test_three_body_exchange_against_oracle_6(engine, method):
     """NZ = 6 is not a multiple of 4: every z consumer splits its pencils over 2 lanes,
-    k_za7_mid with the lane-swap exchange."""
+    k_za7_mid with the lane-swap exchange, and so do the FP64 instances of tc2x passes A
+    and C and hf2x pass C."""
     sys_ = S.tiny_system(method=method, n=(6, 6, 6), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=53,
                          lattice=S.sc_lattice(5.0))
     engine.load_system(sys_)
     nb, phi = S.applied_bands(sys_, nb=[3, 2], seed=54)
     o = O.Oracle(sys_)
-    for term in THREE_BODY_X:
+    for term in ("tc2x",) + THREE_BODY_X:
         print(f"{method} {term} 6^3", flush=True)
         ref = getattr(o, term)(nb, phi)
         assert _close(getattr(engine, term)(nb, phi, precision=tcx.PREC_FP64), ref), term
         got32 = getattr(engine, term)(nb, phi, precision=tcx.PREC_FP32)
         assert per_band_rel(sys_, nb, got32, ref).max() <= 1e-4, term
+    hf = S.System(**{**sys_.__dict__, "method": "HF", "chi_occ": None})
+    engine.load_system(hf)
+    ref = O.Oracle(hf).hf2x(nb, phi)
+    assert _close(engine.hf2x(nb, phi, precision=tcx.PREC_FP64), ref), "hf2x"
+    assert per_band_rel(hf, nb, engine.hf2x(nb, phi, precision=tcx.PREC_FP32), ref).max() <= 3e-5, "hf2x"
 
 
 def test_three_body_exchange_without_divergence_correction(engine):
