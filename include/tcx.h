@@ -240,19 +240,24 @@ int tcx_graph_destroy(tcx_context *ctx, int graph_id);
 /* ---------------------------------------------------------------- consumer */
 /* Block Davidson at fixed occupied orbitals (one Fock operator): replaces
  * Diagonalization::block_davidson (src/include/diagonalization.hpp; src/diagonalization_davidson.cpp:7-689)
- * with make_a_new_trial_vector, Gram_Schmidt and sort_eigen
- * (src/diagonalization_utils.cpp:7-29, 79-118, 182-210).  H*phi is tcx_apply_all on
+ * with make_a_new_trial_vector, Gram_Schmidt, Gram_Schmidt_biortho, sort_eigen and
+ * sort_eigen_overlap (src/diagonalization_utils.cpp:7-211).  H*phi is tcx_apply_all on
  * the GPU in `precision`; the subspace algebra (Gram-Schmidt, preconditioner, V^H H V,
  * the per-k eigensolve, rotations) runs in FP64 on the GPU too.  Host pointers:
  *   phi      in/out complex128 ragged [k][nb[k]][npw_k]: trial bands in (orthonormal per k,
- *            as the reference's bands are; BITC bands are orthonormalised on entry),
- *            eigenvectors out;
- *   phi_left out (BITC only, else may be NULL): left eigenvectors V (X^-1)^H;
+ *            as the reference's bands are; BITC bands are orthonormalised on entry unless
+ *            biortho), eigenvectors out;
+ *   phi_left BITC only, else may be NULL: left eigenvectors out, V (X^-1)^H; with biortho
+ *            also in: the left trial bands (bi-orthonormal to phi, as phik_left is);
  *   eig      in/out complex128 [sum nb]: residual shifts in (the previous eigenvalues),
- *            eigenvalues out, ascending real part;
+ *            eigenvalues out, ascending real part (biortho: ordered by overlap with the
+ *            previous bands);
  *   energies out complex128 [sum nb][3]: e_n = diag(X_L^H H_n X), n = 1, 2, 3.
  * max_blocks >= 2 and num_refresh >= 1 are the reference's max_num_blocks_david and
- * num_refresh_david (defaults 2 and 1); biortho must be 0 (biortho_david = false). */
+ * num_refresh_david (defaults 2 and 1); biortho = 1 is biortho_david = true (BITC only):
+ * left trial vectors grown from H^dagger (u -> -u, right and left occupied orbitals
+ * swapped, the density kept; :234-251), bi-orthogonal Gram-Schmidt, subspace
+ * <V_left|H|V>. */
 int tcx_davidson(tcx_context *ctx, const int *nb, double *phi, double *phi_left, double *eig,
                  double *energies, int max_blocks, int num_refresh, int biortho, int precision);
 

@@ -9,7 +9,8 @@
 // 64-lane workgroup per k with its matrices in LDS: Householder reduction to Hessenberg
 // form, single-shift QR with Wilkinson shifts to the complex Schur form (Golub & Van Loan,
 // Matrix Computations, Alg. 7.4.2 / 7.5), back-substitution for the eigenvectors, the sort
-// by real part and, for BITC, X_L = (X^-1)^H by Gauss-Jordan.  The host version of the
+// by real part (or by overlap, bi-orthogonal variant) and, for BITC, X_L = (X^-1)^H by
+// Gauss-Jordan.  The host version of the
 // same eigensolver is eig_dense (tcx_davidson.cpp).
 #include "kernels_common.hpp"
 
@@ -113,6 +114,8 @@ __global__ __launch_bounds__(kRed) void k_dv_precond(const double* __restrict__ 
 }
 
 // c[k][i] = <A_i | B_bidx[k]> for i < nA[k]; k with bidx[k] < 0 is skipped.  Grid (max nA, nk).
+// (A is the dual set D of launch_dv_project: the set itself for Gram_Schmidt, the other
+// side's set for Gram_Schmidt_biortho.)
 __global__ __launch_bounds__(kRed) void k_dv_dots(const double* __restrict__ A, const long long* __restrict__ aoff,
                                                   const int* __restrict__ nA, const double* __restrict__ B,
                                                   const long long* __restrict__ boff, const int* __restrict__ bidx,
@@ -192,7 +195,8 @@ __global__ __launch_bounds__(kRed) void k_dv_copy(const double* __restrict__ S, 
        ld(S, soff[k] + (long long)((sfirst ? sfirst[k] : 0) + b) * n + l));
 }
 
-// G_q[k][i][j] = <A_i | B_q,j> for i, j < nA[k], q < nB (B_q are sets with A's offsets).
+// G_q[k][i][j] = <A_i | B_q,j> for i, j < nA[k], q < nB (B_q are sets with A's offsets;
+// A is V, or V_left in the bi-orthogonal variant, :465-466).
 // Grid (gs * gs, nk, nB); G_q[k] at G + 2 * (q * nk + k) * gs * gs.
 __global__ __launch_bounds__(kRed) void k_dv_gram(const double* __restrict__ A, const long long* __restrict__ aoff,
                                                   const int* __restrict__ nA, const double* __restrict__ B0,
@@ -235,6 +239,61 @@ __global__ __launch_bounds__(kRed) void k_dv_rotate(const double* __restrict__ A
     }
 }
 
+// c[k] = <X_xidx[k] | Y_yidx[k]> (X at xoff, Y at yoff); k with xidx[k] < 0 is skipped.  Grid nk.
+__global__ __launch_bounds__(kRed) void k_dv_dot(const double* __restrict__ X, const long long* __restrict__ xoff,
+                                                 const int* __restrict__ xidx, const double* __restrict__ Y,
+                                                 const long long* __restrict__ yoff, const int* __restrict__ yidx,
+                                                 const int* __restrict__ npw, double* __restrict__ c)
+{
+    __shared__ double red[2 * kRed];
+    const int k = blockIdx.x;
+    if (xidx[k] < 0) return;
+    const int n = npw[k];
+    const long long x0 = xoff[k] + (long long)xidx[k] * n, y0 = yoff[k] + (long long)yidx[k] * n;
+    z2 s{0.0, 0.0};
+    for (int l = threadIdx.x; l < n; l += kRed) s = zadd(s, zmul(zconj(ld(X, x0 + l)), ld(Y, y0 + l)));
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) st(c, k, s);
+}
+
+// O_odst[k] = B_bidx[k] / conj(c[k]) (Gram_Schmidt_biortho's <t|t> = 1 on the left vector,
+// diagonalization_utils.cpp:69-76).  Grid (ceil(max npw / 256), nk).
+__global__ __launch_bounds__(kRed) void k_dv_cdiv_store(const double* __restrict__ B, const long long* __restrict__ boff,
+                                                        const int* __restrict__ bidx, const double* __restrict__ c,
+                                                        double* __restrict__ O, const long long* __restrict__ ooff,
+                                                        const int* __restrict__ odst, const int* __restrict__ npw)
+{
+    const int k = blockIdx.y;
+    const int n = npw[k];
+    const int l = blockIdx.x * kRed + threadIdx.x;
+    if (l >= n || bidx[k] < 0 || odst[k] < 0) return;
+    const z2 v = ld(B, boff[k] + (long long)bidx[k] * n + l);
+    st(O, ooff[k] + (long long)odst[k] * n + l, zdiv(v, zconj(ld(c, k))));
+}
+
+// Bi-orthogonal pair normalisation after the rotation (diagonalization_davidson.cpp:609-615):
+// for b < nb[k], nrm = ||R_b||, R_b /= nrm, L_b *= nrm, out[band0[k] + b] = nrm.  Grid (nk, max nb).
+__global__ __launch_bounds__(kRed) void k_dv_binorm(double* __restrict__ R, double* __restrict__ L,
+                                                    const long long* __restrict__ off, const int* __restrict__ nb,
+                                                    const int* __restrict__ band0, const int* __restrict__ npw,
+                                                    double* __restrict__ out)
+{
+    __shared__ double red[2 * kRed];
+    const int k = blockIdx.x, b = blockIdx.y;
+    if (b >= nb[k]) return;  // uniform per workgroup
+    const int n = npw[k];
+    const long long o = off[k] + (long long)b * n;
+    double s = 0.0;
+    for (int l = threadIdx.x; l < n; l += kRed) s += znorm(ld(R, o + l));
+    const double nrm = sqrt(block_sum(z2{s, 0.0}, red).re);
+    if (threadIdx.x == 0) out[band0[k] + b] = nrm;
+    if (!(nrm >= 1e-10)) return;  // the host reports the breakdown
+    for (int l = threadIdx.x; l < n; l += kRed) {
+        st(R, o + l, zscale(ld(R, o + l), 1.0 / nrm));
+        st(L, o + l, zscale(ld(L, o + l), nrm));
+    }
+}
+
 // ----------------------------------------------------------------- subspace problem, one workgroup per k
 constexpr int kEigLanes = 64;
 
@@ -244,13 +303,15 @@ __host__ __device__ inline size_t dv_eig_bytes(int gs) { return (size_t)(3 * gs 
 __host__ __device__ inline size_t dv_eig_stride(int gs) { return dv_eig_bytes(gs) / 8 + 1; }  // in doubles
 
 // Hs_q[k] (q < nH) summed and eigen-decomposed: w[k] ascending by real part (sort_eigen,
-// diagonalization_utils.cpp:79-118), X[k] unit-norm right eigenvectors (columns); for
+// diagonalization_utils.cpp:79-118) or, with sort_overlap, ordered by the largest component
+// (sort_eigen_overlap, :121-167), X[k] unit-norm right eigenvectors (columns); for
 // want_left X_L[k] = (X^-1)^H (diagonalization_davidson.cpp:498); with en non-null,
 // en[band0[k] + b][q] = (X_L^H Hs_q X)_bb for b < nb[k] (X_L = X unless want_left,
 // :571-585).  status[k] = 0 ok, 1 QR did not converge, 2 singular eigenvector matrix.
 // Every lane takes the same decisions from the same shared values.
 __global__ __launch_bounds__(kEigLanes) void k_dv_eig(const double* __restrict__ Hs, int nH,
                                                       const int* __restrict__ dim, int gs, int want_left,
+                                                      int sort_overlap,
                                                       double* __restrict__ w, double* __restrict__ X,
                                                       double* __restrict__ XL, double* __restrict__ gwork, int use_lds,
                                                       const int* __restrict__ nb, const int* __restrict__ band0,
@@ -436,24 +497,42 @@ __global__ __launch_bounds__(kEigLanes) void k_dv_eig(const double* __restrict__
     }
     __threadfence_block();
     __syncthreads();
-    // sort by ascending real part in the reference's exchange order
     if (t == 0) {
         for (int i = 0; i < n; ++i) perm[i] = i;
-        for (int i = 0; i < n; ++i)
-            for (int j = i + 1; j < n; ++j)
-                if (ld(wv, i).re > ld(wv, j).re) {
-                    const z2 u = ld(wv, i);
-                    st(wv, i, ld(wv, j));
-                    st(wv, j, u);
-                    const int p = perm[i];
-                    perm[i] = perm[j];
-                    perm[j] = p;
+        if (!sort_overlap) {
+            // sort by ascending real part in the reference's exchange order (sort_eigen)
+            for (int i = 0; i < n; ++i)
+                for (int j = i + 1; j < n; ++j)
+                    if (ld(wv, perm[i]).re > ld(wv, perm[j]).re) {
+                        const int p = perm[i];
+                        perm[i] = perm[j];
+                        perm[j] = p;
+                    }
+        } else {
+            // sort_eigen_overlap (diagonalization_utils.cpp:121-167): position i takes the
+            // not yet chosen eigenvector with the largest |X(i, j)|^2 (the first on ties)
+            for (int i = 0; i < n; ++i) {
+                double best = 0.0;
+                int js = -1, free0 = -1;
+                for (int j = 0; j < n; ++j) {
+                    bool used = false;
+                    for (int q = 0; q < i; ++q) used = used || perm[q] == j;
+                    if (used) continue;
+                    if (free0 < 0) free0 = j;
+                    const double a = znorm(ld(Xk, h(i, j)));
+                    if (best < a) {
+                        best = a;
+                        js = j;
+                    }
                 }
+                perm[i] = js >= 0 ? js : free0;  // an all-zero row keeps the first free column
+            }
+        }
     }
     __syncthreads();
     for (int i = t; i < n; i += kEigLanes) {  // Hm <- X with sorted columns, lane per row
         for (int c = 0; c < n; ++c) st(Hm, h(i, c), ld(Xk, h(i, perm[c])));
-        st(wk, i, ld(wv, i));
+        st(wk, i, ld(wv, perm[i]));
     }
     __syncthreads();
     for (int i = t; i < n; i += kEigLanes)
@@ -544,11 +623,12 @@ void launch_dv_precond(const double* V, const long long* voff, const int* first,
 
 void launch_dv_project(const double* A, const long long* aoff, const int* nA, int max_nA, double* B,
                        const long long* boff, const int* bidx, const int* npw, int max_npw, int nk, double* c,
-                       double* nrm, hipStream_t s)
+                       double* nrm, hipStream_t s, const double* D)
 {
     if (nk <= 0) return;
     if (max_nA > 0) {
-        hipLaunchKernelGGL(k_dv_dots, dim3(max_nA, nk), dim3(kRed), 0, s, A, aoff, nA, B, boff, bidx, npw, c, max_nA);
+        hipLaunchKernelGGL(k_dv_dots, dim3(max_nA, nk), dim3(kRed), 0, s, D ? D : A, aoff, nA, B, boff, bidx, npw, c,
+                           max_nA);
         TCX_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_dv_subtract, dim3(blocks_of(max_npw), nk), dim3(kRed), 0, s, A, aoff, nA, B, boff, bidx,
                            npw, c, max_nA);
@@ -597,13 +677,39 @@ size_t dv_eig_work_bytes(int gs, int nk) { return (size_t)nk * dv_eig_stride(gs)
 
 bool dv_eig_fits_lds(int gs) { return dv_eig_bytes(gs) <= 64 * 1024; }
 
-void launch_dv_eig(const double* Hs, int nH, const int* dim, int gs, int nk, int want_left, double* w, double* X,
-                   double* XL, double* gwork, const int* nb, const int* band0, double* en, int* status, hipStream_t s)
+void launch_dv_eig(const double* Hs, int nH, const int* dim, int gs, int nk, int want_left, int sort_overlap,
+                   double* w, double* X, double* XL, double* gwork, const int* nb, const int* band0, double* en,
+                   int* status, hipStream_t s)
 {
     if (nk <= 0) return;
     const bool lds = dv_eig_fits_lds(gs);
     hipLaunchKernelGGL(k_dv_eig, dim3(nk), dim3(kEigLanes), lds ? dv_eig_bytes(gs) : 0, s, Hs, nH, dim, gs, want_left,
-                       w, X, XL, gwork, lds ? 1 : 0, nb, band0, en, status);
+                       sort_overlap, w, X, XL, gwork, lds ? 1 : 0, nb, band0, en, status);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dv_dot(const double* X, const long long* xoff, const int* xidx, const double* Y, const long long* yoff,
+                   const int* yidx, const int* npw, int nk, double* c, hipStream_t s)
+{
+    if (nk <= 0) return;
+    hipLaunchKernelGGL(k_dv_dot, dim3(nk), dim3(kRed), 0, s, X, xoff, xidx, Y, yoff, yidx, npw, c);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dv_cdiv_store(const double* B, const long long* boff, const int* bidx, const double* c, double* O,
+                          const long long* ooff, const int* odst, const int* npw, int max_npw, int nk, hipStream_t s)
+{
+    if (nk <= 0) return;
+    hipLaunchKernelGGL(k_dv_cdiv_store, dim3(blocks_of(max_npw), nk), dim3(kRed), 0, s, B, boff, bidx, c, O, ooff,
+                       odst, npw);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_dv_binorm(double* R, double* L, const long long* off, const int* nb, int max_nb, const int* band0,
+                      const int* npw, int nk, double* out, hipStream_t s)
+{
+    if (nk <= 0 || max_nb <= 0) return;
+    hipLaunchKernelGGL(k_dv_binorm, dim3(nk, max_nb), dim3(kRed), 0, s, R, L, off, nb, band0, npw, out);
     TCX_HIP_CHECK(hipGetLastError());
 }
 

@@ -62,6 +62,7 @@ void Context::swap_left_right()
         std::swap(d_occR[p], d_chiR[p]);
         std::swap(occR_valid[p], chiR_valid[p]);
         gradR_valid[p] = false;  // gradients are of the right orbitals
+        a_gradR_valid[p] = false;
     }
     density_valid = false;       // rebuilt from the swapped pair: conj of the density
     invalidate_local();

@@ -204,10 +204,33 @@ void Context::set_basis(const int* npw_, const int* gindex)
 
 void Context::invalidate_local()
 {
-    for (LocalCache* c : {&c_a8, &c_hartree, &c_tc2h, &c_tc3a1, &c_tc3b1}) {
+    for (LocalCache* c : {&c_a8, &c_hartree, &c_tc2h, &c_tc3a1, &c_tc3b1, &a_a8, &a_hartree, &a_tc2h, &a_tc3a1,
+                          &a_tc3b1}) {
         c->valid = false;
         c->prec = -1;
     }
+}
+
+void Context::hermitian_conj()
+{
+    // the density of the unswapped orbitals stays (built first if it was not)
+    if (!conj_mode) ensure_density();
+    for (double& a : A) a = -a;
+    std::swap(d_phi_occ, d_chi_occ);
+    for (int p = 0; p < 2; ++p) {
+        std::swap(d_occR[p], d_chiR[p]);
+        std::swap(occR_valid[p], chiR_valid[p]);
+        std::swap(d_gradR[p], a_gradR[p]);
+        std::swap(gradR_valid[p], a_gradR_valid[p]);
+    }
+    std::swap(c_a8, a_a8);
+    std::swap(c_hartree, a_hartree);
+    std::swap(c_tc2h, a_tc2h);
+    std::swap(c_tc3a1, a_tc3a1);
+    std::swap(c_tc3b1, a_tc3b1);
+    std::swap(d_Q4T, a_Q4T);
+    std::swap(d_Q5T, a_Q5T);
+    conj_mode = !conj_mode;
 }
 
 void Context::invalidate_states()
@@ -216,6 +239,7 @@ void Context::invalidate_states()
     occR_valid[0] = occR_valid[1] = false;
     chiR_valid[0] = chiR_valid[1] = false;
     gradR_valid[0] = gradR_valid[1] = false;
+    a_gradR_valid[0] = a_gradR_valid[1] = false;
     density_valid = false;
     sym_jobs_valid = false;
     plans.clear();

@@ -187,8 +187,9 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
     require(states_set, TCX_ERR_STATE, "davidson: call tcx_set_states first");
     require(nb_ && phi_d && eig_d && energies_d, TCX_ERR_ARG, "davidson: null pointer");
     require(max_blocks >= 2 && num_refresh >= 1, TCX_ERR_ARG, "davidson: need max_blocks >= 2, num_refresh >= 1");
-    require(biortho == 0, TCX_ERR_UNSUPPORTED, "davidson: the bi-orthogonal variant is not implemented");
     const bool bitc = method == TCX_METHOD_BITC, uses3 = method != TCX_METHOD_HF;
+    const bool bi = biortho != 0;  // biortho_david_ (diagonalization.cpp:117-121: BITC only)
+    require(!bi || bitc, TCX_ERR_ARG, "davidson: the bi-orthogonal variant needs a BITC cell");
     require(!bitc || phi_left_d, TCX_ERR_ARG, "davidson: BITC needs phi_left");
     if (uses3) require(jastrow_set, TCX_ERR_STATE, "davidson: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);
@@ -237,10 +238,20 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
 
     // device memory of this call (vtot, ptot coefficients of 16 bytes)
     DBuf dV[2], dHV[3], dP, dT, dIn, dO[3], dkG2, deig, dHs, dX, dXL, dw, den, dc, dnrm, dI, dL, dwork;
+    DBuf dVL[2], dHLV[3], dPL, dTL, deigc, dcc, dbn;  // bi-orthogonal variant: left side
     const size_t vb = (size_t)std::max<long long>(1, vtot) * 16, pb = (size_t)std::max<long long>(1, ptot) * 16;
     const size_t mb = (size_t)nk * gs * gs * 16;
     for (auto& b : dV) b.ensure(vb);
     for (int n = 0; n < nH; ++n) dHV[n].ensure(vb);
+    if (bi) {
+        for (auto& b : dVL) b.ensure(vb);
+        for (int n = 0; n < nH; ++n) dHLV[n].ensure(vb);
+        dPL.ensure(pb);
+        dTL.ensure(pb);
+        deigc.ensure((size_t)nbands * 16);
+        dcc.ensure((size_t)nk * 16);
+        dbn.ensure((size_t)nbands * 8);
+    }
     for (DBuf* b : {&dP, &dT, &dIn, &dO[0], &dO[1]}) b->ensure(pb);
     if (uses3) dO[2].ensure(pb);
     dkG2.ensure(kG2.size() * 8);
@@ -277,6 +288,12 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
     h2d(dkG2.p, kG2.data(), kG2.size() * 8);
     h2d(deig.p, eig_d, (size_t)nbands * 16);  // bloch_states.eigenvalues shift the residuals
     h2d(dP.p, phi_d, (size_t)ptot * 16);
+    if (bi) {
+        std::vector<double> ec(eig_d, eig_d + 2 * (size_t)nbands);  // the left residuals take conj(eps) (:322-326)
+        for (int b = 0; b < nbands; ++b) ec[2 * b + 1] = -ec[2 * b + 1];
+        h2d(deigc.p, ec.data(), ec.size() * 8);
+        h2d(dPL.p, phi_left_d, (size_t)ptot * 16);
+    }
 
     // Gram-Schmidt of slots 0..nb[k] of B in place, in band order; a vector whose first
     // projection leaves a norm below 1e-8 is an error (diagonalization_davidson.cpp:527-545)
@@ -304,14 +321,20 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
 
     int cur = 0;
     std::vector<int> size(nk);
-    // trial vectors V (initialised by phi, :48-65; orthonormalised for BITC)
+    // trial vectors V (initialised by phi, :48-65; orthonormalised for BITC unless
+    // bi-orthogonal, where V_left starts from phi_left as it is, :86-103)
     launch_dv_copy(d64(dP), dl(kPoff), nullptr, d64(dV[cur]), dl(kVoff), nullptr, di(kNb), max_nb, di(kNpw), max_npw,
                    nk, stream);
-    if (bitc) orthonormalise(d64(dV[cur]));
+    if (bi)
+        launch_dv_copy(d64(dPL), dl(kPoff), nullptr, d64(dVL[cur]), dl(kVoff), nullptr, di(kNb), max_nb, di(kNpw),
+                       max_npw, nk, stream);
+    else if (bitc)
+        orthonormalise(d64(dV[cur]));
     size = nb;
 
-    // H_n applied to V slots first[k] .. + cnt[k], into the same slots of the H_n V sets
-    auto apply_block = [&]() {
+    // H_n (or, left, H_n^dagger: :234-251) applied to slots first[k] .. + cnt[k] of V (V_left),
+    // into the same slots of the H_n V (H_n^dagger V_left) sets
+    auto apply_block = [&](bool left) {
         long long c = 0;
         int max_cnt = 0;
         for (int k = 0; k < nk; ++k) {
@@ -321,13 +344,65 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
         }
         h2d((void*)dl(kCoff), L.data() + (size_t)kCoff * nk, (size_t)nk * 8);
         push_ints(kFirst, 2);
-        launch_dv_copy(d64(dV[cur]), dl(kVoff), di(kFirst), d64(dIn), dl(kCoff), nullptr, di(kCnt), max_cnt, di(kNpw),
-                       max_npw, nk, stream);
-        apply_all(it(kCnt), d64(dIn), d64(dO[0]), d64(dO[1]), uses3 ? d64(dO[2]) : nullptr, prec,
-                  TCX_FLAG_DEVICE_PTRS);
+        launch_dv_copy(d64(left ? dVL[cur] : dV[cur]), dl(kVoff), di(kFirst), d64(dIn), dl(kCoff), nullptr, di(kCnt),
+                       max_cnt, di(kNpw), max_npw, nk, stream);
+        if (left) hermitian_conj();
+        try {
+            apply_all(it(kCnt), d64(dIn), d64(dO[0]), d64(dO[1]), uses3 ? d64(dO[2]) : nullptr, prec,
+                      TCX_FLAG_DEVICE_PTRS);
+        } catch (...) {
+            if (left) hermitian_conj();
+            throw;
+        }
+        if (left) hermitian_conj();
         for (int n = 0; n < nH; ++n)
-            launch_dv_copy(d64(dO[n]), dl(kCoff), nullptr, d64(dHV[n]), dl(kVoff), di(kFirst), di(kCnt), max_cnt,
-                           di(kNpw), max_npw, nk, stream);
+            launch_dv_copy(d64(dO[n]), dl(kCoff), nullptr, d64(left ? dHLV[n] : dHV[n]), dl(kVoff), di(kFirst),
+                           di(kCnt), max_cnt, di(kNpw), max_npw, nk, stream);
+    };
+
+    // Gram_Schmidt_biortho (diagonalization_utils.cpp:31-77) of candidate b (right in dT, left
+    // in dTL) against the pairs in V, V_left, batched over k; a kept pair enters slot size[k]
+    auto biortho_candidate = [&](int b) {
+        int max_nA = 0;
+        for (int k = 0; k < nk; ++k) {
+            it(kNA)[k] = size[k];
+            it(kBidx)[k] = b < it(kCnt)[k] ? b : -1;
+            it(kOdst)[k] = size[k];
+            max_nA = std::max(max_nA, size[k]);
+        }
+        push_ints(kNA, 3);
+        // right: t -= sum_j <V_left_j|t> V_j, then |t| >= 1e-10 and t /= |t| (:41-55)
+        launch_dv_project(d64(dV[cur]), dl(kVoff), di(kNA), max_nA, d64(dT), dl(kPoff), di(kBidx), di(kNpw), max_npw,
+                          nk, d64(dc), d64(dnrm), stream, d64(dVL[cur]));
+        std::vector<double> nrm(nk);
+        d2h(nrm.data(), dnrm.p, (size_t)nk * 8);
+        bool any = false;
+        for (int k = 0; k < nk; ++k) {
+            if (it(kBidx)[k] >= 0 && !(nrm[k] >= 1e-10)) it(kBidx)[k] = -1;
+            any = any || it(kBidx)[k] >= 0;
+        }
+        if (!any) return;
+        push_ints(kBidx, 1);
+        // a second pass of the same projection, as for Gram_Schmidt (exact arithmetic: no change)
+        launch_dv_project(d64(dV[cur]), dl(kVoff), di(kNA), max_nA, d64(dT), dl(kPoff), di(kBidx), di(kNpw), max_npw,
+                          nk, d64(dc), d64(dnrm), stream, d64(dVL[cur]));
+        launch_dv_store(d64(dT), dl(kPoff), di(kBidx), d64(dnrm), d64(dV[cur]), dl(kVoff), di(kOdst), di(kNpw),
+                        max_npw, nk, stream);  // tentative: slot size[k] counts only if the left passes
+        // left: s -= sum_j <V_j|s> V_left_j (:58-63), then s /= conj(<s|t>) with |<s|t>| >= 1e-10 (:65-76)
+        for (int pass = 0; pass < 2; ++pass)
+            launch_dv_project(d64(dVL[cur]), dl(kVoff), di(kNA), max_nA, d64(dTL), dl(kPoff), di(kBidx), di(kNpw),
+                              max_npw, nk, d64(dc), d64(dnrm), stream, d64(dV[cur]));
+        launch_dv_dot(d64(dTL), dl(kPoff), di(kBidx), d64(dV[cur]), dl(kVoff), di(kOdst), di(kNpw), nk, d64(dcc),
+                      stream);
+        std::vector<double> cc(2 * (size_t)nk);
+        d2h(cc.data(), dcc.p, cc.size() * 8);
+        for (int k = 0; k < nk; ++k)
+            if (it(kBidx)[k] >= 0 && !(std::hypot(cc[2 * k], cc[2 * k + 1]) >= 1e-10)) it(kBidx)[k] = -1;
+        push_ints(kBidx, 1);
+        launch_dv_cdiv_store(d64(dTL), dl(kPoff), di(kBidx), d64(dcc), d64(dVL[cur]), dl(kVoff), di(kOdst), di(kNpw),
+                             max_npw, nk, stream);
+        for (int k = 0; k < nk; ++k)
+            if (it(kBidx)[k] >= 0) ++size[k];
     };
 
     for (int iter = 0; iter < num_refresh; ++iter) {
@@ -339,11 +414,20 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
                 it(kFirst)[k] = size[k] - it(kCnt)[k];
                 max_cnt = std::max(max_cnt, it(kCnt)[k]);
             }
-            apply_block();
+            apply_block(false);
+            if (bi) apply_block(true);
             launch_dv_precond(d64(dV[cur]), dl(kVoff), di(kFirst), di(kCnt), max_cnt, d64(dHV[0]), d64(dHV[1]),
                               uses3 ? d64(dHV[2]) : nullptr, d64(dkG2), dl(kGoff), d64(deig), di(kBand0), d64(dT),
                               dl(kPoff), di(kNpw), nk, stream);
             const std::vector<int> nold = size;
+            if (bi) {
+                launch_dv_precond(d64(dVL[cur]), dl(kVoff), di(kFirst), di(kCnt), max_cnt, d64(dHLV[0]),
+                                  d64(dHLV[1]), uses3 ? d64(dHLV[2]) : nullptr, d64(dkG2), dl(kGoff), d64(deigc),
+                                  di(kBand0), d64(dTL), dl(kPoff), di(kNpw), nk, stream);
+                for (int b = 0; b < max_cnt; ++b) biortho_candidate(b);
+                for (int k = 0; k < nk; ++k) it(kCnt)[k] = size[k] - nold[k];
+                continue;
+            }
             for (int b = 0; b < max_cnt; ++b) {  // candidates in band order, each against the grown V
                 int max_nA = 0;
                 for (int k = 0; k < nk; ++k) {
@@ -374,16 +458,16 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
             for (int k = 0; k < nk; ++k) it(kCnt)[k] = size[k] - nold[k];
         }
         for (int k = 0; k < nk; ++k) it(kFirst)[k] = size[k] - it(kCnt)[k];
-        apply_block();  // remaining matrix elements of H V (:408-437)
+        apply_block(false);  // remaining matrix elements of H V (:408-437)
 
-        // subspace problem (:438-640)
+        // subspace problem (:438-640): H_ij = <V_i|H V_j>, <V_left_i|H V_j> bi-orthogonal (:465-466)
         for (int k = 0; k < nk; ++k) it(kSize)[k] = size[k];
         push_ints(kSize, 1);
-        launch_dv_gram(d64(dV[cur]), dl(kVoff), di(kSize), d64(dHV[0]), d64(dHV[1]), uses3 ? d64(dHV[2]) : nullptr,
-                       nH, di(kNpw), nk, d64(dHs), gs, stream);
-        launch_dv_eig(d64(dHs), nH, di(kSize), gs, nk, bitc && final_loop, d64(dw), d64(dX), bitc ? d64(dXL) : nullptr,
-                      dwork.p ? d64(dwork) : nullptr, di(kNb), di(kBand0), final_loop ? d64(den) : nullptr,
-                      di(kStatus), stream);
+        launch_dv_gram(d64(bi ? dVL[cur] : dV[cur]), dl(kVoff), di(kSize), d64(dHV[0]), d64(dHV[1]),
+                       uses3 ? d64(dHV[2]) : nullptr, nH, di(kNpw), nk, d64(dHs), gs, stream);
+        launch_dv_eig(d64(dHs), nH, di(kSize), gs, nk, (bitc && final_loop) || bi, bi ? 1 : 0, d64(dw), d64(dX),
+                      bitc ? d64(dXL) : nullptr, dwork.p ? d64(dwork) : nullptr, di(kNb), di(kBand0),
+                      final_loop ? d64(den) : nullptr, di(kStatus), stream);
         d2h(it(kStatus), di(kStatus), (size_t)nk * 4);
         for (int k = 0; k < nk; ++k) {
             require(it(kStatus)[k] != 1, TCX_ERR_INTERNAL, "davidson: diagonalization of H_subspace failed");
@@ -392,12 +476,28 @@ void Context::davidson(const int* nb_, double* phi_d, double* phi_left_d, double
         const int alt = 1 - cur;
         launch_dv_rotate(d64(dV[cur]), dl(kVoff), di(kSize), d64(dX), gs, d64(dV[alt]), dl(kVoff), di(kNb), di(kNpw),
                          max_npw, nk, stream);
-        if (!(bitc && final_loop)) orthonormalise(d64(dV[alt]));
+        if (bi) {
+            // V X and V_left X_L, rescaled so that |V X| = 1 with <left|right> kept (:596-615)
+            launch_dv_rotate(d64(dVL[cur]), dl(kVoff), di(kSize), d64(dXL), gs, d64(dVL[alt]), dl(kVoff), di(kNb),
+                             di(kNpw), max_npw, nk, stream);
+            launch_dv_binorm(d64(dV[alt]), d64(dVL[alt]), dl(kVoff), di(kNb), max_nb, di(kBand0), di(kNpw), nk,
+                             d64(dbn), stream);
+            std::vector<double> bn(nbands);
+            d2h(bn.data(), dbn.p, (size_t)nbands * 8);
+            for (int b = 0; b < nbands; ++b)
+                require(bn[b] >= 1e-10, TCX_ERR_INTERNAL, "davidson: linear independence breaks");
+        } else if (!(bitc && final_loop)) {
+            orthonormalise(d64(dV[alt]));
+        }
         if (final_loop) {
             launch_dv_copy(d64(dV[alt]), dl(kVoff), nullptr, d64(dP), dl(kPoff), nullptr, di(kNb), max_nb, di(kNpw),
                            max_npw, nk, stream);
             d2h(phi_d, dP.p, (size_t)ptot * 16);
-            if (bitc) {
+            if (bi) {
+                launch_dv_copy(d64(dVL[alt]), dl(kVoff), nullptr, d64(dPL), dl(kPoff), nullptr, di(kNb), max_nb,
+                               di(kNpw), max_npw, nk, stream);
+                d2h(phi_left_d, dPL.p, (size_t)ptot * 16);
+            } else if (bitc) {
                 launch_dv_rotate(d64(dV[cur]), dl(kVoff), di(kSize), d64(dXL), gs, d64(dT), dl(kPoff), di(kNb),
                                  di(kNpw), max_npw, nk, stream);
                 d2h(phi_left_d, dT.p, (size_t)ptot * 16);

@@ -224,9 +224,11 @@ void launch_dv_precond(const double* V, const long long* voff, const int* first,
                        const double* H0, const double* H1, const double* H2, const double* kG2, const long long* goff,
                        const double* eig, const int* band0, double* T, const long long* toff, const int* npw, int nk,
                        hipStream_t s);
+// B_bidx -= sum_i <D_i|B_bidx> A_i over i < nA (D = A when null: Gram_Schmidt; D = the
+// other side's set: Gram_Schmidt_biortho), then nrm = ||B_bidx||
 void launch_dv_project(const double* A, const long long* aoff, const int* nA, int max_nA, double* B,
                        const long long* boff, const int* bidx, const int* npw, int max_npw, int nk, double* c,
-                       double* nrm, hipStream_t s);
+                       double* nrm, hipStream_t s, const double* D = nullptr);
 void launch_dv_store(const double* B, const long long* boff, const int* bidx, const double* nrm, double* O,
                      const long long* ooff, const int* odst, const int* npw, int max_npw, int nk, hipStream_t s);
 void launch_dv_copy(const double* S, const long long* soff, const int* sfirst, double* D, const long long* doff,
@@ -237,7 +239,14 @@ void launch_dv_rotate(const double* A, const long long* aoff, const int* nA, con
                       const long long* ooff, const int* nout, const int* npw, int max_npw, int nk, hipStream_t s);
 size_t dv_eig_work_bytes(int gs, int nk);
 bool dv_eig_fits_lds(int gs);
-void launch_dv_eig(const double* Hs, int nH, const int* dim, int gs, int nk, int want_left, double* w, double* X,
-                   double* XL, double* gwork, const int* nb, const int* band0, double* en, int* status, hipStream_t s);
+void launch_dv_eig(const double* Hs, int nH, const int* dim, int gs, int nk, int want_left, int sort_overlap,
+                   double* w, double* X, double* XL, double* gwork, const int* nb, const int* band0, double* en,
+                   int* status, hipStream_t s);
+void launch_dv_dot(const double* X, const long long* xoff, const int* xidx, const double* Y, const long long* yoff,
+                   const int* yidx, const int* npw, int nk, double* c, hipStream_t s);
+void launch_dv_cdiv_store(const double* B, const long long* boff, const int* bidx, const double* c, double* O,
+                          const long long* ooff, const int* odst, const int* npw, int max_npw, int nk, hipStream_t s);
+void launch_dv_binorm(double* R, double* L, const long long* off, const int* nb, int max_nb, const int* band0,
+                      const int* npw, int nk, double* out, hipStream_t s);
 
 }  // namespace tcx

@@ -171,6 +171,19 @@ struct Context {
     LocalCache c_a8, c_hartree, c_tc2h, c_tc3a1, c_tc3b1;
     void invalidate_local();
 
+    // The Hermitian conjugate H^dagger that the bi-orthogonal Davidson applies to its left
+    // vectors: u -> -u and the right and left occupied orbitals swapped, with the density
+    // left as it is (Jastrow::take_Hermitian_conj, BlochStates::switch_left_right_orbitals;
+    // diagonalization_davidson.cpp:234-251).  hermitian_conj() toggles it.  The caches of
+    // the other orientation (local potentials, tc3a2a4b2b5's q-sum products, the gradient
+    // grids) are kept beside the current ones and swapped with them, so that alternating
+    // H and H^dagger applies do not rebuild them; every invalidation clears both.
+    bool conj_mode = false;
+    LocalCache a_a8, a_hartree, a_tc2h, a_tc3a1, a_tc3b1;
+    DBuf a_Q4T, a_Q5T, a_gradR[2];
+    bool a_gradR_valid[2] = {false, false};
+    void hermitian_conj();
+
     // HIP graphs captured from this context's stream (tcx_graph_*)
     std::vector<hipGraphExec_t> graphs;
 

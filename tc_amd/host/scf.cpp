@@ -329,12 +329,15 @@ SCFResult Diagonalization::scf(tcx_context* ctx, const QEInputs& qe) const
             phi_cat.insert(phi_cat.end(), phi[ik].begin(), phi[ik].end());
             for (int b = 0; b < nb; ++b) eig_in[(size_t)ik * nb + b] = eig[ik][b];
         }
-        if (bitc) left_cat.assign(phi_cat.size(), 0.0);
+        if (bitc && S.biortho_david)  // the left trial bands are the current left orbitals (:86-103)
+            for (int ik = 0; ik < nirr; ++ik) left_cat.insert(left_cat.end(), chi[ik].begin(), chi[ik].end());
+        else if (bitc)
+            left_cat.assign(phi_cat.size(), 0.0);
         std::vector<Complex> en_out((size_t)nirr * nb * 3);
         check(ctx,
               tcx_davidson(ctx, nb_full.data(), (double*)phi_cat.data(), bitc ? (double*)left_cat.data() : nullptr,
                            (double*)eig_in.data(), (double*)en_out.data(), S.max_num_blocks_david, S.num_refresh_david,
-                           0, S.precision),
+                           bitc && S.biortho_david ? 1 : 0, S.precision),
               "davidson");
         ++out.iterations;
         if (dm) {  // reset_phik keeps the occupied bands before the update (bloch_states_scfloop_phik.cpp:86-99)

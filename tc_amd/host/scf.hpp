@@ -62,6 +62,7 @@ struct SCFSettings {
     double charge_tolerance = 1e-5;
     int max_num_blocks_david = 2;
     int num_refresh_david = 1;
+    bool biortho_david = false;   // BITC: bi-orthogonal block Davidson (diagonalization.cpp:117-122)
     double A_input = 1.0;         // RPA Jastrow scale (jastrow_initialize.cpp:28-50)
     int precision = TCX_PREC_FP64;
 };

@@ -222,6 +222,7 @@ def write_flat(qe: QEInput, path: str) -> None:
         ("charge_tolerance", np.asarray([float(st.get("charge_tolerance", 1e-5))], "<f8")),
         ("max_num_blocks_david", np.asarray([int(st["max_num_blocks_david"])], "<i4")),
         ("num_refresh_david", np.asarray([int(st["num_refresh_david"])], "<i4")),
+        ("biortho_david", np.asarray([int(bool(st.get("biortho_david", False)))], "<i4")),
         ("A_input", np.asarray([float(st.get("A_input", 1.0))], "<f8")),
     ]
     with open(path, "wb") as f:

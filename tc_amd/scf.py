@@ -172,9 +172,11 @@ def run(engine: tcx.Engine, qe: Q.QEInput, precision=tcx.PREC_FP64, max_iteratio
                 seen.add(r.ik)
                 nb_full[i] = nb
         phi_cat = np.concatenate([phi[ik].reshape(-1) for ik in range(nirr)])
+        biortho = bitc and bool(setup.get("biortho_david", False))  # diagonalization.cpp:117-122
+        left_cat = np.concatenate([chi[ik].reshape(-1) for ik in range(nirr)]) if biortho else None
         phi_out, left_out, eig_out, en_out = engine.davidson(
             nb_full, phi_cat, eig.reshape(-1), max_blocks=setup["max_num_blocks_david"],
-            num_refresh=setup["num_refresh_david"], precision=precision)
+            num_refresh=setup["num_refresh_david"], precision=precision, phi_left=left_cat)
         out.iterations += 1
         if dm:  # reset_phik keeps the occupied bands before the update as the old bands
             old_phi = [phi[ik][:nocc[ik]].copy() for ik in range(nirr)]

@@ -335,15 +335,22 @@ class Engine:
     def tc2h(self, nb, phi, out=None):
         return self._term_fp64("tc2h", nb, phi, out)
 
-    def davidson(self, nb, phi, eig, max_blocks=2, num_refresh=1, precision=PREC_FP64):
-        """One Diagonalization::block_davidson call: returns (phi, phi_left or None, eig, energies[:,3])."""
+    def davidson(self, nb, phi, eig, max_blocks=2, num_refresh=1, precision=PREC_FP64, phi_left=None):
+        """One Diagonalization::block_davidson call: returns (phi, phi_left or None, eig, energies[:,3]).
+        phi_left given (BITC): the bi-orthogonal variant (biortho_david = true) from these left bands."""
         nb = np.ascontiguousarray(nb, dtype=np.int32)
         phi = np.array(phi, dtype=np.complex128, copy=True)
         eig = np.array(eig, dtype=np.complex128, copy=True)
-        left = np.zeros_like(phi) if self.method == "BITC" else None
+        bi = phi_left is not None
+        if bi:
+            left = np.array(phi_left, dtype=np.complex128, copy=True)
+            if left.shape != phi.shape:
+                raise ValueError("davidson: phi_left must have the shape of phi")
+        else:
+            left = np.zeros_like(phi) if self.method == "BITC" else None
         en = np.zeros((int(nb.sum()), 3), np.complex128)
         st = self.L.tcx_davidson(self.h, _ptr(nb, _ip), phi.ctypes.data, left.ctypes.data if left is not None else None,
-                                 eig.ctypes.data, en.ctypes.data, int(max_blocks), int(num_refresh), 0,
+                                 eig.ctypes.data, en.ctypes.data, int(max_blocks), int(num_refresh), int(bi),
                                  int(precision))
         self._check(st, "davidson")
         return phi, left, eig, en

@@ -240,6 +240,7 @@ int main(int argc, char** argv)
         s.charge_tolerance = get<double>(m, "charge_tolerance", "<f8")[0];
         s.max_num_blocks_david = get<int32_t>(m, "max_num_blocks_david", "<i4")[0];
         s.num_refresh_david = get<int32_t>(m, "num_refresh_david", "<i4")[0];
+        s.biortho_david = get<int32_t>(m, "biortho_david", "<i4")[0] != 0;
         s.A_input = get<double>(m, "A_input", "<f8")[0];
         if (std::string(argv[2]) == "replicas") {
             for (const auto& r : tcx_host::replicate_kpoints(qe)) {
