@@ -201,6 +201,83 @@ static cplx zdotc(const cplx *a, const cplx *b, int n) /* Eigen a.dot(b) = sum c
 }
 
 /* ------------------------------------------------------------------ */
+/* Zero-weight k-points (BAND mode, fake SCF)                           */
+/* ------------------------------------------------------------------ */
+
+static int zero_weight(const orc_problem *p, int ik) { return p->kweight && p->kweight[ik] < 1e-8; }
+
+/* k is on the weighted mesh (modulo reciprocal lattice vectors):
+ * tc3a3b3b4b6.cpp:396-410, potentials_coulomb.cpp:271-287 */
+static int if_match(const orc_problem *p, int ik)
+{
+    for (int iq = 0; iq < p->nk; ++iq) {
+        if (zero_weight(p, iq)) continue;
+        double kq[3];
+        for (int d = 0; d < 3; ++d) {
+            kq[d] = p->kfrac[3 * ik + d] - p->kfrac[3 * iq + d];
+            while (kq[d] < -1e-8) kq[d] += 1.0;
+            while (kq[d] > 1.0 - 1e-8) kq[d] -= 1.0;
+        }
+        if (kq[0] * kq[0] + kq[1] * kq[1] + kq[2] * kq[2] < 1e-8) return 1;
+    }
+    return 0;
+}
+
+/* The states the divergence terms at k project on (the ibandk loops, e.g.
+ * hf2x.cpp:166-199): the occupied orbitals at a weighted k (their fillings, old bands
+ * included), the given states with their return_band_filling values at a zero-weight k. */
+typedef struct {
+    int n;
+    const double *fill;
+    const cplx *phi, *chi;
+} div_set;
+
+static div_set div_states(const orc_problem *p, int ik)
+{
+    div_set s;
+    if (zero_weight(p, ik)) {
+        long fo = 0, co = 0;
+        for (int k = 0; k < ik; ++k) {
+            fo += p->zw_nstate ? p->zw_nstate[k] : 0;
+            co += (long)(p->zw_nstate ? p->zw_nstate[k] : 0) * p->npw[k];
+        }
+        s.n = p->zw_nstate ? p->zw_nstate[ik] : 0;
+        s.fill = p->zw_fill + fo;
+        s.phi = (const cplx *)p->zw_phi + co;
+        s.chi = (p->method == 2 && p->zw_chi) ? (const cplx *)p->zw_chi + co : s.phi;
+    } else {
+        long fo = 0, co = 0;
+        for (int k = 0; k < ik; ++k) {
+            fo += p->nocc[k];
+            co += (long)p->nocc[k] * p->npw[k];
+        }
+        s.n = p->nocc[ik];
+        s.fill = p->filling + fo;
+        s.phi = (const cplx *)p->phi_occ + co;
+        s.chi = (p->method == 2 && p->chi_occ) ? (const cplx *)p->chi_occ + co : s.phi;
+    }
+    return s;
+}
+
+static void kvaux_of(const orc_problem *p, int ik, double out[3])
+{
+    for (int d = 0; d < 3; ++d) out[d] = p->sum_kvaux ? p->sum_kvaux[3 * ik + d] : 0.0;
+}
+
+/* (k + G)_d at plane wave l of k (Cartesian) */
+static void kg_of(const orc_problem *p, int ik, int l, double out[3])
+{
+    double kc[3];
+    kcart(p, p->kfrac + 3 * ik, kc);
+    const int e = p->gindex[pw_offset(p, ik) + l];
+    const int n1 = p->n[0], n2 = p->n[1], n3 = p->n[2];
+    const int i1 = e % n1, i2 = (e / n1) % n2, i3 = e / (n1 * n2);
+    const int m1 = i1 <= n1 / 2 ? i1 : i1 - n1, m2 = i2 <= n2 / 2 ? i2 : i2 - n2, m3 = i3 <= n3 / 2 ? i3 : i3 - n3;
+    for (int d = 0; d < 3; ++d)
+        out[d] = kc[d] + p->recip[0 * 3 + d] * m1 + p->recip[1 * 3 + d] * m2 + p->recip[2 * 3 + d] * m3;
+}
+
+/* ------------------------------------------------------------------ */
 /* Jastrow (RPA), jastrow_functions.cpp:6-97                            */
 /* ------------------------------------------------------------------ */
 
@@ -283,6 +360,74 @@ void orc_sum_of_vaux(const double recip[9], double volume, int nk, const double 
     *alpha_out = alpha;
 }
 
+/* set_sum_of_Vaux_each and set_sum_of_kVaux_each (potentials_coulomb.cpp:60-254) with the
+ * q-sum over the weighted k only (:86, :198): the shells of every (k, q) until a shell's
+ * sum (squared norm for the vector) drops below 1e-20; the q = k term of the nshell = 0
+ * shell is left out when k matches q; -4 pi alpha for k on the mesh (:149-160); the vector
+ * sum is zero there (:187-191). */
+void orc_sum_of_vaux_w(const double recip[9], double volume, int nk, const double *kfrac, const double *kweight,
+                       double *alpha_out, double *sum_out, double *ksum_out)
+{
+    double alpha = cbrt(volume) / (2.0 * PI);
+    alpha = alpha * alpha;
+    for (int ik = 0; ik < nk; ++ik) {
+        double sum = 0.0, ks[3] = {0.0, 0.0, 0.0};
+        int match_any = 0;
+        for (int iq = 0; iq < nk; ++iq) {
+            if (kweight && kweight[iq] < 1e-8) continue;
+            double kq[3];
+            for (int d = 0; d < 3; ++d) {
+                kq[d] = kfrac[ik * 3 + d] - kfrac[iq * 3 + d];
+                while (kq[d] < -1e-8) kq[d] += 1.0;
+                while (kq[d] > 1.0 - 1e-8) kq[d] -= 1.0;
+            }
+            const int match = (kq[0] * kq[0] + kq[1] * kq[1] + kq[2] * kq[2]) < 1e-8;
+            if (match) match_any = 1;
+            if (!match) {
+                double c[3];
+                for (int d = 0; d < 3; ++d)
+                    c[d] = recip[0 * 3 + d] * kq[0] + recip[1 * 3 + d] * kq[1] + recip[2 * 3 + d] * kq[2];
+                const double kq2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+                const double w = exp(-alpha * kq2) / kq2;
+                sum += w;
+                for (int d = 0; d < 3; ++d) ks[d] += w * c[d];
+            }
+            /* scalar and vector shells stop independently (:135, :245) */
+            int done_s = 0, done_v = 0;
+            for (int nshell = 1; !(done_s && done_v); ++nshell) {
+                double shell = 0.0, vshell[3] = {0.0, 0.0, 0.0};
+                for (int gx = -nshell; gx <= nshell; ++gx)
+                    for (int gy = -nshell; gy <= nshell; ++gy)
+                        for (int gz = -nshell; gz <= nshell; ++gz) {
+                            if (gx != nshell && -gx != nshell && gy != nshell && -gy != nshell &&
+                                gz != nshell && -gz != nshell) continue;
+                            double m[3] = {kq[0] + gx, kq[1] + gy, kq[2] + gz}, c[3];
+                            for (int d = 0; d < 3; ++d)
+                                c[d] = recip[0 * 3 + d] * m[0] + recip[1 * 3 + d] * m[1] + recip[2 * 3 + d] * m[2];
+                            const double g2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+                            const double w = exp(-alpha * g2) / g2;
+                            shell += w;
+                            for (int d = 0; d < 3; ++d) vshell[d] += w * c[d];
+                        }
+                if (!done_s) {
+                    sum += shell;
+                    if (shell < 1e-20) done_s = 1;
+                }
+                if (!done_v) {
+                    for (int d = 0; d < 3; ++d) ks[d] += vshell[d];
+                    if (vshell[0] * vshell[0] + vshell[1] * vshell[1] + vshell[2] * vshell[2] < 1e-20) done_v = 1;
+                }
+            }
+        }
+        sum *= FOURPI;
+        if (match_any) sum -= FOURPI * alpha;
+        sum_out[ik] = sum;
+        if (ksum_out)
+            for (int d = 0; d < 3; ++d) ksum_out[3 * ik + d] = match_any ? 0.0 : FOURPI * ks[d];
+    }
+    *alpha_out = alpha;
+}
+
 /* ------------------------------------------------------------------ */
 /* Density, bloch_states_scfloop_filling_density.cpp:256-358            */
 /* ------------------------------------------------------------------ */
@@ -359,7 +504,7 @@ void orc_hf2x(const orc_problem *p, const int *nb, const double *phi_d, double *
                 double *V = malloc(sizeof(double) * N);
 #pragma omp for schedule(dynamic)
                 for (int iq = 0; iq < p->nk; ++iq) {                  /* :80 */
-                    if (p->nocc[iq] == 0) continue;                    /* :90 */
+                    if (p->nocc[iq] == 0 || zero_weight(p, iq)) continue; /* :90 */
                     double qc[3];
                     kcart(p, p->kfrac + 3 * iq, qc);
                     for (long r = 0; r < N; ++r) {                     /* :91-104 */
@@ -390,11 +535,11 @@ void orc_hf2x(const orc_problem *p, const int *nb, const double *phi_d, double *
 
             if (p->div_corr) {                                          /* :150-201 */
                 const double dc = div_corr_value(p, ik);
-                const long co = occ_coef_offset(p, ik), fo = occ_offset(p, ik);
-                for (int b = 0; b < p->nocc[ik]; ++b) {
-                    const double f = p->filling[fo + b];
+                const div_set D = div_states(p, ik);                    /* :166-191 */
+                for (int b = 0; b < D.n; ++b) {
+                    const double f = D.fill[b];
                     if (f < 1e-8) continue;
-                    const cplx *pk = occ + co + (long)b * npwk;
+                    const cplx *pk = D.phi + (long)b * npwk;
                     cplx prod = zdotc(pk, phi + ao + (long)jb * npwk, npwk) * f * dc;
                     for (int l = 0; l < npwk; ++l) out[l] += prod * pk[l];
                 }
@@ -406,8 +551,8 @@ void orc_hf2x(const orc_problem *p, const int *nb, const double *phi_d, double *
 
 /* ------------------------------------------------------------------ */
 /* tc2x, calc_hamiltonian_tc2x.cpp:11-516                               */
-/* (the kVaux branches :117-222, :483-508 vanish for SCF k on the mesh,  */
-/*  potentials_coulomb.cpp:187-191, and are not restated)               */
+/* (the kVaux branches :117-222, :483-508 act at k off the weighted     */
+/*  mesh only: sum_of_kVaux = 0 on it, potentials_coulomb.cpp:187-191)  */
 /* ------------------------------------------------------------------ */
 
 void orc_tc2x(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d, int nthreads)
@@ -439,7 +584,7 @@ void orc_tc2x(const orc_problem *p, const int *nb, const double *phi_d, double *
             cplx *pgp = malloc(sizeof(cplx) * 3 * N), *hsub = malloc(sizeof(cplx) * N);
 #pragma omp for schedule(dynamic)
             for (int iq = 0; iq < p->nk; ++iq) {                              /* :241 */
-                if (p->nocc[iq] == 0) continue;                                /* :251 */
+                if (p->nocc[iq] == 0 || zero_weight(p, iq)) continue;          /* :251 */
                 double qc[3];
                 kcart(p, p->kfrac + 3 * iq, qc);                               /* :253-255 */
                 for (long r = 0; r < N; ++r) {                                 /* :256-276 */
@@ -508,20 +653,39 @@ void orc_tc2x(const orc_problem *p, const int *nb, const double *phi_d, double *
 
         if (p->div_corr) {                                                      /* :409-513 */
             const double dc = div_corr_value(p, ik);
-            const long co = occ_coef_offset(p, ik), fo = occ_offset(p, ik);
+            const div_set D = div_states(p, ik);                                /* :440-479 */
+            /* kVaux = sum_of_kVaux / (Nk V) (/2 for one spin) * A_long[s][s] (:117-126) */
+            double kv[3];
+            kvaux_of(p, ik, kv);
+            for (int d = 0; d < 3; ++d) kv[d] *= p->A[0] / (p->num_kpoints * p->volume) / (p->nspin == 1 ? 2.0 : 1.0);
+            const int use_kv = !p->jastrow_A_zero && (kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]) > 1e-8;
+            double *kg = malloc(sizeof(double) * 3 * (npwk > 0 ? npwk : 1));
+            for (int l = 0; l < npwk; ++l) kg_of(p, ik, l, kg + 3 * l);
             for (int jb = 0; jb < nb[ik]; ++jb) {
                 cplx *out = hphi + ao + (long)jb * npwk;
-                for (int b = 0; b < p->nocc[ik]; ++b) {
-                    const double f = p->filling[fo + b];
+                const cplx *pj = phi + ao + (long)jb * npwk;
+                for (int b = 0; b < D.n; ++b) {
+                    const double f = D.fill[b];
                     if (f < 1e-8) continue;                                     /* :463 */
-                    const cplx *pk = occ + co + (long)b * npwk;
-                    const cplx *ck = occl + co + (long)b * npwk;                /* :476-479 */
-                    cplx prod = zdotc(ck, phi + ao + (long)jb * npwk, npwk);    /* :481 */
+                    const cplx *pk = D.phi + (long)b * npwk;
+                    const cplx *ck = D.chi + (long)b * npwk;                    /* :476-479 */
+                    cplx prod = zdotc(ck, pj, npwk);                            /* :481 */
+                    if (use_kv)                                                 /* [2b_x1] :483-492, :169-180 */
+                        for (int l = 0; l < npwk; ++l) {
+                            const double kd = kv[0] * kg[3 * l] + kv[1] * kg[3 * l + 1] + kv[2] * kg[3 * l + 2];
+                            out[l] -= f * kd * pk[l] * prod;
+                        }
                     prod *= dc;                                                 /* :496 */
+                    if (use_kv)                                                 /* [2b_x2] :498-507, :193-204 */
+                        for (int l = 0; l < npwk; ++l) {
+                            const double kd = kv[0] * kg[3 * l] + kv[1] * kg[3 * l + 1] + kv[2] * kg[3 * l + 2];
+                            prod += conj(ck[l]) * kd * pj[l];
+                        }
                     prod *= f;                                                  /* :509 */
                     for (int l = 0; l < npwk; ++l) out[l] += prod * pk[l];      /* :510 */
                 }
             }
+            free(kg);
         }
     }
     free(G);
@@ -760,7 +924,7 @@ void orc_tc3b1(const orc_problem *p, const int *nb, const double *phi_d, double 
 
 /* ------------------------------------------------------------------ */
 /* tc3a3b3b4b6, calc_hamiltonian_tc3a3b3b4b6.cpp:11-863                 */
-/* (kVaux branches :537-627, :772-858 vanish for SCF k on the mesh)     */
+/* (kVaux branches :537-627, :772-858 act at k off the weighted mesh)   */
 /* ------------------------------------------------------------------ */
 
 static void dnu_grids(const orc_problem *p, const double *G, cplx *dnu) /* :128-150 */
@@ -842,7 +1006,7 @@ void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi_d, d
             cplx *dnuphi = malloc(sizeof(cplx) * 3 * N), *hsub = malloc(sizeof(cplx) * N);
 #pragma omp for schedule(dynamic)
             for (int iq = 0; iq < p->nk; ++iq) {
-                if (p->nocc[iq] == 0) continue;                        /* :201 */
+                if (p->nocc[iq] == 0 || zero_weight(p, iq)) continue;  /* :201 */
                 double qc[3], kq[3];
                 kcart(p, p->kfrac + 3 * iq, qc);
                 for (int d = 0; d < 3; ++d) kq[d] = kc[d] - qc[d];
@@ -906,7 +1070,14 @@ void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi_d, d
             free(dnuphi); free(hsub);
         }
 
-        /* per band: [3b6] divergence correction (:393-535), [3a3] (:629-642) */
+        /* per band: [3b6] divergence correction (:393-535), the kVaux terms of [3b3] and
+         * [3a3,b4] (:537-627), [3a3] (:629-642) */
+        const div_set D = div_states(p, ik);
+        const int match = if_match(p, ik);                                       /* :396-410 */
+        double kv3[3];                                     /* -sum_of_kVaux A_long[s][s] (:539-543) */
+        kvaux_of(p, ik, kv3);
+        for (int d = 0; d < 3; ++d) kv3[d] *= -p->A[0];
+        const int use_kv = need_div && (kv3[0] * kv3[0] + kv3[1] * kv3[1] + kv3[2] * kv3[2]) > 1e-8;
         {
             cplx *phij = malloc(sizeof(cplx) * N), *phib = malloc(sizeof(cplx) * N), *chib = malloc(sizeof(cplx) * N);
             cplx *pp = malloc(sizeof(cplx) * N), *hsub = malloc(sizeof(cplx) * N);
@@ -915,33 +1086,38 @@ void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi_d, d
                 cplx *out = hphi + ao + (long)jb * npwk;
                 orbital_grid(p, ik, phi + ao + (long)jb * npwk, phij, N);
                 if (need_div) {
-                    /* k is on the SCF mesh, so if_match holds (:396-410) */
                     const double dcp = 1.0 / sqrt(PI * p->alpha) * NkV - p->sum_vaux[ik]; /* :415-419 */
+                    double kvd[3];                                                 /* -sum_of_kVaux (:422-425) */
+                    kvaux_of(p, ik, kvd);
                     cplx t1 = 0.0;
                     memset(tG, 0, sizeof(cplx) * N);
                     for (int js = 0; js < 2; ++js) {                              /* :427-450 */
                         memcpy(nG, dens + js * N, sizeof(cplx) * N);
                         orc_fft3d((double *)nG, p->n, -1);
                         const double Aj = p->A[0 * 2 + js];
-                        const double Adiv = dcp * Aj + orc_uk(p, 0.0, 0, js);
+                        double Adiv = dcp * Aj;
+                        if (match) Adiv += orc_uk(p, 0.0, 0, js);
                         t1 += FOURPI * Adiv * Aj * nG[0];
                         for (long r = 1; r < N; ++r) {                            /* AuG :99-112, G=0 excluded */
                             double g2 = G[3 * r] * G[3 * r] + G[3 * r + 1] * G[3 * r + 1] + G[3 * r + 2] * G[3 * r + 2];
-                            tG[r] += FOURPI * nG[r] * (Aj * orc_uk(p, g2, 0, js));
+                            const double AuG = Aj * orc_uk(p, g2, 0, js);
+                            if (match)
+                                tG[r] += FOURPI * nG[r] * AuG;
+                            else                                                  /* :445-448 */
+                                tG[r] -= (kvd[0] * G[3 * r] + kvd[1] * G[3 * r + 1] + kvd[2] * G[3 * r + 2]) * nG[r] * AuG;
                         }
                     }
                     memcpy(tR, tG, sizeof(cplx) * N);
                     orc_fft3d((double *)tR, p->n, +1);                             /* :451 */
                     memset(hsub, 0, sizeof(cplx) * N);
-                    const long co = occ_coef_offset(p, ik), fo = occ_offset(p, ik);
-                    for (int b = 0; b < p->nocc[ik]; ++b) {                         /* :456-532 */
-                        const double f = p->filling[fo + b];
+                    for (int b = 0; b < D.n; ++b) {                                  /* :456-532 */
+                        const double f = D.fill[b];
                         if (f < 1e-8) continue;
-                        const cplx *pk = occ + co + (long)b * npwk;
+                        const cplx *pk = D.phi + (long)b * npwk;
                         orbital_grid(p, ik, pk, phib, N);
                         const cplx *chi = phib;
                         if (bitc) {
-                            orbital_grid(p, ik, occl + co + (long)b * npwk, chib, N);
+                            orbital_grid(p, ik, D.chi + (long)b * npwk, chib, N);
                             chi = chib;
                         }
                         for (long r = 0; r < N; ++r) pp[r] = chi[r] * conj(phij[r]);   /* :511-512 */
@@ -949,12 +1125,40 @@ void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi_d, d
                         const cplx Ib = conj(pp[0]);
                         cplx dot = 0.0;                                                /* phiphi.dot(temp_G) */
                         for (long r = 0; r < N; ++r) dot += conj(pp[r]) * tG[r];
-                        const cplx c = f * fb6 * t1 * Ib + f * fb6 * dot;                /* contributions 1, 3 */
+                        const double sign = match ? 1.0 : -1.0;                          /* :519 */
+                        const cplx c = f * fb6 * t1 * Ib + sign * f * fb6 * dot;         /* contributions 1, 3 or 5 */
                         for (int l = 0; l < npwk; ++l) out[l] += c * pk[l];
-                        for (long r = 0; r < N; ++r) hsub[r] += (f * Ib * fb6) * tR[r] * phib[r]; /* 2 */
+                        for (long r = 0; r < N; ++r) hsub[r] += (f * Ib * fb6) * tR[r] * phib[r]; /* 2 or 4 */
                     }
                     orc_fft3d((double *)hsub, p->n, -1);
                     for (int l = 0; l < npwk; ++l) out[l] += hsub[gi[l]];
+                }
+                if (use_kv) {                                                      /* :545-623 */
+                    for (int b = 0; b < D.n; ++b) {
+                        const double f = D.fill[b];
+                        if (f < 1e-8) continue;
+                        const cplx *pk = D.phi + (long)b * npwk;
+                        orbital_grid(p, ik, pk, phib, N);
+                        const cplx *chi = phib;
+                        if (bitc) {
+                            orbital_grid(p, ik, D.chi + (long)b * npwk, chib, N);
+                            chi = chib;
+                        }
+                        /* [3b3]: sum_d kVaux_d <chi_b|dnu_d phi_j> f fb3 / N, times phi_b (:596-609) */
+                        cplx t3 = 0.0;
+                        for (int d = 0; d < 3; ++d) {
+                            cplx c = 0.0;
+                            for (long r = 0; r < N; ++r) c += conj(chi[r]) * dnu[d * N + r] * phij[r];
+                            t3 += c * (f * fb3 / (double)N) * kv3[d];
+                        }
+                        for (int l = 0; l < npwk; ++l) out[l] += t3 * pk[l];
+                        /* [3a3,b4]: qsum_d += kVaux_d f <chi_b|phi_j> phi_b(r) (:612-620) */
+                        const cplx pj = zdotc(D.chi + (long)b * npwk, phi + ao + (long)jb * npwk, npwk) * f;
+                        for (int d = 0; d < 3; ++d) {
+                            cplx *qs = qsum + ((size_t)jb * 3 + d) * N;
+                            for (long r = 0; r < N; ++r) qs[r] += pj * kv3[d] * phib[r];
+                        }
+                    }
                 }
                 /* [3a3] */
                 memset(hsub, 0, sizeof(cplx) * N);
@@ -976,7 +1180,7 @@ void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi_d, d
             cplx *pp = malloc(sizeof(cplx) * N), *orb = malloc(sizeof(cplx) * N), *hsub = malloc(sizeof(cplx) * N);
 #pragma omp for schedule(dynamic)
             for (int iq = 0; iq < p->nk; ++iq) {
-                if (p->nocc[iq] == 0) continue;
+                if (p->nocc[iq] == 0 || zero_weight(p, iq)) continue;
                 double qc[3], kq[3];
                 kcart(p, p->kfrac + 3 * iq, qc);
                 for (int d = 0; d < 3; ++d) kq[d] = kc[d] - qc[d];
@@ -1012,6 +1216,27 @@ void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi_d, d
             }
             free(du); free(phiq); free(chiq); free(pp); free(orb); free(hsub);
         }
+        if (use_kv) {                                                        /* [3b4] kVaux (:772-858) */
+            cplx *chib = malloc(sizeof(cplx) * N);
+            for (int jb = 0; jb < nb[ik]; ++jb) {
+                cplx *out = hphi + ao + (long)jb * npwk;
+                for (int b = 0; b < D.n; ++b) {
+                    const double f = D.fill[b];
+                    if (f < 1e-8) continue;
+                    orbital_grid(p, ik, D.chi + (long)b * npwk, chib, N);  /* chi_b (phi_b for TC) */
+                    cplx t4 = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        const cplx *qs = qsum + ((size_t)jb * 3 + d) * N;
+                        cplx c = 0.0;
+                        for (long r = 0; r < N; ++r) c += conj(chib[r]) * qs[r];
+                        t4 += c * (fb4 * f / (double)N) * kv3[d];
+                    }
+                    const cplx *pk = D.phi + (long)b * npwk;
+                    for (int l = 0; l < npwk; ++l) out[l] += t4 * pk[l];
+                }
+            }
+            free(chib);
+        }
         free(qsum);
     }
     free(G); free(dnu);
@@ -1021,8 +1246,8 @@ void orc_tc3a3b3b4b6(const orc_problem *p, const int *nb, const double *phi_d, d
 /* tc3a2a4b2b5, calc_hamiltonian_tc3a2a4b2b5.cpp:11-1300                */
 /* Identity symmetry: the U^+ transforms of :576-643 and :682-765 are   */
 /* the identity (phase 1, Udag_cart = 1), so qsum_temp_3a4_trans is     */
-/* FFT_backward(FFT_forward(qsum_temp_3a4)).  The kVaux branches         */
-/* :1036-1299 vanish for SCF k on the mesh.                              */
+/* FFT_backward(FFT_forward(qsum_temp_3a4)).  The kVaux branch           */
+/* :1036-1299 acts at k off the weighted mesh only.                      */
 /* ------------------------------------------------------------------ */
 
 void orc_tc3a2a4b2b5(const orc_problem *p, const int *nb, const double *phi_d, double *hphi_d, int nthreads)
@@ -1057,7 +1282,7 @@ void orc_tc3a2a4b2b5(const orc_problem *p, const int *nb, const double *phi_d, d
 
     for (int iq2 = 0; iq2 < p->nk; ++iq2) {                            /* :165 */
         const int n2 = p->nocc[iq2];
-        if (n2 == 0) continue;
+        if (n2 == 0 || zero_weight(p, iq2)) continue;
         double q2c[3];
         kcart(p, p->kfrac + 3 * iq2, q2c);
         const long co2 = occ_coef_offset(p, iq2), fo2 = occ_offset(p, iq2);
@@ -1068,7 +1293,7 @@ void orc_tc3a2a4b2b5(const orc_problem *p, const int *nb, const double *phi_d, d
         /* (i) q1-q2 phase (:210-443) */
         double *du = malloc(sizeof(double) * 6 * N);
         for (int iq1 = 0; iq1 < p->nk; ++iq1) {
-            if (p->nocc[iq1] == 0) continue;
+            if (p->nocc[iq1] == 0 || zero_weight(p, iq1)) continue;
             double q1c[3], v[3];
             kcart(p, p->kfrac + 3 * iq1, q1c);
             for (int d = 0; d < 3; ++d) v[d] = q2c[d] - q1c[d];
@@ -1248,5 +1473,94 @@ void orc_tc3a2a4b2b5(const orc_problem *p, const int *nb, const double *phi_d, d
     orc_fft3d((double *)V3a2, p->n, +1);
     apply_local_potential(p, nb, phi_d, hphi_d, X, fb2);
     apply_local_potential(p, nb, phi_d, hphi_d, V3a2, fa2);
+
+    /* [3a4,b5] kVaux terms at k off the weighted mesh (:1036-1299).  The two |kVaux|^2
+     * terms sit inside the (q1) loop, so they count once per q1 with occupied orbitals
+     * (one process: the /num_mpi_processes of :1260-1261, :1278-1279 is 1). */
+    if (need_div) {
+        cplx *phiq2 = malloc(sizeof(cplx) * N), *chiq2 = malloc(sizeof(cplx) * N);
+        cplx *phiq1 = malloc(sizeof(cplx) * N), *chiq1 = malloc(sizeof(cplx) * N);
+        cplx *pp = malloc(sizeof(cplx) * N), *pj = malloc(sizeof(cplx) * N);
+        cplx *orb = malloc(sizeof(cplx) * N), *orb2 = malloc(sizeof(cplx) * N), *phijG = malloc(sizeof(cplx) * N);
+        double *duk = malloc(sizeof(double) * N);
+        for (int ik = 0; ik < p->nk; ++ik) {
+            double kv[3];
+            kvaux_of(p, ik, kv);
+            for (int d = 0; d < 3; ++d) kv[d] *= -p->A[0];                         /* :1064-1068 */
+            const double kv2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+            if (!(kv2 > 1e-8) || nb[ik] == 0) continue;
+            const div_set D = div_states(p, ik);
+            const long ao = app_coef_offset(p, nb, ik);
+            const int npwk = p->npw[ik];
+            const int *gi = p->gindex + pw_offset(p, ik);
+            double kc[3];
+            kcart(p, p->kfrac + 3 * ik, kc);
+            for (int iq1 = 0; iq1 < p->nk; ++iq1) {                                 /* :1073-1104 */
+                if (p->nocc[iq1] == 0 || zero_weight(p, iq1)) continue;
+                double q1c[3];
+                kcart(p, p->kfrac + 3 * iq1, q1c);
+                for (long r = 0; r < N; ++r) {                                      /* :1106-1110 */
+                    double x[3], g2 = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        x[d] = kc[d] - q1c[d] + G[3 * r + d];
+                        g2 += x[d] * x[d];
+                    }
+                    duk[r] = (kv[0] * x[0] + kv[1] * x[1] + kv[2] * x[2]) * orc_uk(p, g2, 0, 0);
+                }
+                const long co1 = occ_coef_offset(p, iq1), fo1 = occ_offset(p, iq1);
+                for (int b2 = 0; b2 < D.n; ++b2) {                                  /* :1114-1152 */
+                    const double f2 = D.fill[b2];
+                    if (f2 < 1e-8) continue;
+                    orbital_grid(p, ik, D.phi + (long)b2 * npwk, phiq2, N);
+                    const cplx *c2 = phiq2;
+                    if (bitc) {
+                        orbital_grid(p, ik, D.chi + (long)b2 * npwk, chiq2, N);
+                        c2 = chiq2;
+                    }
+                    memset(orb, 0, sizeof(cplx) * N);                               /* [3a4] */
+                    memset(orb2, 0, sizeof(cplx) * N);                              /* [3b5] */
+                    for (int ib1 = 0; ib1 < p->nocc[iq1]; ++ib1) {                  /* :1154-1227 */
+                        orbital_grid(p, iq1, occ + co1 + (long)ib1 * p->npw[iq1], phiq1, N);
+                        const cplx *c1 = phiq1;
+                        if (bitc) {
+                            orbital_grid(p, iq1, occl + co1 + (long)ib1 * p->npw[iq1], chiq1, N);
+                            c1 = chiq1;
+                        }
+                        for (long r = 0; r < N; ++r) pp[r] = phiq2[r] * conj(c1[r]);
+                        orc_fft3d((double *)pp, p->n, -1);
+                        for (long r = 0; r < N; ++r) pj[r] = pp[r] * duk[r];
+                        orc_fft3d((double *)pj, p->n, +1);
+                        const double f1 = p->filling[fo1 + ib1];
+                        for (long r = 0; r < N; ++r) orb[r] += f1 * pj[r] * phiq1[r];
+                        if (bitc) {
+                            for (long r = 0; r < N; ++r) pp[r] = c2[r] * conj(phiq1[r]);
+                            orc_fft3d((double *)pp, p->n, -1);
+                            for (long r = 0; r < N; ++r) pj[r] = pp[r] * duk[r];
+                            orc_fft3d((double *)pj, p->n, +1);
+                        }
+                        for (long r = 0; r < N; ++r) orb2[r] -= f1 * pj[r] * c1[r];
+                    }
+                    orc_fft3d((double *)orb, p->n, -1);
+                    orc_fft3d((double *)orb2, p->n, -1);
+                    const cplx *pk2 = D.phi + (long)b2 * npwk;
+                    const cplx *ck2 = D.chi + (long)b2 * npwk;
+                    for (int jb = 0; jb < nb[ik]; ++jb) {                           /* :1231-1290 */
+                        const cplx *pjc = phi + ao + (long)jb * npwk;
+                        cplx *out = hphi + ao + (long)jb * npwk;
+                        scatter(p, ik, pjc, phijG, N);
+                        const cplx c2j = zdotc(ck2, pjc, npwk);
+                        const cplx a1 = fa4 * f2 * c2j, a2 = fa4 * f2 * f2 * c2j * kv2;
+                        for (int l = 0; l < npwk; ++l) out[l] += a1 * orb[gi[l]] + a2 * pk2[l];
+                        cplx c5 = 0.0;
+                        for (long r = 0; r < N; ++r) c5 += conj(orb2[r]) * phijG[r];
+                        c5 = (c5 + f2 * c2j * kv2) * (fb5 * f2);
+                        for (int l = 0; l < npwk; ++l) out[l] += c5 * pk2[l];
+                    }
+                }
+            }
+        }
+        free(phiq2); free(chiq2); free(phiq1); free(chiq1); free(pp); free(pj); free(orb); free(orb2);
+        free(phijG); free(duk);
+    }
     free(G); free(uG); free(V3a2); free(B2); free(X); free(app_k); free(app_j);
 }

@@ -16,8 +16,10 @@
  *   - Cartesian G = recip^T * m with recip rows b_i; Miller m_d = i_d if i_d <= n_d/2 else i_d-n_d.
  *
  * Scope of this restatement: identity-only symmetry (every k irreducible,
- * one isym, no time reversal, phase factor 1), SCF mode, collinear spins
- * with num_independent_spins = 1 (the spin-unpolarised configurations).
+ * one isym, no time reversal, phase factor 1), collinear spins with
+ * num_independent_spins = 1 (the spin-unpolarised configurations); SCF k-points
+ * on the mesh and, through kweight, zero-weight SCF k-points and BAND k-points
+ * (one k list whose zero-weight entries are the band k-points).
  */
 #ifndef TCX_ORACLE_H
 #define TCX_ORACLE_H
@@ -57,6 +59,19 @@ typedef struct {
     /* real-space density [2][N] complex128 (BlochStates::density_); may be NULL
      * for terms that do not use it */
     const double *density;
+    /* k-points off the weighted mesh: BAND k-points and zero-weight SCF k-points
+     * (Kpoints::kweight_scf < 1e-8).  kweight NULL: every k is a weighted mesh point.
+     * A zero-weight k is no source of the q-sums (it holds no occupied orbitals,
+     * nocc = 0), is matched against the weighted k only (if_match), and its divergence
+     * terms project on the states zw_* (phik_band / phik_scf there) with their
+     * Kpoints::return_band_filling values; at a weighted k they project on the
+     * occupied orbitals at k. */
+    const double *kweight;  /* nk or NULL */
+    const double *sum_kvaux;/* nk*3, Potentials::sum_of_kVaux_(scf|band); NULL = 0 */
+    const int *zw_nstate;   /* nk: states per zero-weight k (0 at weighted k); NULL = none */
+    const double *zw_fill;  /* concatenated over k */
+    const double *zw_phi;   /* complex128 [k][b][npw_k] */
+    const double *zw_chi;   /* BITC left states (same layout) or NULL (= zw_phi) */
 } orc_problem;
 
 /* 3D FFT in place on a complex128 grid (interleaved doubles).
@@ -75,6 +90,13 @@ double orc_tc_2body(const orc_problem *p, double G2, int s1, int s2);
  * Writes alpha and sum_of_Vaux[nk] (single process: ibegin..iend covers everything). */
 void orc_sum_of_vaux(const double recip[9], double volume, int nk, const double *kfrac,
                      double *alpha_out, double *sum_out);
+
+/* The same with the mesh restricted to the weighted k (kweight >= 1e-8) and the vector
+ * sums of Potentials::set_sum_of_kVaux_each (potentials_coulomb.cpp:163-254):
+ * ksum_out[nk*3] = 4 pi sum_{q,G} exp(-alpha |k-q+G|^2) (k-q+G) / |k-q+G|^2, zero at k
+ * that match a weighted k.  ksum_out may be NULL. */
+void orc_sum_of_vaux_w(const double recip[9], double volume, int nk, const double *kfrac, const double *kweight,
+                       double *alpha_out, double *sum_out, double *ksum_out);
 
 /* density_s(r), reference src/bloch_states_scfloop_filling_density.cpp:256-358.
  * out: [2][N] complex128. */

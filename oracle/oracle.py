@@ -41,6 +41,12 @@ class OrcProblem(C.Structure):
         ("alpha", C.c_double),
         ("sum_vaux", _dp),
         ("density", _dp),
+        ("kweight", _dp),
+        ("sum_kvaux", _dp),
+        ("zw_nstate", _ip),
+        ("zw_fill", _dp),
+        ("zw_phi", _dp),
+        ("zw_chi", _dp),
     ]
 
 
@@ -67,6 +73,7 @@ def lib():
         L.orc_tc_2body.argtypes = [P, C.c_double, C.c_int, C.c_int]
         L.orc_tc_2body.restype = C.c_double
         L.orc_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.orc_sum_of_vaux_w.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_density.argtypes = [P, _dp]
         for name in ("orc_hf2x", "orc_tc2x", "orc_tc3a3b3b4b6", "orc_tc3a2a4b2b5"):
             getattr(L, name).argtypes = [P, _ip, _dp, _dp, C.c_int]
@@ -107,6 +114,18 @@ def sum_of_vaux(recip, volume, kfrac):
     return float(alpha[0]), out
 
 
+def sum_of_vaux_w(recip, volume, kfrac, kweight):
+    """(alpha, sum_of_Vaux[nk], sum_of_kVaux[nk, 3]) with the q-mesh restricted to the k
+    of weight >= 1e-8 (potentials_coulomb.cpp:60-254)."""
+    r = np.ascontiguousarray(recip, dtype=np.float64).reshape(9)
+    kf = np.ascontiguousarray(kfrac, dtype=np.float64).reshape(-1)
+    kw = np.ascontiguousarray(kweight, dtype=np.float64)
+    nk = len(kf) // 3
+    out, kout, alpha = np.zeros(nk), np.zeros(3 * nk), np.zeros(1)
+    lib().orc_sum_of_vaux_w(_p(r), float(volume), nk, _p(kf), _p(kw), _p(alpha), _p(out), _p(kout))
+    return float(alpha[0]), out, kout.reshape(nk, 3)
+
+
 class Oracle:
     """Holds the ctypes problem struct (and keeps its arrays alive) for one System."""
 
@@ -140,7 +159,26 @@ class Oracle:
         P.F[:] = list(np.asarray(sys.F, dtype=np.float64).reshape(4))
         P.jastrow_A_zero = int(sys.jastrow_A_zero)
         P.div_corr = int(sys.div_corr if div_corr is None else div_corr)
-        alpha, svaux = sum_of_vaux(sys.recip, sys.volume, sys.kfrac)
+        kweight = getattr(sys, "kweight", None)
+        if kweight is None:
+            alpha, svaux = sum_of_vaux(sys.recip, sys.volume, sys.kfrac)
+        else:
+            alpha, svaux, skv = sum_of_vaux_w(sys.recip, sys.volume, sys.kfrac, kweight)
+            self.kweight = self._hold(np.ascontiguousarray(kweight, dtype=np.float64))
+            self.skv = self._hold(np.ascontiguousarray(skv, dtype=np.float64).reshape(-1))
+            P.kweight = _p(self.kweight)
+            P.sum_kvaux = _p(self.skv)
+            zw = sys.zero_weight_states
+            if zw is not None:
+                self.zw_n = self._hold(np.ascontiguousarray(zw["nstate"], dtype=np.int32))
+                self.zw_f = self._hold(np.ascontiguousarray(zw["fill"], dtype=np.float64))
+                self.zw_p = self._hold(np.ascontiguousarray(zw["phi"], dtype=np.complex128))
+                P.zw_nstate = _p(self.zw_n, _ip)
+                P.zw_fill = _p(self.zw_f)
+                P.zw_phi = self.zw_p.view(np.float64).ctypes.data_as(_dp)
+                if zw.get("chi") is not None:
+                    self.zw_c = self._hold(np.ascontiguousarray(zw["chi"], dtype=np.complex128))
+                    P.zw_chi = self.zw_c.view(np.float64).ctypes.data_as(_dp)
         P.alpha = alpha
         self.svaux = self._hold(svaux)
         P.sum_vaux = _p(self.svaux)

@@ -133,6 +133,13 @@ class System:
     # k-mesh symmetry (tcx_set_kpoint_symmetry) or None for identity symmetry:
     # {"k_src": (nk,), "rot": (nk, 3, 3), "trans": (nk, 3), "trev": (nk,)}
     symmetry: dict | None = None
+    # zero-weight k-points (BAND k-points, fake-SCF k-points): kweight[k] < 1e-8 marks
+    # one; None = every k is a weighted mesh point.  Zero-weight k hold no occupied
+    # orbitals (nocc = 0); the states their divergence terms project on, with their
+    # return_band_filling values: {"nstate": (nk,), "fill": concat, "phi": concat
+    # [k][b][npw_k], "chi": BITC left states or None}
+    kweight: np.ndarray | None = None
+    zero_weight_states: dict | None = None
 
     @property
     def nk(self) -> int:
@@ -140,7 +147,10 @@ class System:
 
     @property
     def num_kpoints(self) -> int:
-        return self.nk
+        """Nk of the 1/(Nk V) prefactors: the weighted k-points (Kpoints::num_kpoints)."""
+        if self.kweight is None:
+            return self.nk
+        return int(np.sum(np.asarray(self.kweight) >= 1e-8))
 
     @property
     def ngrid(self) -> int:
@@ -327,3 +337,37 @@ def applied_bands(sys: System, nb=None, seed=None):
         c /= np.linalg.norm(c, axis=1, keepdims=True)
         blocks.append(c.reshape(-1))
     return nb, np.concatenate(blocks).astype(np.complex128)
+
+
+def with_zero_weight_kpoints(sys: System, kfrac_extra, from_k) -> System:
+    """`sys` plus zero-weight k-points (BAND k-points): extra k at fractional coordinates
+    kfrac_extra[e], each carrying the plane-wave index list of the weighted k-point
+    from_k[e] and, as the states its divergence terms project on, the occupied orbitals
+    (right and left) of from_k[e] with their fillings.  The extra k hold no occupied
+    orbitals, so they are no source of the q-sums."""
+    kfrac_extra = np.asarray(kfrac_extra, dtype=np.float64).reshape(-1, 3)
+    from_k = [int(k) for k in from_k]
+    assert sys.kweight is None and len(from_k) == len(kfrac_extra)
+    po, oo = sys.pw_offsets(), sys.occ_offsets()
+    fo = np.concatenate([[0], np.cumsum(sys.nocc)]).astype(np.int64)
+    gidx = [sys.gindex] + [sys.gindex[po[k]:po[k + 1]] for k in from_k]
+    zphi = [sys.phi_occ[oo[k]:oo[k + 1]] for k in from_k]
+    zchi = [sys.chi_occ[oo[k]:oo[k + 1]] for k in from_k] if sys.chi_occ is not None else None
+    zfill = [sys.filling[fo[k]:fo[k + 1]] for k in from_k]
+    nk0 = sys.nk
+    return System(**{
+        **sys.__dict__,
+        "kfrac": np.concatenate([np.asarray(sys.kfrac, np.float64).reshape(-1, 3), kfrac_extra]),
+        "npw": np.concatenate([sys.npw, [sys.npw[k] for k in from_k]]).astype(np.int32),
+        "gindex": np.concatenate(gidx).astype(np.int32),
+        "nocc": np.concatenate([sys.nocc, np.zeros(len(from_k), np.int32)]).astype(np.int32),
+        "kweight": np.concatenate([np.ones(nk0), np.zeros(len(from_k))]),
+        "zero_weight_states": {
+            "nstate": np.concatenate([np.zeros(nk0, np.int32), [sys.nocc[k] for k in from_k]]).astype(np.int32),
+            "fill": np.concatenate(zfill),
+            "phi": np.concatenate(zphi),
+            "chi": np.concatenate(zchi) if zchi is not None else None,
+        },
+        "symmetry": None,
+        "name": sys.name + " + zero-weight k",
+    })
