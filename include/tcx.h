@@ -149,6 +149,20 @@ int tcx_sum_of_vaux(const double recip_rows[9], double volume, int nk, const dou
 int tcx_hf2x(tcx_context *ctx, const int *nb, const double *phi, double *h2phi,
              int precision, unsigned flags);
 
+/* tcx_hf2x without device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory while they run (serving loops, code that must not
+ * call hipMalloc).  phi and h2phi are device pointers of the context's GPU (h2phi is
+ * accumulated into, as for tcx_hf2x); the occupied orbitals must already be in real
+ * space in `precision`, which the first tcx_hf2x call of that precision after
+ * tcx_set_states does (else TCX_ERR_STATE).  The working memory -- launch tables, the
+ * applied bands' grids, the accumulators, the pair records -- is one fixed device array
+ * of 256 MiB per GPU, held until the process ends; the bands are processed in chunks
+ * that fit it, and a grid whose single band pair does not fit is refused with
+ * TCX_ERR_ARG.  Every context on a GPU shares that GPU's array, so calls that overlap
+ * on one GPU race.  Synchronous; not capturable into a graph. */
+int tcx_hf2x_noalloc(tcx_context *ctx, const int *nb, const double *phi, double *h2phi,
+                     int precision);
+
 /* TC/BITC two-body exchange [2a_x, 2b_x1, 2b_x2]: replaces
  * calc_hamiltonian::tc2x (src/include/calc_hamiltonian.hpp:77-86;
  * src/calc_hamiltonian_tc2x.cpp:11-516). */

@@ -33,6 +33,7 @@ SOURCES = [
     "csrc/kernels_tc3xy.hip",
     "csrc/kernels_tc3z.hip",
     "csrc/kernels_davidson.hip",
+    "csrc/tcx_noalloc.hip",
     "csrc/tcx_api.cpp",
     "csrc/tcx_tc.cpp",
     "csrc/tcx_tc3x.cpp",

@@ -822,6 +822,9 @@ int tcx_create(tcx_context** out, int device)
         delete c;
         return TCX_ERR_HIP;
     }
+    // address of the no-allocation entry point's device array on this device (once per
+    // device; a failure is reported by tcx_hf2x_noalloc, the other entry points do not use it)
+    (void)tcx::scratch_setup(device);
     *out = c;
     return TCX_OK;
 }
@@ -894,6 +897,11 @@ int tcx_set_density(tcx_context* ctx, const double* density)
 int tcx_hf2x(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision, unsigned flags)
 {
     return guarded(ctx, [&] { ctx->impl.hf2x(nb, phi, h2phi, precision, flags); });
+}
+
+int tcx_hf2x_noalloc(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision)
+{
+    return guarded(ctx, [&] { ctx->impl.hf2x_noalloc(nb, phi, h2phi, precision); });
 }
 
 int tcx_tc2x(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision, unsigned flags)

@@ -26,6 +26,11 @@ struct Error : std::runtime_error {
                                std::string(#call) + ": " + hipGetErrorString(e_));            \
     } while (0)
 
+// The fixed device array of tcx_hf2x_noalloc (tcx_noalloc.hip): looks up its address on
+// `device` (which must be current) once per device; returns a TCX_ status.
+int scratch_setup(int device);
+size_t scratch_bytes();
+
 // Grid sizes (per axis) with compiled kernels.
 constexpr int kSizes[] = {6, 8, 12, 16, 20, 24, 32, 36, 40, 48};
 bool size_supported(int n);

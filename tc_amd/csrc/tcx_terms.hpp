@@ -252,6 +252,9 @@ struct Context {
     void finish_W(const Plan& P, const ApplyIO& io, double factor);
 
     void hf2x(const int* nb, const double* phi, double* out, int prec, unsigned flags);
+    // hf2x on device pointers with all working memory in the library's fixed device array
+    // (tcx_noalloc.hip); no device allocation
+    void hf2x_noalloc(const int* nb, const double* phi, double* out, int prec);
     void tc2x(const int* nb, const double* phi, double* out, int prec, unsigned flags);
     void tc3_local(int which, const int* nb, const double* phi, double* out, int prec, unsigned flags);
 

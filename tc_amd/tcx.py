@@ -38,7 +38,7 @@ EXPORTS = [
     "tcx_set_cell", "tcx_set_kpoints", "tcx_set_basis", "tcx_set_states",
     "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux", "tcx_set_kpoint_symmetry",
     "tcx_get_density", "tcx_set_density",
-    "tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
+    "tcx_hf2x", "tcx_hf2x_noalloc", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
     "tcx_davidson", "tcx_total_energy", "tcx_eig_dense", "tcx_set_filling",
     "tcx_graph_begin", "tcx_graph_end", "tcx_graph_launch", "tcx_graph_destroy",
@@ -86,6 +86,7 @@ def load_library():
     L.tcx_set_density.argtypes = [vp, vp]
     for name in ("tcx_hf2x", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5"):
         getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_int, C.c_uint]
+    L.tcx_hf2x_noalloc.argtypes = [vp, _ip, vp, vp, C.c_int]
     for name in ("tcx_kinetic", "tcx_hf2h", "tcx_tc2h"):
         getattr(L, name).argtypes = [vp, _ip, vp, vp, C.c_uint]
     L.tcx_apply_all.argtypes = [vp, _ip, vp, vp, vp, vp, C.c_int, C.c_uint]
