@@ -140,6 +140,32 @@ int tcx_set_density(tcx_context *ctx, const double *density);
 int tcx_sum_of_vaux(const double recip_rows[9], double volume, int nk, const double *kfrac,
                     double *alpha, double *sum_of_vaux);
 
+/* The same with zero-weight k-points in the list: replaces Potentials::set_sum_of_Vaux_each
+ * and set_sum_of_kVaux_each (src/potentials_coulomb.cpp:60-254), whose q-sums run over the
+ * weighted k only (kweight[k] >= 1e-8).  Out: alpha, sum_of_vaux[nk] and, when not NULL,
+ * sum_of_kvaux[nk][3] (zero at k on the weighted mesh).  No GPU. */
+int tcx_sum_of_vaux_weighted(const double recip_rows[9], double volume, int nk, const double *kfrac,
+                             const double *kweight, double *alpha, double *sum_of_vaux, double *sum_of_kvaux);
+
+/* Zero-weight k-points: the BAND k-points of a band calculation (kvectors_band) and the
+ * zero-weight SCF k-points (Kpoints::kweight_scf < 1e-8) of the k list.  Replaces the
+ * BAND / zero-weight branches of the terms: calc_hamiltonian_hf2x.cpp:150-201,
+ * tc2x.cpp:117-222 and :408-513, tc3a3b3b4b6.cpp:393-627 and :772-858,
+ * tc3a2a4b2b5.cpp:1036-1299 (BlochStates::phik_band, Kpoints::return_band_filling).
+ * After tcx_set_states (which holds nocc = 0 at these k) and the divergence correction
+ * (sum_of_vaux from tcx_sum_of_vaux_weighted):
+ *   kweight      [nk]: < 1e-8 marks a zero-weight k (no source of the q-sums);
+ *   sum_of_kvaux [nk][3]: the vector shell sums of tcx_sum_of_vaux_weighted;
+ *   nstate       [nk]: states the divergence terms at a zero-weight k project on (0 at
+ *                weighted k, at most 512), with their band fillings `filling` and
+ *                coefficients phi (and BITC chi), complex128 ragged [k][b][npw_k].
+ * Bands applied at a zero-weight k see the occupied orbitals as sources and get the
+ * reference's divergence terms at that k, including the kVaux branches off the mesh.
+ * Cleared by tcx_set_states, tcx_set_basis and tcx_set_kpoints; not combined with
+ * tcx_set_kpoint_symmetry. */
+int tcx_set_zero_weight_kpoints(tcx_context *ctx, const double *kweight, const double *sum_of_kvaux,
+                                const int *nstate, const double *filling, const double *phi, const double *chi);
+
 /* ---------------------------------------------------------------- terms */
 /* nb[k] bands at every k in phi (ragged, see above); hphi has the same
  * layout and is accumulated into. */

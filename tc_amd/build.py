@@ -34,11 +34,13 @@ SOURCES = [
     "csrc/kernels_tc3z.hip",
     "csrc/kernels_davidson.hip",
     "csrc/tcx_noalloc.hip",
+    "csrc/kernels_zw.hip",
     "csrc/tcx_api.cpp",
     "csrc/tcx_tc.cpp",
     "csrc/tcx_tc3x.cpp",
     "csrc/tcx_davidson.cpp",
     "csrc/tcx_adjoint.cpp",
+    "csrc/tcx_zeroweight.cpp",
     "host/calc_hamiltonian.cpp",
     "host/scf.cpp",
 ]

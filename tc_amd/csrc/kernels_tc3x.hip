@@ -210,14 +210,16 @@ __global__ __launch_bounds__(256) void k_b6_tG(const double2* __restrict__ nG, d
     tG[r] = dadd(dscale(nG[r], c0), dscale(nG[N + r], c1));
 }
 
-//   t1[k] = sum_js 4 pi (dcp[k] A_js + u(0, 0, js)) A_js n_js(G=0)
-__global__ void k_b6_t1(const double2* __restrict__ nG, const double* __restrict__ dcp, int nk, long long N,
-                        JastrowRPA j0, JastrowRPA j1, double2* __restrict__ t1)
+//   t1[k] = sum_js 4 pi (dcp[k] A_js + u(0, 0, js) [k on the weighted mesh]) A_js n_js(G=0)
+// (match null: every k is on the mesh)
+__global__ void k_b6_t1(const double2* __restrict__ nG, const double* __restrict__ dcp, const int* __restrict__ match,
+                        int nk, long long N, JastrowRPA j0, JastrowRPA j1, double2* __restrict__ t1)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk) return;
-    const double a0 = kFourPi * (dcp[k] * j0.A + jastrow_uk(j0, 0.0)) * j0.A;
-    const double a1 = kFourPi * (dcp[k] * j1.A + jastrow_uk(j1, 0.0)) * j1.A;
+    const double m = (match == nullptr || match[k]) ? 1.0 : 0.0;
+    const double a0 = kFourPi * (dcp[k] * j0.A + m * jastrow_uk(j0, 0.0)) * j0.A;
+    const double a1 = kFourPi * (dcp[k] * j1.A + m * jastrow_uk(j1, 0.0)) * j1.A;
     t1[k] = dadd(dscale(nG[0], a0), dscale(nG[N], a1));
 }
 
@@ -226,7 +228,8 @@ __global__ void k_b6_t1(const double2* __restrict__ nG, const double* __restrict
 //   Ib    = <chi_b|phi_j>  (= conj(FFT_forward(chi_b conj(phi_j))(G=0)))
 //   dot_b = sum_G conj(FFT_forward(chi_b conj(phi_j))) tG
 //         = (1/N) sum_r conj(chi_b(r)) phi_j(r) tR(r)        (Parseval, tR = FFT_backward(tG))
-//   out_j += f b6 (t1[k] Ib + dot_b) phi_b                   (contributions 1 and 3)
+//   out_j += f b6 (t1[k] Ib + sgn dot_b) phi_b               (contributions 1 and 3, or 5
+//                                                             with sgn = -1 off the mesh)
 //   cR[j][b] = f Ib b6                                       (contribution 2, real space)
 constexpr int kB6MaxOcc = 512;
 
@@ -234,8 +237,9 @@ __global__ __launch_bounds__(256) void k_b6_coef(const double2* __restrict__ phi
                                                  const double2* __restrict__ occ_l, const DivEntry* __restrict__ jobs,
                                                  const double* __restrict__ fill, const double2* __restrict__ t1,
                                                  const double2* __restrict__ appR, const double2* __restrict__ chiR,
-                                                 const double2* __restrict__ tR, double fb6, long long N,
-                                                 double2* __restrict__ out, double2* __restrict__ cR, int maxocc)
+                                                 const double2* __restrict__ tR, double fb6, double sgn,
+                                                 long long N, double2* __restrict__ out, double2* __restrict__ cR,
+                                                 int maxocc)
 {
     const int j = blockIdx.x;
     const DivEntry e = jobs[j];
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(256) void k_b6_coef(const double2* __restrict__ phi
         if (t == 0) {
             double2 c = make_double2(0.0, 0.0), cr = make_double2(0.0, 0.0);
             if (f >= 1e-8) {
-                const double2 Ib = sums[0], dot = dscale(sums[1], invN);
+                const double2 Ib = sums[0], dot = dscale(sums[1], sgn * invN);
                 c = dscale(dadd(dmul(t1[e.kidx], Ib), dot), f * fb6);
                 cr = dscale(Ib, f * fb6);
             }
@@ -305,26 +309,26 @@ __global__ __launch_bounds__(256) void k_b6_W(const double2* __restrict__ cR, in
     *w = dadd(*w, dmul(acc, tR[r]));
 }
 
-void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, int nk, const Geom& g, long long N,
-               const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s)
+void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, const int* match, int nk, const Geom& g,
+               long long N, const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s)
 {
     hipLaunchKernelGGL(k_b6_tG, grid_pts(N, 1), dim3(256), 0, s, (const double2*)nG, (double2*)tG, g, N, j0, j1);
     TCX_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_b6_t1, dim3((nk + 63) / 64), dim3(64), 0, s, (const double2*)nG, dcp, nk, N, j0, j1,
+    hipLaunchKernelGGL(k_b6_t1, dim3((nk + 63) / 64), dim3(64), 0, s, (const double2*)nG, dcp, match, nk, N, j0, j1,
                        (double2*)t1);
     TCX_HIP_CHECK(hipGetLastError());
 }
 
 void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l, const DivEntry* jobs, int njobs,
                     int maxocc, const double* fill, const double* t1, const double* appR, const double* chiR,
-                    const double* occR, const double* tR, double fb6, long long N, double* out, double* cR,
-                    double* W, hipStream_t s)
+                    const double* occR, const double* tR, double fb6, double sgn, long long N, double* out,
+                    double* cR, double* W, hipStream_t s)
 {
     if (njobs == 0) return;
     if (maxocc > kB6MaxOcc) throw Error(TCX_ERR_UNSUPPORTED, "tc3a3b3b4b6: more than 512 occupied bands at one k");
     hipLaunchKernelGGL(k_b6_coef, dim3(njobs), dim3(256), 0, s, (const double2*)phi, (const double2*)occ_r,
                        (const double2*)occ_l, jobs, fill, (const double2*)t1, (const double2*)appR,
-                       (const double2*)chiR, (const double2*)tR, fb6, N, (double2*)out, (double2*)cR, maxocc);
+                       (const double2*)chiR, (const double2*)tR, fb6, sgn, N, (double2*)out, (double2*)cR, maxocc);
     TCX_HIP_CHECK(hipGetLastError());
     for_slices(njobs, [&](int j0, unsigned ny) {
         hipLaunchKernelGGL(k_b6_W, grid_pts(N, ny), dim3(256), 0, s, (const double2*)cR, maxocc, jobs,

@@ -235,6 +235,7 @@ void Context::hermitian_conj()
 
 void Context::invalidate_states()
 {
+    clear_zero_weight();  // its states live in the orbital arrays that a setter replaces
     invalidate_local();
     occR_valid[0] = occR_valid[1] = false;
     chiR_valid[0] = chiR_valid[1] = false;
@@ -505,8 +506,10 @@ void Context::ensure_occR(int prec)
 {
     if (occR_valid[prec]) return;
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
-    d_occR[prec].ensure(std::max<size_t>((size_t)n_occ * N * es, 16));
-    orbitals_to_real((const double*)d_phi_occ.p, (const OrbEntry*)d_occjobs.p, n_occ, prec, d_occR[prec].p);
+    // occupied orbitals, then the states of zero-weight k-points (divergence terms only)
+    const int ng = n_occ + n_zw;
+    d_occR[prec].ensure(std::max<size_t>((size_t)ng * N * es, 16));
+    orbitals_to_real((const double*)d_phi_occ.p, (const OrbEntry*)d_occjobs.p, ng, prec, d_occR[prec].p);
     occR_valid[prec] = true;
 }
 
@@ -514,8 +517,9 @@ void Context::ensure_chiR(int prec)
 {
     if (chiR_valid[prec]) return;
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
-    d_chiR[prec].ensure(std::max<size_t>((size_t)n_occ * N * es, 16));
-    orbitals_to_real((const double*)d_chi_occ.p, (const OrbEntry*)d_occjobs.p, n_occ, prec, d_chiR[prec].p);
+    const int ng = n_occ + n_zw;
+    d_chiR[prec].ensure(std::max<size_t>((size_t)ng * N * es, 16));
+    orbitals_to_real((const double*)d_chi_occ.p, (const OrbEntry*)d_occjobs.p, ng, prec, d_chiR[prec].p);
     chiR_valid[prec] = true;
 }
 
@@ -643,7 +647,10 @@ const Plan& Context::get_plan(const std::vector<int>& nb, int term)
             const OrbEntry e{P.app_coef[k] + (long long)j * npw[k], npw[k], (int)pw_off[k], k, -1};
             appjobs.push_back(e);
             for (int d = 0; d < 3; ++d) appgradjobs.push_back({e.coef_off, e.npw, e.gidx_off, k, d});
-            divjobs.push_back({e.coef_off, occ_coef[k], npw[k], nocc[k], (int)occ_first[k], k});
+            if (zw_set && zw_k[k])  // the states given for a zero-weight k
+                divjobs.push_back({e.coef_off, zw_coef[k], npw[k], zw_n[k], zw_first[k], k});
+            else
+                divjobs.push_back({e.coef_off, occ_coef[k], npw[k], nocc[k], (int)occ_first[k], k});
         }
     upload(P.d_appjobs, appjobs.data(), appjobs.size() * sizeof(OrbEntry));
     upload(P.d_appgradjobs, appgradjobs.data(), appgradjobs.size() * sizeof(OrbEntry));
@@ -897,6 +904,12 @@ int tcx_set_density(tcx_context* ctx, const double* density)
 int tcx_hf2x(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision, unsigned flags)
 {
     return guarded(ctx, [&] { ctx->impl.hf2x(nb, phi, h2phi, precision, flags); });
+}
+
+int tcx_set_zero_weight_kpoints(tcx_context* ctx, const double* kweight, const double* sum_of_kvaux,
+                                const int* nstate, const double* filling, const double* phi, const double* chi)
+{
+    return guarded(ctx, [&] { ctx->impl.set_zero_weight(kweight, sum_of_kvaux, nstate, filling, phi, chi); });
 }
 
 int tcx_hf2x_noalloc(tcx_context* ctx, const int* nb, const double* phi, double* h2phi, int precision)

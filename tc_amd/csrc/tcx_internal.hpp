@@ -190,12 +190,13 @@ void launch_za8_apply(int prec, const Geom& g, const void* Z, const void* occR, 
                       const PairEntry* pairs, const TargetEntry* targets, int ntargets, double* W, hipStream_t s);
 void launch_mul_dnu(int prec, const void* appR, const double* dnu, void* out, int napp, long long N, hipStream_t s);
 void launch_a3(double* W, const double* qsum, const double* dnu, int napp, long long N, double fa3, hipStream_t s);
-void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, int nk, const Geom& g, long long N,
+void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, const int* match, int nk, const Geom& g,
+               long long N,
                const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
 void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l, const DivEntry* jobs, int njobs,
                     int maxocc, const double* fill, const double* t1, const double* appR, const double* chiR,
-                    const double* occR, const double* tR, double fb6, long long N, double* out, double* cR,
-                    double* W, hipStream_t s);
+                    const double* occR, const double* tR, double fb6, double sgn, long long N, double* out,
+                    double* cR, double* W, hipStream_t s);
 void launch_a8_div(double* Tg, const double* rec, const PairEntry* pairs, const TargetEntry* targets, int ntargets,
                    const double* fill, const double* dc2, const Geom& g, long long N, const JastrowRPA& j0,
                    const JastrowRPA& j1, hipStream_t s);
@@ -223,6 +224,27 @@ void launch_tc2h_combine(const double* nG, const double* pgpG, double* V, const 
 void launch_tc2h_apply(double* orb, const double* grad, const double* V2, const double* dnu, int nb, long long N,
                        hipStream_t s);
 void launch_axpby(const double* X, const double* Y, double* out, double a, double b, long long N, hipStream_t s);
+
+// divergence terms at zero-weight k-points (kernels_zw.hip); the states projected on are
+// addressed by DivEntry like occupied orbitals
+void launch_divcorr_kv(const double* phi, const double* occ_right, const double* occ_left, const DivEntry* jobs,
+                       int njobs, int max_states, const double* fill, const double* dc, const double* kv,
+                       const int* gindex, const int* gidx_off, const double* kcart, const Geom& g, double* out,
+                       hipStream_t s);
+void launch_b6_tG_kv(const double* nG, double* tG, const double skv[3], const Geom& g, long long N,
+                     const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
+void launch_zw_b3(const double* phi, const double* occ_l, const DivEntry* jobs, int njobs, int max_states,
+                  const double* fill, const double* chiR, const double* occR, const double* occ_r, const double* dnu,
+                  const double* appR, long long N, double fb3, const double kv[3], double* cA, double* cQ,
+                  double* qsum, double* out, hipStream_t s);
+void launch_zw_b4(const DivEntry* jobs, int njobs, int max_states, const double* fill, const double* chiR,
+                  const double* occ_r, const double* qsum, long long N, double fb4, const double kv[3], double* cA,
+                  double* out, hipStream_t s);
+void launch_duk_table(double* table, const double* kqcart, int row0, int nrows, const Geom& g, const JastrowRPA& j,
+                      const double kv[3], hipStream_t s);
+void launch_zw_a4b5_apply(const double* phi, const double* occ_r, const double* occ_l, const DivEntry* jobs, int njobs,
+                          int max_states, const double* fill, const double* Wa, const double* Wb, const int* gi,
+                          long long N, double fa4, double fb5, double kv2nq, double* out, hipStream_t s);
 
 // block Davidson algebra (FP64, batched over k; vector i of k at base + 2 (off[k] + i npw[k]))
 void launch_dv_precond(const double* V, const long long* voff, const int* first, const int* cnt, int max_cnt,

@@ -76,6 +76,7 @@ void Context::hf2x_noalloc(const int* nb_, const double* phi, double* out, int p
         if (device >= 0 && device < kMaxDevices && g_scratch_status[device] > 0) base_p = g_scratch[device];
     }
     require(base_p != nullptr, TCX_ERR_HIP, "hf2x_noalloc: the scratch array of this GPU was not found at set-up");
+    require(!zw_set, TCX_ERR_UNSUPPORTED, "hf2x_noalloc: zero-weight k-points need tcx_hf2x");
     require(occR_valid[prec], TCX_ERR_STATE,
             "hf2x_noalloc: the occupied orbitals of this precision are not in real space yet; one tcx_hf2x call "
             "in this precision after tcx_set_states builds them");

@@ -55,7 +55,18 @@ void Context::tc2x(const int* nb_, const double* phi, double* out, int prec, uns
         double factor = -1.0 / (num_kpoints * volume);
         if (nspin == 1) factor /= 2.0;
         finish_W(P, io, factor);
-        apply_divcorr(P, io, (const double*)(bitc ? d_chi_occ.p : d_phi_occ.p));
+        std::vector<double> kv;
+        tc2x_kvaux(kv);
+        if (div_enabled && std::any_of(kv.begin(), kv.end(), [](double x) { return x != 0.0; })) {
+            // zero-weight k: [2a_x] with the [2b_x1], [2b_x2] kVaux terms (tc2x.cpp:117-222, :483-508)
+            upload(d_zw_kv, kv.data(), kv.size() * sizeof(double));
+            launch_divcorr_kv(io.phi, (const double*)d_phi_occ.p, (const double*)(bitc ? d_chi_occ.p : d_phi_occ.p),
+                              (const DivEntry*)P.d_divjobs.p, P.n_app, max_div_states(), (const double*)d_fill.p,
+                              (const double*)d_dc.p, (const double*)d_zw_kv.p, (const int*)d_gindex.p,
+                              (const int*)d_pwoff.p, (const double*)d_kcart.p, geo, io.out, stream);
+        } else {
+            apply_divcorr(P, io, (const double*)(bitc ? d_chi_occ.p : d_phi_occ.p));
+        }
     }
     end_apply(io);
 }

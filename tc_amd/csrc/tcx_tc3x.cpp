@@ -8,7 +8,9 @@
 // tc3a2a4b2b5 are formed at every q2 directly; with one (tcx_set_kpoint_symmetry)
 // they are formed at the irreducible q2 only and rotated to the other q2 by the
 // U^+ transforms of tc3a2a4b2b5.cpp:576-765, as the reference does.  The kVaux
-// branches vanish for k-points on the SCF mesh and are not implemented.
+// branches act at zero-weight k-points off the mesh (tcx_set_zero_weight_kpoints):
+// the off-mesh [3b6] table and the [3b3], [3a3,b4], [3b4] terms of tc3a3b3b4b6
+// (kernels_zw.hip) and the [3a4,b5] block of tc3a2a4b2b5 (a8_kvaux).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -178,24 +180,33 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             });
         }
 
-        // [3b6] divergence correction (:393-535); k is on the SCF mesh so if_match holds
+        // [3b6] divergence correction (:393-535): k on the weighted mesh (if_match) takes the
+        // shared tables; a zero-weight k off it its own kVaux . G table with the opposite sign
+        // of contribution 3 (:445-448, :519), and then the [3b3] and [3a3,b4] kVaux terms
+        // (:537-627).  The states projected on are the occupied orbitals at k, or those of
+        // tcx_set_zero_weight_kpoints at a zero-weight k.
+        const double* div_r = (const double*)d_phi_occ.p;
+        const double* div_l = (const double*)(bitc ? d_chi_occ.p : d_phi_occ.p);
+        const DivEntry* djobs = (const DivEntry*)P.d_divjobs.p;
+        const int maxst = std::max(1, max_div_states());
+        const double* occR64 = nullptr;
+        const double* chiR64 = nullptr;
         if (div_enabled) {
             std::vector<double> dcp(nk);
             for (int k = 0; k < nk; ++k) dcp[k] = 1.0 / std::sqrt(kPiHost * alpha) * NkV - sum_vaux[k];
-            int maxocc = 1;
-            for (int k = 0; k < nk; ++k) maxocc = std::max(maxocc, nocc[k]);
-            d_aux.ensure((size_t)nk * 8 + (size_t)nk * 16 + (size_t)P.n_app * maxocc * 16);
+            d_aux.ensure((size_t)nk * 8 + (size_t)nk * 16 + (size_t)P.n_app * maxst * 16 * 2);
             double* d_dcp = (double*)d_aux.p;
             double* t1 = d_dcp + nk;
             double* cR = t1 + 2 * nk;
+            double* cQ = cR + 2 * (size_t)P.n_app * maxst;
             TCX_HIP_CHECK(hipMemcpy(d_dcp, dcp.data(), nk * sizeof(double), hipMemcpyHostToDevice));
-            launch_b6(nG, tG, t1, d_dcp, nk, geo, N, j00, j01, stream);
+            launch_b6(nG, tG, t1, d_dcp, zw_set ? (const int*)d_zw_match.p : nullptr, nk, geo, N, j00, j01, stream);
             launch_copy(tR, tG, g64, stream);
             launch_fft3d(TCX_PREC_FP64, +1, tR, 1, geo, 1.0, stream);
             ensure_occR(TCX_PREC_FP64);
             if (bitc) ensure_chiR(TCX_PREC_FP64);
-            const double* occR64 = (const double*)d_occR[TCX_PREC_FP64].p;
-            const double* chiR64 = bitc ? (const double*)d_chiR[TCX_PREC_FP64].p : occR64;
+            occR64 = (const double*)d_occR[TCX_PREC_FP64].p;
+            chiR64 = bitc ? (const double*)d_chiR[TCX_PREC_FP64].p : occR64;
             const double* appR64;
             if (prec == TCX_PREC_FP64) {
                 appR64 = (const double*)d_appR.p;
@@ -204,9 +215,35 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
                 orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, TCX_PREC_FP64, d_appR64.p);
                 appR64 = (const double*)d_appR64.p;
             }
-            launch_b6_coef(io.phi, (const double*)d_phi_occ.p, (const double*)(bitc ? d_chi_occ.p : d_phi_occ.p),
-                           (const DivEntry*)P.d_divjobs.p, P.n_app, maxocc, fill, t1, appR64, chiR64, occR64, tR, fb6,
-                           N, io.out, cR, W, stream);
+            if (!zw_set) {
+                launch_b6_coef(io.phi, div_r, div_l, djobs, P.n_app, maxst, fill, t1, appR64, chiR64, occR64, tR, fb6,
+                               1.0, N, io.out, cR, W, stream);
+            } else {
+                d_zwtab.ensure(2 * g64);
+                double* tGk = (double*)d_zwtab.p;
+                double* tRk = tGk + 2 * N;
+                for (int k = 0; k < nk; ++k) {
+                    const int j0 = (int)P.app_first[k], nj = nb[k];
+                    if (nj == 0) continue;
+                    const DivEntry* jk = djobs + j0;
+                    double* Wk = W + 2 * (size_t)j0 * N;
+                    const double* aRk = appR64 + 2 * (size_t)j0 * N;
+                    if (zw_match[k]) {
+                        launch_b6_coef(io.phi, div_r, div_l, jk, nj, maxst, fill, t1, aRk, chiR64, occR64, tR, fb6, 1.0,
+                                       N, io.out, cR, Wk, stream);
+                    } else {
+                        launch_b6_tG_kv(nG, tGk, &zw_skv[3 * (size_t)k], geo, N, j00, j01, stream);
+                        launch_copy(tRk, tGk, g64, stream);
+                        launch_fft3d(TCX_PREC_FP64, +1, tRk, 1, geo, 1.0, stream);
+                        launch_b6_coef(io.phi, div_r, div_l, jk, nj, maxst, fill, t1, aRk, chiR64, occR64, tRk, fb6,
+                                       -1.0, N, io.out, cR, Wk, stream);
+                    }
+                    double kv3[3];
+                    if (!jastrow_zero && tc3_kvaux(k, kv3) && zw_n[k] > 0)
+                        launch_zw_b3(io.phi, div_l, jk, nj, zw_n[k], fill, chiR64, occR64, div_r, dnu, aRk, N, fb3, kv3,
+                                     cR, cQ, qsum + 2 * (size_t)j0 * 3 * N, io.out, stream);
+                }
+            }
         }
         // [3a3] (:629-642)
         launch_a3(W, qsum, dnu, P.n_app, N, fa3, stream);
@@ -240,6 +277,17 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             timed(2, np, [&] {
                 launch_hf2x_C(prec, geo, Z, occR, pairs, targets, L.ntargets, nullptr, W, stream);
             });
+        }
+        // [3b4] kVaux term at a zero-weight k (:772-858), on the complete qsum
+        if (div_enabled && zw_set && !jastrow_zero) {
+            double* cA = (double*)d_aux.p + nk + 2 * (size_t)nk;
+            for (int k = 0; k < nk; ++k) {
+                double kv3[3];
+                const int j0 = (int)P.app_first[k];
+                if (nb[k] == 0 || zw_match[k] || zw_n[k] == 0 || !tc3_kvaux(k, kv3)) continue;
+                launch_zw_b4(djobs + j0, nb[k], zw_n[k], fill, chiR64, div_r, qsum + 2 * (size_t)j0 * 3 * N, N, fb4,
+                             kv3, cA, io.out, stream);
+            }
         }
         finish_W(P, io, 1.0);
     }
@@ -433,6 +481,8 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
         finish_W(P, io, 1.0);
         // [3b2] (:922-992) and [3a2] (:995-1034) as one local potential
         apply_local(P, io, (const double*)c_a8.buf.p, 1.0);
+        // [3a4,b5] kVaux block at zero-weight k (:1036-1299)
+        if (div_enabled && zw_set) a8_kvaux(P, io, nb, fa4, fb5);
     }
     end_apply(io);
 }

@@ -178,6 +178,30 @@ struct Context {
     // the other orientation (local potentials, tc3a2a4b2b5's q-sum products, the gradient
     // grids) are kept beside the current ones and swapped with them, so that alternating
     // H and H^dagger applies do not rebuild them; every invalidation clears both.
+    // Zero-weight k-points (tcx_set_zero_weight_kpoints, tcx_zeroweight.cpp): zw_k[k] marks
+    // one; zw_match[k]: k lies on the weighted mesh; zw_skv: sum_of_kVaux [nk][3]; the
+    // states of k are grids zw_first[k] .. + zw_n[k] (behind the n_occ occupied ones) and
+    // coefficients from zw_coef[k] in the orbital arrays.
+    bool zw_set = false;
+    std::vector<char> zw_k;
+    std::vector<int> zw_match, zw_n, zw_first;
+    std::vector<long long> zw_coef;
+    std::vector<double> zw_skv;
+    int n_zw = 0;
+    DBuf d_zw_match, d_zw_kv, d_pwoff, d_zwtab;
+    void set_zero_weight(const double* kweight, const double* skv, const int* nstate, const double* filling,
+                         const double* phi, const double* chi);
+    void clear_zero_weight();
+    void tc2x_kvaux(std::vector<double>& kv) const;
+    bool tc3_kvaux(int k, double kv[3]) const;
+    // states a divergence job projects on at most (occupied orbitals, or zero-weight states)
+    int max_div_states() const
+    {
+        int m = 0;
+        for (int k = 0; k < nk; ++k) m = std::max(m, zw_set && zw_k[k] ? zw_n[k] : nocc[k]);
+        return m;
+    }
+
     bool conj_mode = false;
     LocalCache a_a8, a_hartree, a_tc2h, a_tc3a1, a_tc3b1;
     DBuf a_Q4T, a_Q5T, a_gradR[2];
@@ -267,6 +291,7 @@ struct Context {
     void tc3a3b3b4b6(const int* nb, const double* phi, double* out, int prec, unsigned flags);
     void tc3a2a4b2b5(const int* nb, const double* phi, double* out, int prec, unsigned flags);
     void a8_occupied_phase(const std::vector<int>& nb, int prec);
+    void a8_kvaux(const Plan& P, const ApplyIO& io, const std::vector<int>& nb, double fa4, double fb5);
     void kinetic(const int* nb, const double* phi, double* out, unsigned flags);
     void hf2h(const int* nb, const double* phi, double* out, unsigned flags);
     void tc2h(const int* nb, const double* phi, double* out, unsigned flags);

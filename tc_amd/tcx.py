@@ -37,6 +37,7 @@ EXPORTS = [
     "tcx_create", "tcx_destroy", "tcx_last_error", "tcx_synchronize",
     "tcx_set_cell", "tcx_set_kpoints", "tcx_set_basis", "tcx_set_states",
     "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux", "tcx_set_kpoint_symmetry",
+    "tcx_sum_of_vaux_weighted", "tcx_set_zero_weight_kpoints",
     "tcx_get_density", "tcx_set_density",
     "tcx_hf2x", "tcx_hf2x_noalloc", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
     "tcx_kinetic", "tcx_hf2h", "tcx_tc2h", "tcx_apply_all",
@@ -81,6 +82,8 @@ def load_library():
     L.tcx_set_jastrow_rpa.argtypes = [vp, _dp, _dp]
     L.tcx_set_div_correction.argtypes = [vp, C.c_int, C.c_double, _dp]
     L.tcx_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
+    L.tcx_sum_of_vaux_weighted.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.tcx_set_zero_weight_kpoints.argtypes = [vp, _dp, _dp, _ip, _dp, vp, vp]
     L.tcx_set_kpoint_symmetry.argtypes = [vp, _ip, _ip, _dp, _ip]
     L.tcx_get_density.argtypes = [vp, vp]
     L.tcx_set_density.argtypes = [vp, vp]
@@ -171,6 +174,22 @@ def sum_of_vaux(recip, volume, kfrac):
     return float(alpha[0]), out
 
 
+def sum_of_vaux_weighted(recip, volume, kfrac, kweight):
+    """(alpha, sum_of_Vaux[nk], sum_of_kVaux[nk, 3]) with the q-sums over the k of weight
+    >= 1e-8 (Potentials::set_sum_of_Vaux_each, set_sum_of_kVaux_each), library host code."""
+    L = load_library()
+    r = np.ascontiguousarray(recip, dtype=np.float64).reshape(9)
+    kf = np.ascontiguousarray(kfrac, dtype=np.float64).reshape(-1)
+    kw = np.ascontiguousarray(kweight, dtype=np.float64)
+    nk = kf.size // 3
+    out, kout, alpha = np.zeros(nk), np.zeros(3 * nk), np.zeros(1)
+    st = L.tcx_sum_of_vaux_weighted(_ptr(r), float(volume), nk, _ptr(kf), _ptr(kw), _ptr(alpha), _ptr(out),
+                                    _ptr(kout))
+    if st != TCX_OK:
+        raise TcxError(st, "tcx_sum_of_vaux_weighted")
+    return float(alpha[0]), out, kout.reshape(nk, 3)
+
+
 class Engine:
     """One tcx_context on one GPU, loaded with a synthetic or caller-provided system."""
 
@@ -238,6 +257,20 @@ class Engine:
         self._check(self.L.tcx_set_div_correction(self.h, int(bool(enabled)), float(alpha),
                                                   None if s is None else _ptr(s)), "set_div_correction")
 
+    def set_zero_weight_kpoints(self, kweight, sum_of_kvaux, nstate, filling, phi, chi=None):
+        """Zero-weight k-points (BAND k-points) and the states their divergence terms
+        project on (tcx_set_zero_weight_kpoints)."""
+        kw = np.ascontiguousarray(kweight, dtype=np.float64)
+        skv = np.ascontiguousarray(sum_of_kvaux, dtype=np.float64).reshape(-1)
+        ns = np.ascontiguousarray(nstate, dtype=np.int32)
+        f = np.ascontiguousarray(filling, dtype=np.float64)
+        p = np.ascontiguousarray(phi, dtype=np.complex128)
+        c = None if chi is None else np.ascontiguousarray(chi, dtype=np.complex128)
+        self._check(self.L.tcx_set_zero_weight_kpoints(self.h, _ptr(kw), _ptr(skv), _ptr(ns, _ip), _ptr(f),
+                                                       p.ctypes.data if p.size else None,
+                                                       None if c is None or not c.size else c.ctypes.data),
+                    "set_zero_weight_kpoints")
+
     def set_kpoint_symmetry(self, k_src=None, rot=None, trans=None, trev=None):
         """k-mesh symmetry (None: identity).  k_src (nk,), rot (nk, 3, 3) TC++ rotations,
         trans (nk, 3) fractional translations, trev (nk,) time-reversal flags."""
@@ -277,11 +310,22 @@ class Engine:
             self.set_kpoint_symmetry(sym["k_src"], sym["rot"], sym["trans"], sym["trev"])
         self.set_jastrow_rpa(sys.A, sys.F)
         on = sys.div_corr if div_corr is None else div_corr
-        if on:
+        kweight = getattr(sys, "kweight", None)
+        if kweight is None:
             alpha, sv = sum_of_vaux(sys.recip, sys.volume, sys.kfrac)
+            skv = None
+        else:
+            alpha, sv, skv = sum_of_vaux_weighted(sys.recip, sys.volume, sys.kfrac, kweight)
+        if on:
             self.set_div_correction(True, alpha, sv)
         else:
             self.set_div_correction(False)
+        if kweight is not None:
+            zw = sys.zero_weight_states or {}
+            ns = zw.get("nstate", np.zeros(sys.nk, np.int32))
+            empty = np.zeros(0, np.complex128)
+            self.set_zero_weight_kpoints(kweight, skv, ns, zw.get("fill", np.zeros(0)), zw.get("phi", empty),
+                                         zw.get("chi") if sys.method == "BITC" else None)
 
     # --------------------------------------------------------------- terms
     def _term(self, name, nb, phi, out=None, precision=PREC_FP64):

@@ -24,6 +24,8 @@ def per_band_rel(sys_, nb, got, ref):
     bo = sys_.band_offsets(nb)
     out = []
     for ik in range(sys_.nk):
+        if nb[ik] == 0:
+            continue
         g = got[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
         r = ref[bo[ik]:bo[ik + 1]].reshape(nb[ik], -1)
         out.extend(np.linalg.norm(g - r, axis=1) / np.maximum(np.linalg.norm(r, axis=1), 1e-300))
