@@ -278,24 +278,25 @@ static void kg_of(const orc_problem *p, int ik, int l, double out[3])
 }
 
 /* ------------------------------------------------------------------ */
-/* Jastrow (RPA), jastrow_functions.cpp:6-97                            */
+/* Jastrow, jastrow_functions.cpp:6-97 (polynomial: jastrow_poly.c)     */
 /* ------------------------------------------------------------------ */
 
 double orc_uk(const orc_problem *p, double G2, int s1, int s2)
 {
-    if (p->jastrow_A_zero) return 0.0;                    /* uk :9-16, no polynomial */
-    const double A = p->A[s1 * 2 + s2], F = p->F[s1 * 2 + s2];
-    const double F2 = F * F;
-    if (G2 < 1e-12) return -FOURPI * A * F2;              /* uk_RPA :63-67 */
-    return FOURPI * A / (G2 * (G2 * F2 + 1.0));           /* :70 */
+    const int poly = p->poly && p->poly->deg > 0;
+    double v = 0.0;
+    if (!p->jastrow_A_zero) {                             /* uk :9-13 */
+        const double A = p->A[s1 * 2 + s2], F = p->F[s1 * 2 + s2];
+        const double F2 = F * F;
+        if (G2 < 1e-12) v += -FOURPI * A * F2;            /* uk_RPA :63-67 */
+        else v += FOURPI * A / (G2 * (G2 * F2 + 1.0));    /* :70 */
+    }
+    if (poly) v += orc_uk_poly(p->poly, G2, s1 * 2 + s2); /* :14-17 */
+    return v;
 }
 
-double orc_tc_2body(const orc_problem *p, double G2, int s1, int s2)
+static double tc_2body_rpa(const orc_problem *p, double G2, int s1, int s2)
 {
-    if (p->jastrow_A_zero) {                              /* tc_2body :22-39 */
-        if (G2 > 1e-12) return FOURPI / G2;
-        return 0.0;
-    }
     const double A = p->A[s1 * 2 + s2], F = p->F[s1 * 2 + s2];
     const double F2 = F * F;
     const double PIAAF2 = PI * A * A / F2;                /* PI_AAFinv2_long_, jastrow_initialize.cpp:107 */
@@ -306,6 +307,19 @@ double orc_tc_2body(const orc_problem *p, double G2, int s1, int s2)
     return FOURPI / G2 - FOURPI * A / (1.0 + F2G2)
          + (PIAAF2 / G1) * (PI * F2G2 + (4.0 + 2.0 * F2G2) * atan(FG1 / 2.0)
                             - (4.0 + 4.0 * F2G2) * atan(FG1)); /* :88-95 */
+}
+
+double orc_tc_2body(const orc_problem *p, double G2, int s1, int s2)
+{
+    const int deg = p->poly ? p->poly->deg : 0;
+    if (p->jastrow_A_zero) {                              /* tc_2body :22-39 */
+        if (deg > 0) return orc_tc2body_poly(p->poly, G2, s1 * 2 + s2);
+        if (G2 > 1e-12) return FOURPI / G2;
+        return 0.0;
+    }
+    if (deg > 1)                                          /* :41-46 */
+        return orc_tc2body_rpapoly(p->poly, G2, s1 * 2 + s2, tc_2body_rpa(p, G2, s1, s2));
+    return tc_2body_rpa(p, G2, s1, s2);
 }
 
 /* ------------------------------------------------------------------ */

@@ -72,7 +72,28 @@ typedef struct {
     const double *zw_fill;  /* concatenated over k */
     const double *zw_phi;   /* complex128 [k][b][npw_k] */
     const double *zw_chi;   /* BITC left states (same layout) or NULL (= zw_phi) */
+    /* polynomial part of the Jastrow (NULL or deg == 0: none); with A != 0 it is the
+     * RPA+polynomial form */
+    const struct orc_jastrow_poly *poly;
 } orc_problem;
+
+/* Polynomial Jastrow u_poly(r) = -sum_i c_i r^(i-1) (1 - r/L)^C theta(L - r) per spin
+ * pair [s1*2+s2] with its derived parameters (reference Jastrow members,
+ * src/include/jastrow.hpp:41-69). */
+typedef struct orc_jastrow_poly {
+    int deg;                 /* deg_poly_ (0..16) */
+    int C;                   /* C_damp_poly_ (2 or 3) */
+    int rpa;                 /* RPA terms present: RPA+polynomial derived parameters set */
+    int nr;                  /* num_rmesh_J3_integ_ */
+    double L[4], A[4], F[4];
+    double c_input[4][16];   /* c_poly_input_ after the cusp condition */
+    double c_int[4][19];     /* c_poly_internal_ */
+    double fourpi_L[4][38];  /* FourPI_power_L_ */
+    double exp_LF[4];        /* exp_LF_ */
+    double M[4][47];         /* M_integ_ */
+    double *rmesh;           /* [4][nr] rmesh_J3_integ_ */
+    double *integrand;       /* [4][nr] integrand_stored_J3_integ_ */
+} orc_jastrow_poly;
 
 /* 3D FFT in place on a complex128 grid (interleaved doubles).
  * sign = -1: FFT_forward (with 1/N); sign = +1: FFT_backward (unscaled).
@@ -82,9 +103,26 @@ void orc_fft3d(double *grid, const int n[3], int sign);
 /* Naive O(N^2) 3D DFT with the same convention (for pinning orc_fft3d). */
 void orc_dft3d_naive(const double *in, double *out, const int n[3], int sign);
 
-/* Jastrow RPA kernels, reference src/jastrow_functions.cpp:6-97. */
+/* Jastrow kernels u(G) and tc_2body(G), reference src/jastrow_functions.cpp:6-51
+ * (RPA :53-97, polynomial and RPA+polynomial :99-413 through p->poly). */
 double orc_uk(const orc_problem *p, double G2, int s1, int s2);
 double orc_tc_2body(const orc_problem *p, double G2, int s1, int s2);
+
+/* Polynomial Jastrow set-up (jastrow_initialize_polynomial.cpp:6-354): c_input[p*deg + i]
+ * (entry 0 of each pair is replaced by the cusp condition); rpa: RPA terms present (A, F
+ * used for the RPA+polynomial cross terms); recip, n: the FFT grid whose |G| choose
+ * num_rmesh_J3_integ.  Returns 0, -1 (bad argument) or -2 (J3 mesh did not converge). */
+int orc_jastrow_poly_init(orc_jastrow_poly *jp, int deg, int C, const double L[4], const double *c_input,
+                          int cusp, int rpa, const double A[4], const double F[4], const double recip[9],
+                          const int n[3]);
+void orc_jastrow_poly_free(orc_jastrow_poly *jp);
+void orc_jastrow_poly_set_rmesh(orc_jastrow_poly *jp, int nr);
+void orc_J_integ(const orc_jastrow_poly *jp, double G2, int p, double *J, int nJ, int zeroth);
+void orc_J2_integ(const orc_jastrow_poly *jp, double G1, int p, double *J2, int nJ2);
+double orc_J3_integ(const orc_jastrow_poly *jp, double G1, int p);
+double orc_uk_poly(const orc_jastrow_poly *jp, double G2, int p);
+double orc_tc2body_poly(const orc_jastrow_poly *jp, double G2, int p);
+double orc_tc2body_rpapoly(const orc_jastrow_poly *jp, double G2, int p, double rpa);
 
 /* Divergence-correction shell sums, reference src/potentials_coulomb.cpp:60-161.
  * Writes alpha and sum_of_Vaux[nk] (single process: ibegin..iend covers everything). */

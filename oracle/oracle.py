@@ -47,6 +47,26 @@ class OrcProblem(C.Structure):
         ("zw_fill", _dp),
         ("zw_phi", _dp),
         ("zw_chi", _dp),
+        ("poly", C.c_void_p),
+    ]
+
+
+class OrcJastrowPoly(C.Structure):
+    _fields_ = [
+        ("deg", C.c_int),
+        ("C", C.c_int),
+        ("rpa", C.c_int),
+        ("nr", C.c_int),
+        ("L", C.c_double * 4),
+        ("A", C.c_double * 4),
+        ("F", C.c_double * 4),
+        ("c_input", (C.c_double * 16) * 4),
+        ("c_int", (C.c_double * 19) * 4),
+        ("fourpi_L", (C.c_double * 38) * 4),
+        ("exp_LF", C.c_double * 4),
+        ("M", (C.c_double * 47) * 4),
+        ("rmesh", _dp),
+        ("integrand", _dp),
     ]
 
 
@@ -75,6 +95,17 @@ def lib():
         L.orc_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
         L.orc_sum_of_vaux_w.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_density.argtypes = [P, _dp]
+        JP = C.POINTER(OrcJastrowPoly)
+        L.orc_jastrow_poly_init.argtypes = [JP, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _ip]
+        L.orc_jastrow_poly_init.restype = C.c_int
+        L.orc_jastrow_poly_free.argtypes = [JP]
+        L.orc_J_integ.argtypes = [JP, C.c_double, C.c_int, _dp, C.c_int, C.c_int]
+        L.orc_J2_integ.argtypes = [JP, C.c_double, C.c_int, _dp, C.c_int]
+        for name in ("orc_J3_integ", "orc_uk_poly", "orc_tc2body_poly"):
+            getattr(L, name).argtypes = [JP, C.c_double, C.c_int]
+            getattr(L, name).restype = C.c_double
+        L.orc_tc2body_rpapoly.argtypes = [JP, C.c_double, C.c_int, C.c_double]
+        L.orc_tc2body_rpapoly.restype = C.c_double
         for name in ("orc_hf2x", "orc_tc2x", "orc_tc3a3b3b4b6", "orc_tc3a2a4b2b5"):
             getattr(L, name).argtypes = [P, _ip, _dp, _dp, C.c_int]
         for name in ("orc_tc3a1", "orc_tc3b1", "orc_kinetic", "orc_hf2h", "orc_tc2h"):
@@ -126,6 +157,67 @@ def sum_of_vaux_w(recip, volume, kfrac, kweight):
     return float(alpha[0]), out, kout.reshape(nk, 3)
 
 
+class JastrowPoly:
+    """The oracle's polynomial Jastrow (jastrow_poly.c) for one parameter set.
+
+    spec: {"deg": int, "C": 2|3, "L": (2, 2), "c": (2, 2, deg) c_poly_input (entry 0 of each
+    pair is replaced by the cusp condition), "cusp": bool}; A, F: the RPA parameters (A all
+    zero: polynomial only); recip, n: the FFT grid that fixes num_rmesh_J3_integ."""
+
+    def __init__(self, spec, A, F, recip, n):
+        self.s = OrcJastrowPoly()
+        deg = int(spec["deg"])
+        L = np.ascontiguousarray(spec["L"], dtype=np.float64).reshape(4)
+        c = np.ascontiguousarray(spec["c"], dtype=np.float64).reshape(4 * deg) if deg else np.zeros(1)
+        A = np.ascontiguousarray(A, dtype=np.float64).reshape(4)
+        F = np.ascontiguousarray(F, dtype=np.float64).reshape(4)
+        rpa = int(np.any(np.abs(A) > 1e-8))
+        r = np.ascontiguousarray(recip, dtype=np.float64).reshape(9)
+        nn = np.ascontiguousarray(n, dtype=np.int32)
+        rc = lib().orc_jastrow_poly_init(C.byref(self.s), deg, int(spec["C"]), _p(L), _p(c), int(spec["cusp"]), rpa,
+                                         _p(A), _p(F), _p(r), _p(nn, _ip))
+        if rc != 0:
+            raise ValueError(f"polynomial Jastrow set-up failed ({rc})")
+
+    def __del__(self):
+        try:
+            lib().orc_jastrow_poly_free(C.byref(self.s))
+        except Exception:
+            pass
+
+    @property
+    def num_rmesh(self) -> int:
+        return int(self.s.nr)
+
+    def c_input(self, p):
+        return np.array(self.s.c_input[p][:self.s.deg])
+
+    def c_internal(self, p):
+        return np.array(self.s.c_int[p][:self.s.deg + self.s.C])
+
+    def J(self, G2, p, nJ, zeroth=True):
+        out = np.zeros(nJ)
+        lib().orc_J_integ(C.byref(self.s), float(G2), p, _p(out), nJ, int(zeroth))
+        return out
+
+    def J2(self, G1, p, nJ2):
+        out = np.zeros(nJ2)
+        lib().orc_J2_integ(C.byref(self.s), float(G1), p, _p(out), nJ2)
+        return out
+
+    def J3(self, G1, p):
+        return lib().orc_J3_integ(C.byref(self.s), float(G1), p)
+
+    def uk_poly(self, G2, p):
+        return lib().orc_uk_poly(C.byref(self.s), float(G2), p)
+
+    def tc2body_poly(self, G2, p):
+        return lib().orc_tc2body_poly(C.byref(self.s), float(G2), p)
+
+    def tc2body_rpapoly(self, G2, p, rpa):
+        return lib().orc_tc2body_rpapoly(C.byref(self.s), float(G2), p, float(rpa))
+
+
 class Oracle:
     """Holds the ctypes problem struct (and keeps its arrays alive) for one System."""
 
@@ -158,6 +250,10 @@ class Oracle:
         P.A[:] = list(np.asarray(sys.A, dtype=np.float64).reshape(4))
         P.F[:] = list(np.asarray(sys.F, dtype=np.float64).reshape(4))
         P.jastrow_A_zero = int(sys.jastrow_A_zero)
+        spec = getattr(sys, "jastrow_poly", None)
+        if spec is not None and int(spec["deg"]) > 0:
+            self.poly = JastrowPoly(spec, sys.A, sys.F, sys.recip, sys.n)
+            P.poly = C.cast(C.pointer(self.poly.s), C.c_void_p)
         P.div_corr = int(sys.div_corr if div_corr is None else div_corr)
         kweight = getattr(sys, "kweight", None)
         if kweight is None:

@@ -140,6 +140,10 @@ class System:
     # [k][b][npw_k], "chi": BITC left states or None}
     kweight: np.ndarray | None = None
     zero_weight_states: dict | None = None
+    # polynomial Jastrow (tcx_set_jastrow_polynomial) or None: {"deg": int, "C": 2 or 3,
+    # "L": (2, 2), "c": (2, 2, deg) c_poly_input with entry 0 replaced by the cusp
+    # condition, "cusp": bool}; with A != 0 it is the RPA+polynomial form
+    jastrow_poly: dict | None = None
 
     @property
     def nk(self) -> int:
