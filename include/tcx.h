@@ -98,6 +98,31 @@ int tcx_set_states(tcx_context *ctx, const int *nocc, const double *filling,
  * (Jastrow::is_A_zero_). */
 int tcx_set_jastrow_rpa(tcx_context *ctx, const double A[4], const double F[4]);
 
+/* Polynomial Jastrow u_poly(r) = -sum_{i<deg} c_i r^i (1 - r/L)^C for r < L, per
+ * spin pair [s1*2+s2] (Jastrow::set_polynomial_parameters,
+ * src/jastrow_initialize_polynomial.cpp:6-101, with set_derived_polynomial_parameters
+ * :103-118 and, when the RPA part is non-zero, set_derived_RPA_polynomial_parameters
+ * :121-155).  deg in 0..16 (0 removes the polynomial part), C in {2, 3}, L[4] >= 1e-8,
+ * c_input[p*deg + i] = c_poly_input_ (entry i = 0 of each pair is replaced by the cusp
+ * condition: c_0 = (L/C)(c_1 - cusp), cusp = 1/4 parallel, 1/2 antiparallel, or 0 when
+ * cusp == 0).  u(G) becomes uk_RPA + uk_POLY and tc_2body becomes tc_2body_POLY (no RPA
+ * part) or tc_2body_RPAPOLY (deg > 1) (src/jastrow_functions.cpp:6-51, :99-413).
+ * Needs tcx_set_cell: its FFT grid chooses num_rmesh_J3_integ (:157-204).  Without a
+ * tcx_set_jastrow_rpa call the RPA part is zero; a later one (or tcx_set_cell) rebuilds
+ * the derived parameters.  Errors where the reference stops: TCX_ERR_ARG. */
+int tcx_set_jastrow_polynomial(tcx_context *ctx, int deg, int C, const double L[4], const double *c_input,
+                               int cusp);
+
+/* num_rmesh_J3_integ of the RPA+polynomial Jastrow (Jastrow::print_RPA_polynomial_parameters,
+ * src/jastrow_initialize_polynomial.cpp:426-431); 0 without that form. */
+int tcx_get_jastrow_num_rmesh(tcx_context *ctx, int *num_rmesh);
+
+/* The Jastrow kernels at n points (device pointers of the context's GPU): uk[i] = u(G) and
+ * tc2body[i] = tc_2body(G) of spin pair (s1, s2) at |G|^2 = G2[i] (Jastrow::uk,
+ * Jastrow::tc_2body; src/jastrow_functions.cpp:6-51).  Asynchronous on the context's stream. */
+int tcx_jastrow_kernels(tcx_context *ctx, int s1, int s2, long long n, const double *G2, double *uk,
+                        double *tc2body);
+
 /* Divergence (G=0) correction: Potentials::includes_div_correction,
  * alpha_Vaux and sum_of_Vaux_scf[nk] (src/potentials_coulomb.cpp:9-161). */
 int tcx_set_div_correction(tcx_context *ctx, int enabled, double alpha, const double *sum_of_vaux);

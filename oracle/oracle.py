@@ -178,6 +178,12 @@ class JastrowPoly:
                                          _p(A), _p(F), _p(r), _p(nn, _ip))
         if rc != 0:
             raise ValueError(f"polynomial Jastrow set-up failed ({rc})")
+        if spec.get("hermitian_conj"):
+            # Jastrow::take_Hermitian_conj (jastrow_functions.cpp:417-440) negates c_poly_internal_
+            # (A is negated by the caller, whose RPA part reads it too)
+            for p in range(4):
+                for i in range(19):
+                    self.s.c_int[p][i] = -self.s.c_int[p][i]
 
     def __del__(self):
         try:

@@ -41,6 +41,7 @@ SOURCES = [
     "csrc/tcx_davidson.cpp",
     "csrc/tcx_adjoint.cpp",
     "csrc/tcx_zeroweight.cpp",
+    "csrc/tcx_jastrow.cpp",
     "host/calc_hamiltonian.cpp",
     "host/scf.cpp",
 ]

@@ -19,6 +19,19 @@ namespace tcx {
 constexpr int kPencilBlock = 64;  // threads per z-pencil workgroup
 constexpr int kPlaneBlock = 64;   // threads per xy-plane workgroup (plane side <= 64)
 
+// Launch a kernel template instantiated with kPoly = (polynomial Jastrow present):
+// kernels without the polynomial part carry no call and no stack.
+#define TCX_POLY_DISPATCH(poly, ...)         \
+    do {                                     \
+        if (poly) {                          \
+            constexpr bool kPoly = true;     \
+            __VA_ARGS__;                     \
+        } else {                             \
+            constexpr bool kPoly = false;    \
+            __VA_ARGS__;                     \
+        }                                    \
+    } while (0)
+
 __device__ __forceinline__ int centered(int i, int n) { return i <= n / 2 ? i : i - n; }
 
 

@@ -38,8 +38,9 @@ __global__ __launch_bounds__(256) void k_density(const double2* __restrict__ occ
 }
 
 // out_d(G) = sum_s u(G, 0, s) G_d nG_s(G), s = 0, 1   (3 grids)
+template <bool kPoly>
 __global__ __launch_bounds__(256) void k_dnu(const double2* __restrict__ nG, double2* __restrict__ out, Geom g,
-                                             JastrowRPA j0, JastrowRPA j1, long long N)
+                                             JastrowPair j0, JastrowPair j1, long long N)
 {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= N) return;
@@ -50,15 +51,16 @@ __global__ __launch_bounds__(256) void k_dnu(const double2* __restrict__ nG, dou
         G[d] = g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
         G2 += G[d] * G[d];
     }
-    const double u0 = jastrow_uk(j0, G2), u1 = jastrow_uk(j1, G2);
+    const double u0 = jastrow_uk<kPoly>(j0, G2), u1 = jastrow_uk<kPoly>(j1, G2);
     const double2 a = nG[e], b = nG[N + e];
     const double sx = u0 * a.x + u1 * b.x, sy = u0 * a.y + u1 * b.y;
     for (int d = 0; d < 3; ++d) out[(long long)d * N + e] = make_double2(G[d] * sx, G[d] * sy);
 }
 
 // V(G) = sum_{s2} sum_d u(G, 0, s2) G_d dnu_d(G)  (tc3b1.cpp:86-100; one independent spin)
+template <bool kPoly>
 __global__ __launch_bounds__(256) void k_tc3b1_V(const double2* __restrict__ dnu, double2* __restrict__ V, Geom g,
-                                                 JastrowRPA j0, JastrowRPA j1, long long N)
+                                                 JastrowPair j0, JastrowPair j1, long long N)
 {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= N) return;
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(256) void k_tc3b1_V(const double2* __restrict__ dnu
         G[d] = g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
         G2 += G[d] * G[d];
     }
-    const double u = jastrow_uk(j0, G2) + jastrow_uk(j1, G2);
+    const double u = jastrow_uk<kPoly>(j0, G2) + jastrow_uk<kPoly>(j1, G2);
     double re = 0.0, im = 0.0;
     for (int d = 0; d < 3; ++d) {
         const double2 x = dnu[(long long)d * N + e];
@@ -116,19 +118,39 @@ void launch_density(const double* occR, const double* chiR, const double* fill, 
     TCX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_dnu(const double* nG, double* out, const Geom& g, const JastrowRPA& j0, const JastrowRPA& j1,
-                hipStream_t s)
+// uk[i] = u(G), v[i] = tc_2body(G) at |G|^2 = G2[i] (Jastrow::uk, tc_2body)
+__global__ __launch_bounds__(256) void k_jastrow_eval(const double* __restrict__ G2, double* __restrict__ uk,
+                                                      double* __restrict__ v, JastrowPair j, long long n)
 {
-    const long long N = (long long)g.n1 * g.n2 * g.n3;
-    hipLaunchKernelGGL(k_dnu, grid1(N), dim3(256), 0, s, (const double2*)nG, (double2*)out, g, j0, j1, N);
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double g2 = G2[i];
+    uk[i] = jastrow_uk(j, g2);
+    v[i] = jastrow_tc2body(j, g2);
+}
+
+void launch_jastrow_eval(const double* G2, double* uk, double* v, long long n, const JastrowPair& j, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_jastrow_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, G2, uk, v, j, n);
     TCX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_tc3b1_V(const double* dnu, double* V, const Geom& g, const JastrowRPA& j0, const JastrowRPA& j1,
+void launch_dnu(const double* nG, double* out, const Geom& g, const JastrowPair& j0, const JastrowPair& j1,
+                hipStream_t s)
+{
+    const long long N = (long long)g.n1 * g.n2 * g.n3;
+    TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_dnu<kPoly>), grid1(N), dim3(256), 0, s, (const double2*)nG,
+                                                      (double2*)out, g, j0, j1, N));
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tc3b1_V(const double* dnu, double* V, const Geom& g, const JastrowPair& j0, const JastrowPair& j1,
                     hipStream_t s)
 {
     const long long N = (long long)g.n1 * g.n2 * g.n3;
-    hipLaunchKernelGGL(k_tc3b1_V, grid1(N), dim3(256), 0, s, (const double2*)dnu, (double2*)V, g, j0, j1, N);
+    TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_tc3b1_V<kPoly>), grid1(N), dim3(256), 0, s,
+                                                      (const double2*)dnu, (double2*)V, g, j0, j1, N));
     TCX_HIP_CHECK(hipGetLastError());
 }
 

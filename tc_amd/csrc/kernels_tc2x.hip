@@ -19,9 +19,9 @@
 namespace tcx {
 
 // ----------------------------------------------------------------- kernel table (V2, u) per k-q row
-template <typename T>
+template <typename T, bool kPoly>
 __global__ __launch_bounds__(256) void k_tc_table(cpx<T>* __restrict__ table, const double* __restrict__ kqcart,
-                                                  int row0, Geom g, JastrowRPA jp, long long N)
+                                                  int row0, Geom g, JastrowPair jp, long long N)
 {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= N) return;
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_tc_table(cpx<T>* __restrict__ table, co
         const double c = kq[d] + g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
         G2 += c * c;
     }
-    table[(long long)row * N + e] = cpx<T>{static_cast<T>(jastrow_tc2body(jp, G2)), static_cast<T>(jastrow_uk(jp, G2))};
+    table[(long long)row * N + e] = cpx<T>{static_cast<T>(jastrow_tc2body<kPoly>(jp, G2)), static_cast<T>(jastrow_uk<kPoly>(jp, G2))};
 }
 
 // ----------------------------------------------------------------- pass A
@@ -235,15 +235,17 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_C(const cpx<T>* __restric
 
 // ----------------------------------------------------------------- launchers
 void launch_tc_table(int prec, void* table, const double* kqcart, int row0, int nrows, const Geom& g,
-                     const JastrowRPA& jp, hipStream_t s)
+                     const JastrowPair& jp, hipStream_t s)
 {
     if (nrows == 0) return;
     const long long N = (long long)g.n1 * g.n2 * g.n3;
     const dim3 grid((unsigned)((N + 255) / 256), nrows);
     if (prec == TCX_PREC_FP64)
-        hipLaunchKernelGGL(k_tc_table<double>, grid, dim3(256), 0, s, (cpx<double>*)table, kqcart, row0, g, jp, N);
+        TCX_POLY_DISPATCH(jp.pdeg > 0, hipLaunchKernelGGL((k_tc_table<double, kPoly>), grid, dim3(256), 0, s,
+                                                          (cpx<double>*)table, kqcart, row0, g, jp, N));
     else
-        hipLaunchKernelGGL(k_tc_table<float>, grid, dim3(256), 0, s, (cpx<float>*)table, kqcart, row0, g, jp, N);
+        TCX_POLY_DISPATCH(jp.pdeg > 0, hipLaunchKernelGGL((k_tc_table<float, kPoly>), grid, dim3(256), 0, s,
+                                                          (cpx<float>*)table, kqcart, row0, g, jp, N));
     TCX_HIP_CHECK(hipGetLastError());
 }
 

@@ -195,8 +195,9 @@ void launch_a3(double* W, const double* qsum, const double* dnu, int napp, long 
 
 // [3b6] divergence correction (tc3a3b3b4b6.cpp:424-451):
 //   tG(G) = sum_js 4 pi n_js(G) A_js u(G, 0, js) (G != 0),  tG(0) = 0
+template <bool kPoly>
 __global__ __launch_bounds__(256) void k_b6_tG(const double2* __restrict__ nG, double2* __restrict__ tG, Geom g,
-                                               long long N, JastrowRPA j0, JastrowRPA j1)
+                                               long long N, JastrowPair j0, JastrowPair j1)
 {
     const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
     if (r >= N) return;
@@ -206,20 +207,21 @@ __global__ __launch_bounds__(256) void k_b6_tG(const double2* __restrict__ nG, d
         tG[0] = make_double2(0.0, 0.0);
         return;
     }
-    const double c0 = kFourPi * j0.A * jastrow_uk(j0, x2), c1 = kFourPi * j1.A * jastrow_uk(j1, x2);
+    const double c0 = kFourPi * j0.A * jastrow_uk<kPoly>(j0, x2), c1 = kFourPi * j1.A * jastrow_uk<kPoly>(j1, x2);
     tG[r] = dadd(dscale(nG[r], c0), dscale(nG[N + r], c1));
 }
 
 //   t1[k] = sum_js 4 pi (dcp[k] A_js + u(0, 0, js) [k on the weighted mesh]) A_js n_js(G=0)
 // (match null: every k is on the mesh)
+template <bool kPoly>
 __global__ void k_b6_t1(const double2* __restrict__ nG, const double* __restrict__ dcp, const int* __restrict__ match,
-                        int nk, long long N, JastrowRPA j0, JastrowRPA j1, double2* __restrict__ t1)
+                        int nk, long long N, JastrowPair j0, JastrowPair j1, double2* __restrict__ t1)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nk) return;
     const double m = (match == nullptr || match[k]) ? 1.0 : 0.0;
-    const double a0 = kFourPi * (dcp[k] * j0.A + m * jastrow_uk(j0, 0.0)) * j0.A;
-    const double a1 = kFourPi * (dcp[k] * j1.A + m * jastrow_uk(j1, 0.0)) * j1.A;
+    const double a0 = kFourPi * (dcp[k] * j0.A + m * jastrow_uk<kPoly>(j0, 0.0)) * j0.A;
+    const double a1 = kFourPi * (dcp[k] * j1.A + m * jastrow_uk<kPoly>(j1, 0.0)) * j1.A;
     t1[k] = dadd(dscale(nG[0], a0), dscale(nG[N], a1));
 }
 
@@ -310,12 +312,13 @@ __global__ __launch_bounds__(256) void k_b6_W(const double2* __restrict__ cR, in
 }
 
 void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, const int* match, int nk, const Geom& g,
-               long long N, const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s)
+               long long N, const JastrowPair& j0, const JastrowPair& j1, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_b6_tG, grid_pts(N, 1), dim3(256), 0, s, (const double2*)nG, (double2*)tG, g, N, j0, j1);
+    TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_b6_tG<kPoly>), grid_pts(N, 1), dim3(256), 0, s,
+                                                      (const double2*)nG, (double2*)tG, g, N, j0, j1));
     TCX_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_b6_t1, dim3((nk + 63) / 64), dim3(64), 0, s, (const double2*)nG, dcp, match, nk, N, j0, j1,
-                       (double2*)t1);
+    TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_b6_t1<kPoly>), dim3((nk + 63) / 64), dim3(64), 0, s,
+                                                      (const double2*)nG, dcp, match, nk, N, j0, j1, (double2*)t1));
     TCX_HIP_CHECK(hipGetLastError());
 }
 
@@ -343,11 +346,12 @@ void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l,
 // FFT_forward(conj(chi_q2,i2) phi_q2,i1)):
 //   T(0)  -= sum_s 4 pi A_s (A_s dc2[q2] + u_s(0)) f1 pa(0) pb(0)
 //   T(G)  -= sum_s 4 pi A_s f1 u_s(G) (pa(G) pb(0) + pa(0) pb(G))
+template <bool kPoly>
 __global__ __launch_bounds__(256) void k_a8_div(double2* __restrict__ Tg, const double2* __restrict__ rec,
                                                 const PairEntry* __restrict__ pairs,
                                                 const TargetEntry* __restrict__ targets, int t0,
                                                 const double* __restrict__ fill, const double* __restrict__ dc2,
-                                                Geom g, long long N, JastrowRPA j0, JastrowRPA j1)
+                                                Geom g, long long N, JastrowPair j0, JastrowPair j1)
 {
     const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
     if (r >= N) return;
@@ -355,8 +359,8 @@ __global__ __launch_bounds__(256) void k_a8_div(double2* __restrict__ Tg, const 
     double x[3];
     const double x2 = kq_plus_G(g, r, nullptr, x);
     const double dc = dc2[t.pad];
-    const double w0 = r == 0 ? kFourPi * j0.A * (j0.A * dc + jastrow_uk(j0, 0.0)) : kFourPi * j0.A * jastrow_uk(j0, x2);
-    const double w1 = r == 0 ? kFourPi * j1.A * (j1.A * dc + jastrow_uk(j1, 0.0)) : kFourPi * j1.A * jastrow_uk(j1, x2);
+    const double w0 = r == 0 ? kFourPi * j0.A * (j0.A * dc + jastrow_uk<kPoly>(j0, 0.0)) : kFourPi * j0.A * jastrow_uk<kPoly>(j0, x2);
+    const double w1 = r == 0 ? kFourPi * j1.A * (j1.A * dc + jastrow_uk<kPoly>(j1, 0.0)) : kFourPi * j1.A * jastrow_uk<kPoly>(j1, x2);
     double2 acc = make_double2(0.0, 0.0);
     for (int i = 0; i < t.npair; ++i) {
         const int p = t.pair0 + i;
@@ -475,12 +479,13 @@ void launch_sym_rotate(double* Q, int ncomp, const SymRotJob* jobs, int njobs, c
 }
 
 void launch_a8_div(double* Tg, const double* rec, const PairEntry* pairs, const TargetEntry* targets, int ntargets,
-                   const double* fill, const double* dc2, const Geom& g, long long N, const JastrowRPA& j0,
-                   const JastrowRPA& j1, hipStream_t s)
+                   const double* fill, const double* dc2, const Geom& g, long long N, const JastrowPair& j0,
+                   const JastrowPair& j1, hipStream_t s)
 {
     for_slices(ntargets, [&](int t0, unsigned ny) {
-        hipLaunchKernelGGL(k_a8_div, grid_pts(N, ny), dim3(256), 0, s, (double2*)Tg, (const double2*)rec, pairs,
-                           targets, t0, fill, dc2, g, N, j0, j1);
+        TCX_POLY_DISPATCH(j0.pdeg > 0,
+                          hipLaunchKernelGGL((k_a8_div<kPoly>), grid_pts(N, ny), dim3(256), 0, s, (double2*)Tg,
+                                             (const double2*)rec, pairs, targets, t0, fill, dc2, g, N, j0, j1));
         TCX_HIP_CHECK(hipGetLastError());
     });
 }
@@ -570,17 +575,18 @@ __global__ __launch_bounds__(256) void k_pgp(const double2* __restrict__ chiR, c
 
 // tc2h.cpp:146-171 (one independent spin, spin_factor 2):
 //   V2(G) = sum_js [tc_2body(G,0,js) n_js(G) + u(G,0,js) sum_d G_d pgp_d(G) / 2]
+template <bool kPoly>
 __global__ __launch_bounds__(256) void k_tc2h_V(const double2* __restrict__ nG, const double2* __restrict__ pgpG,
-                                                double2* __restrict__ V, Geom g, long long N, JastrowRPA j0,
-                                                JastrowRPA j1)
+                                                double2* __restrict__ V, Geom g, long long N, JastrowPair j0,
+                                                JastrowPair j1)
 {
     const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
     if (r >= N) return;
     double x[3];
     const double x2 = kq_plus_G(g, r, nullptr, x);
     const double2 n0 = nG[r], n1 = nG[N + r];
-    double2 v = dadd(dscale(n0, jastrow_tc2body(j0, x2)), dscale(n1, jastrow_tc2body(j1, x2)));
-    const double us = jastrow_uk(j0, x2) + jastrow_uk(j1, x2);
+    double2 v = dadd(dscale(n0, jastrow_tc2body<kPoly>(j0, x2)), dscale(n1, jastrow_tc2body<kPoly>(j1, x2)));
+    const double us = jastrow_uk<kPoly>(j0, x2) + jastrow_uk<kPoly>(j1, x2);
     for (int d = 0; d < 3; ++d) v = dadd(v, dscale(pgpG[(long long)d * N + r], 0.5 * us * x[d]));
     V[r] = v;
 }
@@ -594,10 +600,11 @@ void launch_tc2h_V(const double* chiR, const double* gradR, const double* fill, 
 }
 
 void launch_tc2h_combine(const double* nG, const double* pgpG, double* V, const Geom& g, long long N,
-                         const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s)
+                         const JastrowPair& j0, const JastrowPair& j1, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tc2h_V, grid_pts(N, 1), dim3(256), 0, s, (const double2*)nG, (const double2*)pgpG,
-                       (double2*)V, g, N, j0, j1);
+    TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_tc2h_V<kPoly>), grid_pts(N, 1), dim3(256), 0, s,
+                                                      (const double2*)nG, (const double2*)pgpG, (double2*)V, g, N,
+                                                      j0, j1));
     TCX_HIP_CHECK(hipGetLastError());
 }
 
@@ -665,10 +672,10 @@ namespace tcx {
 // the energy) and sum_x dK/dA(x) dE/dK(x) (dE/dA at fixed F).
 constexpr int kMaxKernelBins = 1024;
 
-template <typename T>
+template <typename T, bool kPoly>
 __global__ __launch_bounds__(256) void k_kgrad(const cpx<T>* __restrict__ rec, int tc2, const PairEntry* __restrict__ pairs,
                                                int p0, const double* __restrict__ kqcart, const double* __restrict__ fill,
-                                               Geom g, long long N, JastrowRPA j0, JastrowRPA j1, double w0,
+                                               Geom g, long long N, JastrowPair j0, JastrowPair j1, double w0,
                                                int nbins, double width, double* __restrict__ bins_v,
                                                double* __restrict__ bins_u, double* __restrict__ sums)
 {
@@ -711,10 +718,10 @@ __global__ __launch_bounds__(256) void k_kgrad(const cpx<T>* __restrict__ rec, i
             atomicAdd(&hu[2 * b], sU.x);
             atomicAdd(&hu[2 * b + 1], sU.y);
             // spin pair (ispin, ispin) of tc2x.cpp:269-271
-            JastrowRPA jd = j0;
+            JastrowPair jd = j0;
             jd.dA = 1;
-            const double V2 = jastrow_tc2body(j0, x2), u = jastrow_uk(j0, x2);
-            const double dV2 = jastrow_tc2body(jd, x2), du = jastrow_uk(jd, x2);
+            const double V2 = jastrow_tc2body<kPoly>(j0, x2), u = jastrow_uk<kPoly>(j0, x2);
+            const double dV2 = jastrow_tc2body<kPoly>(jd, x2), du = jastrow_uk<kPoly>(jd, x2);
             s_lin[0] = V2 * sV.x + u * sU.x;
             s_lin[1] = V2 * sV.y + u * sU.y;
             s_dA[0] = dV2 * sV.x + du * sU.x;
@@ -739,17 +746,19 @@ __global__ __launch_bounds__(256) void k_kgrad(const cpx<T>* __restrict__ rec, i
 }
 
 void launch_kgrad(int prec, const void* rec, int tc2, const PairEntry* pairs, int npairs, const double* kqcart,
-                  const double* fill, const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1, double w0,
+                  const double* fill, const Geom& g, long long N, const JastrowPair& j0, const JastrowPair& j1, double w0,
                   int nbins, double width, double* bins_v, double* bins_u, double* sums, hipStream_t s)
 {
     if (nbins < 1 || nbins > kMaxKernelBins) throw Error(TCX_ERR_ARG, "kernel gradient: 1 <= nbins <= 1024");
     for_slices(npairs, [&](int p0, unsigned ny) {
         if (prec == TCX_PREC_FP64)
-            hipLaunchKernelGGL(k_kgrad<double>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<double>*)rec, tc2, pairs,
-                               p0, kqcart, fill, g, N, j0, j1, w0, nbins, width, bins_v, bins_u, sums);
+            TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_kgrad<double, kPoly>), grid_pts(N, ny), dim3(256), 0,
+                                                              s, (const cpx<double>*)rec, tc2, pairs, p0, kqcart, fill,
+                                                              g, N, j0, j1, w0, nbins, width, bins_v, bins_u, sums));
         else
-            hipLaunchKernelGGL(k_kgrad<float>, grid_pts(N, ny), dim3(256), 0, s, (const cpx<float>*)rec, tc2, pairs,
-                               p0, kqcart, fill, g, N, j0, j1, w0, nbins, width, bins_v, bins_u, sums);
+            TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_kgrad<float, kPoly>), grid_pts(N, ny), dim3(256), 0,
+                                                              s, (const cpx<float>*)rec, tc2, pairs, p0, kqcart, fill,
+                                                              g, N, j0, j1, w0, nbins, width, bins_v, bins_u, sums));
         TCX_HIP_CHECK(hipGetLastError());
     });
 }

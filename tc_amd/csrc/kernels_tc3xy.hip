@@ -107,9 +107,9 @@ struct PlanePoint {
 
 // ----------------------------------------------------------------- u table
 // tab[row][e] = (u(kq_row + G_e, j0), u(kq_row + G_e, j1)) in the working precision.
-template <typename T>
+template <typename T, bool kPoly>
 __global__ __launch_bounds__(256) void k_u2_table(cpx<T>* __restrict__ tab, const double* __restrict__ kqcart, int row0,
-                                                  Geom g, JastrowRPA j0, JastrowRPA j1, long long N)
+                                                  Geom g, JastrowPair j0, JastrowPair j1, long long N)
 {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= N) return;
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k_u2_table(cpx<T>* __restrict__ tab, cons
         const double c = kq[d] + g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
         x2 += c * c;
     }
-    tab[(long long)blockIdx.y * N + e] = cpx<T>{(T)jastrow_uk(j0, x2), (T)jastrow_uk(j1, x2)};
+    tab[(long long)blockIdx.y * N + e] = cpx<T>{(T)jastrow_uk<kPoly>(j0, x2), (T)jastrow_uk<kPoly>(j1, x2)};
 }
 
 // ----------------------------------------------------------------- tc3a3b3b4b6
@@ -379,15 +379,17 @@ void check_plane(const Geom& g)
 }  // namespace
 
 void launch_u2_table(int prec, void* tab, const double* kqcart, int row0, int nrows, const Geom& g,
-                     const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s)
+                     const JastrowPair& j0, const JastrowPair& j1, hipStream_t s)
 {
     if (nrows == 0) return;
     const long long N = (long long)g.n1 * g.n2 * g.n3;
     const dim3 grid((unsigned)((N + 255) / 256), nrows);
     if (prec == TCX_PREC_FP64)
-        hipLaunchKernelGGL(k_u2_table<double>, grid, dim3(256), 0, s, (cpx<double>*)tab, kqcart, row0, g, j0, j1, N);
+        TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_u2_table<double, kPoly>), grid, dim3(256), 0, s,
+                                                          (cpx<double>*)tab, kqcart, row0, g, j0, j1, N));
     else
-        hipLaunchKernelGGL(k_u2_table<float>, grid, dim3(256), 0, s, (cpx<float>*)tab, kqcart, row0, g, j0, j1, N);
+        TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_u2_table<float, kPoly>), grid, dim3(256), 0, s,
+                                                          (cpx<float>*)tab, kqcart, row0, g, j0, j1, N));
     TCX_HIP_CHECK(hipGetLastError());
 }
 

@@ -144,8 +144,9 @@ void launch_divcorr_kv(const double* phi, const double* occ_right, const double*
 
 // Off-mesh [3b6] table (tc3a3b3b4b6.cpp:445-448):
 //   tG(G) = kVaux . G sum_js n_js(G) A_js u(G, 0, js) (G != 0), kVaux = -sum_of_kVaux, tG(0) = 0
+template <bool kPoly>
 __global__ __launch_bounds__(256) void k_b6_tG_kv(const double2* __restrict__ nG, double2* __restrict__ tG, Geom g,
-                                                  long long N, JastrowRPA j0, JastrowRPA j1, double s0, double s1,
+                                                  long long N, JastrowPair j0, JastrowPair j1, double s0, double s1,
                                                   double s2)
 {
     const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -157,15 +158,16 @@ __global__ __launch_bounds__(256) void k_b6_tG_kv(const double2* __restrict__ nG
         return;
     }
     const double kg = -(s0 * x[0] + s1 * x[1] + s2 * x[2]);
-    const double c0 = kg * j0.A * jastrow_uk(j0, x2), c1 = kg * j1.A * jastrow_uk(j1, x2);
+    const double c0 = kg * j0.A * jastrow_uk<kPoly>(j0, x2), c1 = kg * j1.A * jastrow_uk<kPoly>(j1, x2);
     tG[r] = dadd(dscale(nG[r], c0), dscale(nG[N + r], c1));
 }
 
 void launch_b6_tG_kv(const double* nG, double* tG, const double skv[3], const Geom& g, long long N,
-                     const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s)
+                     const JastrowPair& j0, const JastrowPair& j1, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_b6_tG_kv, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, (const double2*)nG,
-                       (double2*)tG, g, N, j0, j1, skv[0], skv[1], skv[2]);
+    TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_b6_tG_kv<kPoly>), dim3((unsigned)((N + 255) / 256)), dim3(256),
+                                                      0, s, (const double2*)nG, (double2*)tG, g, N, j0, j1, skv[0],
+                                                      skv[1], skv[2]));
     TCX_HIP_CHECK(hipGetLastError());
 }
 
@@ -307,8 +309,9 @@ void launch_zw_b4(const DivEntry* jobs, int njobs, int max_states, const double*
 //   du(G) / N = (kv . (k - q1 + G)) u(k - q1 + G, s, s) / N
 // in FP64, stored like the Coulomb table of pass B (y fastest within a plane), so the
 // hf2x pair passes carry the block's three transforms.
+template <bool kPoly>
 __global__ __launch_bounds__(256) void k_duk_table(double* __restrict__ table, const double* __restrict__ kqcart,
-                                                   int row0, Geom g, long long N, double invN, JastrowRPA j, double k0,
+                                                   int row0, Geom g, long long N, double invN, JastrowPair j, double k0,
                                                    double k1, double k2)
 {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -322,16 +325,17 @@ __global__ __launch_bounds__(256) void k_duk_table(double* __restrict__ table, c
         x[d] = kq[d] + g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
         x2 += x[d] * x[d];
     }
-    table[(long long)blockIdx.y * N + e] = invN * (k0 * x[0] + k1 * x[1] + k2 * x[2]) * jastrow_uk(j, x2);
+    table[(long long)blockIdx.y * N + e] = invN * (k0 * x[0] + k1 * x[1] + k2 * x[2]) * jastrow_uk<kPoly>(j, x2);
 }
 
-void launch_duk_table(double* table, const double* kqcart, int row0, int nrows, const Geom& g, const JastrowRPA& j,
+void launch_duk_table(double* table, const double* kqcart, int row0, int nrows, const Geom& g, const JastrowPair& j,
                       const double kv[3], hipStream_t s)
 {
     if (nrows == 0) return;
     const long long N = (long long)g.n1 * g.n2 * g.n3;
-    hipLaunchKernelGGL(k_duk_table, dim3((unsigned)((N + 255) / 256), nrows), dim3(256), 0, s, table, kqcart, row0, g,
-                       N, 1.0 / (double)N, j, kv[0], kv[1], kv[2]);
+    TCX_POLY_DISPATCH(j.pdeg > 0, hipLaunchKernelGGL((k_duk_table<kPoly>), dim3((unsigned)((N + 255) / 256), nrows),
+                                                     dim3(256), 0, s, table, kqcart, row0, g, N, 1.0 / (double)N, j,
+                                                     kv[0], kv[1], kv[2]));
     TCX_HIP_CHECK(hipGetLastError());
 }
 

@@ -15,7 +15,7 @@
 // diagonalization_davidson.cpp:234-251); the density is rebuilt from the
 // swapped orbitals, which makes it the exact adjoint.  u and tc_2body enter
 // H_2 linearly, so dE_2/dA at fixed F is E_2 evaluated with the d/dA kernels
-// (JastrowRPA::dA); the divergence correction does not depend on A.
+// (JastrowPair::dA); the divergence correction does not depend on A.
 // The backward pass recomputes everything from the orbitals (no pair densities
 // are stored between the forward and the backward calls).
 #include <algorithm>
@@ -116,19 +116,19 @@ void Context::grad_tc2(double* grad_left, double* grad_right, double* energy, do
 
     // H_2^dagger chi: u -> -u, right <-> left (TC: the orbitals are their own left set)
     // (every change of the kernels or orbitals drops the cached local potentials)
-    for (double& a : A) a = -a;
+    negate_u();
     invalidate_local();
     if (bitc) swap_left_right();  // TC: phi = chi, nothing to swap
     std::vector<cd> Hdchi;
     try {
         Hdchi = H2(chi);
     } catch (...) {
-        for (double& a : A) a = -a;
+        negate_u();
         invalidate_local();
         if (bitc) swap_left_right();
         throw;
     }
-    for (double& a : A) a = -a;
+    negate_u();
     invalidate_local();
     if (bitc) swap_left_right();
 
@@ -212,7 +212,7 @@ void Context::kernel_grad(int term, int nbins, double width, double* dE_dK, doub
         double c = tc2 ? -1.0 / NkV : -4.0 * kPiHost / NkV;  // two_body_factor of tc2x / hf2x
         if (nspin == 1) c /= 2.0;
         const double w0 = c / (2.0 * num_kpoints);
-        const JastrowRPA j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
+        const JastrowPair j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
         for (const Launch& L : S.launches) {
             const PairEntry* pairs = (const PairEntry*)S.d_pairs.p + L.pair0;
             launch_products_fft(prec, pairs, L.npairs, (const ProdChan*)d_chans.p, C, d_rec.p, geo, 1.0 / (double)N,

@@ -140,6 +140,7 @@ void Context::set_cell(const double recip_rows[9], double vol, const int grid[3]
     nspin = ns;
     method = meth;
     cell_set = true;
+    if (poly_deg > 0) poly_dirty = true;  // the J3 mesh follows the grid
     sym_set = false;
     basis_set = false;
     states_set = false;
@@ -215,7 +216,7 @@ void Context::hermitian_conj()
 {
     // the density of the unswapped orbitals stays (built first if it was not)
     if (!conj_mode) ensure_density();
-    for (double& a : A) a = -a;
+    negate_u();
     std::swap(d_phi_occ, d_chi_occ);
     for (int p = 0; p < 2; ++p) {
         std::swap(d_occR[p], d_chiR[p]);
@@ -317,6 +318,7 @@ void Context::set_jastrow(const double A_[4], const double F_[4])
     }
     jastrow_zero = zero;
     jastrow_set = true;
+    if (poly_deg > 0) poly_dirty = true;  // the RPA+polynomial parameters follow A and F
 }
 
 void Context::set_div(int enabled, double alpha_, const double* svaux)
@@ -879,6 +881,25 @@ int tcx_set_states(tcx_context* ctx, const int* nocc, const double* filling, con
 int tcx_set_jastrow_rpa(tcx_context* ctx, const double A[4], const double F[4])
 {
     return guarded(ctx, [&] { ctx->impl.set_jastrow(A, F); });
+}
+
+int tcx_set_jastrow_polynomial(tcx_context* ctx, int deg, int C, const double L[4], const double* c_input, int cusp)
+{
+    return guarded(ctx, [&] { ctx->impl.set_jastrow_poly(deg, C, L, c_input, cusp); });
+}
+
+int tcx_get_jastrow_num_rmesh(tcx_context* ctx, int* num_rmesh)
+{
+    return guarded(ctx, [&] {
+        tcx::Context::require(num_rmesh != nullptr, TCX_ERR_ARG, "get_jastrow_num_rmesh: null pointer");
+        if (ctx->impl.poly_dirty) ctx->impl.build_jastrow_poly();
+        *num_rmesh = ctx->impl.poly_nr;
+    });
+}
+
+int tcx_jastrow_kernels(tcx_context* ctx, int s1, int s2, long long n, const double* G2, double* uk, double* tc2body)
+{
+    return guarded(ctx, [&] { ctx->impl.jastrow_kernels(s1, s2, n, G2, uk, tc2body); });
 }
 
 int tcx_set_div_correction(tcx_context* ctx, int enabled, double alpha, const double* sum_of_vaux)

@@ -137,7 +137,7 @@ void launch_divcorr(const double* phi, const double* occ_right, const double* oc
 
 // tc2x pair pipeline (4-channel pair records)
 void launch_tc_table(int prec, void* table, const double* kqcart, int row0, int nrows, const Geom& g,
-                     const JastrowRPA& jp, hipStream_t s);
+                     const JastrowPair& jp, hipStream_t s);
 void launch_tc2x_A(int prec, const Geom& g, const void* appR, const void* appGrad, const void* chiR,
                    const PairEntry* pairs, int npairs, void* buf, hipStream_t s);
 void launch_tc2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const void* table, int kq0,
@@ -149,9 +149,10 @@ void launch_tc2x_C(int prec, const Geom& g, const void* buf, const void* occR, c
 // density and local three-body potentials (FP64 grids)
 void launch_density(const double* occR, const double* chiR, const double* fill, int nocc, long long N, double* out,
                     hipStream_t s);
-void launch_dnu(const double* nG, double* out, const Geom& g, const JastrowRPA& j0, const JastrowRPA& j1,
+void launch_jastrow_eval(const double* G2, double* uk, double* v, long long n, const JastrowPair& j, hipStream_t s);
+void launch_dnu(const double* nG, double* out, const Geom& g, const JastrowPair& j0, const JastrowPair& j1,
                 hipStream_t s);
-void launch_tc3b1_V(const double* dnu, double* V, const Geom& g, const JastrowRPA& j0, const JastrowRPA& j1,
+void launch_tc3b1_V(const double* dnu, double* V, const Geom& g, const JastrowPair& j0, const JastrowPair& j1,
                     hipStream_t s);
 void launch_sum_squares(const double* dnu, double* V, long long N, hipStream_t s);
 void launch_mul_shared(double* grids, int ngrids, const double* V, long long N, hipStream_t s);
@@ -166,7 +167,7 @@ void launch_products_z(int prec, const PairEntry* pairs, int npairs, const ProdC
 // xy-plane passes with the three-body combines fused in (kernels_tc3xy.hip); tab holds
 // (u(kq+G, j0), u(kq+G, j1)) per k-q row of the launch (launch_u2_table)
 void launch_u2_table(int prec, void* tab, const double* kqcart, int row0, int nrows, const Geom& g,
-                     const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
+                     const JastrowPair& j0, const JastrowPair& j1, hipStream_t s);
 void launch_xy_a7c1(int prec, const Geom& g, const void* rec, void* P, void* Y, const PairEntry* pairs, int npairs,
                     const void* tab, int kq0, const double* kqcart, const double* fill, double fb3, hipStream_t s);
 void launch_xy_a7c2(int prec, const Geom& g, const void* P, void* Y, const PairEntry* pairs, int npairs,
@@ -192,14 +193,14 @@ void launch_mul_dnu(int prec, const void* appR, const double* dnu, void* out, in
 void launch_a3(double* W, const double* qsum, const double* dnu, int napp, long long N, double fa3, hipStream_t s);
 void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, const int* match, int nk, const Geom& g,
                long long N,
-               const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
+               const JastrowPair& j0, const JastrowPair& j1, hipStream_t s);
 void launch_b6_coef(const double* phi, const double* occ_r, const double* occ_l, const DivEntry* jobs, int njobs,
                     int maxocc, const double* fill, const double* t1, const double* appR, const double* chiR,
                     const double* occR, const double* tR, double fb6, double sgn, long long N, double* out,
                     double* cR, double* W, hipStream_t s);
 void launch_a8_div(double* Tg, const double* rec, const PairEntry* pairs, const TargetEntry* targets, int ntargets,
-                   const double* fill, const double* dc2, const Geom& g, long long N, const JastrowRPA& j0,
-                   const JastrowRPA& j1, hipStream_t s);
+                   const double* fill, const double* dc2, const Geom& g, long long N, const JastrowPair& j0,
+                   const JastrowPair& j1, hipStream_t s);
 void launch_a8_reduce(const double* Tg, const double* Q4, const double* Q5, int bitc, const double* occR,
                       const double* fill, int nocc, double* V3a2, double* B2, long long N, hipStream_t s);
 void launch_a8_q45(int prec, const double* Q4, const double* Q5, int bitc, void* Q4T, void* Q5T, int ngrids,
@@ -210,7 +211,7 @@ void launch_sym_rotate(double* Q, int ncomp, const SymRotJob* jobs, int njobs, c
 
 // kernel-gradient reduction of the occupied-set energies (rec: 1 channel E_x, 8 channels E_2x)
 void launch_kgrad(int prec, const void* rec, int tc2, const PairEntry* pairs, int npairs, const double* kqcart,
-                  const double* fill, const Geom& g, long long N, const JastrowRPA& j0, const JastrowRPA& j1, double w0,
+                  const double* fill, const Geom& g, long long N, const JastrowPair& j0, const JastrowPair& j1, double w0,
                   int nbins, double width, double* bins_v, double* bins_u, double* sums, hipStream_t s);
 
 // assembly terms (FP64)
@@ -220,7 +221,7 @@ void launch_hartree(const double* nG, double* V, const Geom& g, long long N, hip
 void launch_tc2h_V(const double* chiR, const double* gradR, const double* fill, int nocc, double* pgp,
                    const Geom& g, long long N, hipStream_t s);
 void launch_tc2h_combine(const double* nG, const double* pgpG, double* V, const Geom& g, long long N,
-                         const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
+                         const JastrowPair& j0, const JastrowPair& j1, hipStream_t s);
 void launch_tc2h_apply(double* orb, const double* grad, const double* V2, const double* dnu, int nb, long long N,
                        hipStream_t s);
 void launch_axpby(const double* X, const double* Y, double* out, double a, double b, long long N, hipStream_t s);
@@ -232,7 +233,7 @@ void launch_divcorr_kv(const double* phi, const double* occ_right, const double*
                        const int* gindex, const int* gidx_off, const double* kcart, const Geom& g, double* out,
                        hipStream_t s);
 void launch_b6_tG_kv(const double* nG, double* tG, const double skv[3], const Geom& g, long long N,
-                     const JastrowRPA& j0, const JastrowRPA& j1, hipStream_t s);
+                     const JastrowPair& j0, const JastrowPair& j1, hipStream_t s);
 void launch_zw_b3(const double* phi, const double* occ_l, const DivEntry* jobs, int njobs, int max_states,
                   const double* fill, const double* chiR, const double* occR, const double* occ_r, const double* dnu,
                   const double* appR, long long N, double fb3, const double kv[3], double* cA, double* cQ,
@@ -240,7 +241,7 @@ void launch_zw_b3(const double* phi, const double* occ_l, const DivEntry* jobs, 
 void launch_zw_b4(const DivEntry* jobs, int njobs, int max_states, const double* fill, const double* chiR,
                   const double* occ_r, const double* qsum, long long N, double fb4, const double kv[3], double* cA,
                   double* out, hipStream_t s);
-void launch_duk_table(double* table, const double* kqcart, int row0, int nrows, const Geom& g, const JastrowRPA& j,
+void launch_duk_table(double* table, const double* kqcart, int row0, int nrows, const Geom& g, const JastrowPair& j,
                       const double kv[3], hipStream_t s);
 void launch_zw_a4b5_apply(const double* phi, const double* occ_r, const double* occ_l, const DivEntry* jobs, int njobs,
                           int max_states, const double* fill, const double* Wa, const double* Wb, const int* gi,

@@ -34,7 +34,7 @@ void Context::tc2x(const int* nb_, const double* phi, double* out, int prec, uns
         int max_rows = 1;
         for (const Launch& L : P.launches) max_rows = std::max(max_rows, L.nkq);
         d_ktab.ensure((size_t)max_rows * N * es);
-        const JastrowRPA jp = jastrow_pair(0, 0);  // tc2x.cpp:269-271 (ispin, ispin)
+        const JastrowPair jp = jastrow_pair(0, 0);  // tc2x.cpp:269-271 (ispin, ispin)
         for (const Launch& L : P.launches) {
             const PairEntry* pairs = (const PairEntry*)P.d_pairs.p + L.pair0;
             const TargetEntry* targets = (const TargetEntry*)P.d_targets.p + L.target0;
@@ -107,7 +107,7 @@ void Context::tc3_local(int which, const int* nb_, const double* phi, double* ou
             double* Vc = (double*)C.buf.p;                     // the local potential, real space
             TCX_HIP_CHECK(hipMemcpyAsync(nG, d_density.p, 2 * g, hipMemcpyDeviceToDevice, stream));
             launch_fft3d(TCX_PREC_FP64, -1, nG, 2, geo, 1.0 / (double)N, stream);
-            const JastrowRPA j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
+            const JastrowPair j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
             launch_dnu(nG, dnu, geo, j0, j1, stream);
             launch_fft3d(TCX_PREC_FP64, +1, dnu, 3, geo, 1.0, stream);
             if (which == 0) {

@@ -98,7 +98,7 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
     const Plan& P = get_plan(nb, 0);
     ApplyIO io = begin_apply(nb_, phi, out, flags, P);
     // u = 0 makes every contribution vanish, and the G=0 correction is skipped with it (:99)
-    if (P.n_app > 0 && n_occ > 0 && !jastrow_zero) {
+    if (P.n_app > 0 && n_occ > 0 && !u_zero()) {
         const bool bitc = method == TCX_METHOD_BITC;
         const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
         const size_t g64 = (size_t)N * 16, gT = (size_t)N * es;
@@ -111,7 +111,7 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
             fb4 /= 4.0;
             fb6 /= 2.0;
         }
-        const JastrowRPA j00 = jastrow_pair(0, 0), j10 = jastrow_pair(1, 0), j01 = jastrow_pair(0, 1);
+        const JastrowPair j00 = jastrow_pair(0, 0), j10 = jastrow_pair(1, 0), j01 = jastrow_pair(0, 1);
         ensure_occR(prec);
         if (bitc) ensure_chiR(prec);
         const void* occR = d_occR[prec].p;
@@ -311,7 +311,7 @@ void Context::a8_occupied_phase(const std::vector<int>& nb, int prec)
         fa2 /= 4.0;
         fb2 /= 4.0;
     }
-    const JastrowRPA j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
+    const JastrowPair j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
     const void* occR = d_occR[prec].p;
     const void* chiR = bitc ? d_chiR[prec].p : occR;
     ensure_occR(TCX_PREC_FP64);
@@ -416,7 +416,7 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
     std::vector<int> nb(nb_, nb_ + nk);
     const Plan& P = get_plan(nb, 0);
     ApplyIO io = begin_apply(nb_, phi, out, flags, P);
-    if (P.n_app > 0 && n_occ > 0 && !jastrow_zero) {
+    if (P.n_app > 0 && n_occ > 0 && !u_zero()) {
         const bool bitc = method == TCX_METHOD_BITC;
         const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
         const size_t g64 = (size_t)N * 16, gT = (size_t)N * es;
@@ -427,7 +427,7 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
             fa4 /= 4.0;
             fb5 /= 4.0;
         }
-        const JastrowRPA j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
+        const JastrowPair j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
         ensure_occR(prec);
         if (bitc) ensure_chiR(prec);
         const void* occR = d_occR[prec].p;
@@ -577,7 +577,7 @@ void Context::tc2h(const int* nb_, const double* phi, double* out, unsigned flag
     if (P.n_app > 0 && n_occ > 0) {
         const bool bitc = method == TCX_METHOD_BITC;
         const size_t g = (size_t)N * 16;
-        const JastrowRPA j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
+        const JastrowPair j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
         ensure_occR(TCX_PREC_FP64);
         ensure_gradR(TCX_PREC_FP64);
         if (bitc) ensure_chiR(TCX_PREC_FP64);

@@ -260,10 +260,22 @@ struct Context {
     void ensure_chiR(int prec);
     void ensure_gradR(int prec);
     bool jastrow_dA = false;  // evaluate d/dA of the Jastrow kernels (adjoint of the A-dependence)
-    JastrowRPA jastrow_pair(int s1, int s2) const
-    {
-        return JastrowRPA{A[s1 * 2 + s2], F[s1 * 2 + s2], jastrow_zero ? 1 : 0, jastrow_dA ? 1 : 0};
-    }
+    JastrowPair jastrow_pair(int s1, int s2);  // builds the polynomial blocks if they are stale
+    // polynomial Jastrow (tcx_set_jastrow_polynomial): deg 0 = none
+    int poly_deg = 0, poly_C = 3;
+    bool poly_cusp = false;
+    double poly_L[4] = {1, 1, 1, 1};
+    std::vector<double> poly_c;  // [4][deg] as given (entry 0 of each pair replaced at build)
+    bool poly_dirty = false;     // derived parameters to rebuild (RPA F or the grid changed)
+    int poly_nr = 0;             // num_rmesh_J3_integ
+    size_t poly_stride = 0;      // doubles per pair block
+    DBuf d_jpoly[2];             // [4] pair blocks: u as given, u negated
+    bool u_negated = false;      // A and the polynomial negated (Hermitian conjugate)
+    void set_jastrow_poly(int deg, int C, const double* L, const double* c_in, int cusp);
+    void build_jastrow_poly();
+    void negate_u();
+    bool u_zero() const { return jastrow_zero && poly_deg == 0; }
+    void jastrow_kernels(int s1, int s2, long long n, const double* G2, double* uk, double* v);
     void ensure_density();
 
     void resolve_shape(int term, long& tc, long& sc, long ntargets, long nsources) const;

@@ -192,7 +192,7 @@ void Context::a8_kvaux(const Plan& P, const ApplyIO& io, const std::vector<int>&
     const double* chiR64 = bitc ? (const double*)d_chiR[TCX_PREC_FP64].p : occR64;
     const double* div_r = (const double*)d_phi_occ.p;
     const double* div_l = (const double*)(bitc ? d_chi_occ.p : d_phi_occ.p);
-    const JastrowRPA j00 = jastrow_pair(0, 0);  // u(k - q1 + G, s, s) (:1109)
+    const JastrowPair j00 = jastrow_pair(0, 0);  // u(k - q1 + G, s, s) (:1109)
     int nq = 0;
     for (int q = 0; q < nk; ++q)
         if (!zw_k[q] && nocc[q] > 0) ++nq;

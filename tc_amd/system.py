@@ -142,7 +142,8 @@ class System:
     zero_weight_states: dict | None = None
     # polynomial Jastrow (tcx_set_jastrow_polynomial) or None: {"deg": int, "C": 2 or 3,
     # "L": (2, 2), "c": (2, 2, deg) c_poly_input with entry 0 replaced by the cusp
-    # condition, "cusp": bool}; with A != 0 it is the RPA+polynomial form
+    # condition, "cusp": bool}; with A != 0 it is the RPA+polynomial form.  The oracle also
+    # reads "hermitian_conj": True (c_poly_internal negated, as for H^dagger)
     jastrow_poly: dict | None = None
 
     @property

@@ -36,7 +36,8 @@ METHODS = {"HF": 0, "TC": 1, "BITC": 2}
 EXPORTS = [
     "tcx_create", "tcx_destroy", "tcx_last_error", "tcx_synchronize",
     "tcx_set_cell", "tcx_set_kpoints", "tcx_set_basis", "tcx_set_states",
-    "tcx_set_jastrow_rpa", "tcx_set_div_correction", "tcx_sum_of_vaux", "tcx_set_kpoint_symmetry",
+    "tcx_set_jastrow_rpa", "tcx_set_jastrow_polynomial", "tcx_get_jastrow_num_rmesh", "tcx_jastrow_kernels",
+    "tcx_set_div_correction", "tcx_sum_of_vaux", "tcx_set_kpoint_symmetry",
     "tcx_sum_of_vaux_weighted", "tcx_set_zero_weight_kpoints",
     "tcx_get_density", "tcx_set_density",
     "tcx_hf2x", "tcx_hf2x_noalloc", "tcx_tc2x", "tcx_tc3a1", "tcx_tc3b1", "tcx_tc3a3b3b4b6", "tcx_tc3a2a4b2b5",
@@ -80,6 +81,9 @@ def load_library():
     L.tcx_set_basis.argtypes = [vp, _ip, _ip]
     L.tcx_set_states.argtypes = [vp, _ip, _dp, _dp, _dp]
     L.tcx_set_jastrow_rpa.argtypes = [vp, _dp, _dp]
+    L.tcx_set_jastrow_polynomial.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, C.c_int]
+    L.tcx_get_jastrow_num_rmesh.argtypes = [vp, _ip]
+    L.tcx_jastrow_kernels.argtypes = [vp, C.c_int, C.c_int, C.c_longlong, vp, vp, vp]
     L.tcx_set_div_correction.argtypes = [vp, C.c_int, C.c_double, _dp]
     L.tcx_sum_of_vaux.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp]
     L.tcx_sum_of_vaux_weighted.argtypes = [_dp, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp]
@@ -252,6 +256,32 @@ class Engine:
         f = np.ascontiguousarray(F, dtype=np.float64).reshape(4)
         self._check(self.L.tcx_set_jastrow_rpa(self.h, _ptr(a), _ptr(f)), "set_jastrow_rpa")
 
+    def set_jastrow_polynomial(self, deg, C=3, L=None, c=None, cusp=False):
+        """Polynomial part of the Jastrow (tcx_set_jastrow_polynomial); deg 0 removes it.
+        L: (2, 2), c: (2, 2, deg) c_poly_input (entry 0 of each pair is replaced by the
+        cusp condition)."""
+        deg = int(deg)
+        Lv = np.ascontiguousarray(np.ones(4) if L is None else L, dtype=np.float64).reshape(4)
+        cv = np.ascontiguousarray(np.zeros(4 * max(deg, 1)) if c is None else c, dtype=np.float64).reshape(-1)
+        self._check(self.L.tcx_set_jastrow_polynomial(self.h, deg, int(C), _ptr(Lv), _ptr(cv) if deg else None,
+                                                      int(bool(cusp))), "set_jastrow_polynomial")
+
+    def jastrow_num_rmesh(self):
+        n = C.c_int(-1)
+        self._check(self.L.tcx_get_jastrow_num_rmesh(self.h, C.byref(n)), "get_jastrow_num_rmesh")
+        return n.value
+
+    def jastrow_kernels(self, G2, s1=0, s2=0):
+        """(u(G), tc_2body(G)) at the |G|^2 values G2, evaluated on the GPU."""
+        g2 = np.ascontiguousarray(G2, dtype=np.float64).reshape(-1)
+        n = g2.size
+        g = self.to_device(g2)
+        u, v = self.device_buffer(max(8 * n, 8)), self.device_buffer(max(8 * n, 8))
+        self._check(self.L.tcx_jastrow_kernels(self.h, int(s1), int(s2), n, C.c_void_p(g.ptr), C.c_void_p(u.ptr),
+                                               C.c_void_p(v.ptr)), "jastrow_kernels")
+        self.synchronize()
+        return u.download(np.float64, n), v.download(np.float64, n)
+
     def set_div_correction(self, enabled, alpha=0.0, sum_vaux=None):
         s = None if sum_vaux is None else np.ascontiguousarray(sum_vaux, dtype=np.float64)
         self._check(self.L.tcx_set_div_correction(self.h, int(bool(enabled)), float(alpha),
@@ -309,6 +339,11 @@ class Engine:
         if sym is not None:
             self.set_kpoint_symmetry(sym["k_src"], sym["rot"], sym["trans"], sym["trev"])
         self.set_jastrow_rpa(sys.A, sys.F)
+        poly = getattr(sys, "jastrow_poly", None)
+        if poly is not None and int(poly["deg"]) > 0:
+            self.set_jastrow_polynomial(poly["deg"], poly["C"], poly["L"], poly["c"], poly["cusp"])
+        else:
+            self.set_jastrow_polynomial(0)
         on = sys.div_corr if div_corr is None else div_corr
         kweight = getattr(sys, "kweight", None)
         if kweight is None:

@@ -4,7 +4,8 @@ and the torch.autograd.Function wrapper of the test harness (CPU tensors; the
 GPU work goes through libtcx).
 
 E_x and E_2 are quartic in the orbitals (central-difference error O(h^2)) and
-E_2 is quadratic in A at fixed F (central differences exact up to rounding)."""
+E_2 is quadratic in A at fixed F (central differences exact up to rounding), also with a
+polynomial part on top of the RPA one (its cross terms with the RPA part are linear in A)."""
 import numpy as np
 import pytest
 
@@ -24,6 +25,12 @@ def _with(sys_, **kw):
     return S.System(**{**sys_.__dict__, **kw})
 
 
+def _poly(sys_):
+    """The RPA Jastrow of sys_ plus example 32's 6th-order polynomial (RPA+polynomial form)."""
+    from test_oracle_jastrow_poly import example_spec
+    return _with(sys_, jastrow_poly=example_spec(32)[0])
+
+
 def test_exchange_gradient_finite_differences(engine):
     sys_ = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=91)
     engine.load_system(sys_)
@@ -40,9 +47,12 @@ def test_exchange_gradient_finite_differences(engine):
     engine.load_system(sys_)
 
 
+@pytest.mark.parametrize("poly", [False, True], ids=["rpa", "rpa_poly"])
 @pytest.mark.parametrize("method", ["TC", "BITC"])
-def test_tc2_gradients_finite_differences(engine, method):
+def test_tc2_gradients_finite_differences(engine, method, poly):
     sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=92)
+    if poly:
+        sys_ = _poly(sys_)
     engine.load_system(sys_)
     e0, gl, gr, dA = engine.grad_tc2()
     chi = sys_.chi_occ if method == "BITC" else sys_.phi_occ
@@ -149,10 +159,13 @@ def test_exchange_kernel_gradient_bins_and_euler(engine):
     assert np.abs(gk.imag).max() == 0.0
 
 
+@pytest.mark.parametrize("poly", [False, True], ids=["rpa", "rpa_poly"])
 @pytest.mark.parametrize("method", ["TC", "BITC"])
 @pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
-def test_tc2_kernel_gradient_euler_and_dA(engine, method, prec):
+def test_tc2_kernel_gradient_euler_and_dA(engine, method, prec, poly):
     sys_ = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=95)
+    if poly:
+        sys_ = _poly(sys_)
     chi = sys_.chi_occ if method == "BITC" else sys_.phi_occ
 
     def e2x(s):

@@ -199,7 +199,11 @@ def hermitian_conj_hamiltonians(sys_):
     (Jastrow::take_Hermitian_conj and BlochStates::switch_left_right_orbitals,
     diagonalization_davidson.cpp:234-251): u -> -u, phi <-> chi, the density unchanged."""
     dens = O.Oracle(sys_).density()
-    sw = S.System(**{**sys_.__dict__, "A": -np.asarray(sys_.A), "phi_occ": sys_.chi_occ, "chi_occ": sys_.phi_occ})
+    poly = sys_.jastrow_poly
+    if poly is not None:
+        poly = {**poly, "hermitian_conj": not poly.get("hermitian_conj", False)}
+    sw = S.System(**{**sys_.__dict__, "A": -np.asarray(sys_.A), "phi_occ": sys_.chi_occ, "chi_occ": sys_.phi_occ,
+                     "jastrow_poly": poly})
     return full_hamiltonians(sw, density=dens)
 
 
