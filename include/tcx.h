@@ -363,6 +363,19 @@ int tcx_memcpy(tcx_context *ctx, void *dst, const void *src, unsigned long long 
 int tcx_memset_zero(tcx_context *ctx, void *ptr, unsigned long long bytes);
 
 /* ---------------------------------------------------------------- tuning / timing */
+/* Pass order of the exchange pair pipeline (term 0: hf2x): 0 z-first (transform along z,
+ * then the xy planes, then z back), 1 plane-first (xy planes, then z pencils, then xy back;
+ * FP32-split only: the pair records then hold only the (kx, ky) columns inside the pair
+ * density's G-space support, about two thirds of them at 40^3), -1 automatic (z-first;
+ * the default: plane-first moves fewer bytes but ran slower at 40^3).  No counterpart in
+ * the reference, whose hf2x runs full 3D FFTs (src/calc_hamiltonian_hf2x.cpp:123-135);
+ * the two orders agree to rounding. */
+int tcx_set_pipeline_order(tcx_context *ctx, int term, int order);
+
+/* Mean number of (kx, ky) columns per plane of the pair records of the last apply of
+ * term 0 (hf2x): n1*n2 for the z-first order, the support columns for plane-first. */
+int tcx_get_record_columns(tcx_context *ctx, int term, double *columns);
+
 /* Launch shape of a pair pipeline (term 0: hf2x, 1: tc2x): each launch pairs
  * `targets` consecutive applied bands (k,j) with `sources` consecutive
  * occupied orbitals (q,i); targets*sources pair records are in flight (the

@@ -27,6 +27,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     "csrc/kernels_fft.hip",
     "csrc/kernels_hf2x.hip",
+    "csrc/kernels_hf2x_xy.hip",
     "csrc/kernels_tc2x.hip",
     "csrc/kernels_local.hip",
     "csrc/kernels_tc3x.hip",
@@ -42,6 +43,7 @@ SOURCES = [
     "csrc/tcx_adjoint.cpp",
     "csrc/tcx_zeroweight.cpp",
     "csrc/tcx_jastrow.cpp",
+    "csrc/tcx_hf2x_xy.cpp",
     "host/calc_hamiltonian.cpp",
     "host/scf.cpp",
 ]

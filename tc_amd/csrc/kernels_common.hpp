@@ -172,6 +172,31 @@ __device__ __forceinline__ void fft_split2(cpx<T> (&v)[NZ / 2], int h)
     });
 }
 
+// The transpose of fft_split2 (decimation in frequency): lane half h holds the samples
+// z = h*NZ/2 + k on entry (the layout fft_split2 leaves); the cross-lane butterfly comes
+// first, then the twiddle of the upper half and the length-NZ/2 transform in registers.
+// On exit lane half h holds outputs z = 2m + h.
+template <typename T, int NZ, int SIGN, bool kLaneSwap = false>
+__device__ __forceinline__ void fft_split2_dif(cpx<T> (&v)[NZ / 2], int h)
+{
+    constexpr int M = NZ / 2;
+    static_for<0, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (kLaneSwap) {
+            v[k] = xbfly<32>(v[k], h != 0);  // h = lane >> 5: lower X[k] + X[k+M], upper X[k] - X[k+M]
+        } else {
+            const cpx<T> o{xshfl(v[k].re, 32), xshfl(v[k].im, 32)};
+            v[k] = csel(h != 0, o - v[k], v[k] + o);
+        }
+    });
+    static_for<1, M>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const cpx<T> t = v[k] * twiddle<T, NZ, SIGN, k>();
+        v[k] = csel(h != 0, t, v[k]);
+    });
+    fft<T, M, SIGN>(v);
+}
+
 // Pencil split over four lanes (lane ^ 16, ^ 32, ^ 48): lane quarter h holds
 // z = 4m + h on entry; the radix-4 cross-lane DFT runs as two radix-2 stages
 // (shuffle with lane ^ 32, twiddle, shuffle with lane ^ 16).  On exit lane

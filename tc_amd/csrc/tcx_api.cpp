@@ -574,8 +574,11 @@ void build_pairs(PairSet& S, const std::vector<PairItem>& tg, const std::vector<
             L.kq0 = (int)S.kq.size() / 3;
             L.nkq = (kl - kf + 1) * (ql - qf + 1);
             for (int k = kf; k <= kl; ++k)
-                for (int q = qf; q <= ql; ++q)
+                for (int q = qf; q <= ql; ++q) {
                     for (int d = 0; d < 3; ++d) S.kq.push_back(kcart[3 * k + d] - kcart[3 * q + d]);
+                    S.kq_kq.push_back(k);
+                    S.kq_kq.push_back(q);
+                }
             int local = 0;
             for (size_t t = t0; t < t1; ++t) {
                 TargetEntry te{tg[t].idx, local, 0, tg[t].k};
@@ -593,6 +596,7 @@ void build_pairs(PairSet& S, const std::vector<PairItem>& tg, const std::vector<
             if (local == 0) {
                 S.targets.resize(L.target0);
                 S.kq.resize(3 * (size_t)L.kq0);
+                S.kq_kq.resize(2 * (size_t)L.kq0);
                 continue;
             }
             S.launches.push_back(L);
@@ -744,6 +748,16 @@ void Context::hf2x(const int* nb_, const double* phi, double* out, int prec, uns
         timed(3, 0, [&] { orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p); });
         d_W.ensure((size_t)P.n_app * N * 16);
         launch_zero(d_W.p, (size_t)P.n_app * N * 16, stream);
+        if (use_plane_first(prec)) {
+            hf2x_plane_first(P);
+            double factor = -4.0 * kPiHost / (num_kpoints * volume);  // two_body_factor, hf2x.cpp:29-30
+            if (nspin == 1) factor /= 2.0;
+            finish_W(P, io, factor);
+            apply_divcorr(P, io, (const double*)d_phi_occ.p);
+            end_apply(io);
+            return;
+        }
+        hf2x_record_cols = (double)geo.n1 * geo.n2;
         d_pairbuf.ensure((size_t)std::max(1, P.max_launch_pairs) * N * es);
         int max_rows = 1;
         for (const Launch& L : P.launches) max_rows = std::max(max_rows, L.nkq);
@@ -1212,6 +1226,24 @@ int tcx_set_launch_shape(tcx_context* ctx, int term, long targets, long sources)
             throw Error(TCX_ERR_ARG, "launch_shape: targets and sources must be in [0, 65535]");
         ctx->impl.shape_targets[term] = targets;
         ctx->impl.shape_sources[term] = sources;
+    });
+}
+
+int tcx_set_pipeline_order(tcx_context* ctx, int term, int order)
+{
+    return guarded(ctx, [&] {
+        if (term != 0) throw Error(TCX_ERR_ARG, "pipeline_order: term must be 0 (hf2x)");
+        if (order < -1 || order > 1) throw Error(TCX_ERR_ARG, "pipeline_order: order must be -1, 0 or 1");
+        ctx->impl.hf2x_order = order;
+    });
+}
+
+int tcx_get_record_columns(tcx_context* ctx, int term, double* columns)
+{
+    return guarded(ctx, [&] {
+        if (term != 0) throw Error(TCX_ERR_ARG, "record_columns: term must be 0 (hf2x)");
+        if (!columns) throw Error(TCX_ERR_ARG, "record_columns: null pointer");
+        *columns = ctx->impl.hf2x_record_cols;
     });
 }
 

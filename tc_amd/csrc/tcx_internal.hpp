@@ -122,6 +122,14 @@ void launch_fft_xy(int prec, int sign, void* grids, int ngrids, const Geom& g, h
 void launch_fft_z(int prec, int sign, void* grids, int ngrids, const Geom& g, double scale, hipStream_t s);
 
 // hf2x pair pipeline
+// plane-first hf2x (kernels_hf2x_xy.hip), FP32-split
+void launch_hf2x_xyA(const Geom& g, const void* app, const void* occ, const PairEntry* pairs, int npairs,
+                     const int* colxy, const int* coloff, void* rec, int cstride, hipStream_t s);
+void launch_hf2x_zB(const Geom& g, void* rec, const PairEntry* pairs, int npairs, const int* colxy, const int* coloff,
+                    const double* kqcart, int cstride, hipStream_t s);
+void launch_hf2x_xyC(const Geom& g, const void* rec, const void* occ, const PairEntry* pairs,
+                     const TargetEntry* targets, int ntargets, const int* rowinfo, const double* fill, double* W,
+                     int cstride, hipStream_t s);
 void launch_hf2x_A(int prec, const Geom& g, const void* phiR_app, const void* phiR_occ, const PairEntry* pairs,
                    int npairs, void* buf, hipStream_t s);
 void launch_coulomb_table(int prec, void* table, const double* kqcart, int row0, int nrows, const Geom& g,

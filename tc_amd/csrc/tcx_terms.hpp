@@ -63,6 +63,7 @@ struct PairSet {
     std::vector<PairEntry> pairs;
     std::vector<TargetEntry> targets;
     std::vector<double> kq;  // Cartesian k - q per (k,q) row
+    std::vector<int> kq_kq;  // (k, q) per row
     std::vector<Launch> launches;
     int max_launch_pairs = 0;
     DBuf d_pairs, d_targets, d_kq;
@@ -78,12 +79,22 @@ struct PairItem {
 void build_pairs(PairSet& S, const std::vector<PairItem>& tg, const std::vector<PairItem>& src, long tc, long sc,
                  const std::vector<double>& kcart, bool same_k);
 
+// Support columns of the plane-first hf2x records, per k-q row of a pair set
+// (kernels_hf2x_xy.hip): colxy[coloff[r] + c] = i1 | i2 << 16, rowinfo[r][i2] = (lo, len).
+struct XyColumns {
+    bool built = false;
+    int cstride = 0;      // record columns per plane (the largest row)
+    double mean_cols = 0; // mean columns per pair
+    DBuf d_colxy, d_coloff, d_rowinfo;
+};
+
 struct Plan : PairSet {
     std::vector<long long> app_first, app_coef;  // per k: first applied orbital, coefficient offset
     int n_app = 0;
     DBuf d_appjobs;      // OrbEntry per applied band (band order)
     DBuf d_appgradjobs;  // three OrbEntry (dir 0..2) per applied band
     DBuf d_divjobs;      // DivEntry per applied band
+    mutable XyColumns xy;
 };
 
 // synchronous host -> device copy into a growing buffer
@@ -153,6 +164,11 @@ struct Context {
     // launch shape per term (0 hf2x, 1 tc2x): targets x sources per launch, 0 = auto
     long shape_targets[2] = {0, 0};
     long shape_sources[2] = {0, 0};
+    int hf2x_order = -1;          // -1 automatic, 0 z-first, 1 plane-first (FP32)
+    double hf2x_record_cols = 0;  // (kx, ky) columns per plane of the last hf2x apply's records
+    bool use_plane_first(int prec) const;
+    void build_xy_columns(const Plan& P);
+    void hf2x_plane_first(const Plan& P);
     std::map<std::tuple<std::vector<int>, long, long>, Plan> plans;
     // pair sets of the three-body exchange terms, keyed by (applied bands, kind)
     std::map<std::tuple<std::vector<int>, int>, PairSet> pairsets;

@@ -46,7 +46,8 @@ EXPORTS = [
     "tcx_graph_begin", "tcx_graph_end", "tcx_graph_launch", "tcx_graph_destroy",
     "tcx_grad_exchange", "tcx_grad_tc2", "tcx_kernel_gradient",
     "tcx_device_alloc", "tcx_device_free", "tcx_memcpy", "tcx_memset_zero",
-    "tcx_set_launch_shape", "tcx_enable_timing", "tcx_get_timing",
+    "tcx_set_launch_shape", "tcx_set_pipeline_order", "tcx_get_record_columns", "tcx_enable_timing",
+    "tcx_get_timing",
 ]
 
 
@@ -113,6 +114,8 @@ def load_library():
     L.tcx_memcpy.argtypes = [vp, vp, vp, C.c_ulonglong, C.c_int]
     L.tcx_memset_zero.argtypes = [vp, vp, C.c_ulonglong]
     L.tcx_set_launch_shape.argtypes = [vp, C.c_int, C.c_long, C.c_long]
+    L.tcx_set_pipeline_order.argtypes = [vp, C.c_int, C.c_int]
+    L.tcx_get_record_columns.argtypes = [vp, C.c_int, _dp]
     L.tcx_enable_timing.argtypes = [vp, C.c_int]
     L.tcx_get_timing.argtypes = [vp, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
     _lib = L
@@ -496,6 +499,18 @@ class Engine:
         """term: "hf2x" or "tc2x"; 0 = automatic."""
         t = {"hf2x": 0, "tc2x": 1}[term] if isinstance(term, str) else int(term)
         self._check(self.L.tcx_set_launch_shape(self.h, t, int(targets), int(sources)), "set_launch_shape")
+
+    def set_pipeline_order(self, term, order):
+        """hf2x pass order: 0 z-first, 1 plane-first (FP32-split), -1 automatic."""
+        t = {"hf2x": 0}[term] if isinstance(term, str) else int(term)
+        self._check(self.L.tcx_set_pipeline_order(self.h, t, int(order)), "set_pipeline_order")
+
+    def record_columns(self, term="hf2x"):
+        """Mean (kx, ky) columns per plane of the last hf2x apply's pair records."""
+        t = {"hf2x": 0}[term] if isinstance(term, str) else int(term)
+        out = C.c_double(0.0)
+        self._check(self.L.tcx_get_record_columns(self.h, t, C.byref(out)), "get_record_columns")
+        return out.value
 
     def enable_timing(self, on=True):
         self._check(self.L.tcx_enable_timing(self.h, int(bool(on))), "enable_timing")
