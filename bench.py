@@ -450,6 +450,7 @@ def bench_batched(log, replays):
     apply captured once into a HIP graph, the three graphs launched together on their own
     streams per replay.  Sequential mode: the same work as separate per-k calls (one k's
     bands at a time), each call synchronous.  Reports p50 / p99 wall time per apply."""
+    from tc_amd import autotune as AT
     from tc_amd import system as S
     from tc_amd import tcx
     systems = S.batched_uneven_systems()
@@ -459,6 +460,8 @@ def bench_batched(log, replays):
         eng = tcx.Engine(0)
         eng.load_system(sys_)
         nb = sys_.nocc.copy()
+        for term in ("hf2x", "tc2x"):  # the autotuner's shapes for this cell (automatic if none)
+            AT.apply_tuned(eng, sys_, nb, term, tcx.PREC_FP32)
         d_phi = eng.to_device(sys_.phi_occ)
         d_out = eng.device_buffer(sys_.phi_occ.nbytes)
         ctxs.append((sys_, eng, nb, d_phi, d_out))
@@ -667,10 +670,19 @@ def bench_tc_legs(eng, log, tc_steps):
         eng.synchronize()
         return time.perf_counter() - t0
 
+    from tc_amd import autotune as AT
     res = {"workload": "BITC fcc a=10.26 bohr, 4x4x4 k, 16 bands/k (all occupied), FFT 40^3, RPA Jastrow A=0.5",
-           "pairs_per_apply": pairs}
+           "pairs_per_apply": pairs, "launch_shapes": {}}
+
+    def tuned(term, prec):
+        """The autotuner's shape for this term and precision (automatic if none)."""
+        shape = AT.apply_tuned(eng, sys_, nb, term, prec)
+        res["launch_shapes"][f"{term}_{'fp32' if prec == tcx.PREC_FP32 else 'fp64'}"] = \
+            list(shape) if shape else "automatic"
+
     # ---- tc2x
     log("bench tc: tc2x FP32-split warm-up")
+    tuned("tc2x", tcx.PREC_FP32)
     apply("tc2x", tcx.PREC_FP32)
     eng.enable_timing(True)
     t32 = []
@@ -680,6 +692,7 @@ def bench_tc_legs(eng, log, tc_steps):
     ms, launches, kpairs = eng.get_timing()
     eng.enable_timing(False)
     h32 = d_out.download(np.complex128)
+    tuned("tc2x", tcx.PREC_FP64)
     apply("tc2x", tcx.PREC_FP64)
     t64 = apply("tc2x", tcx.PREC_FP64)
     log(f"bench tc: tc2x FP64 apply {t64:.3f} s")
@@ -708,6 +721,7 @@ def bench_tc_legs(eng, log, tc_steps):
     res["three_body"] = {}
     for term in ("tc3a3b3b4b6", "tc3a2a4b2b5"):
         eng.load_system(sys_)             # drops the occupied-set caches: the first apply does all the work
+        tuned(term, tcx.PREC_FP32)        # FP64 runs with the same pair sets
         first = apply(term, tcx.PREC_FP32)
         h32 = d_out.download(np.complex128)
         again = apply(term, tcx.PREC_FP32)

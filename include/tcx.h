@@ -376,12 +376,13 @@ int tcx_set_pipeline_order(tcx_context *ctx, int term, int order);
  * term 0 (hf2x): n1*n2 for the z-first order, the support columns for plane-first. */
 int tcx_get_record_columns(tcx_context *ctx, int term, double *columns);
 
-/* Launch shape of a pair pipeline (term 0: hf2x, 1: tc2x): each launch pairs
- * `targets` consecutive applied bands (k,j) with `sources` consecutive
- * occupied orbitals (q,i); targets*sources pair records are in flight (the
- * pair-buffer size).  0 selects the automatic shape.  Set by the autotuner
- * (tc_amd/autotune.py).  No counterpart in the reference (its OpenMP loop
- * has no launch shape). */
+/* Launch shape of a pair pipeline (term 0: hf2x, 1: tc2x, 2: the three-body terms'
+ * pair sets): each launch pairs `targets` consecutive applied bands (k,j) (or
+ * occupied orbitals, for the three-body occupied-set phases) with `sources`
+ * consecutive occupied orbitals (q,i); targets*sources pair records are in flight
+ * (the pair-buffer size; for term 2 capped by a 3 GiB record budget).  0 selects the
+ * automatic shape.  Set by the autotuner (tc_amd/autotune.py).  No counterpart in
+ * the reference (its OpenMP loop has no launch shape). */
 int tcx_set_launch_shape(tcx_context *ctx, int term, long targets, long sources);
 
 /* Per-kernel-class timing with HIP events on the context's stream.

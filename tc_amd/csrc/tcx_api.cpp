@@ -1221,9 +1221,12 @@ int tcx_memset_zero(tcx_context* ctx, void* ptr, unsigned long long bytes)
 int tcx_set_launch_shape(tcx_context* ctx, int term, long targets, long sources)
 {
     return guarded(ctx, [&] {
-        if (term < 0 || term > 1) throw Error(TCX_ERR_ARG, "launch_shape: term must be 0 (hf2x) or 1 (tc2x)");
+        if (term < 0 || term > 2)
+            throw Error(TCX_ERR_ARG, "launch_shape: term must be 0 (hf2x), 1 (tc2x) or 2 (three-body terms)");
         if (targets < 0 || sources < 0 || targets > 65535 || sources > 65535)
             throw Error(TCX_ERR_ARG, "launch_shape: targets and sources must be in [0, 65535]");
+        if (term == 2 && (ctx->impl.shape_targets[2] != targets || ctx->impl.shape_sources[2] != sources))
+            ctx->impl.pairsets.clear();  // the three-body pair sets are cached per shape
         ctx->impl.shape_targets[term] = targets;
         ctx->impl.shape_sources[term] = sources;
     });

@@ -48,9 +48,12 @@ const PairSet& Context::pair_set(const std::vector<int>& nb, int kind, int chann
     const bool irr = kind == kPairsIrrOcc || kind == kPairsIrrSameK;
     const std::vector<PairItem> tg = kind == kPairsAppOcc ? applied_items(nb) : occupied_items(irr);
     const std::vector<PairItem> src = occupied_items();
+    // the launch shape (tcx_set_launch_shape term 2) within the record budget, else automatic
     const long maxpairs = std::max<long>(1, (long)(kRecordBudget / ((size_t)channels * N * es)));
-    const long tc = std::max<long>(1, std::min<long>({kTc3Targets, maxpairs, (long)std::max<size_t>(1, tg.size())}));
+    const long want_t = shape_targets[2] > 0 ? shape_targets[2] : kTc3Targets;
+    const long tc = std::max<long>(1, std::min<long>({want_t, maxpairs, (long)std::max<size_t>(1, tg.size())}));
     long sc = std::max<long>(1, maxpairs / tc);
+    if (shape_sources[2] > 0) sc = std::min<long>(sc, shape_sources[2]);
     sc = std::min<long>(sc, (long)std::max<size_t>(1, src.size()));
     PairSet S;
     build_pairs(S, tg, src, tc, sc, kcart, kind == kPairsOccSameK || kind == kPairsIrrSameK);

@@ -162,8 +162,8 @@ struct Context {
     DBuf d_phi_app, d_out, d_appR, d_appGrad, d_tmp64, d_W, d_pairbuf, d_ktab, d_work;
 
     // launch shape per term (0 hf2x, 1 tc2x): targets x sources per launch, 0 = auto
-    long shape_targets[2] = {0, 0};
-    long shape_sources[2] = {0, 0};
+    long shape_targets[3] = {0, 0, 0};  // hf2x, tc2x, three-body pair sets (0 = automatic)
+    long shape_sources[3] = {0, 0, 0};
     int hf2x_order = -1;          // -1 automatic, 0 z-first, 1 plane-first (FP32)
     double hf2x_record_cols = 0;  // (kx, ky) columns per plane of the last hf2x apply's records
     bool use_plane_first(int prec) const;

@@ -495,9 +495,11 @@ class Engine:
         return h1, h2, h3
 
     # --------------------------------------------------------------- tuning / timing
+    SHAPE_TERMS = {"hf2x": 0, "tc2x": 1, "tc3": 2, "tc3a3b3b4b6": 2, "tc3a2a4b2b5": 2}
+
     def set_launch_shape(self, term, targets=0, sources=0):
-        """term: "hf2x" or "tc2x"; 0 = automatic."""
-        t = {"hf2x": 0, "tc2x": 1}[term] if isinstance(term, str) else int(term)
+        """term: "hf2x", "tc2x" or "tc3" (the three-body terms, also by their names); 0 = automatic."""
+        t = self.SHAPE_TERMS[term] if isinstance(term, str) else int(term)
         self._check(self.L.tcx_set_launch_shape(self.h, t, int(targets), int(sources)), "set_launch_shape")
 
     def set_pipeline_order(self, term, order):

@@ -166,3 +166,24 @@ def test_pair_lists_follow_state_changes(engine):
         o = O.Oracle(sys_)
         for term in THREE_BODY_X:
             assert _close(getattr(engine, term)(nb, phi), getattr(o, term)(nb, phi)), (nocc, term)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (5, 2)])
+def test_three_body_launch_shape(engine, shape):
+    """The three-body pair sets follow tcx_set_launch_shape (term 2) and the results do not
+    depend on it beyond FP64 summation order (1e-12)."""
+    sys_ = S.tiny_system(method="BITC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=131,
+                         A=np.full((2, 2), 0.5))
+    engine.load_system(sys_)
+    nb = np.array([2, 1], np.int32)
+    _, phi = S.applied_bands(sys_, nb=nb, seed=132)
+    engine.set_launch_shape("tc3", 0, 0)
+    base = {t: getattr(engine, t)(nb, phi) for t in THREE_BODY_X}
+    engine.set_launch_shape("tc3", *shape)
+    engine.load_system(sys_)  # fresh occupied-set caches under the new shape
+    for t in THREE_BODY_X:
+        got = getattr(engine, t)(nb, phi)
+        err = np.abs(got - base[t]).max() / np.abs(base[t]).max()
+        print(t, shape, err, flush=True)
+        assert err <= 1e-12, t
+    engine.set_launch_shape("tc3", 0, 0)
