@@ -117,6 +117,17 @@ __global__ __launch_bounds__((Tc2xPlanes<T, NP>::threads)) void k_tc2x_B(cpx<T>*
     const size_t ng = (size_t)NP * NP * nz;
     PlaneTile<T, NP>& tile = S[on ? pc : 0];
     cpx<T>* base = buf + ((size_t)blockIdx.y * 4 + (on ? c : 0)) * ng + (size_t)(on ? z : 0) * NP * NP;
+    // (V2, u) at the thread's combine points e = threadIdx.x + i NT, loaded before the
+    // forward transform so that the loads overlap it (coalesced over e)
+    constexpr int EPT = (P * NP * NP + NT - 1) / NT;
+    const PairEntry pe = pairs[blockIdx.y];
+    const cpx<T>* tab = table + (size_t)(pe.kq - kq0) * ng + (size_t)blockIdx.x * P * NP * NP;
+    const int nvalid = (nz - (int)blockIdx.x * P < P ? nz - (int)blockIdx.x * P : P) * NP * NP;
+    cpx<T> vus[EPT];
+    static_for<0, EPT>([&](auto ic) {
+        const int e = threadIdx.x + decltype(ic)::value * NT;
+        if (e < nvalid) vus[decltype(ic)::value] = tab[e];
+    });
     cpx<T> v[NP];
     if (on) {
         load_pencil_nt<T, NP>(v, base + t, NP);  // column x = t; planes are streamed once
@@ -130,18 +141,16 @@ __global__ __launch_bounds__((Tc2xPlanes<T, NP>::threads)) void k_tc2x_B(cpx<T>*
         tile_put_row<T, NP>(tile, v, t);
     }
     __syncthreads();
-    {
-        // combine at frequency (x, y, z): b = rho V2 + sum_d sigma_d du_d, a_d = rho du_d (1/N folded in)
-        const PairEntry pe = pairs[blockIdx.y];
-        const double* kq = kqcart + 3 * pe.kq;
-        const cpx<T>* tab = table + (size_t)(pe.kq - kq0) * ng;
-        for (int e = threadIdx.x; e < P * NP * NP; e += NT) {
+    // combine at frequency (x, y, z): b = rho V2 + sum_d sigma_d du_d, a_d = rho du_d (1/N folded in)
+    const double* kq = kqcart + 3 * pe.kq;
+    static_for<0, EPT>([&](auto ic) {
+        const int e = threadIdx.x + decltype(ic)::value * NT;
+        if (e < nvalid) {
             const int q = e / (NP * NP), xy = e % (NP * NP);
             const int zz = blockIdx.x * P + q;
-            if (zz >= nz) continue;
             const int x = xy % NP, y = xy / NP;
             const int mx = centered(x, NP), my = centered(y, NP), mz = centered(zz, nz);
-            const cpx<T> vu = tab[(size_t)zz * NP * NP + xy];  // (V2, u), coalesced over e
+            const cpx<T> vu = vus[decltype(ic)::value];
             const double us = (double)vu.im * invN;
             const T du0 = static_cast<T>((kq[0] + g.b[0] * mx + g.b[3] * my + g.b[6] * mz) * us);
             const T du1 = static_cast<T>((kq[1] + g.b[1] * mx + g.b[4] * my + g.b[7] * mz) * us);
@@ -155,7 +164,7 @@ __global__ __launch_bounds__((Tc2xPlanes<T, NP>::threads)) void k_tc2x_B(cpx<T>*
             Q[2].a[y][x] = scale(r, du1);
             Q[3].a[y][x] = scale(r, du2);
         }
-    }
+    });
     __syncthreads();
     if (on) {
         tile_get_row<T, NP>(tile, v, t);

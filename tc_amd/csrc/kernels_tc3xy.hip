@@ -88,6 +88,32 @@ __device__ __forceinline__ void xy_inverse_store(PlaneTile<T, NP>& tile, cpx<T>*
     __syncthreads();
 }
 
+// Plane points per thread: thread t owns e = t + i kXyBlock, i < xy_ept(NP).
+constexpr int xy_ept(int np) { return (np * np + kXyBlock - 1) / kXyBlock; }
+
+// src[e] at the thread's plane points.  Issued before the forward transform so that the
+// loads overlap it: a load inside a combine loop waits once per iteration, and with it
+// for every store issued before it.
+template <typename T, int NP>
+__device__ __forceinline__ void load_points(cpx<T> (&v)[xy_ept(NP)], const cpx<T>* __restrict__ src)
+{
+    static_for<0, xy_ept(NP)>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const int e = threadIdx.x + i * kXyBlock;
+        if (e < NP * NP) v[i] = src[e];
+    });
+}
+
+// f(ic, e, x, y) at the thread's plane points, ic = std::integral_constant<int, i>
+template <int NP, typename F>
+__device__ __forceinline__ void for_points(F&& f)
+{
+    static_for<0, xy_ept(NP)>([&](auto ic) {
+        const int e = threadIdx.x + decltype(ic)::value * kXyBlock;
+        if (e < NP * NP) f(ic, e, e % NP, e / NP);
+    });
+}
+
 // (kq + G)_d at the points of plane z: x_d = k0[d] + b_1d m_x + b_2d m_y
 struct PlanePoint {
     double k0[3];
@@ -143,12 +169,13 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a7c1(const cpx<T>* __restrict__
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
     const double f = fill[pe.occ];
+    cpx<T> u[xy_ept(NP)];
+    load_points<T, NP>(u, tp);
     xy_forward<T, NP>(S.t[w], rec + ((size_t)p * 4 + w) * ng + pz, true, t);
-    for (int e = threadIdx.x; e < NP * NP; e += kXyBlock) {
-        const int x = e % NP, y = e / NP;
+    for_points<NP>([&](auto ic, int e, int x, int y) {
         double xd[3];
         pp.at(g, x, y, NP, xd);
-        const double u0 = (double)tp[e].re;
+        const double u0 = (double)u[decltype(ic)::value].re;
         const cpx<T> r = S.t[0].a[y][x];
         double2 acc = make_double2(0.0, 0.0);
         for (int d = 0; d < 3; ++d) {
@@ -158,18 +185,17 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a7c1(const cpx<T>* __restrict__
         }
         Y[(size_t)p * ng + pz + e] = down2<T>(make_double2(acc.x * (f * fb3), acc.y * (f * fb3)));
         for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r, f * xd[d] * u0);
-    }
+    });
     __syncthreads();
     const int ow = w >= 1 ? w - 1 : 0;  // output slot of wavefront w (wavefront 0 stores nothing)
     xy_inverse_store<T, NP>(S.t[w], P + ((size_t)p * 6 + ow) * ng + pz, w >= 1, t);
-    for (int e = threadIdx.x; e < NP * NP; e += kXyBlock) {
-        const int x = e % NP, y = e / NP;
+    for_points<NP>([&](auto ic, int e, int x, int y) {
         double xd[3];
         pp.at(g, x, y, NP, xd);
-        const double u1 = (double)tp[e].im;
+        const double u1 = (double)u[decltype(ic)::value].im;
         const cpx<T> r = S.t[0].a[y][x];
         for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r, f * xd[d] * u1);
-    }
+    });
     __syncthreads();
     xy_inverse_store<T, NP>(S.t[w], P + ((size_t)p * 6 + 3 + ow) * ng + pz, w >= 1, t);
 }
@@ -190,13 +216,16 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a7c2(const cpx<T>* __restrict__
     const PairEntry pe = pairs[p];
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
+    cpx<T> u[xy_ept(NP)], yv[xy_ept(NP)];
+    load_points<T, NP>(u, tp);
+    load_points<T, NP>(yv, Y + (size_t)p * ng + pz);
     // channels 0..3 = (s,d) (0,0) (0,1) (0,2) (1,0) in tiles 0..3
     xy_forward<T, NP>(S.t[w], P + ((size_t)p * 6 + w) * ng + pz, true, t);
-    for (int e = threadIdx.x; e < NP * NP; e += kXyBlock) {
-        const int x = e % NP, y = e / NP;
+    for_points<NP>([&](auto ic, int e, int x, int y) {
+        constexpr int i = decltype(ic)::value;
         double xd[3];
         pp.at(g, x, y, NP, xd);
-        const double u0 = (double)tp[e].re, u1 = (double)tp[e].im;
+        const double u0 = (double)u[i].re, u1 = (double)u[i].im;
         const double c[4] = {xd[0] * u0, xd[1] * u0, xd[2] * u0, xd[0] * u1};
         double2 acc = make_double2(0.0, 0.0);
         for (int q = 0; q < 4; ++q) {
@@ -204,22 +233,21 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a7c2(const cpx<T>* __restrict__
             acc.x += s.x * c[q];
             acc.y += s.y * c[q];
         }
-        const double2 y0 = up2(Y[(size_t)p * ng + pz + e]);
+        const double2 y0 = up2(yv[i]);
         S.t[0].a[y][x] = down2<T>(make_double2(y0.x + fb6 * acc.x, y0.y + fb6 * acc.y));
-    }
+    });
     __syncthreads();
     // channels 4, 5 = (1,1), (1,2) in tiles 1, 2
     xy_forward<T, NP>(S.t[w], P + ((size_t)p * 6 + 3 + (w == 1 || w == 2 ? w : 0)) * ng + pz, w == 1 || w == 2, t);
-    for (int e = threadIdx.x; e < NP * NP; e += kXyBlock) {
-        const int x = e % NP, y = e / NP;
+    for_points<NP>([&](auto ic, int e, int x, int y) {
         double xd[3];
         pp.at(g, x, y, NP, xd);
-        const double u1 = (double)tp[e].im;
+        const double u1 = (double)u[decltype(ic)::value].im;
         const double2 s1 = up2(S.t[1].a[y][x]), s2 = up2(S.t[2].a[y][x]);
         const double2 y0 = up2(S.t[0].a[y][x]);
         const double c1 = fb6 * xd[1] * u1, c2 = fb6 * xd[2] * u1;
         S.t[0].a[y][x] = down2<T>(make_double2(y0.x + s1.x * c1 + s2.x * c2, y0.y + s1.y * c1 + s2.y * c2));
-    }
+    });
     __syncthreads();
     xy_inverse_store<T, NP>(S.t[0], Y + (size_t)p * ng + pz, w == 0, t);
 }
@@ -240,12 +268,13 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_b4(const cpx<T>* __restrict__ r
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
     const double f = fill[pe.occ];
+    cpx<T> u[xy_ept(NP)];
+    load_points<T, NP>(u, tp);
     xy_forward<T, NP>(S.t[w], rec + ((size_t)p * 3 + (w < 3 ? w : 0)) * ng + pz, w < 3, t);
-    for (int e = threadIdx.x; e < NP * NP; e += kXyBlock) {
-        const int x = e % NP, y = e / NP;
+    for_points<NP>([&](auto ic, int e, int x, int y) {
         double xd[3];
         pp.at(g, x, y, NP, xd);
-        const double u0 = (double)tp[e].re;
+        const double u0 = (double)u[decltype(ic)::value].re;
         double2 acc = make_double2(0.0, 0.0);
         for (int d = 0; d < 3; ++d) {
             const double2 s = up2(S.t[d].a[y][x]);
@@ -253,7 +282,7 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_b4(const cpx<T>* __restrict__ r
             acc.y += s.y * (xd[d] * u0);
         }
         S.t[0].a[y][x] = down2<T>(make_double2(acc.x * (fb4 * f), acc.y * (fb4 * f)));
-    }
+    });
     __syncthreads();
     xy_inverse_store<T, NP>(S.t[0], Z + (size_t)p * ng + pz, w == 0, t);
 }
@@ -265,13 +294,13 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_b4(const cpx<T>* __restrict__ r
 // thread then keeps its points of them in registers, so all four tiles take outputs and
 // the 6 NIN outputs leave in rounds of four (three full rounds for BITC).
 template <typename T, int NP, int NIN>
-__global__ __launch_bounds__(kXyBlock) void k_xy_a8c1(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ O,
+__global__ __launch_bounds__(kXyBlock, sizeof(T) == 4 ? 3 : 1) void k_xy_a8c1(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ O,
                                                       const PairEntry* __restrict__ pairs,
                                                       const cpx<T>* __restrict__ tab, int kq0,
                                                       const double* __restrict__ kqcart, int nz, Geom g)
 {
     constexpr int NOUT = 6 * NIN;
-    constexpr int EPT = (NP * NP + kXyBlock - 1) / kXyBlock;  // plane points per thread
+    constexpr int EPT = xy_ept(NP);
     __shared__ XyTiles<T, NP> S;
     const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
     const int z = blockIdx.x, p = blockIdx.y;
@@ -279,6 +308,8 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a8c1(const cpx<T>* __restrict__
     const PairEntry pe = pairs[p];
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
+    cpx<T> uv[EPT];
+    load_points<T, NP>(uv, tp);
     xy_forward<T, NP>(S.t[w], rec + ((size_t)p * NIN + (w < NIN ? w : 0)) * ng + pz, w < NIN, t);
     cpx<T> r[NIN][EPT];
     static_for<0, EPT>([&](auto ic) {
@@ -303,7 +334,7 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a8c1(const cpx<T>* __restrict__
                 const int x = e % NP, y = e / NP;
                 double xd[3];
                 pp.at(g, x, y, NP, xd);
-                const cpx<T> u = tp[e];
+                const cpx<T> u = uv[i];
                 static_for<0, kXyWaves>([&](auto wc) {
                     constexpr int wi = decltype(wc)::value;
                     const int o = o0 + wi;
@@ -342,12 +373,13 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a8c2(const cpx<T>* __restrict__
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
     const double f2 = fill[pe.occ];
+    cpx<T> uv[xy_ept(NP)];
+    load_points<T, NP>(uv, tp);
     xy_forward<T, NP>(S.t[w], rec + ((size_t)p * 4 + w) * ng + pz, true, t);
-    for (int e = threadIdx.x; e < NP * NP; e += kXyBlock) {
-        const int x = e % NP, y = e / NP;
+    for_points<NP>([&](auto ic, int e, int x, int y) {
         double xd[3];
         pp.at(g, x, y, NP, xd);
-        const double u = (double)tp[e].re;
+        const double u = (double)uv[decltype(ic)::value].re;
         const cpx<T> r0 = S.t[0].a[y][x];
         double2 b = make_double2(0.0, 0.0);
         for (int d = 0; d < 3; ++d) {
@@ -357,7 +389,7 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a8c2(const cpx<T>* __restrict__
         }
         S.t[0].a[y][x] = down2<T>(make_double2(b.x * (fb5 * f2), b.y * (fb5 * f2)));
         for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r0, f2 * fa4 * xd[d] * u);
-    }
+    });
     __syncthreads();
     xy_inverse_store<T, NP>(S.t[w], Z + ((size_t)p * 4 + w) * ng + pz, true, t);
 }
