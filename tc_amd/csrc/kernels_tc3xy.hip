@@ -2,13 +2,16 @@
 // per-term G-space combine fused in.
 //
 // A record channel arrives transformed along z only (the products pass); one workgroup
-// owns plane z of one pair and runs, in LDS tiles of four wavefronts,
+// owns plane z of one pair and four LDS tiles (channel planes), and runs
 //     FFT_y, FFT_x of the input channels        (completing FFT_forward),
 //     the term's combine at every G of the plane,
 //     inverse FFT_x, FFT_y of the output channels (the xy half of FFT_backward),
 // so that every input channel is read once and every output written once; the z pass of
-// FFT_backward follows separately or fused with the consumer.  The combines are the
-// reference's G-space products with du_s,d(G) = (kq+G)_d u(kq+G, s):
+// FFT_backward follows separately or fused with the consumer.  Line t of tile c runs on
+// thread c NP + t, so a phase that transforms R tiles occupies ceil(R NP / 64) wavefronts
+// and the rest skip it; the du_s,d outputs leave in rounds of their three d components.
+// The combines are the reference's G-space products with
+// du_s,d(G) = (kq+G)_d u(kq+G, s):
 //   tc3a3b3b4b6 (src/calc_hamiltonian_tc3a3b3b4b6.cpp): [3b3] and the [3a3,b4,b6] channels
 //     (:280-349), the [3b3]+[3b6] merge (:344-358), [3b4] (:740-754);
 //   tc3a2a4b2b5 (src/calc_hamiltonian_tc3a2a4b2b5.cpp): the q1-q2 channels (:371-375) and
@@ -23,12 +26,37 @@ namespace tcx {
 
 namespace {
 
-constexpr int kXyWaves = 4;
-constexpr int kXyBlock = 64 * kXyWaves;
+// Workgroup of the xy passes at grid NP: NT channel planes (LDS tiles), line t of tile c on
+// thread c NP + t.
+template <int NP, int NT>
+struct XyGroup {
+    static constexpr int threads = ((NT * NP + 63) / 64) * 64;
+    static constexpr int ept = (NP * NP + threads - 1) / threads;  // combine points per thread
+};
 
-template <typename T, int NP>
+template <typename T, int NP, int NT>
 struct XyTiles {
-    PlaneTile<T, NP> t[kXyWaves];
+    PlaneTile<T, NP> t[NT];
+};
+
+// Wavefronts per SIMD that the LDS of the tiles allows (160 KB per CU, 4 SIMDs), as the
+// register bound of the four-tile passes where it is 3 or fewer, so that registers do not
+// lower the occupancy further (above that the bound would make the small grids spill; the
+// three-tile passes spill under it at 24^3-36^3 and take no bound).
+template <typename T, int NP, int NT>
+constexpr int xy_waves_per_simd()
+{
+    const int wgs = (160 * 1024) / (int)sizeof(XyTiles<T, NP, NT>);
+    const int w = (wgs * XyGroup<NP, NT>::threads / 64 + 3) / 4;
+    return w < 1 || w > 3 ? 1 : w;
+}
+
+// The calling thread's tile and line: c = tile (NT or more on padding lanes), t = line.
+template <int NP, int NT>
+struct XyLane {
+    int c, t;
+    __device__ XyLane() : c((int)threadIdx.x / NP), t((int)threadIdx.x % NP) {}
+    __device__ int tile() const { return c < NT ? c : 0; }
 };
 
 template <typename T>
@@ -47,20 +75,19 @@ __device__ __forceinline__ cpx<T> mulr(cpx<T> a, double s)  // FP64 product, rou
     return cpx<T>{(T)((double)a.re * s), (T)((double)a.im * s)};
 }
 
-// Lanes t < NP of a wavefront with a job: column x = t -> FFT_y -> tile -> row y = t ->
-// FFT_x -> tile.  Every thread of the workgroup calls it (two barriers).
+// Threads with a job: column x = t -> FFT_y -> tile -> row y = t -> FFT_x -> tile.  Every
+// thread of the workgroup calls it (two barriers).
 template <typename T, int NP>
 __device__ __forceinline__ void xy_forward(PlaneTile<T, NP>& tile, const cpx<T>* plane, bool job, int t)
 {
     cpx<T> v[NP];
-    const bool on = job && t < NP;
-    if (on) {
+    if (job) {
         load_pencil_nt<T, NP>(v, plane + t, NP);  // record planes are streamed once
         fft<T, NP, -1>(v);
         tile_put_col<T, NP>(tile, v, t);
     }
     __syncthreads();
-    if (on) {
+    if (job) {
         tile_get_row<T, NP>(tile, v, t);
         fft<T, NP, -1>(v);
         tile_put_row<T, NP>(tile, v, t);
@@ -73,14 +100,13 @@ template <typename T, int NP>
 __device__ __forceinline__ void xy_inverse_store(PlaneTile<T, NP>& tile, cpx<T>* plane, bool job, int t)
 {
     cpx<T> v[NP];
-    const bool on = job && t < NP;
-    if (on) {
+    if (job) {
         tile_get_row<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
         tile_put_row<T, NP>(tile, v, t);
     }
     __syncthreads();
-    if (on) {
+    if (job) {
         tile_get_col<T, NP>(tile, v, t);
         fft<T, NP, 1>(v);
         store_pencil_nt<T, NP>(v, plane + t, NP);
@@ -88,28 +114,25 @@ __device__ __forceinline__ void xy_inverse_store(PlaneTile<T, NP>& tile, cpx<T>*
     __syncthreads();
 }
 
-// Plane points per thread: thread t owns e = t + i kXyBlock, i < xy_ept(NP).
-constexpr int xy_ept(int np) { return (np * np + kXyBlock - 1) / kXyBlock; }
-
-// src[e] at the thread's plane points.  Issued before the forward transform so that the
-// loads overlap it: a load inside a combine loop waits once per iteration, and with it
-// for every store issued before it.
-template <typename T, int NP>
-__device__ __forceinline__ void load_points(cpx<T> (&v)[xy_ept(NP)], const cpx<T>* __restrict__ src)
+// src[e] at the thread's plane points e = threadIdx.x + i threads.  Issued before the
+// forward transform so that the loads overlap it: a load inside a combine loop waits once
+// per iteration, and with it for every store issued before it.
+template <typename T, int NP, int NT>
+__device__ __forceinline__ void load_points(cpx<T> (&v)[(XyGroup<NP, NT>::ept)], const cpx<T>* __restrict__ src)
 {
-    static_for<0, xy_ept(NP)>([&](auto ic) {
+    static_for<0, XyGroup<NP, NT>::ept>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        const int e = threadIdx.x + i * kXyBlock;
+        const int e = threadIdx.x + i * XyGroup<NP, NT>::threads;
         if (e < NP * NP) v[i] = src[e];
     });
 }
 
 // f(ic, e, x, y) at the thread's plane points, ic = std::integral_constant<int, i>
-template <int NP, typename F>
+template <int NP, int NT, typename F>
 __device__ __forceinline__ void for_points(F&& f)
 {
-    static_for<0, xy_ept(NP)>([&](auto ic) {
-        const int e = threadIdx.x + decltype(ic)::value * kXyBlock;
+    static_for<0, XyGroup<NP, NT>::ept>([&](auto ic) {
+        const int e = threadIdx.x + decltype(ic)::value * XyGroup<NP, NT>::threads;
         if (e < NP * NP) f(ic, e, e % NP, e / NP);
     });
 }
@@ -128,6 +151,46 @@ struct PlanePoint {
         for (int d = 0; d < 3; ++d) xd[d] = k0[d] + g.b[d] * mx + g.b[3 + d] * my;
     }
 };
+
+// Outputs of the form du_s,d src: NRND rounds of the three d components, round k in
+// tiles 0..2 (tile d), inverse-transformed into dst(3k + d).  Tile d at the thread's point
+// i takes src(ic, k) (kq+G)_d scale(k, uv[i]).  With the d components in fixed tiles the
+// per-point work has no index arithmetic; three tiles of NP lines fill two wavefronts at 40^3.
+template <typename T, int NP, int NT, int NRND, typename Src, typename Scale, typename Dst>
+__device__ __forceinline__ void xy_du_rounds(XyTiles<T, NP, NT>& S, const PlanePoint& pp, const Geom& g,
+                                             const cpx<T> (&uv)[(XyGroup<NP, NT>::ept)], Src&& src, Scale&& scale,
+                                             Dst&& dst)
+{
+    static_assert(NT >= 3, "the three d components need three tiles");
+    const XyLane<NP, NT> L;
+#pragma unroll 1
+    for (int k = 0; k < NRND; ++k) {
+        // the per-point terms are recomputed each round rather than held across rounds:
+        // the empty asm hides that the thread index does not change
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        static_for<0, XyGroup<NP, NT>::ept>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            const int e = tid + i * XyGroup<NP, NT>::threads;
+            if (e < NP * NP) {
+                const int x = e % NP, y = e / NP;
+                double xd[3];
+                pp.at(g, x, y, NP, xd);
+                const cpx<T> r = src(ic, k);
+                const double sc = scale(k, uv[i]);
+                for (int d = 0; d < 3; ++d) S.t[d].a[y][x] = mulr(r, xd[d] * sc);
+            }
+        });
+        __syncthreads();
+        const bool job = L.c < 3;
+        xy_inverse_store<T, NP>(S.t[L.tile()], dst(3 * k + (job ? L.c : 0)), job, L.t);
+    }
+}
+
+// Tiles per workgroup of each pass: as many as its largest group of channels transformed
+// together (three tiles give 4 workgroups of 2 wavefronts per CU at 40^3 in FP32, four
+// tiles 3 of 3).
+constexpr int kTilesA7c1 = 4, kTilesA7c2 = 3, kTilesB4 = 3, kTilesA8c1 = 3, kTilesA8c2 = 4;
 
 }  // namespace
 
@@ -155,28 +218,32 @@ __global__ __launch_bounds__(256) void k_u2_table(cpx<T>* __restrict__ tab, cons
 // Out: Y[p] = f b3 sum_d du_0,d rec[p][1+d] in G space (completed later by the second combine);
 //      P[p][3s+d] = f du_s,d rec[p][0], inverse-transformed along x and y.
 template <typename T, int NP>
-__global__ __launch_bounds__(kXyBlock) void k_xy_a7c1(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ P,
-                                                      cpx<T>* __restrict__ Y, const PairEntry* __restrict__ pairs,
-                                                      const cpx<T>* __restrict__ tab, int kq0,
-                                                      const double* __restrict__ kqcart,
-                                                      const double* __restrict__ fill, int nz, Geom g, double fb3)
+__global__ __launch_bounds__((XyGroup<NP, kTilesA7c1>::threads), (xy_waves_per_simd<T, NP, kTilesA7c1>()))
+void k_xy_a7c1(
+    const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ P, cpx<T>* __restrict__ Y, const PairEntry* __restrict__ pairs,
+    const cpx<T>* __restrict__ tab, int kq0, const double* __restrict__ kqcart, const double* __restrict__ fill, int nz,
+    Geom g, double fb3)
 {
-    __shared__ XyTiles<T, NP> S;
-    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    constexpr int NT = kTilesA7c1;
+    constexpr int EPT = XyGroup<NP, NT>::ept;
+    __shared__ XyTiles<T, NP, NT> S;
+    const XyLane<NP, NT> L;
     const int z = blockIdx.x, p = blockIdx.y;
     const size_t ng = (size_t)NP * NP * nz, pz = (size_t)z * NP * NP;
     const PairEntry pe = pairs[p];
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
     const double f = fill[pe.occ];
-    cpx<T> u[xy_ept(NP)];
-    load_points<T, NP>(u, tp);
-    xy_forward<T, NP>(S.t[w], rec + ((size_t)p * 4 + w) * ng + pz, true, t);
-    for_points<NP>([&](auto ic, int e, int x, int y) {
+    cpx<T> u[EPT];
+    load_points<T, NP, NT>(u, tp);
+    xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * 4 + L.tile()) * ng + pz, L.c < 4, L.t);
+    cpx<T> r0[EPT];
+    for_points<NP, NT>([&](auto ic, int e, int x, int y) {
+        constexpr int i = decltype(ic)::value;
         double xd[3];
         pp.at(g, x, y, NP, xd);
-        const double u0 = (double)u[decltype(ic)::value].re;
-        const cpx<T> r = S.t[0].a[y][x];
+        const double u0 = (double)u[i].re;
+        r0[i] = S.t[0].a[y][x];
         double2 acc = make_double2(0.0, 0.0);
         for (int d = 0; d < 3; ++d) {
             const double2 s = up2(S.t[1 + d].a[y][x]);
@@ -184,94 +251,85 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a7c1(const cpx<T>* __restrict__
             acc.y += s.y * (xd[d] * u0);
         }
         Y[(size_t)p * ng + pz + e] = down2<T>(make_double2(acc.x * (f * fb3), acc.y * (f * fb3)));
-        for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r, f * xd[d] * u0);
     });
-    __syncthreads();
-    const int ow = w >= 1 ? w - 1 : 0;  // output slot of wavefront w (wavefront 0 stores nothing)
-    xy_inverse_store<T, NP>(S.t[w], P + ((size_t)p * 6 + ow) * ng + pz, w >= 1, t);
-    for_points<NP>([&](auto ic, int e, int x, int y) {
-        double xd[3];
-        pp.at(g, x, y, NP, xd);
-        const double u1 = (double)u[decltype(ic)::value].im;
-        const cpx<T> r = S.t[0].a[y][x];
-        for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r, f * xd[d] * u1);
-    });
-    __syncthreads();
-    xy_inverse_store<T, NP>(S.t[w], P + ((size_t)p * 6 + 3 + ow) * ng + pz, w >= 1, t);
+    __syncthreads();  // the inputs have been read before outputs overwrite the tiles
+    xy_du_rounds<T, NP, NT, 2>(
+        S, pp, g, u, [&](auto ic, int) { return r0[decltype(ic)::value]; },
+        [&](int k, cpx<T> uk) { return f * (double)(k == 0 ? uk.re : uk.im); },
+        [&](int o) { return P + ((size_t)p * 6 + o) * ng + pz; });
 }
 
 // Phase-1 second combine ([3b3] and [3b6] merged by linearity).  In: P[p][0..5] z-transformed
 // (after the density step), Y[p] in G space.  Out: Y[p] += b6 sum_{s,d} du_s,d P[p][3s+d],
 // inverse-transformed along x and y (in place).
 template <typename T, int NP>
-__global__ __launch_bounds__(kXyBlock) void k_xy_a7c2(const cpx<T>* __restrict__ P, cpx<T>* __restrict__ Y,
-                                                      const PairEntry* __restrict__ pairs,
-                                                      const cpx<T>* __restrict__ tab, int kq0,
-                                                      const double* __restrict__ kqcart, int nz, Geom g, double fb6)
+__global__ __launch_bounds__((XyGroup<NP, kTilesA7c2>::threads))
+void k_xy_a7c2(
+    const cpx<T>* __restrict__ P, cpx<T>* __restrict__ Y, const PairEntry* __restrict__ pairs,
+    const cpx<T>* __restrict__ tab, int kq0, const double* __restrict__ kqcart, int nz, Geom g, double fb6)
 {
-    __shared__ XyTiles<T, NP> S;
-    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    constexpr int NT = kTilesA7c2;
+    constexpr int EPT = XyGroup<NP, NT>::ept;
+    __shared__ XyTiles<T, NP, NT> S;
+    const XyLane<NP, NT> L;
     const int z = blockIdx.x, p = blockIdx.y;
     const size_t ng = (size_t)NP * NP * nz, pz = (size_t)z * NP * NP;
     const PairEntry pe = pairs[p];
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
-    cpx<T> u[xy_ept(NP)], yv[xy_ept(NP)];
-    load_points<T, NP>(u, tp);
-    load_points<T, NP>(yv, Y + (size_t)p * ng + pz);
-    // channels 0..3 = (s,d) (0,0) (0,1) (0,2) (1,0) in tiles 0..3
-    xy_forward<T, NP>(S.t[w], P + ((size_t)p * 6 + w) * ng + pz, true, t);
-    for_points<NP>([&](auto ic, int e, int x, int y) {
-        constexpr int i = decltype(ic)::value;
-        double xd[3];
-        pp.at(g, x, y, NP, xd);
-        const double u0 = (double)u[i].re, u1 = (double)u[i].im;
-        const double c[4] = {xd[0] * u0, xd[1] * u0, xd[2] * u0, xd[0] * u1};
-        double2 acc = make_double2(0.0, 0.0);
-        for (int q = 0; q < 4; ++q) {
-            const double2 s = up2(S.t[q].a[y][x]);
-            acc.x += s.x * c[q];
-            acc.y += s.y * c[q];
-        }
-        const double2 y0 = up2(yv[i]);
-        S.t[0].a[y][x] = down2<T>(make_double2(y0.x + fb6 * acc.x, y0.y + fb6 * acc.y));
+    cpx<T> u[EPT], yv[EPT];
+    load_points<T, NP, NT>(u, tp);
+    load_points<T, NP, NT>(yv, Y + (size_t)p * ng + pz);
+    // s = 0, 1 in turn: channels 3s + d in tiles d, the partial sum in FP64 registers
+    double2 acc[EPT];
+    static_for<0, 2>([&](auto sc) {
+        constexpr int sp = decltype(sc)::value;
+        if constexpr (sp == 1) __syncthreads();  // the tiles have been read before the next channels overwrite them
+        xy_forward<T, NP>(S.t[L.tile()], P + ((size_t)p * 6 + 3 * sp + L.tile()) * ng + pz, L.c < 3, L.t);
+        for_points<NP, NT>([&](auto ic, int e, int x, int y) {
+            constexpr int i = decltype(ic)::value;
+            double xd[3];
+            pp.at(g, x, y, NP, xd);
+            const double us = fb6 * (double)(sp == 0 ? u[i].re : u[i].im);
+            double2 a = sp == 0 ? up2(yv[i]) : acc[i];
+            for (int d = 0; d < 3; ++d) {
+                const double2 v = up2(S.t[d].a[y][x]);
+                a.x += v.x * (xd[d] * us);
+                a.y += v.y * (xd[d] * us);
+            }
+            if constexpr (sp == 0)
+                acc[i] = a;
+            else
+                S.t[0].a[y][x] = down2<T>(a);
+        });
     });
     __syncthreads();
-    // channels 4, 5 = (1,1), (1,2) in tiles 1, 2
-    xy_forward<T, NP>(S.t[w], P + ((size_t)p * 6 + 3 + (w == 1 || w == 2 ? w : 0)) * ng + pz, w == 1 || w == 2, t);
-    for_points<NP>([&](auto ic, int e, int x, int y) {
-        double xd[3];
-        pp.at(g, x, y, NP, xd);
-        const double u1 = (double)u[decltype(ic)::value].im;
-        const double2 s1 = up2(S.t[1].a[y][x]), s2 = up2(S.t[2].a[y][x]);
-        const double2 y0 = up2(S.t[0].a[y][x]);
-        const double c1 = fb6 * xd[1] * u1, c2 = fb6 * xd[2] * u1;
-        S.t[0].a[y][x] = down2<T>(make_double2(y0.x + s1.x * c1 + s2.x * c2, y0.y + s1.y * c1 + s2.y * c2));
-    });
-    __syncthreads();
-    xy_inverse_store<T, NP>(S.t[0], Y + (size_t)p * ng + pz, w == 0, t);
+    xy_inverse_store<T, NP>(S.t[0], Y + (size_t)p * ng + pz, L.c == 0, L.t);
 }
 
 // Phase-2 combine ([3b4]).  In: rec[p][0..2] z-transformed conj(chi_qi) qsum_j,d.
 // Out: Z[p] = b4 f sum_d du_0,d rec[p][d], inverse-transformed along x and y.
 template <typename T, int NP>
-__global__ __launch_bounds__(kXyBlock) void k_xy_b4(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Z,
-                                                    const PairEntry* __restrict__ pairs, const cpx<T>* __restrict__ tab,
-                                                    int kq0, const double* __restrict__ kqcart,
-                                                    const double* __restrict__ fill, int nz, Geom g, double fb4)
+__global__ __launch_bounds__((XyGroup<NP, kTilesB4>::threads))
+void k_xy_b4(
+    const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Z, const PairEntry* __restrict__ pairs,
+    const cpx<T>* __restrict__ tab, int kq0, const double* __restrict__ kqcart, const double* __restrict__ fill, int nz,
+    Geom g, double fb4)
 {
-    __shared__ XyTiles<T, NP> S;
-    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    constexpr int NT = kTilesB4;
+    constexpr int EPT = XyGroup<NP, NT>::ept;
+    __shared__ XyTiles<T, NP, NT> S;
+    const XyLane<NP, NT> L;
     const int z = blockIdx.x, p = blockIdx.y;
     const size_t ng = (size_t)NP * NP * nz, pz = (size_t)z * NP * NP;
     const PairEntry pe = pairs[p];
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
     const double f = fill[pe.occ];
-    cpx<T> u[xy_ept(NP)];
-    load_points<T, NP>(u, tp);
-    xy_forward<T, NP>(S.t[w], rec + ((size_t)p * 3 + (w < 3 ? w : 0)) * ng + pz, w < 3, t);
-    for_points<NP>([&](auto ic, int e, int x, int y) {
+    cpx<T> u[EPT];
+    load_points<T, NP, NT>(u, tp);
+    xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * 3 + (L.c < 3 ? L.c : 0)) * ng + pz, L.c < 3, L.t);
+    for_points<NP, NT>([&](auto ic, int e, int x, int y) {
         double xd[3];
         pp.at(g, x, y, NP, xd);
         const double u0 = (double)u[decltype(ic)::value].re;
@@ -284,99 +342,75 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_b4(const cpx<T>* __restrict__ r
         S.t[0].a[y][x] = down2<T>(make_double2(acc.x * (fb4 * f), acc.y * (fb4 * f)));
     });
     __syncthreads();
-    xy_inverse_store<T, NP>(S.t[0], Z + (size_t)p * ng + pz, w == 0, t);
+    xy_inverse_store<T, NP>(S.t[0], Z + (size_t)p * ng + pz, L.c == 0, L.t);
 }
 
 // ----------------------------------------------------------------- tc3a2a4b2b5
 // q1-q2 combine.  In: rec[p][h] z-transformed (h = 0: conj(chi_q1) phi_q2; BITC h = 1:
 // conj(phi_q1) chi_q2).  Out: O[p][6h + 3s + d] = du12_s,d rec[p][h], inverse-transformed
-// along x and y.  The NIN inputs are transformed side by side (waves 0..NIN-1); each
-// thread then keeps its points of them in registers, so all four tiles take outputs and
-// the 6 NIN outputs leave in rounds of four (three full rounds for BITC).
+// along x and y.  The NIN inputs are transformed side by side; each thread then keeps its
+// points of them in registers, so every tile can take outputs.
 template <typename T, int NP, int NIN>
-__global__ __launch_bounds__(kXyBlock, sizeof(T) == 4 ? 3 : 1) void k_xy_a8c1(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ O,
-                                                      const PairEntry* __restrict__ pairs,
-                                                      const cpx<T>* __restrict__ tab, int kq0,
-                                                      const double* __restrict__ kqcart, int nz, Geom g)
+__global__ __launch_bounds__((XyGroup<NP, kTilesA8c1>::threads))
+void k_xy_a8c1(
+    const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ O, const PairEntry* __restrict__ pairs,
+    const cpx<T>* __restrict__ tab, int kq0, const double* __restrict__ kqcart, int nz, Geom g)
 {
     constexpr int NOUT = 6 * NIN;
-    constexpr int EPT = xy_ept(NP);
-    __shared__ XyTiles<T, NP> S;
-    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    constexpr int NT = kTilesA8c1;
+    constexpr int EPT = XyGroup<NP, NT>::ept;
+    __shared__ XyTiles<T, NP, NT> S;
+    const XyLane<NP, NT> L;
     const int z = blockIdx.x, p = blockIdx.y;
     const size_t ng = (size_t)NP * NP * nz, pz = (size_t)z * NP * NP;
     const PairEntry pe = pairs[p];
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
     cpx<T> uv[EPT];
-    load_points<T, NP>(uv, tp);
-    xy_forward<T, NP>(S.t[w], rec + ((size_t)p * NIN + (w < NIN ? w : 0)) * ng + pz, w < NIN, t);
+    load_points<T, NP, NT>(uv, tp);
+    xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * NIN + (L.c < NIN ? L.c : 0)) * ng + pz, L.c < NIN, L.t);
     cpx<T> r[NIN][EPT];
-    static_for<0, EPT>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        const int e = threadIdx.x + i * kXyBlock;
-        if (e < NP * NP) {
-            const int x = e % NP, y = e / NP;
-            static_for<0, NIN>([&](auto hc) { r[decltype(hc)::value][i] = S.t[decltype(hc)::value].a[y][x]; });
-        }
+    for_points<NP, NT>([&](auto ic, int e, int x, int y) {
+        static_for<0, NIN>([&](auto hc) {
+            r[decltype(hc)::value][decltype(ic)::value] = S.t[decltype(hc)::value].a[y][x];
+        });
     });
     __syncthreads();  // the inputs have been read before outputs overwrite the tiles
-#pragma unroll 1
-    for (int o0 = 0; o0 < NOUT; o0 += kXyWaves) {
-        // the per-point terms are recomputed each round rather than held across rounds:
-        // the empty asm hides that the thread index does not change
-        int tid = threadIdx.x;
-        asm volatile("" : "+v"(tid));
-        static_for<0, EPT>([&](auto ic) {
+    // round k: input h = k / 2, s = k % 2 (outputs 6h + 3s + d)
+    xy_du_rounds<T, NP, NT, 2 * NIN>(
+        S, pp, g, uv,
+        [&](auto ic, int k) {
             constexpr int i = decltype(ic)::value;
-            const int e = tid + i * kXyBlock;
-            if (e < NP * NP) {
-                const int x = e % NP, y = e / NP;
-                double xd[3];
-                pp.at(g, x, y, NP, xd);
-                const cpx<T> u = uv[i];
-                static_for<0, kXyWaves>([&](auto wc) {
-                    constexpr int wi = decltype(wc)::value;
-                    const int o = o0 + wi;
-                    if (o < NOUT) {
-                        const int sd = o % 6, d = sd % 3;
-                        const cpx<T> rh = (NIN == 1 || o < 6) ? r[0][i] : r[NIN - 1][i];
-                        const double us = sd < 3 ? (double)u.re : (double)u.im;
-                        const double xs = d == 0 ? xd[0] : (d == 1 ? xd[1] : xd[2]);
-                        S.t[wi].a[y][x] = mulr(rh, xs * us);
-                    }
-                });
-            }
-        });
-        __syncthreads();
-        const int o = o0 + w;
-        xy_inverse_store<T, NP>(S.t[w], O + ((size_t)p * NOUT + (o < NOUT ? o : 0)) * ng + pz, o < NOUT, t);
-    }
+            return (NIN == 1 || k < 2) ? r[0][i] : r[NIN - 1][i];
+        },
+        [&](int k, cpx<T> uk) { return (double)((k & 1) ? uk.im : uk.re); },
+        [&](int o) { return O + ((size_t)p * NOUT + o) * ng + pz; });
 }
 
 // Apply-phase combine.  In: rec[p][0] z-transformed conj(chi_t) phi_kj, rec[p][1+d]
 // z-transformed Q5'_t,d phi_kj.  Out (a separate record): Z[p][0] = b5 f2 sum_d du0_d rec[p][1+d],
 // Z[p][1+d] = f2 a4 du0_d rec[p][0], inverse-transformed along x and y.
 template <typename T, int NP>
-__global__ __launch_bounds__(kXyBlock) void k_xy_a8c2(const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Z,
-                                                      const PairEntry* __restrict__ pairs,
-                                                      const cpx<T>* __restrict__ tab, int kq0,
-                                                      const double* __restrict__ kqcart,
-                                                      const double* __restrict__ fill, int nz, Geom g, double fa4,
-                                                      double fb5)
+__global__ __launch_bounds__((XyGroup<NP, kTilesA8c2>::threads), (xy_waves_per_simd<T, NP, kTilesA8c2>()))
+void k_xy_a8c2(
+    const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Z, const PairEntry* __restrict__ pairs,
+    const cpx<T>* __restrict__ tab, int kq0, const double* __restrict__ kqcart, const double* __restrict__ fill, int nz,
+    Geom g, double fa4, double fb5)
 {
-    __shared__ XyTiles<T, NP> S;
-    const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+    constexpr int NT = kTilesA8c2;
+    constexpr int EPT = XyGroup<NP, NT>::ept;
+    __shared__ XyTiles<T, NP, NT> S;
+    const XyLane<NP, NT> L;
     const int z = blockIdx.x, p = blockIdx.y;
     const size_t ng = (size_t)NP * NP * nz, pz = (size_t)z * NP * NP;
     const PairEntry pe = pairs[p];
     const PlanePoint pp(kqcart + 3 * pe.kq, g, z, nz);
     const cpx<T>* tp = tab + (size_t)(pe.kq - kq0) * ng + pz;
     const double f2 = fill[pe.occ];
-    cpx<T> uv[xy_ept(NP)];
-    load_points<T, NP>(uv, tp);
-    xy_forward<T, NP>(S.t[w], rec + ((size_t)p * 4 + w) * ng + pz, true, t);
-    for_points<NP>([&](auto ic, int e, int x, int y) {
+    cpx<T> uv[EPT];
+    load_points<T, NP, NT>(uv, tp);
+    xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * 4 + L.tile()) * ng + pz, L.c < 4, L.t);
+    for_points<NP, NT>([&](auto ic, int e, int x, int y) {
         double xd[3];
         pp.at(g, x, y, NP, xd);
         const double u = (double)uv[decltype(ic)::value].re;
@@ -391,7 +425,7 @@ __global__ __launch_bounds__(kXyBlock) void k_xy_a8c2(const cpx<T>* __restrict__
         for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r0, f2 * fa4 * xd[d] * u);
     });
     __syncthreads();
-    xy_inverse_store<T, NP>(S.t[w], Z + ((size_t)p * 4 + w) * ng + pz, true, t);
+    xy_inverse_store<T, NP>(S.t[L.tile()], Z + ((size_t)p * 4 + L.tile()) * ng + pz, L.c < 4, L.t);
 }
 
 // ----------------------------------------------------------------- launchers
@@ -401,6 +435,12 @@ template <typename F>
 void for_pair_slices(int npairs, F&& f)
 {
     for (int a = 0; a < npairs; a += 65535) f(a, npairs - a < 65535 ? npairs - a : 65535);
+}
+
+template <int NP, int NT>
+dim3 xy_block()
+{
+    return dim3(XyGroup<NP, NT>::threads);
 }
 
 void check_plane(const Geom& g)
@@ -443,11 +483,11 @@ void launch_xy_a7c1(int prec, const Geom& g, const void* rec, void* P, void* Y, 
 {
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
-#define TCX_L(NP)                                                                                                    \
-    hipLaunchKernelGGL((k_xy_a7c1<T, NP>), dim3(g.n3, n), dim3(kXyBlock), 0, s,                                    \
-                       (const cpx<T>*)((const char*)rec + (size_t)a * 4 * gb), (cpx<T>*)((char*)P + (size_t)a * 6 * gb), \
-                       (cpx<T>*)((char*)Y + (size_t)a * gb), pairs + a, (const cpx<T>*)tab, kq0, kqcart, fill, g.n3, g,  \
-                       fb3);
+#define TCX_L(NP)                                                                                               \
+    hipLaunchKernelGGL((k_xy_a7c1<T, NP>), dim3(g.n3, n), (xy_block<NP, kTilesA7c1>()), 0, s,                   \
+                       (const cpx<T>*)((const char*)rec + (size_t)a * 4 * gb),                                  \
+                       (cpx<T>*)((char*)P + (size_t)a * 6 * gb), (cpx<T>*)((char*)Y + (size_t)a * gb), pairs + a, \
+                       (const cpx<T>*)tab, kq0, kqcart, fill, g.n3, g, fb3);
         TCX_XY_DISPATCH(prec, g, TCX_L);
 #undef TCX_L
     });
@@ -459,7 +499,7 @@ void launch_xy_a7c2(int prec, const Geom& g, const void* P, void* Y, const PairE
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
 #define TCX_L(NP)                                                                                              \
-    hipLaunchKernelGGL((k_xy_a7c2<T, NP>), dim3(g.n3, n), dim3(kXyBlock), 0, s,                              \
+    hipLaunchKernelGGL((k_xy_a7c2<T, NP>), dim3(g.n3, n), (xy_block<NP, kTilesA7c2>()), 0, s,                  \
                        (const cpx<T>*)((const char*)P + (size_t)a * 6 * gb), (cpx<T>*)((char*)Y + (size_t)a * gb), \
                        pairs + a, (const cpx<T>*)tab, kq0, kqcart, g.n3, g, fb6);
         TCX_XY_DISPATCH(prec, g, TCX_L);
@@ -473,7 +513,7 @@ void launch_xy_b4(int prec, const Geom& g, const void* rec, void* Z, const PairE
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
 #define TCX_L(NP)                                                                                                \
-    hipLaunchKernelGGL((k_xy_b4<T, NP>), dim3(g.n3, n), dim3(kXyBlock), 0, s,                                  \
+    hipLaunchKernelGGL((k_xy_b4<T, NP>), dim3(g.n3, n), (xy_block<NP, kTilesB4>()), 0, s,                        \
                        (const cpx<T>*)((const char*)rec + (size_t)a * 3 * gb), (cpx<T>*)((char*)Z + (size_t)a * gb), \
                        pairs + a, (const cpx<T>*)tab, kq0, kqcart, fill, g.n3, g, fb4);
         TCX_XY_DISPATCH(prec, g, TCX_L);
@@ -486,16 +526,17 @@ void launch_xy_a8c1(int prec, const Geom& g, const void* rec, int nin, void* O, 
 {
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
-#define TCX_L(NP)                                                                                           \
-    {                                                                                                       \
-        const cpx<T>* in = (const cpx<T>*)((const char*)rec + (size_t)a * nin * gb);                        \
-        cpx<T>* out = (cpx<T>*)((char*)O + (size_t)a * 6 * nin * gb);                                       \
-        if (nin == 2)                                                                                       \
-            hipLaunchKernelGGL((k_xy_a8c1<T, NP, 2>), dim3(g.n3, n), dim3(kXyBlock), 0, s, in, out, pairs + a, \
-                               (const cpx<T>*)tab, kq0, kqcart, g.n3, g);                                   \
-        else                                                                                                \
-            hipLaunchKernelGGL((k_xy_a8c1<T, NP, 1>), dim3(g.n3, n), dim3(kXyBlock), 0, s, in, out, pairs + a, \
-                               (const cpx<T>*)tab, kq0, kqcart, g.n3, g);                                   \
+#define TCX_L(NP)                                                                                   \
+    {                                                                                               \
+        const cpx<T>* in = (const cpx<T>*)((const char*)rec + (size_t)a * nin * gb);                \
+        cpx<T>* out = (cpx<T>*)((char*)O + (size_t)a * 6 * nin * gb);                               \
+        const dim3 grid(g.n3, n), block = xy_block<NP, kTilesA8c1>();                               \
+        if (nin == 2)                                                                               \
+            hipLaunchKernelGGL((k_xy_a8c1<T, NP, 2>), grid, block, 0, s, in, out, pairs + a,        \
+                               (const cpx<T>*)tab, kq0, kqcart, g.n3, g);                           \
+        else                                                                                        \
+            hipLaunchKernelGGL((k_xy_a8c1<T, NP, 1>), grid, block, 0, s, in, out, pairs + a,        \
+                               (const cpx<T>*)tab, kq0, kqcart, g.n3, g);                           \
     }
         TCX_XY_DISPATCH(prec, g, TCX_L);
 #undef TCX_L
@@ -508,10 +549,11 @@ void launch_xy_a8c2(int prec, const Geom& g, const void* rec, void* Z, const Pai
 {
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
-#define TCX_L(NP)                                                                                                    \
-    hipLaunchKernelGGL((k_xy_a8c2<T, NP>), dim3(g.n3, n), dim3(kXyBlock), 0, s,                                    \
-                       (const cpx<T>*)((const char*)rec + (size_t)a * 4 * gb), (cpx<T>*)((char*)Z + (size_t)a * 4 * gb), \
-                       pairs + a, (const cpx<T>*)tab, kq0, kqcart, fill, g.n3, g, fa4, fb5);
+#define TCX_L(NP)                                                                                                 \
+    hipLaunchKernelGGL((k_xy_a8c2<T, NP>), dim3(g.n3, n), (xy_block<NP, kTilesA8c2>()), 0, s,                     \
+                       (const cpx<T>*)((const char*)rec + (size_t)a * 4 * gb),                                    \
+                       (cpx<T>*)((char*)Z + (size_t)a * 4 * gb), pairs + a, (const cpx<T>*)tab, kq0, kqcart, fill, \
+                       g.n3, g, fa4, fb5);
         TCX_XY_DISPATCH(prec, g, TCX_L);
 #undef TCX_L
     });
