@@ -187,10 +187,19 @@ __device__ __forceinline__ void xy_du_rounds(XyTiles<T, NP, NT>& S, const PlaneP
     }
 }
 
+// a7c2 transforms its six channels at once where two workgroups of six tiles fit in a CU's
+// LDS, else in two groups of three (one six-tile workgroup per CU was slower: FP64 40^3
+// tc3a3b3b4b6 13.2 -> 13.8 s; FP32 40^3 gained 6.66 -> 6.61 s from six tiles)
+template <typename T, int NP>
+constexpr int tiles_a7c2()
+{
+    return 2 * 6 * (int)sizeof(PlaneTile<T, NP>) <= 160 * 1024 ? 6 : 3;
+}
+
 // Tiles per workgroup of each pass: as many as its largest group of channels transformed
 // together (three tiles give 4 workgroups of 2 wavefronts per CU at 40^3 in FP32, four
 // tiles 3 of 3).
-constexpr int kTilesA7c1 = 4, kTilesA7c2 = 3, kTilesB4 = 3, kTilesA8c1 = 3, kTilesA8c2 = 4;
+constexpr int kTilesA7c1 = 4, kTilesB4 = 3, kTilesA8c1 = 3, kTilesA8c2 = 4;
 
 }  // namespace
 
@@ -263,12 +272,12 @@ void k_xy_a7c1(
 // (after the density step), Y[p] in G space.  Out: Y[p] += b6 sum_{s,d} du_s,d P[p][3s+d],
 // inverse-transformed along x and y (in place).
 template <typename T, int NP>
-__global__ __launch_bounds__((XyGroup<NP, kTilesA7c2>::threads))
+__global__ __launch_bounds__((XyGroup<NP, tiles_a7c2<T, NP>()>::threads))
 void k_xy_a7c2(
     const cpx<T>* __restrict__ P, cpx<T>* __restrict__ Y, const PairEntry* __restrict__ pairs,
     const cpx<T>* __restrict__ tab, int kq0, const double* __restrict__ kqcart, int nz, Geom g, double fb6)
 {
-    constexpr int NT = kTilesA7c2;
+    constexpr int NT = tiles_a7c2<T, NP>();
     constexpr int EPT = XyGroup<NP, NT>::ept;
     __shared__ XyTiles<T, NP, NT> S;
     const XyLane<NP, NT> L;
@@ -280,29 +289,49 @@ void k_xy_a7c2(
     cpx<T> u[EPT], yv[EPT];
     load_points<T, NP, NT>(u, tp);
     load_points<T, NP, NT>(yv, Y + (size_t)p * ng + pz);
-    // s = 0, 1 in turn: channels 3s + d in tiles d, the partial sum in FP64 registers
-    double2 acc[EPT];
-    static_for<0, 2>([&](auto sc) {
-        constexpr int sp = decltype(sc)::value;
-        if constexpr (sp == 1) __syncthreads();  // the tiles have been read before the next channels overwrite them
-        xy_forward<T, NP>(S.t[L.tile()], P + ((size_t)p * 6 + 3 * sp + L.tile()) * ng + pz, L.c < 3, L.t);
+    if constexpr (NT == 6) {
+        // all six channels at once: channel 3s + d in tile 3s + d
+        xy_forward<T, NP>(S.t[L.tile()], P + ((size_t)p * 6 + L.tile()) * ng + pz, L.c < 6, L.t);
         for_points<NP, NT>([&](auto ic, int e, int x, int y) {
             constexpr int i = decltype(ic)::value;
             double xd[3];
             pp.at(g, x, y, NP, xd);
-            const double us = fb6 * (double)(sp == 0 ? u[i].re : u[i].im);
-            double2 a = sp == 0 ? up2(yv[i]) : acc[i];
-            for (int d = 0; d < 3; ++d) {
-                const double2 v = up2(S.t[d].a[y][x]);
-                a.x += v.x * (xd[d] * us);
-                a.y += v.y * (xd[d] * us);
+            double2 a = up2(yv[i]);
+            for (int sp = 0; sp < 2; ++sp) {
+                const double us = fb6 * (double)(sp == 0 ? u[i].re : u[i].im);
+                for (int d = 0; d < 3; ++d) {
+                    const double2 v = up2(S.t[3 * sp + d].a[y][x]);
+                    a.x += v.x * (xd[d] * us);
+                    a.y += v.y * (xd[d] * us);
+                }
             }
-            if constexpr (sp == 0)
-                acc[i] = a;
-            else
-                S.t[0].a[y][x] = down2<T>(a);
+            S.t[0].a[y][x] = down2<T>(a);
         });
-    });
+    } else {
+        // s = 0, 1 in turn: channels 3s + d in tiles d, the partial sum in FP64 registers
+        double2 acc[EPT];
+        static_for<0, 2>([&](auto sc) {
+            constexpr int sp = decltype(sc)::value;
+            if constexpr (sp == 1) __syncthreads();  // the tiles have been read before the next channels overwrite them
+            xy_forward<T, NP>(S.t[L.tile()], P + ((size_t)p * 6 + 3 * sp + L.tile()) * ng + pz, L.c < 3, L.t);
+            for_points<NP, NT>([&](auto ic, int e, int x, int y) {
+                constexpr int i = decltype(ic)::value;
+                double xd[3];
+                pp.at(g, x, y, NP, xd);
+                const double us = fb6 * (double)(sp == 0 ? u[i].re : u[i].im);
+                double2 a = sp == 0 ? up2(yv[i]) : acc[i];
+                for (int d = 0; d < 3; ++d) {
+                    const double2 v = up2(S.t[d].a[y][x]);
+                    a.x += v.x * (xd[d] * us);
+                    a.y += v.y * (xd[d] * us);
+                }
+                if constexpr (sp == 0)
+                    acc[i] = a;
+                else
+                    S.t[0].a[y][x] = down2<T>(a);
+            });
+        });
+    }
     __syncthreads();
     xy_inverse_store<T, NP>(S.t[0], Y + (size_t)p * ng + pz, L.c == 0, L.t);
 }
@@ -499,7 +528,7 @@ void launch_xy_a7c2(int prec, const Geom& g, const void* P, void* Y, const PairE
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
 #define TCX_L(NP)                                                                                              \
-    hipLaunchKernelGGL((k_xy_a7c2<T, NP>), dim3(g.n3, n), (xy_block<NP, kTilesA7c2>()), 0, s,                  \
+    hipLaunchKernelGGL((k_xy_a7c2<T, NP>), dim3(g.n3, n), (xy_block<NP, tiles_a7c2<T, NP>()>()), 0, s,                  \
                        (const cpx<T>*)((const char*)P + (size_t)a * 6 * gb), (cpx<T>*)((char*)Y + (size_t)a * gb), \
                        pairs + a, (const cpx<T>*)tab, kq0, kqcart, g.n3, g, fb6);
         TCX_XY_DISPATCH(prec, g, TCX_L);
