@@ -103,15 +103,31 @@ __global__ __launch_bounds__(kPencilBlock) void k_za7_mid(cpx<T>* __restrict__ P
         } else {
             load_pencil<T, M>(v, g + L.xy + (size_t)L.h * nxy, S * nxy);  // z = S m + h
         }
+        const double2* n = dens + s * dstride + o0;  // dstride 0: one independent spin
+        // FP32: the orbital and density points of the step are loaded before the transform,
+        // so that their latency overlaps it (FP64 loads them where they are used)
+        double2 dnv[kPrefetch ? M : 1];
+        cpx<T> phv[kPrefetch ? M : 1];
+        if constexpr (kPrefetch)
+            static_for<0, M>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                dnv[k] = n[(size_t)k * nxy];
+                if (s == 0) phv[k] = ph[(size_t)k * nxy];
+            });
         // the exchanges of both transforms use lane swaps here (3.5% faster than ds_bpermute)
         PencilSplit<S>::template fft<T, NZ, 1, true>(v, L.h);  // -> z = ob M + k
         {
-            const double2* n = dens + s * dstride + o0;  // dstride 0: one independent spin
             static_for<0, M>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
                 const double2 x = upd(v[k]);
-                if (s == 0) acc[k] = cmadd(acc[k], x, upd(ph[(size_t)k * nxy]), 1.0);
-                const double2 dn = n[(size_t)k * nxy];
+                double2 dn;
+                if constexpr (kPrefetch) {
+                    if (s == 0) acc[k] = cmadd(acc[k], x, upd(phv[k]), 1.0);
+                    dn = dnv[k];
+                } else {
+                    if (s == 0) acc[k] = cmadd(acc[k], x, upd(ph[(size_t)k * nxy]), 1.0);
+                    dn = n[(size_t)k * nxy];
+                }
                 xs[pi][L.ob * M + k] = cpx<T>{(T)(x.x * dn.x - x.y * dn.y), (T)(x.x * dn.y + x.y * dn.x)};
             });
             __syncthreads();
@@ -247,10 +263,22 @@ __global__ __launch_bounds__(kPencilBlock) void k_za8_q45(const cpx<T>* __restri
         } else {
             load_pencil<T, M>(v, grid_of(pp), S * nxy);  // z = S m + h
         }
+        // FP32 up to 40^3: the orbital points are loaded before the transform, so that their
+        // latency overlaps it (at 48^3 the registers would halve the waves per SIMD; FP64
+        // loads them where they are used)
+        constexpr bool kHoist = kPrefetch && NZ <= 40;
+        cpx<T> xv[kHoist ? M : 1];
+        if constexpr (kHoist)
+            static_for<0, M>([&](auto kc) { xv[decltype(kc)::value] = xg[(size_t)decltype(kc)::value * nxy]; });
         PencilSplit<S>::template fft<T, NZ, 1>(v, L.h);  // -> z = ob M + k
         static_for<0, M>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            const double2 a = upd(v[k]), b = upd(xg[(size_t)k * nxy]);
+            const double2 a = upd(v[k]);
+            double2 b;
+            if constexpr (kHoist)
+                b = upd(xv[k]);
+            else
+                b = upd(xg[(size_t)k * nxy]);
             acc[k].x += (sy * f1) * (a.x * b.x - a.y * b.y);
             acc[k].y += f1 * (a.x * b.y + a.y * b.x);
         });
