@@ -80,18 +80,32 @@ void for_slices(long long n, F&& f)
 
 // ----------------------------------------------------------------- products
 // Products fused with the forward z pass: one thread owns pencil (x, y) of one record
-// channel, forms A_c * B_c along z from the two orbital grids (shared by many records, so
-// mostly cache hits), transforms along z, scales, and stores; launch_fft_xy(-1) over the
-// records completes FFT_forward.  Same records as k_products + launch_fft3d(-1).
+// channel, forms A_c * B_c along z from the two orbital grids (shared by many records),
+// transforms along z, scales, and stores; launch_fft_xy(-1) over the records completes
+// FFT_forward.  Same records as k_products + launch_fft3d(-1).
+//
+// Workgroup order.  Workgroups go to the 8 XCDs in turn (linear id mod 8), each with its
+// own L2.  Workgroup L takes XCD r = L mod 8, channel c = (L / 8) mod nchan and the
+// (pencil group, pair) item u = 8 ((L / 8) / nchan) + r, pencil-group-major over the
+// launch's target-major pairs.  So the channels of one pair and pencil group run back to
+// back on one XCD (the source orbital A is read once into its L2), and an XCD takes every
+// 8th pair of a pencil group (a target's B grids, and with 8 | sources the source's A
+// across targets, are re-read from the same L2).
+constexpr int kXcds = 8;
+
 template <typename T, int NZ>
-__global__ __launch_bounds__(kPencilBlock) void k_products_z(const PairEntry* __restrict__ pairs,
-                                                             const ProdChan* __restrict__ chans, int nchan, int rec0,
-                                                             cpx<T>* __restrict__ out, int nxy, T sc)
+__global__ __launch_bounds__(kPencilBlock) void k_products_z(const PairEntry* __restrict__ pairs, int npairs,
+                                                             const ProdChan* __restrict__ chans, int nchan,
+                                                             int ngroups, cpx<T>* __restrict__ out, int nxy, T sc)
 {
-    const int xy = blockIdx.x * kPencilBlock + threadIdx.x;
+    const unsigned lin = blockIdx.x, r = lin % kXcds, rest = lin / kXcds;
+    const int c = (int)(rest % (unsigned)nchan);
+    const size_t u = (size_t)(rest / (unsigned)nchan) * kXcds + r;
+    if (u >= (size_t)ngroups * npairs) return;
+    const int grp = (int)(u / (size_t)npairs), p = (int)(u % (size_t)npairs);
+    const int xy = grp * kPencilBlock + threadIdx.x;
     if (xy >= nxy) return;
-    const int rec = rec0 + (int)blockIdx.y;
-    const int p = rec / nchan, c = rec - p * nchan;
+    const int rec = p * nchan + c;
     const PairEntry e = pairs[p];
     const ProdChan ch = chans[c];
     const size_t ng = (size_t)nxy * NZ;
@@ -117,15 +131,17 @@ static void products_fft_impl(const PairEntry* pairs, int npairs, const ProdChan
                               const Geom& g, T sc, hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
-    for_slices((long long)npairs * nchan, [&](int r0, unsigned ny) {
-        const dim3 grid((nxy + kPencilBlock - 1) / kPencilBlock, ny);
-#define TCX_PZ(NZ)                                                                                       \
-    hipLaunchKernelGGL((k_products_z<T, NZ>), grid, dim3(kPencilBlock), 0, s, pairs, chans, nchan, r0, \
+    const int ngroups = (nxy + kPencilBlock - 1) / kPencilBlock;
+    const size_t items = (size_t)ngroups * npairs;
+    const size_t nwg = (items + kXcds - 1) / kXcds * kXcds * nchan;
+    if (nwg > 0x7fffffffu) throw Error(TCX_ERR_UNSUPPORTED, "products pass: too many workgroups in one launch");
+    const dim3 grid((unsigned)nwg);
+#define TCX_PZ(NZ)                                                                                                \
+    hipLaunchKernelGGL((k_products_z<T, NZ>), grid, dim3(kPencilBlock), 0, s, pairs, npairs, chans, nchan, ngroups, \
                        (cpx<T>*)out, nxy, sc);
-        TCX_SIZE_SWITCH(g.n3, TCX_PZ)
+    TCX_SIZE_SWITCH(g.n3, TCX_PZ)
 #undef TCX_PZ
-        TCX_HIP_CHECK(hipGetLastError());
-    });
+    TCX_HIP_CHECK(hipGetLastError());
 }
 
 void launch_products_z(int prec, const PairEntry* pairs, int npairs, const ProdChan* chans, int nchan, void* out,
