@@ -116,7 +116,7 @@ def apply_tuned(eng, sys_, nb, term, precision, path=TABLE) -> tuple | None:
 
 def main(argv=None):
     """Tune the benchmark workloads and merge the winners into the table:
-    python -m tc_amd.autotune --workloads c2,c3 [--out PATH]."""
+    python -m tc_amd.autotune --workloads fcc_bitc_40,batched_uneven [--out PATH]."""
     import argparse
     import sys
 
@@ -124,7 +124,7 @@ def main(argv=None):
     from tc_amd import tcx
 
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workloads", default="c2,c3")
+    ap.add_argument("--workloads", default="fcc_bitc_40,batched_uneven")
     ap.add_argument("--out", default=TABLE)
     ap.add_argument("--samples", type=int, default=SAMPLES)
     args = ap.parse_args(argv)
@@ -154,10 +154,10 @@ def main(argv=None):
         d_out.free()
 
     for w in args.workloads.split(","):
-        if w == "c2":   # BITC fcc 40^3, 4x4x4 k, 16 bands: the two-body and three-body terms
+        if w == "fcc_bitc_40":  # BITC fcc 40^3, 4x4x4 k, 16 bands: the two-body and three-body terms
             run(S.fcc_bitc_40(), [("tc2x", tcx.PREC_FP32), ("tc2x", tcx.PREC_FP64),
                                   ("tc3a2a4b2b5", tcx.PREC_FP32), ("tc3a3b3b4b6", tcx.PREC_FP32)])
-        elif w == "c3":  # the batched workload's cells at 32^3, 36^3 and 48^3
+        elif w == "batched_uneven":  # the batched workload's cells at 32^3, 36^3 and 48^3
             for sys_ in S.batched_uneven_systems():
                 run(sys_, [("hf2x", tcx.PREC_FP32), ("tc2x", tcx.PREC_FP32)] if sys_.method != "HF"
                     else [("hf2x", tcx.PREC_FP32)])
