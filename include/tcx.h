@@ -50,7 +50,8 @@ typedef enum { TCX_METHOD_HF = 0, TCX_METHOD_TC = 1, TCX_METHOD_BITC = 2 } tcx_m
 /* Arithmetic of the pair pipeline.  FP64: every FFT and product in double.
  * FP32: FFTs and pointwise products in float, band-pair sums accumulated in
  * double, orbital transforms, final transforms, divergence corrections and
- * all outputs in double. */
+ * all outputs in double.  Both: the band-pair sums of a launch are added into
+ * the accumulators with compensation (TwoSum) across launches. */
 typedef enum { TCX_PREC_FP64 = 0, TCX_PREC_FP32 = 1 } tcx_precision;
 
 /* flags of the term entry points */

@@ -34,6 +34,36 @@ constexpr int kPlaneBlock = 64;   // threads per xy-plane workgroup (plane side 
 
 __device__ __forceinline__ int centered(int i, int n) { return i <= n / 2 ? i : i - n; }
 
+// w += a; returns the rounding error of that addition (Knuth's TwoSum: exact in binary
+// floating point, so the code must not be reassociated; it has no products to contract).
+__device__ __forceinline__ double two_sum_add(double& w, double a)
+{
+    const double s = w + a;
+    const double ap = s - w;
+    const double e = (w - (s - ap)) + (a - ap);
+    w = s;
+    return e;
+}
+
+// Adds the partial band-pair sum (are, aim) of a launch into the FP64 accumulator point w.
+// With a compensation grid (wc non-null) the rounding errors of these additions are summed
+// into wc, so that w + wc carries the sum with about one rounding of the result instead of
+// one per launch.  The errors are below 1.2e-16 |w| each, so their sum is kept in FP32: its
+// own rounding is 1e-7 of a correction of order 1e-16 |w|, at half the traffic of an FP64 grid.
+__device__ __forceinline__ void add_partial(double2* w, float2* wc, double are, double aim)
+{
+    double2 a = *w;
+    if (wc) {
+        const double ex = two_sum_add(a.x, are), ey = two_sum_add(a.y, aim);
+        const float2 c = *wc;
+        *wc = make_float2((float)((double)c.x + ex), (float)((double)c.y + ey));
+    } else {
+        a.x += are;
+        a.y += aim;
+    }
+    *w = a;
+}
+
 
 template <typename T, int NZ>
 __device__ __forceinline__ void load_pencil(cpx<T> (&v)[NZ], const cpx<T>* __restrict__ p, int stride)

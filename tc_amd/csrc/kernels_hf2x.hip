@@ -173,7 +173,8 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_C(const cpx<T>* __restric
                                                          const PairEntry* __restrict__ pairs,
                                                          const TargetEntry* __restrict__ targets,
                                                          const double* __restrict__ fill,
-                                                         cpx<double>* __restrict__ W, int nxy)
+                                                         cpx<double>* __restrict__ W, float2* __restrict__ Wc,
+                                                         int nxy)
 {
     constexpr int S = SplitOf<T, NZ>::value;
     using Split = PencilSplit<S>;
@@ -213,10 +214,11 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_C(const cpx<T>* __restric
         });
     }
     if (!valid) return;
-    cpx<double>* w = W + (size_t)te.app * ng + xy + (size_t)ob * M * nxy;
+    const size_t o = (size_t)te.app * ng + xy + (size_t)ob * M * nxy;
     static_for<0, M>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        w[(size_t)k * nxy] = w[(size_t)k * nxy] + acc[k];
+        add_partial((double2*)(W + o + (size_t)k * nxy), Wc ? Wc + o + (size_t)k * nxy : nullptr, acc[k].re,
+                    acc[k].im);
     });
 }
 
@@ -289,15 +291,17 @@ void launch_hf2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, i
 
 template <typename T>
 static void hf2x_C_impl(const Geom& g, const void* buf, const void* occ, const PairEntry* pairs,
-                        const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s)
+                        const TargetEntry* targets, int ntargets, const double* fill, double* W, float* Wc,
+                        hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
     // 64/S split pencils per 64-lane workgroup
-#define TCX_C(NZ)                                                                                             \
-    {                                                                                                         \
-        constexpr int P = 64 / SplitOf<T, NZ>::value;                                                         \
-        hipLaunchKernelGGL((k_hf2x_C<T, NZ>), dim3((nxy + P - 1) / P, ntargets), dim3(kPencilBlock), 0, s,    \
-                           (const cpx<T>*)buf, (const cpx<T>*)occ, pairs, targets, fill, (cpx<double>*)W, nxy); \
+#define TCX_C(NZ)                                                                                          \
+    {                                                                                                      \
+        constexpr int P = 64 / SplitOf<T, NZ>::value;                                                      \
+        hipLaunchKernelGGL((k_hf2x_C<T, NZ>), dim3((nxy + P - 1) / P, ntargets), dim3(kPencilBlock), 0, s, \
+                           (const cpx<T>*)buf, (const cpx<T>*)occ, pairs, targets, fill, (cpx<double>*)W,  \
+                           (float2*)Wc, nxy);                                                              \
     }
     TCX_SIZE_SWITCH(g.n3, TCX_C)
 #undef TCX_C
@@ -305,14 +309,14 @@ static void hf2x_C_impl(const Geom& g, const void* buf, const void* occ, const P
 }
 
 void launch_hf2x_C(int prec, const Geom& g, const void* buf, const void* occ, const PairEntry* pairs,
-                   const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s)
+                   const TargetEntry* targets, int ntargets, const double* fill, double* W, float* Wc, hipStream_t s)
 {
     if (ntargets == 0) return;
     if (ntargets > 65535) throw Error(TCX_ERR_INTERNAL, "hf2x: too many targets in one launch");
     if (prec == TCX_PREC_FP64)
-        hf2x_C_impl<double>(g, buf, occ, pairs, targets, ntargets, fill, W, s);
+        hf2x_C_impl<double>(g, buf, occ, pairs, targets, ntargets, fill, W, Wc, s);
     else
-        hf2x_C_impl<float>(g, buf, occ, pairs, targets, ntargets, fill, W, s);
+        hf2x_C_impl<float>(g, buf, occ, pairs, targets, ntargets, fill, W, Wc, s);
 }
 
 }  // namespace tcx

@@ -193,7 +193,7 @@ template <typename T, int NP>
 __global__ __launch_bounds__((PlaneGroupC<T, NP>::threads), 3) void k_hf2x_xyC(
     const cpx<T>* __restrict__ rec, const cpx<T>* __restrict__ phiR_occ, const PairEntry* __restrict__ pairs,
     const TargetEntry* __restrict__ targets, const int4* __restrict__ rowinfo, const double* __restrict__ fill,
-    cpx<double>* __restrict__ W, int cstride, int nz)
+    cpx<double>* __restrict__ W, float2* __restrict__ Wc, int cstride, int nz)
 {
     using PG = PlaneGroupC<T, NP>;
     constexpr int PL = PG::planes, S = PG::split, LW = PG::lines_per_wave, M = NP / S;
@@ -265,10 +265,11 @@ __global__ __launch_bounds__((PlaneGroupC<T, NP>::threads), 3) void k_hf2x_xyC(
             });
     }
     if (!active) return;
-    cpx<double>* o = W + (size_t)te.app * ng + (size_t)z * NP * NP + (size_t)ob * M * NP + line;
+    const size_t o = (size_t)te.app * ng + (size_t)z * NP * NP + (size_t)ob * M * NP + line;
     static_for<0, M>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        o[(size_t)k * NP] = o[(size_t)k * NP] + acc[k];
+        add_partial((double2*)(W + o + (size_t)k * NP), Wc ? Wc + o + (size_t)k * NP : nullptr, acc[k].re,
+                    acc[k].im);
     });
 }
 
@@ -306,7 +307,7 @@ void launch_hf2x_zB(const Geom& g, void* rec, const PairEntry* pairs, int npairs
 
 void launch_hf2x_xyC(const Geom& g, const void* rec, const void* occ, const PairEntry* pairs,
                      const TargetEntry* targets, int ntargets, const int* rowinfo, const double* fill, double* W,
-                     int cstride, hipStream_t s)
+                     float* Wc, int cstride, hipStream_t s)
 {
     if (ntargets == 0) return;
     if (ntargets > 65535) throw Error(TCX_ERR_INTERNAL, "hf2x: too many targets in one launch");
@@ -315,7 +316,7 @@ void launch_hf2x_xyC(const Geom& g, const void* rec, const void* occ, const Pair
         using PG = PlaneGroupC<float, NP>;                                                                       \
         hipLaunchKernelGGL((k_hf2x_xyC<float, NP>), dim3((g.n3 + PG::planes - 1) / PG::planes, ntargets),        \
                            dim3(PG::threads), 0, s, (const cpx<float>*)rec, (const cpx<float>*)occ, pairs, targets, \
-                           (const int4*)rowinfo, fill, (cpx<double>*)W, cstride, g.n3);                           \
+                           (const int4*)rowinfo, fill, (cpx<double>*)W, (float2*)Wc, cstride, g.n3);              \
     }
     TCX_SIZE_SWITCH(g.n1, TCX_XC)
 #undef TCX_XC

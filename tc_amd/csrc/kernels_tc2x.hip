@@ -187,7 +187,8 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_C(const cpx<T>* __restric
                                                          const PairEntry* __restrict__ pairs,
                                                          const TargetEntry* __restrict__ targets,
                                                          const double* __restrict__ fill,
-                                                         cpx<double>* __restrict__ W, int nxy)
+                                                         cpx<double>* __restrict__ W, float2* __restrict__ Wc,
+                                                         int nxy)
 {
     constexpr int S = SplitOf<T, NZ>::value;  // pencil split over S lanes, see PencilSplit
     using Split = PencilSplit<S>;
@@ -235,10 +236,11 @@ __global__ __launch_bounds__(kPencilBlock) void k_tc2x_C(const cpx<T>* __restric
         }
     }
     if (!valid) return;
-    cpx<double>* w = W + (size_t)te.app * ng + xy + (size_t)ob * M * nxy;
+    const size_t o = (size_t)te.app * ng + xy + (size_t)ob * M * nxy;
     static_for<0, M>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        w[(size_t)k * nxy] = w[(size_t)k * nxy] + acc[k];
+        add_partial((double2*)(W + o + (size_t)k * nxy), Wc ? Wc + o + (size_t)k * nxy : nullptr, acc[k].re,
+                    acc[k].im);
     });
 }
 
@@ -316,7 +318,8 @@ void launch_tc2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, i
 
 template <typename T>
 static void tc2x_C_impl(const Geom& g, const void* buf, const void* occR, const void* gradR, const PairEntry* pairs,
-                        const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s)
+                        const TargetEntry* targets, int ntargets, const double* fill, double* W, float* Wc,
+                        hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
 #define TCX_C(NZ)                                                                                           \
@@ -324,7 +327,7 @@ static void tc2x_C_impl(const Geom& g, const void* buf, const void* occR, const 
         constexpr int P = 64 / SplitOf<T, NZ>::value;                                                       \
         hipLaunchKernelGGL((k_tc2x_C<T, NZ>), dim3((nxy + P - 1) / P, ntargets), dim3(kPencilBlock), 0, s,  \
                            (const cpx<T>*)buf, (const cpx<T>*)occR, (const cpx<T>*)gradR, pairs, targets, fill, \
-                           (cpx<double>*)W, nxy);                                                           \
+                           (cpx<double>*)W, (float2*)Wc, nxy);                                              \
     }
     TCX_SIZE_SWITCH(g.n3, TCX_C)
 #undef TCX_C
@@ -333,14 +336,14 @@ static void tc2x_C_impl(const Geom& g, const void* buf, const void* occR, const 
 
 void launch_tc2x_C(int prec, const Geom& g, const void* buf, const void* occR, const void* gradR,
                    const PairEntry* pairs, const TargetEntry* targets, int ntargets, const double* fill, double* W,
-                   hipStream_t s)
+                   float* Wc, hipStream_t s)
 {
     if (ntargets == 0) return;
     if (ntargets > 65535) throw Error(TCX_ERR_INTERNAL, "tc2x: too many targets in one launch");
     if (prec == TCX_PREC_FP64)
-        tc2x_C_impl<double>(g, buf, occR, gradR, pairs, targets, ntargets, fill, W, s);
+        tc2x_C_impl<double>(g, buf, occR, gradR, pairs, targets, ntargets, fill, W, Wc, s);
     else
-        tc2x_C_impl<float>(g, buf, occR, gradR, pairs, targets, ntargets, fill, W, s);
+        tc2x_C_impl<float>(g, buf, occR, gradR, pairs, targets, ntargets, fill, W, Wc, s);
 }
 
 }  // namespace tcx

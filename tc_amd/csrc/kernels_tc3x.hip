@@ -667,6 +667,22 @@ void launch_axpby(const double* X, const double* Y, double* out, double a, doubl
     TCX_HIP_CHECK(hipGetLastError());
 }
 
+// W += C (W FP64, C FP32 complex grids)
+__global__ __launch_bounds__(256) void k_add_into(double2* __restrict__ W, const float2* __restrict__ C, long long N)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= N) return;
+    const float2 c = C[r];
+    W[r] = dadd(W[r], make_double2((double)c.x, (double)c.y));
+}
+
+void launch_add_into(double* W, const float* C, long long N, hipStream_t s)
+{
+    if (N == 0) return;
+    hipLaunchKernelGGL(k_add_into, grid_pts(N, 1), dim3(256), 0, s, (double2*)W, (const float2*)C, N);
+    TCX_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace tcx
 
 namespace tcx {

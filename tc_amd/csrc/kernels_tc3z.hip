@@ -299,7 +299,8 @@ __global__ __launch_bounds__(kPencilBlock) void k_za8_apply(const cpx<T>* __rest
                                                             const cpx<T>* __restrict__ Q4T,
                                                             const PairEntry* __restrict__ pairs,
                                                             const TargetEntry* __restrict__ targets,
-                                                            double2* __restrict__ W, int nxy)
+                                                            double2* __restrict__ W, float2* __restrict__ Wc,
+                                                            int nxy)
 {
     constexpr int M = NZ / S;
     const LaneOf<S> L(nxy);
@@ -336,11 +337,10 @@ __global__ __launch_bounds__(kPencilBlock) void k_za8_apply(const cpx<T>* __rest
         }
     }
     if (!L.valid) return;
-    double2* w = W + (size_t)te.app * ng + o0;
+    const size_t o = (size_t)te.app * ng + o0;
     static_for<0, M>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const double2 a = w[(size_t)k * nxy];
-        w[(size_t)k * nxy] = make_double2(a.x + acc[k].x, a.y + acc[k].y);
+        add_partial(W + o + (size_t)k * nxy, Wc ? Wc + o + (size_t)k * nxy : nullptr, acc[k].x, acc[k].y);
     });
 }
 
@@ -416,7 +416,8 @@ void launch_za8_q(int prec, const Geom& g, const void* O, int bitc, const void* 
 }
 
 void launch_za8_apply(int prec, const Geom& g, const void* Z, const void* occR, const void* Q4T,
-                      const PairEntry* pairs, const TargetEntry* targets, int ntargets, double* W, hipStream_t s)
+                      const PairEntry* pairs, const TargetEntry* targets, int ntargets, double* W, float* Wc,
+                      hipStream_t s)
 {
     const int nxy = g.n1 * g.n2;
     for_target_slices(ntargets, [&](int a, int n) {
@@ -426,7 +427,7 @@ void launch_za8_apply(int prec, const Geom& g, const void* Z, const void* occR, 
         constexpr int PN = PencilSplit<S>::pencils;                                                            \
         hipLaunchKernelGGL((k_za8_apply<T, NZ, S>), dim3((nxy + PN - 1) / PN, n), dim3(kPencilBlock), 0, s,   \
                            (const cpx<T>*)Z, (const cpx<T>*)occR, (const cpx<T>*)Q4T, pairs, targets + a,      \
-                           (double2*)W, nxy);                                                                  \
+                           (double2*)W, (float2*)Wc, nxy);                                                     \
     }
         TCX_Z_DISPATCH(prec, g, TCX_L);
 #undef TCX_L

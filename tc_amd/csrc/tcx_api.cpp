@@ -727,9 +727,22 @@ void Context::apply_divcorr(const Plan& P, const ApplyIO& io, const double* occ_
                    (const double*)d_fill.p, (const double*)d_dc.p, io.out, stream);
 }
 
-// W (FP64 real-space accumulators, one grid per applied band) -> FFT_forward -> gather with factor
+// W (FP64 real-space accumulators, one grid per applied band) and Wc (FP32: the summed
+// rounding errors of the band-pair sums added into W across launches, see add_partial)
+double* Context::begin_W(int n_app)
+{
+    const size_t bytes = (size_t)n_app * N * 16;
+    d_W.ensure(bytes);
+    d_Wc.ensure(bytes / 2);
+    launch_zero(d_W.p, bytes, stream);
+    launch_zero(d_Wc.p, bytes / 2, stream);
+    return (double*)d_W.p;
+}
+
+// W + Wc -> FFT_forward -> gather with factor
 void Context::finish_W(const Plan& P, const ApplyIO& io, double factor)
 {
+    launch_add_into((double*)d_W.p, (const float*)d_Wc.p, (long long)P.n_app * N, stream);
     timed(3, 0, [&] { launch_fft3d(TCX_PREC_FP64, -1, d_W.p, P.n_app, geo, 1.0 / (double)N, stream); });
     launch_gather_add((const double*)d_W.p, (const int*)d_gindex.p, (const OrbEntry*)P.d_appjobs.p, P.n_app,
                       max_npw, io.out, N, factor, stream);
@@ -746,8 +759,7 @@ void Context::hf2x(const int* nb_, const double* phi, double* out, int prec, uns
         const size_t es = prec == TCX_PREC_FP64 ? 16 : 8;
         d_appR.ensure((size_t)P.n_app * N * es);
         timed(3, 0, [&] { orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p); });
-        d_W.ensure((size_t)P.n_app * N * 16);
-        launch_zero(d_W.p, (size_t)P.n_app * N * 16, stream);
+        begin_W(P.n_app);
         if (use_plane_first(prec)) {
             hf2x_plane_first(P);
             double factor = -4.0 * kPiHost / (num_kpoints * volume);  // two_body_factor, hf2x.cpp:29-30
@@ -774,7 +786,7 @@ void Context::hf2x(const int* nb_, const double* phi, double* out, int prec, uns
             });
             timed(2, L.npairs, [&] {
                 launch_hf2x_C(prec, geo, d_pairbuf.p, d_occR[prec].p, pairs, targets, L.ntargets,
-                              (const double*)d_fill.p, (double*)d_W.p, stream);
+                              (const double*)d_fill.p, (double*)d_W.p, (float*)d_Wc.p, stream);
             });
         }
         // two_body_factor, src/calc_hamiltonian_hf2x.cpp:29-30

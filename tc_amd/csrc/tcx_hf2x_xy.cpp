@@ -130,7 +130,7 @@ void Context::hf2x_plane_first(const Plan& P)
         });
         timed(2, L.npairs, [&] {
             launch_hf2x_xyC(geo, d_pairbuf.p, d_occR[TCX_PREC_FP32].p, pairs, targets, L.ntargets, rowinfo,
-                            (const double*)d_fill.p, (double*)d_W.p, cs, stream);
+                            (const double*)d_fill.p, (double*)d_W.p, (float*)d_Wc.p, cs, stream);
         });
     }
 }

@@ -129,15 +129,19 @@ void launch_hf2x_zB(const Geom& g, void* rec, const PairEntry* pairs, int npairs
                     const double* kqcart, int cstride, hipStream_t s);
 void launch_hf2x_xyC(const Geom& g, const void* rec, const void* occ, const PairEntry* pairs,
                      const TargetEntry* targets, int ntargets, const int* rowinfo, const double* fill, double* W,
-                     int cstride, hipStream_t s);
+                     float* Wc, int cstride, hipStream_t s);
 void launch_hf2x_A(int prec, const Geom& g, const void* phiR_app, const void* phiR_occ, const PairEntry* pairs,
                    int npairs, void* buf, hipStream_t s);
 void launch_coulomb_table(int prec, void* table, const double* kqcart, int row0, int nrows, const Geom& g,
                           hipStream_t s);
 void launch_hf2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, int npairs, const void* table,
                    int kq0, hipStream_t s);
+// W[target] += the launch's band-pair sums; with Wc (complex FP32, the layout of W) non-null
+// the rounding errors of these additions are summed into Wc (W + Wc carries the sum); the
+// same for the other pair accumulators below
 void launch_hf2x_C(int prec, const Geom& g, const void* buf, const void* phiR_occ, const PairEntry* pairs,
-                   const TargetEntry* targets, int ntargets, const double* fill, double* W, hipStream_t s);
+                   const TargetEntry* targets, int ntargets, const double* fill, double* W, float* Wc,
+                   hipStream_t s);
 
 // divergence correction: hphi += sum_b dc f_b <chi_b|phi_j> phi_b
 void launch_divcorr(const double* phi, const double* occ_right, const double* occ_left, const DivEntry* jobs,
@@ -152,7 +156,7 @@ void launch_tc2x_B(int prec, const Geom& g, void* buf, const PairEntry* pairs, i
                    const double* kqcart, hipStream_t s);
 void launch_tc2x_C(int prec, const Geom& g, const void* buf, const void* occR, const void* gradR,
                    const PairEntry* pairs, const TargetEntry* targets, int ntargets, const double* fill, double* W,
-                   hipStream_t s);
+                   float* Wc, hipStream_t s);
 
 // density and local three-body potentials (FP64 grids)
 void launch_density(const double* occR, const double* chiR, const double* fill, int nocc, long long N, double* out,
@@ -196,7 +200,8 @@ void launch_za8_q(int prec, const Geom& g, const void* O, int bitc, const void* 
                   const PairEntry* pairs, const TargetEntry* targets, int ntargets, const double* fill, double* Q2,
                   double* Q4, double* Q5, hipStream_t s);
 void launch_za8_apply(int prec, const Geom& g, const void* Z, const void* occR, const void* Q4T,
-                      const PairEntry* pairs, const TargetEntry* targets, int ntargets, double* W, hipStream_t s);
+                      const PairEntry* pairs, const TargetEntry* targets, int ntargets, double* W, float* Wc,
+                      hipStream_t s);
 void launch_mul_dnu(int prec, const void* appR, const double* dnu, void* out, int napp, long long N, hipStream_t s);
 void launch_a3(double* W, const double* qsum, const double* dnu, int napp, long long N, double fa3, hipStream_t s);
 void launch_b6(const double* nG, double* tG, double* t1, const double* dcp, const int* match, int nk, const Geom& g,
@@ -233,6 +238,8 @@ void launch_tc2h_combine(const double* nG, const double* pgpG, double* V, const 
 void launch_tc2h_apply(double* orb, const double* grad, const double* V2, const double* dnu, int nb, long long N,
                        hipStream_t s);
 void launch_axpby(const double* X, const double* Y, double* out, double a, double b, long long N, hipStream_t s);
+// W += C over N complex points (W FP64, C FP32)
+void launch_add_into(double* W, const float* C, long long N, hipStream_t s);
 
 // divergence terms at zero-weight k-points (kernels_zw.hip); the states projected on are
 // addressed by DivEntry like occupied orbitals

@@ -234,7 +234,9 @@ void Context::hf2x_noalloc(const int* nb_, const double* phi, double* out, int p
             launch_coulomb_table(prec, ktab, d_kq, l.kq0, l.nkq, geo, stream);
             launch_hf2x_A(prec, geo, appR, d_occR[prec].p, lp, l.npairs, pairbuf, stream);
             launch_hf2x_B(prec, geo, pairbuf, lp, l.npairs, ktab, l.kq0, stream);
-            launch_hf2x_C(prec, geo, pairbuf, d_occR[prec].p, lp, lt, l.ntargets, (const double*)d_fill.p, W, stream);
+            // plain FP64 band-pair sums: the working array holds no compensation grids
+            launch_hf2x_C(prec, geo, pairbuf, d_occR[prec].p, lp, lt, l.ntargets, (const double*)d_fill.p, W, nullptr,
+                          stream);
         }
         launch_fft3d(TCX_PREC_FP64, -1, W, nt, geo, 1.0 / (double)N, stream);
         launch_gather_add(W, (const int*)d_gindex.p, d_jobs, nt, max_npw, out, N, factor, stream);

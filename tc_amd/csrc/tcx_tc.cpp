@@ -28,8 +28,7 @@ void Context::tc2x(const int* nb_, const double* phi, double* out, int prec, uns
             orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p);
             orbitals_to_real(io.phi, (const OrbEntry*)P.d_appgradjobs.p, 3 * P.n_app, prec, d_appGrad.p);
         });
-        d_W.ensure((size_t)P.n_app * N * 16);
-        launch_zero(d_W.p, (size_t)P.n_app * N * 16, stream);
+        begin_W(P.n_app);
         d_pairbuf.ensure((size_t)std::max(1, P.max_launch_pairs) * 4 * N * es);
         int max_rows = 1;
         for (const Launch& L : P.launches) max_rows = std::max(max_rows, L.nkq);
@@ -48,7 +47,7 @@ void Context::tc2x(const int* nb_, const double* phi, double* out, int prec, uns
             });
             timed(2, L.npairs, [&] {
                 launch_tc2x_C(prec, geo, d_pairbuf.p, d_occR[prec].p, d_gradR[prec].p, pairs, targets, L.ntargets,
-                              (const double*)d_fill.p, (double*)d_W.p, stream);
+                              (const double*)d_fill.p, (double*)d_W.p, (float*)d_Wc.p, stream);
             });
         }
         // two_body_factor, src/calc_hamiltonian_tc2x.cpp:32-33

@@ -137,8 +137,7 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
         orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p);
         launch_mul_dnu(prec, d_appR.p, dnu, d_appGrad.p, P.n_app, N, stream);
 
-        d_W.ensure((size_t)P.n_app * g64);
-        launch_zero(d_W.p, (size_t)P.n_app * g64, stream);
+        begin_W(P.n_app);
         d_qsum.ensure((size_t)3 * P.n_app * g64);
         launch_zero(d_qsum.p, (size_t)3 * P.n_app * g64, stream);
         double* W = (double*)d_W.p;
@@ -179,7 +178,7 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
                 launch_xy_a7c2(prec, geo, Pb, Y, pairs, np, d_ktab.p, L.kq0, kq, fb6, stream);
             });
             timed(2, np, [&] {  // z pass of FFT_backward fused with the accumulation
-                launch_hf2x_C(prec, geo, Y, occR, pairs, targets, L.ntargets, nullptr, W, stream);
+                launch_hf2x_C(prec, geo, Y, occR, pairs, targets, L.ntargets, nullptr, W, (float*)d_Wc.p, stream);
             });
         }
 
@@ -278,7 +277,7 @@ void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int pr
                 launch_xy_b4(prec, geo, rec3, Z, pairs, np, d_ktab.p, L.kq0, kq, fill, fb4, stream);
             });
             timed(2, np, [&] {
-                launch_hf2x_C(prec, geo, Z, occR, pairs, targets, L.ntargets, nullptr, W, stream);
+                launch_hf2x_C(prec, geo, Z, occR, pairs, targets, L.ntargets, nullptr, W, (float*)d_Wc.p, stream);
             });
         }
         // [3b4] kVaux term at a zero-weight k (:772-858), on the complete qsum
@@ -449,8 +448,7 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
         // [3a4], [3b5] applied to (k,j) (:801-913): targets (k,j) x sources (q2,i2)
         d_appR.ensure((size_t)P.n_app * gT);
         orbitals_to_real(io.phi, (const OrbEntry*)P.d_appjobs.p, P.n_app, prec, d_appR.p);
-        d_W.ensure((size_t)P.n_app * g64);
-        launch_zero(d_W.p, (size_t)P.n_app * g64, stream);
+        begin_W(P.n_app);
         {
             const PairSet& S = pair_set(nb, kPairsAppOcc, 8, prec);  // 4 input + 4 output channels
             d_rec.ensure((size_t)std::max(1, S.max_launch_pairs) * 8 * gT);
@@ -477,7 +475,8 @@ void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int pr
                                    fb5, stream);
                 });
                 timed(2, np, [&] {
-                    launch_za8_apply(prec, geo, Zo, occR, d_Q4T.p, pairs, targets, L.ntargets, (double*)d_W.p, stream);
+                    launch_za8_apply(prec, geo, Zo, occR, d_Q4T.p, pairs, targets, L.ntargets, (double*)d_W.p,
+                                     (float*)d_Wc.p, stream);
                 });
             }
         }

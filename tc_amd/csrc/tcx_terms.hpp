@@ -160,6 +160,7 @@ struct Context {
     DBuf d_dc;       // divergence factor per k
     DBuf d_symjobs;  // SymRotJob per occupied orbital at a replica k
     DBuf d_phi_app, d_out, d_appR, d_appGrad, d_tmp64, d_W, d_pairbuf, d_ktab, d_work;
+    DBuf d_Wc;  // compensation grids of the band-pair sums in d_W
 
     // launch shape per term (0 hf2x, 1 tc2x): targets x sources per launch, 0 = auto
     long shape_targets[3] = {0, 0, 0};  // hf2x, tc2x, three-body pair sets (0 = automatic)
@@ -301,6 +302,8 @@ struct Context {
     void end_apply(const ApplyIO& io);
     double div_factor(int k) const;
     void apply_divcorr(const Plan& P, const ApplyIO& io, const double* occ_left);
+    // clears W and its compensation grids for n_app applied bands; returns W
+    double* begin_W(int n_app);
     void finish_W(const Plan& P, const ApplyIO& io, double factor);
 
     void hf2x(const int* nb, const double* phi, double* out, int prec, unsigned flags);

@@ -227,12 +227,12 @@ void Context::a8_kvaux(const Plan& P, const ApplyIO& io, const std::vector<int>&
             launch_hf2x_A(TCX_PREC_FP64, geo, phib, chiR64, pairs, L.npairs, d_rec.p, stream);
             launch_hf2x_B(TCX_PREC_FP64, geo, d_rec.p, pairs, L.npairs, d_ktab.p, L.kq0, stream);
             launch_hf2x_C(TCX_PREC_FP64, geo, d_rec.p, occR64, pairs, targets, L.ntargets, (const double*)d_fill.p, Wa,
-                          stream);
+                          nullptr, stream);
             if (bitc) {  // [3b5]: chi_b conj(phi_q1i1) -> x du -> x f1 chi_q1i1, summed into Wb_b
                 launch_hf2x_A(TCX_PREC_FP64, geo, chib, occR64, pairs, L.npairs, d_rec.p, stream);
                 launch_hf2x_B(TCX_PREC_FP64, geo, d_rec.p, pairs, L.npairs, d_ktab.p, L.kq0, stream);
                 launch_hf2x_C(TCX_PREC_FP64, geo, d_rec.p, chiR64, pairs, targets, L.ntargets,
-                              (const double*)d_fill.p, Wb, stream);
+                              (const double*)d_fill.p, Wb, nullptr, stream);
             }
         }
         launch_fft3d(TCX_PREC_FP64, -1, Wa, ns, geo, 1.0 / (double)N, stream);
