@@ -271,3 +271,22 @@ def test_two_body_adjoint_is_minus_u_with_swapped_orbitals(method):
     Ha = oa.tc2x(nb, a) + oa.tc2h(nb, a)
     lhs, rhs = np.vdot(a, Hb), np.conj(np.vdot(b, Ha))
     assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_hf2x_batched_restatement(golden):
+    """The CPU baseline's batched SciPy restatement of hf2x (oracle/hf2x_batched.py) against
+    the golden fixtures held without the divergence correction and against orc_hf2x, on
+    ragged band counts (an empty k included) and two grids."""
+    from oracle import hf2x_batched as HB
+    for name in [n for n in golden_names() if n.startswith("hf_")]:
+        z = golden[name]
+        sys_ = MG.load_system(z)
+        got = HB.hf2x(sys_, z["nb"], z["phi"], workers=2)
+        ref = z["hf2x_nodiv"]
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), name
+    for n, km, nb in (((8, 8, 8), (2, 2, 1), [3, 0, 2, 4]), ((12, 12, 12), (2, 1, 1), [1, 4])):
+        sys_ = S.tiny_system(method="HF", n=n, kmesh=km, nbands=4, nocc=3, div_corr=False)
+        nb, phi = S.applied_bands(sys_, nb=nb, seed=5)
+        ref = O.Oracle(sys_).hf2x(nb, phi)
+        got = HB.hf2x(sys_, nb, phi, workers=3)
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), n
