@@ -68,7 +68,18 @@ int tcx_synchronize(tcx_context *ctx);
  * CrystalStructure (reciprocal_vectors, unit_cell_volume:
  * src/include/crystal_structure.hpp), PlaneWaveBasis::setup_FFT
  * (src/plane_wave_basis.cpp:175-198), Spin::num_independent_spins and
- * Method::calc_method.  nspin must be 1 (spin-unpolarised) in this build. */
+ * Method::calc_method.
+ *
+ * nspin = 1 or 2 (num_independent_spins).  With 2, every per-k array of the
+ * states and of the term calls (nocc, filling, phi_occ, chi_occ, the applied
+ * band counts nb and the coefficient arrays) runs over (spin, k), spin-major:
+ * 2 nk entries, spin 0's k-blocks first.  The k-points, the basis and
+ * sum_of_vaux are given once for the nk k-points.  kinetic, hf2h (the sum of
+ * the two spin densities), hf2x and tc2x (states of one spin paired, u(s, s),
+ * no spin-sum halving of the prefactors) and the HF all / Davidson run with
+ * two spins; tc2h, the three-body terms, the k-mesh symmetry, zero-weight
+ * k-points, the gradients and tcx_hf2x_noalloc return TCX_ERR_UNSUPPORTED.
+ * Changing nspin clears the k-points: call tcx_set_kpoints again. */
 int tcx_set_cell(tcx_context *ctx, const double recip_rows[9], double volume,
                  const int grid[3], int nspin, int method);
 

@@ -16,7 +16,7 @@ namespace tcx {
 
 __global__ __launch_bounds__(256) void k_density(const double2* __restrict__ occR, const double2* __restrict__ chiR,
                                                  const double* __restrict__ fill, int nocc, long long N,
-                                                 double2* __restrict__ out)
+                                                 double2* __restrict__ out, int nspin)
 {
     const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
     if (r >= N) return;
@@ -31,6 +31,10 @@ __global__ __launch_bounds__(256) void k_density(const double2* __restrict__ occ
         } else {
             re += f * (p.x * p.x + p.y * p.y);
         }
+    }
+    if (nspin == 2) {  // one spin channel, fillings up to 1
+        out[r] = make_double2(re, im);
+        return;
     }
     // one independent spin: the spin factor 2 is in the filling (filling_density.cpp:343, :356-358)
     out[r] = make_double2(0.5 * re, 0.5 * im);
@@ -111,10 +115,10 @@ __global__ __launch_bounds__(256) void k_mul_shared(double2* __restrict__ grids,
 static dim3 grid1(long long N, unsigned y = 1) { return dim3((unsigned)((N + 255) / 256), y); }
 
 void launch_density(const double* occR, const double* chiR, const double* fill, int nocc, long long N, double* out,
-                    hipStream_t s)
+                    int nspin, hipStream_t s)
 {
     hipLaunchKernelGGL(k_density, grid1(N), dim3(256), 0, s, (const double2*)occR, (const double2*)chiR, fill, nocc,
-                       N, (double2*)out);
+                       N, (double2*)out, nspin);
     TCX_HIP_CHECK(hipGetLastError());
 }
 

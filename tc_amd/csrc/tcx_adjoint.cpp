@@ -71,6 +71,7 @@ void Context::swap_left_right()
 void Context::grad_x(double* grad, double* energy, int prec)
 {
     require(states_set, TCX_ERR_STATE, "grad_x: call tcx_set_states first");
+    require_one_spin("grad_exchange");
     require(grad && energy, TCX_ERR_ARG, "grad_x: null pointer");
     const long long ncoef = occ_coef[nk];
     std::vector<cd> phi(ncoef), h(ncoef, cd(0.0));
@@ -93,6 +94,7 @@ void Context::grad_x(double* grad, double* energy, int prec)
 void Context::grad_tc2(double* grad_left, double* grad_right, double* energy, double* dE_dA, int prec)
 {
     require(states_set, TCX_ERR_STATE, "grad_tc2: call tcx_set_states first");
+    require_one_spin("grad_tc2");
     require(method != TCX_METHOD_HF && jastrow_set, TCX_ERR_STATE, "grad_tc2: needs a TC/BITC cell and a Jastrow");
     require(grad_left && grad_right && energy && dE_dA, TCX_ERR_ARG, "grad_tc2: null pointer");
     const bool bitc = method == TCX_METHOD_BITC;
@@ -176,6 +178,7 @@ void Context::grad_tc2(double* grad_left, double* grad_right, double* energy, do
 void Context::kernel_grad(int term, int nbins, double width, double* dE_dK, double* dE_du, double* sums, int prec)
 {
     require(states_set, TCX_ERR_STATE, "kernel_grad: call tcx_set_states first");
+    require_one_spin("kernel_gradient");
     require(term == 0 || term == 1, TCX_ERR_ARG, "kernel_grad: term is 0 (exchange) or 1 (TC two-body)");
     require(dE_dK && sums && (term == 0 || dE_du), TCX_ERR_ARG, "kernel_grad: null pointer");
     require(nbins >= 1 && nbins <= 1024 && width > 0.0, TCX_ERR_ARG, "kernel_grad: 1 <= nbins <= 1024, width > 0");

@@ -127,7 +127,7 @@ void Context::set_cell(const double recip_rows[9], double vol, const int grid[3]
     require(vol > 1e-8, TCX_ERR_ARG, "set_cell: non-positive volume");
     require(grid[0] > 0 && grid[1] > 0 && grid[2] > 0, TCX_ERR_ARG, "set_cell: bad grid");
     require(meth >= TCX_METHOD_HF && meth <= TCX_METHOD_BITC, TCX_ERR_ARG, "set_cell: bad method");
-    require(ns == 1, TCX_ERR_UNSUPPORTED, "set_cell: only num_independent_spins == 1 is implemented");
+    require(ns == 1 || ns == 2, TCX_ERR_ARG, "set_cell: num_independent_spins must be 1 or 2");
     require(grid[0] == grid[1], TCX_ERR_UNSUPPORTED, "set_cell: grid must have n1 == n2");
     require(size_supported(grid[0]) && size_supported(grid[2]), TCX_ERR_UNSUPPORTED,
             "set_cell: grid size not compiled (supported: 6 8 12 16 20 24 32 36 40 48)");
@@ -137,6 +137,7 @@ void Context::set_cell(const double recip_rows[9], double vol, const int grid[3]
     for (int i = 0; i < 9; ++i) geo.b[i] = recip_rows[i];
     N = (long long)grid[0] * grid[1] * grid[2];
     volume = vol;
+    if (ns != nspin) k_set = false;  // the k-blocks repeat per spin: set the k-points again
     nspin = ns;
     method = meth;
     cell_set = true;
@@ -152,9 +153,11 @@ void Context::set_kpoints(int nk_, const double* kf, int numk)
     require(cell_set, TCX_ERR_STATE, "set_kpoints: call tcx_set_cell first");
     require(nk_ > 0 && kf, TCX_ERR_ARG, "set_kpoints: bad k list");
     require(numk > 0, TCX_ERR_ARG, "set_kpoints: num_kpoints must be positive");
-    nk = nk_;
+    nk_phys = nk_;
+    nk = nspin * nk_;
     num_kpoints = numk;
-    kfrac.assign(kf, kf + 3 * nk);
+    kfrac.clear();
+    for (int s = 0; s < nspin; ++s) kfrac.insert(kfrac.end(), kf, kf + 3 * nk_);  // the k list per spin
     kcart.assign(3 * nk, 0.0);
     for (int k = 0; k < nk; ++k)
         for (int d = 0; d < 3; ++d)
@@ -186,20 +189,25 @@ void Context::set_basis(const int* npw_, const int* gindex)
     states_set = false;
     invalidate_states();
     require(npw_ && gindex, TCX_ERR_ARG, "set_basis: null pointer");
-    std::vector<int> n_(npw_, npw_ + nk);
+    // the basis of the nk_phys k-points, repeated for the second spin
+    std::vector<int> n_(nk);
+    for (int k = 0; k < nk; ++k) n_[k] = npw_[k % nk_phys];
     std::vector<long long> off(nk + 1, 0);
     for (int k = 0; k < nk; ++k) {
         require(n_[k] > 0, TCX_ERR_ARG, "set_basis: npw must be positive");
         off[k + 1] = off[k] + n_[k];
     }
-    for (long long l = 0; l < off[nk]; ++l)
+    const long long nphys = off[nk_phys];
+    for (long long l = 0; l < nphys; ++l)
         require(gindex[l] >= 0 && gindex[l] < N, TCX_ERR_ARG, "set_basis: gindex outside the FFT grid");
-    d_gindex.ensure(off[nk] * sizeof(int));
-    TCX_HIP_CHECK(hipMemcpy(d_gindex.p, gindex, off[nk] * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<int> gi;
+    for (int s = 0; s < nspin; ++s) gi.insert(gi.end(), gindex, gindex + nphys);
+    d_gindex.ensure(gi.size() * sizeof(int));
+    TCX_HIP_CHECK(hipMemcpy(d_gindex.p, gi.data(), gi.size() * sizeof(int), hipMemcpyHostToDevice));
     npw = std::move(n_);
     pw_off = std::move(off);
     max_npw = *std::max_element(npw.begin(), npw.end());
-    gindex_h.assign(gindex, gindex + pw_off[nk]);
+    gindex_h = std::move(gi);
     basis_set = true;
 }
 
@@ -329,10 +337,11 @@ void Context::set_div(int enabled, double alpha_, const double* svaux)
     if (enabled) {
         require(svaux != nullptr, TCX_ERR_ARG, "set_div_correction: sum_of_vaux is null");
         require(std::isfinite(alpha_) && alpha_ > 0.0, TCX_ERR_ARG, "set_div_correction: alpha must be positive");
-        for (int k = 0; k < nk; ++k)
+        for (int k = 0; k < nk_phys; ++k)
             require(std::isfinite(svaux[k]), TCX_ERR_ARG, "set_div_correction: non-finite sum_of_vaux");
         alpha = alpha_;
-        sum_vaux.assign(svaux, svaux + nk);
+        sum_vaux.resize(nk);  // per k-block: the value of its k-point
+        for (int k = 0; k < nk; ++k) sum_vaux[k] = svaux[k % nk_phys];
         std::vector<double> dc(nk);
         for (int k = 0; k < nk; ++k) dc[k] = div_factor(k);
         upload(d_dc, dc.data(), dc.size() * sizeof(double));
@@ -350,6 +359,7 @@ void Context::set_symmetry(const int* k_src, const int* rot, const double* trans
     sym_set = false;
     invalidate_states();
     if (k_src == nullptr) return;  // identity symmetry
+    require_one_spin("set_kpoint_symmetry");
     require(rot && trans && trev, TCX_ERR_ARG, "set_kpoint_symmetry: null pointer");
     // lattice rows a_i = 2 pi (B^-1)^T, B = reciprocal rows
     const double* B = geo.b;
@@ -644,7 +654,22 @@ const Plan& Context::get_plan(const std::vector<int>& nb, int term)
         P.app_coef[k + 1] = P.app_coef[k] + (long long)nb[k] * npw[k];
     }
     P.n_app = (int)P.app_first[nk];
-    build_pairs(P, applied_items(nb), occupied_items(), tc, sc, kcart, false);
+    if (nspin == 1) {
+        build_pairs(P, applied_items(nb), occupied_items(), tc, sc, kcart, false);
+    } else {
+        // the exchange pairs states of one spin only (calc_hamiltonian_hf2x.cpp:46, :90-134)
+        const std::vector<PairItem> app = applied_items(nb), occ = occupied_items();
+        for (int s = 0; s < nspin; ++s) {
+            std::vector<PairItem> a, o;
+            for (const PairItem& x : app)
+                if (x.k / nk_phys == s) a.push_back(x);
+            for (const PairItem& x : occ)
+                if (x.k / nk_phys == s) o.push_back(x);
+            const size_t l0 = P.launches.size();
+            build_pairs(P, a, o, tc, sc, kcart, false);
+            for (size_t l = l0; l < P.launches.size(); ++l) P.launches[l].spin = s;
+        }
+    }
     // one scatter/gather job and one divergence job per applied band, in band order
     std::vector<OrbEntry> appjobs, appgradjobs;
     std::vector<DivEntry> divjobs;
@@ -663,6 +688,14 @@ const Plan& Context::get_plan(const std::vector<int>& nb, int term)
     upload(P.d_divjobs, divjobs.data(), divjobs.size() * sizeof(DivEntry));
     auto res = plans.emplace(key, std::move(P));
     return res.first->second;
+}
+
+void Context::require_one_spin(const char* what) const
+{
+    if (nspin == 1) return;
+    throw Error(TCX_ERR_UNSUPPORTED, std::string(what) +
+                                         ": two independent spins are implemented for kinetic, hf2h, hf2x, tc2x "
+                                         "and the HF all / Davidson only");
 }
 
 // ------------------------------------------------------------------ term driver

@@ -159,8 +159,10 @@ void launch_tc2x_C(int prec, const Geom& g, const void* buf, const void* occR, c
                    float* Wc, hipStream_t s);
 
 // density and local three-body potentials (FP64 grids)
+// nspin 1: out[0] = out[1] = half the filled density of the nocc orbitals (the filling holds
+// the spin factor 2); nspin 2: out[0] = the filled density (one spin channel)
 void launch_density(const double* occR, const double* chiR, const double* fill, int nocc, long long N, double* out,
-                    hipStream_t s);
+                    int nspin, hipStream_t s);
 void launch_jastrow_eval(const double* G2, double* uk, double* v, long long n, const JastrowPair& j, hipStream_t s);
 void launch_dnu(const double* nG, double* out, const Geom& g, const JastrowPair& j0, const JastrowPair& j1,
                 hipStream_t s);

@@ -63,6 +63,7 @@ void Context::hf2x_noalloc(const int* nb_, const double* phi, double* out, int p
 {
     require(!g_capturing, TCX_ERR_STATE, "hf2x_noalloc: not capturable (it waits between band chunks)");
     check_apply_args(nb_, phi, out, prec);
+    require_one_spin("hf2x_noalloc");
     for (const void* p : {(const void*)phi, (const void*)out}) {
         hipPointerAttribute_t at{};
         const hipError_t e = hipPointerGetAttributes(&at, p);

@@ -33,10 +33,10 @@ void Context::tc2x(const int* nb_, const double* phi, double* out, int prec, uns
         int max_rows = 1;
         for (const Launch& L : P.launches) max_rows = std::max(max_rows, L.nkq);
         d_ktab.ensure((size_t)max_rows * N * es);
-        const JastrowPair jp = jastrow_pair(0, 0);  // tc2x.cpp:269-271 (ispin, ispin)
         for (const Launch& L : P.launches) {
             const PairEntry* pairs = (const PairEntry*)P.d_pairs.p + L.pair0;
             const TargetEntry* targets = (const TargetEntry*)P.d_targets.p + L.target0;
+            const JastrowPair jp = jastrow_pair(L.spin, L.spin);  // tc2x.cpp:269-271 (ispin, ispin)
             launch_tc_table(prec, d_ktab.p, (const double*)P.d_kq.p, L.kq0, L.nkq, geo, jp, stream);
             timed(0, L.npairs, [&] {
                 launch_tc2x_A(prec, geo, d_appR.p, d_appGrad.p, chiR, pairs, L.npairs, d_pairbuf.p, stream);
@@ -77,8 +77,20 @@ void Context::ensure_density()
     ensure_occR(TCX_PREC_FP64);
     if (bitc) ensure_chiR(TCX_PREC_FP64);
     d_density.ensure((size_t)2 * N * 16);
-    launch_density((const double*)d_occR[TCX_PREC_FP64].p, bitc ? (const double*)d_chiR[TCX_PREC_FP64].p : nullptr,
-                   (const double*)d_fill.p, n_occ, N, (double*)d_density.p, stream);
+    if (nspin == 1) {
+        launch_density((const double*)d_occR[TCX_PREC_FP64].p,
+                       bitc ? (const double*)d_chiR[TCX_PREC_FP64].p : nullptr, (const double*)d_fill.p, n_occ, N,
+                       (double*)d_density.p, 1, stream);
+    } else {
+        // density_[s] from the occupied states of spin s (the k-blocks s nk_phys .. (s+1) nk_phys - 1)
+        for (int s = 0; s < 2; ++s) {
+            const long long b0 = occ_first[(size_t)s * nk_phys], b1 = occ_first[(size_t)(s + 1) * nk_phys];
+            launch_density((const double*)d_occR[TCX_PREC_FP64].p + 2 * b0 * N,
+                           bitc ? (const double*)d_chiR[TCX_PREC_FP64].p + 2 * b0 * N : nullptr,
+                           (const double*)d_fill.p + b0, (int)(b1 - b0), N, (double*)d_density.p + 2 * (size_t)s * N,
+                           2, stream);
+        }
+    }
     density_valid = true;
 }
 
@@ -86,6 +98,7 @@ void Context::ensure_density()
 void Context::tc3_local(int which, const int* nb_, const double* phi, double* out, int prec, unsigned flags)
 {
     check_apply_args(nb_, phi, out, prec);
+    require_one_spin(which == 0 ? "tc3a1" : "tc3b1");
     require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc3: the cell was set up for HF");
     require(jastrow_set, TCX_ERR_STATE, "tc3: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);

@@ -95,6 +95,7 @@ void Context::apply_local(const Plan& P, const ApplyIO& io, const double* V, dou
 void Context::tc3a3b3b4b6(const int* nb_, const double* phi, double* out, int prec, unsigned flags)
 {
     check_apply_args(nb_, phi, out, prec);
+    require_one_spin("tc3a3b3b4b6");
     require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc3a3b3b4b6: the cell was set up for HF");
     require(jastrow_set, TCX_ERR_STATE, "tc3a3b3b4b6: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);
@@ -413,6 +414,7 @@ void Context::a8_occupied_phase(const std::vector<int>& nb, int prec)
 void Context::tc3a2a4b2b5(const int* nb_, const double* phi, double* out, int prec, unsigned flags)
 {
     check_apply_args(nb_, phi, out, prec);
+    require_one_spin("tc3a2a4b2b5");
     require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc3a2a4b2b5: the cell was set up for HF");
     require(jastrow_set, TCX_ERR_STATE, "tc3a2a4b2b5: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);
@@ -571,6 +573,7 @@ void Context::hf2h(const int* nb_, const double* phi, double* out, unsigned flag
 void Context::tc2h(const int* nb_, const double* phi, double* out, unsigned flags)
 {
     check_apply_args(nb_, phi, out, TCX_PREC_FP64);
+    require_one_spin("tc2h");
     require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc2h: the cell was set up for HF");
     require(jastrow_set, TCX_ERR_STATE, "tc2h: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);

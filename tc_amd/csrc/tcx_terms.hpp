@@ -55,6 +55,7 @@ struct Launch {
     int pair0 = 0, npairs = 0;      // slice of Plan::pairs; slot p of the pair buffer = pair0 + p
     int target0 = 0, ntargets = 0;  // slice of Plan::targets (pair indices are launch-local)
     int kq0 = 0, nkq = 0;           // rows of Plan::kq used by this launch
+    int spin = 0;                   // spin channel of its targets and sources (two independent spins)
 };
 
 // (target, source) pairs grouped into launches; TargetEntry::pad holds the
@@ -125,7 +126,11 @@ struct Context {
     long long N = 0;
     double volume = 0.0;
     int nspin = 1, method = TCX_METHOD_HF;
-    int nk = 0, num_kpoints = 0;
+    // Two independent spins: every per-k array of the states and of the applies (occupied
+    // counts, fillings, orbitals, applied band counts) runs over (spin, k), spin-major, and
+    // the k-points and the basis are those of the nk_phys given k-points repeated per spin.
+    // nk = nspin * nk_phys; spin of the k-block k = k / nk_phys.
+    int nk = 0, nk_phys = 0, num_kpoints = 0;
     std::vector<double> kfrac, kcart;
     std::vector<int> npw;
     std::vector<long long> pw_off;
@@ -304,6 +309,8 @@ struct Context {
     void apply_divcorr(const Plan& P, const ApplyIO& io, const double* occ_left);
     // clears W and its compensation grids for n_app applied bands; returns W
     double* begin_W(int n_app);
+    // the terms whose spin structure is not the same-spin pairing of the two-body exchange
+    void require_one_spin(const char* what) const;
     void finish_W(const Plan& P, const ApplyIO& io, double factor);
 
     void hf2x(const int* nb, const double* phi, double* out, int prec, unsigned flags);

@@ -26,6 +26,7 @@ void Context::set_zero_weight(const double* kweight, const double* skv, const in
 {
     require(!g_capturing, TCX_ERR_STATE, "set_zero_weight_kpoints: not capturable");
     require(states_set, TCX_ERR_STATE, "set_zero_weight_kpoints: call tcx_set_states first");
+    require_one_spin("set_zero_weight_kpoints");
     require(!sym_set, TCX_ERR_UNSUPPORTED, "set_zero_weight_kpoints: not combined with a k-mesh symmetry");
     require(kweight && skv && nstate, TCX_ERR_ARG, "set_zero_weight_kpoints: null pointer");
     const bool bitc = method == TCX_METHOD_BITC;

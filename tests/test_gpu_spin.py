@@ -1,0 +1,153 @@
+"""Two independent spins (num_independent_spins = 2) through the C ABI.
+
+The states, fillings and applied bands run over (spin, k), spin-major, with the k-points
+and the basis given once.  The exchange-type terms pair states of one spin only and drop
+the spin-sum halving of their prefactors (src/calc_hamiltonian_hf2x.cpp:29-30, :46, :90-134;
+tc2x.cpp:32-33 with u(ispin, ispin), :269-271); the Hartree term reads the sum of the two
+spin densities (hf2h.cpp:30).  Pins:
+  * two identical spin channels with fillings f/2 reproduce the one-spin result with
+    fillings f, term by term (FP64, 1e-12 of the largest value);
+  * different states per spin and a spin-dependent Jastrow: spin channel s equals the
+    one-spin oracle on that channel's states with fillings 2 f_s (and u(s, s) moved to
+    u(0, 0) for tc2x), the Hartree term the one-spin oracle on both channels' states with
+    fillings f_s (FP64 1e-10, FP32-split per band 3e-5);
+  * the terms that couple the spins otherwise are refused with TCX_ERR_UNSUPPORTED.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tc_amd import system as S
+from tc_amd import tcx
+from test_gpu_hf2x import per_band_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def two_spin(sa, sb, fa=None, fb=None):
+    """A two-spin System with the states of sa as spin 0 and of sb as spin 1 (same cell,
+    k-points and basis), fillings fa / fb (default: half of each system's)."""
+    assert np.array_equal(sa.npw, sb.npw) and np.array_equal(sa.gindex, sb.gindex)
+    fa = sa.filling / 2 if fa is None else fa
+    fb = sb.filling / 2 if fb is None else fb
+    chi = None
+    if sa.chi_occ is not None:
+        chi = np.concatenate([sa.chi_occ, sb.chi_occ])
+    return dataclasses.replace(sa, nspin=2, nocc=np.concatenate([sa.nocc, sb.nocc]).astype(np.int32),
+                               filling=np.concatenate([fa, fb]), phi_occ=np.concatenate([sa.phi_occ, sb.phi_occ]),
+                               chi_occ=chi)
+
+
+def with_states(sys_, nocc, filling, phi, chi):
+    return dataclasses.replace(sys_, nocc=np.asarray(nocc, np.int32), filling=np.asarray(filling, np.float64),
+                               phi_occ=phi, chi_occ=chi)
+
+
+def both_channels(sys_, nb, seed):
+    """Applied bands: the same nb per k in both channels, different coefficients."""
+    nb0, phi0 = S.applied_bands(sys_, nb=nb, seed=seed)
+    nb1, phi1 = S.applied_bands(sys_, nb=nb, seed=seed + 1)
+    return nb0, phi0, nb1, phi1, np.concatenate([nb0, nb1]).astype(np.int32), np.concatenate([phi0, phi1])
+
+
+@pytest.mark.parametrize("method", ["HF", "TC", "BITC"])
+def test_identical_channels_match_one_spin(engine, method):
+    s1 = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 2, 1), nbands=3, nocc=2, seed_stream=151)
+    nb0, phi0 = S.applied_bands(s1, nb=[3, 0, 2, 1], seed=152)
+    engine.load_system(s1)
+    terms = ["kinetic", "hf2h", "hf2x"] + (["tc2x"] if method != "HF" else [])
+    one = {t: getattr(engine, t)(nb0, phi0) for t in terms}
+    s2 = two_spin(s1, s1)
+    engine.load_system(s2)
+    nb2, phi2 = np.concatenate([nb0, nb0]).astype(np.int32), np.concatenate([phi0, phi0])
+    n = phi0.size
+    for t in terms:
+        got = getattr(engine, t)(nb2, phi2)
+        scale = np.abs(one[t]).max()
+        for s in range(2):
+            err = np.abs(got[s * n:(s + 1) * n] - one[t]).max() / scale
+            print(f"{method} {t} spin {s}: {err:.2e}", flush=True)
+            assert err <= 1e-12, (t, s)
+
+
+@pytest.mark.parametrize("method", ["HF", "TC", "BITC"])
+def test_channels_against_oracle(engine, method):
+    A = np.array([[0.5, 0.4], [0.4, 0.8]]) if method != "HF" else None
+    sa = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 1, 1), nbands=4, nocc=3, seed_stream=161, A=A)
+    sb = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 1, 1), nbands=4, nocc=2, seed_stream=162, A=A)
+    fa = np.tile([1.0, 1.0, 0.6], sa.nk)
+    fb = np.tile([1.0, 0.3], sb.nk)
+    s2 = two_spin(sa, sb, fa, fb)
+    engine.load_system(s2)
+    nb0, phi0, nb1, phi1, nb2, phi2 = both_channels(sa, [3, 2], 163)
+    n0 = phi0.size
+    # one-spin oracle systems: channel s with fillings 2 f_s, u(s, s) in the (0, 0) slot
+    ref_sys = [with_states(sa, sa.nocc, 2 * fa, sa.phi_occ, sa.chi_occ),
+               with_states(sb, sb.nocc, 2 * fb, sb.phi_occ, sb.chi_occ)]
+    if method != "HF":
+        A1, F1 = sb.A.copy(), sb.F.copy()
+        A1[0, 0], F1[0, 0] = sb.A[1, 1], sb.F[1, 1]
+        ref_sys[1] = dataclasses.replace(ref_sys[1], A=A1, F=F1)
+    chans = [(nb0, phi0), (nb1, phi1)]
+    terms = ["hf2x"] + (["tc2x"] if method != "HF" else [])
+    for t in terms:
+        refs = [getattr(O.Oracle(ref_sys[s]), t)(*chans[s]) for s in range(2)]
+        for prec in (tcx.PREC_FP64, tcx.PREC_FP32):
+            got = getattr(engine, t)(nb2, phi2, precision=prec)
+            for s, (lo, hi) in enumerate(((0, n0), (n0, phi2.size))):
+                g, r = got[lo:hi], refs[s]
+                if prec == tcx.PREC_FP64:
+                    err = np.abs(g - r).max() / np.abs(r).max()
+                    print(f"{method} {t} fp64 spin {s}: {err:.2e}", flush=True)
+                    assert err <= 1e-10, (t, s)
+                else:
+                    rel = per_band_rel(sa, chans[s][0], g, r)
+                    print(f"{method} {t} fp32 spin {s}: per-band {rel.max():.2e}", flush=True)
+                    assert rel.max() <= 3e-5, (t, s)
+    # Hartree: both channels' states with their fillings in one one-spin system
+    nocc_all, f_all, phi_all, chi_all = [], [], [], []
+    oa, ob = sa.occ_offsets(), sb.occ_offsets()
+    fo_a = np.concatenate([[0], np.cumsum(sa.nocc)])
+    fo_b = np.concatenate([[0], np.cumsum(sb.nocc)])
+    for k in range(sa.nk):
+        nocc_all.append(sa.nocc[k] + sb.nocc[k])
+        f_all += [fa[fo_a[k]:fo_a[k + 1]], fb[fo_b[k]:fo_b[k + 1]]]
+        phi_all += [sa.phi_occ[oa[k]:oa[k + 1]], sb.phi_occ[ob[k]:ob[k + 1]]]
+        if sa.chi_occ is not None:
+            chi_all += [sa.chi_occ[oa[k]:oa[k + 1]], sb.chi_occ[ob[k]:ob[k + 1]]]
+    hsys = with_states(sa, nocc_all, np.concatenate(f_all), np.concatenate(phi_all),
+                       np.concatenate(chi_all) if chi_all else None)
+    got = engine.hf2h(nb2, phi2)
+    for s, (lo, hi) in enumerate(((0, n0), (n0, phi2.size))):
+        r = O.Oracle(hsys).hf2h(*chans[s])
+        err = np.abs(got[lo:hi] - r).max() / np.abs(r).max()
+        print(f"{method} hf2h spin {s}: {err:.2e}", flush=True)
+        assert err <= 1e-10, s
+
+
+def test_hf_all_is_the_sum_of_its_terms(engine):
+    sa = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=171)
+    sb = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=1, seed_stream=172)
+    engine.load_system(two_spin(sa, sb))
+    _, _, _, _, nb2, phi2 = both_channels(sa, [2, 3], 173)
+    h1, h2, _ = engine.apply_all(nb2, phi2)
+    k = engine.kinetic(nb2, phi2)
+    h = engine.hf2h(nb2, phi2) + engine.hf2x(nb2, phi2)
+    assert np.abs(h1 - k).max() <= 1e-14 * np.abs(k).max()
+    assert np.abs(h2 - h).max() <= 1e-12 * np.abs(h).max()
+
+
+def test_spin_coupling_terms_are_refused(engine):
+    s1 = S.tiny_system(method="BITC", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=181)
+    engine.load_system(two_spin(s1, s1))
+    nb, phi = S.applied_bands(s1, nb=[1, 1], seed=182)
+    nb2, phi2 = np.concatenate([nb, nb]).astype(np.int32), np.concatenate([phi, phi])
+    for t in ("tc3a1", "tc3b1", "tc3a3b3b4b6", "tc3a2a4b2b5", "tc2h"):
+        with pytest.raises(tcx.TcxError) as e:
+            getattr(engine, t)(nb2, phi2)
+        assert e.value.code == tcx.TCX_ERR_UNSUPPORTED, t
+    # the two-body terms still run on the same context
+    assert np.isfinite(engine.tc2x(nb2, phi2)).all()
+    engine.load_system(s1)  # one spin again
