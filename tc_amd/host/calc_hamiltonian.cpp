@@ -87,11 +87,10 @@ int method_id(const std::string& m)
     throw std::runtime_error("calc_hamiltonian: unknown calc_method " + m);
 }
 
-// pack [k][band][spinor=0] coefficient vectors of spin 0 into one ragged array
+// append [k][band < count[k]][spinor=0] coefficient vectors of one spin to a ragged array
 void pack_bands(const std::vector<std::vector<std::vector<VectorXcd>>>& bands_of_k, const std::vector<int>& count,
                 std::vector<double>& out)
 {
-    out.clear();
     for (size_t k = 0; k < bands_of_k.size(); ++k)
         for (int b = 0; b < count[k]; ++b) {
             const VectorXcd& v = bands_of_k[k][b][0];
@@ -100,10 +99,26 @@ void pack_bands(const std::vector<std::vector<std::vector<VectorXcd>>>& bands_of
         }
 }
 
+// The ABI's (spin, k)-blocked arrays of phi[spin][k][band][0] for nspin spins, spin-major:
+// band counts nb[s nk + k] and the packed coefficients.
+void pack_spins(const Phi4& phi, int nspin, int nk, std::vector<int>& nb, std::vector<double>& out)
+{
+    if ((int)phi.size() < nspin) throw std::runtime_error("calc_hamiltonian: phi does not cover every spin");
+    nb.assign((size_t)nspin * nk, 0);
+    out.clear();
+    for (int s = 0; s < nspin; ++s) {
+        if ((int)phi[s].size() != nk) throw std::runtime_error("calc_hamiltonian: phi does not cover every k");
+        std::vector<int> c(nk);
+        for (int k = 0; k < nk; ++k) c[k] = nb[(size_t)s * nk + k] = (int)phi[s][k].size();
+        pack_bands(phi[s], c, out);
+    }
+}
+
 void push_geometry(tcx_context* ctx, const Method& method, const CrystalStructure& cs, const Spin& spin,
                    const Kpoints& kp, const PlaneWaveBasis& pw)
 {
-    if (spin.num_independent_spins != 1) throw std::runtime_error("calc_hamiltonian: only one independent spin");
+    const int nspin = spin.num_independent_spins;
+    if (nspin != 1 && nspin != 2) throw std::runtime_error("calc_hamiltonian: num_independent_spins must be 1 or 2");
     if (method.calc_mode != "SCF") throw std::runtime_error("calc_hamiltonian: only SCF mode");
     const int nk = (int)kp.kvectors_scf.size();
     double recip[9];
@@ -125,6 +140,7 @@ void push_geometry(tcx_context* ctx, const Method& method, const CrystalStructur
     f.add(&cs.unit_cell_volume, sizeof(double));
     f.add(pw.size_FFT_grid_vec, sizeof(pw.size_FFT_grid_vec));
     f.add(&meth, sizeof(meth));
+    f.add(&nspin, sizeof(nspin));
     f.add(&kp.num_kpoints, sizeof(int));
     f.add_vec(kf);
     f.add_vec(npw);
@@ -133,7 +149,7 @@ void push_geometry(tcx_context* ctx, const Method& method, const CrystalStructur
     if (P.geometry != 0 && P.geometry == f.value()) return;
     // set_cell and set_kpoints drop everything that depends on them
     P = Pushed();
-    check(ctx, tcx_set_cell(ctx, recip, cs.unit_cell_volume, pw.size_FFT_grid_vec, 1, meth), "set_cell");
+    check(ctx, tcx_set_cell(ctx, recip, cs.unit_cell_volume, pw.size_FFT_grid_vec, nspin, meth), "set_cell");
     check(ctx, tcx_set_kpoints(ctx, nk, kf.data(), kp.num_kpoints), "set_kpoints");
     check(ctx, tcx_set_basis(ctx, npw.data(), gidx.data()), "set_basis");
     P.geometry = f.value();
@@ -200,25 +216,36 @@ void push_state(tcx_context* ctx, const Method& method, const CrystalStructure& 
     if (sym) push_symmetry(ctx, *sym, kp);
     Pushed& P = pushed_state()[ctx];
     const int nk = (int)kp.kvectors_scf.size();
-    // occupied orbitals: bands < num_occupied_bands (BlochStates::set_filling, f > 1e-8)
-    std::vector<int> nocc(bs.num_occupied_bands.at(0).begin(), bs.num_occupied_bands.at(0).end());
-    if ((int)nocc.size() != nk) throw std::runtime_error("calc_hamiltonian: num_occupied_bands does not cover every k");
+    const int nspin = spin.num_independent_spins;
+    // occupied orbitals: bands < num_occupied_bands (BlochStates::set_filling, f > 1e-8),
+    // over (spin, k), spin-major
+    std::vector<int> nocc;
     std::vector<double> fill;
-    for (int k = 0; k < nk; ++k)
-        for (int b = 0; b < nocc[k]; ++b) fill.push_back(bs.filling.at(0).at(k).at(b));
     const bool bitc = method.calc_method == "BITC";
     Fingerprint fs;
+    for (int s = 0; s < nspin; ++s) {
+        const std::vector<int>& ns = bs.num_occupied_bands.at(s);
+        if ((int)ns.size() != nk) throw std::runtime_error("calc_hamiltonian: num_occupied_bands does not cover every k");
+        nocc.insert(nocc.end(), ns.begin(), ns.end());
+        for (int k = 0; k < nk; ++k)
+            for (int b = 0; b < ns[k]; ++b) fill.push_back(bs.filling.at(s).at(k).at(b));
+        add_bands(fs, bs.phik_scf.at(s), ns);
+        if (bitc) add_bands(fs, bs.phik_left_scf.at(s), ns);
+    }
     fs.add_vec(nocc);
     fs.add_vec(fill);
-    add_bands(fs, bs.phik_scf.at(0), nocc);
-    if (bitc) add_bands(fs, bs.phik_left_scf.at(0), nocc);
-    if (P.states == 0 || P.states != fs.value()) {
-        P.states = 0;
+    auto push_orbitals = [&] {
         std::vector<double> phi_occ, chi_occ;
-        pack_bands(bs.phik_scf[0], nocc, phi_occ);
-        if (bitc) pack_bands(bs.phik_left_scf[0], nocc, chi_occ);
+        for (int s = 0; s < nspin; ++s) {
+            pack_bands(bs.phik_scf[s], bs.num_occupied_bands[s], phi_occ);
+            if (bitc) pack_bands(bs.phik_left_scf[s], bs.num_occupied_bands[s], chi_occ);
+        }
         check(ctx, tcx_set_states(ctx, nocc.data(), fill.data(), phi_occ.data(), bitc ? chi_occ.data() : nullptr),
               "set_states");
+    };
+    if (P.states == 0 || P.states != fs.value()) {
+        P.states = 0;
+        push_orbitals();
         P.states = fs.value();
         P.density = 0;  // the context now holds the density of these orbitals
     }
@@ -246,11 +273,7 @@ void push_state(tcx_context* ctx, const Method& method, const CrystalStructure& 
         // back to the orbital density: push the states again
         P.states = 0;
         P.density = 0;
-        std::vector<double> phi_occ, chi_occ;
-        pack_bands(bs.phik_scf[0], nocc, phi_occ);
-        if (bitc) pack_bands(bs.phik_left_scf[0], nocc, chi_occ);
-        check(ctx, tcx_set_states(ctx, nocc.data(), fill.data(), phi_occ.data(), bitc ? chi_occ.data() : nullptr),
-              "set_states");
+        push_orbitals();
         P.states = fs.value();
     }
     double A[4], F[4];
@@ -281,6 +304,19 @@ void push_state(tcx_context* ctx, const Method& method, const CrystalStructure& 
     }
 }
 
+// the (spin, k)-blocked ragged array back into H[spin][k][band][0], spins < nspin
+void unpack_bands(const std::vector<double>& buf, Phi4& H, int nspin)
+{
+    size_t off = 0;
+    for (int s = 0; s < nspin; ++s)
+        for (auto& k : H[s])
+            for (auto& b : k) {
+                VectorXcd& v = b[0];
+                std::memcpy(reinterpret_cast<double*>(v.data()), buf.data() + off, sizeof(double) * 2 * v.size());
+                off += 2 * v.size();
+            }
+}
+
 using TermFn = int (*)(tcx_context*, const int*, const double*, double*, int, unsigned);
 
 void run_term(TermFn fn, const char* name, const Parallelization& par, const Method& method,
@@ -290,20 +326,14 @@ void run_term(TermFn fn, const char* name, const Parallelization& par, const Met
 {
     tcx_context* ctx = context_for(par.device);
     push_state(ctx, method, cs, pot, spin, kp, pw, bs, sym);
-    const int nk = (int)kp.kvectors_scf.size();
-    std::vector<int> nb(nk);
-    for (int k = 0; k < nk; ++k) nb[k] = (int)phi[0][k].size();
+    const int nk = (int)kp.kvectors_scf.size(), nspin = spin.num_independent_spins;
+    std::vector<int> nb, nbo;
     std::vector<double> in, out;
-    pack_bands(phi[0], nb, in);
-    pack_bands(HNphi[0], nb, out);  // the term accumulates into the current H*phi
+    pack_spins(phi, nspin, nk, nb, in);
+    pack_spins(HNphi, nspin, nk, nbo, out);  // the term accumulates into the current H*phi
+    if (nbo != nb) throw std::runtime_error(std::string("calc_hamiltonian::") + name + ": H*phi does not match phi");
     check(ctx, fn(ctx, nb.data(), in.data(), out.data(), method.precision, 0u), name);
-    size_t off = 0;
-    for (int k = 0; k < nk; ++k)
-        for (int b = 0; b < nb[k]; ++b) {
-            VectorXcd& v = HNphi[0][k][b][0];
-            std::memcpy(reinterpret_cast<double*>(v.data()), out.data() + off, sizeof(double) * 2 * v.size());
-            off += 2 * v.size();
-        }
+    unpack_bands(out, HNphi, nspin);
 }
 
 int hf2h_term(tcx_context* c, const int* nb, const double* phi, double* out, int, unsigned f)
@@ -315,16 +345,6 @@ int tc2h_term(tcx_context* c, const int* nb, const double* phi, double* out, int
     return tcx_tc2h(c, nb, phi, out, f);
 }
 
-void unpack_bands(const std::vector<double>& buf, Phi4& H)
-{
-    size_t off = 0;
-    for (auto& k : H[0])
-        for (auto& b : k) {
-            VectorXcd& v = b[0];
-            std::memcpy(reinterpret_cast<double*>(v.data()), buf.data() + off, sizeof(double) * 2 * v.size());
-            off += 2 * v.size();
-        }
-}
 
 }  // namespace
 
@@ -336,12 +356,11 @@ void all(const Parallelization& par, MyClock&, const Method& m, const CrystalStr
 {
     tcx_context* ctx = context_for(par.device);
     push_state(ctx, m, cs, pot, spin, kp, pw, bs, &sym);
-    const int nk = (int)kp.kvectors_scf.size();
-    std::vector<int> nb(nk);
-    for (int k = 0; k < nk; ++k) nb[k] = (int)phi[0][k].size();
+    const int nk = (int)kp.kvectors_scf.size(), nspin = spin.num_independent_spins;
+    std::vector<int> nb;
     const bool tc = m.calc_method != "HF";
     std::vector<double> in, h1, h2, h3;
-    pack_bands(phi[0], nb, in);
+    pack_spins(phi, nspin, nk, nb, in);
     h1.assign(in.size(), 0.0);
     h2.assign(in.size(), 0.0);
     if (tc) h3.assign(in.size(), 0.0);
@@ -351,15 +370,17 @@ void all(const Parallelization& par, MyClock&, const Method& m, const CrystalStr
     // H*phi arrays take the shape of phi (calc_hamiltonian_all.cpp:28-49)
     for (Phi4* H : {&H1phi, &H2phi, &H3phi}) {
         if (H == &H3phi && !tc) continue;
-        H->assign(1, std::vector<std::vector<std::vector<VectorXcd>>>(nk));
-        for (int k = 0; k < nk; ++k) {
-            (*H)[0][k].resize(nb[k]);
-            for (int b = 0; b < nb[k]; ++b) (*H)[0][k][b].assign(1, VectorXcd(phi[0][k][b][0].size()));
-        }
+        H->assign(nspin, std::vector<std::vector<std::vector<VectorXcd>>>(nk));
+        for (int s = 0; s < nspin; ++s)
+            for (int k = 0; k < nk; ++k) {
+                const int n = nb[(size_t)s * nk + k];
+                (*H)[s][k].resize(n);
+                for (int b = 0; b < n; ++b) (*H)[s][k][b].assign(1, VectorXcd(phi[s][k][b][0].size()));
+            }
     }
-    unpack_bands(h1, H1phi);
-    unpack_bands(h2, H2phi);
-    if (tc) unpack_bands(h3, H3phi);
+    unpack_bands(h1, H1phi, nspin);
+    unpack_bands(h2, H2phi, nspin);
+    if (tc) unpack_bands(h3, H3phi, nspin);
 }
 
 void kinetic(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Spin& spin,
@@ -368,14 +389,14 @@ void kinetic(const Parallelization& par, const Method& m, const CrystalStructure
     // the one-body term reads no orbital state: only the cell, k-points and basis are pushed
     tcx_context* ctx = context_for(par.device);
     push_geometry(ctx, m, cs, spin, kp, pw);
-    const int nk = (int)kp.kvectors_scf.size();
-    std::vector<int> nb(nk);
-    for (int k = 0; k < nk; ++k) nb[k] = (int)phi[0][k].size();
+    const int nk = (int)kp.kvectors_scf.size(), nspin = spin.num_independent_spins;
+    std::vector<int> nb, nbo;
     std::vector<double> in, out;
-    pack_bands(phi[0], nb, in);
-    pack_bands(H1phi[0], nb, out);
+    pack_spins(phi, nspin, nk, nb, in);
+    pack_spins(H1phi, nspin, nk, nbo, out);
+    if (nbo != nb) throw std::runtime_error("calc_hamiltonian::kinetic: H*phi does not match phi");
     check(ctx, tcx_kinetic(ctx, nb.data(), in.data(), out.data(), 0u), "kinetic");
-    unpack_bands(out, H1phi);
+    unpack_bands(out, H1phi, nspin);
 }
 
 void hf2h(const Parallelization& par, const Method& m, const CrystalStructure& cs, const Potentials& pot,

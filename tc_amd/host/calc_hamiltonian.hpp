@@ -13,9 +13,10 @@
 //
 // Error behaviour: the reference stops the run (error_messages::stop ->
 // MPI_Abort, src/error_messages.cpp:60-64); here a failing call throws
-// std::runtime_error carrying tcx_last_error().  Scope: SCF mode,
-// num_independent_spins == 1, no spinors; the k-mesh is passed flattened (every
-// symmetry replica a k of its own), with its symmetry relation for tc3a2a4b2b5.
+// std::runtime_error carrying tcx_last_error().  Scope: SCF mode, no spinors;
+// num_independent_spins 1, or 2 for kinetic, hf2h, hf2x, tc2x and the HF all (the
+// engine refuses the spin-coupling terms with two spins); the k-mesh is passed flattened
+// (every symmetry replica a k of its own), with its symmetry relation for tc3a2a4b2b5.
 #pragma once
 
 #include <array>

@@ -5,8 +5,9 @@
 //   plane wave of the sphere at k  =>  Vx is diagonal with
 //   <k+G0|Vx|k+G0> = c_x sum_{q,b} f_qb / |k+G0-q-G_b|^2  (0 below 1e-8),
 //   c_x = -4 pi/(Nk V) / 2   (src/calc_hamiltonian_hf2x.cpp:29-30, 91-147).
-// Also checks tc2x with A = 0 against hf2x (src/jastrow_functions.cpp:30-38) and
-// calc_hamiltonian::all (kinetic closed form, Hartree = 0 for a uniform density).
+// Also checks tc2x with A = 0 against hf2x (src/jastrow_functions.cpp:30-38),
+// calc_hamiltonian::all (kinetic closed form, Hartree = 0 for a uniform density) and
+// hf2x with two independent spins holding different occupied sets.
 // Exit status 0 on success.
 #include <algorithm>
 #include <cmath>
@@ -171,6 +172,54 @@ int main()
             }
         std::printf("all(%s): kinetic |err| = %.3e, H2 |err| = %.3e, |H3| = %.3e\n", m, e1, e2, e3);
         if (!(e1 <= 1e-13 && e2 <= 1e-11 * scale && e3 <= 1e-14)) return 1;
+    }
+
+    // Two independent spins: spin 0 holds both plane waves, spin 1 only the first, each with
+    // filling 1.  Channel s of hf2x then follows the closed form over the occupied set of
+    // spin s, with c_x without the spin-sum halving (src/calc_hamiltonian_hf2x.cpp:29-30, :46).
+    {
+        method.calc_method = "HF";
+        Spin spin2;
+        spin2.num_independent_spins = 2;
+        const int nocc_s[2] = {2, 1};
+        BlochStates bs2;
+        bs2.phik_scf.resize(2);
+        bs2.filling.resize(2);
+        bs2.num_occupied_bands.resize(2);
+        Phi4 phi2(2, phi[0]), Hs(2, H[0]);
+        for (int s = 0; s < 2; ++s)
+            for (int k = 0; k < 3; ++k) {
+                bs2.phik_scf[s].emplace_back(bs.phik_scf[0][k].begin(), bs.phik_scf[0][k].begin() + nocc_s[s]);
+                bs2.filling[s].emplace_back(nocc_s[s], 1.0);
+                bs2.num_occupied_bands[s].push_back(nocc_s[s]);
+                for (auto& b : Hs[s][k]) std::fill(b[0].begin(), b[0].end(), Complex(0.0));
+            }
+        calc_hamiltonian::hf2x(par, method, cs, pot, spin2, kp, pw, bs2, phi2, Hs, nullptr);
+        const double c2 = -4 * PI / (kp.num_kpoints * cs.unit_cell_volume);
+        double e = 0.0;
+        for (int s = 0; s < 2; ++s)
+            for (int k = 0; k < 3; ++k) {
+                const int npw = (int)pw.Gindex_at_k_scf[k].size();
+                for (int j = 0; j < npw; ++j) {
+                    double expect = 0.0;
+                    for (int q = 0; q < 3; ++q)
+                        for (int b = 0; b < nocc_s[s]; ++b) {
+                            double d2 = 0;
+                            for (int d = 0; d < 3; ++d) {
+                                const double c = 2 * PI / a *
+                                                 (kp.kvectors_scf[k][d] + mill[k][j][d] - kp.kvectors_scf[q][d] -
+                                                  mill[q][b][d]);
+                                d2 += c * c;
+                            }
+                            if (d2 >= 1e-8) expect += 1.0 / d2;
+                        }
+                    expect *= c2;
+                    for (int l = 0; l < npw; ++l)
+                        e = std::fmax(e, std::abs(Hs[s][k][j][0][l] - (l == j ? expect : 0.0)));
+                }
+            }
+        std::printf("hf2x, two spins (2 and 1 occupied plane waves): max |err| = %.3e\n", e);
+        if (!(e <= 1e-11 * scale)) return 1;
     }
     std::printf("OK\n");
     return 0;
