@@ -40,7 +40,7 @@ struct XyTiles {
 };
 
 // Wavefronts per SIMD that the LDS of the tiles allows (160 KB per CU, 4 SIMDs), as the
-// register bound of the four-tile passes where it is 3 or fewer, so that registers do not
+// register bound of the four-tile pass (a8c2) where it is 3 or fewer, so that registers do not
 // lower the occupancy further (above that the bound would make the small grids spill; the
 // three-tile passes spill under it at 24^3-36^3 and take no bound).
 template <typename T, int NP, int NT>
@@ -198,8 +198,8 @@ constexpr int tiles_a7c2()
 
 // Tiles per workgroup of each pass: as many as its largest group of channels transformed
 // together (three tiles give 4 workgroups of 2 wavefronts per CU at 40^3 in FP32, four
-// tiles 3 of 3).
-constexpr int kTilesA7c1 = 4, kTilesB4 = 3, kTilesA8c1 = 3, kTilesA8c2 = 4;
+// tiles 3 of 3).  a7c1 transforms its four inputs as three, then one.
+constexpr int kTilesA7c1 = 3, kTilesB4 = 3, kTilesA8c1 = 3, kTilesA8c2 = 4;
 
 }  // namespace
 
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_u2_table(cpx<T>* __restrict__ tab, cons
 // Out: Y[p] = f b3 sum_d du_0,d rec[p][1+d] in G space (completed later by the second combine);
 //      P[p][3s+d] = f du_s,d rec[p][0], inverse-transformed along x and y.
 template <typename T, int NP>
-__global__ __launch_bounds__((XyGroup<NP, kTilesA7c1>::threads), (xy_waves_per_simd<T, NP, kTilesA7c1>()))
+__global__ __launch_bounds__((XyGroup<NP, kTilesA7c1>::threads))
 void k_xy_a7c1(
     const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ P, cpx<T>* __restrict__ Y, const PairEntry* __restrict__ pairs,
     const cpx<T>* __restrict__ tab, int kq0, const double* __restrict__ kqcart, const double* __restrict__ fill, int nz,
@@ -245,23 +245,26 @@ void k_xy_a7c1(
     const double f = fill[pe.occ];
     cpx<T> u[EPT];
     load_points<T, NP, NT>(u, tp);
-    xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * 4 + L.tile()) * ng + pz, L.c < 4, L.t);
-    cpx<T> r0[EPT];
+    // rec[p][1+d] in tiles d -> Y; then rec[p][0] in tile 0, kept in registers
+    xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * 4 + 1 + L.tile()) * ng + pz, L.c < 3, L.t);
     for_points<NP, NT>([&](auto ic, int e, int x, int y) {
         constexpr int i = decltype(ic)::value;
         double xd[3];
         pp.at(g, x, y, NP, xd);
         const double u0 = (double)u[i].re;
-        r0[i] = S.t[0].a[y][x];
         double2 acc = make_double2(0.0, 0.0);
         for (int d = 0; d < 3; ++d) {
-            const double2 s = up2(S.t[1 + d].a[y][x]);
+            const double2 s = up2(S.t[d].a[y][x]);
             acc.x += s.x * (xd[d] * u0);
             acc.y += s.y * (xd[d] * u0);
         }
         Y[(size_t)p * ng + pz + e] = down2<T>(make_double2(acc.x * (f * fb3), acc.y * (f * fb3)));
     });
-    __syncthreads();  // the inputs have been read before outputs overwrite the tiles
+    __syncthreads();  // the tiles have been read before rec[p][0] overwrites tile 0
+    xy_forward<T, NP>(S.t[0], rec + (size_t)p * 4 * ng + pz, L.c == 0, L.t);
+    cpx<T> r0[EPT];
+    for_points<NP, NT>([&](auto ic, int e, int x, int y) { r0[decltype(ic)::value] = S.t[0].a[y][x]; });
+    __syncthreads();  // the input has been read before outputs overwrite the tiles
     xy_du_rounds<T, NP, NT, 2>(
         S, pp, g, u, [&](auto ic, int) { return r0[decltype(ic)::value]; },
         [&](int k, cpx<T> uk) { return f * (double)(k == 0 ? uk.re : uk.im); },
