@@ -531,7 +531,7 @@ void launch_xy_a7c2(int prec, const Geom& g, const void* P, void* Y, const PairE
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
 #define TCX_L(NP)                                                                                              \
-    hipLaunchKernelGGL((k_xy_a7c2<T, NP>), dim3(g.n3, n), (xy_block<NP, tiles_a7c2<T, NP>()>()), 0, s,                  \
+    hipLaunchKernelGGL((k_xy_a7c2<T, NP>), dim3(g.n3, n), (xy_block<NP, tiles_a7c2<T, NP>()>()), 0, s,         \
                        (const cpx<T>*)((const char*)P + (size_t)a * 6 * gb), (cpx<T>*)((char*)Y + (size_t)a * gb), \
                        pairs + a, (const cpx<T>*)tab, kq0, kqcart, g.n3, g, fb6);
         TCX_XY_DISPATCH(prec, g, TCX_L);
