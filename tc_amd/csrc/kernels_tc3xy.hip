@@ -2,7 +2,7 @@
 // per-term G-space combine fused in.
 //
 // A record channel arrives transformed along z only (the products pass); one workgroup
-// owns plane z of one pair and four LDS tiles (channel planes), and runs
+// owns plane z of one pair and three to six LDS tiles (channel planes), and runs
 //     FFT_y, FFT_x of the input channels        (completing FFT_forward),
 //     the term's combine at every G of the plane,
 //     inverse FFT_x, FFT_y of the output channels (the xy half of FFT_backward),
@@ -40,12 +40,13 @@ struct XyTiles {
 };
 
 // Wavefronts per SIMD that the LDS of the tiles allows (160 KB per CU, 4 SIMDs), as the
-// register bound of the four-tile pass (a8c2) where it is 3 or fewer, so that registers do not
-// lower the occupancy further (above that the bound would make the small grids spill; the
-// three-tile passes spill under it at 24^3-36^3 and take no bound).
+// register bound of the four-tile pass (a8c2 in FP32) where it is 3 or fewer, so that registers
+// do not lower the occupancy further (above that the bound would make the small grids spill;
+// the three-tile passes spill under it at 24^3-36^3 and take no bound).
 template <typename T, int NP, int NT>
 constexpr int xy_waves_per_simd()
 {
+    if (NT < 4) return 0;  // no bound
     const int wgs = (160 * 1024) / (int)sizeof(XyTiles<T, NP, NT>);
     const int w = (wgs * XyGroup<NP, NT>::threads / 64 + 3) / 4;
     return w < 1 || w > 3 ? 1 : w;
@@ -199,7 +200,16 @@ constexpr int tiles_a7c2()
 // Tiles per workgroup of each pass: as many as its largest group of channels transformed
 // together (three tiles give 4 workgroups of 2 wavefronts per CU at 40^3 in FP32, four
 // tiles 3 of 3).  a7c1 transforms its four inputs as three, then one.
-constexpr int kTilesA7c1 = 3, kTilesB4 = 3, kTilesA8c1 = 3, kTilesA8c2 = 4;
+constexpr int kTilesA7c1 = 3, kTilesB4 = 3, kTilesA8c1 = 3;
+
+// a8c2 transforms its four inputs at once in FP32 and as three, then one, in FP64 (at 40^3
+// two workgroups of 2 wavefronts per CU against one of 3): tc3a2a4b2b5 at 40^3 took 6.59 s
+// (four) against 6.69 s (three) in FP32, 13.67 s against 13.10 s in FP64
+template <typename T>
+constexpr int tiles_a8c2()
+{
+    return sizeof(T) == 4 ? 4 : 3;
+}
 
 }  // namespace
 
@@ -423,13 +433,14 @@ void k_xy_a8c1(
 // z-transformed Q5'_t,d phi_kj.  Out (a separate record): Z[p][0] = b5 f2 sum_d du0_d rec[p][1+d],
 // Z[p][1+d] = f2 a4 du0_d rec[p][0], inverse-transformed along x and y.
 template <typename T, int NP>
-__global__ __launch_bounds__((XyGroup<NP, kTilesA8c2>::threads), (xy_waves_per_simd<T, NP, kTilesA8c2>()))
+__global__ __launch_bounds__((XyGroup<NP, tiles_a8c2<T>()>::threads),
+                             (xy_waves_per_simd<T, NP, tiles_a8c2<T>()>()))
 void k_xy_a8c2(
     const cpx<T>* __restrict__ rec, cpx<T>* __restrict__ Z, const PairEntry* __restrict__ pairs,
     const cpx<T>* __restrict__ tab, int kq0, const double* __restrict__ kqcart, const double* __restrict__ fill, int nz,
     Geom g, double fa4, double fb5)
 {
-    constexpr int NT = kTilesA8c2;
+    constexpr int NT = tiles_a8c2<T>();
     constexpr int EPT = XyGroup<NP, NT>::ept;
     __shared__ XyTiles<T, NP, NT> S;
     const XyLane<NP, NT> L;
@@ -441,23 +452,50 @@ void k_xy_a8c2(
     const double f2 = fill[pe.occ];
     cpx<T> uv[EPT];
     load_points<T, NP, NT>(uv, tp);
-    xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * 4 + L.tile()) * ng + pz, L.c < 4, L.t);
-    for_points<NP, NT>([&](auto ic, int e, int x, int y) {
-        double xd[3];
-        pp.at(g, x, y, NP, xd);
-        const double u = (double)uv[decltype(ic)::value].re;
-        const cpx<T> r0 = S.t[0].a[y][x];
-        double2 b = make_double2(0.0, 0.0);
-        for (int d = 0; d < 3; ++d) {
-            const double2 s = up2(S.t[1 + d].a[y][x]);
-            b.x += s.x * (xd[d] * u);
-            b.y += s.y * (xd[d] * u);
-        }
-        S.t[0].a[y][x] = down2<T>(make_double2(b.x * (fb5 * f2), b.y * (fb5 * f2)));
-        for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r0, f2 * fa4 * xd[d] * u);
-    });
-    __syncthreads();
-    xy_inverse_store<T, NP>(S.t[L.tile()], Z + ((size_t)p * 4 + L.tile()) * ng + pz, L.c < 4, L.t);
+    if constexpr (NT == 4) {
+        xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * 4 + L.tile()) * ng + pz, L.c < 4, L.t);
+        for_points<NP, NT>([&](auto ic, int e, int x, int y) {
+            double xd[3];
+            pp.at(g, x, y, NP, xd);
+            const double u = (double)uv[decltype(ic)::value].re;
+            const cpx<T> r0 = S.t[0].a[y][x];
+            double2 b = make_double2(0.0, 0.0);
+            for (int d = 0; d < 3; ++d) {
+                const double2 s = up2(S.t[1 + d].a[y][x]);
+                b.x += s.x * (xd[d] * u);
+                b.y += s.y * (xd[d] * u);
+            }
+            S.t[0].a[y][x] = down2<T>(make_double2(b.x * (fb5 * f2), b.y * (fb5 * f2)));
+            for (int d = 0; d < 3; ++d) S.t[1 + d].a[y][x] = mulr(r0, f2 * fa4 * xd[d] * u);
+        });
+        __syncthreads();
+        xy_inverse_store<T, NP>(S.t[L.tile()], Z + ((size_t)p * 4 + L.tile()) * ng + pz, L.c < 4, L.t);
+    } else {
+        // rec[p][1+d] in tiles d -> Z[p][0] (tile 0); then rec[p][0] in tile 0, kept in registers
+        xy_forward<T, NP>(S.t[L.tile()], rec + ((size_t)p * 4 + 1 + L.tile()) * ng + pz, L.c < 3, L.t);
+        for_points<NP, NT>([&](auto ic, int e, int x, int y) {
+            double xd[3];
+            pp.at(g, x, y, NP, xd);
+            const double u = (double)uv[decltype(ic)::value].re;
+            double2 b = make_double2(0.0, 0.0);
+            for (int d = 0; d < 3; ++d) {
+                const double2 s = up2(S.t[d].a[y][x]);
+                b.x += s.x * (xd[d] * u);
+                b.y += s.y * (xd[d] * u);
+            }
+            S.t[0].a[y][x] = down2<T>(make_double2(b.x * (fb5 * f2), b.y * (fb5 * f2)));
+        });
+        __syncthreads();
+        xy_inverse_store<T, NP>(S.t[0], Z + (size_t)p * 4 * ng + pz, L.c == 0, L.t);
+        xy_forward<T, NP>(S.t[0], rec + (size_t)p * 4 * ng + pz, L.c == 0, L.t);
+        cpx<T> r0[EPT];
+        for_points<NP, NT>([&](auto ic, int e, int x, int y) { r0[decltype(ic)::value] = S.t[0].a[y][x]; });
+        __syncthreads();  // the input has been read before outputs overwrite the tiles
+        xy_du_rounds<T, NP, NT, 1>(
+            S, pp, g, uv, [&](auto ic, int) { return r0[decltype(ic)::value]; },
+            [&](int, cpx<T> uk) { return f2 * fa4 * (double)uk.re; },
+            [&](int o) { return Z + ((size_t)p * 4 + 1 + o) * ng + pz; });
+    }
 }
 
 // ----------------------------------------------------------------- launchers
@@ -582,7 +620,7 @@ void launch_xy_a8c2(int prec, const Geom& g, const void* rec, void* Z, const Pai
     const size_t es = prec == TCX_PREC_FP64 ? 16 : 8, gb = (size_t)g.n1 * g.n2 * g.n3 * es;
     for_pair_slices(npairs, [&](int a, int n) {
 #define TCX_L(NP)                                                                                                 \
-    hipLaunchKernelGGL((k_xy_a8c2<T, NP>), dim3(g.n3, n), (xy_block<NP, kTilesA8c2>()), 0, s,                     \
+    hipLaunchKernelGGL((k_xy_a8c2<T, NP>), dim3(g.n3, n), (xy_block<NP, tiles_a8c2<T>()>()), 0, s,                     \
                        (const cpx<T>*)((const char*)rec + (size_t)a * 4 * gb),                                    \
                        (cpx<T>*)((char*)Z + (size_t)a * 4 * gb), pairs + a, (const cpx<T>*)tab, kq0, kqcart, fill, \
                        g.n3, g, fa4, fb5);
