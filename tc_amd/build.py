@@ -46,6 +46,7 @@ SOURCES = [
     "csrc/tcx_hf2x_xy.cpp",
     "host/calc_hamiltonian.cpp",
     "host/scf.cpp",
+    "host/host_api.cpp",
 ]
 ROOT = os.path.dirname(HERE)
 SHIM_TEST_SRC = os.path.join(ROOT, "tests", "host_shim_test.cpp")

@@ -96,6 +96,10 @@ std::vector<KReplica> replicate_kpoints(const QEInputs& qe);
 // FFT-grid indices and phases of the irreducible plane waves at a replica
 void replica_basis(const QEInputs& qe, const KReplica& r, std::vector<int>& gidx, std::vector<Complex>& phase);
 
+// Reads the flat input file of include/tcx_host.h (QE inputs and SCF settings); throws
+// std::invalid_argument for a missing or malformed file.
+void read_qe_flat(const std::string& path, QEInputs& qe, SCFSettings& s);
+
 class Diagonalization {
 public:
     explicit Diagonalization(const SCFSettings& s) : set_(s) {}

@@ -7,9 +7,10 @@ It then works on the full k-mesh without storing it: every k of the mesh is an
 irreducible k with a rotation (and possibly time reversal), and each orbital at that k
 is produced on the fly by re-indexing the irreducible coefficients
 (plane_wave_basis.cpp:81-164, 250-270).  This module builds that full mesh once,
-explicitly, as a `tc_amd.system.System` with identity symmetry, so that the engine's
-operators see exactly the orbitals the reference's loops see:
+explicitly, as a `tc_amd.system.System`, so that the engine's operators see exactly the
+orbitals the reference's loops see.
 
+The expansion itself runs in the C++ host (include/tcx_host.h, tc_amd/host/scf.cpp):
   * TC++ rotation U = (QE rotation)^-1, integer, crystal coordinates (symmetry.cpp:17-33);
   * k-replicas k' = U^T k (crystal), kept when not equal to an earlier one modulo the
     reciprocal lattice, and -k' by time reversal unless QE's noinv is true
@@ -17,19 +18,24 @@ operators see exactly the orbitals the reference's loops see:
   * orbital at k': coefficient c(G) placed at G' = U^T G (G' -> -G' and c -> conj(c)
     under time reversal), times exp(2 pi i G'.t) with t the fractional translation
     (plane_wave_basis.cpp:81-164, 250-270).
+This module hands it the inputs (`write_flat`) and assembles the System from its output.
 
 The fixtures (`tests/golden/qe_example*.npz`) hold the same data as the QE files of the
 reference's examples.
 """
 from __future__ import annotations
 
+import ctypes as C
 import dataclasses
 import json
 import math
+import os
+import tempfile
 
 import numpy as np
 
 from . import system as S
+from . import tcx
 
 HT_IN_EV = 27.21138505  # src/include/header.hpp:47
 
@@ -80,63 +86,91 @@ def load_fixture(path: str) -> QEInput:
                    setup=json.loads(str(z["setup"])), expected=json.loads(str(z["expected"])))
 
 
-def tc_rotations(qe: QEInput) -> np.ndarray:
-    """U = inverse of the QE rotation, rounded to integers (symmetry.cpp:17-33)."""
-    return np.array([np.rint(np.linalg.inv(r.astype(np.float64))).astype(np.int64) for r in qe.rot_qe])
-
-
-def _matched(v, lst):
-    """kpoints.cpp:10-26: equal modulo reciprocal-lattice vectors."""
-    for w in lst:
-        d = v - w
-        d = d - np.round(d)  # components folded to [-0.5, 0.5]
-        if float(np.sum(d * d)) < 1e-8:
-            return True
-    return False
-
-
 @dataclasses.dataclass
 class KReplica:
     ik: int        # irreducible k
     isym: int      # symmetry operation (TC++ index)
     trev: bool     # time reversal used
     kcrys: np.ndarray
+    index: int     # position in the expanded mesh
+
+
+class HostInputs:
+    """The QE inputs loaded into the C++ host (tcxh_qe_load), which expands the k-mesh
+    and runs the SCF loop; a context manager that frees the host's copy."""
+
+    def __init__(self, qe: QEInput):
+        self.L = tcx.load_library()
+        fd, path = tempfile.mkstemp(suffix=".tcxqe")
+        os.close(fd)
+        h = C.c_void_p()
+        try:
+            write_flat(qe, path)
+            st = self.L.tcxh_qe_load(path.encode(), C.byref(h))
+        finally:
+            os.unlink(path)
+        tcx.host_check(st, "tcxh_qe_load")
+        self.h = h
+        self.nsym = len(qe.rot_qe)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tcxh_qe_free(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    def replicas(self) -> list:
+        """Kpoints::set_kvectors_by_symmetry (kpoints.cpp:34-110), in index_all_kscf order."""
+        n = C.c_int()
+        tcx.host_check(self.L.tcxh_qe_num_replicas(self.h, C.byref(n)), "tcxh_qe_num_replicas")
+        ik, isym, trev = (np.zeros(n.value, np.int32) for _ in range(3))
+        kc = np.zeros((n.value, 3))
+        tcx.host_check(self.L.tcxh_qe_replicas(self.h, tcx._ptr(ik, tcx._ip), tcx._ptr(isym, tcx._ip),
+                                               tcx._ptr(trev, tcx._ip), tcx._ptr(kc)), "tcxh_qe_replicas")
+        return [KReplica(int(ik[r]), int(isym[r]), bool(trev[r]), kc[r].copy(), r) for r in range(n.value)]
+
+    def rotation(self, isym: int) -> np.ndarray:
+        """U = (QE rotation)^-1, integer (symmetry.cpp:17-33)."""
+        U = np.zeros(9, np.int32)
+        tcx.host_check(self.L.tcxh_qe_rotation(self.h, int(isym), tcx._ptr(U, tcx._ip)), "tcxh_qe_rotation")
+        return U.reshape(3, 3).astype(np.int64)
+
+    def replica_basis(self, rep: KReplica):
+        """(FFT-grid linear indices, phase factors) of the irreducible plane waves at a
+        replica (plane_wave_basis.cpp:81-164)."""
+        n = C.c_int()
+        tcx.host_check(self.L.tcxh_qe_replica_npw(self.h, rep.index, C.byref(n)), "tcxh_qe_replica_npw")
+        gidx = np.zeros(n.value, np.int32)
+        ph = np.zeros(n.value, np.complex128)
+        tcx.host_check(self.L.tcxh_qe_replica_basis(self.h, rep.index, tcx._ptr(gidx, tcx._ip), ph.ctypes.data),
+                       "tcxh_qe_replica_basis")
+        return gidx, ph
+
+
+def tc_rotations(qe: QEInput) -> np.ndarray:
+    """U = inverse of the QE rotation, integer (symmetry.cpp:17-33), per operation."""
+    with HostInputs(qe) as h:
+        return np.array([h.rotation(i) for i in range(h.nsym)])
 
 
 def replicate_kpoints(qe: QEInput) -> list:
-    """Kpoints::set_kvectors_by_symmetry (kpoints.cpp:34-110), in index_all_kscf order."""
-    U = tc_rotations(qe)
-    kcrys = [np.linalg.solve(qe.b_norm.T, k) for k in qe.kcart]  # kcart = (b1 b2 b3) kcrys
-    per_k = [[KReplica(ik, 0, False, kcrys[ik])] for ik in range(qe.nirr)]
-    vec_list = [kcrys[ik] for ik in range(qe.nirr) if qe.kweight[ik] >= 1e-8]
-    for ik in range(qe.nirr):
-        if qe.kweight[ik] < 1e-8:
-            continue
-        for isym in range(len(U)):
-            v = U[isym].T.astype(np.float64) @ kcrys[ik]
-            v[np.abs(v) < 1e-4] = 0.0
-            if not _matched(v, vec_list):
-                vec_list.append(v)
-                per_k[ik].append(KReplica(ik, isym, False, v))
-            if qe.time_reversal and not _matched(-v, vec_list):
-                vec_list.append(-v)
-                per_k[ik].append(KReplica(ik, isym, True, -v))
-    return [r for lst in per_k for r in lst]
+    """The expanded k-mesh (kpoints.cpp:34-110), in index_all_kscf order."""
+    with HostInputs(qe) as h:
+        return h.replicas()
 
 
 def replica_basis(qe: QEInput, rep: KReplica):
-    """(FFT-grid linear indices, phase factors) of the irreducible plane waves at a replica
-    (plane_wave_basis.cpp:81-164)."""
-    U = tc_rotations(qe)[rep.isym]
-    G = qe.mill[rep.ik].astype(np.int64)
-    Gs = G @ U  # rows: (U^T G)^T = G^T U
-    if rep.trev:
-        Gs = -Gs
-    phase = np.exp(2j * math.pi * (Gs.astype(np.float64) @ qe.trans[rep.isym]))
-    n1, n2, n3 = qe.grid
-    w = Gs % np.array([n1, n2, n3])
-    gidx = (w[:, 0] + n1 * (w[:, 1] + n2 * w[:, 2])).astype(np.int32)
-    return gidx, phase
+    """(FFT-grid linear indices, phase factors) of the irreducible plane waves at a replica."""
+    with HostInputs(qe) as h:
+        return h.replica_basis(rep)
 
 
 def jastrow_rpa(qe: QEInput, volume: float):
@@ -155,13 +189,15 @@ def build_system(qe: QEInput, phi_irr: list, filling_irr: np.ndarray, nocc_irr: 
     filling_irr (nirr, nb), nocc_irr (nirr,) occupied bands (f > 1e-8) per irreducible k.
     symmetric: carry the mesh symmetry, so that tcx_tc3a2a4b2b5 rotates its q2 sums from
     the irreducible q2 as the reference does; False treats every replica as independent."""
-    reps = replicate_kpoints(qe)
+    with HostInputs(qe) as h:
+        reps = h.replicas()
+        bases = [h.replica_basis(r) for r in reps]
+        U = np.array([h.rotation(i) for i in range(h.nsym)])
     recip = S.reciprocal_rows(qe.cell)
     volume = abs(float(np.linalg.det(qe.cell)))
     method = method or qe.setup["method"]
     npw, gidx_all, kfrac, nocc, fill, occ, left = [], [], [], [], [], [], []
-    for r in reps:
-        gidx, ph = replica_basis(qe, r)
+    for r, (gidx, ph) in zip(reps, bases):
         npw.append(len(gidx))
         gidx_all.append(gidx)
         kfrac.append(r.kcrys)
@@ -179,14 +215,14 @@ def build_system(qe: QEInput, phi_irr: list, filling_irr: np.ndarray, nocc_irr: 
                     nocc=np.array(nocc, np.int32), filling=np.concatenate(fill), phi_occ=np.concatenate(occ),
                     chi_occ=np.concatenate(left) if chi_irr is not None else None, method=method, A=A, F=F,
                     div_corr=True, name=qe.setup.get("source", "qe"),
-                    symmetry=mesh_symmetry(qe, reps) if symmetric else None)
+                    symmetry=mesh_symmetry(qe, reps, U) if symmetric else None)
     return sys_, reps
 
 
-def mesh_symmetry(qe: QEInput, reps: list) -> dict:
+def mesh_symmetry(qe: QEInput, reps: list, U: np.ndarray) -> dict:
     """The symmetry relation of the replicated mesh for tcx_set_kpoint_symmetry: each
-    replica built from the first replica (isym 0, the irreducible k itself) of its ik."""
-    U = tc_rotations(qe)
+    replica built from the first replica (isym 0, the irreducible k itself) of its ik;
+    U[isym] the TC++ rotations."""
     first = {}
     for i, r in enumerate(reps):
         first.setdefault(r.ik, i)
@@ -198,7 +234,7 @@ def mesh_symmetry(qe: QEInput, reps: list) -> dict:
 
 def write_flat(qe: QEInput, path: str) -> None:
     """The QE inputs and SCF settings as named little-endian records for the C++ host
-    (tc_amd/host/scf.hpp; read by tests/scf_host_test.cpp): after the 8-byte magic
+    (include/tcx_host.h, read by tcxh_qe_load): after the 8-byte magic
     "TCXQE001", each record is a 32-byte name, an 8-byte numpy type tag ("<f8", "<i4",
     "<c16"), an int64 element count and the data."""
     st = qe.setup

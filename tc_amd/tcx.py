@@ -50,6 +50,13 @@ EXPORTS = [
     "tcx_get_timing",
 ]
 
+# Every entry point declared in include/tcx_host.h (the C++ host's k-mesh expansion and SCF loop).
+HOST_EXPORTS = [
+    "tcxh_last_error", "tcxh_qe_load", "tcxh_qe_free", "tcxh_qe_num_replicas", "tcxh_qe_replicas",
+    "tcxh_qe_rotation", "tcxh_qe_replica_npw", "tcxh_qe_replica_basis", "tcxh_scf_run", "tcxh_scf_result_sizes",
+    "tcxh_scf_result_fill", "tcxh_scf_result_energy", "tcxh_scf_result_ddiff", "tcxh_scf_result_free",
+]
+
 
 class TcxError(RuntimeError):
     def __init__(self, code, msg):
@@ -118,8 +125,32 @@ def load_library():
     L.tcx_get_record_columns.argtypes = [vp, C.c_int, _dp]
     L.tcx_enable_timing.argtypes = [vp, C.c_int]
     L.tcx_get_timing.argtypes = [vp, _dp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+    L.tcxh_last_error.argtypes = []
+    L.tcxh_last_error.restype = C.c_char_p
+    L.tcxh_qe_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.tcxh_qe_free.argtypes = [vp]
+    L.tcxh_qe_free.restype = None
+    L.tcxh_qe_num_replicas.argtypes = [vp, _ip]
+    L.tcxh_qe_replicas.argtypes = [vp, _ip, _ip, _ip, _dp]
+    L.tcxh_qe_rotation.argtypes = [vp, C.c_int, _ip]
+    L.tcxh_qe_replica_npw.argtypes = [vp, C.c_int, _ip]
+    L.tcxh_qe_replica_basis.argtypes = [vp, C.c_int, _ip, vp]
+    L.tcxh_scf_run.argtypes = [vp, vp, C.c_int, C.c_int, C.POINTER(vp)]
+    L.tcxh_scf_result_sizes.argtypes = [vp] + [_ip] * 7
+    L.tcxh_scf_result_fill.argtypes = [vp, C.c_int, _dp, vp, _dp]
+    L.tcxh_scf_result_energy.argtypes = [vp, C.c_int, vp]
+    L.tcxh_scf_result_ddiff.argtypes = [vp, _dp]
+    L.tcxh_scf_result_free.argtypes = [vp]
+    L.tcxh_scf_result_free.restype = None
     _lib = L
     return L
+
+
+def host_check(st, what):
+    """Raise TcxError for a failed tcxh_ call, with the host's message."""
+    if st != TCX_OK:
+        msg = load_library().tcxh_last_error()
+        raise TcxError(st, f"{what}: {msg.decode() if msg else ''}")
 
 
 def _ptr(a, t=_dp):

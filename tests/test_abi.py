@@ -13,19 +13,34 @@ from oracle import oracle as O
 from tc_amd import system as S
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "tcx.h")).read()
-    return sorted(set(re.findall(r"\b(tcx_[a-z0-9_]+)\s*\(", text)))
+def declared_symbols(header="tcx.h", prefix="tcx_"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_and_binding_agree():
     assert declared_symbols() == sorted(tcx.EXPORTS)
+    assert declared_symbols("tcx_host.h", "tcxh_") == sorted(tcx.HOST_EXPORTS)
+
+
+def test_every_header_is_checked():
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["tcx.h", "tcx_host.h"]
 
 
 def test_library_exports_every_declared_symbol():
     L = tcx.load_library()
-    for name in declared_symbols():
+    for name in declared_symbols() + declared_symbols("tcx_host.h", "tcxh_"):
         assert hasattr(L, name), name
+
+
+def test_host_abi_errors():
+    """The host entry points report bad input through a status and tcxh_last_error."""
+    L = tcx.load_library()
+    h = ctypes.c_void_p()
+    st = L.tcxh_qe_load(b"/nonexistent/in.tcxqe", ctypes.byref(h))
+    assert st == tcx.TCX_ERR_ARG and not h.value
+    assert b"cannot open" in L.tcxh_last_error()
+    assert L.tcxh_qe_num_replicas(None, None) == tcx.TCX_ERR_ARG
 
 
 def test_library_is_gfx950_only():

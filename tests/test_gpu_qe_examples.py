@@ -201,19 +201,19 @@ def _check_run(qe, ex, prec, label, fermi, eigs, energies, ddiff, iterations, co
 @pytest.mark.parametrize("ex", [15, 16, 14])
 def test_scf_run_matches_reference(engine, ex, prec):
     """The whole SCF run of the example (max_num_iterations Davidson calls, density mixing
-    for example14/15, density-matrix mixing with old bands for example16) through the
-    Python harness (tc_amd.scf)."""
+    for example14/15, density-matrix mixing with old bands for example16): the C++ host's
+    loop through its C ABI (tcxh_scf_run, called by tc_amd.scf)."""
     qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
     r = scf.run(engine, qe, precision=prec)
     _check_run(qe, ex, prec, "python", [f.fermi for f in r.fillings], [f.eigenvalues for f in r.fillings],
                r.energies, r.density_difference, r.iterations, r.converged)
 
 
-@pytest.mark.parametrize("prec", [tcx.PREC_FP64, tcx.PREC_FP32])
-@pytest.mark.parametrize("ex", [15, 16, 14])
+@pytest.mark.parametrize("prec", [tcx.PREC_FP64])
+@pytest.mark.parametrize("ex", [16])
 def test_cpp_host_scf_matches_reference(ex, prec, tmp_path):
-    """The same runs through the C++ host (tc_amd/host/scf.cpp: Diagonalization::scf with
-    the symmetry expansion in C++), driven by tests/scf_host_test.cpp in a child process."""
+    """The same loop driven from C++: tests/scf_host_test.cpp calls
+    tcx_host::Diagonalization::scf directly, in a child process."""
     import subprocess
     binp = os.path.join(os.path.dirname(HERE), "tests", "_bin", "scf_host_test")
     if not os.path.exists(binp):
@@ -272,46 +272,25 @@ def test_cpp_host_mirror_tc3a2a4b2b5_with_symmetry(engine, tmp_path):
     assert gap > 1e-6
 
 
-def _run_cpp(qe, prec, tmp_path):
-    import subprocess
-    binp = os.path.join(os.path.dirname(HERE), "tests", "_bin", "scf_host_test")
-    if not os.path.exists(binp):
-        pytest.fail(f"{binp} is not built (run __graft_entry__.build())")
-    path = str(tmp_path / "in.tcxqe")
-    Q.write_flat(qe, path)
-    res = subprocess.run([binp, path, str(prec)], capture_output=True, text=True, timeout=300)
-    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-2000:]
-    energies, ddiff, done = [], [], None
-    for ln in res.stdout.splitlines():
-        t = ln.split()
-        if t[0] == "ENERGY":
-            v = [float(x) for x in t[1:]]
-            energies.append(np.array([complex(v[2 * i], v[2 * i + 1]) for i in range(4)]))
-        elif t[0] == "DDIFF":
-            ddiff.append(float(t[1]))
-        elif t[0] == "DONE":
-            done = (int(t[1]), bool(int(t[2])))
-    return energies, ddiff, done
-
-
-def test_scf_drivers_converge_alike(engine, tmp_path):
+def test_scf_convergence_branch(engine):
     """The convergence branch of Diagonalization::scf (diagonalization.cpp:186-227: energy and
     charge converged in two consecutive iterations ends the loop without the final
     set_filling), which none of the reference's examples reach: example15 with loose
-    tolerances converges, and the C++ host and the Python harness take the same
-    trajectory and stop at the same iteration."""
+    tolerances converges before max_num_iterations, with one energy and one density
+    difference per iteration after the first and none after the last."""
     qe = Q.load_fixture(os.path.join(HERE, "golden", "qe_example15.npz"))
     qe.setup.update({"max_num_iterations": 12, "energy_tolerance": 5e-3, "charge_tolerance": 0.8})
     r = scf.run(engine, qe, precision=tcx.PREC_FP64)
-    energies, ddiff, done = _run_cpp(qe, tcx.PREC_FP64, tmp_path)
-    print(f"python: {r.iterations} iterations, converged {r.converged}, dE "
-          f"{[f'{abs(b[3] - a[3]):.1e}' for a, b in zip(r.energies, r.energies[1:])]}, ddiff "
-          f"{[f'{x:.3f}' for x in r.density_difference]}; c++: {done}", flush=True)
-    assert r.converged and done == (r.iterations, True)
-    assert r.iterations < 12
-    # converged in iteration it: energies of iterations 1..it-1, no final one
-    assert len(r.energies) == r.iterations and len(energies) == len(r.energies)
-    assert len(ddiff) == len(r.density_difference) == r.iterations
-    assert np.abs(np.array(energies) - np.array(r.energies)).max() < 1e-9
-    assert np.abs(np.array(ddiff) - np.array(r.density_difference)).max() < 1e-9
-    assert abs(r.energies[-1][3] - r.energies[-2][3]) < 5e-3 and r.density_difference[-1] < 0.8
+    dE = [abs(b[3] - a[3]) for a, b in zip(r.energies, r.energies[1:])]
+    print(f"{r.iterations} iterations, converged {r.converged}, dE {[f'{x:.1e}' for x in dE]}, ddiff "
+          f"{[f'{x:.3f}' for x in r.density_difference]}", flush=True)
+    assert r.converged and r.iterations < 12
+    # converged in iteration it: one energy per iteration 1..it and no final one
+    assert len(r.energies) == r.iterations and len(r.density_difference) == r.iterations
+    assert len(r.fillings) == r.iterations + 1
+    # the energy and charge checks passed in the last two iterations; the loop did not stop
+    # one iteration earlier
+    etol, ctol, dd = 5e-3, 0.8, r.density_difference
+    assert dE[-1] < etol and dE[-2] < etol and dd[-1] < ctol and dd[-2] < ctol
+    if len(dE) >= 3:
+        assert not (dE[-2] < etol and dE[-3] < etol and dd[-2] < ctol and dd[-3] < ctol)

@@ -71,24 +71,27 @@ def test_initial_fermi_energy_matches_reference(ex):
 
 
 @pytest.mark.parametrize("ex", [14, 15, 16])
-def test_cpp_host_replication_matches(ex, tmp_path):
-    """The C++ host (tc_amd/host/scf.cpp) builds the same symmetry-expanded k-mesh and
-    replica plane-wave indices / phases as tc_amd.qe (no GPU: the program's replicas mode)."""
-    import subprocess
-    binp = os.path.join(os.path.dirname(HERE), "tests", "_bin", "scf_host_test")
-    if not os.path.exists(binp):
-        pytest.skip("scf_host_test not built (run tc_amd/build.py)")
-    qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
-    path = str(tmp_path / "in.tcxqe")
-    Q.write_flat(qe, path)
-    res = subprocess.run([binp, path, "replicas"], capture_output=True, text=True, timeout=120)
-    assert res.returncode == 0, res.stderr
-    rows = [ln.split()[1:] for ln in res.stdout.splitlines() if ln.startswith("REP")]
-    reps = Q.replicate_kpoints(qe)
-    assert len(rows) == len(reps)
-    for row, r in zip(rows, reps):
-        gidx, ph = Q.replica_basis(qe, r)
-        assert (int(row[0]), int(row[1]), int(row[2])) == (r.ik, r.isym, int(r.trev))
-        assert np.allclose([float(x) for x in row[3:6]], r.kcrys, atol=1e-12)
-        assert int(row[6]) == len(gidx) and int(row[7]) == int(gidx.astype(np.int64).sum())
-        assert abs(complex(float(row[8]), float(row[9])) - ph.sum()) < 1e-9
+def test_host_rotations_and_replica_bases(ex):
+    """The C++ host's rotations and replica plane waves against a numpy restatement of
+    symmetry.cpp:17-33 and plane_wave_basis.cpp:81-164: U = (QE rotation)^-1; at a replica,
+    G' = +-U^T G of every irreducible plane wave, folded onto the grid, with phase
+    exp(2 pi i G'.tau).  The replica's k itself is U^T k (or its negative) modulo the
+    reciprocal lattice."""
+    qe = fixture(ex)
+    U = Q.tc_rotations(qe)
+    for i, r in enumerate(qe.rot_qe):
+        assert np.array_equal(U[i] @ r.astype(np.int64), np.eye(3, dtype=np.int64)), i
+    n = np.array(qe.grid)
+    kcrys = [np.linalg.solve(qe.b_norm.T, k) for k in qe.kcart]
+    with Q.HostInputs(qe) as h:
+        reps = h.replicas()
+        assert [r.index for r in reps] == list(range(len(reps)))
+        for r in reps:
+            gidx, ph = h.replica_basis(r)
+            Gs = (-1 if r.trev else 1) * (qe.mill[r.ik].astype(np.int64) @ U[r.isym])
+            w = Gs % n
+            assert np.array_equal(gidx, w[:, 0] + n[0] * (w[:, 1] + n[1] * w[:, 2])), r.index
+            assert np.abs(ph - np.exp(2j * np.pi * (Gs @ qe.trans[r.isym]))).max() < 1e-12, r.index
+            v = (-1 if r.trev else 1) * (U[r.isym].T @ kcrys[r.ik])
+            d = v - r.kcrys
+            assert np.abs(d - np.round(d)).max() < 1e-4, r.index
