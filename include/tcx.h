@@ -70,6 +70,9 @@ int tcx_synchronize(tcx_context *ctx);
  * (src/plane_wave_basis.cpp:175-198), Spin::num_independent_spins and
  * Method::calc_method.
  *
+ * grid: n1 = n2 and n1, n3 in {6, 8, 10, 12, 16, 18, 20, 24, 30, 32, 36, 40, 48}
+ * (the compiled FFT lengths), else TCX_ERR_UNSUPPORTED.
+ *
  * nspin = 1 or 2 (num_independent_spins).  With 2, every per-k array of the
  * states and of the term calls (nocc, filling, phi_occ, chi_occ, the applied
  * band counts nb and the coefficient arrays) runs over (spin, k), spin-major:

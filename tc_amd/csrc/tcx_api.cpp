@@ -130,7 +130,7 @@ void Context::set_cell(const double recip_rows[9], double vol, const int grid[3]
     require(ns == 1 || ns == 2, TCX_ERR_ARG, "set_cell: num_independent_spins must be 1 or 2");
     require(grid[0] == grid[1], TCX_ERR_UNSUPPORTED, "set_cell: grid must have n1 == n2");
     require(size_supported(grid[0]) && size_supported(grid[2]), TCX_ERR_UNSUPPORTED,
-            "set_cell: grid size not compiled (supported: 6 8 12 16 20 24 32 36 40 48)");
+            "set_cell: grid size not compiled (supported: 6 8 10 12 16 18 20 24 30 32 36 40 48)");
     geo.n1 = grid[0];
     geo.n2 = grid[1];
     geo.n3 = grid[2];

@@ -32,7 +32,7 @@ int scratch_setup(int device);
 size_t scratch_bytes();
 
 // Grid sizes (per axis) with compiled kernels.
-constexpr int kSizes[] = {6, 8, 12, 16, 20, 24, 32, 36, 40, 48};
+constexpr int kSizes[] = {6, 8, 10, 12, 16, 18, 20, 24, 30, 32, 36, 40, 48};
 bool size_supported(int n);
 
 // Geometry passed by value to kernels.
