@@ -136,7 +136,7 @@ def test_errors_are_status_codes(engine):
         engine.hf2x(np.array([1, 1], np.int32), np.zeros(sys_.npw.sum(), np.complex128), precision=7)
     assert ei.value.code == tcx.TCX_ERR_ARG
     with pytest.raises(tcx.TcxError) as ei:
-        engine.set_cell(sys_.recip, sys_.volume, (10, 10, 10))
+        engine.set_cell(sys_.recip, sys_.volume, (14, 14, 14))  # 14 = 2 x 7: no compiled length
     assert ei.value.code == tcx.TCX_ERR_UNSUPPORTED
     engine.set_kpoints(sys_.kfrac)  # invalidates basis and states
     with pytest.raises(tcx.TcxError) as ei:
