@@ -173,6 +173,8 @@ SCFResult Diagonalization::scf(tcx_context* ctx, const QEInputs& qe) const
         F[1] = F[2] = std::sqrt(a);
     }
     check(ctx, tcx_set_jastrow_rpa(ctx, A, F), "set_jastrow_rpa");
+    // the run's Jastrow is the RPA form alone: clear a polynomial part left on the context
+    check(ctx, tcx_set_jastrow_polynomial(ctx, 0, 0, nullptr, nullptr, 0), "set_jastrow_polynomial");
     double alpha = 0.0;
     std::vector<double> sv(nk);
     check(nullptr, tcx_sum_of_vaux(recip, volume, nk, kfrac.data(), &alpha, sv.data()), "sum_of_vaux");

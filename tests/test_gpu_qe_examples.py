@@ -20,6 +20,7 @@ import pytest
 
 from tc_amd import qe as Q
 from tc_amd import scf
+from tc_amd import system as S
 from tc_amd import tcx
 
 pytestmark = pytest.mark.gpu
@@ -48,6 +49,15 @@ def test_first_scf_iteration_matches_reference(engine, ex, prec):
     assert d_eig < tol
     assert d_e.max() < tol
     assert d_ef < tol
+
+
+def test_scf_run_clears_a_polynomial_jastrow_left_on_the_context(engine):
+    """The SCF run sets its whole Jastrow (the RPA form): a polynomial part that an earlier
+    caller left on the shared context must not enter it (example15, TC, FP64, against the
+    reference's printed first iteration)."""
+    engine.load_system(S.tiny_system(method="TC", n=(8, 8, 8)))
+    engine.set_jastrow_polynomial(2, 3, np.full((2, 2), 2.0), np.full((2, 2, 2), 0.3), False)
+    test_first_scf_iteration_matches_reference(engine, 15, tcx.PREC_FP64)
 
 
 def _tc_system_from(ex, symmetric):
