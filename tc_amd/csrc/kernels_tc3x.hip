@@ -589,10 +589,13 @@ __global__ __launch_bounds__(256) void k_pgp(const double2* __restrict__ chiR, c
     pgp[2 * N + r] = a2;
 }
 
-// tc2h.cpp:146-171 (one independent spin, spin_factor 2):
-//   V2(G) = sum_js [tc_2body(G,0,js) n_js(G) + u(G,0,js) sum_d G_d pgp_d(G) / 2]
+// tc2h.cpp:146-171 for spin s (j0 = u(s, 0), j1 = u(s, 1)):
+//   V2(G) = sum_js [tc_2body(G,s,js) n_js(G) + u(G,s,js) sum_d G_d pgp_js,d(G) / spin_factor]
+// pgp0/pgp1: the <q|nabla|q> sums of spin 0 / 1 (one independent spin: both the one sum,
+// pf = 1 / spin_factor = 1/2; two: pf = 1)
 template <bool kPoly>
-__global__ __launch_bounds__(256) void k_tc2h_V(const double2* __restrict__ nG, const double2* __restrict__ pgpG,
+__global__ __launch_bounds__(256) void k_tc2h_V(const double2* __restrict__ nG, const double2* __restrict__ pgp0,
+                                                const double2* __restrict__ pgp1, double pf,
                                                 double2* __restrict__ V, Geom g, long long N, JastrowPair j0,
                                                 JastrowPair j1)
 {
@@ -602,8 +605,9 @@ __global__ __launch_bounds__(256) void k_tc2h_V(const double2* __restrict__ nG, 
     const double x2 = kq_plus_G(g, r, nullptr, x);
     const double2 n0 = nG[r], n1 = nG[N + r];
     double2 v = dadd(dscale(n0, jastrow_tc2body<kPoly>(j0, x2)), dscale(n1, jastrow_tc2body<kPoly>(j1, x2)));
-    const double us = jastrow_uk<kPoly>(j0, x2) + jastrow_uk<kPoly>(j1, x2);
-    for (int d = 0; d < 3; ++d) v = dadd(v, dscale(pgpG[(long long)d * N + r], 0.5 * us * x[d]));
+    const double u0 = pf * jastrow_uk<kPoly>(j0, x2), u1 = pf * jastrow_uk<kPoly>(j1, x2);
+    for (int d = 0; d < 3; ++d)
+        v = dadd(v, dadd(dscale(pgp0[(long long)d * N + r], u0 * x[d]), dscale(pgp1[(long long)d * N + r], u1 * x[d])));
     V[r] = v;
 }
 
@@ -615,12 +619,12 @@ void launch_tc2h_V(const double* chiR, const double* gradR, const double* fill, 
     TCX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_tc2h_combine(const double* nG, const double* pgpG, double* V, const Geom& g, long long N,
-                         const JastrowPair& j0, const JastrowPair& j1, hipStream_t s)
+void launch_tc2h_combine(const double* nG, const double* pgp0, const double* pgp1, double pf, double* V,
+                         const Geom& g, long long N, const JastrowPair& j0, const JastrowPair& j1, hipStream_t s)
 {
     TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_tc2h_V<kPoly>), grid_pts(N, 1), dim3(256), 0, s,
-                                                      (const double2*)nG, (const double2*)pgpG, (double2*)V, g, N,
-                                                      j0, j1));
+                                                      (const double2*)nG, (const double2*)pgp0, (const double2*)pgp1,
+                                                      pf, (double2*)V, g, N, j0, j1));
     TCX_HIP_CHECK(hipGetLastError());
 }
 

@@ -235,8 +235,9 @@ void launch_kinetic(const double* phi, const int* gindex, const OrbEntry* jobs, 
 void launch_hartree(const double* nG, double* V, const Geom& g, long long N, hipStream_t s);
 void launch_tc2h_V(const double* chiR, const double* gradR, const double* fill, int nocc, double* pgp,
                    const Geom& g, long long N, hipStream_t s);
-void launch_tc2h_combine(const double* nG, const double* pgpG, double* V, const Geom& g, long long N,
-                         const JastrowPair& j0, const JastrowPair& j1, hipStream_t s);
+// pgp0 / pgp1: the <q|nabla|q> sums (G space) of spin 0 / 1; pf = 1 / spin_factor
+void launch_tc2h_combine(const double* nG, const double* pgp0, const double* pgp1, double pf, double* V,
+                         const Geom& g, long long N, const JastrowPair& j0, const JastrowPair& j1, hipStream_t s);
 void launch_tc2h_apply(double* orb, const double* grad, const double* V2, const double* dnu, int nb, long long N,
                        hipStream_t s);
 void launch_axpby(const double* X, const double* Y, double* out, double a, double b, long long N, hipStream_t s);

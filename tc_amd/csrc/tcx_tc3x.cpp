@@ -573,7 +573,6 @@ void Context::hf2h(const int* nb_, const double* phi, double* out, unsigned flag
 void Context::tc2h(const int* nb_, const double* phi, double* out, unsigned flags)
 {
     check_apply_args(nb_, phi, out, TCX_PREC_FP64);
-    require_one_spin("tc2h");
     require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc2h: the cell was set up for HF");
     require(jastrow_set, TCX_ERR_STATE, "tc2h: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);
@@ -582,44 +581,56 @@ void Context::tc2h(const int* nb_, const double* phi, double* out, unsigned flag
     if (P.n_app > 0 && n_occ > 0) {
         const bool bitc = method == TCX_METHOD_BITC;
         const size_t g = (size_t)N * 16;
-        const JastrowPair j00 = jastrow_pair(0, 0), j01 = jastrow_pair(0, 1);
         ensure_occR(TCX_PREC_FP64);
         ensure_gradR(TCX_PREC_FP64);
         if (bitc) ensure_chiR(TCX_PREC_FP64);
         const double* chiR = (const double*)(bitc ? d_chiR[TCX_PREC_FP64].p : d_occR[TCX_PREC_FP64].p);
-        c_tc2h.buf.ensure(4 * g);
-        double* V2 = (double*)c_tc2h.buf.p;  // 1 grid
-        double* dnu = V2 + 2 * N;            // 3 grids
+        const double* gradR = (const double*)d_gradR[TCX_PREC_FP64].p;
+        // per spin s: V2_s (1 grid) and dnu_s (3 grids), 4 consecutive grids
+        c_tc2h.buf.ensure((size_t)4 * nspin * g);
+        auto V2_of = [&](int s) { return (double*)c_tc2h.buf.p + (size_t)s * 8 * N; };
         if (!c_tc2h.valid) {
-            d_local.ensure(5 * g);
-            double* nG = (double*)d_local.p;   // 2 grids
-            double* pgp = nG + 4 * N;          // 3 grids
-            // phiq_gphiq (tc2h.cpp:60-127)
-            launch_tc2h_V(chiR, (const double*)d_gradR[TCX_PREC_FP64].p, (const double*)d_fill.p, n_occ, pgp, geo,
-                          N, stream);
-            launch_fft3d(TCX_PREC_FP64, -1, pgp, 3, geo, 1.0 / (double)N, stream);
+            d_local.ensure((size_t)(2 + 3 * nspin) * g);
+            double* nG = (double*)d_local.p;  // 2 grids: n_0, n_1
+            double* pgp = nG + 4 * N;         // 3 grids per spin
+            // phiq_gphiq of each spin from its occupied states (tc2h.cpp:60-127)
+            for (int s = 0; s < nspin; ++s) {
+                const long long b0 = occ_first[(size_t)s * nk_phys], b1 = occ_first[(size_t)(s + 1) * nk_phys];
+                launch_tc2h_V(chiR + 2 * b0 * N, gradR + 6 * b0 * N, (const double*)d_fill.p + b0, (int)(b1 - b0),
+                              pgp + (size_t)s * 6 * N, geo, N, stream);
+            }
+            launch_fft3d(TCX_PREC_FP64, -1, pgp, 3 * nspin, geo, 1.0 / (double)N, stream);
             density_G(nG);
-            launch_tc2h_combine(nG, pgp, V2, geo, N, j00, j01, stream);  // tc2h.cpp:146-171
-            launch_dnu(nG, dnu, geo, j00, j01, stream);
-            launch_fft3d(TCX_PREC_FP64, +1, V2, 1, geo, 1.0, stream);
-            launch_fft3d(TCX_PREC_FP64, +1, dnu, 3, geo, 1.0, stream);
+            const double* pgp1 = nspin == 1 ? pgp : pgp + 6 * N;  // one spin: jspin_ref = 0
+            const double pf = nspin == 1 ? 0.5 : 1.0;             // 1 / spin_factor (tc2h.cpp:144)
+            for (int s = 0; s < nspin; ++s) {
+                const JastrowPair js0 = jastrow_pair(s, 0), js1 = jastrow_pair(s, 1);
+                launch_tc2h_combine(nG, pgp, pgp1, pf, V2_of(s), geo, N, js0, js1, stream);  // tc2h.cpp:146-171
+                launch_dnu(nG, V2_of(s) + 2 * N, geo, js0, js1, stream);
+            }
+            launch_fft3d(TCX_PREC_FP64, +1, V2_of(0), 4 * nspin, geo, 1.0, stream);
             c_tc2h.valid = true;
         }
-        // per band: FFT_forward[V2 phi_j - sum_d gphi_j,d dnu_d] (tc2h.cpp:178-234)
+        // per band: FFT_forward[V2_s phi_j - sum_d gphi_j,d dnu_s,d] (tc2h.cpp:178-234), the
+        // bands of spin s being the jobs of its k-blocks
         const double factor = 1.0 / (num_kpoints * volume);  // tc2h.cpp:32
         const int chunk = (int)std::max<size_t>(1, std::min<size_t>(P.n_app, ((size_t)1 << 30) / (4 * g)));
         d_W.ensure((size_t)chunk * g);
         d_appGrad.ensure((size_t)3 * chunk * g);
         const OrbEntry* jobs = (const OrbEntry*)P.d_appjobs.p;
         const OrbEntry* gjobs = (const OrbEntry*)P.d_appgradjobs.p;
-        for (int c0 = 0; c0 < P.n_app; c0 += chunk) {
-            const int nc = std::min(chunk, P.n_app - c0);
-            double* Wc = (double*)d_W.p;
-            orbitals_to_real(io.phi, jobs + c0, nc, TCX_PREC_FP64, Wc);
-            orbitals_to_real(io.phi, gjobs + 3 * c0, 3 * nc, TCX_PREC_FP64, d_appGrad.p);
-            launch_tc2h_apply(Wc, (const double*)d_appGrad.p, V2, dnu, nc, N, stream);
-            launch_fft3d(TCX_PREC_FP64, -1, Wc, nc, geo, 1.0 / (double)N, stream);
-            launch_gather_add(Wc, (const int*)d_gindex.p, jobs + c0, nc, max_npw, io.out, N, factor, stream);
+        for (int s = 0; s < nspin; ++s) {
+            const int a0 = (int)P.app_first[(size_t)s * nk_phys], a1 = (int)P.app_first[(size_t)(s + 1) * nk_phys];
+            const double* V2 = V2_of(s);
+            for (int c0 = a0; c0 < a1; c0 += chunk) {
+                const int nc = std::min(chunk, a1 - c0);
+                double* Wc = (double*)d_W.p;
+                orbitals_to_real(io.phi, jobs + c0, nc, TCX_PREC_FP64, Wc);
+                orbitals_to_real(io.phi, gjobs + 3 * c0, 3 * nc, TCX_PREC_FP64, d_appGrad.p);
+                launch_tc2h_apply(Wc, (const double*)d_appGrad.p, V2, V2 + 2 * N, nc, N, stream);
+                launch_fft3d(TCX_PREC_FP64, -1, Wc, nc, geo, 1.0 / (double)N, stream);
+                launch_gather_add(Wc, (const int*)d_gindex.p, jobs + c0, nc, max_npw, io.out, N, factor, stream);
+            }
         }
     }
     end_apply(io);

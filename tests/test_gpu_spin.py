@@ -11,7 +11,9 @@ spin densities (hf2h.cpp:30).  Pins:
     one-spin oracle on that channel's states with fillings 2 f_s (and u(s, s) moved to
     u(0, 0) for tc2x), the Hartree term the one-spin oracle on both channels' states with
     fillings f_s (FP64 1e-10, FP32-split per band 3e-5);
-  * the terms that couple the spins otherwise are refused with TCX_ERR_UNSUPPORTED.
+  * tc2h, which couples the spins through u(s, s'), equals per channel a sum of one-spin
+    oracle applies, one per spin's states (FP64 1e-10);
+  * the three-body terms are refused with TCX_ERR_UNSUPPORTED.
 """
 import dataclasses
 
@@ -57,7 +59,7 @@ def test_identical_channels_match_one_spin(engine, method):
     s1 = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 2, 1), nbands=3, nocc=2, seed_stream=151)
     nb0, phi0 = S.applied_bands(s1, nb=[3, 0, 2, 1], seed=152)
     engine.load_system(s1)
-    terms = ["kinetic", "hf2h", "hf2x"] + (["tc2x"] if method != "HF" else [])
+    terms = ["kinetic", "hf2h", "hf2x"] + (["tc2x", "tc2h"] if method != "HF" else [])
     one = {t: getattr(engine, t)(nb0, phi0) for t in terms}
     s2 = two_spin(s1, s1)
     engine.load_system(s2)
@@ -127,6 +129,34 @@ def test_channels_against_oracle(engine, method):
         assert err <= 1e-10, s
 
 
+@pytest.mark.parametrize("method", ["TC", "BITC"])
+def test_tc2h_channels_against_oracle(engine, method):
+    """tc2h with two spins (tc2h.cpp:133-171, spin_factor 1): channel s's potential is
+    sum_js [tc_2body(s,js) n_js + u(s,js) G.pgp_js] and its dnu sum_js u(s,js) G n_js, each
+    js term linear in the states of spin js.  A one-spin system holding spin js's states
+    with their fillings and u(s, js) in every Jastrow slot gives exactly that js term, so
+    channel s is the sum over js of two one-spin oracle applies (FP64, 1e-10)."""
+    A = np.array([[0.5, 0.4], [0.4, 0.8]])
+    sa = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 1, 1), nbands=4, nocc=3, seed_stream=191, A=A)
+    sb = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 1, 1), nbands=4, nocc=2, seed_stream=192, A=A)
+    fa = np.tile([1.0, 1.0, 0.6], sa.nk)
+    fb = np.tile([1.0, 0.3], sb.nk)
+    engine.load_system(two_spin(sa, sb, fa, fb))
+    nb0, phi0, nb1, phi1, nb2, phi2 = both_channels(sa, [3, 2], 193)
+    got = engine.tc2h(nb2, phi2)
+    states = [(sa, fa), (sb, fb)]
+    n0 = phi0.size
+    for s, (nbs, phis, lo, hi) in enumerate(((nb0, phi0, 0, n0), (nb1, phi1, n0, phi2.size))):
+        ref = np.zeros_like(phis)
+        for js, (sys_js, f_js) in enumerate(states):
+            one = dataclasses.replace(with_states(sys_js, sys_js.nocc, f_js, sys_js.phi_occ, sys_js.chi_occ),
+                                      A=np.full((2, 2), sa.A[s, js]), F=np.full((2, 2), sa.F[s, js]))
+            ref = ref + O.Oracle(one).tc2h(nbs, phis)
+        err = np.abs(got[lo:hi] - ref).max() / np.abs(ref).max()
+        print(f"{method} tc2h spin {s}: {err:.2e}", flush=True)
+        assert err <= 1e-10, s
+
+
 def test_hf_all_is_the_sum_of_its_terms(engine):
     sa = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=171)
     sb = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=1, seed_stream=172)
@@ -144,10 +174,10 @@ def test_spin_coupling_terms_are_refused(engine):
     engine.load_system(two_spin(s1, s1))
     nb, phi = S.applied_bands(s1, nb=[1, 1], seed=182)
     nb2, phi2 = np.concatenate([nb, nb]).astype(np.int32), np.concatenate([phi, phi])
-    for t in ("tc3a1", "tc3b1", "tc3a3b3b4b6", "tc3a2a4b2b5", "tc2h"):
+    for t in ("tc3a1", "tc3b1", "tc3a3b3b4b6", "tc3a2a4b2b5"):
         with pytest.raises(tcx.TcxError) as e:
             getattr(engine, t)(nb2, phi2)
         assert e.value.code == tcx.TCX_ERR_UNSUPPORTED, t
     # the two-body terms still run on the same context
-    assert np.isfinite(engine.tc2x(nb2, phi2)).all()
+    assert np.isfinite(engine.tc2x(nb2, phi2)).all() and np.isfinite(engine.tc2h(nb2, phi2)).all()
     engine.load_system(s1)  # one spin again
