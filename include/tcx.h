@@ -79,10 +79,11 @@ int tcx_synchronize(tcx_context *ctx);
  * 2 nk entries, spin 0's k-blocks first.  The k-points, the basis and
  * sum_of_vaux are given once for the nk k-points.  kinetic, hf2h (the sum of
  * the two spin densities), hf2x and tc2x (states of one spin paired, u(s, s),
- * no spin-sum halving of the prefactors), tc2h (u(s, s') with each spin's
- * density and <q|nabla|q> sum) and the HF all / Davidson run with two spins;
- * the three-body terms, the k-mesh symmetry, zero-weight k-points, the
- * gradients and tcx_hf2x_noalloc return TCX_ERR_UNSUPPORTED.
+ * no spin-sum halving of the prefactors), tc2h, tc3a1 and tc3b1 (u(s, s')
+ * with each spin's density and <q|nabla|q> sum) and the HF all / Davidson run
+ * with two spins; the three-body exchange terms, the k-mesh symmetry,
+ * zero-weight k-points, the gradients and tcx_hf2x_noalloc return
+ * TCX_ERR_UNSUPPORTED.
  * Changing nspin clears the k-points: call tcx_set_kpoints again. */
 int tcx_set_cell(tcx_context *ctx, const double recip_rows[9], double volume,
                  const int grid[3], int nspin, int method);

@@ -61,10 +61,12 @@ __global__ __launch_bounds__(256) void k_dnu(const double2* __restrict__ nG, dou
     for (int d = 0; d < 3; ++d) out[(long long)d * N + e] = make_double2(G[d] * sx, G[d] * sy);
 }
 
-// V(G) = sum_{s2} sum_d u(G, 0, s2) G_d dnu_d(G)  (tc3b1.cpp:86-100; one independent spin)
+// V(G) = sum_{s2} sum_d u(G, s1, s2) G_d w_s2,d(G)  (tc3b1.cpp:83-100): dnu = w_0, dnu1 = w_1, or
+// dnu1 = nullptr for one independent spin (s2_ref = 0: both s2 read w_0)
 template <bool kPoly>
-__global__ __launch_bounds__(256) void k_tc3b1_V(const double2* __restrict__ dnu, double2* __restrict__ V, Geom g,
-                                                 JastrowPair j0, JastrowPair j1, long long N)
+__global__ __launch_bounds__(256) void k_tc3b1_V(const double2* __restrict__ dnu, const double2* __restrict__ dnu1,
+                                                 double2* __restrict__ V, Geom g, JastrowPair j0, JastrowPair j1,
+                                                 long long N)
 {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= N) return;
@@ -75,12 +77,21 @@ __global__ __launch_bounds__(256) void k_tc3b1_V(const double2* __restrict__ dnu
         G[d] = g.b[d] * m1 + g.b[3 + d] * m2 + g.b[6 + d] * m3;
         G2 += G[d] * G[d];
     }
-    const double u = jastrow_uk<kPoly>(j0, G2) + jastrow_uk<kPoly>(j1, G2);
+    const double u0 = jastrow_uk<kPoly>(j0, G2), u1 = jastrow_uk<kPoly>(j1, G2);
     double re = 0.0, im = 0.0;
-    for (int d = 0; d < 3; ++d) {
-        const double2 x = dnu[(long long)d * N + e];
-        re += u * G[d] * x.x;
-        im += u * G[d] * x.y;
+    if (dnu1 == nullptr) {  // one independent spin: both s2 read the one sum
+        const double u = u0 + u1;
+        for (int d = 0; d < 3; ++d) {
+            const double2 x = dnu[(long long)d * N + e];
+            re += u * G[d] * x.x;
+            im += u * G[d] * x.y;
+        }
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            const double2 x = dnu[(long long)d * N + e], y = dnu1[(long long)d * N + e];
+            re += G[d] * (u0 * x.x + u1 * y.x);
+            im += G[d] * (u0 * x.y + u1 * y.y);
+        }
     }
     V[e] = make_double2(re, im);
 }
@@ -149,12 +160,13 @@ void launch_dnu(const double* nG, double* out, const Geom& g, const JastrowPair&
     TCX_HIP_CHECK(hipGetLastError());
 }
 
-void launch_tc3b1_V(const double* dnu, double* V, const Geom& g, const JastrowPair& j0, const JastrowPair& j1,
-                    hipStream_t s)
+void launch_tc3b1_V(const double* dnu, const double* dnu1, double* V, const Geom& g, const JastrowPair& j0,
+                    const JastrowPair& j1, hipStream_t s)
 {
     const long long N = (long long)g.n1 * g.n2 * g.n3;
     TCX_POLY_DISPATCH(j0.pdeg > 0, hipLaunchKernelGGL((k_tc3b1_V<kPoly>), grid1(N), dim3(256), 0, s,
-                                                      (const double2*)dnu, (double2*)V, g, j0, j1, N));
+                                                      (const double2*)dnu, (const double2*)dnu1, (double2*)V, g,
+                                                      j0, j1, N));
     TCX_HIP_CHECK(hipGetLastError());
 }
 

@@ -166,8 +166,8 @@ void launch_density(const double* occR, const double* chiR, const double* fill, 
 void launch_jastrow_eval(const double* G2, double* uk, double* v, long long n, const JastrowPair& j, hipStream_t s);
 void launch_dnu(const double* nG, double* out, const Geom& g, const JastrowPair& j0, const JastrowPair& j1,
                 hipStream_t s);
-void launch_tc3b1_V(const double* dnu, double* V, const Geom& g, const JastrowPair& j0, const JastrowPair& j1,
-                    hipStream_t s);
+void launch_tc3b1_V(const double* dnu, const double* dnu1, double* V, const Geom& g, const JastrowPair& j0,
+                    const JastrowPair& j1, hipStream_t s);
 void launch_sum_squares(const double* dnu, double* V, long long N, hipStream_t s);
 void launch_mul_shared(double* grids, int ngrids, const double* V, long long N, hipStream_t s);
 

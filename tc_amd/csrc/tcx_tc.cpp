@@ -98,7 +98,6 @@ void Context::ensure_density()
 void Context::tc3_local(int which, const int* nb_, const double* phi, double* out, int prec, unsigned flags)
 {
     check_apply_args(nb_, phi, out, prec);
-    require_one_spin(which == 0 ? "tc3a1" : "tc3b1");
     require(method != TCX_METHOD_HF, TCX_ERR_STATE, "tc3: the cell was set up for HF");
     require(jastrow_set, TCX_ERR_STATE, "tc3: call tcx_set_jastrow_rpa first");
     std::vector<int> nb(nb_, nb_ + nk);
@@ -110,44 +109,61 @@ void Context::tc3_local(int which, const int* nb_, const double* phi, double* ou
         // tc3a1.cpp:26-28, tc3b1.cpp:26-28
         const double factor = which == 0 ? 0.5 / (NkV * NkV) : -1.0 / (NkV * NkV);
         LocalCache& C = which == 0 ? c_tc3a1 : c_tc3b1;
+        C.buf.ensure((size_t)nspin * g);  // the local potential of each spin s1, real space
         if (!C.valid) {
             ensure_density();
-            d_work.ensure(5 * g);
-            double* nG = (double*)d_work.p;                    // 2 grids
-            double* dnu = (double*)((char*)d_work.p + 2 * g);  // 3 grids
-            C.buf.ensure(g);
-            double* Vc = (double*)C.buf.p;                     // the local potential, real space
+            d_work.ensure((size_t)(2 + 3 * nspin) * g);
+            double* nG = (double*)d_work.p;                    // 2 grids: n_0, n_1
+            double* dnu = (double*)((char*)d_work.p + 2 * g);  // 3 grids per spin
             TCX_HIP_CHECK(hipMemcpyAsync(nG, d_density.p, 2 * g, hipMemcpyDeviceToDevice, stream));
             launch_fft3d(TCX_PREC_FP64, -1, nG, 2, geo, 1.0 / (double)N, stream);
-            const JastrowPair j0 = jastrow_pair(0, 0), j1 = jastrow_pair(0, 1);
-            launch_dnu(nG, dnu, geo, j0, j1, stream);
-            launch_fft3d(TCX_PREC_FP64, +1, dnu, 3, geo, 1.0, stream);
-            if (which == 0) {
-                launch_sum_squares(dnu, Vc, N, stream);        // tc3a1.cpp:78-84
-            } else {
-                launch_mul_shared(dnu, 3, (const double*)d_density.p, N, stream);  // tc3b1.cpp:77-81 (spin 0)
-                launch_fft3d(TCX_PREC_FP64, -1, dnu, 3, geo, 1.0 / (double)N, stream);
-                launch_tc3b1_V(dnu, Vc, geo, j0, j1, stream);  // tc3b1.cpp:86-100
-                launch_fft3d(TCX_PREC_FP64, +1, Vc, 1, geo, 1.0, stream);
+            // dnu_s = sum_s' u(s, s') G n_s' (tc3a1.cpp:54-75, tc3b1.cpp:52-71)
+            for (int s = 0; s < nspin; ++s)
+                launch_dnu(nG, dnu + (size_t)s * 6 * N, geo, jastrow_pair(s, 0), jastrow_pair(s, 1), stream);
+            launch_fft3d(TCX_PREC_FP64, +1, dnu, 3 * nspin, geo, 1.0, stream);
+            for (int s = 0; s < nspin; ++s) {
+                double* Vc = (double*)C.buf.p + (size_t)s * 2 * N;
+                double* ds = dnu + (size_t)s * 6 * N;
+                if (which == 0) {
+                    launch_sum_squares(ds, Vc, N, stream);  // tc3a1.cpp:78-84
+                } else {
+                    // w_s = FFT_forward[n_s dnu_s] (tc3b1.cpp:73-81)
+                    launch_mul_shared(ds, 3, (const double*)d_density.p + (size_t)s * 2 * N, N, stream);
+                }
+            }
+            if (which == 1) {
+                launch_fft3d(TCX_PREC_FP64, -1, dnu, 3 * nspin, geo, 1.0 / (double)N, stream);
+                // V_s1 = sum_s2 u(s1, s2) G.w_s2 (tc3b1.cpp:83-100; one spin: s2_ref = 0)
+                for (int s = 0; s < nspin; ++s) {
+                    double* Vc = (double*)C.buf.p + (size_t)s * 2 * N;
+                    launch_tc3b1_V(dnu, nspin == 1 ? nullptr : dnu + 6 * N, Vc, geo, jastrow_pair(s, 0),
+                                   jastrow_pair(s, 1), stream);
+                }
+                launch_fft3d(TCX_PREC_FP64, +1, (double*)C.buf.p, nspin, geo, 1.0, stream);
             }
             C.valid = true;
         }
-        const double* V = (const double*)C.buf.p;
-        // H3 phi_kj += factor * FFT_forward[V phi_kj] on the sphere of k (tc3a1.cpp:86-116)
+        // H3 phi_kj += factor * FFT_forward[V_s phi_kj] on the sphere of k (tc3a1.cpp:86-116), the
+        // bands of spin s being the jobs of its k-blocks
         const int chunk = (int)std::max<size_t>(1, std::min<size_t>(P.n_app, ((size_t)1 << 30) / g));
         d_W.ensure((size_t)chunk * g);
         const OrbEntry* jobs = (const OrbEntry*)P.d_appjobs.p;
         timed(3, 0, [&] {
-            for (int c0 = 0; c0 < P.n_app; c0 += chunk) {
-                const int nc = std::min(chunk, P.n_app - c0);
-                double* Wc = (double*)d_W.p;
-                launch_zero(Wc, (size_t)nc * g, stream);
-                launch_scatter(io.phi, (const int*)d_gindex.p, jobs + c0, nc, max_npw, Wc, N,
-                               (const double*)d_kcart.p, geo, stream);
-                launch_fft3d(TCX_PREC_FP64, +1, Wc, nc, geo, 1.0, stream);
-                launch_mul_shared(Wc, nc, V, N, stream);
-                launch_fft3d(TCX_PREC_FP64, -1, Wc, nc, geo, 1.0 / (double)N, stream);
-                launch_gather_add(Wc, (const int*)d_gindex.p, jobs + c0, nc, max_npw, io.out, N, factor, stream);
+            for (int s = 0; s < nspin; ++s) {
+                const double* V = (const double*)C.buf.p + (size_t)s * 2 * N;
+                const int a0 = (int)P.app_first[(size_t)s * nk_phys], a1 = (int)P.app_first[(size_t)(s + 1) * nk_phys];
+                for (int c0 = a0; c0 < a1; c0 += chunk) {
+                    const int nc = std::min(chunk, a1 - c0);
+                    double* Wc = (double*)d_W.p;
+                    launch_zero(Wc, (size_t)nc * g, stream);
+                    launch_scatter(io.phi, (const int*)d_gindex.p, jobs + c0, nc, max_npw, Wc, N,
+                                   (const double*)d_kcart.p, geo, stream);
+                    launch_fft3d(TCX_PREC_FP64, +1, Wc, nc, geo, 1.0, stream);
+                    launch_mul_shared(Wc, nc, V, N, stream);
+                    launch_fft3d(TCX_PREC_FP64, -1, Wc, nc, geo, 1.0 / (double)N, stream);
+                    launch_gather_add(Wc, (const int*)d_gindex.p, jobs + c0, nc, max_npw, io.out, N, factor,
+                                      stream);
+                }
             }
         });
     }

@@ -405,7 +405,7 @@ void Context::a8_occupied_phase(const std::vector<int>& nb, int prec)
     c_a8.buf.ensure(g64);
     double* X = (double*)c_a8.buf.p;
     launch_fft3d(TCX_PREC_FP64, -1, B2, 3, geo, 1.0 / (double)N, stream);
-    launch_tc3b1_V(B2, X, geo, j00, j01, stream);  // sum_d G_d (u(G,0,0) + u(G,0,1)) B2_d(G)
+    launch_tc3b1_V(B2, nullptr, X, geo, j00, j01, stream);  // sum_d G_d (u(G,0,0) + u(G,0,1)) B2_d(G)
     launch_fft3d(TCX_PREC_FP64, +1, X, 1, geo, 1.0, stream);
     launch_fft3d(TCX_PREC_FP64, +1, V3a2, 1, geo, 1.0, stream);
     launch_axpby(X, V3a2, X, fb2, fa2, N, stream);

@@ -12,7 +12,8 @@ spin densities (hf2h.cpp:30).  Pins:
     u(0, 0) for tc2x), the Hartree term the one-spin oracle on both channels' states with
     fillings f_s (FP64 1e-10, FP32-split per band 3e-5);
   * tc2h, which couples the spins through u(s, s'), equals per channel a sum of one-spin
-    oracle applies, one per spin's states (FP64 1e-10);
+    oracle applies, one per spin's states (FP64 1e-10); tc3a1 and tc3b1 equal the one-spin
+    oracle on both spins' states together where the Jastrow makes that exact (FP64 1e-10);
   * the three-body terms are refused with TCX_ERR_UNSUPPORTED.
 """
 import dataclasses
@@ -47,6 +48,23 @@ def with_states(sys_, nocc, filling, phi, chi):
                                phi_occ=phi, chi_occ=chi)
 
 
+def combined(sa, sb, fa, fb):
+    """One one-spin system holding, at every k, the states of sa (fillings fa) and then those
+    of sb (fillings fb): its density is the sum of the two spin densities."""
+    nocc_all, f_all, phi_all, chi_all = [], [], [], []
+    oa, ob = sa.occ_offsets(), sb.occ_offsets()
+    fo_a = np.concatenate([[0], np.cumsum(sa.nocc)])
+    fo_b = np.concatenate([[0], np.cumsum(sb.nocc)])
+    for k in range(sa.nk):
+        nocc_all.append(sa.nocc[k] + sb.nocc[k])
+        f_all += [fa[fo_a[k]:fo_a[k + 1]], fb[fo_b[k]:fo_b[k + 1]]]
+        phi_all += [sa.phi_occ[oa[k]:oa[k + 1]], sb.phi_occ[ob[k]:ob[k + 1]]]
+        if sa.chi_occ is not None:
+            chi_all += [sa.chi_occ[oa[k]:oa[k + 1]], sb.chi_occ[ob[k]:ob[k + 1]]]
+    return with_states(sa, nocc_all, np.concatenate(f_all), np.concatenate(phi_all),
+                       np.concatenate(chi_all) if chi_all else None)
+
+
 def both_channels(sys_, nb, seed):
     """Applied bands: the same nb per k in both channels, different coefficients."""
     nb0, phi0 = S.applied_bands(sys_, nb=nb, seed=seed)
@@ -59,7 +77,7 @@ def test_identical_channels_match_one_spin(engine, method):
     s1 = S.tiny_system(method=method, n=(8, 8, 8), kmesh=(2, 2, 1), nbands=3, nocc=2, seed_stream=151)
     nb0, phi0 = S.applied_bands(s1, nb=[3, 0, 2, 1], seed=152)
     engine.load_system(s1)
-    terms = ["kinetic", "hf2h", "hf2x"] + (["tc2x", "tc2h"] if method != "HF" else [])
+    terms = ["kinetic", "hf2h", "hf2x"] + (["tc2x", "tc2h", "tc3a1", "tc3b1"] if method != "HF" else [])
     one = {t: getattr(engine, t)(nb0, phi0) for t in terms}
     s2 = two_spin(s1, s1)
     engine.load_system(s2)
@@ -109,18 +127,7 @@ def test_channels_against_oracle(engine, method):
                     print(f"{method} {t} fp32 spin {s}: per-band {rel.max():.2e}", flush=True)
                     assert rel.max() <= 3e-5, (t, s)
     # Hartree: both channels' states with their fillings in one one-spin system
-    nocc_all, f_all, phi_all, chi_all = [], [], [], []
-    oa, ob = sa.occ_offsets(), sb.occ_offsets()
-    fo_a = np.concatenate([[0], np.cumsum(sa.nocc)])
-    fo_b = np.concatenate([[0], np.cumsum(sb.nocc)])
-    for k in range(sa.nk):
-        nocc_all.append(sa.nocc[k] + sb.nocc[k])
-        f_all += [fa[fo_a[k]:fo_a[k + 1]], fb[fo_b[k]:fo_b[k + 1]]]
-        phi_all += [sa.phi_occ[oa[k]:oa[k + 1]], sb.phi_occ[ob[k]:ob[k + 1]]]
-        if sa.chi_occ is not None:
-            chi_all += [sa.chi_occ[oa[k]:oa[k + 1]], sb.chi_occ[ob[k]:ob[k + 1]]]
-    hsys = with_states(sa, nocc_all, np.concatenate(f_all), np.concatenate(phi_all),
-                       np.concatenate(chi_all) if chi_all else None)
+    hsys = combined(sa, sb, fa, fb)
     got = engine.hf2h(nb2, phi2)
     for s, (lo, hi) in enumerate(((0, n0), (n0, phi2.size))):
         r = O.Oracle(hsys).hf2h(*chans[s])
@@ -157,6 +164,35 @@ def test_tc2h_channels_against_oracle(engine, method):
         assert err <= 1e-10, s
 
 
+@pytest.mark.parametrize("method", ["TC", "BITC"])
+def test_tc3_local_channels_against_oracle(engine, method):
+    """tc3a1 and tc3b1 with two spins (tc3a1.cpp:54-84, tc3b1.cpp:52-100): dnu_s =
+    sum_s' u(s,s') G n_s'; tc3a1's V_s = sum_d dnu_s,d^2, tc3b1's V_s1 = sum_s2 u(s1,s2)
+    G.FFT[n_s2 dnu_s2].  With u(s, 0) = u(s, 1) = u_s, channel s of tc3a1 equals the one-spin
+    oracle on both spins' states together with u_s in every Jastrow slot; with one u for
+    every spin pair, so does tc3b1, which then checks that each n_s2 weights its own term
+    (FP64, 1e-10)."""
+    sa = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 1, 1), nbands=4, nocc=3, seed_stream=195)
+    sb = S.tiny_system(method=method, n=(12, 12, 12), kmesh=(2, 1, 1), nbands=4, nocc=2, seed_stream=196)
+    fa = np.tile([1.0, 1.0, 0.6], sa.nk)
+    fb = np.tile([1.0, 0.3], sb.nk)
+    nb0, phi0, nb1, phi1, nb2, phi2 = both_channels(sa, [3, 2], 197)
+    n0 = phi0.size
+    chans = ((nb0, phi0, 0, n0), (nb1, phi1, n0, phi2.size))
+    rows = {"tc3a1": (np.array([[0.5, 0.5], [0.8, 0.8]]), np.array([[1.1, 1.1], [0.7, 0.7]])),
+            "tc3b1": (np.full((2, 2), 0.6), np.full((2, 2), 0.9))}
+    for term, (A, F) in rows.items():
+        two = two_spin(dataclasses.replace(sa, A=A, F=F), dataclasses.replace(sb, A=A, F=F), fa, fb)
+        engine.load_system(two)
+        got = getattr(engine, term)(nb2, phi2)
+        for s, (nbs, phis, lo, hi) in enumerate(chans):
+            one = dataclasses.replace(combined(sa, sb, fa, fb), A=np.full((2, 2), A[s, 0]), F=np.full((2, 2), F[s, 0]))
+            ref = getattr(O.Oracle(one), term)(nbs, phis)
+            err = np.abs(got[lo:hi] - ref).max() / np.abs(ref).max()
+            print(f"{method} {term} spin {s}: {err:.2e}", flush=True)
+            assert err <= 1e-10, (term, s)
+
+
 def test_hf_all_is_the_sum_of_its_terms(engine):
     sa = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=2, seed_stream=171)
     sb = S.tiny_system(method="HF", n=(8, 8, 8), kmesh=(2, 1, 1), nbands=3, nocc=1, seed_stream=172)
@@ -174,7 +210,7 @@ def test_spin_coupling_terms_are_refused(engine):
     engine.load_system(two_spin(s1, s1))
     nb, phi = S.applied_bands(s1, nb=[1, 1], seed=182)
     nb2, phi2 = np.concatenate([nb, nb]).astype(np.int32), np.concatenate([phi, phi])
-    for t in ("tc3a1", "tc3b1", "tc3a3b3b4b6", "tc3a2a4b2b5"):
+    for t in ("tc3a3b3b4b6", "tc3a2a4b2b5"):
         with pytest.raises(tcx.TcxError) as e:
             getattr(engine, t)(nb2, phi2)
         assert e.value.code == tcx.TCX_ERR_UNSUPPORTED, t
