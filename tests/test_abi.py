@@ -43,6 +43,27 @@ def test_host_abi_errors():
     assert L.tcxh_qe_num_replicas(None, None) == tcx.TCX_ERR_ARG
 
 
+def test_host_abi_rejects_truncated_and_malformed_inputs(tmp_path):
+    """A flat input cut anywhere, or with a bad magic or record tag, is TCX_ERR_ARG; the whole
+    file loads and expands (no GPU calls)."""
+    from tc_amd import qe as Q
+    qe = Q.load_fixture(os.path.join(ROOT, "tests", "golden", "qe_example14.npz"))
+    path = tmp_path / "in.tcxqe"
+    Q.write_flat(qe, str(path))
+    data = path.read_bytes()
+    L = tcx.load_library()
+    cuts = [4, 8 + 20, 8 + 48 + 3, len(data) // 2, len(data) - 1]
+    bad = [b"TCXQE002" + data[8:], data[:8 + 32] + b"<x9\0\0\0\0\0" + data[8 + 40:]]
+    for blob in [data[:c] for c in cuts] + bad:
+        path.write_bytes(blob)
+        h = ctypes.c_void_p()
+        assert L.tcxh_qe_load(str(path).encode(), ctypes.byref(h)) == tcx.TCX_ERR_ARG
+        assert not h.value
+    path.write_bytes(data)
+    with Q.HostInputs(qe) as h:
+        assert len(h.replicas()) > qe.nirr
+
+
 def test_library_is_gfx950_only():
     data = open(tcx.LIB_PATH, "rb").read()
     assert b"gfx950" in data
