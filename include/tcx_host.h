@@ -16,7 +16,9 @@
  * 32-byte name, an 8-byte type tag ("<f8", "<i4", "<c16"), an int64 element count and
  * the little-endian data.
  *
- * Errors: every function returns a tcx_status from tcx.h; tcxh_last_error() gives the
+ * Errors: every function returns a tcx_status from tcx.h: TCX_ERR_ARG for bad input
+ * (including a malformed file or a rotation that is not unimodular), and from
+ * tcxh_scf_run the status of an engine call that failed.  tcxh_last_error() gives the
  * message of the calling thread's last failure.
  */
 #ifndef TCX_HOST_H

@@ -148,6 +148,9 @@ int guarded(const char* what, F&& f)
         const int st = f();
         if (st == TCX_OK) g_error.clear();
         return st;
+    } catch (const tcx_host::StatusError& e) {  // an engine call failed: keep its status
+        g_error = std::string(what) + ": " + e.what();
+        return e.status;
     } catch (const std::invalid_argument& e) {
         g_error = std::string(what) + ": " + e.what();
         return TCX_ERR_ARG;
@@ -179,7 +182,11 @@ int tcxh_qe_load(const char* path, tcxh_qe** out)
     return guarded("tcxh_qe_load", [&] {
         auto q = std::make_unique<tcxh_qe>();
         tcx_host::read_qe_flat(path, q->qe, q->settings);
-        q->reps = tcx_host::replicate_kpoints(q->qe);
+        try {
+            q->reps = tcx_host::replicate_kpoints(q->qe);
+        } catch (const std::runtime_error& e) {  // singular cell or rotation: bad input
+            throw std::invalid_argument(e.what());
+        }
         *out = q.release();
         return (int)TCX_OK;
     });

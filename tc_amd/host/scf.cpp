@@ -13,7 +13,7 @@ constexpr double kPi = 3.14159265358979323846;
 void check(tcx_context* ctx, int st, const char* what)
 {
     if (st != TCX_OK)
-        throw std::runtime_error(std::string(what) + ": " + (ctx ? tcx_last_error(ctx) : "status " + std::to_string(st)));
+        throw StatusError(st, std::string(what) + ": " + (ctx ? tcx_last_error(ctx) : "status " + std::to_string(st)));
 }
 
 // Equal modulo reciprocal-lattice vectors (kpoints.cpp:10-26).

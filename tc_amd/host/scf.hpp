@@ -17,11 +17,13 @@
 // no pseudopotentials (is_heg), no forces, biortho_david = false.
 //
 // Error behaviour: the reference stops the run (error_messages::stop); here a failing
-// call throws std::runtime_error carrying tcx_last_error().
+// engine call throws tcx_host::StatusError (a std::runtime_error) carrying its tcx_status
+// and tcx_last_error(), and invalid inputs throw std::runtime_error.
 #pragma once
 
 #include <array>
 #include <complex>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -30,6 +32,12 @@
 namespace tcx_host {
 
 using Complex = std::complex<double>;
+
+// A failed engine call: its status and message.
+struct StatusError : std::runtime_error {
+    int status;
+    StatusError(int st, const std::string& msg) : std::runtime_error(msg), status(st) {}
+};
 
 // What TC++ reads from the QE save directory (io_qe_files_read_xml.cpp:55-273,
 // io_qe_files_read_wfc.f90:69-140), for one independent spin.

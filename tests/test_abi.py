@@ -62,6 +62,13 @@ def test_host_abi_rejects_truncated_and_malformed_inputs(tmp_path):
     path.write_bytes(data)
     with Q.HostInputs(qe) as h:
         assert len(h.replicas()) > qe.nirr
+    # a rotation that is not unimodular is bad input too
+    import dataclasses
+    rot = qe.rot_qe.copy()
+    rot[1] = 2 * np.eye(3, dtype=rot.dtype)
+    with pytest.raises(tcx.TcxError) as e:
+        Q.HostInputs(dataclasses.replace(qe, rot_qe=rot))
+    assert e.value.code == tcx.TCX_ERR_ARG and "unimodular" in str(e.value)
 
 
 def test_library_is_gfx950_only():

@@ -60,6 +60,16 @@ def test_scf_run_clears_a_polynomial_jastrow_left_on_the_context(engine):
     test_first_scf_iteration_matches_reference(engine, 15, tcx.PREC_FP64)
 
 
+def test_scf_run_reports_the_engine_status(engine):
+    """An engine call that fails inside the SCF run keeps its status through the host ABI:
+    a grid with no compiled FFT length is TCX_ERR_UNSUPPORTED, as from tcx_set_cell."""
+    import dataclasses
+    qe = Q.load_fixture(os.path.join(HERE, "golden", "qe_example15.npz"))
+    with pytest.raises(tcx.TcxError) as e:
+        scf.run(engine, dataclasses.replace(qe, grid=(14, 14, 14)), precision=tcx.PREC_FP64)
+    assert e.value.code == tcx.TCX_ERR_UNSUPPORTED and "set_cell" in str(e.value)
+
+
 def _tc_system_from(ex, symmetric):
     """The QE orbitals of a reference example with the QE fillings, as a TC system."""
     qe = Q.load_fixture(os.path.join(HERE, "golden", f"qe_example{ex}.npz"))
