@@ -30,7 +30,8 @@ namespace {
 
 // Pass C splits every line of its planes over S lanes (PencilSplit: lanes l, l ^ 16,
 // l ^ 32, l ^ 48 for S = 4), which keeps its NP/S FP64 accumulators per lane within
-// the registers of three waves per SIMD; a wavefront then holds 64/S lines.  Planes
+// the registers of three waves per SIMD (two at NP = 30, whose lines split over 2 lanes
+// only: under three it spilled 380 bytes per lane); a wavefront then holds 64/S lines.  Planes
 // per workgroup: the best fill of the wavefronts with at most 56 KiB of LDS tiles (two
 // per plane).
 template <typename T, int NP>
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(kPencilBlock) void k_hf2x_zB(cpx<T>* __restrict__ r
 // and the next pair's row and phi_qi values are loaded before the current pair is
 // transformed.  No atomics: each (target, plane) belongs to one workgroup.
 template <typename T, int NP>
-__global__ __launch_bounds__((PlaneGroupC<T, NP>::threads), 3) void k_hf2x_xyC(
+__global__ __launch_bounds__((PlaneGroupC<T, NP>::threads), (NP == 30 ? 2 : 3)) void k_hf2x_xyC(
     const cpx<T>* __restrict__ rec, const cpx<T>* __restrict__ phiR_occ, const PairEntry* __restrict__ pairs,
     const TargetEntry* __restrict__ targets, const int4* __restrict__ rowinfo, const double* __restrict__ fill,
     cpx<double>* __restrict__ W, float2* __restrict__ Wc, int cstride, int nz)
